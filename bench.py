@@ -1,0 +1,157 @@
+"""Benchmark: VBD vertex-updates/s on the 512x512 cloth (one MI355X).
+
+Workload: the 512x512 cloth (262,144 vertices, 522,242 StVK triangles, 782,341 bending hinges,
+512 anchored vertices), Y drawn log-uniform in [1e3, 1e5] from --seed (default 3: Y = 4.95e4, a draw
+for which the reference VBD converges), dt = 1/60, 20 iterations per step, FP32 on the GPU.
+Inputs are resident in HBM before the timed region. A step = one VBD::step (inertial predict, 20
+iterations x 4 colour sweeps, velocity update), captured once in a hipGraph and replayed.
+
+value      = vertices x iterations x steps / wall time of the timed steps (synchronised both sides)
+roofline   = the vertex-solve kernel: compulsory bytes per launch / mean launch duration, launch
+             durations from HIP events on the solver's stream over an event-timed replay of the same
+             steps right after the throughput region
+cpu_baseline = the FP64 OpenMP oracle (restatement of spg's VBD::step), closed-form derivatives,
+             one step of the same scene on the host cores
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--timing-steps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs the VBD step on one GPU; multi-GPU runs are not part of it "
+              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    from spg_amd import native, scenes
+
+    s = scenes.cloth_config2(seed=args.seed, n=args.n)
+    ctx = native.VbdContext([s.vbd_object()], device=0)
+    tuned = None
+    if args.lanes or args.block:
+        kw = {}
+        if args.lanes:
+            kw["lanes_per_vertex"] = args.lanes
+        if args.block:
+            kw["block_size"] = args.block
+        ctx.set_launch(**kw)
+    elif not args.no_autotune:
+        tuned = ctx.autotune(trials=3)
+    ctx.set_launch(use_graph=1)
+    V = s.n
+    iters = s.iterations
+    print(f"# scene {s.meta} colours {ctx.num_colors()} launch {ctx.launch()}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        ctx.step(s.dt, 1, iters, s.gravity)
+    ctx.synchronize()
+
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.step(s.dt, 1, iters, s.gravity)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    value = V * iters * args.steps / wall
+    print(f"# timed {args.steps} steps in {wall * 1e3:.2f} ms", file=sys.stderr, flush=True)
+
+    # event-timed replay of the same steps (non-graph launches, one event pair per vertex-solve launch)
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    for _ in range(args.timing_steps):
+        ctx.step(s.dt, 1, iters, s.gravity)
+    ctx.synchronize()
+    st = ctx.stats()
+    ctx.set_timing(False)
+    mean_ms = st["solve_ms"] / max(1, st["solve_launches"])
+    bytes_per_launch = st["algorithmic_bytes"] / max(1, st["solve_launches"])
+    achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
+    x, v = ctx.state()
+    finite = bool(np.all(np.isfinite(x)))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(s)
+
+    out = {
+        "metric": "VBD vertex-updates/s on 512² cloth; time to 1e-4 residual, learned vs plain",
+        "value": value,
+        "unit": "vertex-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {"workload": f"cloth {args.n}x{args.n} (V={V}, {s.meta['triangles']} StVK triangles, "
+                               f"{s.meta['hinges']} bending hinges, top row anchored), Y={s.meta['young']:.5g} "
+                               f"(seed {args.seed}), dt=1/60, {iters} iterations/step, hipGraph replay",
+                   "vertices": V, "iterations": iters, "colors": ctx.num_colors(),
+                   "launch": ctx.launch(), "autotuned": tuned},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "solveColorKernel", "mean_launch_us": mean_ms * 1e3,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "launches_timed": st["solve_launches"]},
+        "solver_stats": {"fp64_fallbacks": st["fp64_fallbacks"], "skipped_blocks": st["skipped_blocks"],
+                         "state_finite": finite},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(scene):
+    """FP64 oracle (OpenMP restatement of VBD::step, closed-form derivatives) on the host cores:
+    one step of the same scene, colouring/incidence excluded (done before timing)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ol.set_threads(cores)
+    o = ol.from_scene(scene)
+    o.colors()  # first-call precomputation outside the timed region (vbd.cpp:14-18)
+    t0 = time.perf_counter()
+    o.step(scene.dt, 1, scene.iterations, scene.gravity, mode=ol.CLOSED_FORM)
+    dt_cf = time.perf_counter() - t0
+    # AD-mode (TinyAD-like cost: two full-element derivative passes per incidence), one iteration
+    o2 = ol.from_scene(scene)
+    o2.colors()
+    t0 = time.perf_counter()
+    o2.step(scene.dt, 1, 1, scene.gravity, mode=ol.AD)
+    dt_ad = time.perf_counter() - t0
+    return {"value": scene.n * scene.iterations / dt_cf, "unit": "vertex-updates/s", "cores": cores, "kind": "port",
+            "sample": f"one step ({scene.iterations} iterations) of the same {scene.n}-vertex cloth, FP64 "
+                      f"closed-form derivatives, {dt_cf:.2f} s",
+            "ad_mode_value": scene.n * 1 / dt_ad,
+            "ad_mode_sample": f"one step of 1 iteration, FP64 forward-mode dual numbers, {dt_ad:.2f} s"}
+
+
+if __name__ == "__main__":
+    main()

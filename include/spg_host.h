@@ -1,0 +1,74 @@
+/*
+ * spg_host.h — C binding of the C++ spg API mirror (libspg_host.so), for callers that are not C++
+ * (Python ctypes in this repository, or a cgo/JNI layer). It exposes scene construction through the
+ * mirrored spg classes and the VBDGpu drop-in solver:
+ *
+ *   spg_pg_*          spg::ParticleGroup (spg/include/spg/sim/simObject/particleGroup.h:14-60) with
+ *                     addParticle (particleGroup.cpp:74-82), addEnergy and each energy's addStencil +
+ *                     preparePrecomputations (the five energy headers under spg/include/spg/sim/energy/).
+ *   spg_trimesh_*     spg::TriangleMesh edge/opposing-vertex enumeration (spg/include/spg/geom/triangleMesh.h:16-50)
+ *                     and the cloth bending wiring of apps/solverCompare.cpp:441-452.
+ *   spg_solver_*      spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41) implemented by
+ *                     spg::solver::VBDGpu, the GPU drop-in for spg::solver::VBD (vbd.h:14-41).
+ *
+ * Errors: 0 on success, nonzero on failure with the message (the reference's exception text where
+ * the reference throws) in spg_host_last_error().
+ */
+#ifndef SPG_HOST_H
+#define SPG_HOST_H
+
+#include <stdint.h>
+
+#include "spg_vbd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct spg_pgroup spg_pgroup;
+typedef struct spg_solver spg_solver;
+
+const char *spg_host_last_error(void);
+
+spg_pgroup *spg_pg_create(void);
+void spg_pg_destroy(spg_pgroup *pg);
+/* addParticle(x[i], x0[i], mass[i]) for i < n */
+int spg_pg_add_particles(spg_pgroup *pg, int64_t n, const double *x, const double *x0, const double *mass);
+/* Creates one energy of `type` (SPG_VBD_* of spg_vbd.h), calls addStencil for each of n stencils with the
+   per-stencil arguments `args` and, if prepare != 0, preparePrecomputations(pg); then addEnergy. args:
+     SPRING [restLength, stiffness]; MEMBRANE_STVK [young, poisson]; DISCRETE_BENDING [restTheta, stiffness];
+     STABLE_NEO_HOOKEAN [young, poisson]; SPRING_ANCHOR [ax, ay, az, stiffness].
+   Returns the energy index in *index. */
+int spg_pg_add_energy(spg_pgroup *pg, int32_t type, int64_t n, const int32_t *stencils, const double *args,
+                      int32_t prepare, int32_t *index);
+int spg_pg_prepare_energy(spg_pgroup *pg, int32_t index);
+int spg_pg_num_particles(const spg_pgroup *pg, int64_t *n);
+/* Prepared data of energy `index` in the params layout of spg_vbd_energy (spg_vbd.h). */
+int spg_pg_energy_params(const spg_pgroup *pg, int32_t index, double *params);
+int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v);
+int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v);
+
+/* TriangleMesh(faces).bendingStencils(): out may be NULL to query the count. */
+int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int32_t *faces, int32_t *out,
+                                 int64_t *count);
+
+spg_solver *spg_solver_create_vbd_gpu(int32_t device);
+void spg_solver_destroy(spg_solver *s);
+int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg); /* copies the group, shares its energies */
+int spg_solver_set_dt(spg_solver *s, double dt);
+int spg_solver_set_num_substeps(spg_solver *s, int32_t n);
+int spg_solver_set_gravity(spg_solver *s, const double g[3]);
+int spg_solver_set_iterations(spg_solver *s, int32_t iterations);
+int spg_solver_set_resident_state(spg_solver *s, int32_t resident);
+int spg_solver_step(spg_solver *s);
+int spg_solver_reset(spg_solver *s);
+int spg_solver_sync_to_host(spg_solver *s);
+int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v);
+/* The C-ABI context behind the solver (NULL before the first step). */
+spg_vbd_ctx *spg_solver_context(spg_solver *s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPG_HOST_H */

@@ -1,0 +1,1199 @@
+// FP64 CPU oracle for the spg VBD step — TEST INFRASTRUCTURE ONLY (see vbd_oracle.h).
+// Every function cites the reference file:line it restates. Nothing here is part of the product.
+#include "vbd_oracle.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+thread_local std::string g_err;
+
+// ---------------------------------------------------------------------------------------------
+// Forward-mode dual numbers with dense gradient and (optionally) dense Hessian over N inputs.
+// Textbook second-order forward AD (Griewank & Walther, "Evaluating Derivatives", ch. 13):
+// every elementary op propagates value, gradient and Hessian by the chain rule, like
+// TinyAD::Scalar<N, double, hess> does for the reference (energy.h:84-86,348-362).
+// ---------------------------------------------------------------------------------------------
+template <int N, bool H>
+struct Dual {
+    double v;
+    double g[N];
+    double h[H ? N * N : 1];
+    Dual() = default;
+    Dual(double c) : v(c)  // NOLINT: implicit on purpose (constants inside expressions)
+    {
+        std::memset(g, 0, sizeof(g));
+        if (H) std::memset(h, 0, sizeof(h));
+    }
+    static Dual var(double c, int i)
+    {
+        Dual r(c);
+        r.g[i] = 1.0;
+        return r;
+    }
+};
+
+template <int N, bool H>
+inline Dual<N, H> chain1(const Dual<N, H> &u, double f, double fp, double fpp)
+{
+    Dual<N, H> r;
+    r.v = f;
+    for (int i = 0; i < N; ++i) r.g[i] = fp * u.g[i];
+    if (H) {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j) r.h[i * N + j] = fp * u.h[i * N + j] + fpp * u.g[i] * u.g[j];
+    }
+    return r;
+}
+
+template <int N, bool H>
+inline Dual<N, H> operator+(const Dual<N, H> &a, const Dual<N, H> &b)
+{
+    Dual<N, H> r;
+    r.v = a.v + b.v;
+    for (int i = 0; i < N; ++i) r.g[i] = a.g[i] + b.g[i];
+    if (H)
+        for (int i = 0; i < N * N; ++i) r.h[i] = a.h[i] + b.h[i];
+    return r;
+}
+template <int N, bool H>
+inline Dual<N, H> operator-(const Dual<N, H> &a, const Dual<N, H> &b)
+{
+    Dual<N, H> r;
+    r.v = a.v - b.v;
+    for (int i = 0; i < N; ++i) r.g[i] = a.g[i] - b.g[i];
+    if (H)
+        for (int i = 0; i < N * N; ++i) r.h[i] = a.h[i] - b.h[i];
+    return r;
+}
+template <int N, bool H>
+inline Dual<N, H> operator-(const Dual<N, H> &a)
+{
+    return chain1(a, -a.v, -1.0, 0.0);
+}
+template <int N, bool H>
+inline Dual<N, H> operator*(const Dual<N, H> &a, const Dual<N, H> &b)
+{
+    Dual<N, H> r;
+    r.v = a.v * b.v;
+    for (int i = 0; i < N; ++i) r.g[i] = a.v * b.g[i] + b.v * a.g[i];
+    if (H) {
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                r.h[i * N + j] = a.v * b.h[i * N + j] + b.v * a.h[i * N + j] + a.g[i] * b.g[j] + b.g[i] * a.g[j];
+    }
+    return r;
+}
+template <int N, bool H>
+inline Dual<N, H> operator*(double c, const Dual<N, H> &a)
+{
+    return chain1(a, c * a.v, c, 0.0);
+}
+template <int N, bool H>
+inline Dual<N, H> operator*(const Dual<N, H> &a, double c)
+{
+    return chain1(a, c * a.v, c, 0.0);
+}
+template <int N, bool H>
+inline Dual<N, H> operator+(const Dual<N, H> &a, double c)
+{
+    Dual<N, H> r = a;
+    r.v += c;
+    return r;
+}
+template <int N, bool H>
+inline Dual<N, H> operator-(const Dual<N, H> &a, double c)
+{
+    Dual<N, H> r = a;
+    r.v -= c;
+    return r;
+}
+template <int N, bool H>
+inline Dual<N, H> operator/(const Dual<N, H> &a, const Dual<N, H> &b)
+{
+    const double iv = 1.0 / b.v;
+    return a * chain1(b, iv, -iv * iv, 2.0 * iv * iv * iv);
+}
+template <int N, bool H>
+inline Dual<N, H> sqrt(const Dual<N, H> &u)
+{
+    const double s = std::sqrt(u.v);
+    return chain1(u, s, 0.5 / s, -0.25 / (s * u.v));
+}
+// atan2(y, x): partials f_y = x/r^2, f_x = -y/r^2, f_yy = -2xy/r^4, f_xx = 2xy/r^4, f_xy = (y^2-x^2)/r^4
+template <int N, bool H>
+inline Dual<N, H> atan2(const Dual<N, H> &y, const Dual<N, H> &x)
+{
+    Dual<N, H> r;
+    const double r2 = x.v * x.v + y.v * y.v;
+    const double r4 = r2 * r2;
+    const double fy = x.v / r2, fx = -y.v / r2;
+    r.v = std::atan2(y.v, x.v);
+    for (int i = 0; i < N; ++i) r.g[i] = fy * y.g[i] + fx * x.g[i];
+    if (H) {
+        const double fyy = -2.0 * x.v * y.v / r4, fxx = 2.0 * x.v * y.v / r4, fxy = (y.v * y.v - x.v * x.v) / r4;
+        for (int i = 0; i < N; ++i)
+            for (int j = 0; j < N; ++j)
+                r.h[i * N + j] = fy * y.h[i * N + j] + fx * x.h[i * N + j] + fyy * y.g[i] * y.g[j] +
+                                 fxx * x.g[i] * x.g[j] + fxy * (x.g[i] * y.g[j] + y.g[i] * x.g[j]);
+    }
+    return r;
+}
+inline double sqrt(double x) { return std::sqrt(x); }
+inline double atan2(double y, double x) { return std::atan2(y, x); }
+
+template <class S>
+struct V3 {
+    S x, y, z;
+};
+template <class S>
+V3<S> sub(const V3<S> &a, const V3<S> &b)
+{
+    return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class S>
+S dot(const V3<S> &a, const V3<S> &b)
+{
+    return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+template <class S>
+V3<S> cross(const V3<S> &a, const V3<S> &b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class S>
+V3<S> normalized(const V3<S> &a)
+{
+    const S n = sqrt(dot(a, a));
+    return {a.x / n, a.y / n, a.z / n};
+}
+template <class S>
+V3<S> neg(const V3<S> &a)
+{
+    return {-a.x, -a.y, -a.z};
+}
+
+// ---------------------------------------------------------------------------------------------
+// Element energies, written over a generic scalar S (double or Dual) exactly as the reference
+// states them. p[] holds the prepared parameters in the layout of vbd_oracle.h.
+// ---------------------------------------------------------------------------------------------
+
+// springEnergy.cpp:20-30 — E = 1/2 k (|x0 - x1| - L0)^2
+template <class S>
+S energySpring(const V3<S> *x, const double *p)
+{
+    const V3<S> d = sub(x[0], x[1]);
+    const S diff = sqrt(dot(d, d)) - p[0];
+    return 0.5 * p[1] * diff * diff;
+}
+
+// springAnchorEnergy.cpp:21-26 — E = 1/2 k_model |x - a|^2
+template <class S>
+S energyAnchor(const V3<S> *x, const double *p)
+{
+    const V3<S> d = {x[0].x - p[0], x[0].y - p[1], x[0].z - p[2]};
+    return 0.5 * p[3] * dot(d, d);
+}
+
+// membraneStvkEnergy.cpp:9-28 + energy.h:336-341 — C = (E00, E11, 2 E01), E = 1/2 C^T Keff C
+template <class S>
+S energyStvk(const V3<S> *x, const double *p)
+{
+    const double *Di = p;      // 2x2 row-major
+    const double *K = p + 4;   // 3x3 row-major
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    // F = [u v] * Dinv (3x2); column c: u * Di[0][c] + v * Di[1][c]
+    const V3<S> f0 = {u.x * Di[0] + v.x * Di[2], u.y * Di[0] + v.y * Di[2], u.z * Di[0] + v.z * Di[2]};
+    const V3<S> f1 = {u.x * Di[1] + v.x * Di[3], u.y * Di[1] + v.y * Di[3], u.z * Di[1] + v.z * Di[3]};
+    const S e00 = 0.5 * (dot(f0, f0) - 1.0);
+    const S e11 = 0.5 * (dot(f1, f1) - 1.0);
+    const S e01 = 0.5 * dot(f0, f1);
+    const S c[3] = {e00, e11, 2.0 * e01};
+    S kc[3] = {S(0.0), S(0.0), S(0.0)};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) kc[i] = kc[i] + K[i * 3 + j] * c[j];
+    return 0.5 * (c[0] * kc[0] + c[1] * kc[1] + c[2] * kc[2]);
+}
+
+// discreteBendingEnergy.cpp:9-30 + energy.h:336-341 — C = theta - theta0, E = 1/2 keff C^2
+template <class S>
+S bendingTheta(const V3<S> *x)
+{
+    const V3<S> e0 = sub(x[1], x[0]);
+    const V3<S> e3 = sub(x[2], x[1]);
+    const V3<S> e4 = sub(x[3], x[1]);
+    const V3<S> n1 = normalized(cross(e0, e3));
+    const V3<S> n2 = neg(normalized(cross(e0, e4)));
+    return atan2(dot(cross(n1, n2), normalized(e0)), dot(n1, n2));
+}
+template <class S>
+S energyBending(const V3<S> *x, const double *p)
+{
+    const S c = bendingTheta(x) - p[0];
+    return 0.5 * c * (p[1] * c);
+}
+
+// stableNeoHookeanEnergy.cpp:33-61 — E = 1/2 lambda (det F - alpha)^2 + 1/2 mu (tr F^T F - 3)
+template <class S>
+S energySnh(const V3<S> *x, const double *p)
+{
+    const double *Di = p;  // 3x3 row-major
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    const V3<S> w = sub(x[3], x[0]);
+    V3<S> f[3];
+    for (int c = 0; c < 3; ++c)
+        f[c] = {u.x * Di[0 * 3 + c] + v.x * Di[1 * 3 + c] + w.x * Di[2 * 3 + c],
+                u.y * Di[0 * 3 + c] + v.y * Di[1 * 3 + c] + w.y * Di[2 * 3 + c],
+                u.z * Di[0 * 3 + c] + v.z * Di[1 * 3 + c] + w.z * Di[2 * 3 + c]};
+    const S J = dot(f[0], cross(f[1], f[2]));
+    const S hyd = J - p[11];
+    const S dev = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.0;
+    return 0.5 * p[9] * hyd * hyd + 0.5 * p[10] * dev;
+}
+
+int stencilSize(int type)
+{
+    switch (type) {
+    case ORC_SPRING: return 2;
+    case ORC_MEMBRANE_STVK: return 3;
+    case ORC_DISCRETE_BENDING: return 4;
+    case ORC_STABLE_NEO_HOOKEAN: return 4;
+    case ORC_SPRING_ANCHOR: return 1;
+    }
+    return 0;
+}
+int rawParams(int type)
+{
+    switch (type) {
+    case ORC_SPRING: return 2;
+    case ORC_MEMBRANE_STVK: return 2;
+    case ORC_DISCRETE_BENDING: return 2;
+    case ORC_STABLE_NEO_HOOKEAN: return 2;
+    case ORC_SPRING_ANCHOR: return 4;
+    }
+    return 0;
+}
+int preparedParams(int type)
+{
+    switch (type) {
+    case ORC_SPRING: return 2;
+    case ORC_MEMBRANE_STVK: return 13;
+    case ORC_DISCRETE_BENDING: return 2;
+    case ORC_STABLE_NEO_HOOKEAN: return 12;
+    case ORC_SPRING_ANCHOR: return 4;
+    }
+    return 0;
+}
+
+template <class S>
+S evalEnergy(int type, const V3<S> *x, const double *p)
+{
+    switch (type) {
+    case ORC_SPRING: return energySpring(x, p);
+    case ORC_MEMBRANE_STVK: return energyStvk(x, p);
+    case ORC_DISCRETE_BENDING: return energyBending(x, p);
+    case ORC_STABLE_NEO_HOOKEAN: return energySnh(x, p);
+    case ORC_SPRING_ANCHOR: return energyAnchor(x, p);
+    }
+    return S(0.0);
+}
+
+// Same as evalEnergy, restricted at compile time to the energies of stencil size NS.
+template <int NS, class S>
+S evalEnergyN(int type, const V3<S> *x, const double *p)
+{
+    if constexpr (NS == 1) {
+        return energyAnchor(x, p);
+    } else if constexpr (NS == 2) {
+        return energySpring(x, p);
+    } else if constexpr (NS == 3) {
+        return energyStvk(x, p);
+    } else {
+        return type == ORC_DISCRETE_BENDING ? energyBending(x, p) : energySnh(x, p);
+    }
+}
+
+// AD evaluation of the full element, as StencilBlockEnergy::dEnergyGradient / dEnergyHessian
+// (energy.h:348-362). NS = stencil size; the gradient pass and the Hessian pass are separate, as
+// accumulateVertexForce and accumulateVertexHessian are (energy.h:240-258).
+template <int NS, bool H>
+void adElement(int type, const double *xs, const double *p, double *E, double *grad, double *hess)
+{
+    constexpr int N = 3 * NS;
+    using D = Dual<N, H>;
+    V3<D> x[NS];
+    for (int s = 0; s < NS; ++s) {
+        x[s].x = D::var(xs[3 * s + 0], 3 * s + 0);
+        x[s].y = D::var(xs[3 * s + 1], 3 * s + 1);
+        x[s].z = D::var(xs[3 * s + 2], 3 * s + 2);
+    }
+    const D e = evalEnergyN<NS>(type, x, p);
+    if (E) *E = e.v;
+    if (grad)
+        for (int i = 0; i < N; ++i) grad[i] = e.g[i];
+    if (H && hess)
+        for (int i = 0; i < N * N; ++i) hess[i] = e.h[i];
+}
+
+// ---------------------------------------------------------------------------------------------
+// Closed-form per-vertex derivatives (gradient w.r.t. x_slot and its 3x3 diagonal Hessian block).
+// ---------------------------------------------------------------------------------------------
+using M3 = std::array<double, 9>;
+using A3 = std::array<double, 3>;
+
+inline A3 a3sub(const double *a, const double *b) { return {a[0] - b[0], a[1] - b[1], a[2] - b[2]}; }
+inline double a3dot(const A3 &a, const A3 &b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+inline A3 a3cross(const A3 &a, const A3 &b)
+{
+    return {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+}
+inline M3 skew(const A3 &a) { return {0, -a[2], a[1], a[2], 0, -a[0], -a[1], a[0], 0}; }
+inline M3 mmul(const M3 &a, const M3 &b)
+{
+    M3 r{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+            for (int k = 0; k < 3; ++k) r[i * 3 + j] += a[i * 3 + k] * b[k * 3 + j];
+    return r;
+}
+inline M3 mT(const M3 &a) { return {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]}; }
+inline A3 mvec(const M3 &a, const A3 &v)
+{
+    return {a[0] * v[0] + a[1] * v[1] + a[2] * v[2], a[3] * v[0] + a[4] * v[1] + a[5] * v[2],
+            a[6] * v[0] + a[7] * v[1] + a[8] * v[2]};
+}
+inline M3 outer(const A3 &a, const A3 &b)
+{
+    M3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[i * 3 + j] = a[i] * b[j];
+    return r;
+}
+
+// Spring: the reference's own closed form (springEnergy.cpp:101-134).
+void cfSpring(const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    const double k = p[1], L0 = p[0];
+    const A3 s = a3sub(xs, xs + 3);
+    const double n = std::sqrt(a3dot(s, s));
+    const double sg = slot == 0 ? 1.0 : -1.0;
+    for (int i = 0; i < 3; ++i) g[i] = sg * k * (n - L0) * s[i] / n;
+    const A3 sh = {s[0] / n, s[1] / n, s[2] / n};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double sst = sh[i] * sh[j];
+            Hb[i * 3 + j] = k * ((1.0 - L0 / n) * ((i == j ? 1.0 : 0.0) - sst) + sst);
+        }
+}
+
+void cfAnchor(const double *xs, const double *p, int, double *g, double *Hb)
+{
+    for (int i = 0; i < 3; ++i) g[i] = p[3] * (xs[i] - p[i]);
+    for (int i = 0; i < 9; ++i) Hb[i] = (i % 4 == 0) ? p[3] : 0.0;
+}
+
+// StVK membrane: P = F S with S = [[s0, s2], [s2, s1]], s = Keff C; grad_s = P g_s;
+// H_ss = (g_s^T S g_s) I + F (B^T K B) F^T with B = [[g0, 0], [0, g1], [g1, g0]].
+void cfStvk(const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    const double *Di = p, *K = p + 4;
+    const A3 u = a3sub(xs + 3, xs), v = a3sub(xs + 6, xs);
+    A3 f0, f1;
+    for (int i = 0; i < 3; ++i) {
+        f0[i] = u[i] * Di[0] + v[i] * Di[2];
+        f1[i] = u[i] * Di[1] + v[i] * Di[3];
+    }
+    const double C[3] = {0.5 * (a3dot(f0, f0) - 1.0), 0.5 * (a3dot(f1, f1) - 1.0), a3dot(f0, f1)};
+    double sg[3];
+    for (int i = 0; i < 3; ++i) sg[i] = K[i * 3] * C[0] + K[i * 3 + 1] * C[1] + K[i * 3 + 2] * C[2];
+    double gs[2];
+    if (slot == 0) {
+        gs[0] = -(Di[0] + Di[2]);
+        gs[1] = -(Di[1] + Di[3]);
+    } else {
+        gs[0] = Di[(slot - 1) * 2 + 0];
+        gs[1] = Di[(slot - 1) * 2 + 1];
+    }
+    // P g_s = f0 (s0 g0 + s2 g1) + f1 (s2 g0 + s1 g1)
+    const double a0 = sg[0] * gs[0] + sg[2] * gs[1], a1 = sg[2] * gs[0] + sg[1] * gs[1];
+    for (int i = 0; i < 3; ++i) g[i] = f0[i] * a0 + f1[i] * a1;
+    const double gSg = gs[0] * a0 + gs[1] * a1;
+    // B (3x2)
+    const double B[6] = {gs[0], 0.0, 0.0, gs[1], gs[1], gs[0]};
+    double KB[6] = {0};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 3; ++k) KB[i * 2 + j] += K[i * 3 + k] * B[k * 2 + j];
+    double M[4] = {0};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int k = 0; k < 3; ++k) M[i * 2 + j] += B[k * 2 + i] * KB[k * 2 + j];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double FMFt = f0[i] * (M[0] * f0[j] + M[1] * f1[j]) + f1[i] * (M[2] * f0[j] + M[3] * f1[j]);
+            Hb[i * 3 + j] = (i == j ? gSg : 0.0) + FMFt;
+        }
+}
+
+// Stable Neo-Hookean: P = lambda (J - alpha) cof(F) + mu F; grad_s = P g_s;
+// H_ss = mu |g_s|^2 I + lambda c c^T with c = cof(F) g_s (det is affine along a rank-one update,
+// so its second derivative along x_s vanishes).
+void cfSnh(const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    const double *Di = p;
+    const double lambda = p[9], mu = p[10], alpha = p[11];
+    const A3 u = a3sub(xs + 3, xs), v = a3sub(xs + 6, xs), w = a3sub(xs + 9, xs);
+    A3 f[3];
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i) f[c][i] = u[i] * Di[0 * 3 + c] + v[i] * Di[1 * 3 + c] + w[i] * Di[2 * 3 + c];
+    const A3 cof0 = a3cross(f[1], f[2]), cof1 = a3cross(f[2], f[0]), cof2 = a3cross(f[0], f[1]);
+    const double J = a3dot(f[0], cof0);
+    double gs[3];
+    if (slot == 0) {
+        for (int c = 0; c < 3; ++c) gs[c] = -(Di[0 * 3 + c] + Di[1 * 3 + c] + Di[2 * 3 + c]);
+    } else {
+        for (int c = 0; c < 3; ++c) gs[c] = Di[(slot - 1) * 3 + c];
+    }
+    A3 cg, fg;
+    for (int i = 0; i < 3; ++i) {
+        cg[i] = cof0[i] * gs[0] + cof1[i] * gs[1] + cof2[i] * gs[2];
+        fg[i] = f[0][i] * gs[0] + f[1][i] * gs[1] + f[2][i] * gs[2];
+    }
+    const double hyd = lambda * (J - alpha);
+    for (int i = 0; i < 3; ++i) g[i] = hyd * cg[i] + mu * fg[i];
+    const double gg = gs[0] * gs[0] + gs[1] * gs[1] + gs[2] * gs[2];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Hb[i * 3 + j] = (i == j ? mu * gg : 0.0) + lambda * cg[i] * cg[j];
+}
+
+// Discrete bending. With a = e0 x e3, b = e0 x e4, the reference's angle is
+// theta = atan2(s, c), s = -(a x b).e0/|e0|, c = -(a.b) (the common factor 1/(|a||b|) is positive).
+// For the slot's position, a, b, e0 are affine: da = A dx, db = B dx, de0 = sigma dx, so s and c
+// have closed-form gradients and Hessians; theta's follow from the atan2 chain rule.
+void cfBending(const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    const double theta0 = p[0], k = p[1];
+    const double *x0 = xs, *x1 = xs + 3, *x2 = xs + 6, *x3 = xs + 9;
+    const A3 e = a3sub(x1, x0);
+    const A3 a = a3cross(e, a3sub(x2, x1));
+    const A3 b = a3cross(e, a3sub(x3, x1));
+    M3 A{}, B{};
+    double sigma = 0.0;
+    switch (slot) {
+    case 0:
+        A = skew(a3sub(x2, x1));
+        B = skew(a3sub(x3, x1));
+        sigma = -1.0;
+        break;
+    case 1: {
+        const A3 P = a3sub(x2, x0), Q = a3sub(x3, x0);
+        A = skew({-P[0], -P[1], -P[2]});
+        B = skew({-Q[0], -Q[1], -Q[2]});
+        sigma = 1.0;
+        break;
+    }
+    case 2: A = skew(e); break;
+    case 3: B = skew(e); break;
+    }
+    const M3 At = mT(A), Bt = mT(B);
+    // c = -a.b
+    const double c = -a3dot(a, b);
+    A3 gc;
+    {
+        const A3 t1 = mvec(At, b), t2 = mvec(Bt, a);
+        for (int i = 0; i < 3; ++i) gc[i] = -(t1[i] + t2[i]);
+    }
+    M3 Hc;
+    {
+        const M3 m1 = mmul(At, B), m2 = mmul(Bt, A);
+        for (int i = 0; i < 9; ++i) Hc[i] = -(m1[i] + m2[i]);
+    }
+    // T = (a x b).e
+    const A3 axb = a3cross(a, b);
+    const double T = a3dot(axb, e);
+    A3 gT;
+    {
+        const A3 t1 = mvec(At, a3cross(b, e)), t2 = mvec(Bt, a3cross(e, a));
+        for (int i = 0; i < 3; ++i) gT[i] = t1[i] + t2[i] + sigma * axb[i];
+    }
+    M3 HT;
+    {
+        const M3 m1 = mmul(mmul(At, skew(e)), B);
+        const M3 m2 = mmul(At, skew(b));
+        const M3 m3 = mmul(Bt, skew(a));
+        M3 M;
+        for (int i = 0; i < 9; ++i) M[i] = -m1[i] + sigma * m2[i] - sigma * m3[i];
+        const M3 Mt = mT(M);
+        for (int i = 0; i < 9; ++i) HT[i] = M[i] + Mt[i];
+    }
+    // s = -T / L
+    const double L = std::sqrt(a3dot(e, e));
+    const A3 eh = {e[0] / L, e[1] / L, e[2] / L};
+    const double s = -T / L;
+    A3 gs;
+    for (int i = 0; i < 3; ++i) gs[i] = -gT[i] / L + T * sigma * eh[i] / (L * L);
+    M3 Hs;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double id = i == j ? 1.0 : 0.0;
+            Hs[i * 3 + j] = -HT[i * 3 + j] / L + sigma * (gT[i] * eh[j] + eh[i] * gT[j]) / (L * L) -
+                            T * sigma * sigma * (3.0 * eh[i] * eh[j] - id) / (L * L * L);
+        }
+    // theta = atan2(s, c)
+    const double r2 = s * s + c * c, r4 = r2 * r2;
+    const double ts = c / r2, tc = -s / r2, tss = -2.0 * s * c / r4, tcc = 2.0 * s * c / r4,
+                 tsc = (s * s - c * c) / r4;
+    A3 gth;
+    for (int i = 0; i < 3; ++i) gth[i] = ts * gs[i] + tc * gc[i];
+    const double theta = bendingTheta<double>(reinterpret_cast<const V3<double> *>(xs));
+    const double C = theta - theta0;
+    for (int i = 0; i < 3; ++i) g[i] = k * C * gth[i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double hth = ts * Hs[i * 3 + j] + tc * Hc[i * 3 + j] + tss * gs[i] * gs[j] +
+                               tcc * gc[i] * gc[j] + tsc * (gs[i] * gc[j] + gc[i] * gs[j]);
+            Hb[i * 3 + j] = k * (gth[i] * gth[j] + C * hth);
+        }
+}
+
+void cfVertex(int type, const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    switch (type) {
+    case ORC_SPRING: cfSpring(xs, p, slot, g, Hb); return;
+    case ORC_MEMBRANE_STVK: cfStvk(xs, p, slot, g, Hb); return;
+    case ORC_DISCRETE_BENDING: cfBending(xs, p, slot, g, Hb); return;
+    case ORC_STABLE_NEO_HOOKEAN: cfSnh(xs, p, slot, g, Hb); return;
+    case ORC_SPRING_ANCHOR: cfAnchor(xs, p, slot, g, Hb); return;
+    }
+}
+
+// AD per-vertex: full element gradient pass + full element Hessian pass, then slice
+// (energy.h:240-258). The spring keeps its closed-form override (springEnergy.cpp:101-134).
+template <int NS>
+void adVertexN(int type, const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    constexpr int N = 3 * NS;
+    double grad[N];
+    static thread_local std::vector<double> hess(144);
+    adElement<NS, false>(type, xs, p, nullptr, grad, nullptr);
+    adElement<NS, true>(type, xs, p, nullptr, nullptr, hess.data());
+    for (int i = 0; i < 3; ++i) g[i] = grad[3 * slot + i];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Hb[i * 3 + j] = hess[(3 * slot + i) * N + 3 * slot + j];
+}
+void adVertex(int type, const double *xs, const double *p, int slot, double *g, double *Hb)
+{
+    switch (type) {
+    case ORC_SPRING: cfSpring(xs, p, slot, g, Hb); return;
+    case ORC_MEMBRANE_STVK: adVertexN<3>(type, xs, p, slot, g, Hb); return;
+    case ORC_DISCRETE_BENDING: adVertexN<4>(type, xs, p, slot, g, Hb); return;
+    case ORC_STABLE_NEO_HOOKEAN: adVertexN<4>(type, xs, p, slot, g, Hb); return;
+    case ORC_SPRING_ANCHOR: adVertexN<1>(type, xs, p, slot, g, Hb); return;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Object: ParticleGroup + energies + VBD solver state
+// ---------------------------------------------------------------------------------------------
+struct Energy {
+    int type = 0;
+    int ssize = 0;
+    std::vector<int32_t> st;   // flat stencils (Energy::flatStencils, energy.h:102-110)
+    std::vector<double> raw;   // addStencil arguments
+    std::vector<double> prm;   // prepared parameters
+    bool prepared = false;
+    int64_t n() const { return ssize ? static_cast<int64_t>(st.size()) / ssize : 0; }
+};
+
+struct Object {
+    int64_t nv = 0;
+    std::vector<double> x, xInitial, x0, v, m;
+    std::vector<Energy> energies;
+    // VBD precomputation (vbd.cpp:203-268)
+    bool precompDone = false;
+    std::vector<int32_t> colors;
+    std::vector<std::vector<int32_t>> groups;
+    // incidence: per vertex, entries (energy, stencil, slot) in energy-major, ascending stencil order
+    std::vector<int64_t> incPtr;
+    std::vector<std::array<int32_t, 3>> inc;
+};
+
+struct OracleError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Restates the validation in each energy's addStencil (springEnergy.cpp:38-40,
+// discreteBendingEnergy.cpp:38-40) and stores the addStencil arguments; the stiffness-matrix
+// construction of addStencil (membraneStvkEnergy.cpp:35-42, stableNeoHookeanEnergy.cpp:71-79)
+// happens together with preparePrecomputations below.
+void addEnergy(Object &o, int type, int64_t n, const int32_t *st, const double *raw)
+{
+    Energy e;
+    e.type = type;
+    e.ssize = stencilSize(type);
+    if (!e.ssize) throw OracleError("unknown energy type");
+    const int nr = rawParams(type);
+    e.st.assign(st, st + n * e.ssize);
+    e.raw.assign(raw, raw + n * nr);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int s = 0; s < e.ssize; ++s)
+            if (e.st[i * e.ssize + s] < 0 || e.st[i * e.ssize + s] >= o.nv) throw OracleError("stencil index out of range");
+        if (type == ORC_SPRING && e.raw[i * nr] <= 0) throw OracleError("Zero/negative rest length invalid");
+        if (type == ORC_DISCRETE_BENDING && (e.raw[i * nr] < -3.12 || e.raw[i * nr] > 3.12))
+            throw OracleError("Invalid restTheta");
+    }
+    o.energies.push_back(std::move(e));
+}
+
+// preparePrecomputations of each energy (membraneStvkEnergy.cpp:45-71, discreteBendingEnergy.cpp:48-72,
+// stableNeoHookeanEnergy.cpp:82-108; spring/anchor only take the addStencil values, springEnergy.cpp:33-46,
+// springAnchorEnergy.cpp:29-39). Uses the rest positions x0.
+void prepareEnergy(const Object &o, Energy &e)
+{
+    if (e.prepared) throw OracleError("Energy was already prepared");  // energy.h:54-61
+    const int64_t n = e.n();
+    const int np = preparedParams(e.type);
+    e.prm.assign(n * np, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t *s = &e.st[i * e.ssize];
+        const double *r = &e.raw[i * rawParams(e.type)];
+        double *p = &e.prm[i * np];
+        auto X0 = [&](int k) { return &o.x0[3 * static_cast<int64_t>(s[k])]; };
+        switch (e.type) {
+        case ORC_SPRING:
+            p[0] = r[0];
+            p[1] = r[1];
+            break;
+        case ORC_SPRING_ANCHOR:
+            p[0] = r[0];
+            p[1] = r[1];
+            p[2] = r[2];
+            p[3] = r[3];
+            break;
+        case ORC_MEMBRANE_STVK: {
+            const double young = r[0], poisson = r[1];
+            const double yc = young / (1 - poisson * poisson);
+            const double K[9] = {yc, yc * poisson, 0, yc * poisson, yc, 0, 0, 0, yc * (1 - poisson) / 2};
+            const double *a = X0(0), *b = X0(1), *c = X0(2);
+            if (a[2] != 0.0 || b[2] != 0.0 || c[2] != 0.0)
+                throw OracleError("Rest positions of triangles for membrane StVK energy must have coordinate z=0");
+            const double u0 = b[0] - a[0], u1 = b[1] - a[1], v0 = c[0] - a[0], v1 = c[1] - a[1];
+            // Dm = [u v] (columns), inverse of a 2x2
+            const double det = u0 * v1 - v0 * u1;
+            p[0] = v1 / det;
+            p[1] = -v0 / det;
+            p[2] = -u1 / det;
+            p[3] = u0 / det;
+            const double area = (u0 * v1 - v0 * u1) * 0.5;
+            if (area <= 0) throw OracleError("Rest area of membrane StVK triangle is not positive");
+            for (int k = 0; k < 9; ++k) p[4 + k] = K[k] * area;
+            break;
+        }
+        case ORC_DISCRETE_BENDING: {
+            const double *a = X0(0), *b = X0(1), *c = X0(2), *d = X0(3);
+            const A3 e0 = a3sub(b, a), e3 = a3sub(c, b), e4 = a3sub(d, b);
+            const A3 c03 = a3cross(e0, e3), c04 = a3cross(e0, e4);
+            const double factor =
+                2. * 3. * a3dot(e0, e0) / (0.5 * (std::sqrt(a3dot(c03, c03)) + std::sqrt(a3dot(c04, c04))));
+            const double compliance = (1.0 / r[1]) / factor;
+            p[1] = 1.0 / compliance;
+            const double xsr[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+            p[0] = bendingTheta<double>(reinterpret_cast<const V3<double> *>(xsr));
+            break;
+        }
+        case ORC_STABLE_NEO_HOOKEAN: {
+            const double young = r[0], poisson = r[1];
+            const double mu = young / (2 * (1 + poisson));
+            const double lambda = mu / (1 - 2 * poisson);
+            const double *a = X0(0), *b = X0(1), *c = X0(2), *d = X0(3);
+            const A3 u = a3sub(b, a), v = a3sub(c, a), w = a3sub(d, a);
+            // Dm columns u, v, w; Dm(r, c)
+            const double Dm[9] = {u[0], v[0], w[0], u[1], v[1], w[1], u[2], v[2], w[2]};
+            const double c00 = Dm[4] * Dm[8] - Dm[5] * Dm[7];
+            const double c01 = Dm[5] * Dm[6] - Dm[3] * Dm[8];
+            const double c02 = Dm[3] * Dm[7] - Dm[4] * Dm[6];
+            const double det = Dm[0] * c00 + Dm[1] * c01 + Dm[2] * c02;
+            const double volume = 1. / 6. * det;
+            if (volume <= 0) throw OracleError("SNH tetrahedron rest volume is not positive");
+            // inverse = adj / det, adj(i,j) = cofactor(j,i)
+            const double cof[9] = {c00,
+                                   c01,
+                                   c02,
+                                   Dm[2] * Dm[7] - Dm[1] * Dm[8],
+                                   Dm[0] * Dm[8] - Dm[2] * Dm[6],
+                                   Dm[1] * Dm[6] - Dm[0] * Dm[7],
+                                   Dm[1] * Dm[5] - Dm[2] * Dm[4],
+                                   Dm[2] * Dm[3] - Dm[0] * Dm[5],
+                                   Dm[0] * Dm[4] - Dm[1] * Dm[3]};
+            for (int rr = 0; rr < 3; ++rr)
+                for (int cc = 0; cc < 3; ++cc) p[rr * 3 + cc] = cof[cc * 3 + rr] / det;
+            p[9] = lambda * volume;
+            p[10] = mu * volume;
+            p[11] = 1 + mu / lambda;
+            break;
+        }
+        }
+    }
+    e.prepared = true;
+}
+
+// coloring::colorVertices + Graph::greedyColoring (graphColoring.cpp:21-49,72-84): a clique over
+// every stencil of every energy; node 0 gets colour 0; each later node takes the smallest colour
+// not used by an already coloured (lower-index) neighbour. The per-node availability reset is
+// done over the touched colours only (the reference fills a V-sized array), same result.
+std::vector<int32_t> colorVertices(const Object &o)
+{
+    const int64_t nv = o.nv;
+    std::vector<std::vector<int32_t>> adj(nv);
+    for (const auto &e : o.energies) {
+        const int ss = e.ssize;
+        for (int64_t i = 0; i < static_cast<int64_t>(e.st.size()); ++i)
+            for (int64_t j = i + 1; j % ss != 0; ++j) {
+                adj[e.st[i]].push_back(e.st[j]);
+                adj[e.st[j]].push_back(e.st[i]);
+            }
+    }
+    std::vector<int32_t> col(nv, -1);
+    if (nv == 0) return col;
+    col[0] = 0;
+    std::vector<char> used(nv + 1, 0);
+    for (int64_t n = 1; n < nv; ++n) {
+        for (int32_t q : adj[n])
+            if (col[q] != -1) used[col[q]] = 1;
+        int32_t c = 0;
+        while (used[c]) ++c;
+        col[n] = c;
+        for (int32_t q : adj[n])
+            if (col[q] != -1) used[col[q]] = 0;
+    }
+    return col;
+}
+
+void precompute(Object &o)
+{
+    // computeParallelVertexGroups (vbd.cpp:203-229): groups in ascending vertex index
+    o.colors = colorVertices(o);
+    o.groups.clear();
+    for (int64_t i = 0; i < o.nv; ++i) {
+        const int32_t c = o.colors[i];
+        if (static_cast<int32_t>(o.groups.size()) < c + 1) o.groups.resize(c + 1);
+        o.groups[c].push_back(static_cast<int32_t>(i));
+    }
+    // computeStencilInfoPerVertex (vbd.cpp:231-268): per vertex, per energy, (s/size, s%size) ascending s
+    std::vector<std::vector<std::array<int32_t, 3>>> per(o.nv);
+    for (int32_t ei = 0; ei < static_cast<int32_t>(o.energies.size()); ++ei) {
+        const auto &e = o.energies[ei];
+        for (int64_t s = 0; s < static_cast<int64_t>(e.st.size()); ++s)
+            per[e.st[s]].push_back({ei, static_cast<int32_t>(s / e.ssize), static_cast<int32_t>(s % e.ssize)});
+    }
+    o.incPtr.assign(o.nv + 1, 0);
+    o.inc.clear();
+    for (int64_t v = 0; v < o.nv; ++v) {
+        o.incPtr[v] = static_cast<int64_t>(o.inc.size());
+        o.inc.insert(o.inc.end(), per[v].begin(), per[v].end());
+    }
+    o.incPtr[o.nv] = static_cast<int64_t>(o.inc.size());
+    o.precompDone = true;
+}
+
+void gatherStencil(const Object &o, const double *x, const Energy &e, int64_t si, double *xs)
+{
+    for (int k = 0; k < e.ssize; ++k) {
+        const int64_t vi = e.st[si * e.ssize + k];
+        xs[3 * k + 0] = x[3 * vi + 0];
+        xs[3 * k + 1] = x[3 * vi + 1];
+        xs[3 * k + 2] = x[3 * vi + 2];
+    }
+}
+
+// l_vertexDescent (vbd.cpp:99-147) for one particle.
+void vertexDescent(Object &o, int64_t vi, const std::vector<double> &xt, double invdt2, int mode)
+{
+    const double mdt2 = o.m[vi] * invdt2;
+    double f[3], H[9] = {0};
+    for (int i = 0; i < 3; ++i) f[i] = -mdt2 * (o.x[3 * vi + i] - xt[3 * vi + i]);
+    H[0] = H[4] = H[8] = mdt2;
+    double xs[12], g[3], Hb[9];
+    for (int64_t k = o.incPtr[vi]; k < o.incPtr[vi + 1]; ++k) {
+        const auto &en = o.inc[k];
+        const Energy &e = o.energies[en[0]];
+        gatherStencil(o, o.x.data(), e, en[1], xs);
+        const double *p = &e.prm[static_cast<int64_t>(en[1]) * preparedParams(e.type)];
+        if (mode == ORC_MODE_AD)
+            adVertex(e.type, xs, p, en[2], g, Hb);
+        else
+            cfVertex(e.type, xs, p, en[2], g, Hb);
+        for (int i = 0; i < 3; ++i) f[i] -= g[i];
+        for (int i = 0; i < 9; ++i) H[i] += Hb[i];
+    }
+    // H.determinant() > 1e-10 (vbd.cpp:22,139); cofactor expansion and adjugate inverse
+    const double c00 = H[4] * H[8] - H[5] * H[7];
+    const double c10 = H[5] * H[6] - H[3] * H[8];
+    const double c20 = H[3] * H[7] - H[4] * H[6];
+    const double det = H[0] * c00 + H[1] * c10 + H[2] * c20;
+    if (det > 1e-10) {
+        const double inv[9] = {c00,
+                               H[2] * H[7] - H[1] * H[8],
+                               H[1] * H[5] - H[2] * H[4],
+                               c10,
+                               H[0] * H[8] - H[2] * H[6],
+                               H[2] * H[3] - H[0] * H[5],
+                               c20,
+                               H[1] * H[6] - H[0] * H[7],
+                               H[0] * H[4] - H[1] * H[3]};
+        for (int i = 0; i < 3; ++i)
+            o.x[3 * vi + i] += (inv[i * 3] * f[0] + inv[i * 3 + 1] * f[1] + inv[i * 3 + 2] * f[2]) / det;
+    }
+}
+
+void sweep(Object &o, const std::vector<double> &xt, double invdt2, int mode)
+{
+    // colour groups in order, OpenMP over the vertices of a group (vbd.cpp:148-158)
+    for (const auto &grp : o.groups) {
+        const int64_t n = static_cast<int64_t>(grp.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) vertexDescent(o, grp[i], xt, invdt2, mode);
+    }
+}
+
+void residual(Object &o, const double *x, const double *xt, double dt, int mode, double *norm, double *rout)
+{
+    // r = -grad E(x) - (M/h^2)(x - xt) (implicitEulerBase.cpp:102-143 with x-form of
+    // implicitEulerNewtonRobust.cpp:65-66), gathered per vertex over the incidence lists.
+    if (!o.precompDone) precompute(o);
+    const double invdt2 = 1.0 / (dt * dt);
+    double sum = 0.0;
+#pragma omp parallel for reduction(+ : sum) schedule(static)
+    for (int64_t vi = 0; vi < o.nv; ++vi) {
+        double r[3];
+        for (int i = 0; i < 3; ++i) r[i] = -o.m[vi] * invdt2 * (x[3 * vi + i] - xt[3 * vi + i]);
+        double xs[12], g[3], Hb[9];
+        for (int64_t k = o.incPtr[vi]; k < o.incPtr[vi + 1]; ++k) {
+            const auto &en = o.inc[k];
+            const Energy &e = o.energies[en[0]];
+            gatherStencil(o, x, e, en[1], xs);
+            const double *p = &e.prm[static_cast<int64_t>(en[1]) * preparedParams(e.type)];
+            if (mode == ORC_MODE_AD)
+                adVertex(e.type, xs, p, en[2], g, Hb);
+            else
+                cfVertex(e.type, xs, p, en[2], g, Hb);
+            for (int i = 0; i < 3; ++i) r[i] -= g[i];
+        }
+        if (rout)
+            for (int i = 0; i < 3; ++i) rout[3 * vi + i] = r[i];
+        sum += r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+    }
+    *norm = std::sqrt(sum);
+}
+
+void checkPrepared(const Object &o)
+{
+    for (const auto &e : o.energies)
+        if (!e.prepared) throw OracleError("energy not prepared");
+}
+
+// One substep of VBD::step (vbd.cpp:46-185).
+void substep(Object &o, double h, int iterations, const double *g, int mode, const double *startGuess,
+             double *rhoCurve)
+{
+    const int64_t nv = o.nv;
+    std::vector<double> xOld(o.x), xt(3 * nv);
+    for (int64_t i = 0; i < nv; ++i)
+        for (int d = 0; d < 3; ++d) {
+            o.v[3 * i + d] += h * g[d];             // vbd.cpp:68 (every particle, also m = 0)
+            o.x[3 * i + d] += o.v[3 * i + d] * h;   // particleGroup.cpp:57-63
+            xt[3 * i + d] = o.x[3 * i + d];
+        }
+    double rho0 = 0.0;
+    if (rhoCurve) {
+        residual(o, xt.data(), xt.data(), h, mode, &rho0, nullptr);
+        if (rho0 == 0.0) rho0 = 1.0;
+    }
+    if (startGuess) std::copy(startGuess, startGuess + 3 * nv, o.x.begin());
+    const double invdt2 = 1.0 / (h * h);
+    for (int it = 0; it < iterations; ++it) {
+        sweep(o, xt, invdt2, mode);
+        if (rhoCurve) {
+            double r;
+            residual(o, o.x.data(), xt.data(), h, mode, &r, nullptr);
+            rhoCurve[it] = r / rho0;
+        }
+    }
+    const double invdt = 1.0 / h;
+    for (int64_t i = 0; i < 3 * nv; ++i) o.v[i] = (o.x[i] - xOld[i]) * invdt;  // particleGroup.cpp:65-72
+}
+
+template <class F>
+int guard(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+int orc_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+}
+
+void *orc_new(int64_t nv, const double *x, const double *x0, const double *m)
+{
+    auto *o = new Object;
+    o->nv = nv;
+    o->x.assign(x, x + 3 * nv);
+    o->xInitial = o->x;
+    o->x0.assign(x0, x0 + 3 * nv);
+    o->v.assign(3 * nv, 0.0);
+    o->m.assign(m, m + nv);
+    return o;
+}
+
+void orc_free(void *h) { delete static_cast<Object *>(h); }
+
+int orc_add_energy(void *h, int type, int64_t n, const int32_t *stencils, const double *raw)
+{
+    return guard([&] {
+        addEnergy(*static_cast<Object *>(h), type, n, stencils, raw);
+        static_cast<Object *>(h)->precompDone = false;
+    });
+}
+
+int orc_prepare(void *h)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        for (auto &e : o.energies)
+            if (!e.prepared) prepareEnergy(o, e);
+    });
+}
+
+int orc_num_prepared_params(int type) { return preparedParams(type); }
+
+int orc_get_prepared(void *h, int energy, double *out)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (energy < 0 || energy >= static_cast<int>(o.energies.size())) throw OracleError("bad energy index");
+        const auto &e = o.energies[energy];
+        if (!e.prepared) throw OracleError("energy not prepared");
+        std::copy(e.prm.begin(), e.prm.end(), out);
+    });
+}
+
+int orc_set_prepared(void *h, int energy, const double *in)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (energy < 0 || energy >= static_cast<int>(o.energies.size())) throw OracleError("bad energy index");
+        auto &e = o.energies[energy];
+        e.prm.assign(in, in + e.n() * preparedParams(e.type));
+        e.prepared = true;
+    });
+}
+
+int orc_colors(void *h, int32_t *out, int32_t *ncolors)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (!o.precompDone) precompute(o);
+        if (out) std::copy(o.colors.begin(), o.colors.end(), out);
+        if (ncolors) *ncolors = static_cast<int32_t>(o.groups.size());
+    });
+}
+
+int orc_incidence_count(void *h, int64_t *count)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (!o.precompDone) precompute(o);
+        *count = static_cast<int64_t>(o.inc.size());
+    });
+}
+
+int orc_incidence(void *h, int32_t *vertex, int32_t *energy, int32_t *stencil, int32_t *slot)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (!o.precompDone) precompute(o);
+        for (int64_t v = 0; v < o.nv; ++v)
+            for (int64_t k = o.incPtr[v]; k < o.incPtr[v + 1]; ++k) {
+                vertex[k] = static_cast<int32_t>(v);
+                energy[k] = o.inc[k][0];
+                stencil[k] = o.inc[k][1];
+                slot[k] = o.inc[k][2];
+            }
+    });
+}
+
+int orc_step(void *h, double dt, int nsubsteps, int iterations, const double *gravity, int mode)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        if (!o.precompDone) precompute(o);
+        const double hh = dt / nsubsteps;  // vbd.cpp:19
+        for (int s = 0; s < nsubsteps; ++s) substep(o, hh, iterations, gravity, mode, nullptr, nullptr);
+    });
+}
+
+int orc_substep_curve(void *h, double dt, int iterations, const double *gravity, int mode, const double *start_guess,
+                      double *rho_per_iter)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        if (!o.precompDone) precompute(o);
+        substep(o, dt, iterations, gravity, mode, start_guess, rho_per_iter);
+    });
+}
+
+int orc_get_state(void *h, double *x, double *v)
+{
+    auto &o = *static_cast<Object *>(h);
+    if (x) std::copy(o.x.begin(), o.x.end(), x);
+    if (v) std::copy(o.v.begin(), o.v.end(), v);
+    return 0;
+}
+
+int orc_set_state(void *h, const double *x, const double *v)
+{
+    auto &o = *static_cast<Object *>(h);
+    if (x) std::copy(x, x + 3 * o.nv, o.x.begin());
+    if (v) std::copy(v, v + 3 * o.nv, o.v.begin());
+    return 0;
+}
+
+int orc_reset(void *h)
+{
+    // ParticleGroup::reset (particleGroup.cpp:97-103) + VBD::reset (vbd.cpp:197-201)
+    auto &o = *static_cast<Object *>(h);
+    o.x = o.xInitial;
+    std::fill(o.v.begin(), o.v.end(), 0.0);
+    o.precompDone = false;
+    return 0;
+}
+
+int orc_residual(void *h, const double *x, const double *xt, double dt, int mode, double *norm, double *r_out)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        residual(o, x, xt, dt, mode, norm, r_out);
+    });
+}
+
+int orc_energy(void *h, const double *x, double *eout)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        double sum = 0.0;
+        for (const auto &e : o.energies) {
+            const int np = preparedParams(e.type);
+            for (int64_t i = 0; i < e.n(); ++i) {
+                double xs[12];
+                gatherStencil(o, x, e, i, xs);
+                sum += evalEnergy<double>(e.type, reinterpret_cast<const V3<double> *>(xs), &e.prm[i * np]);
+            }
+        }
+        *eout = sum;
+    });
+}
+
+int orc_element(int type, const double *xs, const double *prepared, int mode, double *energy, double *grad,
+                double *diag_blocks, double *full_hess)
+{
+    return guard([&] {
+        const int ns = stencilSize(type);
+        if (!ns) throw OracleError("unknown energy type");
+        if (energy) *energy = evalEnergy<double>(type, reinterpret_cast<const V3<double> *>(xs), prepared);
+        for (int s = 0; s < ns; ++s) {
+            double g[3], Hb[9];
+            if (mode == ORC_MODE_AD)
+                adVertex(type, xs, prepared, s, g, Hb);
+            else
+                cfVertex(type, xs, prepared, s, g, Hb);
+            if (grad)
+                for (int i = 0; i < 3; ++i) grad[3 * s + i] = g[i];
+            if (diag_blocks)
+                for (int i = 0; i < 9; ++i) diag_blocks[9 * s + i] = Hb[i];
+        }
+        if (full_hess && mode == ORC_MODE_AD) {
+            switch (ns) {
+            case 1: adElement<1, true>(type, xs, prepared, nullptr, nullptr, full_hess); break;
+            case 2: adElement<2, true>(type, xs, prepared, nullptr, nullptr, full_hess); break;
+            case 3: adElement<3, true>(type, xs, prepared, nullptr, nullptr, full_hess); break;
+            case 4: adElement<4, true>(type, xs, prepared, nullptr, nullptr, full_hess); break;
+            }
+        }
+    });
+}
+
+int orc_bending_stencils(int64_t nf, const int32_t *faces, int64_t nv, int32_t *out, int64_t *count)
+{
+    // TriangleMesh constructor (triangleMesh.h:16-50) + createCloth's bending wiring
+    // (apps/solverCompare.cpp:441-452: edges with exactly two opposing vertices).
+    return guard([&] {
+        std::map<std::array<int32_t, 2>, std::vector<int32_t>> edges;
+        for (int64_t f = 0; f < nf; ++f) {
+            const int32_t *fc = faces + 3 * f;
+            for (int i = 0; i < 3; ++i)
+                if (fc[i] < 0 || fc[i] >= nv) throw OracleError("Triangle mesh face contains index to invalid vertex");
+            if (fc[0] == fc[1] || fc[1] == fc[2] || fc[2] == fc[0])
+                throw OracleError("Triangle mesh face contains repeated vertex index");
+            for (int i = 0; i < 3; ++i) {
+                const int32_t a = fc[i], b = fc[(i + 1) % 3], c = fc[(i + 2) % 3];
+                edges[{std::min(a, b), std::max(a, b)}].push_back(c);
+            }
+        }
+        int64_t k = 0;
+        for (const auto &kv : edges) {
+            if (kv.second.size() != 2) continue;
+            if (out) {
+                out[4 * k + 0] = kv.first[0];
+                out[4 * k + 1] = kv.first[1];
+                out[4 * k + 2] = kv.second[0];
+                out[4 * k + 3] = kv.second[1];
+            }
+            ++k;
+        }
+        *count = k;
+    });
+}
+
+int orc_coloring_valid(void *h, const int32_t *colors)
+{
+    auto &o = *static_cast<Object *>(h);
+    for (const auto &e : o.energies)
+        for (int64_t i = 0; i < e.n(); ++i)
+            for (int a = 0; a < e.ssize; ++a)
+                for (int b = a + 1; b < e.ssize; ++b) {
+                    const int32_t va = e.st[i * e.ssize + a], vb = e.st[i * e.ssize + b];
+                    if (va != vb && colors[va] == colors[vb]) return 0;
+                }
+    return 1;
+}
+
+}  // extern "C"

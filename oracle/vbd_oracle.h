@@ -1,0 +1,91 @@
+/*
+ * FP64 CPU oracle for the spg Vertex Block Descent step — TEST INFRASTRUCTURE ONLY.
+ *
+ * This library is the checker, not the product. Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it. The product (libspg_vbd.so / libspg_host.so)
+ * never links, loads or calls it.
+ *
+ * It restates, in plain FP64 C++ with OpenMP, the reference algorithm of
+ *   spg/source/sim/solver/vbd.cpp:12-268            (VBD::step, colouring groups, incidence lists)
+ *   spg/include/spg/sim/energy/energy.h:240-270,336-362 (per-vertex force/Hessian slicing, TinyAD path)
+ *   spg/source/sim/energy/{spring,membraneStvk,discreteBending,stableNeoHookean,springAnchor}Energy.cpp
+ *   spg/source/sim/simObject/particleGroup.cpp:57-82,97-103
+ *   spg/source/utils/graphColoring.cpp:21-49,72-84
+ *   spg/include/spg/geom/triangleMesh.h:37-50       (edge map order, opposing vertices)
+ *   spg/source/sim/solver/implicitEulerBase.cpp:102-143 (system force, residual convention)
+ * with two derivative back-ends: closed-form per-vertex blocks, and the oracle's own
+ * forward-mode second-order dual numbers that evaluate the full element derivative twice per
+ * incidence, as the reference does through TinyAD.
+ *
+ * Raw per-stencil parameters (what each energy's addStencil receives):
+ *   ORC_SPRING           [restLength, stiffness]                 stencil size 2
+ *   ORC_MEMBRANE_STVK    [young, poisson]                        stencil size 3
+ *   ORC_DISCRETE_BENDING [restTheta, stiffness]                  stencil size 4
+ *   ORC_STABLE_NEO_HOOKEAN [young, poisson]                      stencil size 4
+ *   ORC_SPRING_ANCHOR    [ax, ay, az, stiffness]                 stencil size 1
+ * Prepared per-stencil parameters (what the energy accessors expose after preparePrecomputations):
+ *   SPRING   [restLength, k]                       (restLengths(), effectiveStiffness())
+ *   STVK     [Dinv 2x2 row-major (4), Keff 3x3 row-major (9)]  (inverseReferenceMats(), effectiveStiffness())
+ *   BENDING  [restTheta, keff]                     (restThetas(), effectiveStiffness())
+ *   SNH      [Dinv 3x3 row-major (9), lambda, mu, alpha]       (inverseReferenceMats(), effectiveStiffness(),
+ *                                                   hydrostaticCorrectionTerms())
+ *   ANCHOR   [ax, ay, az, k]                       (anchors(), modelStiffness())
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_SPRING = 1,
+    ORC_MEMBRANE_STVK = 2,
+    ORC_DISCRETE_BENDING = 3,
+    ORC_STABLE_NEO_HOOKEAN = 4,
+    ORC_SPRING_ANCHOR = 5
+};
+enum { ORC_MODE_CLOSED_FORM = 0, ORC_MODE_AD = 1 };
+
+const char *orc_last_error(void);
+
+/* object lifetime (a ParticleGroup plus its energies, plus the VBD solver state) */
+void *orc_new(int64_t nv, const double *x, const double *x0, const double *m);
+void orc_free(void *h);
+int orc_add_energy(void *h, int type, int64_t n, const int32_t *stencils, const double *raw);
+int orc_prepare(void *h);
+int orc_num_prepared_params(int type);
+int orc_get_prepared(void *h, int energy, double *out);
+int orc_set_prepared(void *h, int energy, const double *in);
+int orc_set_threads(int n);
+
+/* solver */
+int orc_colors(void *h, int32_t *out, int32_t *ncolors);
+int orc_incidence_count(void *h, int64_t *count);
+int orc_incidence(void *h, int32_t *vertex, int32_t *energy, int32_t *stencil, int32_t *slot);
+int orc_step(void *h, double dt, int nsubsteps, int iterations, const double *gravity, int mode);
+int orc_substep_curve(void *h, double dt, int iterations, const double *gravity, int mode,
+                      const double *start_guess, double *rho_per_iter);
+int orc_get_state(void *h, double *x, double *v);
+int orc_set_state(void *h, const double *x, const double *v);
+int orc_reset(void *h);
+
+/* residual r(x) = -grad E(x) - (M/h^2)(x - xt), returns ||r||_2; r_out may be NULL */
+int orc_residual(void *h, const double *x, const double *xt, double dt, int mode, double *norm, double *r_out);
+int orc_energy(void *h, const double *x, double *e);
+
+/* single element: value, full gradient (3N) and the N diagonal 3x3 Hessian blocks (N*9, row-major);
+   full_hess (3N x 3N) only in AD mode, may be NULL */
+int orc_element(int type, const double *xs, const double *prepared, int mode, double *energy,
+                double *grad, double *diag_blocks, double *full_hess);
+
+/* TriangleMesh restatement: bending stencils {e0, e1, opp0, opp1} for edges with two opposite vertices,
+   in std::map<(min,max)> order. Call with out=NULL to get the count. */
+int orc_bending_stencils(int64_t nf, const int32_t *faces, int64_t nv, int32_t *out, int64_t *count);
+
+/* brute force colouring check: 1 if no stencil has two vertices of the same colour */
+int orc_coloring_valid(void *h, const int32_t *colors);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,96 @@
+// Mirror of spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41,
+// source/sim/solver/baseSolver.cpp:10-35) for particle groups, and the GPU drop-in for
+// spg::solver::VBD (spg/include/spg/sim/solver/vbd.h:14-41).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include <spg/sim/simObject/particleGroup.h>
+#include <spg/types.h>
+
+struct spg_vbd_ctx;
+
+namespace spg {
+
+enum class Verbosity { None, Performance, Debug };  // spg/include/spg/constants.h:5
+
+namespace solver {
+
+class BaseSolver
+{
+public:
+    virtual ~BaseSolver() = default;
+    virtual void step() = 0;
+    // copies the group; its energies stay shared (baseSolver.cpp:10-14)
+    void addObject(const ParticleGroup &object) { m_objects.push_back(object); }
+    std::vector<ParticleGroup> &particleGroups() { return m_objects; }
+    const std::vector<ParticleGroup> &particleGroups() const { return m_objects; }
+    int numSimObjects() const { return static_cast<int>(m_objects.size()); }
+    void setDt(Real dt) { m_dtStep = dt; }
+    Real dt() const { return m_dtStep; }
+    void setNumSubsteps(int nsubsteps) { m_nsubsteps = nsubsteps; }
+    int numSubsteps() const { return m_nsubsteps; }
+    virtual void reset();
+    void setVerbosity(Verbosity verbosity) { m_verbosity = verbosity; }
+    void setGravity(const Vector3 &gravity) { m_gravity = gravity; }
+
+protected:
+    std::vector<ParticleGroup> m_objects;
+    Vector3 m_gravity{0, -9.8, 0};
+    Real m_time{0};
+    Real m_dtStep{0.01};
+    int m_nsubsteps{1};
+    Verbosity m_verbosity{Verbosity::None};
+};
+
+// VBD on one MI355X. Same call pattern as spg::solver::VBD: add objects, set dt/substeps/gravity,
+// call step() in a loop. The first step (or the first after reset()/requirePrecomputationUpdate())
+// colours the graph, builds the incidence streams and uploads them (vbd.cpp:14-18).
+//
+// Every object's energies must be among SpringEnergy, MembraneStvkEnergy, DiscreteBendingEnergy,
+// StableNeoHookeanEnergy and SpringAnchorEnergy; any other energy type throws std::runtime_error at
+// the first step (there is no CPU fallback).
+//
+// State ownership: by default the host ParticleGroups stay the source of truth, as with the CPU
+// solver: step() uploads their positions and velocities, advances on the GPU and writes the result
+// back. setResidentState(true) keeps the state on the GPU between steps (no per-step transfers);
+// syncToHost() then copies it into particleGroups() on demand, and markHostStateDirty() uploads host
+// edits before the next step.
+class VBDGpu : public BaseSolver
+{
+public:
+    enum class InitialGuessType { InertialWithAcceleration };
+    explicit VBDGpu(int device = 0);
+    ~VBDGpu() override;
+    VBDGpu(const VBDGpu &) = delete;
+    VBDGpu &operator=(const VBDGpu &) = delete;
+
+    void step() override;
+    void reset() override;
+    void setInitialGuessType(const InitialGuessType type) { m_initialGuessType = type; }
+    void requirePrecomputationUpdate() { m_precomputationUpdateRequired = true; }
+    // m_iterations of the reference (vbd.h:33) has no setter there; here it has one (default 1).
+    void setIterations(int iterations) { m_iterations = iterations; }
+    int iterations() const { return m_iterations; }
+    void setResidentState(bool resident) { m_resident = resident; }
+    void syncToHost();
+    void markHostStateDirty() { m_hostDirty = true; }
+    // The C-ABI context (include/spg_vbd.h) behind this solver, or nullptr before the first step.
+    spg_vbd_ctx *context() { return m_ctx; }
+
+protected:
+    void buildContext();
+    void uploadState();
+
+    InitialGuessType m_initialGuessType{InitialGuessType::InertialWithAcceleration};
+    int m_iterations{1};
+    int m_device{0};
+    bool m_precomputationUpdateRequired{true};
+    bool m_resident{false};
+    bool m_hostDirty{true};
+    spg_vbd_ctx *m_ctx{nullptr};
+};
+
+}  // namespace solver
+}  // namespace spg

@@ -1,0 +1,282 @@
+// C binding of the C++ spg mirror (include/spg_host.h).
+#include <memory>
+#include <string>
+
+#include <spg/geom/triangleMesh.h>
+#include <spg/sim/energy/energies.h>
+#include <spg/sim/solver/baseSolver.h>
+
+#include "spg_host.h"
+
+struct spg_pgroup {
+    spg::ParticleGroup group;
+};
+
+struct spg_solver {
+    std::unique_ptr<spg::solver::VBDGpu> solver;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+
+spg::Vector3 v3(const double *p) { return {p[0], p[1], p[2]}; }
+
+template <class E, int N>
+std::array<int, N> stencilAt(const int32_t *s, int64_t i)
+{
+    std::array<int, N> a;
+    for (int k = 0; k < N; ++k) a[k] = s[i * N + k];
+    return a;
+}
+
+void checkIndex(const spg_pgroup *pg, int32_t index)
+{
+    if (!pg || index < 0 || index >= static_cast<int32_t>(pg->group.energies().size()))
+        throw std::runtime_error("energy index out of range");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *spg_host_last_error(void) { return g_err.c_str(); }
+
+spg_pgroup *spg_pg_create(void) { return new spg_pgroup; }
+void spg_pg_destroy(spg_pgroup *pg) { delete pg; }
+
+int spg_pg_add_particles(spg_pgroup *pg, int64_t n, const double *x, const double *x0, const double *mass)
+{
+    return guarded([&] {
+        if (!pg || n < 0 || (n > 0 && (!x || !x0 || !mass))) throw std::runtime_error("invalid arguments");
+        for (int64_t i = 0; i < n; ++i) pg->group.addParticle(v3(x + 3 * i), v3(x0 + 3 * i), mass[i]);
+    });
+}
+
+int spg_pg_add_energy(spg_pgroup *pg, int32_t type, int64_t n, const int32_t *st, const double *a, int32_t prepare,
+                      int32_t *index)
+{
+    return guarded([&] {
+        if (!pg || n < 0 || (n > 0 && (!st || !a))) throw std::runtime_error("invalid arguments");
+        std::shared_ptr<spg::ParticleGroup::EnergyT> energy;
+        switch (type) {
+        case SPG_VBD_SPRING: {
+            auto e = std::make_shared<spg::SpringEnergy>();
+            for (int64_t i = 0; i < n; ++i) e->addStencil(stencilAt<spg::SpringEnergy, 2>(st, i), a[2 * i], a[2 * i + 1]);
+            energy = e;
+            break;
+        }
+        case SPG_VBD_MEMBRANE_STVK: {
+            auto e = std::make_shared<spg::MembraneStvkEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                e->addStencil(stencilAt<spg::MembraneStvkEnergy, 3>(st, i), a[2 * i], a[2 * i + 1]);
+            energy = e;
+            break;
+        }
+        case SPG_VBD_DISCRETE_BENDING: {
+            auto e = std::make_shared<spg::DiscreteBendingEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                e->addStencil(stencilAt<spg::DiscreteBendingEnergy, 4>(st, i), a[2 * i], a[2 * i + 1]);
+            energy = e;
+            break;
+        }
+        case SPG_VBD_STABLE_NEO_HOOKEAN: {
+            auto e = std::make_shared<spg::StableNeoHookeanEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                e->addStencil(stencilAt<spg::StableNeoHookeanEnergy, 4>(st, i), a[2 * i], a[2 * i + 1]);
+            energy = e;
+            break;
+        }
+        case SPG_VBD_SPRING_ANCHOR: {
+            auto e = std::make_shared<spg::SpringAnchorEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                e->addStencil(stencilAt<spg::SpringAnchorEnergy, 1>(st, i), v3(a + 4 * i), a[4 * i + 3]);
+            energy = e;
+            break;
+        }
+        default: throw std::runtime_error("unsupported energy type");
+        }
+        for (int64_t i = 0; i < n * energy->stencilSize(); ++i)
+            if (st[i] < 0 || st[i] >= pg->group.size()) throw std::runtime_error("stencil index out of range");
+        if (prepare) energy->preparePrecomputations(pg->group);
+        pg->group.addEnergy(energy);
+        if (index) *index = static_cast<int32_t>(pg->group.energies().size()) - 1;
+    });
+}
+
+int spg_pg_prepare_energy(spg_pgroup *pg, int32_t index)
+{
+    return guarded([&] {
+        checkIndex(pg, index);
+        pg->group.energies()[index]->preparePrecomputations(pg->group);
+    });
+}
+
+int spg_pg_num_particles(const spg_pgroup *pg, int64_t *n)
+{
+    return guarded([&] {
+        if (!pg || !n) throw std::runtime_error("invalid arguments");
+        *n = pg->group.size();
+    });
+}
+
+int spg_pg_energy_params(const spg_pgroup *pg, int32_t index, double *out)
+{
+    return guarded([&] {
+        checkIndex(pg, index);
+        const auto *energy = pg->group.energies()[index].get();
+        size_t k = 0;
+        if (auto *e = dynamic_cast<const spg::SpringEnergy *>(energy)) {
+            for (int i = 0; i < e->nStencils(); ++i) {
+                out[k++] = e->restLengths()[i];
+                out[k++] = e->effectiveStiffness()[i](0, 0);
+            }
+        } else if (auto *e = dynamic_cast<const spg::MembraneStvkEnergy *>(energy)) {
+            for (int i = 0; i < e->nStencils(); ++i) {
+                for (double v : e->inverseReferenceMats()[i].m) out[k++] = v;
+                for (double v : e->effectiveStiffness()[i].m) out[k++] = v;
+            }
+        } else if (auto *e = dynamic_cast<const spg::DiscreteBendingEnergy *>(energy)) {
+            for (int i = 0; i < e->nStencils(); ++i) {
+                out[k++] = e->restThetas()[i];
+                out[k++] = e->effectiveStiffness()[i](0, 0);
+            }
+        } else if (auto *e = dynamic_cast<const spg::StableNeoHookeanEnergy *>(energy)) {
+            for (int i = 0; i < e->nStencils(); ++i) {
+                for (double v : e->inverseReferenceMats()[i].m) out[k++] = v;
+                out[k++] = e->effectiveStiffness()[i](0, 0);
+                out[k++] = e->effectiveStiffness()[i](1, 1);
+                out[k++] = e->hydrostaticCorrectionTerms()[i];
+            }
+        } else if (auto *e = dynamic_cast<const spg::SpringAnchorEnergy *>(energy)) {
+            for (int i = 0; i < e->nStencils(); ++i) {
+                for (int d = 0; d < 3; ++d) out[k++] = e->anchors()[i][d];
+                out[k++] = e->modelStiffness()[i](0, 0);
+            }
+        } else {
+            throw std::runtime_error("unsupported energy type");
+        }
+    });
+}
+
+int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v)
+{
+    return guarded([&] {
+        if (!pg) throw std::runtime_error("invalid arguments");
+        for (int i = 0; i < pg->group.size(); ++i)
+            for (int d = 0; d < 3; ++d) {
+                if (x) x[3 * i + d] = pg->group.positions()[i][d];
+                if (v) v[3 * i + d] = pg->group.velocities()[i][d];
+            }
+    });
+}
+
+int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v)
+{
+    return guarded([&] {
+        if (!pg) throw std::runtime_error("invalid arguments");
+        for (int i = 0; i < pg->group.size(); ++i)
+            for (int d = 0; d < 3; ++d) {
+                if (x) pg->group.positions()[i][d] = x[3 * i + d];
+                if (v) pg->group.velocities()[i][d] = v[3 * i + d];
+            }
+    });
+}
+
+int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int32_t *faces, int32_t *out,
+                                 int64_t *count)
+{
+    return guarded([&] {
+        if (n_vertices < 0 || n_faces < 0 || (n_faces > 0 && !faces) || !count)
+            throw std::runtime_error("invalid arguments");
+        std::vector<spg::Vector3> verts(n_vertices);
+        std::vector<spg::Int3> f(n_faces);
+        for (int64_t i = 0; i < n_faces; ++i) f[i] = {faces[3 * i], faces[3 * i + 1], faces[3 * i + 2]};
+        const spg::TriangleMesh mesh(verts, verts, f);
+        const auto st = mesh.bendingStencils();
+        *count = static_cast<int64_t>(st.size());
+        if (out)
+            for (size_t i = 0; i < st.size(); ++i)
+                for (int k = 0; k < 4; ++k) out[4 * i + k] = st[i][k];
+    });
+}
+
+spg_solver *spg_solver_create_vbd_gpu(int32_t device)
+{
+    auto *s = new spg_solver;
+    s->solver = std::make_unique<spg::solver::VBDGpu>(device);
+    return s;
+}
+
+void spg_solver_destroy(spg_solver *s) { delete s; }
+
+int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg)
+{
+    return guarded([&] {
+        if (!s || !pg) throw std::runtime_error("invalid arguments");
+        s->solver->addObject(pg->group);
+        s->solver->requirePrecomputationUpdate();
+    });
+}
+
+int spg_solver_set_dt(spg_solver *s, double dt)
+{
+    return guarded([&] { s->solver->setDt(dt); });
+}
+int spg_solver_set_num_substeps(spg_solver *s, int32_t n)
+{
+    return guarded([&] { s->solver->setNumSubsteps(n); });
+}
+int spg_solver_set_gravity(spg_solver *s, const double g[3])
+{
+    return guarded([&] { s->solver->setGravity(spg::Vector3{g[0], g[1], g[2]}); });
+}
+int spg_solver_set_iterations(spg_solver *s, int32_t iterations)
+{
+    return guarded([&] { s->solver->setIterations(iterations); });
+}
+int spg_solver_set_resident_state(spg_solver *s, int32_t resident)
+{
+    return guarded([&] { s->solver->setResidentState(resident != 0); });
+}
+int spg_solver_step(spg_solver *s)
+{
+    return guarded([&] { s->solver->step(); });
+}
+int spg_solver_reset(spg_solver *s)
+{
+    return guarded([&] { s->solver->reset(); });
+}
+int spg_solver_sync_to_host(spg_solver *s)
+{
+    return guarded([&] { s->solver->syncToHost(); });
+}
+
+int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v)
+{
+    return guarded([&] {
+        if (!s || object < 0 || object >= s->solver->numSimObjects()) throw std::runtime_error("invalid object");
+        const auto &pg = s->solver->particleGroups()[object];
+        for (int i = 0; i < pg.size(); ++i)
+            for (int d = 0; d < 3; ++d) {
+                if (x) x[3 * i + d] = pg.positions()[i][d];
+                if (v) v[3 * i + d] = pg.velocities()[i][d];
+            }
+    });
+}
+
+spg_vbd_ctx *spg_solver_context(spg_solver *s) { return s ? s->solver->context() : nullptr; }
+
+}  // extern "C"

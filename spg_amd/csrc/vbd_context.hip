@@ -1,0 +1,1076 @@
+// Host side of libspg_vbd.so: builds the device layout of a VBD context from the spg particle
+// groups and energies, orchestrates the step on one HIP stream, and implements the C-ABI of
+// include/spg_vbd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "coloring.h"
+#include "spg_vbd.h"
+#include "vbd_device.h"
+
+using namespace spgvbd;
+
+namespace {
+
+thread_local std::string g_lastError;
+
+struct VbdError : std::runtime_error {
+    int code;
+    VbdError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) throw VbdError(SPG_VBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        release();
+        if (count == 0) return;
+        hipCheck(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)), "hipMalloc");
+        n = count;
+    }
+    void upload(const std::vector<T> &h, hipStream_t s)
+    {
+        alloc(h.size());
+        if (!h.empty()) hipCheck(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+};
+
+int stencilSizeOf(int type)
+{
+    switch (type) {
+    case SPG_VBD_SPRING: return 2;
+    case SPG_VBD_MEMBRANE_STVK: return 3;
+    case SPG_VBD_DISCRETE_BENDING: return 4;
+    case SPG_VBD_STABLE_NEO_HOOKEAN: return 4;
+    case SPG_VBD_SPRING_ANCHOR: return 1;
+    }
+    return 0;
+}
+int paramsOf(int type)
+{
+    switch (type) {
+    case SPG_VBD_SPRING: return 2;
+    case SPG_VBD_MEMBRANE_STVK: return 13;
+    case SPG_VBD_DISCRETE_BENDING: return 2;
+    case SPG_VBD_STABLE_NEO_HOOKEAN: return 12;
+    case SPG_VBD_SPRING_ANCHOR: return 4;
+    }
+    return 0;
+}
+// compulsory bytes of one element record read per incidence (roofline model)
+double recordBytes(int type)
+{
+    switch (type) {
+    case SPG_VBD_SPRING: return 16;
+    case SPG_VBD_MEMBRANE_STVK: return 40;
+    case SPG_VBD_DISCRETE_BENDING: return 24;
+    case SPG_VBD_STABLE_NEO_HOOKEAN: return 64;
+    case SPG_VBD_SPRING_ANCHOR: return 20;
+    }
+    return 0;
+}
+
+bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+float intBitsAsFloat(int32_t i)
+{
+    float f;
+    std::memcpy(&f, &i, sizeof(f));
+    return f;
+}
+
+}  // namespace
+
+struct spg_vbd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // objects and vertex numbering
+    int32_t nObjects = 0;
+    std::vector<int64_t> objOffset;     // global particle id of object o's particle 0
+    int64_t nv = 0;
+    std::vector<int32_t> slotOfGid, gidOfSlot;
+    std::vector<int32_t> localColors;   // per global particle
+    std::vector<int32_t> numColors;     // per object
+    int32_t maxColors = 0;
+    std::vector<int32_t> colorBegin;    // slot range of each colour, size maxColors + 1
+    std::vector<int64_t> colorActiveV;  // vertices of objects that have this colour (roofline model)
+    std::vector<double> colorBytes;     // compulsory bytes of one sweep of this colour
+    std::vector<int64_t> objSlotsOffsets;
+
+    // device state
+    DevBuf<float4> pos, xt, xold, vel;
+    DevBuf<float> mass;
+    DevBuf<double> massd;  // FP64 masses for the residual
+    DevBuf<unsigned long long> counters;
+    DevBuf<double> r2, norms;
+    DevBuf<int32_t> objSlots;
+    DevBuf<int64_t> objOffsets;
+
+    // incidence streams
+    DevBuf<int32_t> sprPtr, ancPtr, triPtr, hinPtr, tetPtr;
+    DevBuf<float4> spr, anc;
+    DevBuf<int2> triO;
+    DevBuf<float4> triG, triK;
+    DevBuf<int4> hinV;
+    DevBuf<float2> hinP;
+    DevBuf<int4> tetO;
+    DevBuf<float4> tetGa, tetGb, tetGc;
+    // FP64 tables for the residual
+    DevBuf<int32_t> sprElem, ancElem, triElem, hinElem, tetElem;
+    DevBuf<int2> sprV;
+    DevBuf<double2> sprP;
+    DevBuf<double4> ancP;
+    DevBuf<int4> triV;
+    DevBuf<double> triP;
+    DevBuf<int4> hinVe;
+    DevBuf<double2> hinPe;
+    DevBuf<int4> tetV;
+    DevBuf<double> tetP;
+
+    spg_vbd_launch launch{4, 256, 0, 0};
+    bool timing = false;
+    spg_vbd_stats stats{};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> events;  // recorded, not yet accumulated
+    std::vector<int32_t> eventColors;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool;
+
+    bool inSubstep = false;
+    double h = 0.0;
+
+    hipGraphExec_t graphExec = nullptr;
+    hipGraph_t graph = nullptr;
+    std::array<double, 6> graphKey{};  // h, gx, gy, gz, iterations, nsubsteps
+    int graphLaunchHash = -1;
+
+    ~spg_vbd_ctx()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        dropGraph();
+        for (auto &e : events) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (auto &e : eventPool) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void dropGraph()
+    {
+        if (graphExec) (void)hipGraphExecDestroy(graphExec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graphExec = nullptr;
+        graph = nullptr;
+    }
+
+    SolveArgs solveArgs(int32_t color, float invdt2)
+    {
+        SolveArgs a{};
+        a.pos = pos.p;
+        a.xt = xt.p;
+        a.sprPtr = sprPtr.p;
+        a.spr = spr.p;
+        a.ancPtr = ancPtr.p;
+        a.anc = anc.p;
+        a.triPtr = triPtr.p;
+        a.triO = triO.p;
+        a.triG = triG.p;
+        a.triK = triK.p;
+        a.hinPtr = hinPtr.p;
+        a.hinV = hinV.p;
+        a.hinP = hinP.p;
+        a.tetPtr = tetPtr.p;
+        a.tetO = tetO.p;
+        a.tetGa = tetGa.p;
+        a.tetGb = tetGb.p;
+        a.tetGc = tetGc.p;
+        a.counters = counters.p;
+        a.begin = colorBegin[color];
+        a.end = colorBegin[color + 1];
+        a.invdt2 = invdt2;
+        return a;
+    }
+
+    std::pair<hipEvent_t, hipEvent_t> takeEvents()
+    {
+        if (eventPool.empty()) {
+            hipEvent_t a, b;
+            hipCheck(hipEventCreate(&a), "hipEventCreate");
+            hipCheck(hipEventCreate(&b), "hipEventCreate");
+            return {a, b};
+        }
+        auto e = eventPool.back();
+        eventPool.pop_back();
+        return e;
+    }
+
+    void drainEvents()
+    {
+        if (events.empty()) return;
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (size_t i = 0; i < events.size(); ++i) {
+            float ms = 0.f;
+            hipCheck(hipEventElapsedTime(&ms, events[i].first, events[i].second), "hipEventElapsedTime");
+            stats.solve_ms += ms;
+            stats.solve_launches += 1;
+            const int32_t c = eventColors[i];
+            stats.vertex_updates += colorBegin[c + 1] - colorBegin[c];
+            stats.algorithmic_bytes += colorBytes[c];
+            eventPool.push_back(events[i]);
+        }
+        events.clear();
+        eventColors.clear();
+    }
+
+    void sweepColors(int32_t iterations, bool timed)
+    {
+        const float invdt2 = static_cast<float>(1.0 / (h * h));
+        for (int32_t it = 0; it < iterations; ++it)
+            for (int32_t c = 0; c < maxColors; ++c) {
+                const SolveArgs a = solveArgs(c, invdt2);
+                std::pair<hipEvent_t, hipEvent_t> ev{};
+                if (timed) {
+                    ev = takeEvents();
+                    hipCheck(hipEventRecord(ev.first, stream), "hipEventRecord");
+                }
+                hipCheck(launchSolveColor(a, launch.lanes_per_vertex, launch.block_size, launch.lds_staging, stream),
+                         "vertex solve launch");
+                if (timed) {
+                    hipCheck(hipEventRecord(ev.second, stream), "hipEventRecord");
+                    events.push_back(ev);
+                    eventColors.push_back(c);
+                    if (events.size() >= 8192) drainEvents();
+                }
+            }
+    }
+
+    void beginSubstep(double hh, const double *g)
+    {
+        h = hh;
+        hipCheck(launchBeginSubstep(pos.p, xold.p, vel.p, xt.p, mass.p, static_cast<int32_t>(nv), static_cast<float>(hh),
+                                    static_cast<float>(g[0]), static_cast<float>(g[1]), static_cast<float>(g[2]),
+                                    stream),
+                 "begin substep");
+        inSubstep = true;
+    }
+
+    void endSubstep()
+    {
+        hipCheck(launchEndSubstep(pos.p, xold.p, vel.p, static_cast<int32_t>(nv), static_cast<float>(1.0 / h), stream),
+                 "end substep");
+        inSubstep = false;
+    }
+
+    void residual(bool atInertial, double *out)
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
+        ResidualArgs a{};
+        a.pos = atInertial ? xt.p : pos.p;
+        a.xt = xt.p;
+        a.mass = massd.p;
+        a.invdt2 = 1.0 / (h * h);
+        a.n = static_cast<int32_t>(nv);
+        a.sprPtr = sprPtr.p;
+        a.sprElem = sprElem.p;
+        a.sprV = sprV.p;
+        a.sprP = sprP.p;
+        a.ancPtr = ancPtr.p;
+        a.ancElem = ancElem.p;
+        a.ancP = ancP.p;
+        a.triPtr = triPtr.p;
+        a.triElem = triElem.p;
+        a.triV = triV.p;
+        a.triP = triP.p;
+        a.hinPtr = hinPtr.p;
+        a.hinElem = hinElem.p;
+        a.hinV = hinVe.p;
+        a.hinP = hinPe.p;
+        a.tetPtr = tetPtr.p;
+        a.tetElem = tetElem.p;
+        a.tetV = tetV.p;
+        a.tetP = tetP.p;
+        a.r2 = r2.p;
+        hipCheck(launchResidual(a, stream), "residual launch");
+        hipCheck(launchReduceObjects(r2.p, objSlots.p, objOffsets.p, nObjects, norms.p, stream), "reduce launch");
+        hipCheck(hipMemcpyAsync(out, norms.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    void enqueueStep(double dt, int32_t nsub, int32_t iterations, const double *g, bool timed)
+    {
+        const double hh = dt / nsub;
+        for (int32_t s = 0; s < nsub; ++s) {
+            beginSubstep(hh, g);
+            sweepColors(iterations, timed);
+            endSubstep();
+        }
+    }
+
+    void step(double dt, int32_t nsub, int32_t iterations, const double *g)
+    {
+        if (inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "step called inside an open substep");
+        if (!(dt > 0) || nsub < 1 || iterations < 0) throw VbdError(SPG_VBD_ERR_INVALID, "invalid dt/substeps/iterations");
+        if (launch.use_graph && !timing) {
+            const std::array<double, 6> key{dt / nsub, g[0], g[1], g[2], double(iterations), double(nsub)};
+            const int lh = launch.lanes_per_vertex * 100000 + launch.block_size * 10 + launch.lds_staging;
+            if (!graphExec || key != graphKey || lh != graphLaunchHash) {
+                dropGraph();
+                hipCheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+                try {
+                    enqueueStep(dt, nsub, iterations, g, false);
+                } catch (...) {
+                    hipGraph_t tmp = nullptr;
+                    (void)hipStreamEndCapture(stream, &tmp);
+                    if (tmp) (void)hipGraphDestroy(tmp);
+                    throw;
+                }
+                hipCheck(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
+                hipCheck(hipGraphInstantiate(&graphExec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+                graphKey = key;
+                graphLaunchHash = lh;
+            }
+            hipCheck(hipGraphLaunch(graphExec, stream), "hipGraphLaunch");
+            h = dt / nsub;
+            return;
+        }
+        enqueueStep(dt, nsub, iterations, g, timing);
+    }
+};
+
+namespace {
+
+template <class F>
+int guarded(spg_vbd_ctx *ctx, F &&f)
+{
+    try {
+        f();
+        if (ctx) ctx->err.clear();
+        return SPG_VBD_OK;
+    } catch (const VbdError &e) {
+        if (ctx) ctx->err = e.what();
+        g_lastError = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->err = e.what();
+        g_lastError = e.what();
+        return SPG_VBD_ERR_INVALID;
+    }
+}
+
+struct Incidence {
+    int32_t slot;    // owning vertex slot
+    int32_t elem;    // element id within its type table
+    int32_t local;   // position of the owning vertex in the stencil
+};
+
+// Counting sort of incidences by owning slot, stable: keeps energy order then ascending stencil.
+std::vector<int32_t> sortBySlot(const std::vector<Incidence> &inc, int64_t nv, std::vector<int32_t> &ptr)
+{
+    ptr.assign(nv + 1, 0);
+    for (const auto &i : inc) ++ptr[i.slot + 1];
+    for (int64_t v = 0; v < nv; ++v) ptr[v + 1] += ptr[v];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1), order(inc.size());
+    for (int32_t k = 0; k < static_cast<int32_t>(inc.size()); ++k) order[fill[inc[k].slot]++] = k;
+    return order;
+}
+
+void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
+{
+    if (nObjects < 1 || !objs) throw VbdError(SPG_VBD_ERR_INVALID, "at least one object required");
+    c.nObjects = nObjects;
+    c.objOffset.assign(nObjects + 1, 0);
+    for (int32_t o = 0; o < nObjects; ++o) {
+        const auto &ob = objs[o];
+        if (ob.n_particles < 1) throw VbdError(SPG_VBD_ERR_INVALID, "object without particles");
+        if (!ob.x || !ob.mass) throw VbdError(SPG_VBD_ERR_INVALID, "object without positions or masses");
+        if (ob.n_energies < 0 || (ob.n_energies > 0 && !ob.energies))
+            throw VbdError(SPG_VBD_ERR_INVALID, "invalid energy list");
+        c.objOffset[o + 1] = c.objOffset[o] + ob.n_particles;
+    }
+    c.nv = c.objOffset[nObjects];
+    if (c.nv >= (int64_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many particles");
+
+    // validate energies, colour each object (VBD::computeParallelVertexGroups, vbd.cpp:203-229)
+    c.localColors.assign(c.nv, 0);
+    c.numColors.assign(nObjects, 0);
+    for (int32_t o = 0; o < nObjects; ++o) {
+        const auto &ob = objs[o];
+        std::vector<FlatStencilSet> sets;
+        for (int32_t e = 0; e < ob.n_energies; ++e) {
+            const auto &en = ob.energies[e];
+            const int ss = stencilSizeOf(en.type);
+            if (!ss) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "unsupported energy type " + std::to_string(en.type));
+            if (en.n_stencils < 0 || (en.n_stencils > 0 && (!en.stencils || !en.params)))
+                throw VbdError(SPG_VBD_ERR_INVALID, "invalid energy arrays");
+            for (int64_t i = 0; i < en.n_stencils * ss; ++i)
+                if (en.stencils[i] < 0 || en.stencils[i] >= ob.n_particles)
+                    throw VbdError(SPG_VBD_ERR_INVALID, "stencil index out of range");
+            const int np = paramsOf(en.type);
+            for (int64_t i = 0; i < en.n_stencils * np; ++i)
+                if (!std::isfinite(en.params[i])) throw VbdError(SPG_VBD_ERR_INVALID, "non-finite energy parameter");
+            if (en.type == SPG_VBD_MEMBRANE_STVK) {
+                for (int64_t i = 0; i < en.n_stencils; ++i) {
+                    const double *K = en.params + i * np + 4;
+                    if (K[0] != K[4] || K[1] != K[3] || K[2] != 0 || K[5] != 0 || K[6] != 0 || K[7] != 0)
+                        throw VbdError(SPG_VBD_ERR_UNSUPPORTED,
+                                       "membrane stiffness matrix is not of the StVK form [[a,b,0],[b,a,0],[0,0,c]]");
+                }
+            }
+            if (en.type == SPG_VBD_SPRING)
+                for (int64_t i = 0; i < en.n_stencils; ++i)
+                    if (en.params[i * np] <= 0) throw VbdError(SPG_VBD_ERR_INVALID, "Zero/negative rest length invalid");
+            sets.push_back({en.stencils, en.n_stencils * ss, ss});
+        }
+        const auto col = colorVertices(ob.n_particles, sets);
+        int32_t nc = 0;
+        for (int64_t i = 0; i < ob.n_particles; ++i) {
+            c.localColors[c.objOffset[o] + i] = col[i];
+            nc = std::max(nc, col[i] + 1);
+        }
+        c.numColors[o] = nc;
+        c.maxColors = std::max(c.maxColors, nc);
+    }
+
+    // colour-major slot numbering
+    c.slotOfGid.assign(c.nv, -1);
+    c.gidOfSlot.assign(c.nv, -1);
+    c.colorBegin.assign(c.maxColors + 1, 0);
+    {
+        std::vector<int64_t> cnt(c.maxColors, 0);
+        for (int64_t g = 0; g < c.nv; ++g) ++cnt[c.localColors[g]];
+        for (int32_t k = 0; k < c.maxColors; ++k) c.colorBegin[k + 1] = static_cast<int32_t>(c.colorBegin[k] + cnt[k]);
+        std::vector<int32_t> fill(c.colorBegin.begin(), c.colorBegin.end() - 1);
+        for (int32_t k = 0; k < c.maxColors; ++k) (void)k;
+        // objects in order, particles ascending: within colour k the runs are object-major
+        for (int64_t g = 0; g < c.nv; ++g) {
+            const int32_t s = fill[c.localColors[g]]++;
+            c.slotOfGid[g] = s;
+            c.gidOfSlot[s] = static_cast<int32_t>(g);
+        }
+    }
+    // object-major slot lists for per-object reductions
+    {
+        std::vector<int32_t> objSlots(c.nv);
+        c.objSlotsOffsets.assign(nObjects + 1, 0);
+        for (int32_t o = 0; o < nObjects; ++o) {
+            c.objSlotsOffsets[o + 1] = c.objOffset[o + 1];
+            for (int64_t g = c.objOffset[o]; g < c.objOffset[o + 1]; ++g) objSlots[g] = c.slotOfGid[g];
+        }
+        c.objSlots.upload(objSlots, c.stream);
+        c.objOffsets.upload(c.objSlotsOffsets, c.stream);
+    }
+
+    // per-vertex state
+    std::vector<float4> hpos(c.nv), hvel(c.nv), hzero(c.nv, make_float4(0, 0, 0, 0));
+    std::vector<float> hmass(c.nv);
+    std::vector<double> hmassd(c.nv);
+    for (int32_t o = 0; o < nObjects; ++o) {
+        const auto &ob = objs[o];
+        for (int64_t i = 0; i < ob.n_particles; ++i) {
+            const int32_t s = c.slotOfGid[c.objOffset[o] + i];
+            if (!finite3(ob.x + 3 * i) || (ob.v && !finite3(ob.v + 3 * i)) || !std::isfinite(ob.mass[i]))
+                throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite particle state");
+            hpos[s] = make_float4(float(ob.x[3 * i]), float(ob.x[3 * i + 1]), float(ob.x[3 * i + 2]), 0.f);
+            hvel[s] = ob.v ? make_float4(float(ob.v[3 * i]), float(ob.v[3 * i + 1]), float(ob.v[3 * i + 2]), 0.f)
+                           : make_float4(0, 0, 0, 0);
+            hmass[s] = float(ob.mass[i]);
+            hmassd[s] = ob.mass[i];
+        }
+    }
+    c.massd.upload(hmassd, c.stream);
+    c.pos.upload(hpos, c.stream);
+    c.vel.upload(hvel, c.stream);
+    c.xt.upload(hpos, c.stream);
+    c.xold.upload(hzero, c.stream);
+    c.mass.upload(hmass, c.stream);
+    c.counters.upload(std::vector<unsigned long long>(2, 0ull), c.stream);
+    c.r2.alloc(c.nv);
+    c.norms.alloc(nObjects);
+
+    // incidence streams per type (VBD::computeStencilInfoPerVertex, vbd.cpp:231-268)
+    std::array<std::vector<Incidence>, 6> inc;
+    std::vector<int2> sprV;
+    std::vector<double2> sprP;
+    std::vector<double4> ancP;
+    std::vector<int4> triV, hinV, tetV;
+    std::vector<double> triP, tetP;
+    std::vector<double2> hinP;
+    std::vector<double> colorRecordBytes(c.maxColors, 0.0);
+    std::vector<int64_t> colorIncidences(c.maxColors, 0);
+    for (int32_t o = 0; o < nObjects; ++o) {
+        const auto &ob = objs[o];
+        const int64_t base = c.objOffset[o];
+        for (int32_t e = 0; e < ob.n_energies; ++e) {
+            const auto &en = ob.energies[e];
+            const int ss = stencilSizeOf(en.type), np = paramsOf(en.type);
+            for (int64_t i = 0; i < en.n_stencils; ++i) {
+                const int32_t *st = en.stencils + i * ss;
+                const double *p = en.params + i * np;
+                int32_t sl[4] = {0, 0, 0, 0};
+                for (int k = 0; k < ss; ++k) sl[k] = c.slotOfGid[base + st[k]];
+                int32_t elem = 0;
+                switch (en.type) {
+                case SPG_VBD_SPRING:
+                    elem = static_cast<int32_t>(sprV.size());
+                    sprV.push_back(make_int2(sl[0], sl[1]));
+                    sprP.push_back(make_double2(p[0], p[1]));
+                    break;
+                case SPG_VBD_SPRING_ANCHOR:
+                    elem = static_cast<int32_t>(ancP.size());
+                    ancP.push_back(make_double4(p[0], p[1], p[2], p[3]));
+                    break;
+                case SPG_VBD_MEMBRANE_STVK:
+                    elem = static_cast<int32_t>(triV.size());
+                    triV.push_back(make_int4(sl[0], sl[1], sl[2], 0));
+                    triP.insert(triP.end(), {p[0], p[1], p[2], p[3], p[4], p[5], p[12], 0.0});
+                    break;
+                case SPG_VBD_DISCRETE_BENDING:
+                    elem = static_cast<int32_t>(hinV.size());
+                    hinV.push_back(make_int4(sl[0], sl[1], sl[2], sl[3]));
+                    hinP.push_back(make_double2(p[0], p[1]));
+                    break;
+                case SPG_VBD_STABLE_NEO_HOOKEAN:
+                    elem = static_cast<int32_t>(tetV.size());
+                    tetV.push_back(make_int4(sl[0], sl[1], sl[2], sl[3]));
+                    tetP.insert(tetP.end(), p, p + 12);
+                    break;
+                }
+                for (int k = 0; k < ss; ++k) {
+                    inc[en.type].push_back({sl[k], elem, k});
+                    const int32_t col = c.localColors[base + st[k]];
+                    colorRecordBytes[col] += recordBytes(en.type) + 4.0;
+                    colorIncidences[col] += 1;
+                }
+            }
+        }
+    }
+    auto checkSize = [](size_t n) {
+        if (n >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many incidences");
+    };
+    // springs
+    if (!inc[SPG_VBD_SPRING].empty()) {
+        checkSize(inc[SPG_VBD_SPRING].size());
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(inc[SPG_VBD_SPRING], c.nv, ptr);
+        std::vector<float4> rec(order.size());
+        std::vector<int32_t> el(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            const auto &in = inc[SPG_VBD_SPRING][order[k]];
+            const int2 v = sprV[in.elem];
+            const int32_t other = in.local == 0 ? v.y : v.x;
+            rec[k] = make_float4(intBitsAsFloat(other), float(sprP[in.elem].x), float(sprP[in.elem].y), 0.f);
+            el[k] = in.elem;
+        }
+        c.sprPtr.upload(ptr, c.stream);
+        c.spr.upload(rec, c.stream);
+        c.sprElem.upload(el, c.stream);
+        c.sprV.upload(sprV, c.stream);
+        c.sprP.upload(sprP, c.stream);
+    }
+    // anchors
+    if (!inc[SPG_VBD_SPRING_ANCHOR].empty()) {
+        checkSize(inc[SPG_VBD_SPRING_ANCHOR].size());
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(inc[SPG_VBD_SPRING_ANCHOR], c.nv, ptr);
+        std::vector<float4> rec(order.size());
+        std::vector<int32_t> el(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            const auto &in = inc[SPG_VBD_SPRING_ANCHOR][order[k]];
+            const double4 p = ancP[in.elem];
+            rec[k] = make_float4(float(p.x), float(p.y), float(p.z), float(p.w));
+            el[k] = in.elem;
+        }
+        c.ancPtr.upload(ptr, c.stream);
+        c.anc.upload(rec, c.stream);
+        c.ancElem.upload(el, c.stream);
+        c.ancP.upload(ancP, c.stream);
+    }
+    // StVK membrane: rows of Dinv are the shape vectors g_1, g_2; g_0 = -(g_1 + g_2)
+    if (!inc[SPG_VBD_MEMBRANE_STVK].empty()) {
+        checkSize(inc[SPG_VBD_MEMBRANE_STVK].size());
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(inc[SPG_VBD_MEMBRANE_STVK], c.nv, ptr);
+        std::vector<int2> ro(order.size());
+        std::vector<float4> rg(order.size()), rk(order.size());
+        std::vector<int32_t> el(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            const auto &in = inc[SPG_VBD_MEMBRANE_STVK][order[k]];
+            const int4 v = triV[in.elem];
+            const double *p = &triP[8 * static_cast<size_t>(in.elem)];
+            const double g[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
+            const int32_t vv[3] = {v.x, v.y, v.z};
+            const int ia = in.local == 0 ? 1 : 0, ib = in.local == 2 ? 1 : 2;
+            ro[k] = make_int2(vv[ia], vv[ib]);
+            rg[k] = make_float4(float(g[ia][0]), float(g[ia][1]), float(g[ib][0]), float(g[ib][1]));
+            rk[k] = make_float4(float(p[4]), float(p[5]), float(p[6]), 0.f);
+            el[k] = in.elem;
+        }
+        c.triPtr.upload(ptr, c.stream);
+        c.triO.upload(ro, c.stream);
+        c.triG.upload(rg, c.stream);
+        c.triK.upload(rk, c.stream);
+        c.triElem.upload(el, c.stream);
+        c.triV.upload(triV, c.stream);
+        c.triP.upload(triP, c.stream);
+    }
+    // discrete bending: canonical slots, the solve finds its own position in the hinge
+    if (!inc[SPG_VBD_DISCRETE_BENDING].empty()) {
+        checkSize(inc[SPG_VBD_DISCRETE_BENDING].size());
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(inc[SPG_VBD_DISCRETE_BENDING], c.nv, ptr);
+        std::vector<int4> rv(order.size());
+        std::vector<float2> rp(order.size());
+        std::vector<int32_t> el(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            const auto &in = inc[SPG_VBD_DISCRETE_BENDING][order[k]];
+            rv[k] = hinV[in.elem];
+            rp[k] = make_float2(float(hinP[in.elem].x), float(hinP[in.elem].y));
+            el[k] = in.elem;
+        }
+        // a hinge with a repeated vertex cannot tell its slot from the vertex id
+        for (const auto &v : hinV)
+            if (v.x == v.y || v.x == v.z || v.x == v.w || v.y == v.z || v.y == v.w || v.z == v.w)
+                throw VbdError(SPG_VBD_ERR_INVALID, "bending stencil with a repeated vertex");
+        c.hinPtr.upload(ptr, c.stream);
+        c.hinV.upload(rv, c.stream);
+        c.hinP.upload(rp, c.stream);
+        c.hinElem.upload(el, c.stream);
+        c.hinVe.upload(hinV, c.stream);
+        c.hinPe.upload(hinP, c.stream);
+    }
+    // Stable Neo-Hookean: rows of Dinv are g_1..g_3; g_0 = -(g_1 + g_2 + g_3)
+    if (!inc[SPG_VBD_STABLE_NEO_HOOKEAN].empty()) {
+        checkSize(inc[SPG_VBD_STABLE_NEO_HOOKEAN].size());
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(inc[SPG_VBD_STABLE_NEO_HOOKEAN], c.nv, ptr);
+        std::vector<int4> ro(order.size());
+        std::vector<float4> ga(order.size()), gb(order.size()), gc(order.size());
+        std::vector<int32_t> el(order.size());
+        for (size_t k = 0; k < order.size(); ++k) {
+            const auto &in = inc[SPG_VBD_STABLE_NEO_HOOKEAN][order[k]];
+            const int4 v = tetV[in.elem];
+            const double *p = &tetP[12 * static_cast<size_t>(in.elem)];
+            double g[4][3];
+            for (int q = 0; q < 3; ++q) {
+                g[1][q] = p[q];
+                g[2][q] = p[3 + q];
+                g[3][q] = p[6 + q];
+                g[0][q] = -(p[q] + p[3 + q] + p[6 + q]);
+            }
+            const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+            int oth[3], m = 0;
+            for (int q = 0; q < 4; ++q)
+                if (q != in.local) oth[m++] = q;
+            ro[k] = make_int4(vv[oth[0]], vv[oth[1]], vv[oth[2]], 0);
+            auto G = [&](int q, double w) { return make_float4(float(g[q][0]), float(g[q][1]), float(g[q][2]), float(w)); };
+            ga[k] = G(oth[0], p[9]);
+            gb[k] = G(oth[1], p[10]);
+            gc[k] = G(oth[2], p[11]);
+            el[k] = in.elem;
+        }
+        c.tetPtr.upload(ptr, c.stream);
+        c.tetO.upload(ro, c.stream);
+        c.tetGa.upload(ga, c.stream);
+        c.tetGb.upload(gb, c.stream);
+        c.tetGc.upload(gc, c.stream);
+        c.tetElem.upload(el, c.stream);
+        c.tetV.upload(tetV, c.stream);
+        c.tetP.upload(tetP, c.stream);
+    }
+
+    // compulsory bytes per colour sweep: 12 B per vertex of every object that has this colour (all
+    // positions read once), 32 B per updated vertex (x write, x_tilde read, mass, row offset), and
+    // per incidence a 4 B CSR entry plus the element record.
+    c.colorActiveV.assign(c.maxColors, 0);
+    c.colorBytes.assign(c.maxColors, 0.0);
+    for (int32_t o = 0; o < nObjects; ++o)
+        for (int32_t k = 0; k < c.numColors[o]; ++k) c.colorActiveV[k] += objs[o].n_particles;
+    for (int32_t k = 0; k < c.maxColors; ++k)
+        c.colorBytes[k] = 12.0 * double(c.colorActiveV[k]) + 32.0 * double(c.colorBegin[k + 1] - c.colorBegin[k]) +
+                          colorRecordBytes[k];
+    hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+}
+
+void checkObject(const spg_vbd_ctx *ctx, int32_t object)
+{
+    if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+    if (object < 0 || object >= ctx->nObjects) throw VbdError(SPG_VBD_ERR_INVALID, "object index out of range");
+}
+
+void uploadPositions(spg_vbd_ctx &c, int32_t object, const double *x, float4 *dst)
+{
+    const int64_t n = c.objOffset[object + 1] - c.objOffset[object];
+    for (int64_t i = 0; i < n; ++i)
+        if (!finite3(x + 3 * i)) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite positions");
+    // read-modify-write of the object's slots
+    std::vector<float4> h(c.nv);
+    hipCheck(hipMemcpyAsync(h.data(), dst, sizeof(float4) * c.nv, hipMemcpyDeviceToHost, c.stream), "download");
+    hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t s = c.slotOfGid[c.objOffset[object] + i];
+        h[s] = make_float4(float(x[3 * i]), float(x[3 * i + 1]), float(x[3 * i + 2]), 0.f);
+    }
+    hipCheck(hipMemcpyAsync(dst, h.data(), sizeof(float4) * c.nv, hipMemcpyHostToDevice, c.stream), "upload");
+    hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+}
+
+}  // namespace
+
+extern "C" {
+
+int spg_vbd_create(spg_vbd_ctx **out, const spg_vbd_object *objects, int32_t n_objects, int32_t device)
+{
+    if (!out) return SPG_VBD_ERR_INVALID;
+    *out = nullptr;
+    auto c = std::make_unique<spg_vbd_ctx>();
+    const int rc = guarded(nullptr, [&] {
+        int ndev = 0;
+        hipCheck(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        if (device < 0 || device >= ndev) throw VbdError(SPG_VBD_ERR_INVALID, "invalid HIP device");
+        c->device = device;
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
+        buildContext(*c, objects, n_objects);
+    });
+    if (rc == SPG_VBD_OK) *out = c.release();
+    return rc;
+}
+
+void spg_vbd_destroy(spg_vbd_ctx *ctx)
+{
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        delete ctx;
+    }
+}
+
+const char *spg_vbd_last_error(const spg_vbd_ctx *ctx) { return ctx ? ctx->err.c_str() : g_lastError.c_str(); }
+
+int spg_vbd_num_objects(const spg_vbd_ctx *ctx, int32_t *n)
+{
+    if (!ctx || !n) return SPG_VBD_ERR_INVALID;
+    *n = ctx->nObjects;
+    return SPG_VBD_OK;
+}
+
+int spg_vbd_num_particles(const spg_vbd_ctx *ctx, int32_t object, int64_t *n)
+{
+    return guarded(const_cast<spg_vbd_ctx *>(ctx), [&] {
+        checkObject(ctx, object);
+        *n = ctx->objOffset[object + 1] - ctx->objOffset[object];
+    });
+}
+
+int spg_vbd_num_colors(const spg_vbd_ctx *ctx, int32_t object, int32_t *n)
+{
+    return guarded(const_cast<spg_vbd_ctx *>(ctx), [&] {
+        checkObject(ctx, object);
+        *n = ctx->numColors[object];
+    });
+}
+
+int spg_vbd_get_colors(const spg_vbd_ctx *ctx, int32_t object, int32_t *colors)
+{
+    return guarded(const_cast<spg_vbd_ctx *>(ctx), [&] {
+        checkObject(ctx, object);
+        std::copy(ctx->localColors.begin() + ctx->objOffset[object], ctx->localColors.begin() + ctx->objOffset[object + 1],
+                  colors);
+    });
+}
+
+int spg_vbd_set_state(spg_vbd_ctx *ctx, int32_t object, const double *x, const double *v)
+{
+    return guarded(ctx, [&] {
+        checkObject(ctx, object);
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        const int64_t n = ctx->objOffset[object + 1] - ctx->objOffset[object];
+        for (int64_t i = 0; i < n; ++i)
+            if ((x && !finite3(x + 3 * i)) || (v && !finite3(v + 3 * i)))
+                throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
+        if (x) uploadPositions(*ctx, object, x, ctx->pos.p);
+        if (v) uploadPositions(*ctx, object, v, ctx->vel.p);
+    });
+}
+
+int spg_vbd_get_state(spg_vbd_ctx *ctx, int32_t object, double *x, double *v)
+{
+    return guarded(ctx, [&] {
+        checkObject(ctx, object);
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<float4> hp(ctx->nv), hv(ctx->nv);
+        hipCheck(hipMemcpyAsync(hp.data(), ctx->pos.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToHost, ctx->stream),
+                 "download");
+        hipCheck(hipMemcpyAsync(hv.data(), ctx->vel.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToHost, ctx->stream),
+                 "download");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        const int64_t n = ctx->objOffset[object + 1] - ctx->objOffset[object];
+        for (int64_t i = 0; i < n; ++i) {
+            const int32_t s = ctx->slotOfGid[ctx->objOffset[object] + i];
+            if (x) {
+                x[3 * i] = hp[s].x;
+                x[3 * i + 1] = hp[s].y;
+                x[3 * i + 2] = hp[s].z;
+            }
+            if (v) {
+                v[3 * i] = hv[s].x;
+                v[3 * i + 1] = hv[s].y;
+                v[3 * i + 2] = hv[s].z;
+            }
+        }
+    });
+}
+
+int spg_vbd_step(spg_vbd_ctx *ctx, double dt, int32_t nsubsteps, int32_t iterations, const double gravity[3])
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !gravity) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->step(dt, nsubsteps, iterations, gravity);
+    });
+}
+
+int spg_vbd_begin_substep(spg_vbd_ctx *ctx, double h, const double gravity[3])
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !gravity) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        if (ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "substep already open");
+        if (!(h > 0)) throw VbdError(SPG_VBD_ERR_INVALID, "invalid h");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->beginSubstep(h, gravity);
+    });
+}
+
+int spg_vbd_set_guess(spg_vbd_ctx *ctx, int32_t object, const double *x)
+{
+    return guarded(ctx, [&] {
+        checkObject(ctx, object);
+        if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "set_guess needs an open substep");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        uploadPositions(*ctx, object, x, ctx->pos.p);
+    });
+}
+
+int spg_vbd_iterate(spg_vbd_ctx *ctx, int32_t iterations)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "iterate needs an open substep");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->sweepColors(iterations, ctx->timing);
+    });
+}
+
+int spg_vbd_end_substep(spg_vbd_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "no open substep");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->endSubstep();
+    });
+}
+
+int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !norms) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->residual(false, norms);
+    });
+}
+
+int spg_vbd_residual_at_inertial(spg_vbd_ctx *ctx, double *norms)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !norms) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->residual(true, norms);
+    });
+}
+
+int spg_vbd_get_launch(const spg_vbd_ctx *ctx, spg_vbd_launch *cfg)
+{
+    if (!ctx || !cfg) return SPG_VBD_ERR_INVALID;
+    *cfg = ctx->launch;
+    return SPG_VBD_OK;
+}
+
+int spg_vbd_set_launch(spg_vbd_ctx *ctx, const spg_vbd_launch *cfg)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !cfg) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        const int g = cfg->lanes_per_vertex;
+        if (g != 1 && g != 2 && g != 4 && g != 8 && g != 16)
+            throw VbdError(SPG_VBD_ERR_INVALID, "lanes_per_vertex must be 1, 2, 4, 8 or 16");
+        if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
+            throw VbdError(SPG_VBD_ERR_INVALID, "block_size must be 64, 128 or 256");
+        ctx->launch = *cfg;
+        ctx->dropGraph();
+    });
+}
+
+int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "autotune inside an open substep");
+        if (trials < 1) trials = 1;
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        // save state, tune inside a scratch substep, restore
+        DevBuf<float4> spos, svel;
+        spos.alloc(ctx->nv);
+        svel.alloc(ctx->nv);
+        hipCheck(hipMemcpyAsync(spos.p, ctx->pos.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "save");
+        hipCheck(hipMemcpyAsync(svel.p, ctx->vel.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "save");
+        const spg_vbd_launch keep = ctx->launch;
+        const double g[3] = {0.0, -9.81, 0.0};
+        ctx->beginSubstep(1.0 / 60.0, g);
+        hipEvent_t e0, e1;
+        hipCheck(hipEventCreate(&e0), "hipEventCreate");
+        hipCheck(hipEventCreate(&e1), "hipEventCreate");
+        float bestMs = 1e30f;
+        spg_vbd_launch bestCfg = keep;
+        const int lanes[] = {1, 2, 4, 8, 16};
+        const int blocks[] = {64, 128, 256};
+        for (int l : lanes)
+            for (int b : blocks) {
+                ctx->launch = keep;
+                ctx->launch.lanes_per_vertex = l;
+                ctx->launch.block_size = b;
+                ctx->sweepColors(1, false);  // warm
+                hipCheck(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+                ctx->sweepColors(trials, false);
+                hipCheck(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+                hipCheck(hipEventSynchronize(e1), "hipEventSynchronize");
+                float ms = 0.f;
+                hipCheck(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                if (ms < bestMs) {
+                    bestMs = ms;
+                    bestCfg = ctx->launch;
+                }
+            }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ctx->inSubstep = false;
+        hipCheck(hipMemcpyAsync(ctx->pos.p, spos.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "restore");
+        hipCheck(hipMemcpyAsync(ctx->vel.p, svel.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "restore");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        ctx->launch = bestCfg;
+        ctx->dropGraph();
+        if (best) *best = bestCfg;
+    });
+}
+
+int spg_vbd_get_stats(const spg_vbd_ctx *ctx, spg_vbd_stats *stats)
+{
+    auto *c = const_cast<spg_vbd_ctx *>(ctx);
+    return guarded(c, [&] {
+        if (!ctx || !stats) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(c->device), "hipSetDevice");
+        c->drainEvents();
+        unsigned long long cnt[2] = {0, 0};
+        hipCheck(hipMemcpyAsync(cnt, c->counters.p, sizeof(cnt), hipMemcpyDeviceToHost, c->stream), "download");
+        hipCheck(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        *stats = c->stats;
+        stats->fp64_fallbacks = static_cast<int64_t>(cnt[0]);
+        stats->skipped_blocks = static_cast<int64_t>(cnt[1]);
+    });
+}
+
+int spg_vbd_reset_stats(spg_vbd_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->drainEvents();
+        ctx->stats = spg_vbd_stats{};
+        const unsigned long long zero[2] = {0, 0};
+        hipCheck(hipMemcpyAsync(ctx->counters.p, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream), "upload");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int spg_vbd_set_timing(spg_vbd_ctx *ctx, int32_t enabled)
+{
+    if (!ctx) return SPG_VBD_ERR_INVALID;
+    ctx->timing = enabled != 0;
+    return SPG_VBD_OK;
+}
+
+int spg_vbd_synchronize(spg_vbd_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
+int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *const *flat_stencils,
+                           const int64_t *lengths, const int32_t *stencil_sizes, int32_t *colors)
+{
+    return guarded(nullptr, [&] {
+        if (n_sets < 0 || (n_sets > 0 && (!flat_stencils || !lengths || !stencil_sizes)) || !colors)
+            throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        std::vector<FlatStencilSet> sets;
+        for (int32_t i = 0; i < n_sets; ++i) sets.push_back({flat_stencils[i], lengths[i], stencil_sizes[i]});
+        const auto col = colorVertices(n_vertices, sets);
+        std::copy(col.begin(), col.end(), colors);
+    });
+}
+
+int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params, int32_t precision,
+                                double *grad, double *hess_blocks, int32_t device)
+{
+    return guarded(nullptr, [&] {
+        const int ss = stencilSizeOf(type), np = paramsOf(type);
+        if (!ss) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "unsupported energy type");
+        if (n < 0 || (n > 0 && (!xs || !params || !grad || !hess_blocks)))
+            throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        if (n == 0) return;
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s;
+        hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        DevBuf<double> dx, dp, dg, dh;
+        dx.upload(std::vector<double>(xs, xs + n * ss * 3), s);
+        dp.upload(std::vector<double>(params, params + n * np), s);
+        dg.alloc(n * ss * 3);
+        dh.alloc(n * ss * 9);
+        hipCheck(launchElementDerivatives(type, n, dx.p, dp.p, precision, dg.p, dh.p, s), "element launch");
+        hipCheck(hipMemcpyAsync(grad, dg.p, sizeof(double) * n * ss * 3, hipMemcpyDeviceToHost, s), "download");
+        hipCheck(hipMemcpyAsync(hess_blocks, dh.p, sizeof(double) * n * ss * 9, hipMemcpyDeviceToHost, s), "download");
+        hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+        (void)hipStreamDestroy(s);
+    });
+}
+
+}  // extern "C"

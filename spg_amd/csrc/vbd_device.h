@@ -1,0 +1,98 @@
+// Device-side data layout of a VBD context and the kernel launch interface.
+//
+// Layout in HBM (one context, all objects concatenated):
+//   * Vertex slots are numbered colour-major: colour 0 of object 0, colour 0 of object 1, ...,
+//     colour 1 of object 0, ... with each (object, colour) run in ascending particle index, the order
+//     of the reference's colour groups (spg/source/sim/solver/vbd.cpp:220-223). A colour sweep is one
+//     contiguous slot range, so the vertex solve writes positions with unit stride.
+//   * Per-vertex state: float4 streams pos (x), xt (inertial target, w = mass), xold, vel.
+//   * Per energy type, an incidence stream in vertex-slot order: each incidence (vertex, element)
+//     carries its own copy of the element record, rewritten relative to the owning vertex, so the
+//     solve reads its element data as contiguous SoA streams; row pointers give each vertex's range.
+//     Within a vertex the order is the reference's (energy in addEnergy order, ascending stencil,
+//     vbd.cpp:250-259).
+//   * FP64 element tables (vertex slots + exact prepared parameters) and per-incidence element ids,
+//     read only by the FP64 residual kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spgvbd {
+
+enum EnergyType { kSpring = 1, kStvk = 2, kBending = 3, kSnh = 4, kAnchor = 5 };
+
+struct SolveArgs {
+    float4 *pos;
+    const float4 *xt;
+    // spring: (other slot bits, L0, k, 0)
+    const int32_t *sprPtr;
+    const float4 *spr;
+    // anchor: (ax, ay, az, k)
+    const int32_t *ancPtr;
+    const float4 *anc;
+    // StVK: other slots, (g_a, g_b), (kd, ko, ks, 0)
+    const int32_t *triPtr;
+    const int2 *triO;
+    const float4 *triG;
+    const float4 *triK;
+    // bending: canonical slots v0..v3, (theta0, keff)
+    const int32_t *hinPtr;
+    const int4 *hinV;
+    const float2 *hinP;
+    // SNH: other slots (3), (g_a, lambda), (g_b, mu), (g_c, alpha)
+    const int32_t *tetPtr;
+    const int4 *tetO;
+    const float4 *tetGa;
+    const float4 *tetGb;
+    const float4 *tetGc;
+    unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
+    int32_t begin, end;            // slot range of the colour
+    float invdt2;
+};
+
+struct ResidualArgs {
+    const float4 *pos;
+    const float4 *xt;
+    const double *mass;  // exact FP64 masses per slot
+    double invdt2;
+    int32_t n;
+    // per type: incidence row pointers (shared with the solve) and element ids; element tables
+    const int32_t *sprPtr;
+    const int32_t *sprElem;
+    const int2 *sprV;
+    const double2 *sprP;  // (L0, k)
+    const int32_t *ancPtr;
+    const int32_t *ancElem;
+    const double4 *ancP;  // (a, k)
+    const int32_t *triPtr;
+    const int32_t *triElem;
+    const int4 *triV;
+    const double *triP;   // 8 per element: Dinv (4), kd, ko, ks, 0
+    const int32_t *hinPtr;
+    const int32_t *hinElem;
+    const int4 *hinV;
+    const double2 *hinP;  // (theta0, keff)
+    const int32_t *tetPtr;
+    const int32_t *tetElem;
+    const int4 *tetV;
+    const double *tetP;   // 12 per element: Dinv (9), lambda, mu, alpha
+    double *r2;           // out: |r_v|^2 per slot
+};
+
+// kernel launchers (vbd_kernels.hip); all asynchronous on `stream`
+hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt, const float *mass, int32_t n,
+                              float h, float gx, float gy, float gz, hipStream_t stream);
+hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
+                            hipStream_t stream);
+hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
+                            hipStream_t stream);
+hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream);
+hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *objOffsets, int32_t nObjects,
+                               double *out, hipStream_t stream);
+hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t stream);
+hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,
+                                    double *grad, double *hess, hipStream_t stream);
+
+}  // namespace spgvbd

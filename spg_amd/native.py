@@ -1,0 +1,472 @@
+"""ctypes bindings of the two native libraries.
+
+  libspg_vbd.so  — include/spg_vbd.h : the GPU VBD solver's C-ABI (the drop-in boundary)
+  libspg_host.so — include/spg_host.h: the C++ mirror of spg's ParticleGroup / energies / BaseSolver
+
+There is no CPU or PyTorch fallback: if the libraries are not built, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(PKG, "lib")
+VBD_LIB = os.path.join(LIBDIR, "libspg_vbd.so")
+HOST_LIB = os.path.join(LIBDIR, "libspg_host.so")
+
+SPRING, MEMBRANE_STVK, DISCRETE_BENDING, STABLE_NEO_HOOKEAN, SPRING_ANCHOR = 1, 2, 3, 4, 5
+KIND = {"spring": SPRING, "stvk": MEMBRANE_STVK, "bending": DISCRETE_BENDING, "snh": STABLE_NEO_HOOKEAN,
+        "anchor": SPRING_ANCHOR}
+STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1}
+PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 12, SPRING_ANCHOR: 4}
+ERR_NONFINITE = 5
+ERR_STATE = 4
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+
+class VbdEnergy(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("n_stencils", ctypes.c_int64), ("stencils", _ip), ("params", _dp)]
+
+
+class VbdObject(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int64), ("x", _dp), ("v", _dp), ("mass", _dp),
+                ("n_energies", ctypes.c_int32), ("energies", ctypes.POINTER(VbdEnergy))]
+
+
+class VbdLaunch(ctypes.Structure):
+    _fields_ = [("lanes_per_vertex", ctypes.c_int32), ("block_size", ctypes.c_int32), ("use_graph", ctypes.c_int32),
+                ("lds_staging", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class VbdStats(ctypes.Structure):
+    _fields_ = [("solve_launches", ctypes.c_int64), ("solve_ms", ctypes.c_double), ("vertex_updates", ctypes.c_int64),
+                ("algorithmic_bytes", ctypes.c_double), ("fp64_fallbacks", ctypes.c_int64),
+                ("skipped_blocks", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class NativeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+_vbd = None
+_host = None
+
+
+def _proto(lib, name, res, args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def vbd_lib():
+    global _vbd
+    if _vbd is None:
+        if not os.path.exists(VBD_LIB):
+            raise RuntimeError(f"{VBD_LIB} is not built: run `python -m spg_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(VBD_LIB)
+        c = ctypes.c_void_p
+        i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        _proto(L, "spg_vbd_create", ctypes.c_int, [ctypes.POINTER(c), ctypes.POINTER(VbdObject), i32, i32])
+        _proto(L, "spg_vbd_destroy", None, [c])
+        _proto(L, "spg_vbd_last_error", ctypes.c_char_p, [c])
+        _proto(L, "spg_vbd_num_objects", ctypes.c_int, [c, _ip])
+        _proto(L, "spg_vbd_num_particles", ctypes.c_int, [c, i32, _lp])
+        _proto(L, "spg_vbd_num_colors", ctypes.c_int, [c, i32, _ip])
+        _proto(L, "spg_vbd_get_colors", ctypes.c_int, [c, i32, _ip])
+        _proto(L, "spg_vbd_set_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_vbd_get_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_vbd_step", ctypes.c_int, [c, dbl, i32, i32, _dp])
+        _proto(L, "spg_vbd_begin_substep", ctypes.c_int, [c, dbl, _dp])
+        _proto(L, "spg_vbd_set_guess", ctypes.c_int, [c, i32, _dp])
+        _proto(L, "spg_vbd_iterate", ctypes.c_int, [c, i32])
+        _proto(L, "spg_vbd_end_substep", ctypes.c_int, [c])
+        _proto(L, "spg_vbd_residual", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_vbd_residual_at_inertial", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_vbd_get_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
+        _proto(L, "spg_vbd_set_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
+        _proto(L, "spg_vbd_autotune", ctypes.c_int, [c, i32, ctypes.POINTER(VbdLaunch)])
+        _proto(L, "spg_vbd_get_stats", ctypes.c_int, [c, ctypes.POINTER(VbdStats)])
+        _proto(L, "spg_vbd_reset_stats", ctypes.c_int, [c])
+        _proto(L, "spg_vbd_set_timing", ctypes.c_int, [c, i32])
+        _proto(L, "spg_vbd_synchronize", ctypes.c_int, [c])
+        _proto(L, "spg_vbd_color_vertices", ctypes.c_int,
+               [i64, i32, ctypes.POINTER(_ip), _lp, _ip, _ip])
+        _proto(L, "spg_vbd_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, i32, _dp, _dp, i32])
+        _vbd = L
+    return _vbd
+
+
+def host_lib():
+    global _host
+    if _host is None:
+        vbd_lib()
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is not built: run `python -m spg_amd.build`")
+        L = ctypes.CDLL(HOST_LIB)
+        c = ctypes.c_void_p
+        i32, i64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        _proto(L, "spg_host_last_error", ctypes.c_char_p, [])
+        _proto(L, "spg_pg_create", c, [])
+        _proto(L, "spg_pg_destroy", None, [c])
+        _proto(L, "spg_pg_add_particles", ctypes.c_int, [c, i64, _dp, _dp, _dp])
+        _proto(L, "spg_pg_add_energy", ctypes.c_int, [c, i32, i64, _ip, _dp, i32, _ip])
+        _proto(L, "spg_pg_prepare_energy", ctypes.c_int, [c, i32])
+        _proto(L, "spg_pg_num_particles", ctypes.c_int, [c, _lp])
+        _proto(L, "spg_pg_energy_params", ctypes.c_int, [c, i32, _dp])
+        _proto(L, "spg_pg_get_state", ctypes.c_int, [c, _dp, _dp])
+        _proto(L, "spg_pg_set_state", ctypes.c_int, [c, _dp, _dp])
+        _proto(L, "spg_trimesh_bending_stencils", ctypes.c_int, [i64, i64, _ip, _ip, _lp])
+        _proto(L, "spg_solver_create_vbd_gpu", c, [i32])
+        _proto(L, "spg_solver_destroy", None, [c])
+        _proto(L, "spg_solver_add_object", ctypes.c_int, [c, c])
+        _proto(L, "spg_solver_set_dt", ctypes.c_int, [c, dbl])
+        _proto(L, "spg_solver_set_num_substeps", ctypes.c_int, [c, i32])
+        _proto(L, "spg_solver_set_gravity", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_solver_set_iterations", ctypes.c_int, [c, i32])
+        _proto(L, "spg_solver_set_resident_state", ctypes.c_int, [c, i32])
+        _proto(L, "spg_solver_step", ctypes.c_int, [c])
+        _proto(L, "spg_solver_reset", ctypes.c_int, [c])
+        _proto(L, "spg_solver_sync_to_host", ctypes.c_int, [c])
+        _proto(L, "spg_solver_get_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_solver_context", c, [c])
+        _host = L
+    return _host
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _i32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a.reshape(shape) if shape is not None else a
+
+
+def _kind(k):
+    return KIND[k] if isinstance(k, str) else int(k)
+
+
+# ------------------------------------------------------------------------------------------------
+# Host mirror: ParticleGroup / TriangleMesh / VBDGpu
+# ------------------------------------------------------------------------------------------------
+
+def _host_check(rc):
+    if rc != 0:
+        raise NativeError(rc, host_lib().spg_host_last_error().decode())
+
+
+def trimesh_bending_stencils(n_vertices, faces):
+    """spg::TriangleMesh(faces).bendingStencils(): {e0, e1, opp0, opp1} for interior edges."""
+    L = host_lib()
+    faces = _i32(faces, (-1, 3))
+    n = ctypes.c_int64(0)
+    _host_check(L.spg_trimesh_bending_stencils(n_vertices, faces.shape[0], _i(faces), None, ctypes.byref(n)))
+    out = np.zeros((n.value, 4), dtype=np.int32)
+    _host_check(L.spg_trimesh_bending_stencils(n_vertices, faces.shape[0], _i(faces), _i(out), ctypes.byref(n)))
+    return out
+
+
+class ParticleGroup:
+    """spg::ParticleGroup built through the C++ mirror (addParticle, addStencil, preparePrecomputations)."""
+
+    def __init__(self):
+        self._L = host_lib()
+        self.h = self._L.spg_pg_create()
+        self.energy_types = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_pg_destroy(self.h)
+            self.h = None
+
+    def add_particles(self, x, x0, m):
+        x, x0, m = _f64(x, (-1, 3)), _f64(x0, (-1, 3)), _f64(m)
+        _host_check(self._L.spg_pg_add_particles(self.h, x.shape[0], _d(x), _d(x0), _d(m)))
+
+    def add_energy(self, kind, stencils, args, prepare=True):
+        t = _kind(kind)
+        st = _i32(stencils, (-1, STENCIL_SIZE[t]))
+        a = _f64(args, (st.shape[0], -1))
+        idx = ctypes.c_int32(0)
+        _host_check(self._L.spg_pg_add_energy(self.h, t, st.shape[0], _i(st), _d(a), int(prepare), ctypes.byref(idx)))
+        self.energy_types.append((t, st.shape[0], st))
+        return idx.value
+
+    def prepare_energy(self, index):
+        _host_check(self._L.spg_pg_prepare_energy(self.h, index))
+
+    def num_particles(self):
+        n = ctypes.c_int64(0)
+        _host_check(self._L.spg_pg_num_particles(self.h, ctypes.byref(n)))
+        return n.value
+
+    def energy_params(self, index):
+        t, n, _ = self.energy_types[index]
+        out = np.zeros((n, PARAMS[t]))
+        _host_check(self._L.spg_pg_energy_params(self.h, index, _d(out)))
+        return out
+
+    def state(self):
+        n = self.num_particles()
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        _host_check(self._L.spg_pg_get_state(self.h, _d(x), _d(v)))
+        return x, v
+
+    def set_state(self, x=None, v=None):
+        _host_check(self._L.spg_pg_set_state(self.h, _d(None if x is None else _f64(x, (-1, 3))),
+                                             _d(None if v is None else _f64(v, (-1, 3)))))
+
+    def vbd_object(self, x=None, v=None, m=None):
+        """Prepared arrays of this group in the layout of spg_vbd_object (for direct C-ABI use)."""
+        xx, vv = self.state()
+        return {"x": xx if x is None else x, "v": vv if v is None else v, "m": m,
+                "energies": [(t, st, self.energy_params(i)) for i, (t, n, st) in enumerate(self.energy_types)]}
+
+
+class VBDGpu:
+    """spg::solver::VBDGpu (the GPU drop-in for spg::solver::VBD) through the C++ mirror."""
+
+    def __init__(self, device=0):
+        self._L = host_lib()
+        self.h = self._L.spg_solver_create_vbd_gpu(device)
+        self._groups = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_solver_destroy(self.h)
+            self.h = None
+
+    def add_object(self, pg):
+        _host_check(self._L.spg_solver_add_object(self.h, pg.h))
+        self._groups.append(pg)
+
+    def set_dt(self, dt):
+        _host_check(self._L.spg_solver_set_dt(self.h, dt))
+
+    def set_num_substeps(self, n):
+        _host_check(self._L.spg_solver_set_num_substeps(self.h, n))
+
+    def set_gravity(self, g):
+        g = _f64(g)
+        _host_check(self._L.spg_solver_set_gravity(self.h, _d(g)))
+
+    def set_iterations(self, n):
+        _host_check(self._L.spg_solver_set_iterations(self.h, n))
+
+    def set_resident_state(self, resident):
+        _host_check(self._L.spg_solver_set_resident_state(self.h, int(resident)))
+
+    def step(self):
+        _host_check(self._L.spg_solver_step(self.h))
+
+    def reset(self):
+        _host_check(self._L.spg_solver_reset(self.h))
+
+    def sync_to_host(self):
+        _host_check(self._L.spg_solver_sync_to_host(self.h))
+
+    def state(self, obj=0):
+        n = self._groups[obj].num_particles()
+        x, v = np.zeros((n, 3)), np.zeros((n, 3))
+        _host_check(self._L.spg_solver_get_state(self.h, obj, _d(x), _d(v)))
+        return x, v
+
+    def context(self):
+        """The C-ABI context behind this solver (valid after the first step), borrowed."""
+        p = self._L.spg_solver_context(self.h)
+        if not p:
+            raise RuntimeError("no context before the first step")
+        return VbdContext._borrow(p, self)
+
+
+# ------------------------------------------------------------------------------------------------
+# Direct C-ABI context
+# ------------------------------------------------------------------------------------------------
+
+class VbdContext:
+    """A spg_vbd_ctx: one or more particle groups advanced together on one GPU."""
+
+    def __init__(self, objects, device=0):
+        """objects: list of dicts with x (n,3), v (n,3) or None, m (n,), energies [(type, stencils, params)]."""
+        self._L = vbd_lib()
+        keep = []
+        objs = (VbdObject * len(objects))()
+        for k, o in enumerate(objects):
+            x = _f64(o["x"], (-1, 3))
+            v = None if o.get("v") is None else _f64(o["v"], (-1, 3))
+            m = _f64(o["m"])
+            ens = (VbdEnergy * max(1, len(o["energies"])))()
+            for j, (t, st, prm) in enumerate(o["energies"]):
+                t = _kind(t)
+                st = _i32(st, (-1, STENCIL_SIZE[t]))
+                prm = _f64(prm, (st.shape[0], PARAMS[t]))
+                ens[j] = VbdEnergy(t, st.shape[0], _i(st), _d(prm))
+                keep += [st, prm]
+            objs[k] = VbdObject(x.shape[0], _d(x), _d(v), _d(m), len(o["energies"]), ens)
+            keep += [x, v, m, ens]
+        h = ctypes.c_void_p()
+        rc = self._L.spg_vbd_create(ctypes.byref(h), objs, len(objects), device)
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_vbd_last_error(None).decode())
+        self.h = h.value
+        self._owned = True
+        self._owner = None
+        self.n_objects = len(objects)
+        self._n = [int(_f64(o["x"]).size // 3) for o in objects]
+
+    @classmethod
+    def _borrow(cls, ptr, owner):
+        self = cls.__new__(cls)
+        self._L = vbd_lib()
+        self.h = ptr
+        self._owned = False
+        self._owner = owner
+        n = ctypes.c_int32(0)
+        self._L.spg_vbd_num_objects(ptr, ctypes.byref(n))
+        self.n_objects = n.value
+        self._n = []
+        for o in range(self.n_objects):
+            k = ctypes.c_int64(0)
+            self._L.spg_vbd_num_particles(ptr, o, ctypes.byref(k))
+            self._n.append(k.value)
+        return self
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "h", None):
+            self._L.spg_vbd_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_vbd_last_error(self.h).decode())
+
+    def num_particles(self, obj=0):
+        return self._n[obj]
+
+    def colors(self, obj=0):
+        out = np.zeros(self._n[obj], dtype=np.int32)
+        self._check(self._L.spg_vbd_get_colors(self.h, obj, _i(out)))
+        return out
+
+    def num_colors(self, obj=0):
+        n = ctypes.c_int32(0)
+        self._check(self._L.spg_vbd_num_colors(self.h, obj, ctypes.byref(n)))
+        return n.value
+
+    def state(self, obj=0):
+        x, v = np.zeros((self._n[obj], 3)), np.zeros((self._n[obj], 3))
+        self._check(self._L.spg_vbd_get_state(self.h, obj, _d(x), _d(v)))
+        return x, v
+
+    def set_state(self, obj, x=None, v=None):
+        self._check(self._L.spg_vbd_set_state(self.h, obj, _d(None if x is None else _f64(x, (-1, 3))),
+                                              _d(None if v is None else _f64(v, (-1, 3)))))
+
+    def step(self, dt, nsubsteps=1, iterations=1, gravity=(0.0, -9.8, 0.0)):
+        g = _f64(gravity)
+        self._check(self._L.spg_vbd_step(self.h, dt, nsubsteps, iterations, _d(g)))
+
+    def begin_substep(self, h, gravity):
+        g = _f64(gravity)
+        self._check(self._L.spg_vbd_begin_substep(self.h, h, _d(g)))
+
+    def set_guess(self, obj, x):
+        self._check(self._L.spg_vbd_set_guess(self.h, obj, _d(_f64(x, (-1, 3)))))
+
+    def iterate(self, iterations):
+        self._check(self._L.spg_vbd_iterate(self.h, iterations))
+
+    def end_substep(self):
+        self._check(self._L.spg_vbd_end_substep(self.h))
+
+    def residual(self):
+        out = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_residual(self.h, _d(out)))
+        return out
+
+    def residual_at_inertial(self):
+        out = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_residual_at_inertial(self.h, _d(out)))
+        return out
+
+    def launch(self):
+        c = VbdLaunch()
+        self._check(self._L.spg_vbd_get_launch(self.h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def set_launch(self, **kw):
+        c = VbdLaunch()
+        self._check(self._L.spg_vbd_get_launch(self.h, ctypes.byref(c)))
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        self._check(self._L.spg_vbd_set_launch(self.h, ctypes.byref(c)))
+
+    def autotune(self, trials=3):
+        c = VbdLaunch()
+        self._check(self._L.spg_vbd_autotune(self.h, trials, ctypes.byref(c)))
+        return c.as_dict()
+
+    def stats(self):
+        s = VbdStats()
+        self._check(self._L.spg_vbd_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+    def reset_stats(self):
+        self._check(self._L.spg_vbd_reset_stats(self.h))
+
+    def set_timing(self, enabled):
+        self._check(self._L.spg_vbd_set_timing(self.h, int(enabled)))
+
+    def synchronize(self):
+        self._check(self._L.spg_vbd_synchronize(self.h))
+
+
+def color_vertices(n_vertices, sets):
+    """coloring::colorVertices over [(flat stencils, stencil size)]."""
+    L = vbd_lib()
+    arrs = [_i32(s).ravel() for s, _ in sets]
+    ptrs = (_ip * max(1, len(arrs)))(*[_i(a) for a in arrs])
+    lens = np.array([a.size for a in arrs] or [0], dtype=np.int64)
+    sizes = np.array([k for _, k in sets] or [1], dtype=np.int32)
+    out = np.zeros(n_vertices, dtype=np.int32)
+    rc = L.spg_vbd_color_vertices(n_vertices, len(arrs), ptrs, lens.ctypes.data_as(_lp), _i(sizes), _i(out))
+    if rc != 0:
+        raise NativeError(rc, L.spg_vbd_last_error(None).decode())
+    return out
+
+
+def element_derivatives(kind, xs, params, precision=0, device=0):
+    """GPU evaluation of per-vertex gradients (n, N, 3) and diagonal Hessian blocks (n, N, 3, 3)."""
+    L = vbd_lib()
+    t = _kind(kind)
+    ns = STENCIL_SIZE[t]
+    xs = _f64(xs, (-1, ns * 3))
+    n = xs.shape[0]
+    p = _f64(params, (n, PARAMS[t]))
+    g = np.zeros((n, ns, 3))
+    h = np.zeros((n, ns, 3, 3))
+    rc = L.spg_vbd_element_derivatives(t, n, _d(xs), _d(p), precision, _d(g), _d(h), device)
+    if rc != 0:
+        raise NativeError(rc, L.spg_vbd_last_error(None).decode())
+    return g, h

@@ -1,0 +1,203 @@
+"""ctypes access to the FP64 CPU oracle (oracle/liboracle.so) — test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use this module.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+SPRING, MEMBRANE_STVK, DISCRETE_BENDING, STABLE_NEO_HOOKEAN, SPRING_ANCHOR = 1, 2, 3, 4, 5
+CLOSED_FORM, AD = 0, 1
+STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1}
+KIND = {"spring": SPRING, "stvk": MEMBRANE_STVK, "bending": DISCRETE_BENDING, "snh": STABLE_NEO_HOOKEAN,
+        "anchor": SPRING_ANCHOR}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_lp = ctypes.POINTER(ctypes.c_int64)
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle not built: {LIB_PATH} (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_last_error.restype = ctypes.c_char_p
+        L.orc_new.restype = ctypes.c_void_p
+        L.orc_new.argtypes = [ctypes.c_int64, _dp, _dp, _dp]
+        L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_add_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _ip, _dp]
+        L.orc_prepare.argtypes = [ctypes.c_void_p]
+        L.orc_get_prepared.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_set_prepared.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_colors.argtypes = [ctypes.c_void_p, _ip, _ip]
+        L.orc_incidence_count.argtypes = [ctypes.c_void_p, _lp]
+        L.orc_incidence.argtypes = [ctypes.c_void_p, _ip, _ip, _ip, _ip]
+        L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_substep_curve.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_get_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_set_state.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_reset.argtypes = [ctypes.c_void_p]
+        L.orc_residual.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp]
+        L.orc_energy.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.orc_element.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_bending_stencils.argtypes = [ctypes.c_int64, _ip, ctypes.c_int64, _ip, _lp]
+        L.orc_coloring_valid.argtypes = [ctypes.c_void_p, _ip]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_num_prepared_params.argtypes = [ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+def _check(rc):
+    if rc != 0:
+        raise OracleError(lib().orc_last_error().decode())
+
+
+def set_threads(n):
+    return lib().orc_set_threads(int(n))
+
+
+def element(kind, xs, prepared, mode=CLOSED_FORM, full_hess=False):
+    """Energy, gradient (3N) and diagonal 3x3 blocks (N,3,3) of one element; full Hessian in AD mode."""
+    t = KIND[kind] if isinstance(kind, str) else kind
+    ns = STENCIL_SIZE[t]
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    p = np.ascontiguousarray(prepared, dtype=np.float64)
+    E = np.zeros(1)
+    g = np.zeros(3 * ns)
+    hb = np.zeros(9 * ns)
+    fh = np.zeros((3 * ns) ** 2) if full_hess else None
+    _check(lib().orc_element(t, _d(xs), _d(p), mode, _d(E), _d(g), _d(hb), _d(fh) if full_hess else None))
+    out = (E[0], g, hb.reshape(ns, 3, 3))
+    if full_hess:
+        out = out + (fh.reshape(3 * ns, 3 * ns),)
+    return out
+
+
+def bending_stencils(faces, nv):
+    faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
+    n = ctypes.c_int64(0)
+    _check(lib().orc_bending_stencils(faces.shape[0], _i(faces), nv, None, ctypes.byref(n)))
+    out = np.zeros((n.value, 4), dtype=np.int32)
+    _check(lib().orc_bending_stencils(faces.shape[0], _i(faces), nv, _i(out), ctypes.byref(n)))
+    return out
+
+
+class Oracle:
+    """One ParticleGroup with its energies and the VBD solver state, FP64 on the CPU."""
+
+    def __init__(self, x, x0, m):
+        self.x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        self.nv = x.shape[0]
+        self.h = lib().orc_new(self.nv, _d(x), _d(self.x0), _d(m))
+        self.types = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_free(self.h)
+            self.h = None
+
+    def add_energy(self, kind, stencils, raw):
+        t = KIND[kind] if isinstance(kind, str) else kind
+        st = np.ascontiguousarray(stencils, dtype=np.int32).reshape(-1, STENCIL_SIZE[t])
+        raw = np.ascontiguousarray(raw, dtype=np.float64).reshape(st.shape[0], -1)
+        _check(lib().orc_add_energy(self.h, t, st.shape[0], _i(st), _d(raw)))
+        self.types.append((t, st.shape[0]))
+
+    def prepare(self):
+        _check(lib().orc_prepare(self.h))
+
+    def prepared(self, e):
+        t, n = self.types[e]
+        out = np.zeros((n, lib().orc_num_prepared_params(t)))
+        _check(lib().orc_get_prepared(self.h, e, _d(out)))
+        return out
+
+    def colors(self):
+        out = np.zeros(self.nv, dtype=np.int32)
+        nc = ctypes.c_int32(0)
+        _check(lib().orc_colors(self.h, _i(out), ctypes.byref(nc)))
+        return out, nc.value
+
+    def incidence(self):
+        n = ctypes.c_int64(0)
+        _check(lib().orc_incidence_count(self.h, ctypes.byref(n)))
+        arrs = [np.zeros(n.value, dtype=np.int32) for _ in range(4)]
+        _check(lib().orc_incidence(self.h, *[_i(a) for a in arrs]))
+        return tuple(arrs)
+
+    def coloring_valid(self, colors):
+        c = np.ascontiguousarray(colors, dtype=np.int32)
+        return bool(lib().orc_coloring_valid(self.h, _i(c)))
+
+    def step(self, dt, nsubsteps=1, iterations=1, gravity=(0.0, -9.8, 0.0), mode=CLOSED_FORM):
+        g = np.asarray(gravity, dtype=np.float64)
+        _check(lib().orc_step(self.h, dt, nsubsteps, iterations, _d(g), mode))
+
+    def substep_curve(self, dt, iterations, gravity, mode=CLOSED_FORM, start_guess=None):
+        g = np.asarray(gravity, dtype=np.float64)
+        rho = np.zeros(iterations)
+        sg = None
+        if start_guess is not None:
+            sg = np.ascontiguousarray(start_guess, dtype=np.float64).reshape(-1, 3)
+        _check(lib().orc_substep_curve(self.h, dt, iterations, _d(g), mode, _d(sg) if sg is not None else None,
+                                       _d(rho)))
+        return rho
+
+    def state(self):
+        x = np.zeros((self.nv, 3))
+        v = np.zeros((self.nv, 3))
+        lib().orc_get_state(self.h, _d(x), _d(v))
+        return x, v
+
+    def set_state(self, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1, 3)
+        lib().orc_set_state(self.h, _d(x), _d(v))
+
+    def reset(self):
+        lib().orc_reset(self.h)
+
+    def residual(self, x, xt, dt, mode=CLOSED_FORM, return_vector=False):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        xt = np.ascontiguousarray(xt, dtype=np.float64).reshape(-1, 3)
+        nrm = np.zeros(1)
+        r = np.zeros((self.nv, 3)) if return_vector else None
+        _check(lib().orc_residual(self.h, _d(x), _d(xt), dt, mode, _d(nrm), _d(r) if return_vector else None))
+        return (nrm[0], r) if return_vector else nrm[0]
+
+    def energy(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        e = np.zeros(1)
+        _check(lib().orc_energy(self.h, _d(x), _d(e)))
+        return e[0]
+
+
+def from_scene(scene):
+    """Build an oracle object from a raw scene description (spg_amd.scenes.Scene)."""
+    o = Oracle(scene.x, scene.x0, scene.m)
+    for kind, st, raw in scene.energies:
+        o.add_energy(kind, st, raw)
+    o.prepare()
+    return o

@@ -1,0 +1,292 @@
+"""GPU parity: the HIP VBD path (through the C-ABI of include/spg_vbd.h) against the FP64 oracle on
+the same inputs.
+
+Bars (from the workload spec):
+  * colouring: bit-exact integer equality with the oracle's greedy colouring;
+  * element derivatives: the device closed forms, instantiated in FP64 on the GPU, within 1e-6
+    relative of the 40-digit golden values; the FP32 instantiation within FP32 rounding (1e-4
+    relative to the block's scale);
+  * one step: positions within 1e-5 relative (max-norm error / max-norm of the positions) of the
+    FP64 oracle started from the same FP32-rounded state;
+  * residual: the GPU FP64 residual of a state within 1e-9 relative of the oracle's residual of the
+    same state.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_common import disp_rel_err, gpu_context, oracle_from_gpu_start, rel_err
+from spg_amd import native, scenes
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = json.load(open(os.path.join(HERE, "golden", "element_golden.json")))["cases"]
+
+
+def _golden_arrays(kind):
+    cases = [c for c in GOLD if c["kind"] == kind]
+    ns = native.STENCIL_SIZE[native.KIND[kind]]
+    xs = np.array([c["x"] for c in cases])
+    prm = np.array([[float(v) for v in c["prepared"]] for c in cases])
+    g = np.array([[float(v) for v in c["grad"]] for c in cases]).reshape(len(cases), ns, 3)
+    H = np.array([[[float(v) for v in r] for r in c["hess"]] for c in cases])
+    hb = np.array([[H[k, 3 * s:3 * s + 3, 3 * s:3 * s + 3] for s in range(ns)] for k in range(len(cases))])
+    return xs, prm, g, hb
+
+
+@pytest.mark.parametrize("kind", ["spring", "anchor", "stvk", "bending", "snh"])
+def test_element_derivatives_fp64_on_gpu(kind, oracle_threads):
+    xs, prm, g_ref, h_ref = _golden_arrays(kind)
+    g, h = native.element_derivatives(kind, xs, prm, precision=1)
+    for k in range(len(xs)):
+        sg = max(np.max(np.abs(g_ref[k])), 1e-300)
+        sh = max(np.max(np.abs(h_ref[k])), 1e-300)
+        if np.max(np.abs(g_ref[k])) > 0:
+            assert np.max(np.abs(g[k] - g_ref[k])) / sg < 1e-6, (kind, k)
+        else:
+            assert np.max(np.abs(g[k])) < 1e-10
+        assert np.max(np.abs(h[k] - h_ref[k])) / sh < 1e-6, (kind, k)
+
+
+@pytest.mark.parametrize("kind", ["spring", "anchor", "stvk", "bending", "snh"])
+def test_element_derivatives_fp32_on_gpu(kind):
+    xs, prm, g_ref, h_ref = _golden_arrays(kind)
+    g, h = native.element_derivatives(kind, xs, prm, precision=0)
+    worst_g = worst_h = 0.0
+    for k in range(len(xs)):
+        sg = np.max(np.abs(g_ref[k]))
+        sh = max(np.max(np.abs(h_ref[k])), 1e-300)
+        if sg > 1e-6 * sh:  # skip gradients that are zero up to the element's stiffness scale
+            worst_g = max(worst_g, np.max(np.abs(g[k] - g_ref[k])) / sg)
+        worst_h = max(worst_h, np.max(np.abs(h[k] - h_ref[k])) / sh)
+    print(f"{kind}: fp32 worst rel grad {worst_g:.2e} hess {worst_h:.2e}")
+    assert worst_g < 1e-4 and worst_h < 1e-4
+
+
+def test_colors_bit_exact_config1():
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    o = ol.from_scene(s)
+    c, nc = o.colors()
+    assert ctx.num_colors() == nc == 4
+    assert np.array_equal(ctx.colors(), c)
+
+
+def test_config1_single_step(oracle_threads):
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    x_start = o.state()[0]
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    xg, vg = ctx.state()
+    xo, vo = o.state()
+    e, ed = rel_err(xg, xo), disp_rel_err(xg, xo, x_start)
+    print(f"config1 step: rel {e:.2e} disp-rel {ed:.2e}")
+    assert e < 1e-5
+    assert np.max(np.abs(vg - vo)) < 1e-5 * 60 * np.max(np.abs(xo)) * 10
+
+
+def test_config1_ad_mode_oracle_agrees(oracle_threads):
+    """The oracle's AD back-end (TinyAD cost profile) and closed-form back-end give the same step."""
+    s = scenes.cloth_config1()
+    o1, o2 = ol.from_scene(s), ol.from_scene(s)
+    o1.step(s.dt, 1, 2, s.gravity, mode=ol.CLOSED_FORM)
+    o2.step(s.dt, 1, 2, s.gravity, mode=ol.AD)
+    assert rel_err(o1.state()[0], o2.state()[0]) < 1e-12
+
+
+def test_config1_trajectory(oracle_threads):
+    """100 steps: FP32 GPU and FP64 oracle stay close (no blow-up, same motion)."""
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    worst = 0.0
+    for k in range(100):
+        ctx.step(s.dt, 1, s.iterations, s.gravity)
+        o.step(s.dt, 1, s.iterations, s.gravity)
+        if k % 10 == 9:
+            worst = max(worst, rel_err(ctx.state()[0], o.state()[0]))
+    print(f"config1 100 steps: worst rel {worst:.2e}")
+    assert np.all(np.isfinite(ctx.state()[0]))
+    assert worst < 1e-3
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_config2_single_step_512(seed, oracle_threads):
+    """512^2 cloth, 20 iterations. Seeds 3 and 4 draw Y = 4.95e4 and 5.68e4, for which the reference
+    VBD (FP64, exact Hessians, no PSD projection) converges; see the divergence test below."""
+    s = scenes.cloth_config2(seed=seed)
+    ctx = gpu_context(s)
+    assert np.bincount(ctx.colors()).tolist() == [65536] * 4
+    o = oracle_from_gpu_start(s, ctx)
+    x_start = o.state()[0]
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    xg, xo = ctx.state()[0], o.state()[0]
+    e, ed = rel_err(xg, xo), disp_rel_err(xg, xo, x_start)
+    print(f"config2 (Y={s.meta['young']:.4g}) step: rel {e:.2e} disp-rel {ed:.2e}")
+    assert e < 1e-5
+
+
+def test_config2_diverging_draw_matches_before_divergence(oracle_threads):
+    """Seed 1 draws Y = 3.06e4, for which the reference VBD diverges within the first step (rho grows
+    from 7e-3 after one sweep to ~7e2 after 20). Before the divergence GPU and oracle agree, and both
+    then leave the basin."""
+    s = scenes.cloth_config2(seed=1)
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    ctx.begin_substep(s.dt, s.gravity)
+    ctx.iterate(1)
+    xg = ctx.state()[0]
+    rg = ctx.residual()[0] / ctx.residual_at_inertial()[0]
+    ctx.iterate(19)
+    rg20 = ctx.residual()[0] / ctx.residual_at_inertial()[0]
+    ctx.end_substep()
+    rho = o.substep_curve(s.dt, 1, s.gravity)
+    e = rel_err(xg, o.state()[0])
+    print(f"seed1: after 1 sweep rel {e:.2e}, rho gpu {rg:.3e} oracle {rho[0]:.3e}; gpu rho after 20 {rg20:.3e}")
+    assert e < 1e-5
+    assert abs(rg - rho[0]) < 1e-3 * rho[0]
+    assert rg20 > 1.0
+
+
+def test_config3_cube_single_step(oracle_threads):
+    """64^3 Kuhn cube (274,625 vertices, 1,572,864 SNH tets), 20 iterations."""
+    s = scenes.cube_config3()
+    ctx = gpu_context(s)
+    assert np.bincount(ctx.colors()).tolist() == [35937, 51520, 39813, 48895, 35925, 28263, 17136, 17136]
+    o = oracle_from_gpu_start(s, ctx)
+    x_start = o.state()[0]
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    xg, xo = ctx.state()[0], o.state()[0]
+    e, ed = rel_err(xg, xo), disp_rel_err(xg, xo, x_start)
+    print(f"config3 step: rel {e:.2e} disp-rel {ed:.2e}")
+    assert e < 1e-5
+
+
+def test_cube_single_step_small(oracle_threads):
+    s = scenes.cube(8, iterations=20)
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    c, _ = o.colors()
+    assert np.array_equal(ctx.colors(), c)
+    x_start = o.state()[0]
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    xg, xo = ctx.state()[0], o.state()[0]
+    e, ed = rel_err(xg, xo), disp_rel_err(xg, xo, x_start)
+    print(f"cube8 step: rel {e:.2e} disp-rel {ed:.2e}")
+    assert e < 1e-5
+
+
+def test_springs_single_step(oracle_threads):
+    """Mass-spring network on the cube's tet edges (apps/solverCompare.cpp:260-330 style)."""
+    x0, tets = scenes.kuhn_cube_mesh(6)
+    edges = np.unique(np.sort(np.concatenate([tets[:, [a, b]] for a in range(4) for b in range(a + 1, 4)]), axis=1),
+                      axis=0)
+    L0 = np.linalg.norm(x0[edges[:, 0]] - x0[edges[:, 1]], axis=1)
+    pins = np.flatnonzero(x0[:, 0] == 0.0)
+    x = x0 * np.array([1.0, 1.03, 1.0])
+    s = scenes.Scene(x=x, x0=x0, m=np.full(len(x0), 1e-2),
+                     energies=[("anchor", pins[:, None], np.column_stack([x0[pins], np.full(len(pins), 1e5)])),
+                               ("spring", edges, np.column_stack([L0, np.full(len(edges), 1e2)]))],
+                     iterations=10)
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    assert np.array_equal(ctx.colors(), o.colors()[0])
+    for _ in range(3):
+        ctx.step(s.dt, 1, s.iterations, s.gravity)
+        o.step(s.dt, 1, s.iterations, s.gravity)
+    e = rel_err(ctx.state()[0], o.state()[0])
+    print(f"springs 3 steps: rel {e:.2e}")
+    assert e < 1e-5
+
+
+def test_residual_matches_oracle(oracle_threads):
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    o = ol.from_scene(s)
+    ctx.begin_substep(s.dt, s.gravity)
+    # x_tilde on the GPU is x at this point
+    xt, _ = ctx.state()
+    r0 = ctx.residual_at_inertial()[0]
+    ctx.iterate(3)
+    r = ctx.residual()[0]
+    x, _ = ctx.state()
+    ro0 = o.residual(xt, xt, s.dt)
+    ro = o.residual(x, xt, s.dt)
+    ctx.end_substep()
+    print(f"residual gpu {r:.10e} oracle {ro:.10e}; at x_tilde {r0:.6e} / {ro0:.6e}")
+    assert abs(r - ro) <= 1e-9 * ro
+    assert abs(r0 - ro0) <= 1e-9 * ro0
+
+
+def test_graph_replay_bitwise():
+    s = scenes.cloth_config1()
+    a, b = gpu_context(s), gpu_context(s, use_graph=1)
+    for _ in range(5):
+        a.step(s.dt, 1, s.iterations, s.gravity)
+        b.step(s.dt, 1, s.iterations, s.gravity)
+        # eager work between replays
+        assert np.array_equal(a.state()[0], b.state()[0])
+    assert np.array_equal(a.state()[1], b.state()[1])
+
+
+def test_vbdgpu_mirror_matches_capi():
+    """spg::solver::VBDGpu (C++ mirror of the BaseSolver API) gives the same result as the C-ABI."""
+    s = scenes.cloth_config1()
+    pg = s.particle_group()
+    solver = native.VBDGpu()
+    solver.add_object(pg)
+    solver.set_dt(s.dt)
+    solver.set_gravity(s.gravity)
+    solver.set_iterations(s.iterations)
+    ctx = gpu_context(s)
+    for _ in range(3):
+        solver.step()
+        ctx.step(s.dt, 1, s.iterations, s.gravity)
+    xm, vm = solver.state()
+    xc, vc = ctx.state()
+    assert np.array_equal(xm, xc) and np.array_equal(vm, vc)
+
+
+def test_batched_objects_match_independent_oracles(oracle_threads):
+    batch = scenes.cloth_batch(count=6, seed=3, lo=8, hi=24, iterations=15)
+    ctx = gpu_context(batch)
+    oracles = [oracle_from_gpu_start(sc, ctx, k) for k, sc in enumerate(batch)]
+    for k, sc in enumerate(batch):
+        assert np.array_equal(ctx.colors(k), oracles[k].colors()[0])
+    for _ in range(2):
+        ctx.step(scenes.DT, 1, 15, scenes.GRAVITY)
+        for o in oracles:
+            o.step(scenes.DT, 1, 15, scenes.GRAVITY)
+    for k, o in enumerate(oracles):
+        e = rel_err(ctx.state(k)[0], o.state()[0])
+        assert e < 1e-5, (k, e)
+
+
+def test_nonfinite_state_rejected():
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    x0, v0 = ctx.state()
+    bad = x0.copy()
+    bad[5, 1] = np.nan
+    with pytest.raises(native.NativeError) as ei:
+        ctx.set_state(0, bad, None)
+    assert ei.value.code == native.ERR_NONFINITE
+    assert np.array_equal(ctx.state()[0], x0)
+
+
+def test_substep_protocol_errors():
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    with pytest.raises(native.NativeError) as ei:
+        ctx.iterate(1)
+    assert ei.value.code == native.ERR_STATE
+    with pytest.raises(native.NativeError):
+        ctx.residual()

@@ -85,7 +85,7 @@ typedef struct {
 
 /* Launch shape of the per-colour vertex solve (chosen by spg_vbd_autotune or set by hand). */
 typedef struct {
-    int32_t lanes_per_vertex;    /* 1, 2, 4, 8 or 16 lanes of a wavefront cooperate on one vertex */
+    int32_t lanes_per_vertex;    /* 1, 2, 3, 4, 6, 8 or 16 lanes of a wavefront cooperate on one vertex */
     int32_t block_size;          /* threads per workgroup: 64, 128, 256, 512 or 1024 */
     int32_t use_graph;           /* 1: capture a whole step in a hipGraph and replay it */
     int32_t lds_staging;         /* 1: stage the block's element records through LDS */
@@ -128,6 +128,13 @@ int spg_vbd_set_guess(spg_vbd_ctx *ctx, int32_t object, const double *x);
 int spg_vbd_iterate(spg_vbd_ctx *ctx, int32_t iterations);
 int spg_vbd_end_substep(spg_vbd_ctx *ctx);
 
+/* Checkpoint / resume on the device (VBD is Markov in (x, v), particleGroup.cpp:97-103 restores the
+   initial state the same way): save copies the current positions and velocities of all objects into
+   a device snapshot, restore copies them back. Both are enqueued on the context's stream (no host
+   synchronisation) and restore-then-step reproduces the saved step bit for bit. */
+int spg_vbd_save_state(spg_vbd_ctx *ctx);
+int spg_vbd_restore_state(spg_vbd_ctx *ctx);
+
 /* FP64 force residual of the current substep: ||-grad E(x) - (M/h^2)(x - x_tilde)||_2 per object
    (norms has n_objects entries), computed on the GPU in FP64. Valid between begin and end. */
 int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms);
@@ -152,11 +159,21 @@ int spg_vbd_synchronize(spg_vbd_ctx *ctx);
 int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *const *flat_stencils,
                            const int64_t *lengths, const int32_t *stencil_sizes, int32_t *colors);
 
-/* Per-element derivatives on the GPU (test hook for the element kernels): for each of n stencils of
-   one type (positions xs: n * size * 3, prepared params: n * nparams), the gradient w.r.t. each stencil
-   vertex (n*size*3) and its diagonal 3x3 Hessian block (n*size*9), in FP32 (precision 0) or FP64 (1). */
+/* Plain float4 device-to-device copy of `bytes` (multiple of 16) repeated `repeats` times, for
+   calibrating HBM traffic counters against a kernel of known traffic (bytes read + bytes written per
+   launch). *ms receives the mean launch time from HIP events. Not part of the solver. */
+int spg_vbd_bench_copy(int64_t bytes, int32_t repeats, int32_t device, double *ms);
+
+/* Per-element derivatives (check hook for the element closed forms; not used by the solver): for each
+   of n stencils of one type (positions xs: n * size * 3, prepared params: n * nparams), the gradient
+   w.r.t. each stencil vertex (n*size*3) and its diagonal 3x3 Hessian block (n*size*9), evaluated in
+   FP32 (precision 0) or FP64 (1) by the same device functions the solve uses, on GPU `device`, or
+   on the host when device is -1. form 0 selects the closed forms of the solve; forms 1 (general
+   (a, b, sigma) hinge form) and 2 (single-expression compact hinge form) are independent derivations
+   of the hinge derivatives, for cross-checking. */
 int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params,
-                                int32_t precision, double *grad, double *hess_blocks, int32_t device);
+                                int32_t precision, int32_t form, int32_t device, double *grad,
+                                double *hess_blocks);
 
 #ifdef __cplusplus
 }

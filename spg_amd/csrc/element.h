@@ -191,6 +191,111 @@ SPG_HD void snhVertex(V3<R> xo, V3<R> xa, V3<R> xb, V3<R> xc, V3<R> ga, V3<R> gb
 }
 
 // ---------------------------------------------------------------------------------------------
+// Discrete bending, compact form used by the solve. Same angle theta = atan2(s, c) as below. With
+// the signed angles alpha, beta of a and b about the hinge axis, theta = beta - alpha + pi and the
+// frame rotation cancels in the difference, so grad_v theta = B^T (e^ x b)/|b|^2 - A^T (e^ x a)/|a|^2.
+// For every slot this collapses to
+//   grad theta = sg (ra.e^)/|a|^2 a - sg (rb.e^)/|b|^2 b
+//   Hess theta = 1/2 sym(a, ua) + 1/2 sym(b, ub),
+//   ua =  ra_perp/(L |a|^2) + 2 (ra.e^) (a x ra)/|a|^4,
+//   ub = -rb_perp/(L |b|^2) - 2 (rb.e^) (b x rb)/|b|^4,
+// (r_perp = r - (r.e^) e^; the skew parts of the Jacobian cancel, so only symmetric parts remain) with
+//   slot 0: ra = x2 - x1, rb = x3 - x1, sg = -1     slot 1: ra = x2 - x0, rb = x3 - x0, sg = +1
+//   slot 2: ra = e, rb = 0, sg = -1                 slot 3: ra = 0, rb = e, sg = -1
+// ---------------------------------------------------------------------------------------------
+// Hinge slots 0 and 1 (the hinge edge's vertices): ra, rb of the compact form. Both slots share
+// ra_perp = (x2 - x1)_perp and rb_perp = (x3 - x1)_perp since e_perp = 0.
+template <class R>
+SPG_HD void bendingEdge(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot1, R theta0, R k, V3<R> &g, Sym3<R> &H)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> p = x2 - x1, q = x3 - x1;
+    const V3<R> a = cross(e, p), b = cross(e, q);
+    const R L = sqrt(dot(e, e));
+    const R iL = R(1) / L;
+    const V3<R> eh = iL * e;
+    const R pe = dot(p, eh), qe = dot(q, eh);
+    const V3<R> pp = p - pe * eh, qp = q - qe * eh;  // perpendicular parts
+    const R rae = slot1 ? pe + L : pe, rbe = slot1 ? qe + L : qe;
+    const V3<R> ra = slot1 ? p + e : p, rb = slot1 ? q + e : q;
+    const R sg = slot1 ? R(1) : R(-1);
+    const R ia2 = R(1) / dot(a, a), ib2 = R(1) / dot(b, b);
+    const V3<R> gth = (sg * rae * ia2) * a + (-sg * rbe * ib2) * b;
+    const V3<R> ua = (iL * ia2) * pp + (R(2) * rae * ia2 * ia2) * cross(a, ra);
+    const V3<R> ub = (-iL * ib2) * qp + (R(-2) * rbe * ib2 * ib2) * cross(b, rb);
+    const R theta = atan2(-dot(cross(a, b), eh), -dot(a, b));
+    const R kC = k * (theta - theta0);
+    g = kC * gth;
+    H = zeroSym<R>();
+    addOuter(H, k, gth);
+    addSymOuter(H, R(0.5) * kC, a, ua);
+    addSymOuter(H, R(0.5) * kC, b, ub);
+}
+
+// Hinge slots 2 and 3 (the wing vertices): with w the normal of the own wing (a for slot 2, b for
+// slot 3) and s_w = -1 (slot 2) / +1 (slot 3): grad theta = s_w L w/|w|^2,
+// Hess theta = -s_w L sym(w, w x e)/|w|^4.
+template <class R>
+SPG_HD void bendingWing(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot3, R theta0, R k, V3<R> &g, Sym3<R> &H)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> a = cross(e, x2 - x1), b = cross(e, x3 - x1);
+    const R L = sqrt(dot(e, e));
+    const R iL = R(1) / L;
+    const R theta = atan2(-dot(cross(a, b), e) * iL, -dot(a, b));
+    const V3<R> w = slot3 ? b : a;
+    const R sw = slot3 ? R(1) : R(-1);
+    const R iw2 = R(1) / dot(w, w);
+    const V3<R> gth = (sw * L * iw2) * w;
+    const R kC = k * (theta - theta0);
+    g = kC * gth;
+    H = zeroSym<R>();
+    addOuter(H, k, gth);
+    addSymOuter(H, -kC * sw * L * iw2 * iw2, w, cross(w, e));
+}
+
+template <class R>
+SPG_HD void bendingVertexCompact(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, R theta0, R k, V3<R> &g,
+                                 Sym3<R> &H)
+{
+    if (slot < 2)
+        bendingEdge<R>(x0, x1, x2, x3, slot == 1, theta0, k, g, H);
+    else
+        bendingWing<R>(x0, x1, x2, x3, slot == 3, theta0, k, g, H);
+}
+
+// Single-expression version of the compact form for every slot (kept for cross-checks).
+template <class R>
+SPG_HD void bendingVertexUnified(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, R theta0, R k, V3<R> &g,
+                                 Sym3<R> &H)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> p = x2 - x1, q = x3 - x1;
+    const V3<R> a = cross(e, p), b = cross(e, q);
+    const R L = sqrt(dot(e, e));
+    const R iL = R(1) / L;
+    const V3<R> eh = iL * e;
+    const V3<R> zero = {R(0), R(0), R(0)};
+    const V3<R> ra = slot == 0 ? p : (slot == 1 ? x2 - x0 : (slot == 2 ? e : zero));
+    const V3<R> rb = slot == 0 ? q : (slot == 1 ? x3 - x0 : (slot == 3 ? e : zero));
+    const R sg = slot == 1 ? R(1) : R(-1);
+    const R ia2 = R(1) / dot(a, a), ib2 = R(1) / dot(b, b);
+    const R rae = dot(ra, eh), rbe = dot(rb, eh);
+    const V3<R> gth = (sg * rae * ia2) * a + (-sg * rbe * ib2) * b;
+    const V3<R> ua = (iL * ia2) * (ra - rae * eh) + (R(2) * rae * ia2 * ia2) * cross(a, ra);
+    const V3<R> ub = (-iL * ib2) * (rb - rbe * eh) + (R(-2) * rbe * ib2 * ib2) * cross(b, rb);
+    // theta as atan2(s, c): s = -(a x b).e^, c = -a.b
+    const R theta = atan2(-dot(cross(a, b), eh), -dot(a, b));
+    const R C = theta - theta0;
+    const R kC = k * C;
+    g = kC * gth;
+    H = zeroSym<R>();
+    addOuter(H, k, gth);
+    addSymOuter(H, R(0.5) * kC, a, ua);
+    addSymOuter(H, R(0.5) * kC, b, ub);
+}
+
+// ---------------------------------------------------------------------------------------------
 // Discrete bending (discreteBendingEnergy.cpp:9-30): hinge x0-x1, wings x2, x3;
 // a = e x (x2 - x1), b = e x (x3 - x1), e = x1 - x0. The reference's angle
 // theta = atan2((n1 x n2).e^, n1.n2) with n1 = a^, n2 = -b^ equals atan2(s, c) with

@@ -14,6 +14,7 @@
 
 #include "coloring.h"
 #include "spg_vbd.h"
+#include "element_eval.h"
 #include "vbd_device.h"
 
 using namespace spgvbd;
@@ -126,6 +127,7 @@ struct spg_vbd_ctx {
 
     // device state
     DevBuf<float4> pos, xt, xold, vel;
+    DevBuf<float4> savedPos, savedVel;  // device checkpoint
     DevBuf<float> mass;
     DevBuf<double> massd;  // FP64 masses for the residual
     DevBuf<unsigned long long> counters;
@@ -134,16 +136,16 @@ struct spg_vbd_ctx {
     DevBuf<int64_t> objOffsets;
 
     // incidence streams
-    DevBuf<int32_t> sprPtr, ancPtr, triPtr, hinPtr, tetPtr;
+    DevBuf<int32_t> sprPtr, ancPtr, triPtr, hinEPtr, hinWPtr, tetPtr;
     DevBuf<float4> spr, anc;
     DevBuf<int2> triO;
     DevBuf<float4> triG, triK;
-    DevBuf<int4> hinV;
-    DevBuf<float2> hinP;
+    DevBuf<int4> hinEV, hinWV;
+    DevBuf<float2> hinEP, hinWP;
     DevBuf<int4> tetO;
     DevBuf<float4> tetGa, tetGb, tetGc;
     // FP64 tables for the residual
-    DevBuf<int32_t> sprElem, ancElem, triElem, hinElem, tetElem;
+    DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int2> sprV;
     DevBuf<double2> sprP;
     DevBuf<double4> ancP;
@@ -205,9 +207,12 @@ struct spg_vbd_ctx {
         a.triO = triO.p;
         a.triG = triG.p;
         a.triK = triK.p;
-        a.hinPtr = hinPtr.p;
-        a.hinV = hinV.p;
-        a.hinP = hinP.p;
+        a.hinEPtr = hinEPtr.p;
+        a.hinEV = hinEV.p;
+        a.hinEP = hinEP.p;
+        a.hinWPtr = hinWPtr.p;
+        a.hinWV = hinWV.p;
+        a.hinWP = hinWP.p;
         a.tetPtr = tetPtr.p;
         a.tetO = tetO.p;
         a.tetGa = tetGa.p;
@@ -310,8 +315,10 @@ struct spg_vbd_ctx {
         a.triElem = triElem.p;
         a.triV = triV.p;
         a.triP = triP.p;
-        a.hinPtr = hinPtr.p;
-        a.hinElem = hinElem.p;
+        a.hinEPtr = hinEPtr.p;
+        a.hinEElem = hinEElem.p;
+        a.hinWPtr = hinWPtr.p;
+        a.hinWElem = hinWElem.p;
         a.hinV = hinVe.p;
         a.hinP = hinPe.p;
         a.tetPtr = tetPtr.p;
@@ -642,28 +649,35 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.triV.upload(triV, c.stream);
         c.triP.upload(triP, c.stream);
     }
-    // discrete bending: canonical slots, the solve finds its own position in the hinge
+    // discrete bending: two incidence classes by the owner's slot (hinge edge: 0, 1; wing: 2, 3), so
+    // that the lanes of a wavefront run one closed form at a time. Records keep the canonical slots;
+    // the solve finds its own position in the hinge by comparing slot ids.
     if (!inc[SPG_VBD_DISCRETE_BENDING].empty()) {
         checkSize(inc[SPG_VBD_DISCRETE_BENDING].size());
-        std::vector<int32_t> ptr;
-        const auto order = sortBySlot(inc[SPG_VBD_DISCRETE_BENDING], c.nv, ptr);
-        std::vector<int4> rv(order.size());
-        std::vector<float2> rp(order.size());
-        std::vector<int32_t> el(order.size());
-        for (size_t k = 0; k < order.size(); ++k) {
-            const auto &in = inc[SPG_VBD_DISCRETE_BENDING][order[k]];
-            rv[k] = hinV[in.elem];
-            rp[k] = make_float2(float(hinP[in.elem].x), float(hinP[in.elem].y));
-            el[k] = in.elem;
-        }
         // a hinge with a repeated vertex cannot tell its slot from the vertex id
         for (const auto &v : hinV)
             if (v.x == v.y || v.x == v.z || v.x == v.w || v.y == v.z || v.y == v.w || v.z == v.w)
                 throw VbdError(SPG_VBD_ERR_INVALID, "bending stencil with a repeated vertex");
-        c.hinPtr.upload(ptr, c.stream);
-        c.hinV.upload(rv, c.stream);
-        c.hinP.upload(rp, c.stream);
-        c.hinElem.upload(el, c.stream);
+        for (int cls = 0; cls < 2; ++cls) {
+            std::vector<Incidence> sub;
+            for (const auto &in : inc[SPG_VBD_DISCRETE_BENDING])
+                if ((in.local >= 2) == (cls == 1)) sub.push_back(in);
+            std::vector<int32_t> ptr;
+            const auto order = sortBySlot(sub, c.nv, ptr);
+            std::vector<int4> rv(order.size());
+            std::vector<float2> rp(order.size());
+            std::vector<int32_t> el(order.size());
+            for (size_t k = 0; k < order.size(); ++k) {
+                const auto &in = sub[order[k]];
+                rv[k] = hinV[in.elem];
+                rp[k] = make_float2(float(hinP[in.elem].x), float(hinP[in.elem].y));
+                el[k] = in.elem;
+            }
+            (cls == 0 ? c.hinEPtr : c.hinWPtr).upload(ptr, c.stream);
+            (cls == 0 ? c.hinEV : c.hinWV).upload(rv, c.stream);
+            (cls == 0 ? c.hinEP : c.hinWP).upload(rp, c.stream);
+            (cls == 0 ? c.hinEElem : c.hinWElem).upload(el, c.stream);
+        }
         c.hinVe.upload(hinV, c.stream);
         c.hinPe.upload(hinP, c.stream);
     }
@@ -899,6 +913,33 @@ int spg_vbd_end_substep(spg_vbd_ctx *ctx)
     });
 }
 
+int spg_vbd_save_state(spg_vbd_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "save_state inside an open substep");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        if (!ctx->savedPos.p) {
+            ctx->savedPos.alloc(ctx->nv);
+            ctx->savedVel.alloc(ctx->nv);
+        }
+        hipCheck(launchCopy(ctx->pos.p, ctx->savedPos.p, ctx->nv, ctx->stream), "save");
+        hipCheck(launchCopy(ctx->vel.p, ctx->savedVel.p, ctx->nv, ctx->stream), "save");
+    });
+}
+
+int spg_vbd_restore_state(spg_vbd_ctx *ctx)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "restore_state inside an open substep");
+        if (!ctx->savedPos.p) throw VbdError(SPG_VBD_ERR_STATE, "no saved state");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        hipCheck(launchCopy(ctx->savedPos.p, ctx->pos.p, ctx->nv, ctx->stream), "restore");
+        hipCheck(launchCopy(ctx->savedVel.p, ctx->vel.p, ctx->nv, ctx->stream), "restore");
+    });
+}
+
 int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms)
 {
     return guarded(ctx, [&] {
@@ -929,8 +970,8 @@ int spg_vbd_set_launch(spg_vbd_ctx *ctx, const spg_vbd_launch *cfg)
     return guarded(ctx, [&] {
         if (!ctx || !cfg) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         const int g = cfg->lanes_per_vertex;
-        if (g != 1 && g != 2 && g != 4 && g != 8 && g != 16)
-            throw VbdError(SPG_VBD_ERR_INVALID, "lanes_per_vertex must be 1, 2, 4, 8 or 16");
+        if (g != 1 && g != 2 && g != 3 && g != 4 && g != 6 && g != 8 && g != 16)
+            throw VbdError(SPG_VBD_ERR_INVALID, "lanes_per_vertex must be 1, 2, 3, 4, 6, 8 or 16");
         if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
             throw VbdError(SPG_VBD_ERR_INVALID, "block_size must be 64, 128 or 256");
         ctx->launch = *cfg;
@@ -959,7 +1000,7 @@ int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best)
         hipCheck(hipEventCreate(&e1), "hipEventCreate");
         float bestMs = 1e30f;
         spg_vbd_launch bestCfg = keep;
-        const int lanes[] = {1, 2, 4, 8, 16};
+        const int lanes[] = {1, 2, 3, 4, 6, 8, 16};
         const int blocks[] = {64, 128, 256};
         for (int l : lanes)
             for (int b : blocks) {
@@ -1048,15 +1089,57 @@ int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *co
     });
 }
 
+int spg_vbd_bench_copy(int64_t bytes, int32_t repeats, int32_t device, double *ms)
+{
+    return guarded(nullptr, [&] {
+        if (bytes <= 0 || bytes % 16 || repeats < 1 || !ms) throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipStream_t s;
+        hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        const int64_t n = bytes / 16;
+        DevBuf<float4> a, b;
+        a.alloc(n);
+        b.alloc(n);
+        hipCheck(launchCopy(b.p, a.p, n, s), "copy launch");  // touch both buffers once
+        hipEvent_t e0, e1;
+        hipCheck(hipEventCreate(&e0), "hipEventCreate");
+        hipCheck(hipEventCreate(&e1), "hipEventCreate");
+        hipCheck(hipEventRecord(e0, s), "hipEventRecord");
+        for (int32_t r = 0; r < repeats; ++r) hipCheck(launchCopy(a.p, b.p, n, s), "copy launch");
+        hipCheck(hipEventRecord(e1, s), "hipEventRecord");
+        hipCheck(hipEventSynchronize(e1), "hipEventSynchronize");
+        float t = 0.f;
+        hipCheck(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        *ms = t / repeats;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+        (void)hipStreamDestroy(s);
+    });
+}
+
 int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params, int32_t precision,
-                                double *grad, double *hess_blocks, int32_t device)
+                                int32_t form, int32_t device, double *grad, double *hess_blocks)
 {
     return guarded(nullptr, [&] {
         const int ss = stencilSizeOf(type), np = paramsOf(type);
         if (!ss) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "unsupported energy type");
         if (n < 0 || (n > 0 && (!xs || !params || !grad || !hess_blocks)))
             throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        if (form < 0 || form > 2) throw VbdError(SPG_VBD_ERR_INVALID, "form must be 0, 1 or 2");
         if (n == 0) return;
+        if (device == -1) {
+            // host instantiation of the same closed forms (derivative checks only)
+            for (int64_t i = 0; i < n; ++i) {
+                if (precision == 0)
+                    elementAll<float>(type, xs + i * ss * 3, params + i * np, form, grad + i * ss * 3,
+                                      hess_blocks + i * ss * 9);
+                else
+                    elementAll<double>(type, xs + i * ss * 3, params + i * np, form, grad + i * ss * 3,
+                                       hess_blocks + i * ss * 9);
+            }
+            return;
+        }
         hipCheck(hipSetDevice(device), "hipSetDevice");
         hipStream_t s;
         hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
@@ -1065,7 +1148,7 @@ int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const
         dp.upload(std::vector<double>(params, params + n * np), s);
         dg.alloc(n * ss * 3);
         dh.alloc(n * ss * 9);
-        hipCheck(launchElementDerivatives(type, n, dx.p, dp.p, precision, dg.p, dh.p, s), "element launch");
+        hipCheck(launchElementDerivatives(type, n, dx.p, dp.p, precision, form, dg.p, dh.p, s), "element launch");
         hipCheck(hipMemcpyAsync(grad, dg.p, sizeof(double) * n * ss * 3, hipMemcpyDeviceToHost, s), "download");
         hipCheck(hipMemcpyAsync(hess_blocks, dh.p, sizeof(double) * n * ss * 9, hipMemcpyDeviceToHost, s), "download");
         hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");

@@ -37,10 +37,14 @@ struct SolveArgs {
     const int2 *triO;
     const float4 *triG;
     const float4 *triK;
-    // bending: canonical slots v0..v3, (theta0, keff)
-    const int32_t *hinPtr;
-    const int4 *hinV;
-    const float2 *hinP;
+    // bending, split by the owner's slot: hinge-edge (slots 0, 1) and hinge-wing (slots 2, 3)
+    // incidences; canonical slots v0..v3, (theta0, keff)
+    const int32_t *hinEPtr;
+    const int4 *hinEV;
+    const float2 *hinEP;
+    const int32_t *hinWPtr;
+    const int4 *hinWV;
+    const float2 *hinWP;
     // SNH: other slots (3), (g_a, lambda), (g_b, mu), (g_c, alpha)
     const int32_t *tetPtr;
     const int4 *tetO;
@@ -70,8 +74,10 @@ struct ResidualArgs {
     const int32_t *triElem;
     const int4 *triV;
     const double *triP;   // 8 per element: Dinv (4), kd, ko, ks, 0
-    const int32_t *hinPtr;
-    const int32_t *hinElem;
+    const int32_t *hinEPtr;
+    const int32_t *hinEElem;
+    const int32_t *hinWPtr;
+    const int32_t *hinWElem;
     const int4 *hinV;
     const double2 *hinP;  // (theta0, keff)
     const int32_t *tetPtr;
@@ -92,7 +98,8 @@ hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream);
 hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *objOffsets, int32_t nObjects,
                                double *out, hipStream_t stream);
 hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t stream);
+hipError_t launchCopy(const float4 *src, float4 *dst, int64_t n, hipStream_t stream);
 hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,
-                                    double *grad, double *hess, hipStream_t stream);
+                                    int form, double *grad, double *hess, hipStream_t stream);
 
 }  // namespace spgvbd

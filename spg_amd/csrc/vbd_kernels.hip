@@ -1,6 +1,7 @@
 // CDNA4 kernels of the VBD step: inertial predict (K1), per-colour vertex block solve (K2),
 // velocity update (K3), FP64 residual (K4) and the per-element derivative hook.
 #include "element.h"
+#include "element_eval.h"
 #include "vbd_device.h"
 
 namespace spgvbd {
@@ -23,14 +24,6 @@ __device__ __forceinline__ void acc(V3f &g, S3f &H, const V3f &ge, const S3f &He
     H.yy += He.yy;
     H.yz += He.yz;
     H.zz += He.zz;
-}
-
-template <int G>
-__device__ __forceinline__ float groupSum(float v)
-{
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, G);
-    return v;
 }
 
 // K1 (vbd.cpp:48-83, particleGroup.cpp:57-63): x_old = x; v += h g; x += h v; x_tilde = x.
@@ -112,16 +105,41 @@ __device__ __forceinline__ bool blockSolve(const S3f &H, const V3f &f, V3f &dx, 
     return false;
 }
 
-// K2: one colour sweep. G lanes of a wavefront cooperate on one vertex: lane l of the group takes
-// incidences l, l+G, ... of each energy type, then the 3-vector force and the 6 Hessian entries
-// are summed over the group with cross-lane shuffles and lane 0 solves and writes.
+// K2: one colour sweep. A group of G lanes of one wavefront cooperates on one vertex; each
+// wavefront holds floor(64 / G) groups (the remaining lanes idle when G does not divide 64). Lane l
+// of a group takes incidences l, l+G, ... of each incidence class (StVK triangles, hinge-edge and
+// hinge-wing slots of bending elements, SNH tetrahedra, springs, anchors), then the force and the
+// six Hessian entries are summed over the group in a fixed order with cross-lane shuffles and the
+// group's first lane solves the 3x3 block and writes the position.
+// Blocks are renumbered so that each XCD (blocks are dealt round-robin over the 8 XCDs) works on
+// one contiguous range of the colour, keeping neighbouring positions in that XCD's L2.
+__device__ __forceinline__ int32_t xcdContiguousBlock(int32_t b, int32_t nb)
+{
+    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
+
+template <int G>
+__device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t lig)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        const float o = __shfl(v, lane64 + off, 64);
+        if (lig + off < G) v += o;
+    }
+    return v;
+}
+
 template <int G>
 __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
 {
-    const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int32_t vid = a.begin + t / G;
-    const int32_t lane = t & (G - 1);
-    const bool active = vid < a.end;
+    constexpr int GPW = 64 / G;  // groups per wavefront
+    const int32_t lane64 = threadIdx.x & 63;
+    const int32_t wave = xcdContiguousBlock(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int32_t grp = lane64 / G;
+    const int32_t lig = lane64 - grp * G;
+    const int32_t vid = a.begin + wave * GPW + grp;
+    const bool active = grp < GPW && vid < a.end;
     V3f g = {0.f, 0.f, 0.f};
     S3f H = zeroSym<float>();
     V3f xo = {0.f, 0.f, 0.f};
@@ -129,7 +147,7 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         xo = xyz(a.pos[vid]);
         if (a.triPtr) {
             const int32_t e = a.triPtr[vid + 1];
-            for (int32_t k = a.triPtr[vid] + lane; k < e; k += G) {
+            for (int32_t k = a.triPtr[vid] + lig; k < e; k += G) {
                 const int2 o = a.triO[k];
                 const float4 gg = a.triG[k];
                 const float4 kk = a.triK[k];
@@ -140,22 +158,33 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
                 acc(g, H, ge, He);
             }
         }
-        if (a.hinPtr) {
-            const int32_t e = a.hinPtr[vid + 1];
-            for (int32_t k = a.hinPtr[vid] + lane; k < e; k += G) {
-                const int4 v = a.hinV[k];
-                const float2 p = a.hinP[k];
-                const int slot = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
+        if (a.hinEPtr) {
+            const int32_t e = a.hinEPtr[vid + 1];
+            for (int32_t k = a.hinEPtr[vid] + lig; k < e; k += G) {
+                const int4 v = a.hinEV[k];
+                const float2 p = a.hinEP[k];
                 V3f ge;
                 S3f He;
-                bendingVertex<float>(xyz(a.pos[v.x]), xyz(a.pos[v.y]), xyz(a.pos[v.z]), xyz(a.pos[v.w]), slot, p.x,
-                                     p.y, ge, He);
+                bendingEdge<float>(xyz(a.pos[v.x]), xyz(a.pos[v.y]), xyz(a.pos[v.z]), xyz(a.pos[v.w]), v.y == vid, p.x,
+                                   p.y, ge, He);
+                acc(g, H, ge, He);
+            }
+        }
+        if (a.hinWPtr) {
+            const int32_t e = a.hinWPtr[vid + 1];
+            for (int32_t k = a.hinWPtr[vid] + lig; k < e; k += G) {
+                const int4 v = a.hinWV[k];
+                const float2 p = a.hinWP[k];
+                V3f ge;
+                S3f He;
+                bendingWing<float>(xyz(a.pos[v.x]), xyz(a.pos[v.y]), xyz(a.pos[v.z]), xyz(a.pos[v.w]), v.w == vid, p.x,
+                                   p.y, ge, He);
                 acc(g, H, ge, He);
             }
         }
         if (a.tetPtr) {
             const int32_t e = a.tetPtr[vid + 1];
-            for (int32_t k = a.tetPtr[vid] + lane; k < e; k += G) {
+            for (int32_t k = a.tetPtr[vid] + lig; k < e; k += G) {
                 const int4 o = a.tetO[k];
                 const float4 ga = a.tetGa[k], gb = a.tetGb[k], gc = a.tetGc[k];
                 V3f ge;
@@ -167,7 +196,7 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         }
         if (a.sprPtr) {
             const int32_t e = a.sprPtr[vid + 1];
-            for (int32_t k = a.sprPtr[vid] + lane; k < e; k += G) {
+            for (int32_t k = a.sprPtr[vid] + lig; k < e; k += G) {
                 const float4 s = a.spr[k];
                 V3f ge;
                 S3f He;
@@ -177,7 +206,7 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         }
         if (a.ancPtr) {
             const int32_t e = a.ancPtr[vid + 1];
-            for (int32_t k = a.ancPtr[vid] + lane; k < e; k += G) {
+            for (int32_t k = a.ancPtr[vid] + lig; k < e; k += G) {
                 const float4 s = a.anc[k];
                 V3f ge;
                 S3f He;
@@ -187,17 +216,17 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         }
     }
     if (G > 1) {
-        g.x = groupSum<G>(g.x);
-        g.y = groupSum<G>(g.y);
-        g.z = groupSum<G>(g.z);
-        H.xx = groupSum<G>(H.xx);
-        H.xy = groupSum<G>(H.xy);
-        H.xz = groupSum<G>(H.xz);
-        H.yy = groupSum<G>(H.yy);
-        H.yz = groupSum<G>(H.yz);
-        H.zz = groupSum<G>(H.zz);
+        g.x = groupReduce<G>(g.x, lane64, lig);
+        g.y = groupReduce<G>(g.y, lane64, lig);
+        g.z = groupReduce<G>(g.z, lane64, lig);
+        H.xx = groupReduce<G>(H.xx, lane64, lig);
+        H.xy = groupReduce<G>(H.xy, lane64, lig);
+        H.xz = groupReduce<G>(H.xz, lane64, lig);
+        H.yy = groupReduce<G>(H.yy, lane64, lig);
+        H.yz = groupReduce<G>(H.yz, lane64, lig);
+        H.zz = groupReduce<G>(H.zz, lane64, lig);
     }
-    if (active && lane == 0) {
+    if (active && lig == 0) {
         const float4 xt = a.xt[vid];
         const float mdt2 = xt.w * a.invdt2;
         // f = -(m/h^2)(x - x_tilde) - sum grad; H = (m/h^2) I + sum H_e (vbd.cpp:102-138)
@@ -243,14 +272,17 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
             r = r - ge;
         }
     }
-    if (a.hinPtr) {
-        for (int32_t k = a.hinPtr[vid]; k < a.hinPtr[vid + 1]; ++k) {
-            const int32_t el = a.hinElem[k];
+    for (int cls = 0; cls < 2; ++cls) {  // hinge-edge and hinge-wing incidences
+        const int32_t *ptr = cls == 0 ? a.hinEPtr : a.hinWPtr;
+        const int32_t *elem = cls == 0 ? a.hinEElem : a.hinWElem;
+        if (!ptr) continue;
+        for (int32_t k = ptr[vid]; k < ptr[vid + 1]; ++k) {
+            const int32_t el = elem[k];
             const int4 v = a.hinV[el];
             const double2 p = a.hinP[el];
             const int slot = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
-            bendingVertex<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), slot, p.x,
-                                  p.y, ge, He);
+            bendingVertexCompact<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), slot,
+                                         p.x, p.y, ge, He);
             r = r - ge;
         }
     }
@@ -318,72 +350,32 @@ __global__ void __launch_bounds__(256) reduceObjectsKernel(const double *__restr
 // vertex of each element, from the prepared FP64 parameters (params layout of spg_vbd.h).
 template <class R>
 __global__ void __launch_bounds__(64) elementKernel(int type, int64_t n, const double *xs, const double *params,
-                                                    double *grad, double *hess)
+                                                    int form, double *grad, double *hess)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
-    int ns = 0, np = 0;
-    switch (type) {
-    case kSpring: ns = 2; np = 2; break;
-    case kStvk: ns = 3; np = 13; break;
-    case kBending: ns = 4; np = 2; break;
-    case kSnh: ns = 4; np = 12; break;
-    case kAnchor: ns = 1; np = 4; break;
-    default: return;
-    }
-    const double *xe = xs + i * ns * 3;
-    const double *p = params + i * np;
-    V3<R> x[4];
-    for (int s = 0; s < ns; ++s) x[s] = {R(xe[3 * s]), R(xe[3 * s + 1]), R(xe[3 * s + 2])};
-    for (int s = 0; s < ns; ++s) {
-        V3<R> ge = {R(0), R(0), R(0)};
-        Sym3<R> He = zeroSym<R>();
-        if (type == kSpring) {
-            springVertex<R>(x[s], x[1 - s], R(p[0]), R(p[1]), ge, He);
-        } else if (type == kAnchor) {
-            anchorVertex<R>(x[0], V3<R>{R(p[0]), R(p[1]), R(p[2])}, R(p[3]), ge, He);
-        } else if (type == kStvk) {
-            const double g[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
-            const int ia = s == 0 ? 1 : 0, ib = s == 2 ? 1 : 2;
-            stvkVertex<R>(x[s], x[ia], x[ib], R(g[ia][0]), R(g[ia][1]), R(g[ib][0]), R(g[ib][1]), R(p[4]), R(p[5]),
-                          R(p[12]), ge, He);
-        } else if (type == kBending) {
-            bendingVertex<R>(x[0], x[1], x[2], x[3], s, R(p[0]), R(p[1]), ge, He);
-        } else {
-            double gr[4][3];
-            for (int c = 0; c < 3; ++c) {
-                gr[1][c] = p[c];
-                gr[2][c] = p[3 + c];
-                gr[3][c] = p[6 + c];
-                gr[0][c] = -(p[c] + p[3 + c] + p[6 + c]);
-            }
-            int o[3], m = 0;
-            for (int q = 0; q < 4; ++q)
-                if (q != s) o[m++] = q;
-            auto G = [&](int q) { return V3<R>{R(gr[q][0]), R(gr[q][1]), R(gr[q][2])}; };
-            snhVertex<R>(x[s], x[o[0]], x[o[1]], x[o[2]], G(o[0]), G(o[1]), G(o[2]), R(p[9]), R(p[10]), R(p[11]), ge,
-                         He);
-        }
-        double *gd = grad + (i * ns + s) * 3;
-        gd[0] = ge.x;
-        gd[1] = ge.y;
-        gd[2] = ge.z;
-        double *hd = hess + (i * ns + s) * 9;
-        hd[0] = He.xx;
-        hd[1] = He.xy;
-        hd[2] = He.xz;
-        hd[3] = He.xy;
-        hd[4] = He.yy;
-        hd[5] = He.yz;
-        hd[6] = He.xz;
-        hd[7] = He.yz;
-        hd[8] = He.zz;
-    }
+    const int ns = elemStencilSize(type), np = elemParams(type);
+    if (!ns) return;
+    elementAll<R>(type, xs + i * ns * 3, params + i * np, form, grad + i * ns * 3, hess + i * ns * 9);
+}
+
+// Calibration copy: one float4 read and one float4 write per thread.
+__global__ void __launch_bounds__(256) copyKernel(const float4 *__restrict__ src, float4 *__restrict__ dst, int64_t n)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 inline int blocksFor(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 }  // namespace
+
+hipError_t launchCopy(const float4 *src, float4 *dst, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    copyKernel<<<blocksFor(n, 256), 256, 0, stream>>>(src, dst, n);
+    return hipGetLastError();
+}
 
 hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt, const float *mass, int32_t n,
                               float h, float gx, float gy, float gz, hipStream_t stream)
@@ -414,11 +406,16 @@ hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSiz
     const int64_t nv = a.end - a.begin;
     if (nv <= 0) return hipSuccess;
     if (blockSize > 256 || blockSize < 64 || blockSize % 64) return hipErrorInvalidValue;
-    const int grid = blocksFor(nv * lanesPerVertex, blockSize);
+    if (lanesPerVertex < 1 || lanesPerVertex > 16) return hipErrorInvalidValue;
+    // wavefronts of 64 / G vertex groups, blockSize / 64 wavefronts per block
+    const int64_t waves = (nv + 64 / lanesPerVertex - 1) / (64 / lanesPerVertex);
+    const int grid = blocksFor(waves, blockSize / 64);
     switch (lanesPerVertex) {
     case 1: solveColorKernel<1><<<grid, blockSize, 0, stream>>>(a); break;
     case 2: solveColorKernel<2><<<grid, blockSize, 0, stream>>>(a); break;
+    case 3: solveColorKernel<3><<<grid, blockSize, 0, stream>>>(a); break;
     case 4: solveColorKernel<4><<<grid, blockSize, 0, stream>>>(a); break;
+    case 6: solveColorKernel<6><<<grid, blockSize, 0, stream>>>(a); break;
     case 8: solveColorKernel<8><<<grid, blockSize, 0, stream>>>(a); break;
     case 16: solveColorKernel<16><<<grid, blockSize, 0, stream>>>(a); break;
     default: return hipErrorInvalidValue;
@@ -442,13 +439,13 @@ hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const 
 }
 
 hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,
-                                    double *grad, double *hess, hipStream_t stream)
+                                    int form, double *grad, double *hess, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
     if (precision == 0)
-        elementKernel<float><<<blocksFor(n, 64), 64, 0, stream>>>(type, n, xs, params, grad, hess);
+        elementKernel<float><<<blocksFor(n, 64), 64, 0, stream>>>(type, n, xs, params, form, grad, hess);
     else
-        elementKernel<double><<<blocksFor(n, 64), 64, 0, stream>>>(type, n, xs, params, grad, hess);
+        elementKernel<double><<<blocksFor(n, 64), 64, 0, stream>>>(type, n, xs, params, form, grad, hess);
     return hipGetLastError();
 }
 

@@ -92,6 +92,8 @@ def vbd_lib():
         _proto(L, "spg_vbd_set_guess", ctypes.c_int, [c, i32, _dp])
         _proto(L, "spg_vbd_iterate", ctypes.c_int, [c, i32])
         _proto(L, "spg_vbd_end_substep", ctypes.c_int, [c])
+        _proto(L, "spg_vbd_save_state", ctypes.c_int, [c])
+        _proto(L, "spg_vbd_restore_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_residual", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_residual_at_inertial", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_get_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
@@ -103,7 +105,8 @@ def vbd_lib():
         _proto(L, "spg_vbd_synchronize", ctypes.c_int, [c])
         _proto(L, "spg_vbd_color_vertices", ctypes.c_int,
                [i64, i32, ctypes.POINTER(_ip), _lp, _ip, _ip])
-        _proto(L, "spg_vbd_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, i32, _dp, _dp, i32])
+        _proto(L, "spg_vbd_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, i32, i32, i32, _dp, _dp])
+        _proto(L, "spg_vbd_bench_copy", ctypes.c_int, [i64, i32, i32, _dp])
         _vbd = L
     return _vbd
 
@@ -400,6 +403,12 @@ class VbdContext:
     def end_substep(self):
         self._check(self._L.spg_vbd_end_substep(self.h))
 
+    def save_state(self):
+        self._check(self._L.spg_vbd_save_state(self.h))
+
+    def restore_state(self):
+        self._check(self._L.spg_vbd_restore_state(self.h))
+
     def residual(self):
         out = np.zeros(self.n_objects)
         self._check(self._L.spg_vbd_residual(self.h, _d(out)))
@@ -456,8 +465,19 @@ def color_vertices(n_vertices, sets):
     return out
 
 
-def element_derivatives(kind, xs, params, precision=0, device=0):
-    """GPU evaluation of per-vertex gradients (n, N, 3) and diagonal Hessian blocks (n, N, 3, 3)."""
+def bench_copy(nbytes, repeats=10, device=0):
+    """Mean time (ms) of a plain float4 copy of nbytes (calibration kernel for traffic counters)."""
+    L = vbd_lib()
+    ms = ctypes.c_double(0.0)
+    rc = L.spg_vbd_bench_copy(int(nbytes), int(repeats), device, ctypes.byref(ms))
+    if rc != 0:
+        raise NativeError(rc, L.spg_vbd_last_error(None).decode())
+    return ms.value
+
+
+def element_derivatives(kind, xs, params, precision=0, device=0, form=0):
+    """Per-vertex gradients (n, N, 3) and diagonal Hessian blocks (n, N, 3, 3) from the solver's closed
+    forms, on GPU `device` or, with device=-1, their host instantiation (derivative checks)."""
     L = vbd_lib()
     t = _kind(kind)
     ns = STENCIL_SIZE[t]
@@ -466,7 +486,7 @@ def element_derivatives(kind, xs, params, precision=0, device=0):
     p = _f64(params, (n, PARAMS[t]))
     g = np.zeros((n, ns, 3))
     h = np.zeros((n, ns, 3, 3))
-    rc = L.spg_vbd_element_derivatives(t, n, _d(xs), _d(p), precision, _d(g), _d(h), device)
+    rc = L.spg_vbd_element_derivatives(t, n, _d(xs), _d(p), precision, form, device, _d(g), _d(h))
     if rc != 0:
         raise NativeError(rc, L.spg_vbd_last_error(None).decode())
     return g, h

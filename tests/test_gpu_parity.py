@@ -237,6 +237,33 @@ def test_graph_replay_bitwise():
     assert np.array_equal(a.state()[1], b.state()[1])
 
 
+def test_checkpoint_restore_bitwise():
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s, use_graph=1)
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    ctx.save_state()
+    saved = ctx.state()
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    first = ctx.state()
+    for _ in range(3):
+        ctx.step(s.dt, 1, s.iterations, s.gravity)
+    ctx.restore_state()
+    assert all(np.array_equal(a, b) for a, b in zip(ctx.state(), saved))
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    assert all(np.array_equal(a, b) for a, b in zip(ctx.state(), first))
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 3, 4, 6, 8, 16])
+def test_lane_group_shapes_agree(lanes, oracle_threads):
+    """Every lanes-per-vertex shape the autotuner may pick gives the oracle's step."""
+    s = scenes.cloth(40, young=1e4, pin="top_row", iterations=10)
+    ctx = gpu_context(s, lanes_per_vertex=lanes, block_size=128)
+    o = oracle_from_gpu_start(s, ctx)
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    assert rel_err(ctx.state()[0], o.state()[0]) < 1e-5
+
+
 def test_vbdgpu_mirror_matches_capi():
     """spg::solver::VBDGpu (C++ mirror of the BaseSolver API) gives the same result as the C-ABI."""
     s = scenes.cloth_config1()

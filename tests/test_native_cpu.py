@@ -117,3 +117,34 @@ def test_double_prepare_throws():
 def test_coloring_bad_index_message():
     with pytest.raises(native.NativeError, match="invalid node indices"):
         native.color_vertices(3, [(np.array([0, 5], dtype=np.int32), 2)])
+
+
+def _golden(kind):
+    import json
+    G = json.load(open(os.path.join(ROOT, "tests", "golden", "element_golden.json")))["cases"]
+    cs = [c for c in G if c["kind"] == kind]
+    ns = native.STENCIL_SIZE[native.KIND[kind]]
+    xs = np.array([c["x"] for c in cs])
+    prm = np.array([[float(v) for v in c["prepared"]] for c in cs])
+    g = np.array([[float(v) for v in c["grad"]] for c in cs]).reshape(len(cs), ns, 3)
+    H = np.array([[[float(v) for v in r] for r in c["hess"]] for c in cs])
+    hb = np.array([[H[k, 3 * s:3 * s + 3, 3 * s:3 * s + 3] for s in range(ns)] for k in range(len(cs))])
+    return xs, prm, g, hb
+
+
+@pytest.mark.parametrize("kind", ["spring", "anchor", "stvk", "bending", "snh"])
+@pytest.mark.parametrize("form", [0, 1, 2], ids=["solve-form", "general-form", "unified-form"])
+def test_device_closed_forms_on_host(kind, form):
+    """The solver's closed forms (element.h), instantiated on the host, against the 40-digit golden
+    derivatives: FP64 within 1e-12 relative, FP32 within 2e-5 relative (per element, max-norm)."""
+    xs, prm, g_ref, h_ref = _golden(kind)
+    for prec, tol in ((1, 1e-12), (0, 2e-5)):
+        g, h = native.element_derivatives(kind, xs, prm, precision=prec, device=-1, form=form)
+        for k in range(len(xs)):
+            sh = np.max(np.abs(h_ref[k]))
+            sg = np.max(np.abs(g_ref[k]))
+            assert np.max(np.abs(h[k] - h_ref[k])) <= tol * sh, (kind, k, prec)
+            if sg > 1e-9 * sh:
+                assert np.max(np.abs(g[k] - g_ref[k])) <= tol * sg, (kind, k, prec)
+            else:
+                assert np.max(np.abs(g[k])) <= 1e-6 * sh

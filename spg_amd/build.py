@@ -21,8 +21,12 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 # Packed FP32 instructions stay off: they are formed by the compiler from plain float code and
 # some of their forms are unreliable on this hardware.
+# FP32 division and square root use the hardware reciprocal / square root (about 1 ulp) instead of
+# the correctly rounded sequences, and FP32 denormals flush to zero: the solver's FP32 magnitudes
+# stay above 1e-30 and the FP64 paths are unaffected.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
-             "-packed-fp32-ops", "-Wall", "-Wno-unused-result"]
+             "-packed-fp32-ops", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
+             "-Wall", "-Wno-unused-result"]
 
 VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "coloring.cpp"]
 HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "vbdGpu.cpp", "host_capi.cpp"]

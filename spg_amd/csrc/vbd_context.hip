@@ -136,10 +136,13 @@ struct spg_vbd_ctx {
     DevBuf<int64_t> objOffsets;
 
     // incidence streams
+    // row offsets (CSR, used by the residual; anchors also by the solve)
     DevBuf<int32_t> sprPtr, ancPtr, triPtr, hinEPtr, hinWPtr, tetPtr;
+    // fixed-width record rows of the solve (anchors: CSR records)
+    int32_t sprW = 0, triW = 0, hinEW = 0, hinWW = 0, tetW = 0;
     DevBuf<float4> spr, anc;
     DevBuf<int2> triO;
-    DevBuf<float4> triG, triK;
+    DevBuf<float4> triG, triM;
     DevBuf<int4> hinEV, hinWV;
     DevBuf<float2> hinEP, hinWP;
     DevBuf<int4> tetO;
@@ -199,21 +202,21 @@ struct spg_vbd_ctx {
         SolveArgs a{};
         a.pos = pos.p;
         a.xt = xt.p;
-        a.sprPtr = sprPtr.p;
+        a.sprW = sprW;
         a.spr = spr.p;
         a.ancPtr = ancPtr.p;
         a.anc = anc.p;
-        a.triPtr = triPtr.p;
+        a.triW = triW;
         a.triO = triO.p;
         a.triG = triG.p;
-        a.triK = triK.p;
-        a.hinEPtr = hinEPtr.p;
+        a.triM = triM.p;
+        a.hinEW = hinEW;
         a.hinEV = hinEV.p;
         a.hinEP = hinEP.p;
-        a.hinWPtr = hinWPtr.p;
+        a.hinWW = hinWW;
         a.hinWV = hinWV.p;
         a.hinWP = hinWP.p;
-        a.tetPtr = tetPtr.p;
+        a.tetW = tetW;
         a.tetO = tetO.p;
         a.tetGa = tetGa.p;
         a.tetGb = tetGb.p;
@@ -263,14 +266,11 @@ struct spg_vbd_ctx {
             for (int32_t c = 0; c < maxColors; ++c) {
                 const SolveArgs a = solveArgs(c, invdt2);
                 std::pair<hipEvent_t, hipEvent_t> ev{};
-                if (timed) {
-                    ev = takeEvents();
-                    hipCheck(hipEventRecord(ev.first, stream), "hipEventRecord");
-                }
-                hipCheck(launchSolveColor(a, launch.lanes_per_vertex, launch.block_size, launch.lds_staging, stream),
+                if (timed) ev = takeEvents();
+                hipCheck(launchSolveColor(a, launch.lanes_per_vertex, launch.block_size, launch.lds_staging, stream,
+                                          ev.first, ev.second),
                          "vertex solve launch");
                 if (timed) {
-                    hipCheck(hipEventRecord(ev.second, stream), "hipEventRecord");
                     events.push_back(ev);
                     eventColors.push_back(c);
                     if (events.size() >= 8192) drainEvents();
@@ -408,6 +408,25 @@ std::vector<int32_t> sortBySlot(const std::vector<Incidence> &inc, int64_t nv, s
     std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1), order(inc.size());
     for (int32_t k = 0; k < static_cast<int32_t>(inc.size()); ++k) order[fill[inc[k].slot]++] = k;
     return order;
+}
+
+int32_t rowWidth(const std::vector<int32_t> &ptr)
+{
+    int32_t w = 0;
+    for (size_t v = 0; v + 1 < ptr.size(); ++v) w = std::max(w, ptr[v + 1] - ptr[v]);
+    return w;
+}
+
+// Fixed-width rows of the solve: record j of vertex slot v at v * width + j, padded with `pad`;
+// make(k) builds the record of the k-th incidence in slot order.
+template <class Rec, class Make>
+std::vector<Rec> toRows(const std::vector<int32_t> &ptr, int32_t width, Rec pad, Make make)
+{
+    const size_t nv = ptr.size() - 1;
+    std::vector<Rec> rows(nv * static_cast<size_t>(width), pad);
+    for (size_t v = 0; v < nv; ++v)
+        for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k) rows[v * width + (k - ptr[v])] = make(k);
+    return rows;
 }
 
 void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
@@ -588,17 +607,19 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         checkSize(inc[SPG_VBD_SPRING].size());
         std::vector<int32_t> ptr;
         const auto order = sortBySlot(inc[SPG_VBD_SPRING], c.nv, ptr);
-        std::vector<float4> rec(order.size());
         std::vector<int32_t> el(order.size());
-        for (size_t k = 0; k < order.size(); ++k) {
-            const auto &in = inc[SPG_VBD_SPRING][order[k]];
-            const int2 v = sprV[in.elem];
-            const int32_t other = in.local == 0 ? v.y : v.x;
-            rec[k] = make_float4(intBitsAsFloat(other), float(sprP[in.elem].x), float(sprP[in.elem].y), 0.f);
-            el[k] = in.elem;
-        }
+        for (size_t k = 0; k < order.size(); ++k) el[k] = inc[SPG_VBD_SPRING][order[k]].elem;
+        c.sprW = rowWidth(ptr);
+        c.spr.upload(toRows(ptr, c.sprW, make_float4(intBitsAsFloat(-1), 0.f, 0.f, 0.f),
+                            [&](int32_t k) {
+                                const auto &in = inc[SPG_VBD_SPRING][order[k]];
+                                const int2 v = sprV[in.elem];
+                                const int32_t other = in.local == 0 ? v.y : v.x;
+                                return make_float4(intBitsAsFloat(other), float(sprP[in.elem].x),
+                                                   float(sprP[in.elem].y), 0.f);
+                            }),
+                     c.stream);
         c.sprPtr.upload(ptr, c.stream);
-        c.spr.upload(rec, c.stream);
         c.sprElem.upload(el, c.stream);
         c.sprV.upload(sprV, c.stream);
         c.sprP.upload(sprP, c.stream);
@@ -641,10 +662,12 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             rk[k] = make_float4(float(p[4]), float(p[5]), float(p[6]), 0.f);
             el[k] = in.elem;
         }
+        c.triW = rowWidth(ptr);
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        c.triO.upload(toRows(ptr, c.triW, make_int2(-1, -1), [&](int32_t k) { return ro[k]; }), c.stream);
+        c.triG.upload(toRows(ptr, c.triW, z4, [&](int32_t k) { return rg[k]; }), c.stream);
+        c.triM.upload(toRows(ptr, c.triW, z4, [&](int32_t k) { return rk[k]; }), c.stream);
         c.triPtr.upload(ptr, c.stream);
-        c.triO.upload(ro, c.stream);
-        c.triG.upload(rg, c.stream);
-        c.triK.upload(rk, c.stream);
         c.triElem.upload(el, c.stream);
         c.triV.upload(triV, c.stream);
         c.triP.upload(triP, c.stream);
@@ -673,9 +696,13 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                 rp[k] = make_float2(float(hinP[in.elem].x), float(hinP[in.elem].y));
                 el[k] = in.elem;
             }
+            const int32_t w = rowWidth(ptr);
+            (cls == 0 ? c.hinEW : c.hinWW) = w;
             (cls == 0 ? c.hinEPtr : c.hinWPtr).upload(ptr, c.stream);
-            (cls == 0 ? c.hinEV : c.hinWV).upload(rv, c.stream);
-            (cls == 0 ? c.hinEP : c.hinWP).upload(rp, c.stream);
+            (cls == 0 ? c.hinEV : c.hinWV)
+                .upload(toRows(ptr, w, make_int4(-1, -1, -1, -1), [&](int32_t k) { return rv[k]; }), c.stream);
+            (cls == 0 ? c.hinEP : c.hinWP)
+                .upload(toRows(ptr, w, make_float2(0.f, 0.f), [&](int32_t k) { return rp[k]; }), c.stream);
             (cls == 0 ? c.hinEElem : c.hinWElem).upload(el, c.stream);
         }
         c.hinVe.upload(hinV, c.stream);
@@ -711,11 +738,13 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             gc[k] = G(oth[2], p[11]);
             el[k] = in.elem;
         }
+        c.tetW = rowWidth(ptr);
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         c.tetPtr.upload(ptr, c.stream);
-        c.tetO.upload(ro, c.stream);
-        c.tetGa.upload(ga, c.stream);
-        c.tetGb.upload(gb, c.stream);
-        c.tetGc.upload(gc, c.stream);
+        c.tetO.upload(toRows(ptr, c.tetW, make_int4(-1, -1, -1, -1), [&](int32_t k) { return ro[k]; }), c.stream);
+        c.tetGa.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return ga[k]; }), c.stream);
+        c.tetGb.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gb[k]; }), c.stream);
+        c.tetGc.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gc[k]; }), c.stream);
         c.tetElem.upload(el, c.stream);
         c.tetV.upload(tetV, c.stream);
         c.tetP.upload(tetP, c.stream);

@@ -23,30 +23,33 @@ namespace spgvbd {
 
 enum EnergyType { kSpring = 1, kStvk = 2, kBending = 3, kSnh = 4, kAnchor = 5 };
 
+// Incidence rows of the solve. Dense classes use fixed-width rows: the j-th incidence of vertex slot v
+// is record v * width + j, and rows shorter than the width are padded with records whose first
+// index is -1 (their contribution is dropped). Anchors, which few vertices have, use row offsets.
 struct SolveArgs {
     float4 *pos;
     const float4 *xt;
     // spring: (other slot bits, L0, k, 0)
-    const int32_t *sprPtr;
+    int32_t sprW;
     const float4 *spr;
-    // anchor: (ax, ay, az, k)
+    // anchor: row offsets, (ax, ay, az, k)
     const int32_t *ancPtr;
     const float4 *anc;
     // StVK: other slots, (g_a, g_b), (kd, ko, ks, 0)
-    const int32_t *triPtr;
+    int32_t triW;
     const int2 *triO;
     const float4 *triG;
-    const float4 *triK;
+    const float4 *triM;
     // bending, split by the owner's slot: hinge-edge (slots 0, 1) and hinge-wing (slots 2, 3)
     // incidences; canonical slots v0..v3, (theta0, keff)
-    const int32_t *hinEPtr;
+    int32_t hinEW;
     const int4 *hinEV;
     const float2 *hinEP;
-    const int32_t *hinWPtr;
+    int32_t hinWW;
     const int4 *hinWV;
     const float2 *hinWP;
     // SNH: other slots (3), (g_a, lambda), (g_b, mu), (g_c, alpha)
-    const int32_t *tetPtr;
+    int32_t tetW;
     const int4 *tetO;
     const float4 *tetGa;
     const float4 *tetGb;
@@ -92,8 +95,9 @@ hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt
                               float h, float gx, float gy, float gz, hipStream_t stream);
 hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
                             hipStream_t stream);
+// start/stop: optional events stamped with the dispatch's own begin/end (nullptr: untimed launch)
 hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
-                            hipStream_t stream);
+                            hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream);
 hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *objOffsets, int32_t nObjects,
                                double *out, hipStream_t stream);

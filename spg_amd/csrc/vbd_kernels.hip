@@ -1,5 +1,7 @@
 // CDNA4 kernels of the VBD step: inertial predict (K1), per-colour vertex block solve (K2),
 // velocity update (K3), FP64 residual (K4) and the per-element derivative hook.
+#include <hip/hip_ext.h>
+
 #include "element.h"
 #include "element_eval.h"
 #include "vbd_device.h"
@@ -12,6 +14,20 @@ using V3f = V3<float>;
 using S3f = Sym3<float>;
 
 __device__ __forceinline__ V3f xyz(float4 p) { return {p.x, p.y, p.z}; }
+
+// accumulate only when `take` (a select: nothing from a dropped contribution reaches g or H)
+__device__ __forceinline__ void accIf(bool take, V3f &g, S3f &H, const V3f &ge, const S3f &He)
+{
+    g.x += take ? ge.x : 0.f;
+    g.y += take ? ge.y : 0.f;
+    g.z += take ? ge.z : 0.f;
+    H.xx += take ? He.xx : 0.f;
+    H.xy += take ? He.xy : 0.f;
+    H.xz += take ? He.xz : 0.f;
+    H.yy += take ? He.yy : 0.f;
+    H.yz += take ? He.yz : 0.f;
+    H.zz += take ? He.zz : 0.f;
+}
 
 __device__ __forceinline__ void acc(V3f &g, S3f &H, const V3f &ge, const S3f &He)
 {
@@ -143,76 +159,77 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
     V3f g = {0.f, 0.f, 0.f};
     S3f H = zeroSym<float>();
     V3f xo = {0.f, 0.f, 0.f};
+    float4 xt4 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (active) {
-        xo = xyz(a.pos[vid]);
-        if (a.triPtr) {
-            const int32_t e = a.triPtr[vid + 1];
-            for (int32_t k = a.triPtr[vid] + lig; k < e; k += G) {
-                const int2 o = a.triO[k];
-                const float4 gg = a.triG[k];
-                const float4 kk = a.triK[k];
-                V3f ge;
-                S3f He;
-                stvkVertex<float>(xo, xyz(a.pos[o.x]), xyz(a.pos[o.y]), gg.x, gg.y, gg.z, gg.w, kk.x, kk.y, kk.z, ge,
-                                  He);
-                acc(g, H, ge, He);
-            }
-        }
-        if (a.hinEPtr) {
-            const int32_t e = a.hinEPtr[vid + 1];
-            for (int32_t k = a.hinEPtr[vid] + lig; k < e; k += G) {
-                const int4 v = a.hinEV[k];
-                const float2 p = a.hinEP[k];
-                V3f ge;
-                S3f He;
-                bendingEdge<float>(xyz(a.pos[v.x]), xyz(a.pos[v.y]), xyz(a.pos[v.z]), xyz(a.pos[v.w]), v.y == vid, p.x,
-                                   p.y, ge, He);
-                acc(g, H, ge, He);
-            }
-        }
-        if (a.hinWPtr) {
-            const int32_t e = a.hinWPtr[vid + 1];
-            for (int32_t k = a.hinWPtr[vid] + lig; k < e; k += G) {
-                const int4 v = a.hinWV[k];
-                const float2 p = a.hinWP[k];
-                V3f ge;
-                S3f He;
-                bendingWing<float>(xyz(a.pos[v.x]), xyz(a.pos[v.y]), xyz(a.pos[v.z]), xyz(a.pos[v.w]), v.w == vid, p.x,
-                                   p.y, ge, He);
-                acc(g, H, ge, He);
-            }
-        }
-        if (a.tetPtr) {
-            const int32_t e = a.tetPtr[vid + 1];
-            for (int32_t k = a.tetPtr[vid] + lig; k < e; k += G) {
-                const int4 o = a.tetO[k];
-                const float4 ga = a.tetGa[k], gb = a.tetGb[k], gc = a.tetGc[k];
-                V3f ge;
-                S3f He;
-                snhVertex<float>(xo, xyz(a.pos[o.x]), xyz(a.pos[o.y]), xyz(a.pos[o.z]), xyz(ga), xyz(gb), xyz(gc),
-                                 ga.w, gb.w, gc.w, ge, He);
-                acc(g, H, ge, He);
-            }
-        }
-        if (a.sprPtr) {
-            const int32_t e = a.sprPtr[vid + 1];
-            for (int32_t k = a.sprPtr[vid] + lig; k < e; k += G) {
-                const float4 s = a.spr[k];
-                V3f ge;
-                S3f He;
-                springVertex<float>(xo, xyz(a.pos[__float_as_int(s.x)]), s.y, s.z, ge, He);
-                acc(g, H, ge, He);
-            }
-        }
+        const float4 xo4 = a.pos[vid];
+        xt4 = a.xt[vid];
+        int32_t ab = 0, ae = 0;
         if (a.ancPtr) {
-            const int32_t e = a.ancPtr[vid + 1];
-            for (int32_t k = a.ancPtr[vid] + lig; k < e; k += G) {
-                const float4 s = a.anc[k];
-                V3f ge;
-                S3f He;
-                anchorVertex<float>(xo, xyz(s), s.w, ge, He);
-                acc(g, H, ge, He);
-            }
+            ab = a.ancPtr[vid];
+            ae = a.ancPtr[vid + 1];
+        }
+        xo = xyz(xo4);
+        const int64_t row = vid;
+        for (int32_t j = lig; j < a.triW; j += G) {
+            const int64_t r = row * a.triW + j;
+            const int2 o = a.triO[r];
+            const float4 gg = a.triG[r];
+            const float4 kk = a.triM[r];
+            const bool ok = o.x >= 0;
+            V3f ge;
+            S3f He;
+            stvkVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), gg.x, gg.y, gg.z, gg.w, kk.x,
+                              kk.y, kk.z, ge, He);
+            accIf(ok, g, H, ge, He);
+        }
+        for (int32_t j = lig; j < a.hinEW; j += G) {
+            const int64_t r = row * a.hinEW + j;
+            const int4 v = a.hinEV[r];
+            const float2 p = a.hinEP[r];
+            const bool ok = v.x >= 0;
+            V3f ge;
+            S3f He;
+            bendingEdge<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), xyz(a.pos[ok ? v.z : vid]),
+                               xyz(a.pos[ok ? v.w : vid]), v.y == vid, p.x, p.y, ge, He);
+            accIf(ok, g, H, ge, He);
+        }
+        for (int32_t j = lig; j < a.hinWW; j += G) {
+            const int64_t r = row * a.hinWW + j;
+            const int4 v = a.hinWV[r];
+            const float2 p = a.hinWP[r];
+            const bool ok = v.x >= 0;
+            V3f ge;
+            S3f He;
+            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), xyz(a.pos[ok ? v.z : vid]),
+                               xyz(a.pos[ok ? v.w : vid]), v.w == vid, p.x, p.y, ge, He);
+            accIf(ok, g, H, ge, He);
+        }
+        for (int32_t j = lig; j < a.tetW; j += G) {
+            const int64_t r = row * a.tetW + j;
+            const int4 o = a.tetO[r];
+            const float4 ga = a.tetGa[r], gb = a.tetGb[r], gc = a.tetGc[r];
+            const bool ok = o.x >= 0;
+            V3f ge;
+            S3f He;
+            snhVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), xyz(a.pos[ok ? o.z : vid]),
+                             xyz(ga), xyz(gb), xyz(gc), ga.w, gb.w, gc.w, ge, He);
+            accIf(ok, g, H, ge, He);
+        }
+        for (int32_t j = lig; j < a.sprW; j += G) {
+            const float4 s = a.spr[row * a.sprW + j];
+            const int32_t other = __float_as_int(s.x);
+            const bool ok = other >= 0;
+            V3f ge;
+            S3f He;
+            springVertex<float>(xo, xyz(a.pos[ok ? other : vid]), s.y, s.z, ge, He);
+            accIf(ok, g, H, ge, He);
+        }
+        for (int32_t k = ab + lig; k < ae; k += G) {
+            const float4 s = a.anc[k];
+            V3f ge;
+            S3f He;
+            anchorVertex<float>(xo, xyz(s), s.w, ge, He);
+            acc(g, H, ge, He);
         }
     }
     if (G > 1) {
@@ -227,7 +244,7 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         H.zz = groupReduce<G>(H.zz, lane64, lig);
     }
     if (active && lig == 0) {
-        const float4 xt = a.xt[vid];
+        const float4 xt = xt4;
         const float mdt2 = xt.w * a.invdt2;
         // f = -(m/h^2)(x - x_tilde) - sum grad; H = (m/h^2) I + sum H_e (vbd.cpp:102-138)
         const V3f f = {-mdt2 * (xo.x - xt.x) - g.x, -mdt2 * (xo.y - xt.y) - g.y, -mdt2 * (xo.z - xt.z) - g.z};
@@ -400,8 +417,17 @@ hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t st
     return hipGetLastError();
 }
 
+template <int G>
+void solveColorLaunch(const SolveArgs &a, int grid, int blockSize, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+{
+    if (start)  // timestamps of the dispatch itself (no extra packets between the events and the kernel)
+        hipExtLaunchKernelGGL(solveColorKernel<G>, dim3(grid), dim3(blockSize), 0, stream, start, stop, 0, a);
+    else
+        solveColorKernel<G><<<grid, blockSize, 0, stream>>>(a);
+}
+
 hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int /*ldsStaging*/,
-                            hipStream_t stream)
+                            hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
     const int64_t nv = a.end - a.begin;
     if (nv <= 0) return hipSuccess;
@@ -411,13 +437,13 @@ hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSiz
     const int64_t waves = (nv + 64 / lanesPerVertex - 1) / (64 / lanesPerVertex);
     const int grid = blocksFor(waves, blockSize / 64);
     switch (lanesPerVertex) {
-    case 1: solveColorKernel<1><<<grid, blockSize, 0, stream>>>(a); break;
-    case 2: solveColorKernel<2><<<grid, blockSize, 0, stream>>>(a); break;
-    case 3: solveColorKernel<3><<<grid, blockSize, 0, stream>>>(a); break;
-    case 4: solveColorKernel<4><<<grid, blockSize, 0, stream>>>(a); break;
-    case 6: solveColorKernel<6><<<grid, blockSize, 0, stream>>>(a); break;
-    case 8: solveColorKernel<8><<<grid, blockSize, 0, stream>>>(a); break;
-    case 16: solveColorKernel<16><<<grid, blockSize, 0, stream>>>(a); break;
+    case 1: solveColorLaunch<1>(a, grid, blockSize, stream, start, stop); break;
+    case 2: solveColorLaunch<2>(a, grid, blockSize, stream, start, stop); break;
+    case 3: solveColorLaunch<3>(a, grid, blockSize, stream, start, stop); break;
+    case 4: solveColorLaunch<4>(a, grid, blockSize, stream, start, stop); break;
+    case 6: solveColorLaunch<6>(a, grid, blockSize, stream, start, stop); break;
+    case 8: solveColorLaunch<8>(a, grid, blockSize, stream, start, stop); break;
+    case 16: solveColorLaunch<16>(a, grid, blockSize, stream, start, stop); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

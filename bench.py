@@ -51,8 +51,9 @@ def main():
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs the VBD step on one GPU; multi-GPU runs are not part of it "
-              "(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print("bench.py: the VBD step runs on one GPU here; runs on several GPUs are not supported "
+              "(instances would shard by object, a single mesh by vertex ownership with per-colour halo "
+              "exchange)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -73,7 +73,9 @@ typedef struct {
     const double *params;
 } spg_vbd_energy;
 
-/* One ParticleGroup: positions, velocities (3 per particle, AoS) and masses, energies in addEnergy order. */
+/* One ParticleGroup: positions, velocities (3 per particle, AoS) and masses, energies in addEnergy order,
+   and optionally the rest positions (ParticleGroup::positions0(); used only by the learned warm start's
+   strain features, zero strain features when NULL). */
 typedef struct {
     int64_t n_particles;
     const double *x;
@@ -81,6 +83,7 @@ typedef struct {
     const double *mass;
     int32_t n_energies;
     const spg_vbd_energy *energies;
+    const double *x0; /* may be NULL */
 } spg_vbd_object;
 
 /* Launch shape of the per-colour vertex solve (chosen by spg_vbd_autotune or set by hand). */
@@ -96,7 +99,9 @@ typedef struct {
     int64_t solve_launches;      /* per-colour vertex-solve kernel launches timed */
     double solve_ms;             /* summed duration of those launches (HIP events on the ctx stream) */
     int64_t vertex_updates;      /* vertex block updates performed by those launches */
-    double algorithmic_bytes;    /* compulsory bytes of those launches (see DESIGN roofline model) */
+    double algorithmic_bytes;    /* compulsory bytes of those launches: 12 B x every vertex of the objects
+                                    with that colour + 32 B x updated vertices + per incidence 4 B plus the
+                                    element record (triangle 40, hinge 24, tet 64, spring 16, anchor 20 B) */
     int64_t fp64_fallbacks;      /* block solves that took the FP64 near-singular path */
     int64_t skipped_blocks;      /* block solves skipped by det(H) <= 1e-10 */
 } spg_vbd_stats;
@@ -140,6 +145,12 @@ int spg_vbd_restore_state(spg_vbd_ctx *ctx);
 int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms);
 /* Same, at x = x_tilde (the inertial guess), for relative residuals. */
 int spg_vbd_residual_at_inertial(spg_vbd_ctx *ctx, double *norms);
+
+/* Colour sweeps of the open substep until every object's relative FP64 residual
+   rho = ||r(x)|| / ||r(x_tilde)|| is <= rel_tol, checking after every `check_every` sweeps, at most
+   max_iterations sweeps. Reports the sweeps done and the final max rho over objects. */
+int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_iterations, int32_t check_every,
+                                 int32_t *iterations, double *rho);
 
 /* Launch configuration and autotuning. */
 int spg_vbd_get_launch(const spg_vbd_ctx *ctx, spg_vbd_launch *cfg);

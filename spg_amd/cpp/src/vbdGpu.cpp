@@ -124,7 +124,7 @@ void VBDGpu::buildContext()
     }
     std::vector<std::vector<FlatEnergy>> flat(m_objects.size());
     std::vector<std::vector<spg_vbd_energy>> descs(m_objects.size());
-    std::vector<std::vector<double>> xs(m_objects.size()), vs(m_objects.size());
+    std::vector<std::vector<double>> xs(m_objects.size()), vs(m_objects.size()), x0s(m_objects.size());
     std::vector<spg_vbd_object> objs(m_objects.size());
     for (size_t o = 0; o < m_objects.size(); ++o) {
         const auto &pg = m_objects[o];
@@ -134,8 +134,9 @@ void VBDGpu::buildContext()
                                 f.stencils.data(), f.params.data()});
         xs[o] = packVec3(pg.positions());
         vs[o] = packVec3(pg.velocities());
-        objs[o] = {pg.size(), xs[o].data(), vs[o].data(), pg.masses().data(), static_cast<int32_t>(descs[o].size()),
-                   descs[o].data()};
+        x0s[o] = packVec3(pg.positions0());
+        objs[o] = {pg.size(),         xs[o].data(),   vs[o].data(), pg.masses().data(),
+                   static_cast<int32_t>(descs[o].size()), descs[o].data(), x0s[o].data()};
     }
     spg_vbd_ctx *ctx = nullptr;
     check(spg_vbd_create(&ctx, objs.data(), static_cast<int32_t>(objs.size()), m_device), nullptr);

@@ -987,6 +987,36 @@ int spg_vbd_residual_at_inertial(spg_vbd_ctx *ctx, double *norms)
     });
 }
 
+int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_iterations, int32_t check_every,
+                                 int32_t *iterations, double *rho)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !iterations || !rho) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "iterate_to_tolerance needs an open substep");
+        if (max_iterations < 0 || check_every < 1) throw VbdError(SPG_VBD_ERR_INVALID, "invalid iteration counts");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<double> r0(ctx->nObjects), r(ctx->nObjects);
+        ctx->residual(true, r0.data());
+        auto relMax = [&] {
+            double m = 0.0;
+            for (int32_t o = 0; o < ctx->nObjects; ++o) m = std::max(m, r0[o] > 0 ? r[o] / r0[o] : r[o]);
+            return m;
+        };
+        ctx->residual(false, r.data());
+        double cur = relMax();
+        int32_t done = 0;
+        while (!(cur <= rel_tol) && done < max_iterations) {
+            const int32_t n = std::min(check_every, max_iterations - done);
+            ctx->sweepColors(n, ctx->timing);
+            done += n;
+            ctx->residual(false, r.data());
+            cur = relMax();
+        }
+        *iterations = done;
+        *rho = cur;
+    });
+}
+
 int spg_vbd_get_launch(const spg_vbd_ctx *ctx, spg_vbd_launch *cfg)
 {
     if (!ctx || !cfg) return SPG_VBD_ERR_INVALID;

@@ -34,7 +34,7 @@ class VbdEnergy(ctypes.Structure):
 
 class VbdObject(ctypes.Structure):
     _fields_ = [("n_particles", ctypes.c_int64), ("x", _dp), ("v", _dp), ("mass", _dp),
-                ("n_energies", ctypes.c_int32), ("energies", ctypes.POINTER(VbdEnergy))]
+                ("n_energies", ctypes.c_int32), ("energies", ctypes.POINTER(VbdEnergy)), ("x0", _dp)]
 
 
 class VbdLaunch(ctypes.Structure):
@@ -96,6 +96,7 @@ def vbd_lib():
         _proto(L, "spg_vbd_restore_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_residual", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_residual_at_inertial", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_vbd_iterate_to_tolerance", ctypes.c_int, [c, dbl, i32, i32, _ip, _dp])
         _proto(L, "spg_vbd_get_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
         _proto(L, "spg_vbd_set_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
         _proto(L, "spg_vbd_autotune", ctypes.c_int, [c, i32, ctypes.POINTER(VbdLaunch)])
@@ -318,6 +319,7 @@ class VbdContext:
         for k, o in enumerate(objects):
             x = _f64(o["x"], (-1, 3))
             v = None if o.get("v") is None else _f64(o["v"], (-1, 3))
+            x0 = None if o.get("x0") is None else _f64(o["x0"], (-1, 3))
             m = _f64(o["m"])
             ens = (VbdEnergy * max(1, len(o["energies"])))()
             for j, (t, st, prm) in enumerate(o["energies"]):
@@ -326,8 +328,8 @@ class VbdContext:
                 prm = _f64(prm, (st.shape[0], PARAMS[t]))
                 ens[j] = VbdEnergy(t, st.shape[0], _i(st), _d(prm))
                 keep += [st, prm]
-            objs[k] = VbdObject(x.shape[0], _d(x), _d(v), _d(m), len(o["energies"]), ens)
-            keep += [x, v, m, ens]
+            objs[k] = VbdObject(x.shape[0], _d(x), _d(v), _d(m), len(o["energies"]), ens, _d(x0))
+            keep += [x, v, x0, m, ens]
         h = ctypes.c_void_p()
         rc = self._L.spg_vbd_create(ctypes.byref(h), objs, len(objects), device)
         if rc != 0:
@@ -418,6 +420,13 @@ class VbdContext:
         out = np.zeros(self.n_objects)
         self._check(self._L.spg_vbd_residual_at_inertial(self.h, _d(out)))
         return out
+
+    def iterate_to_tolerance(self, rel_tol, max_iterations=1000, check_every=1):
+        it = ctypes.c_int32(0)
+        rho = ctypes.c_double(0.0)
+        self._check(self._L.spg_vbd_iterate_to_tolerance(self.h, rel_tol, max_iterations, check_every,
+                                                          ctypes.byref(it), ctypes.byref(rho)))
+        return it.value, rho.value
 
     def launch(self):
         c = VbdLaunch()

@@ -83,7 +83,7 @@ class Scene:
     def vbd_object(self):
         """Prepared arrays (spg_vbd_object layout) computed by the C++ mirror's preparePrecomputations."""
         pg = self.particle_group()
-        return {"x": self.x, "v": None, "m": self.m,
+        return {"x": self.x, "v": None, "m": self.m, "x0": self.x0,
                 "energies": [(kind, st, pg.energy_params(i)) for i, (kind, st, _) in enumerate(self.energies)]}
 
 

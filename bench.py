@@ -11,7 +11,10 @@ roofline   = the vertex-solve kernel: compulsory bytes per launch / mean launch 
              durations from HIP events on the solver's stream over an event-timed replay of the same
              steps right after the throughput region
 cpu_baseline = the FP64 OpenMP oracle (restatement of spg's VBD::step), closed-form derivatives,
-             one step of the same scene on the host cores
+             repeated first steps of the same scene on the host cores (about 10 s of CPU work), plus a
+             one-thread sample and a dual-number (AD-cost) sample
+traffic    = HBM bytes per vertex-solve launch from the committed rocprofv3 counter passes of this
+             workload and launch shape (profiles/), null when the shape differs
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu]
 """
@@ -109,6 +112,7 @@ def main():
     achieved = bytes_per_launch / (mean_ms * 1e-3) / 1e9
     x, v = ctx.state()
     finite = bool(np.all(np.isfinite(x)))
+    traffic, traffic_src = measured_traffic(ctx.launch(), args)
 
     cpu = None
     if not args.no_cpu:
@@ -133,7 +137,8 @@ def main():
                    "vertices": V, "iterations": iters, "colors": ctx.num_colors(),
                    "launch": ctx.launch(), "autotuned": tuned},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_src,
                      "kernel": "solveColorKernel", "mean_launch_us": mean_ms * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "launches_timed": st["solve_launches"]},
@@ -144,30 +149,57 @@ def main():
     print(json.dumps(out))
 
 
+def measured_traffic(launch, args):
+    """HBM bytes per vertex-solve launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
+    this workload (scaled by the same counters on a copy kernel of known size), when the launch shape
+    and scene match the profiled ones; None otherwise."""
+    path = os.path.join(ROOT, "profiles", "solveColorKernel_cloth512_traffic.json")
+    if args.n != 512 or args.seed != 3 or not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        t = json.load(f)
+    if f"lanes {launch['lanes_per_vertex']}, block {launch['block_size']}" not in t["workload"]:
+        return None, None
+    return t["traffic_bytes"], "profiles/solveColorKernel_cloth512_traffic.json (rocprofv3 --pmc, same kernel and launch)"
+
+
 def cpu_baseline(scene):
-    """FP64 oracle (OpenMP restatement of VBD::step, closed-form derivatives) on the host cores:
-    one step of the same scene, colouring/incidence excluded (done before timing)."""
+    """FP64 oracle (OpenMP restatement of VBD::step) on the host cores: repeated first steps of the same
+    scene for a bounded sample (~10 s closed form on all threads, ~5 s on one thread, ~8 s with
+    forward-mode dual numbers), colouring/incidence excluded (done before timing)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
+    import numpy as np
 
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-    ol.set_threads(cores)
     o = ol.from_scene(scene)
     o.colors()  # first-call precomputation outside the timed region (vbd.cpp:14-18)
-    t0 = time.perf_counter()
-    o.step(scene.dt, 1, scene.iterations, scene.gravity, mode=ol.CLOSED_FORM)
-    dt_cf = time.perf_counter() - t0
-    # AD-mode (TinyAD-like cost: two full-element derivative passes per incidence), one iteration
-    o2 = ol.from_scene(scene)
-    o2.colors()
-    t0 = time.perf_counter()
-    o2.step(scene.dt, 1, 1, scene.gravity, mode=ol.AD)
-    dt_ad = time.perf_counter() - t0
-    return {"value": scene.n * scene.iterations / dt_cf, "unit": "vertex-updates/s", "cores": cores, "kind": "port",
-            "sample": f"one step ({scene.iterations} iterations) of the same {scene.n}-vertex cloth, FP64 "
-                      f"closed-form derivatives, {dt_cf:.2f} s",
-            "ad_mode_value": scene.n * 1 / dt_ad,
-            "ad_mode_sample": f"one step of 1 iteration, FP64 forward-mode dual numbers, {dt_ad:.2f} s"}
+    v0 = np.zeros_like(scene.x)
+
+    def timed(threads, iterations, mode, budget_s):
+        """Repeated first steps from the initial state until about budget_s of CPU work."""
+        ol.set_threads(threads)
+        n, t = 0, 0.0
+        while t < budget_s:
+            o.set_state(scene.x, v0)
+            t0 = time.perf_counter()
+            o.step(scene.dt, 1, iterations, scene.gravity, mode=mode)
+            t += time.perf_counter() - t0
+            n += 1
+        return scene.n * iterations * n / t, n, t
+
+    cf, n_cf, t_cf = timed(cores, scene.iterations, ol.CLOSED_FORM, 10.0)
+    cf1, n_cf1, t_cf1 = timed(1, 2, ol.CLOSED_FORM, 5.0)
+    # AD mode (TinyAD-like cost: two full-element derivative passes per incidence)
+    ad, n_ad, t_ad = timed(cores, 1, ol.AD, 8.0)
+    return {"value": cf, "unit": "vertex-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n_cf} first steps ({scene.iterations} iterations each) of the same {scene.n}-vertex cloth, "
+                      f"FP64 closed-form derivatives, {t_cf:.1f} s",
+            "one_core_value": cf1,
+            "one_core_sample": f"{n_cf1} first steps of 2 iterations on 1 thread, closed form, {t_cf1:.1f} s",
+            "ad_mode_value": ad,
+            "ad_mode_sample": f"{n_ad} first steps of 1 iteration on {cores} threads, FP64 forward-mode dual numbers, "
+                              f"{t_ad:.1f} s"}
 
 
 if __name__ == "__main__":

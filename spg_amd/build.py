@@ -9,6 +9,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
@@ -28,7 +29,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-packed-fp32-ops", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fgpu-flush-denormals-to-zero",
              "-Wall", "-Wno-unused-result"]
 
-VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "coloring.cpp"]
+VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "mlp_context.hip", "warmstart_kernels.hip",
+               "coloring.cpp"]
 HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "vbdGpu.cpp", "host_capi.cpp"]
 
 
@@ -44,10 +46,20 @@ def _atomic_link(cmd, out):
 
 
 def build_vbd():
-    os.makedirs(LIBDIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, s) for s in VBD_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    """Compiles every source to an object in parallel (each one is a separate hipcc process), then links."""
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    srcs = [os.path.join(CSRC, s) for s in VBD_SOURCES]
+    objs = [os.path.join(objdir, os.path.basename(s) + ".o") for s in srcs]
+
+    def compile_one(pair):
+        src, obj = pair
+        _run([HIPCC, *HIP_FLAGS, f"-I{INCLUDE}", f"-I{CSRC}", "-c", src, "-o", obj])
+
+    with ThreadPoolExecutor(max_workers=min(len(srcs), os.cpu_count() or 1, 8)) as pool:
+        list(pool.map(compile_one, zip(srcs, objs)))
     out = os.path.join(LIBDIR, "libspg_vbd.so")
-    _atomic_link([HIPCC, *HIP_FLAGS, "-shared", f"-I{INCLUDE}", f"-I{CSRC}", *srcs], out)
+    _atomic_link([HIPCC, *HIP_FLAGS, "-shared", *objs], out)
     return out
 
 

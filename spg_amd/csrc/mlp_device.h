@@ -5,13 +5,15 @@
 
 #include <cstdint>
 
+#include "spg_vbd_learned.h"
+
 namespace spgvbd {
 
 constexpr int kMlpFeatures = 24;  // per-vertex input features
 constexpr int kMlpIn = 32;        // features zero-padded to one MFMA K step
 constexpr int kMlpHidden = 128;
 constexpr int kMlpOut = 3;
-constexpr int kMlpOutPad = 16;  // output rows of the last weight padded to one MFMA tile
+constexpr int kMlpOutPad = 32;  // output rows of the last weight padded to one MFMA K step
 constexpr int kMlpLayers = 4;
 
 // What the GEMM does with an accumulator tile C[row][col] before storing it:
@@ -40,7 +42,36 @@ hipError_t launchColSum(const float *m, int ld, int rows, int cols, float *out, 
 hipError_t launchAdam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
                       int64_t t, hipStream_t stream);
 hipError_t launchWeightCopies(const float *w, int rows, int cols, __bf16 *wb, __bf16 *wtb, hipStream_t stream);
-hipError_t launchPackInput(const float *f, const float *t, const int32_t *index, int n, int rows, __bf16 *X, __bf16 *XT,
-                           float *tOut, hipStream_t stream);
+hipError_t launchPackInput(const float *f, const float *t, const int32_t *index, const float *mean, const float *invStd,
+                           int n, int rows, __bf16 *X, __bf16 *XT, float *tOut, hipStream_t stream);
+
+// Inputs of the feature kernel (warmstart_kernels.hip), slot numbering of a VBD context.
+struct FeatureArgs {
+    const float4 *xt = nullptr;    // inertial target (w: mass)
+    const float4 *vel = nullptr;   // velocity after the substep's gravity kick
+    const float4 *corr = nullptr;  // x* - x~ of the previous substep
+    const int32_t *nbPtr = nullptr;
+    const int32_t *nbIdx = nullptr;
+    const float *nbRest = nullptr;  // rest length of each one-ring edge
+    const float4 *stat = nullptr;   // (log10 Y, log10 k_bending, anchored, 0)
+    int32_t n = 0;
+    float h = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    float invS = 0.f, invU = 0.f;  // 1 / (h^2 a_ref), 1 / (h a_ref)
+    float *out = nullptr;          // n x kMlpFeatures
+};
+
+hipError_t launchFeatures(const FeatureArgs &a, hipStream_t stream);
+hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float *y, int32_t ldy, float s, int32_t n,
+                                hipStream_t stream);
+hipError_t launchGate(float4 *pos, const float4 *xt, const int32_t *objOfSlot, const int32_t *reject, int32_t n,
+                      hipStream_t stream);
+hipError_t launchCorrection(const float4 *pos, const float4 *xt, float4 *corr, int32_t n, hipStream_t stream);
+
+// Network inference on device-resident features (n x kMlpFeatures FP32) into y (n rows, stride
+// ldy >= kMlpInferCols; the first kMlpOut columns are the outputs), ordered on `stream`. Throws
+// VbdError. Defined with the network object.
+constexpr int kMlpInferCols = 16;
+void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipStream_t stream);
+int mlpDevice(const spg_mlp *net);
 
 }  // namespace spgvbd

@@ -161,11 +161,13 @@ __global__ void __launch_bounds__(256) weightCopiesKernel(const float *__restric
     wtb[c * rows + r] = b;
 }
 
-// Rows of an FP32 (n x 24) feature table -> BF16 (rows x 32) zero-padded, optional transpose; rows of
-// the table are picked through `index` (a minibatch) or taken in order when index is null.
+// Rows of an FP32 (n x 24) feature table, standardised ((f - mean) / std), -> BF16 (rows x 32)
+// zero-padded, optional transpose; rows of the table are picked through `index` (a minibatch) or taken
+// in order when index is null. The matching targets are gathered into tOut (rows x 3) when given.
 __global__ void __launch_bounds__(256) packInputKernel(const float *__restrict__ f, const float *__restrict__ t,
-                                                       const int32_t *__restrict__ index, int n, int rows,
-                                                       __bf16 *__restrict__ X, __bf16 *__restrict__ XT,
+                                                       const int32_t *__restrict__ index,
+                                                       const float *__restrict__ mean, const float *__restrict__ invStd,
+                                                       int n, int rows, __bf16 *__restrict__ X, __bf16 *__restrict__ XT,
                                                        float *__restrict__ tOut)
 {
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
@@ -174,7 +176,7 @@ __global__ void __launch_bounds__(256) packInputKernel(const float *__restrict__
     float v = 0.f;
     int src = -1;
     if (row < n) src = index ? index[row] : row;
-    if (src >= 0 && col < kMlpFeatures) v = f[static_cast<int64_t>(src) * kMlpFeatures + col];
+    if (src >= 0 && col < kMlpFeatures) v = (f[static_cast<int64_t>(src) * kMlpFeatures + col] - mean[col]) * invStd[col];
     const __bf16 b = static_cast<__bf16>(v);
     X[i] = b;
     if (XT) XT[static_cast<int64_t>(col) * rows + row] = b;
@@ -223,11 +225,11 @@ hipError_t launchWeightCopies(const float *w, int rows, int cols, __bf16 *wb, __
     return hipGetLastError();
 }
 
-hipError_t launchPackInput(const float *f, const float *t, const int32_t *index, int n, int rows, __bf16 *X, __bf16 *XT,
-                           float *tOut, hipStream_t stream)
+hipError_t launchPackInput(const float *f, const float *t, const int32_t *index, const float *mean, const float *invStd,
+                           int n, int rows, __bf16 *X, __bf16 *XT, float *tOut, hipStream_t stream)
 {
-    packInputKernel<<<blocksFor(static_cast<int64_t>(rows) * kMlpIn, 256), 256, 0, stream>>>(f, t, index, n, rows, X, XT,
-                                                                                           tOut);
+    packInputKernel<<<blocksFor(static_cast<int64_t>(rows) * kMlpIn, 256), 256, 0, stream>>>(f, t, index, mean, invStd,
+                                                                                           n, rows, X, XT, tOut);
     return hipGetLastError();
 }
 

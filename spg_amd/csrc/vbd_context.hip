@@ -14,52 +14,17 @@
 
 #include "coloring.h"
 #include "spg_vbd.h"
+#include "spg_vbd_learned.h"
 #include "element_eval.h"
 #include "vbd_device.h"
+#include "host_util.h"
+#include "mlp_device.h"
 
 using namespace spgvbd;
 
 namespace {
 
 thread_local std::string g_lastError;
-
-struct VbdError : std::runtime_error {
-    int code;
-    VbdError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
-};
-
-void hipCheck(hipError_t e, const char *what)
-{
-    if (e != hipSuccess) throw VbdError(SPG_VBD_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count)
-    {
-        release();
-        if (count == 0) return;
-        hipCheck(hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T)), "hipMalloc");
-        n = count;
-    }
-    void upload(const std::vector<T> &h, hipStream_t s)
-    {
-        alloc(h.size());
-        if (!h.empty()) hipCheck(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
-    }
-};
 
 int stencilSizeOf(int type)
 {
@@ -158,6 +123,13 @@ struct spg_vbd_ctx {
     DevBuf<double2> hinPe;
     DevBuf<int4> tetV;
     DevBuf<double> tetP;
+
+    // learned warm start: x* - x~ of the last substep, one-ring tables, static features, workspaces
+    DevBuf<float4> corr, savedCorr;
+    DevBuf<int32_t> nbPtr, nbIdx, objOfSlot, reject;
+    DevBuf<float> nbRest, feat, pred;
+    DevBuf<float4> stat;
+    double g[3] = {0.0, 0.0, 0.0};  // gravity of the open substep
 
     spg_vbd_launch launch{4, 256, 0, 0};
     bool timing = false;
@@ -281,6 +253,7 @@ struct spg_vbd_ctx {
     void beginSubstep(double hh, const double *g)
     {
         h = hh;
+        for (int k = 0; k < 3; ++k) this->g[k] = g[k];
         hipCheck(launchBeginSubstep(pos.p, xold.p, vel.p, xt.p, mass.p, static_cast<int32_t>(nv), static_cast<float>(hh),
                                     static_cast<float>(g[0]), static_cast<float>(g[1]), static_cast<float>(g[2]),
                                     stream),
@@ -290,6 +263,7 @@ struct spg_vbd_ctx {
 
     void endSubstep()
     {
+        hipCheck(launchCorrection(pos.p, xt.p, corr.p, static_cast<int32_t>(nv), stream), "correction");
         hipCheck(launchEndSubstep(pos.p, xold.p, vel.p, static_cast<int32_t>(nv), static_cast<float>(1.0 / h), stream),
                  "end substep");
         inSubstep = false;
@@ -329,6 +303,72 @@ struct spg_vbd_ctx {
         hipCheck(launchResidual(a, stream), "residual launch");
         hipCheck(launchReduceObjects(r2.p, objSlots.p, objOffsets.p, nObjects, norms.p, stream), "reduce launch");
         hipCheck(hipMemcpyAsync(out, norms.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    // acceleration scale of the learned features: |g|, or 1 without gravity
+    double refAccel() const
+    {
+        const double a = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+        return a > 0.0 ? a : 1.0;
+    }
+
+    void computeFeatures()
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "features need an open substep (begin_substep)");
+        feat.reserve(static_cast<size_t>(nv) * kMlpFeatures);
+        FeatureArgs a;
+        a.xt = xt.p;
+        a.vel = vel.p;
+        a.corr = corr.p;
+        a.nbPtr = nbPtr.p;
+        a.nbIdx = nbIdx.p;
+        a.nbRest = nbRest.p;
+        a.stat = stat.p;
+        a.n = static_cast<int32_t>(nv);
+        a.h = static_cast<float>(h);
+        a.gx = static_cast<float>(g[0]);
+        a.gy = static_cast<float>(g[1]);
+        a.gz = static_cast<float>(g[2]);
+        a.invS = static_cast<float>(1.0 / (h * h * refAccel()));
+        a.invU = static_cast<float>(1.0 / (h * refAccel()));
+        a.out = feat.p;
+        hipCheck(launchFeatures(a, stream), "features launch");
+    }
+
+    void warmStart(spg_mlp *net, bool gate, int32_t *accepted, double *rho)
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
+        if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
+        computeFeatures();
+        pred.reserve(static_cast<size_t>(nv) * kMlpInferCols);
+        mlpInferDevice(net, feat.p, nv, pred.p, kMlpInferCols, stream);
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipCheck(launchApplyWarmStart(pos.p, xt.p, pred.p, kMlpInferCols, static_cast<float>(h * h * refAccel()),
+                                      static_cast<int32_t>(nv), stream),
+                 "warm start launch");
+        std::vector<int32_t> rej(nObjects, 0);
+        std::vector<double> r0(nObjects, 0.0), rp(nObjects, 0.0);
+        if (gate || rho) {
+            residual(true, r0.data());
+            residual(false, rp.data());
+        }
+        if (gate) {
+            bool any = false;
+            for (int32_t o = 0; o < nObjects; ++o) {
+                rej[o] = !(rp[o] <= r0[o]);
+                any = any || rej[o];
+            }
+            if (any) {
+                hipCheck(hipMemcpyAsync(reject.p, rej.data(), sizeof(int32_t) * nObjects, hipMemcpyHostToDevice, stream),
+                         "upload");
+                hipCheck(launchGate(pos.p, xt.p, objOfSlot.p, reject.p, static_cast<int32_t>(nv), stream), "gate launch");
+            }
+        }
+        for (int32_t o = 0; o < nObjects; ++o) {
+            if (accepted) accepted[o] = !rej[o];
+            if (rho) rho[o] = r0[o] > 0.0 ? rp[o] / r0[o] : rp[o];
+        }
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
@@ -428,6 +468,8 @@ std::vector<Rec> toRows(const std::vector<int32_t> &ptr, int32_t width, Rec pad,
         for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k) rows[v * width + (k - ptr[v])] = make(k);
     return rows;
 }
+
+void buildFeatureTables(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects);
 
 void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
 {
@@ -761,6 +803,136 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.colorBytes[k] = 12.0 * double(c.colorActiveV[k]) + 32.0 * double(c.colorBegin[k + 1] - c.colorBegin[k]) +
                           colorRecordBytes[k];
     hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+    buildFeatureTables(c, objs, nObjects);
+}
+
+// Static inputs of the learned warm start: one-ring neighbours (vertices sharing a triangle,
+// tetrahedron or spring) with rest lengths from x0 (x when x0 is NULL), per-vertex log10 stiffness of
+// the incident membrane / solid / spring elements and of the incident hinges (recovered from the
+// prepared parameters), the anchored flag, and the slot -> object map of the warm-start gate.
+void buildFeatureTables(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
+{
+    std::vector<int64_t> deg(c.nv + 1, 0);
+    std::vector<double> matY(c.nv, 0.0), matB(c.nv, 0.0);
+    std::vector<int32_t> nY(c.nv, 0), nB(c.nv, 0), anchored(c.nv, 0);
+    std::vector<std::pair<int32_t, int32_t>> pairs;
+    auto rest = [&](const spg_vbd_object &ob) { return ob.x0 ? ob.x0 : ob.x; };
+    for (int32_t o = 0; o < nObjects; ++o) {
+        const auto &ob = objs[o];
+        const int64_t base = c.objOffset[o];
+        const double *x0 = rest(ob);
+        for (int32_t e = 0; e < ob.n_energies; ++e) {
+            const auto &en = ob.energies[e];
+            const int ss = stencilSizeOf(en.type), np = paramsOf(en.type);
+            for (int64_t i = 0; i < en.n_stencils; ++i) {
+                const int32_t *st = en.stencils + i * ss;
+                const double *p = en.params + i * np;
+                double logY = 0.0;
+                bool hasY = false, edges = false;
+                switch (en.type) {
+                case SPG_VBD_SPRING:
+                    logY = std::log10(std::max(p[1], 1e-30));
+                    hasY = edges = true;
+                    break;
+                case SPG_VBD_MEMBRANE_STVK: {
+                    // K = A Y / (1 - nu^2) [[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]], A = 1 / (2 |det Dm^-1|)
+                    const double area = 0.5 / std::max(std::fabs(p[0] * p[3] - p[1] * p[2]), 1e-300);
+                    const double nu = p[4] != 0.0 ? p[5] / p[4] : 0.0;
+                    logY = std::log10(std::max(p[4] * (1.0 - nu * nu) / area, 1e-30));
+                    hasY = edges = true;
+                    break;
+                }
+                case SPG_VBD_STABLE_NEO_HOOKEAN: {
+                    // lambda' = V lambda, mu' = V mu with lambda = mu / (1 - 2 nu), Y = 2 mu (1 + nu)
+                    const double d = p[0] * (p[4] * p[8] - p[5] * p[7]) - p[1] * (p[3] * p[8] - p[5] * p[6]) +
+                                     p[2] * (p[3] * p[7] - p[4] * p[6]);
+                    const double vol = 1.0 / std::max(6.0 * std::fabs(d), 1e-300);
+                    const double lam = p[9] / vol, mu = p[10] / vol;
+                    const double nu = lam != 0.0 ? 0.5 * (1.0 - mu / lam) : 0.0;
+                    logY = std::log10(std::max(2.0 * mu * (1.0 + nu), 1e-30));
+                    hasY = edges = true;
+                    break;
+                }
+                case SPG_VBD_DISCRETE_BENDING: {
+                    // k_eff = k 12 |e|^2 / (|e x e3| + |e x e4|) at the rest shape
+                    double a[3], b[3], d[3];
+                    for (int q = 0; q < 3; ++q) {
+                        a[q] = x0[3 * st[1] + q] - x0[3 * st[0] + q];
+                        b[q] = x0[3 * st[2] + q] - x0[3 * st[0] + q];
+                        d[q] = x0[3 * st[3] + q] - x0[3 * st[0] + q];
+                    }
+                    auto crossNorm = [](const double *u, const double *v) {
+                        const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2],
+                                     cz = u[0] * v[1] - u[1] * v[0];
+                        return std::sqrt(cx * cx + cy * cy + cz * cz);
+                    };
+                    const double e2 = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+                    const double k = e2 > 0 ? p[1] * (crossNorm(a, b) + crossNorm(a, d)) / (12.0 * e2) : 0.0;
+                    for (int q = 0; q < 4; ++q) {
+                        matB[base + st[q]] += std::log10(std::max(k, 1e-30));
+                        ++nB[base + st[q]];
+                    }
+                    break;
+                }
+                case SPG_VBD_SPRING_ANCHOR:
+                    anchored[base + st[0]] = 1;
+                    break;
+                }
+                if (hasY)
+                    for (int q = 0; q < ss; ++q) {
+                        matY[base + st[q]] += logY;
+                        ++nY[base + st[q]];
+                    }
+                if (edges)
+                    for (int q = 0; q < ss; ++q)
+                        for (int r = 0; r < ss; ++r)
+                            if (q != r && st[q] != st[r])
+                                pairs.push_back({static_cast<int32_t>(base + st[q]), static_cast<int32_t>(base + st[r])});
+            }
+        }
+    }
+    // neighbour rows in gid order, sorted and unique, then renumbered to slots
+    std::sort(pairs.begin(), pairs.end());
+    pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+    for (const auto &pr : pairs) ++deg[c.slotOfGid[pr.first] + 1];
+    for (int64_t s = 0; s < c.nv; ++s) deg[s + 1] += deg[s];
+    if (deg[c.nv] >= (int64_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many neighbour pairs");
+    std::vector<int32_t> ptr(c.nv + 1), idx(deg[c.nv]);
+    std::vector<float> restLen(deg[c.nv]);
+    for (int64_t s = 0; s <= c.nv; ++s) ptr[s] = static_cast<int32_t>(deg[s]);
+    std::vector<int64_t> fill(deg.begin(), deg.end() - 1);
+    std::vector<int32_t> objOfGid(c.nv);
+    for (int32_t o = 0; o < nObjects; ++o)
+        for (int64_t g = c.objOffset[o]; g < c.objOffset[o + 1]; ++g) objOfGid[g] = o;
+    for (const auto &pr : pairs) {
+        const int32_t o = objOfGid[pr.first];
+        const double *x0 = rest(objs[o]);
+        const int64_t ia = pr.first - c.objOffset[o], ib = pr.second - c.objOffset[o];
+        double d2 = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            const double d = x0[3 * ib + q] - x0[3 * ia + q];
+            d2 += d * d;
+        }
+        const int64_t k = fill[c.slotOfGid[pr.first]]++;
+        idx[k] = c.slotOfGid[pr.second];
+        restLen[k] = static_cast<float>(std::max(std::sqrt(d2), 1e-12));
+    }
+    std::vector<float4> st(c.nv);
+    std::vector<int32_t> objOfSlot(c.nv);
+    for (int64_t g = 0; g < c.nv; ++g) {
+        const int32_t s = c.slotOfGid[g];
+        st[s] = make_float4(nY[g] ? float(matY[g] / nY[g]) : 0.f, nB[g] ? float(matB[g] / nB[g]) : 0.f,
+                            float(anchored[g]), 0.f);
+        objOfSlot[s] = objOfGid[g];
+    }
+    c.nbPtr.upload(ptr, c.stream);
+    c.nbIdx.upload(idx, c.stream);
+    c.nbRest.upload(restLen, c.stream);
+    c.stat.upload(st, c.stream);
+    c.objOfSlot.upload(objOfSlot, c.stream);
+    c.reject.alloc(nObjects);
+    c.corr.upload(std::vector<float4>(c.nv, make_float4(0.f, 0.f, 0.f, 0.f)), c.stream);
+    hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
 }
 
 void checkObject(const spg_vbd_ctx *ctx, int32_t object)
@@ -769,10 +941,11 @@ void checkObject(const spg_vbd_ctx *ctx, int32_t object)
     if (object < 0 || object >= ctx->nObjects) throw VbdError(SPG_VBD_ERR_INVALID, "object index out of range");
 }
 
+// Writes object `object`'s rows of dst from x (zeros when x is NULL).
 void uploadPositions(spg_vbd_ctx &c, int32_t object, const double *x, float4 *dst)
 {
     const int64_t n = c.objOffset[object + 1] - c.objOffset[object];
-    for (int64_t i = 0; i < n; ++i)
+    for (int64_t i = 0; x && i < n; ++i)
         if (!finite3(x + 3 * i)) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite positions");
     // read-modify-write of the object's slots
     std::vector<float4> h(c.nv);
@@ -780,7 +953,8 @@ void uploadPositions(spg_vbd_ctx &c, int32_t object, const double *x, float4 *ds
     hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
     for (int64_t i = 0; i < n; ++i) {
         const int32_t s = c.slotOfGid[c.objOffset[object] + i];
-        h[s] = make_float4(float(x[3 * i]), float(x[3 * i + 1]), float(x[3 * i + 2]), 0.f);
+        h[s] = x ? make_float4(float(x[3 * i]), float(x[3 * i + 1]), float(x[3 * i + 2]), 0.f)
+                 : make_float4(0.f, 0.f, 0.f, 0.f);
     }
     hipCheck(hipMemcpyAsync(dst, h.data(), sizeof(float4) * c.nv, hipMemcpyHostToDevice, c.stream), "upload");
     hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
@@ -861,6 +1035,7 @@ int spg_vbd_set_state(spg_vbd_ctx *ctx, int32_t object, const double *x, const d
                 throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
         if (x) uploadPositions(*ctx, object, x, ctx->pos.p);
         if (v) uploadPositions(*ctx, object, v, ctx->vel.p);
+        if (x || v) uploadPositions(*ctx, object, nullptr, ctx->corr.p);  // no substep history
     });
 }
 
@@ -951,9 +1126,11 @@ int spg_vbd_save_state(spg_vbd_ctx *ctx)
         if (!ctx->savedPos.p) {
             ctx->savedPos.alloc(ctx->nv);
             ctx->savedVel.alloc(ctx->nv);
+            ctx->savedCorr.alloc(ctx->nv);
         }
         hipCheck(launchCopy(ctx->pos.p, ctx->savedPos.p, ctx->nv, ctx->stream), "save");
         hipCheck(launchCopy(ctx->vel.p, ctx->savedVel.p, ctx->nv, ctx->stream), "save");
+        hipCheck(launchCopy(ctx->corr.p, ctx->savedCorr.p, ctx->nv, ctx->stream), "save");
     });
 }
 
@@ -966,6 +1143,7 @@ int spg_vbd_restore_state(spg_vbd_ctx *ctx)
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         hipCheck(launchCopy(ctx->savedPos.p, ctx->pos.p, ctx->nv, ctx->stream), "restore");
         hipCheck(launchCopy(ctx->savedVel.p, ctx->vel.p, ctx->nv, ctx->stream), "restore");
+        hipCheck(launchCopy(ctx->savedCorr.p, ctx->corr.p, ctx->nv, ctx->stream), "restore");
     });
 }
 
@@ -1014,6 +1192,34 @@ int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_i
         }
         *iterations = done;
         *rho = cur;
+    });
+}
+
+int spg_vbd_features(spg_vbd_ctx *ctx, int32_t object, float *features)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !features) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        if (object < -1 || object >= ctx->nObjects) throw VbdError(SPG_VBD_ERR_INVALID, "object index out of range");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->computeFeatures();
+        std::vector<float> all(static_cast<size_t>(ctx->nv) * kMlpFeatures);
+        hipCheck(hipMemcpyAsync(all.data(), ctx->feat.p, sizeof(float) * all.size(), hipMemcpyDeviceToHost, ctx->stream),
+                 "download");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        const int64_t g0 = object < 0 ? 0 : ctx->objOffset[object];
+        const int64_t g1 = object < 0 ? ctx->nv : ctx->objOffset[object + 1];
+        for (int64_t gid = g0; gid < g1; ++gid)
+            std::memcpy(features + (gid - g0) * kMlpFeatures, all.data() + int64_t(ctx->slotOfGid[gid]) * kMlpFeatures,
+                        sizeof(float) * kMlpFeatures);
+    });
+}
+
+int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t gate, int32_t *accepted, double *rho)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !net) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->warmStart(net, gate != 0, accepted, rho);
     });
 }
 

@@ -26,6 +26,7 @@ ERR_STATE = 4
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)
 _lp = ctypes.POINTER(ctypes.c_int64)
+_fp = ctypes.POINTER(ctypes.c_float)
 
 
 class VbdEnergy(ctypes.Structure):
@@ -108,6 +109,20 @@ def vbd_lib():
                [i64, i32, ctypes.POINTER(_ip), _lp, _ip, _ip])
         _proto(L, "spg_vbd_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, i32, i32, i32, _dp, _dp])
         _proto(L, "spg_vbd_bench_copy", ctypes.c_int, [i64, i32, i32, _dp])
+        # learned warm start (include/spg_vbd_learned.h)
+        u64 = ctypes.c_uint64
+        _proto(L, "spg_mlp_create", ctypes.c_int, [ctypes.POINTER(c), i32, u64])
+        _proto(L, "spg_mlp_destroy", None, [c])
+        _proto(L, "spg_mlp_last_error", ctypes.c_char_p, [c])
+        _proto(L, "spg_mlp_get_params", ctypes.c_int, [c, _fp])
+        _proto(L, "spg_mlp_set_params", ctypes.c_int, [c, _fp])
+        _proto(L, "spg_mlp_get_normalization", ctypes.c_int, [c, _fp, _fp])
+        _proto(L, "spg_mlp_set_normalization", ctypes.c_int, [c, _fp, _fp])
+        _proto(L, "spg_mlp_forward", ctypes.c_int, [c, _fp, i64, _fp])
+        _proto(L, "spg_mlp_loss_grad", ctypes.c_int, [c, _fp, _fp, i64, _dp, _fp])
+        _proto(L, "spg_mlp_train", ctypes.c_int, [c, _fp, _fp, i64, i32, i32, ctypes.c_float, u64, i32, _dp])
+        _proto(L, "spg_vbd_features", ctypes.c_int, [c, i32, _fp])
+        _proto(L, "spg_vbd_warm_start", ctypes.c_int, [c, c, i32, _ip, _dp])
         _vbd = L
     return _vbd
 
@@ -155,6 +170,15 @@ def _d(a):
 
 def _i(a):
     return None if a is None else a.ctypes.data_as(_ip)
+
+
+def _fl(a):
+    return None if a is None else a.ctypes.data_as(_fp)
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a.reshape(shape) if shape is not None else a
 
 
 def _f64(a, shape=None):
@@ -428,6 +452,21 @@ class VbdContext:
                                                           ctypes.byref(it), ctypes.byref(rho)))
         return it.value, rho.value
 
+    def features(self, obj=-1):
+        """Learned-warm-start input rows of the open substep: (n, 24) float32 for object `obj`, or for
+        all objects concatenated (obj = -1)."""
+        n = sum(self._n) if obj < 0 else self._n[obj]
+        out = np.zeros((n, MLP_FEATURES), dtype=np.float32)
+        self._check(self._L.spg_vbd_features(self.h, obj, _fl(out)))
+        return out
+
+    def warm_start(self, net, gate=True):
+        """Start the open substep from the network's prediction; returns (accepted, rho) per object."""
+        acc = np.zeros(self.n_objects, dtype=np.int32)
+        rho = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(bool(gate)), _i(acc), _d(rho)))
+        return acc.astype(bool), rho
+
     def launch(self):
         c = VbdLaunch()
         self._check(self._L.spg_vbd_get_launch(self.h, ctypes.byref(c)))
@@ -499,3 +538,99 @@ def element_derivatives(kind, xs, params, precision=0, device=0, form=0):
     if rc != 0:
         raise NativeError(rc, L.spg_vbd_last_error(None).decode())
     return g, h
+
+
+# ------------------------------------------------------------------------------------------------
+# Learned warm start network (include/spg_vbd_learned.h)
+# ------------------------------------------------------------------------------------------------
+
+MLP_FEATURES, MLP_HIDDEN, MLP_OUTPUTS, MLP_NUM_PARAMS = 24, 128, 3, 36611
+MLP_SHAPES = [(128, 24), (128, 128), (128, 128), (3, 128)]  # (out, in) of each layer
+
+
+class Mlp:
+    """spg_mlp: the per-vertex network, trained and evaluated by the library's own GPU kernels."""
+
+    def __init__(self, device=0, seed=1):
+        self._L = vbd_lib()
+        h = ctypes.c_void_p()
+        rc = self._L.spg_mlp_create(ctypes.byref(h), device, seed)
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_mlp_last_error(None).decode())
+        self.h = h.value
+        self.device = device
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_mlp_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_mlp_last_error(self.h).decode())
+
+    def params(self):
+        out = np.zeros(MLP_NUM_PARAMS, dtype=np.float32)
+        self._check(self._L.spg_mlp_get_params(self.h, _fl(out)))
+        return out
+
+    def set_params(self, p):
+        p = _f32(p)
+        if p.size != MLP_NUM_PARAMS:
+            raise ValueError(f"expected {MLP_NUM_PARAMS} parameters")
+        self._check(self._L.spg_mlp_set_params(self.h, _fl(p)))
+
+    def normalization(self):
+        m = np.zeros(MLP_FEATURES, dtype=np.float32)
+        s = np.zeros(MLP_FEATURES, dtype=np.float32)
+        self._check(self._L.spg_mlp_get_normalization(self.h, _fl(m), _fl(s)))
+        return m, s
+
+    def set_normalization(self, mean, std):
+        m, s = _f32(mean, (MLP_FEATURES,)), _f32(std, (MLP_FEATURES,))
+        self._check(self._L.spg_mlp_set_normalization(self.h, _fl(m), _fl(s)))
+
+    def forward(self, features):
+        f = _f32(features, (-1, MLP_FEATURES))
+        out = np.zeros((f.shape[0], MLP_OUTPUTS), dtype=np.float32)
+        self._check(self._L.spg_mlp_forward(self.h, _fl(f), f.shape[0], _fl(out)))
+        return out
+
+    def loss_grad(self, features, targets):
+        f = _f32(features, (-1, MLP_FEATURES))
+        t = _f32(targets, (f.shape[0], MLP_OUTPUTS))
+        loss = ctypes.c_double(0.0)
+        g = np.zeros(MLP_NUM_PARAMS, dtype=np.float32)
+        self._check(self._L.spg_mlp_loss_grad(self.h, _fl(f), _fl(t), f.shape[0], ctypes.byref(loss), _fl(g)))
+        return loss.value, g
+
+    def train(self, features, targets, epochs=10, batch=4096, lr=1e-3, seed=1, standardize=True):
+        f = _f32(features, (-1, MLP_FEATURES))
+        t = _f32(targets, (f.shape[0], MLP_OUTPUTS))
+        losses = np.zeros(max(1, epochs))
+        self._check(self._L.spg_mlp_train(self.h, _fl(f), _fl(t), f.shape[0], epochs, batch, lr, seed,
+                                          int(bool(standardize)), _d(losses)))
+        return losses[:epochs]
+
+    def save(self, path):
+        m, s = self.normalization()
+        np.savez(path, params=self.params(), mean=m, std=s)
+
+    def load(self, path):
+        with np.load(path, allow_pickle=False) as z:
+            self.set_params(z["params"])
+            self.set_normalization(z["mean"], z["std"])
+
+
+def mlp_unflatten(p):
+    """Flat parameter vector -> [(W (out, in), b (out,)), ...] in layer order."""
+    p = np.asarray(p)
+    out, k = [], 0
+    for o, i in MLP_SHAPES:
+        w = p[k:k + o * i].reshape(o, i)
+        k += o * i
+        b = p[k:k + o]
+        k += o
+        out.append((w, b))
+    assert k == p.size
+    return out

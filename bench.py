@@ -16,7 +16,12 @@ cpu_baseline = the FP64 OpenMP oracle (restatement of spg's VBD::step), closed-f
 traffic    = HBM bytes per vertex-solve launch from the committed rocprofv3 counter passes of this
              workload and launch shape (profiles/), null when the shape differs
 
-Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu]
+learned    = time to a 1e-4 relative FP64 force residual, learned warm start vs plain VBD, on the same
+             cloth (network trained in this run on the GPU from 256 config-4-style cloths)
+batched    = 256-cloth batch (config 4) throughput, one launch per colour for all instances
+
+Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu] [--no-learned]
+                       [--no-batched]
 """
 import argparse
 import json
@@ -42,6 +47,9 @@ def main():
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start comparison")
+    ap.add_argument("--learned-steps", type=int, default=8, help="trajectory steps of the learned comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloth batched measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -114,6 +122,9 @@ def main():
     finite = bool(np.all(np.isfinite(x)))
     traffic, traffic_src = measured_traffic(ctx.launch(), args)
 
+    batched = None if args.no_batched else batched_throughput(args)
+    learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(s)
@@ -145,8 +156,87 @@ def main():
         "solver_stats": {"fp64_fallbacks": st["fp64_fallbacks"], "skipped_blocks": st["skipped_blocks"],
                          "state_finite": finite},
         "cpu_baseline": cpu,
+        "learned": learned_cmp,
+        "batched": batched,
     }
     print(json.dumps(out))
+
+
+def batched_throughput(args, count=256, steps=30):
+    """256 independent cloths of side U{16..128} advanced together: one launch per colour covers every
+    instance, launch shape from the autotuner, each step captured in a hipGraph. Throughput in
+    vertex-updates/s over `steps` steps (each from the initial state, as above)."""
+    import numpy as np
+    from spg_amd import native, scenes
+
+    t0 = time.perf_counter()
+    batch = scenes.cloth_batch(count=count, seed=7)
+    ctx = native.VbdContext([b.vbd_object() for b in batch], device=0)
+    setup = time.perf_counter() - t0
+    tuned = ctx.autotune(trials=2)
+    ctx.set_launch(use_graph=1)
+    V = sum(b.n for b in batch)
+    iters = batch[0].iterations
+    dt, g = batch[0].dt, batch[0].gravity
+    ctx.save_state()
+    for _ in range(3):
+        ctx.restore_state()
+        ctx.step(dt, 1, iters, g)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.restore_state()
+        ctx.step(dt, 1, iters, g)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.restore_state()
+    ctx.step(dt, 1, iters, g)
+    ctx.synchronize()
+    st = ctx.stats()
+    ctx.set_timing(False)
+    finite = all(bool(np.isfinite(ctx.state(o)[0]).all()) for o in range(len(batch)))
+    mean_ms = st["solve_ms"] / max(1, st["solve_launches"])
+    return {"workload": f"{count} cloths, side U{{16..128}} (seed 7), Y log-U[1e3,1e5], bending log-U[1e-5,1e-3], "
+                        f"top row anchored, {iters} iterations/step, one launch per colour for all instances, "
+                        f"hipGraph replay",
+            "vertices": V, "colors": ctx.num_colors(), "launch": tuned, "value": V * iters * steps / wall,
+            "unit": "vertex-updates/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
+            "solve_launch_us": mean_ms * 1e3,
+            "solve_achieved_GBs": st["algorithmic_bytes"] / max(1, st["solve_launches"]) / (mean_ms * 1e-3) / 1e9,
+            "setup_s": setup, "state_finite": finite}
+
+
+def learned_comparison(scene, args, launch):
+    """Learned warm start against plain VBD on the bench's own cloth. The network is trained here, on
+    the GPU, from the solver's own converged trajectories of 256 config-4-style cloths (2M vertex
+    samples); then for each trajectory step, from the same device checkpoint, plain VBD iterates from
+    x~ and the learned path from the gated prediction until the FP64 relative force residual is at
+    most 1e-4 (checked every iteration), timing both with the host clock around the whole substep."""
+    import numpy as np
+    from spg_amd import learned, scenes
+
+    t0 = time.perf_counter()
+    train_set = scenes.cloth_batch(count=256, seed=7)
+    f, t = learned.collect(train_set, steps=10, samples=2_000_000, rel_tol=3e-5, max_iterations=1000, check_every=10)
+    collect_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net, losses = learned.train(f, t, epochs=12, batch=4096, lr=2e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    rows = learned.compare(scene, net, steps=args.learned_steps, rel_tol=1e-4, max_iterations=3000, check_every=1,
+                           launch={k: launch[k] for k in ("lanes_per_vertex", "block_size", "lds_staging")})
+    sm = learned.summarize(rows)
+    return {"metric": "wall time to a 1e-4 relative FP64 force residual per substep, learned warm start vs plain VBD",
+            "workload": f"same cloth as above, {args.learned_steps} consecutive steps (trajectory continues from the "
+                        f"learned solve), residual checked every iteration",
+            "plain_ms": sm["plain_ms"], "learned_ms": sm["learned_ms"], "time_ratio": sm["time_ratio"],
+            "plain_iterations": sm["plain_iterations"], "learned_iterations": sm["learned_iterations"],
+            "iteration_ratio": sm["iteration_ratio"], "accepted": sm["accepted"], "all_converged": sm["all_converged"],
+            "per_step": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+            "training": {"samples": int(f.shape[0]), "scenes": len(train_set), "collect_s": collect_s,
+                         "train_s": train_s, "epochs": len(losses), "final_loss": float(losses[-1]),
+                         "target_variance": float(np.var(t, axis=0).sum())}}
 
 
 def measured_traffic(launch, args):

@@ -89,9 +89,11 @@ typedef struct {
 /* Launch shape of the per-colour vertex solve (chosen by spg_vbd_autotune or set by hand). */
 typedef struct {
     int32_t lanes_per_vertex;    /* 1, 2, 3, 4, 6, 8 or 16 lanes of a wavefront cooperate on one vertex */
-    int32_t block_size;          /* threads per workgroup: 64, 128, 256, 512 or 1024 */
+    int32_t block_size;          /* threads per workgroup: 64, 128 or 256 */
     int32_t use_graph;           /* 1: capture a whole step in a hipGraph and replay it */
-    int32_t lds_staging;         /* 1: stage the block's element records through LDS */
+    int32_t lds_staging;         /* 1: each workgroup first copies the element-record rows of its vertices
+                                    into LDS (coalesced), then reads them from there; available while
+                                    they fit 64 KB */
 } spg_vbd_launch;
 
 /* Counters and HIP-event timings kept by a context since creation (or the last reset_stats). */

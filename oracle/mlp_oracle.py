@@ -119,9 +119,12 @@ def static_tables(scene):
     sy, cy = np.zeros(n), np.zeros(n)
     sb, cb = np.zeros(n), np.zeros(n)
     anchored = np.zeros(n)
+    size = {"spring": 2, "stvk": 3, "bending": 4, "snh": 4, "anchor": 1}
     for kind, st, args in scene.energies:
-        st = np.asarray(st, dtype=np.int64).reshape(len(st), -1)
-        args = np.asarray(args, dtype=np.float64).reshape(len(st), -1)
+        st = np.asarray(st, dtype=np.int64).reshape(-1, size[kind])
+        if st.shape[0] == 0:
+            continue
+        args = np.asarray(args, dtype=np.float64).reshape(st.shape[0], -1)
         if kind in ("stvk", "snh", "spring"):
             logy = np.log10(args[:, 0] if kind != "spring" else args[:, 1])
             for q in range(st.shape[1]):

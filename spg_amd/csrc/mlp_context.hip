@@ -22,7 +22,6 @@ constexpr int kLayerIn[kMlpLayers] = {kMlpIn, kMlpHidden, kMlpHidden, kMlpHidden
 constexpr int kLayerOut[kMlpLayers] = {kMlpHidden, kMlpHidden, kMlpHidden, kMlpOutPad};
 constexpr int kFlatIn[kMlpLayers] = {kMlpFeatures, kMlpHidden, kMlpHidden, kMlpHidden};
 constexpr int kFlatOut[kMlpLayers] = {kMlpHidden, kMlpHidden, kMlpHidden, kMlpOut};
-constexpr int64_t kInferChunk = 262144;  // rows per inference pass (bounds the activation buffers)
 
 uint64_t splitmix64(uint64_t &s)
 {
@@ -47,11 +46,13 @@ struct spg_mlp {
     DevBuf<float> P, G, M, V;                                   // parameters, gradients, Adam moments
     DevBuf<__bf16> wb[kMlpLayers], wtb[kMlpLayers];              // BF16 W (out x in) and W^T (in x out)
     DevBuf<float> mean, invStd;
+    DevBuf<unsigned char> frag;  // weights in the fused inference kernel's operand order
+    int maxBlocks = 512;         // persistent workgroups of the fused inference kernel
     std::vector<float> hMean, hStd;
     int64_t adamT = 0;
 
     // workspaces
-    int64_t trainRows = 0, inferRows = 0;
+    int64_t trainRows = 0;
     DevBuf<__bf16> X, XT, H[2], HT[3], dY, dYT, dZb[2], dZT;
     DevBuf<float> T, Z[3], Yf, dYf, dZf;
     DevBuf<double> loss;
@@ -83,6 +84,10 @@ struct spg_mlp {
         }
         mean.alloc(kMlpFeatures);
         invStd.alloc(kMlpFeatures);
+        frag.alloc(mlpFragBytes());
+        int cus = 0;
+        hipCheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+        maxBlocks = 2 * std::max(cus, 1);  // two 80 KB-LDS workgroups fit a CU
         std::vector<float> flat(SPG_MLP_NUM_PARAMS, 0.f);
         uint64_t s = seed;
         int64_t k = 0;
@@ -152,6 +157,8 @@ struct spg_mlp {
         for (int l = 0; l < kMlpLayers; ++l)
             hipCheck(launchWeightCopies(P.p + wOff[l], kLayerOut[l], kLayerIn[l], wb[l].p, wtb[l].p, stream),
                      "weight copies");
+        hipCheck(launchMlpFrag(P.p + wOff[0], P.p + wOff[1], P.p + wOff[2], P.p + wOff[3], frag.p, stream),
+                 "fragment launch");
     }
 
     void reserveTrain(int64_t rows)
@@ -174,17 +181,6 @@ struct spg_mlp {
         dZf.alloc(rows * kMlpHidden);
         loss.alloc(1);
         trainRows = rows;
-        inferRows = rows;
-    }
-
-    void reserveInfer(int64_t rows)
-    {
-        if (rows <= inferRows) return;
-        X.alloc(rows * kMlpIn);
-        for (int i = 0; i < 2; ++i) H[i].alloc(rows * kMlpHidden);
-        Yf.alloc(rows * kMlpInferCols);
-        inferRows = rows;
-        trainRows = 0;  // training buffers are no longer consistent in size
     }
 
     void gemm(const __bf16 *A, int lda, const __bf16 *B, int ldb, int64_t Mr, int N, int K, const GemmEpilogue &ep)
@@ -261,30 +257,18 @@ struct spg_mlp {
         }
     }
 
+    // Fused inference on device features (n x 24) into y (stride ldy >= 3), ordered on the caller's
+    // stream `s` (runs on the network's stream between two events).
     void infer(const float *features, int64_t n, float *y, int ldy, hipStream_t s)
     {
-        // order after the caller's stream, run on ours, then make the caller's stream wait for us
+        if (n > (int64_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many rows");
         hipEvent_t ev;
         hipCheck(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
         hipCheck(hipEventRecord(ev, s), "hipEventRecord");
         hipCheck(hipStreamWaitEvent(stream, ev, 0), "hipStreamWaitEvent");
-        const int64_t chunk = std::min<int64_t>(roundUp(n, 64), kInferChunk);
-        reserveInfer(chunk);
-        for (int64_t r0 = 0; r0 < n; r0 += chunk) {
-            const int64_t cnt = std::min<int64_t>(chunk, n - r0);
-            const int64_t rows = roundUp(cnt, 64);
-            hipCheck(launchPackInput(features + r0 * kMlpFeatures, nullptr, nullptr, mean.p, invStd.p,
-                                     static_cast<int>(cnt), static_cast<int>(rows), X.p, nullptr, nullptr, stream),
-                     "pack launch");
-            if (cnt == rows) {
-                forward(rows, false, y + r0 * ldy, ldy);
-            } else {  // partial tail: compute into the workspace, copy the real rows
-                forward(rows, false, Yf.p, kMlpInferCols);
-                hipCheck(hipMemcpy2DAsync(y + r0 * ldy, sizeof(float) * ldy, Yf.p, sizeof(float) * kMlpInferCols,
-                                          sizeof(float) * kMlpInferCols, cnt, hipMemcpyDeviceToDevice, stream),
-                         "copy");
-            }
-        }
+        hipCheck(launchMlpInfer(features, static_cast<int>(n), mean.p, invStd.p, frag.p, P.p + bOff[0], P.p + bOff[1],
+                                P.p + bOff[2], P.p + bOff[3], y, ldy, maxBlocks, stream),
+                 "inference launch");
         hipCheck(hipEventRecord(ev, stream), "hipEventRecord");
         hipCheck(hipStreamWaitEvent(s, ev, 0), "hipStreamWaitEvent");
         hipCheck(hipEventDestroy(ev), "hipEventDestroy");
@@ -295,7 +279,7 @@ namespace spgvbd {
 
 void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipStream_t stream)
 {
-    if (ldy < kMlpInferCols) throw VbdError(SPG_VBD_ERR_INVALID, "output stride below 16");
+    if (ldy < kMlpOut) throw VbdError(SPG_VBD_ERR_INVALID, "output stride below 3");
     hipCheck(hipSetDevice(net->device), "hipSetDevice");
     net->infer(features, n, y, ldy, stream);
 }
@@ -412,13 +396,11 @@ int spg_mlp_forward(spg_mlp *net, const float *features, int64_t n, float *out)
         requireFinite(features, n * kMlpFeatures, "features");
         DevBuf<float> f, y;
         f.alloc(n * kMlpFeatures);
-        y.alloc(n * kMlpInferCols);
+        y.alloc(n * kMlpOut);
         hipCheck(hipMemcpyAsync(f.p, features, sizeof(float) * n * kMlpFeatures, hipMemcpyHostToDevice, net->stream),
                  "upload");
-        net->infer(f.p, n, y.p, kMlpInferCols, net->stream);
-        hipCheck(hipMemcpy2DAsync(out, sizeof(float) * kMlpOut, y.p, sizeof(float) * kMlpInferCols,
-                                  sizeof(float) * kMlpOut, n, hipMemcpyDeviceToHost, net->stream),
-                 "download");
+        net->infer(f.p, n, y.p, kMlpOut, net->stream);
+        hipCheck(hipMemcpyAsync(out, y.p, sizeof(float) * n * kMlpOut, hipMemcpyDeviceToHost, net->stream), "download");
         hipCheck(hipStreamSynchronize(net->stream), "hipStreamSynchronize");
     });
 }

@@ -45,6 +45,17 @@ hipError_t launchWeightCopies(const float *w, int rows, int cols, __bf16 *wb, __
 hipError_t launchPackInput(const float *f, const float *t, const int32_t *index, const float *mean, const float *invStd,
                            int n, int rows, __bf16 *X, __bf16 *XT, float *tOut, hipStream_t stream);
 
+// Fused inference (all layers in registers, weights staged in LDS): fragments of the weights in the
+// kernel's operand order are rebuilt from the FP32 master weights by launchMlpFrag after every change.
+size_t mlpFragBytes();
+hipError_t launchMlpFrag(const float *w1, const float *w2, const float *w3, const float *w4, void *frag,
+                         hipStream_t stream);
+// y[row][0..2] = network(f[row]) for rows < n (f: n x 24 FP32, y stride ldy >= 3); at most maxBlocks
+// persistent workgroups of 256 threads.
+hipError_t launchMlpInfer(const float *f, int n, const float *mean, const float *invStd, const void *frag,
+                          const float *b1, const float *b2, const float *b3, const float *b4, float *y, int ldy,
+                          int maxBlocks, hipStream_t stream);
+
 // Inputs of the feature kernel (warmstart_kernels.hip), slot numbering of a VBD context.
 struct FeatureArgs {
     const float4 *xt = nullptr;    // inertial target (w: mass)
@@ -67,10 +78,11 @@ hipError_t launchGate(float4 *pos, const float4 *xt, const int32_t *objOfSlot, c
                       hipStream_t stream);
 hipError_t launchCorrection(const float4 *pos, const float4 *xt, float4 *corr, int32_t n, hipStream_t stream);
 
-// Network inference on device-resident features (n x kMlpFeatures FP32) into y (n rows, stride
-// ldy >= kMlpInferCols; the first kMlpOut columns are the outputs), ordered on `stream`. Throws
-// VbdError. Defined with the network object.
+// Columns of the training forward pass's output buffer (the last layer is computed 16 wide).
 constexpr int kMlpInferCols = 16;
+// Network inference on device-resident features (n x kMlpFeatures FP32) into y (n rows, stride
+// ldy >= kMlpOut), ordered on `stream`, with the fused kernel. Throws VbdError. Defined with the
+// network object.
 void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipStream_t stream);
 int mlpDevice(const spg_mlp *net);
 

@@ -9,6 +9,7 @@
 // plus the feature kernel, the loss gradient, bias gradients, Adam, and the fused inference kernel.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "mlp_device.h"
@@ -184,6 +185,165 @@ __global__ void __launch_bounds__(256) packInputKernel(const float *__restrict__
         src >= 0 ? t[static_cast<int64_t>(src) * kMlpOut + col] : 0.f;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Fused inference: all four layers in registers, one wavefront per kNT x 16 rows.
+//
+// Each layer is computed transposed, H^T = W X^T: the weights are the MFMA A operand (M = output
+// features) and the rows are the B operand (N = 16 rows per tile), so the accumulator of tile i holds,
+// in lane l, output features 16 i + 4 (l / 16) + r (r = 0..3) of row l % 16. The B fragment a layer
+// needs for K step kk is, in lane l, 8 features of row l % 16; the lane already holds features
+// 32 kk + 4 q + r (tile 2 kk) and 32 kk + 16 + 4 q + r (tile 2 kk + 1), q = l / 16. Using exactly those 8
+// (a fixed permutation of K inside the step) needs no data exchange between lanes; the weight
+// fragments are stored with the same permutation (mlpFragKernel), so the products pair up. The input
+// layer uses the plain order: lane l holds features 8 q .. 8 q + 7.
+// Weight fragments (76 KB) and biases are staged in LDS once per persistent workgroup.
+constexpr int kFragL1 = 0, kFragL2 = 512, kFragL3 = 2560, kFragL4 = 4608, kFragTotal = 4864;
+constexpr int kNT = 4;  // 16-row tiles per wavefront task (each weight fragment read from LDS feeds kNT MFMAs)
+constexpr int kTaskRows = 16 * kNT;
+constexpr int kBiasTotal = 3 * kMlpHidden + kMlpOutPad;
+
+__device__ __forceinline__ int permutedK(int kk, int q, int p) { return 32 * kk + (p < 4 ? 4 * q + p : 16 + 4 * q + p - 4); }
+
+// frag[f][p] for every fragment of the fused kernel, from the padded FP32 master weights.
+__global__ void __launch_bounds__(256) mlpFragKernel(const float *__restrict__ w1, const float *__restrict__ w2,
+                                                     const float *__restrict__ w3, const float *__restrict__ w4,
+                                                     bf16x8 *__restrict__ frag)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= kFragTotal) return;
+    const int lane = f & 63, q = lane >> 4, r16 = lane & 15;
+    const int t = f >> 6;  // (tile, kk) index within its layer
+    bf16x8 v;
+    if (f < kFragL2) {  // layer 1: 8 tiles x 1 step, plain K order, 32 padded inputs
+        const int i = t;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) v[p] = static_cast<__bf16>(w1[(16 * i + r16) * kMlpIn + 8 * q + p]);
+    } else if (f < kFragL4) {  // layers 2, 3: 8 tiles x 4 steps
+        const float *w = f < kFragL3 ? w2 : w3;
+        const int tt = f < kFragL3 ? t - kFragL2 / 64 : t - kFragL3 / 64;
+        const int i = tt >> 2, kk = tt & 3;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) v[p] = static_cast<__bf16>(w[(16 * i + r16) * kMlpHidden + permutedK(kk, q, p)]);
+    } else {  // layer 4: 1 tile (outputs 0..15 of the padded 32) x 4 steps
+        const int kk = t - kFragL4 / 64;
+#pragma unroll
+        for (int p = 0; p < 8; ++p) v[p] = static_cast<__bf16>(w4[r16 * kMlpHidden + permutedK(kk, q, p)]);
+    }
+    frag[f] = v;
+}
+
+__device__ __forceinline__ void hiddenEpilogue(const f32x4 (&acc)[8][kNT], const float *bias, int q,
+                                               bf16x8 (&hb)[4][kNT])
+{
+#pragma unroll
+    for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                hb[kk][nt][r] = static_cast<__bf16>(silu(acc[2 * kk][nt][r] + bias[32 * kk + 4 * q + r]));
+                hb[kk][nt][4 + r] = static_cast<__bf16>(silu(acc[2 * kk + 1][nt][r] + bias[32 * kk + 16 + 4 * q + r]));
+            }
+}
+
+__device__ __forceinline__ void hiddenLayer(const bf16x8 *wl, const bf16x8 (&hb)[4][kNT], int lane,
+                                            f32x4 (&acc)[8][kNT])
+{
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) acc[i][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const bf16x8 a = wl[(i * 4 + kk) * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < kNT; ++nt) acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[kk][nt], acc[i][nt], 0, 0, 0);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) mlpInferKernel(const float *__restrict__ f, int n, const float *__restrict__ mean,
+                                                      const float *__restrict__ invStd, const bf16x8 *__restrict__ frag,
+                                                      const float *__restrict__ b1, const float *__restrict__ b2,
+                                                      const float *__restrict__ b3, const float *__restrict__ b4,
+                                                      float *__restrict__ y, int ldy)
+{
+    __shared__ bf16x8 wl[kFragTotal];
+    __shared__ float bl[kBiasTotal];
+    __shared__ float nl[2 * kMlpIn];
+    for (int i = threadIdx.x; i < kFragTotal; i += blockDim.x) wl[i] = frag[i];
+    for (int i = threadIdx.x; i < kMlpHidden; i += blockDim.x) {
+        bl[i] = b1[i];
+        bl[kMlpHidden + i] = b2[i];
+        bl[2 * kMlpHidden + i] = b3[i];
+    }
+    if (threadIdx.x < kMlpOutPad) bl[3 * kMlpHidden + threadIdx.x] = b4[threadIdx.x];
+    if (threadIdx.x < kMlpIn) {
+        const bool real = threadIdx.x < kMlpFeatures;
+        nl[threadIdx.x] = real ? mean[threadIdx.x] : 0.f;
+        nl[kMlpIn + threadIdx.x] = real ? invStd[threadIdx.x] : 0.f;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15;
+    const int ntask = (n + kTaskRows - 1) / kTaskRows;
+    const int waves = gridDim.x * (blockDim.x >> 6);
+    for (int task = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); task < ntask; task += waves) {
+        // compiler-level memory barrier: keeps the weight fragments in LDS (re-read per task) instead of
+        // hoisting all 76 of them out of the loop into registers
+        asm volatile("" ::: "memory");
+        const int row0 = task * kTaskRows;
+        bf16x8 hb[4][kNT];
+        // standardised inputs, plain K order; features 24..31 and rows past n are zero
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) {
+            const int row = row0 + 16 * nt + r16;
+            bf16x8 x = zeroFrag();
+            if (row < n && q < 3) {
+                const float4 *src = reinterpret_cast<const float4 *>(f + static_cast<int64_t>(row) * kMlpFeatures + 8 * q);
+                const float4 u0 = src[0], u1 = src[1];
+                const float u[8] = {u0.x, u0.y, u0.z, u0.w, u1.x, u1.y, u1.z, u1.w};
+#pragma unroll
+                for (int p = 0; p < 8; ++p) x[p] = static_cast<__bf16>((u[p] - nl[8 * q + p]) * nl[kMlpIn + 8 * q + p]);
+            }
+            hb[0][nt] = x;
+            hb[1][nt] = zeroFrag();
+            hb[2][nt] = zeroFrag();
+            hb[3][nt] = zeroFrag();
+        }
+        f32x4 acc[8][kNT];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bf16x8 a = wl[kFragL1 + i * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < kNT; ++nt)
+                acc[i][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[0][nt], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+        hiddenEpilogue(acc, bl, q, hb);
+        hiddenLayer(wl + kFragL2, hb, lane, acc);
+        hiddenEpilogue(acc, bl + kMlpHidden, q, hb);
+        hiddenLayer(wl + kFragL3, hb, lane, acc);
+        hiddenEpilogue(acc, bl + 2 * kMlpHidden, q, hb);
+        f32x4 o[kNT];
+#pragma unroll
+        for (int nt = 0; nt < kNT; ++nt) o[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const bf16x8 a = wl[kFragL4 + kk * 64 + lane];
+#pragma unroll
+            for (int nt = 0; nt < kNT; ++nt) o[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, hb[kk][nt], o[nt], 0, 0, 0);
+        }
+        if (q == 0) {
+#pragma unroll
+            for (int nt = 0; nt < kNT; ++nt) {
+                const int row = row0 + 16 * nt + r16;
+                if (row < n)
+#pragma unroll
+                    for (int r = 0; r < kMlpOut; ++r) y[static_cast<int64_t>(row) * ldy + r] = o[nt][r] + bl[3 * kMlpHidden + r];
+            }
+        }
+    }
+}
+
 inline int blocksFor(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 }  // namespace
@@ -230,6 +390,32 @@ hipError_t launchPackInput(const float *f, const float *t, const int32_t *index,
 {
     packInputKernel<<<blocksFor(static_cast<int64_t>(rows) * kMlpIn, 256), 256, 0, stream>>>(f, t, index, mean, invStd,
                                                                                            n, rows, X, XT, tOut);
+    return hipGetLastError();
+}
+
+}  // namespace spgvbd
+
+namespace spgvbd {
+
+hipError_t launchMlpFrag(const float *w1, const float *w2, const float *w3, const float *w4, void *frag,
+                         hipStream_t stream)
+{
+    mlpFragKernel<<<blocksFor(kFragTotal, 256), 256, 0, stream>>>(w1, w2, w3, w4, static_cast<bf16x8 *>(frag));
+    return hipGetLastError();
+}
+
+size_t mlpFragBytes() { return sizeof(bf16x8) * kFragTotal; }
+
+hipError_t launchMlpInfer(const float *f, int n, const float *mean, const float *invStd, const void *frag,
+                          const float *b1, const float *b2, const float *b3, const float *b4, float *y, int ldy,
+                          int maxBlocks, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if (ldy < kMlpOut) return hipErrorInvalidValue;
+    const int tasks = (n + kTaskRows - 1) / kTaskRows;
+    const int grid = std::max(1, std::min((tasks + 3) / 4, maxBlocks));
+    mlpInferKernel<<<grid, 256, 0, stream>>>(f, n, mean, invStd, static_cast<const bf16x8 *>(frag), b1, b2, b3, b4, y,
+                                             ldy);
     return hipGetLastError();
 }
 

@@ -99,6 +99,14 @@ struct spg_vbd_ctx {
     DevBuf<double> r2, norms;
     DevBuf<int32_t> objSlots;
     DevBuf<int64_t> objOffsets;
+    // segments of the per-object residual reduction: objSlots[segBegin[k] .. segBegin[k + 1]) belongs to
+    // one object; object o owns segments objSeg[o] .. objSeg[o + 1]
+    DevBuf<int64_t> segBegin;
+    DevBuf<int32_t> objSeg;
+    DevBuf<double> segPartial;
+    int32_t nSegments = 0;
+    std::vector<double> r0Cache;  // |r(x~)| per object of the open substep
+    bool r0Valid = false;
 
     // incidence streams
     // row offsets (CSR, used by the residual; anchors also by the solve)
@@ -253,6 +261,7 @@ struct spg_vbd_ctx {
     void beginSubstep(double hh, const double *g)
     {
         h = hh;
+        r0Valid = false;
         for (int k = 0; k < 3; ++k) this->g[k] = g[k];
         hipCheck(launchBeginSubstep(pos.p, xold.p, vel.p, xt.p, mass.p, static_cast<int32_t>(nv), static_cast<float>(hh),
                                     static_cast<float>(g[0]), static_cast<float>(g[1]), static_cast<float>(g[2]),
@@ -272,6 +281,10 @@ struct spg_vbd_ctx {
     void residual(bool atInertial, double *out)
     {
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
+        if (atInertial && r0Valid) {  // x~ is fixed for the whole substep
+            std::copy(r0Cache.begin(), r0Cache.end(), out);
+            return;
+        }
         ResidualArgs a{};
         a.pos = atInertial ? xt.p : pos.p;
         a.xt = xt.p;
@@ -301,9 +314,15 @@ struct spg_vbd_ctx {
         a.tetP = tetP.p;
         a.r2 = r2.p;
         hipCheck(launchResidual(a, stream), "residual launch");
-        hipCheck(launchReduceObjects(r2.p, objSlots.p, objOffsets.p, nObjects, norms.p, stream), "reduce launch");
+        hipCheck(launchReduceObjects(r2.p, objSlots.p, segBegin.p, nSegments, objSeg.p, nObjects, segPartial.p, norms.p,
+                                     stream),
+                 "reduce launch");
         hipCheck(hipMemcpyAsync(out, norms.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        if (atInertial) {
+            r0Cache.assign(out, out + nObjects);
+            r0Valid = true;
+        }
     }
 
     // acceleration scale of the learned features: |g|, or 1 without gravity
@@ -341,10 +360,10 @@ struct spg_vbd_ctx {
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
         if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
         computeFeatures();
-        pred.reserve(static_cast<size_t>(nv) * kMlpInferCols);
-        mlpInferDevice(net, feat.p, nv, pred.p, kMlpInferCols, stream);
+        pred.reserve(static_cast<size_t>(nv) * 4);
+        mlpInferDevice(net, feat.p, nv, pred.p, 4, stream);
         hipCheck(hipSetDevice(device), "hipSetDevice");
-        hipCheck(launchApplyWarmStart(pos.p, xt.p, pred.p, kMlpInferCols, static_cast<float>(h * h * refAccel()),
+        hipCheck(launchApplyWarmStart(pos.p, xt.p, pred.p, 4, static_cast<float>(h * h * refAccel()),
                                       static_cast<int32_t>(nv), stream),
                  "warm start launch");
         std::vector<int32_t> rej(nObjects, 0);
@@ -555,6 +574,17 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         }
         c.objSlots.upload(objSlots, c.stream);
         c.objOffsets.upload(c.objSlotsOffsets, c.stream);
+        std::vector<int64_t> seg{0};
+        std::vector<int32_t> objSeg{0};
+        for (int32_t o = 0; o < nObjects; ++o) {
+            for (int64_t b = c.objSlotsOffsets[o]; b < c.objSlotsOffsets[o + 1]; b += kReduceSegment)
+                seg.push_back(std::min<int64_t>(b + kReduceSegment, c.objSlotsOffsets[o + 1]));
+            objSeg.push_back(static_cast<int32_t>(seg.size() - 1));
+        }
+        c.nSegments = static_cast<int32_t>(seg.size() - 1);
+        c.segBegin.upload(seg, c.stream);
+        c.objSeg.upload(objSeg, c.stream);
+        c.segPartial.alloc(c.nSegments);
     }
 
     // per-vertex state
@@ -1239,6 +1269,10 @@ int spg_vbd_set_launch(spg_vbd_ctx *ctx, const spg_vbd_launch *cfg)
             throw VbdError(SPG_VBD_ERR_INVALID, "lanes_per_vertex must be 1, 2, 3, 4, 6, 8 or 16");
         if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
             throw VbdError(SPG_VBD_ERR_INVALID, "block_size must be 64, 128 or 256");
+        if (cfg->lds_staging != 0 && cfg->lds_staging != 1)
+            throw VbdError(SPG_VBD_ERR_INVALID, "lds_staging must be 0 or 1");
+        if (cfg->lds_staging && solveStagedLdsBytes(ctx->solveArgs(0, 1.f), g, cfg->block_size) > kSolveMaxLds)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "the element-record rows of one workgroup exceed 64 KB of LDS");
         ctx->launch = *cfg;
         ctx->dropGraph();
     });
@@ -1267,11 +1301,14 @@ int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best)
         spg_vbd_launch bestCfg = keep;
         const int lanes[] = {1, 2, 3, 4, 6, 8, 16};
         const int blocks[] = {64, 128, 256};
+        for (int staged = 0; staged < 2; ++staged)
         for (int l : lanes)
             for (int b : blocks) {
+                if (staged && solveStagedLdsBytes(ctx->solveArgs(0, 1.f), l, b) > kSolveMaxLds) continue;
                 ctx->launch = keep;
                 ctx->launch.lanes_per_vertex = l;
                 ctx->launch.block_size = b;
+                ctx->launch.lds_staging = staged;
                 ctx->sweepColors(1, false);  // warm
                 hipCheck(hipEventRecord(e0, ctx->stream), "hipEventRecord");
                 ctx->sweepColors(trials, false);

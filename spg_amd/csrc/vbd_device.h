@@ -95,12 +95,18 @@ hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt
                               float h, float gx, float gy, float gz, hipStream_t stream);
 hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
                             hipStream_t stream);
+// LDS staging of the element-record rows is available while one workgroup's rows fit this many bytes
+constexpr size_t kSolveMaxLds = 65536;
+size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize);
 // start/stop: optional events stamped with the dispatch's own begin/end (nullptr: untimed launch)
 hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
                             hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream);
-hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *objOffsets, int32_t nObjects,
-                               double *out, hipStream_t stream);
+// entries of one object's slot list summed by one block of the per-object residual reduction
+constexpr int64_t kReduceSegment = 4096;
+hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *segBegin, int32_t nSegments,
+                               const int32_t *objSeg, int32_t nObjects, double *partial, double *out,
+                               hipStream_t stream);
 hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t stream);
 hipError_t launchCopy(const float4 *src, float4 *dst, int64_t n, hipStream_t stream);
 hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,

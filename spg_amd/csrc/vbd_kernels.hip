@@ -146,16 +146,61 @@ __device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t li
     return v;
 }
 
-template <int G>
+// Copies count records starting at src[first] into LDS at *cursor (16-byte aligned), with the whole
+// workgroup, and advances the cursor. T is 8 or 16 bytes.
+template <class T>
+__device__ __forceinline__ const T *stageRows(const T *src, int64_t first, int32_t count, unsigned char *&cursor)
+{
+    T *dst = reinterpret_cast<T *>(cursor);
+    for (int32_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[first + i];
+    cursor += (static_cast<size_t>(count) * sizeof(T) + 15) & ~size_t(15);
+    return dst;
+}
+
+// LDS bytes the staged variant needs for one workgroup (host side mirrors the kernel's layout).
+__host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int blockSize)
+{
+    const int64_t nvb = static_cast<int64_t>(blockSize / 64) * (64 / G);
+    auto seg = [&](int32_t w, size_t bytes) { return (static_cast<size_t>(nvb * w) * bytes + 15) & ~size_t(15); };
+    return seg(a.triW, 8) + 2 * seg(a.triW, 16) + seg(a.hinEW, 16) + seg(a.hinEW, 8) + seg(a.hinWW, 16) +
+           seg(a.hinWW, 8) + 4 * seg(a.tetW, 16) + seg(a.sprW, 16);
+}
+
+// LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
+// contiguous span per record array) into LDS with coalesced loads, then every lane reads its records
+// from LDS; positions are gathered from global memory as in the direct variant.
+template <int G, bool LDS>
 __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
 {
     constexpr int GPW = 64 / G;  // groups per wavefront
+    extern __shared__ float4 ldsRows[];
     const int32_t lane64 = threadIdx.x & 63;
-    const int32_t wave = xcdContiguousBlock(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int32_t blk = xcdContiguousBlock(blockIdx.x, gridDim.x);
+    const int32_t wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int32_t grp = lane64 / G;
     const int32_t lig = lane64 - grp * G;
     const int32_t vid = a.begin + wave * GPW + grp;
     const bool active = grp < GPW && vid < a.end;
+    int64_t rowBase = 0;  // record row of vertex vid is vid - rowBase
+    if constexpr (LDS) {
+        const int32_t v0 = a.begin + blk * (blockDim.x >> 6) * GPW;
+        const int32_t nvb = max(0, min(static_cast<int32_t>(blockDim.x >> 6) * GPW, a.end - v0));
+        unsigned char *cur = reinterpret_cast<unsigned char *>(ldsRows);
+        a.triO = stageRows(a.triO, int64_t(v0) * a.triW, nvb * a.triW, cur);
+        a.triG = stageRows(a.triG, int64_t(v0) * a.triW, nvb * a.triW, cur);
+        a.triM = stageRows(a.triM, int64_t(v0) * a.triW, nvb * a.triW, cur);
+        a.hinEV = stageRows(a.hinEV, int64_t(v0) * a.hinEW, nvb * a.hinEW, cur);
+        a.hinEP = stageRows(a.hinEP, int64_t(v0) * a.hinEW, nvb * a.hinEW, cur);
+        a.hinWV = stageRows(a.hinWV, int64_t(v0) * a.hinWW, nvb * a.hinWW, cur);
+        a.hinWP = stageRows(a.hinWP, int64_t(v0) * a.hinWW, nvb * a.hinWW, cur);
+        a.tetO = stageRows(a.tetO, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
+        a.tetGa = stageRows(a.tetGa, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
+        a.tetGb = stageRows(a.tetGb, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
+        a.tetGc = stageRows(a.tetGc, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
+        a.spr = stageRows(a.spr, int64_t(v0) * a.sprW, nvb * a.sprW, cur);
+        rowBase = v0;
+        __syncthreads();
+    }
     V3f g = {0.f, 0.f, 0.f};
     S3f H = zeroSym<float>();
     V3f xo = {0.f, 0.f, 0.f};
@@ -169,7 +214,7 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             ae = a.ancPtr[vid + 1];
         }
         xo = xyz(xo4);
-        const int64_t row = vid;
+        const int64_t row = vid - rowBase;
         for (int32_t j = lig; j < a.triW; j += G) {
             const int64_t r = row * a.triW + j;
             const int2 o = a.triO[r];
@@ -345,22 +390,36 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
 }
 
 // Sum of r2 over each object's slots, in a fixed order (one block per object).
-__global__ void __launch_bounds__(256) reduceObjectsKernel(const double *__restrict__ r2,
-                                                           const int32_t *__restrict__ objSlots,
-                                                           const int64_t *__restrict__ objOffsets,
-                                                           double *__restrict__ out)
+// Per-object |r| in two deterministic stages. Stage 1: block b sums the squared residuals of segment b
+// (at most kReduceSegment entries of one object's slot list) in a fixed tree order. Stage 2: one
+// thread per object sums its segments' partials in segment order and takes the square root.
+__global__ void __launch_bounds__(256) reduceSegmentsKernel(const double *__restrict__ r2,
+                                                            const int32_t *__restrict__ objSlots,
+                                                            const int64_t *__restrict__ segBegin,
+                                                            double *__restrict__ partial)
 {
     __shared__ double part[256];
-    const int32_t obj = blockIdx.x;
+    const int64_t b = segBegin[blockIdx.x], e = segBegin[blockIdx.x + 1];
     double s = 0.0;
-    for (int64_t j = objOffsets[obj] + threadIdx.x; j < objOffsets[obj + 1]; j += blockDim.x) s += r2[objSlots[j]];
+    for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) s += r2[objSlots[j]];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[obj] = sqrt(part[0]);
+    if (threadIdx.x == 0) partial[blockIdx.x] = part[0];
+}
+
+__global__ void __launch_bounds__(256) reduceObjectsKernel(const double *__restrict__ partial,
+                                                           const int32_t *__restrict__ objSeg, int32_t nObjects,
+                                                           double *__restrict__ out)
+{
+    const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nObjects) return;
+    double s = 0.0;
+    for (int32_t k = objSeg[o]; k < objSeg[o + 1]; ++k) s += partial[k];
+    out[o] = sqrt(s);
 }
 
 // Per-element derivative hook: evaluates the same device functions the solve uses, for every
@@ -417,36 +476,54 @@ hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t st
     return hipGetLastError();
 }
 
-template <int G>
-void solveColorLaunch(const SolveArgs &a, int grid, int blockSize, hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+template <int G, bool LDS>
+void solveColorLaunch(const SolveArgs &a, int grid, int blockSize, size_t lds, hipStream_t stream, hipEvent_t start,
+                      hipEvent_t stop)
 {
     if (start)  // timestamps of the dispatch itself (no extra packets between the events and the kernel)
-        hipExtLaunchKernelGGL(solveColorKernel<G>, dim3(grid), dim3(blockSize), 0, stream, start, stop, 0, a);
+        hipExtLaunchKernelGGL(solveColorKernel<G, LDS>, dim3(grid), dim3(blockSize), lds, stream, start, stop, 0, a);
     else
-        solveColorKernel<G><<<grid, blockSize, 0, stream>>>(a);
+        solveColorKernel<G, LDS><<<grid, blockSize, lds, stream>>>(a);
 }
 
-hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int /*ldsStaging*/,
+template <int G>
+void solveColorLaunchG(const SolveArgs &a, int grid, int blockSize, bool staged, hipStream_t stream, hipEvent_t start,
+                       hipEvent_t stop)
+{
+    if (staged)
+        solveColorLaunch<G, true>(a, grid, blockSize, solveLdsBytes(a, G, blockSize), stream, start, stop);
+    else
+        solveColorLaunch<G, false>(a, grid, blockSize, 0, stream, start, stop);
+}
+
+hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
                             hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
     const int64_t nv = a.end - a.begin;
     if (nv <= 0) return hipSuccess;
     if (blockSize > 256 || blockSize < 64 || blockSize % 64) return hipErrorInvalidValue;
     if (lanesPerVertex < 1 || lanesPerVertex > 16) return hipErrorInvalidValue;
+    if (ldsStaging && solveLdsBytes(a, lanesPerVertex, blockSize) > kSolveMaxLds) return hipErrorInvalidValue;
     // wavefronts of 64 / G vertex groups, blockSize / 64 wavefronts per block
     const int64_t waves = (nv + 64 / lanesPerVertex - 1) / (64 / lanesPerVertex);
     const int grid = blocksFor(waves, blockSize / 64);
+    const bool st = ldsStaging != 0;
     switch (lanesPerVertex) {
-    case 1: solveColorLaunch<1>(a, grid, blockSize, stream, start, stop); break;
-    case 2: solveColorLaunch<2>(a, grid, blockSize, stream, start, stop); break;
-    case 3: solveColorLaunch<3>(a, grid, blockSize, stream, start, stop); break;
-    case 4: solveColorLaunch<4>(a, grid, blockSize, stream, start, stop); break;
-    case 6: solveColorLaunch<6>(a, grid, blockSize, stream, start, stop); break;
-    case 8: solveColorLaunch<8>(a, grid, blockSize, stream, start, stop); break;
-    case 16: solveColorLaunch<16>(a, grid, blockSize, stream, start, stop); break;
+    case 1: solveColorLaunchG<1>(a, grid, blockSize, st, stream, start, stop); break;
+    case 2: solveColorLaunchG<2>(a, grid, blockSize, st, stream, start, stop); break;
+    case 3: solveColorLaunchG<3>(a, grid, blockSize, st, stream, start, stop); break;
+    case 4: solveColorLaunchG<4>(a, grid, blockSize, st, stream, start, stop); break;
+    case 6: solveColorLaunchG<6>(a, grid, blockSize, st, stream, start, stop); break;
+    case 8: solveColorLaunchG<8>(a, grid, blockSize, st, stream, start, stop); break;
+    case 16: solveColorLaunchG<16>(a, grid, blockSize, st, stream, start, stop); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize)
+{
+    return solveLdsBytes(a, lanesPerVertex, blockSize);
 }
 
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
@@ -456,11 +533,13 @@ hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *objOffsets, int32_t nObjects,
-                               double *out, hipStream_t stream)
+hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *segBegin, int32_t nSegments,
+                               const int32_t *objSeg, int32_t nObjects, double *partial, double *out,
+                               hipStream_t stream)
 {
-    if (nObjects <= 0) return hipSuccess;
-    reduceObjectsKernel<<<nObjects, 256, 0, stream>>>(r2, objSlots, objOffsets, out);
+    if (nObjects <= 0 || nSegments <= 0) return hipSuccess;
+    reduceSegmentsKernel<<<nSegments, 256, 0, stream>>>(r2, objSlots, segBegin, partial);
+    reduceObjectsKernel<<<blocksFor(nObjects, 256), 256, 0, stream>>>(partial, objSeg, nObjects, out);
     return hipGetLastError();
 }
 

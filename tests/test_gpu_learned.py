@@ -1,0 +1,209 @@
+"""GPU tests of the learned warm start, through the C-ABI of include/spg_vbd_learned.h:
+  * network forward pass and loss gradient (BF16 MFMA kernels, FP32 accumulation) against the FP64
+    oracle of oracle/mlp_oracle.py, both with the oracle rounding to bfloat16 where the kernels do
+    (tight) and in exact FP64 (loose: the bfloat16 rounding itself);
+  * parameters round-trip exactly; training reduces the loss of a smooth regression;
+  * the solver's feature rows against the oracle's restatement from the downloaded states;
+  * the gate: a prediction with a larger residual than the inertial guess is discarded exactly;
+  * what the trained network gains: fewer VBD iterations to a 1e-4 relative residual than plain VBD.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from spg_amd import learned, native, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _net_with_random_inputs(seed, n):
+    rng = np.random.default_rng(seed)
+    net = native.Mlp(seed=seed)
+    mean = rng.normal(size=24).astype(np.float32) * 0.2
+    std = rng.uniform(0.5, 2.0, size=24).astype(np.float32)
+    net.set_normalization(mean, std)
+    f = (rng.normal(size=(n, 24)) * std + mean).astype(np.float32)
+    t = rng.normal(size=(n, 3)).astype(np.float32)
+    return net, f, t, mean, std
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def test_params_roundtrip_exact():
+    net = native.Mlp(seed=5)
+    p = np.random.default_rng(1).normal(size=native.MLP_NUM_PARAMS).astype(np.float32) * 0.1
+    net.set_params(p)
+    assert np.array_equal(net.params(), p)
+    m, s = np.arange(24, dtype=np.float32), np.arange(1, 25, dtype=np.float32)
+    net.set_normalization(m, s)
+    m2, s2 = net.normalization()
+    assert np.array_equal(m2, m) and np.array_equal(s2, s)
+
+
+@pytest.mark.parametrize("n", [1, 64, 333, 5000])
+def test_forward_matches_oracle(n):
+    net, f, _, mean, std = _net_with_random_inputs(10 + n, n)
+    y = net.forward(f)
+    p = net.params()
+    ye = mo.forward(p, f, mean, std, emulate=True)
+    yx = mo.forward(p, f, mean, std)
+    print(f"n={n} vs bf16-emulating oracle {_rel(y, ye):.2e}, vs FP64 {_rel(y, yx):.2e}", flush=True)
+    assert _rel(y, ye) < 5e-3
+    assert _rel(y, yx) < 3e-2
+
+
+@pytest.mark.parametrize("n", [64, 500, 4096])
+def test_loss_gradient_matches_oracle(n):
+    net, f, t, mean, std = _net_with_random_inputs(20 + n, n)
+    loss, g = net.loss_grad(f, t)
+    p = net.params()
+    le, ge = mo.loss_grad(p, f, t, mean, std, emulate=True)
+    lx, gx = mo.loss_grad(p, f, t, mean, std)
+    print(f"n={n} loss {loss:.6f} emul {le:.6f} fp64 {lx:.6f}; grad vs emul {_rel(g, ge):.2e} vs fp64 {_rel(g, gx):.2e}",
+          flush=True)
+    assert abs(loss - le) <= 2e-3 * le
+    assert _rel(g, ge) < 1e-2
+    assert _rel(g, gx) < 5e-2
+    # every layer's block on its own (a transposed or misplaced block fails here)
+    k = 0
+    for o, i in mo.SHAPES:
+        for size in (o * i, o):
+            assert _rel(g[k:k + size], ge[k:k + size]) < 3e-2, (k, size)
+            k += size
+
+
+def test_training_reduces_loss():
+    rng = np.random.default_rng(3)
+    n = 40000
+    f = rng.normal(size=(n, 24)).astype(np.float32)
+    a = rng.normal(size=(24, 3)) / np.sqrt(24)
+    t = np.tanh(f @ a).astype(np.float32)
+    net = native.Mlp(seed=4)
+    l0, _ = net.loss_grad(f[:4096], t[:4096])
+    losses = net.train(f, t, epochs=6, batch=1024, lr=2e-3, seed=5)
+    l1, _ = net.loss_grad(f[:4096], t[:4096])
+    print(f"loss {l0:.4f} -> {l1:.5f}, epochs {np.round(losses, 5).tolist()}", flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-1] < losses[0]
+    assert l1 < 0.05 * l0
+
+
+def _step_states(scene, steps):
+    ctx = native.VbdContext([scene.vbd_object()])
+    states = [ctx.state()]
+    for _ in range(steps):
+        ctx.step(scene.dt, 1, scene.iterations, scene.gravity)
+        states.append(ctx.state())
+    return ctx, states
+
+
+def test_features_match_oracle():
+    s = scenes.cloth(20, young=3e3, bending=3e-4, pin="top_row", seed=4, iterations=6, mass=2e-3)
+    ctx, st = _step_states(s, 3)
+    h, g = s.dt, np.array(s.gravity)
+    (xp, vp), (x, v) = st[-2], st[-1]
+    xt_prev = xp + h * (vp + h * g)
+    corr = x - xt_prev
+    ctx.begin_substep(h, s.gravity)
+    F = ctx.features(0)
+    ref = mo.features(x + h * (v + h * g), v + h * g, corr, mo.static_tables(s), h, g)
+    err = np.abs(F - ref)
+    print(f"max |F - ref| per column {np.round(err.max(axis=0), 6).tolist()}", flush=True)
+    assert np.allclose(F[:, 18:21], ref[:, 18:21], rtol=1e-5, atol=1e-5)
+    assert np.allclose(F, ref, rtol=2e-3, atol=2e-3)
+    # the inertial-offset columns are not trivially zero in this state
+    assert np.abs(ref[:, 0:3]).max() > 0.1
+    ctx.end_substep()
+
+
+def test_features_batched_and_checkpointed():
+    a = scenes.cloth(12, young=2e3, pin="top_row", seed=1, iterations=5)
+    b = scenes.cloth(17, young=5e4, bending=1e-3, pin="corners", seed=2, iterations=5)
+    ctx = native.VbdContext([a.vbd_object(), b.vbd_object()])
+    for _ in range(2):
+        ctx.step(a.dt, 1, 5, a.gravity)
+    ctx.save_state()
+    ctx.begin_substep(a.dt, a.gravity)
+    F = ctx.features(-1)
+    Fa, Fb = ctx.features(0), ctx.features(1)
+    assert np.array_equal(F, np.concatenate([Fa, Fb]))
+    ctx.iterate(5)
+    ctx.end_substep()
+    ctx.restore_state()
+    ctx.begin_substep(a.dt, a.gravity)
+    assert np.array_equal(ctx.features(-1), F)  # the checkpoint carries the substep history
+    ctx.end_substep()
+    # set_state clears the history of that object only
+    x, v = ctx.state(0)
+    ctx.set_state(0, x, v)
+    ctx.begin_substep(a.dt, a.gravity)
+    F2 = ctx.features(-1)
+    assert np.all(F2[:a.n, 0:3] == 0) and np.all(F2[:a.n, 6:12] == 0)
+    ctx.end_substep()
+
+
+def _constant_net(bias):
+    p = np.zeros(native.MLP_NUM_PARAMS, dtype=np.float32)
+    p[-3:] = bias
+    net = native.Mlp(seed=1)
+    net.set_params(p)
+    return net
+
+
+def test_warm_start_zero_prediction_is_the_inertial_guess():
+    s = scenes.cloth(16, young=1e4, pin="top_row", seed=3, iterations=5)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.begin_substep(s.dt, s.gravity)
+    acc, rho = ctx.warm_start(_constant_net([0.0, 0.0, 0.0]))
+    assert acc[0] and rho[0] == 1.0
+    assert np.array_equal(ctx.residual(), ctx.residual_at_inertial())
+    ctx.end_substep()
+
+
+def test_warm_start_gate_discards_a_worse_prediction():
+    s = scenes.cloth(16, young=1e4, pin="top_row", seed=3, iterations=5)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.save_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    acc, rho = ctx.warm_start(_constant_net([0.0, 50.0, 0.0]), gate=False)
+    assert acc[0] and rho[0] > 1.0  # an upward jump of 50 h^2 g: worse than x~, kept without the gate
+    ctx.end_substep()
+    ctx.restore_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    acc, rho = ctx.warm_start(_constant_net([0.0, 50.0, 0.0]), gate=True)
+    assert not acc[0] and rho[0] > 1.0
+    assert np.array_equal(ctx.residual(), ctx.residual_at_inertial())
+    x, _ = ctx.state()
+    ctx.end_substep()
+    # with the prediction discarded the solve continues from x~ exactly like plain VBD
+    ctx.restore_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    x2, _ = ctx.state()
+    ctx.end_substep()
+    assert np.array_equal(x, x2)
+
+
+def test_trained_network_needs_fewer_iterations():
+    """Train on converged trajectories of small random cloths (config-4 style) and compare iterations
+    to a 1e-4 relative force residual on an unseen cloth of the 512x512 construction at 96x96."""
+    train_set = scenes.cloth_batch(count=24, seed=11, lo=16, hi=48)
+    f, t = learned.collect(train_set, steps=8, samples=120_000, rel_tol=1e-5, max_iterations=800, check_every=5)
+    net, losses = learned.train(f, t, epochs=10, batch=2048, lr=2e-3, seed=2)
+    print(f"{f.shape[0]} samples, losses {np.round(losses, 5).tolist()}", flush=True)
+    assert losses[-1] < 0.5 * losses[0]
+    scene = scenes.cloth_config2(seed=3, n=96)
+    rows = learned.compare(scene, net, steps=6, rel_tol=1e-4, max_iterations=3000, check_every=1,
+                           log=lambda m: print(m, flush=True))
+    sm = learned.summarize(rows)
+    print(sm, flush=True)
+    assert sm["all_converged"]
+    assert all(r["finite"] for r in rows)
+    assert sm["learned_iterations"] < sm["plain_iterations"]

@@ -264,6 +264,35 @@ def test_lane_group_shapes_agree(lanes, oracle_threads):
     assert rel_err(ctx.state()[0], o.state()[0]) < 1e-5
 
 
+@pytest.mark.parametrize("lanes,block", [(1, 64), (4, 128), (6, 256), (8, 64), (16, 256)])
+def test_lds_staged_rows_bitwise_equal_direct(lanes, block):
+    """The LDS-staged variant reads the same records in the same order: bitwise the same step, on
+    cloth (triangles + hinges + anchors), a tetrahedral cube (tets + anchors) and a spring net."""
+    cases = [scenes.cloth(40, young=1e4, pin="top_row", iterations=6), scenes.cube(5, iterations=6)]
+    rng = np.random.default_rng(0)
+    x = rng.uniform(size=(60, 3))
+    pairs = np.array([(i, j) for i in range(60) for j in range(i + 1, 60) if np.linalg.norm(x[i] - x[j]) < 0.35],
+                     dtype=np.int32)
+    L = np.linalg.norm(x[pairs[:, 0]] - x[pairs[:, 1]], axis=1) * 0.9
+    cases.append(scenes.Scene(x=x, x0=x, m=np.full(60, 0.01), iterations=6,
+                              energies=[("spring", pairs, np.column_stack([L, np.full(len(L), 50.0)])),
+                                        ("anchor", np.array([[0]], np.int32), np.array([[*x[0], 1e4]]))]))
+    for s in cases:
+        out = []
+        for staged in (0, 1):
+            ctx = gpu_context(s, lanes_per_vertex=lanes, block_size=block)
+            try:
+                ctx.set_launch(lds_staging=staged)
+            except native.NativeError as e:  # rows of one workgroup larger than 64 KB: not offered
+                assert staged and e.code == 3
+                break
+            for _ in range(2):
+                ctx.step(s.dt, 1, s.iterations, s.gravity)
+            out.append(ctx.state())
+        if len(out) == 2:
+            assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_vbdgpu_mirror_matches_capi():
     """spg::solver::VBDGpu (C++ mirror of the BaseSolver API) gives the same result as the C-ABI."""
     s = scenes.cloth_config1()

@@ -19,9 +19,10 @@ traffic    = HBM bytes per vertex-solve launch from the committed rocprofv3 coun
 learned    = time to a 1e-4 relative FP64 force residual, learned warm start vs plain VBD, on the same
              cloth (network trained in this run on the GPU from 256 config-4-style cloths)
 batched    = 256-cloth batch (config 4) throughput, one launch per colour for all instances
+volumetric = 64^3-cell Stable Neo-Hookean cube (config 3) throughput
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu] [--no-learned]
-                       [--no-batched]
+                       [--no-batched] [--no-volumetric]
 """
 import argparse
 import json
@@ -46,10 +47,12 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--lds", type=int, default=-1, help="lds_staging of the vertex solve (with --lanes/--block)")
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start comparison")
     ap.add_argument("--learned-steps", type=int, default=8, help="trajectory steps of the learned comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloth batched measurement")
+    ap.add_argument("--no-volumetric", action="store_true", help="skip the 64^3 cube measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -80,6 +83,8 @@ def main():
             kw["lanes_per_vertex"] = args.lanes
         if args.block:
             kw["block_size"] = args.block
+        if args.lds >= 0:
+            kw["lds_staging"] = args.lds
         ctx.set_launch(**kw)
     elif not args.no_autotune:
         tuned = ctx.autotune(trials=3)
@@ -123,6 +128,7 @@ def main():
     traffic, traffic_src = measured_traffic(ctx.launch(), args)
 
     batched = None if args.no_batched else batched_throughput(args)
+    volumetric = None if args.no_volumetric else volumetric_throughput(args)
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
 
     cpu = None
@@ -158,26 +164,26 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned_cmp,
         "batched": batched,
+        "volumetric": volumetric,
     }
     print(json.dumps(out))
 
 
-def batched_throughput(args, count=256, steps=30):
-    """256 independent cloths of side U{16..128} advanced together: one launch per colour covers every
-    instance, launch shape from the autotuner, each step captured in a hipGraph. Throughput in
-    vertex-updates/s over `steps` steps (each from the initial state, as above)."""
+def _throughput(scene_list, steps):
+    """Autotuned, graph-captured steps of the scenes advanced together (each step from the initial
+    state); returns (vertex-updates/s, ms per step, launch, event-timed solve launch us, algorithmic
+    GB/s of the solve, all positions finite, context setup s)."""
     import numpy as np
-    from spg_amd import native, scenes
+    from spg_amd import native
 
     t0 = time.perf_counter()
-    batch = scenes.cloth_batch(count=count, seed=7)
-    ctx = native.VbdContext([b.vbd_object() for b in batch], device=0)
+    ctx = native.VbdContext([b.vbd_object() for b in scene_list], device=0)
     setup = time.perf_counter() - t0
     tuned = ctx.autotune(trials=2)
     ctx.set_launch(use_graph=1)
-    V = sum(b.n for b in batch)
-    iters = batch[0].iterations
-    dt, g = batch[0].dt, batch[0].gravity
+    V = sum(b.n for b in scene_list)
+    iters = scene_list[0].iterations
+    dt, g = scene_list[0].dt, scene_list[0].gravity
     ctx.save_state()
     for _ in range(3):
         ctx.restore_state()
@@ -196,16 +202,38 @@ def batched_throughput(args, count=256, steps=30):
     ctx.synchronize()
     st = ctx.stats()
     ctx.set_timing(False)
-    finite = all(bool(np.isfinite(ctx.state(o)[0]).all()) for o in range(len(batch)))
+    finite = all(bool(np.isfinite(ctx.state(o)[0]).all()) for o in range(len(scene_list)))
     mean_ms = st["solve_ms"] / max(1, st["solve_launches"])
-    return {"workload": f"{count} cloths, side U{{16..128}} (seed 7), Y log-U[1e3,1e5], bending log-U[1e-5,1e-3], "
-                        f"top row anchored, {iters} iterations/step, one launch per colour for all instances, "
-                        f"hipGraph replay",
-            "vertices": V, "colors": ctx.num_colors(), "launch": tuned, "value": V * iters * steps / wall,
+    return {"vertices": V, "colors": ctx.num_colors(), "launch": tuned, "value": V * iters * steps / wall,
             "unit": "vertex-updates/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
             "solve_launch_us": mean_ms * 1e3,
             "solve_achieved_GBs": st["algorithmic_bytes"] / max(1, st["solve_launches"]) / (mean_ms * 1e-3) / 1e9,
+            "fp64_fallbacks": st["fp64_fallbacks"], "skipped_blocks": st["skipped_blocks"],
             "setup_s": setup, "state_finite": finite}
+
+
+def batched_throughput(args, count=256, steps=30):
+    """256 independent cloths of side U{16..128} advanced together: one launch per colour covers every
+    instance, launch shape from the autotuner, each step captured in a hipGraph."""
+    from spg_amd import scenes
+
+    batch = scenes.cloth_batch(count=count, seed=7)
+    out = {"workload": f"{count} cloths, side U{{16..128}} (seed 7), Y log-U[1e3,1e5], bending log-U[1e-5,1e-3], "
+                       f"top row anchored, {batch[0].iterations} iterations/step, one launch per colour for all "
+                       f"instances, hipGraph replay"}
+    out.update(_throughput(batch, steps))
+    return out
+
+
+def volumetric_throughput(args, steps=20):
+    """The 64^3-cell Stable Neo-Hookean cube (274,625 vertices, 1,572,864 tets, 8 colours)."""
+    from spg_amd import scenes
+
+    cube = scenes.cube_config3()
+    out = {"workload": "cube 64^3 cells, 6 tets per cell (V=274625, 1572864 SNH tets, Y 1e5, nu 0.45), bottom face "
+                       "anchored, rest scaled 1.05 in y, 20 iterations/step, hipGraph replay"}
+    out.update(_throughput([cube], steps))
+    return out
 
 
 def learned_comparison(scene, args, launch):
@@ -232,7 +260,8 @@ def learned_comparison(scene, args, launch):
                         f"learned solve), residual checked every iteration",
             "plain_ms": sm["plain_ms"], "learned_ms": sm["learned_ms"], "time_ratio": sm["time_ratio"],
             "plain_iterations": sm["plain_iterations"], "learned_iterations": sm["learned_iterations"],
-            "iteration_ratio": sm["iteration_ratio"], "accepted": sm["accepted"], "all_converged": sm["all_converged"],
+            "iteration_ratio": sm["iteration_ratio"], "accepted": sm["accepted"], "fell_back": sm["fell_back"],
+            "all_converged": sm["all_converged"],
             "per_step": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
             "training": {"samples": int(f.shape[0]), "scenes": len(train_set), "collect_s": collect_s,
                          "train_s": train_s, "epochs": len(losses), "final_loss": float(losses[-1]),
@@ -248,7 +277,8 @@ def measured_traffic(launch, args):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    if f"lanes {launch['lanes_per_vertex']}, block {launch['block_size']}" not in t["workload"]:
+    shape = f"lanes {launch['lanes_per_vertex']}, block {launch['block_size']}, lds {launch['lds_staging']}"
+    if not t["workload"].endswith(shape):
         return None, None
     return t["traffic_bytes"], "profiles/solveColorKernel_cloth512_traffic.json (rocprofv3 --pmc, same kernel and launch)"
 

@@ -106,6 +106,9 @@ typedef struct {
                                     element record (triangle 40, hinge 24, tet 64, spring 16, anchor 20 B) */
     int64_t fp64_fallbacks;      /* block solves that took the FP64 near-singular path */
     int64_t skipped_blocks;      /* block solves skipped by det(H) <= 1e-10 */
+    int64_t prediction_fallbacks; /* objects whose learned start (spg_vbd_learned.h) was dropped during
+                                    spg_vbd_iterate_to_tolerance because their residual rose above the
+                                    inertial guess's or became non-finite */
 } spg_vbd_stats;
 
 typedef struct spg_vbd_ctx spg_vbd_ctx;
@@ -153,6 +156,11 @@ int spg_vbd_residual_at_inertial(spg_vbd_ctx *ctx, double *norms);
    max_iterations sweeps. Reports the sweeps done and the final max rho over objects. */
 int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_iterations, int32_t check_every,
                                  int32_t *iterations, double *rho);
+
+/* spg_vbd_iterate_to_tolerance: iterates (check_every sweeps between FP64 residual checks) until the
+   largest per-object |r(x)| / |r(x~)| is at most rel_tol or max_iterations sweeps are done; *rho is that
+   ratio (NaN when a residual is not finite). An object that started from a learned prediction whose
+   residual rises above |r(x~)| goes back to x~ and continues as plain VBD. */
 
 /* Launch configuration and autotuning. */
 int spg_vbd_get_launch(const spg_vbd_ctx *ctx, spg_vbd_launch *cfg);

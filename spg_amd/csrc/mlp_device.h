@@ -74,6 +74,9 @@ struct FeatureArgs {
 hipError_t launchFeatures(const FeatureArgs &a, hipStream_t stream);
 hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float *y, int32_t ldy, float s, int32_t n,
                                 hipStream_t stream);
+// y, out: n rows of 4 floats (the first three are the displacement)
+hipError_t launchSmoothPrediction(const float *y, float *out, const int32_t *nbPtr, const int32_t *nbIdx, int32_t n,
+                                  hipStream_t stream);
 hipError_t launchGate(float4 *pos, const float4 *xt, const int32_t *objOfSlot, const int32_t *reject, int32_t n,
                       hipStream_t stream);
 hipError_t launchCorrection(const float4 *pos, const float4 *xt, float4 *corr, int32_t n, hipStream_t stream);

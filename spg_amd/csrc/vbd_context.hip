@@ -7,6 +7,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -138,6 +139,9 @@ struct spg_vbd_ctx {
     DevBuf<float> nbRest, feat, pred;
     DevBuf<float4> stat;
     double g[3] = {0.0, 0.0, 0.0};  // gravity of the open substep
+    std::vector<int32_t> predicted;  // per object: the open substep started from a network prediction
+    bool anyPredicted = false;
+    int64_t predictionFallbacks = 0;
 
     spg_vbd_launch launch{4, 256, 0, 0};
     bool timing = false;
@@ -262,6 +266,8 @@ struct spg_vbd_ctx {
     {
         h = hh;
         r0Valid = false;
+        anyPredicted = false;
+        predicted.assign(nObjects, 0);
         for (int k = 0; k < 3; ++k) this->g[k] = g[k];
         hipCheck(launchBeginSubstep(pos.p, xold.p, vel.p, xt.p, mass.p, static_cast<int32_t>(nv), static_cast<float>(hh),
                                     static_cast<float>(g[0]), static_cast<float>(g[1]), static_cast<float>(g[2]),
@@ -355,15 +361,47 @@ struct spg_vbd_ctx {
         hipCheck(launchFeatures(a, stream), "features launch");
     }
 
-    void warmStart(spg_mlp *net, bool gate, int32_t *accepted, double *rho)
+    // Objects whose substep started from a network prediction: while one of them has a residual larger
+    // than at x~ (or a non-finite one) its start is dropped and it continues from x~, as plain VBD.
+    // Returns whether any object fell back (their positions are x~ again).
+    bool fallBackFromPredictions(const std::vector<double> &r0, const std::vector<double> &r)
+    {
+        if (!anyPredicted) return false;
+        std::vector<int32_t> rej(nObjects, 0);
+        bool any = false;
+        for (int32_t o = 0; o < nObjects; ++o)
+            if (predicted[o] && !(r[o] <= r0[o])) {
+                rej[o] = 1;
+                predicted[o] = 0;
+                ++predictionFallbacks;
+                any = true;
+            }
+        anyPredicted = std::find(predicted.begin(), predicted.end(), 1) != predicted.end();
+        if (any) {
+            hipCheck(hipMemcpyAsync(reject.p, rej.data(), sizeof(int32_t) * nObjects, hipMemcpyHostToDevice, stream),
+                     "upload");
+            hipCheck(launchGate(pos.p, xt.p, objOfSlot.p, reject.p, static_cast<int32_t>(nv), stream), "gate launch");
+            hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        }
+        return any;
+    }
+
+    void warmStart(spg_mlp *net, int32_t smoothing, bool gate, int32_t *accepted, double *rho)
     {
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
         if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
+        if (smoothing < 0 || smoothing > 64) throw VbdError(SPG_VBD_ERR_INVALID, "smoothing passes must be in 0..64");
         computeFeatures();
-        pred.reserve(static_cast<size_t>(nv) * 4);
-        mlpInferDevice(net, feat.p, nv, pred.p, 4, stream);
+        pred.reserve(static_cast<size_t>(nv) * 8);  // two ping-pong rows of 4 floats per vertex
+        float *y = pred.p, *ySpare = pred.p + static_cast<size_t>(nv) * 4;
+        hipCheck(hipMemsetAsync(y, 0, sizeof(float) * 4 * nv, stream), "memset");
+        mlpInferDevice(net, feat.p, nv, y, 4, stream);
         hipCheck(hipSetDevice(device), "hipSetDevice");
-        hipCheck(launchApplyWarmStart(pos.p, xt.p, pred.p, 4, static_cast<float>(h * h * refAccel()),
+        for (int32_t k = 0; k < smoothing; ++k) {
+            hipCheck(launchSmoothPrediction(y, ySpare, nbPtr.p, nbIdx.p, static_cast<int32_t>(nv), stream), "smooth");
+            std::swap(y, ySpare);
+        }
+        hipCheck(launchApplyWarmStart(pos.p, xt.p, y, 4, static_cast<float>(h * h * refAccel()),
                                       static_cast<int32_t>(nv), stream),
                  "warm start launch");
         std::vector<int32_t> rej(nObjects, 0);
@@ -384,10 +422,13 @@ struct spg_vbd_ctx {
                 hipCheck(launchGate(pos.p, xt.p, objOfSlot.p, reject.p, static_cast<int32_t>(nv), stream), "gate launch");
             }
         }
+        predicted.assign(nObjects, 0);
         for (int32_t o = 0; o < nObjects; ++o) {
+            predicted[o] = !rej[o];
             if (accepted) accepted[o] = !rej[o];
             if (rho) rho[o] = r0[o] > 0.0 ? rp[o] / r0[o] : rp[o];
         }
+        anyPredicted = std::find(predicted.begin(), predicted.end(), 1) != predicted.end();
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
@@ -1205,12 +1246,18 @@ int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_i
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         std::vector<double> r0(ctx->nObjects), r(ctx->nObjects);
         ctx->residual(true, r0.data());
+        // largest per-object relative residual; NaN when any object's residual is not finite
         auto relMax = [&] {
             double m = 0.0;
-            for (int32_t o = 0; o < ctx->nObjects; ++o) m = std::max(m, r0[o] > 0 ? r[o] / r0[o] : r[o]);
+            for (int32_t o = 0; o < ctx->nObjects; ++o) {
+                const double q = r0[o] > 0 ? r[o] / r0[o] : r[o];
+                if (!std::isfinite(q)) return std::numeric_limits<double>::quiet_NaN();
+                m = std::max(m, q);
+            }
             return m;
         };
         ctx->residual(false, r.data());
+        ctx->fallBackFromPredictions(r0, r);
         double cur = relMax();
         int32_t done = 0;
         while (!(cur <= rel_tol) && done < max_iterations) {
@@ -1218,6 +1265,7 @@ int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_i
             ctx->sweepColors(n, ctx->timing);
             done += n;
             ctx->residual(false, r.data());
+            if (ctx->fallBackFromPredictions(r0, r)) ctx->residual(false, r.data());
             cur = relMax();
         }
         *iterations = done;
@@ -1244,12 +1292,12 @@ int spg_vbd_features(spg_vbd_ctx *ctx, int32_t object, float *features)
     });
 }
 
-int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t gate, int32_t *accepted, double *rho)
+int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t smoothing, int32_t gate, int32_t *accepted, double *rho)
 {
     return guarded(ctx, [&] {
         if (!ctx || !net) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
-        ctx->warmStart(net, gate != 0, accepted, rho);
+        ctx->warmStart(net, smoothing, gate != 0, accepted, rho);
     });
 }
 
@@ -1346,6 +1394,7 @@ int spg_vbd_get_stats(const spg_vbd_ctx *ctx, spg_vbd_stats *stats)
         *stats = c->stats;
         stats->fp64_fallbacks = static_cast<int64_t>(cnt[0]);
         stats->skipped_blocks = static_cast<int64_t>(cnt[1]);
+        stats->prediction_fallbacks = c->predictionFallbacks;
     });
 }
 
@@ -1356,6 +1405,7 @@ int spg_vbd_reset_stats(spg_vbd_ctx *ctx)
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         ctx->drainEvents();
         ctx->stats = spg_vbd_stats{};
+        ctx->predictionFallbacks = 0;
         const unsigned long long zero[2] = {0, 0};
         hipCheck(hipMemcpyAsync(ctx->counters.p, zero, sizeof(zero), hipMemcpyHostToDevice, ctx->stream), "upload");
         hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");

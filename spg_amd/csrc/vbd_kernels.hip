@@ -223,8 +223,8 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const bool ok = o.x >= 0;
             V3f ge;
             S3f He;
-            stvkVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), gg.x, gg.y, gg.z, gg.w, kk.x,
-                              kk.y, kk.z, ge, He);
+            stvkVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), gg.x, gg.y, gg.z, gg.w,
+                              kk.x, kk.y, kk.z, ge, He);
             accIf(ok, g, H, ge, He);
         }
         for (int32_t j = lig; j < a.hinEW; j += G) {
@@ -234,8 +234,9 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const bool ok = v.x >= 0;
             V3f ge;
             S3f He;
-            bendingEdge<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), xyz(a.pos[ok ? v.z : vid]),
-                               xyz(a.pos[ok ? v.w : vid]), v.y == vid, p.x, p.y, ge, He);
+            bendingEdge<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]),
+                               xyz(a.pos[ok ? v.z : vid]), xyz(a.pos[ok ? v.w : vid]), v.y == vid, p.x, p.y, ge,
+                               He);
             accIf(ok, g, H, ge, He);
         }
         for (int32_t j = lig; j < a.hinWW; j += G) {
@@ -245,8 +246,9 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const bool ok = v.x >= 0;
             V3f ge;
             S3f He;
-            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), xyz(a.pos[ok ? v.z : vid]),
-                               xyz(a.pos[ok ? v.w : vid]), v.w == vid, p.x, p.y, ge, He);
+            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]),
+                               xyz(a.pos[ok ? v.z : vid]), xyz(a.pos[ok ? v.w : vid]), v.w == vid, p.x, p.y, ge,
+                               He);
             accIf(ok, g, H, ge, He);
         }
         for (int32_t j = lig; j < a.tetW; j += G) {
@@ -306,90 +308,116 @@ using S3d = Sym3<double>;
 
 __device__ __forceinline__ V3d xyzd(float4 p) { return {double(p.x), double(p.y), double(p.z)}; }
 
+template <int G>
+__device__ __forceinline__ double groupReduceD(double v, int32_t lane64, int32_t lig)
+{
+#pragma unroll
+    for (int off = 1; off < G; off <<= 1) {
+        const double o = __shfl(v, lane64 + off, 64);
+        if (lig + off < G) v += o;
+    }
+    return v;
+}
+
 // K4: r_v = -(m/h^2)(x_v - x_tilde_v) - sum_e grad_v E_e(x), all in FP64 from the FP32 state and the
 // exact prepared parameters (implicitEulerBase.cpp:102-143 force, x-form of
 // implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per vertex slot.
+// A group of kResidualLanes lanes per vertex: lane l of the group takes entries l, l + G, ... of each
+// incidence class, and the group sums its partial forces in a fixed order.
+constexpr int kResidualLanes = 8;
+
 __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
 {
-    const int32_t vid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (vid >= a.n) return;
-    const float4 xf = a.pos[vid], tf = a.xt[vid];
-    const V3d xo = xyzd(xf);
-    const double mdt2 = a.mass[vid] * a.invdt2;
-    V3d r = {-mdt2 * (xo.x - double(tf.x)), -mdt2 * (xo.y - double(tf.y)), -mdt2 * (xo.z - double(tf.z))};
-    V3d ge;
-    S3d He;
-    if (a.triPtr) {
-        for (int32_t k = a.triPtr[vid]; k < a.triPtr[vid + 1]; ++k) {
-            const int32_t el = a.triElem[k];
-            const int4 v = a.triV[el];
-            const double *p = a.triP + 8 * static_cast<int64_t>(el);
-            // rows of Dinv: g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2)
-            const double g[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
-            const int s = v.x == vid ? 0 : (v.y == vid ? 1 : 2);
-            const int ia = s == 0 ? 1 : 0, ib = s == 2 ? 1 : 2;
-            const int vv[3] = {v.x, v.y, v.z};
-            stvkVertex<double>(xo, xyzd(a.pos[vv[ia]]), xyzd(a.pos[vv[ib]]), g[ia][0], g[ia][1], g[ib][0], g[ib][1],
-                               p[4], p[5], p[6], ge, He);
-            r = r - ge;
+    constexpr int G = kResidualLanes, GPW = 64 / G;
+    const int32_t lane64 = threadIdx.x & 63;
+    const int32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int32_t grp = lane64 / G, lig = lane64 - grp * G;
+    const int32_t vid = wave * GPW + grp;
+    const bool active = vid < a.n;
+    V3d r = {0.0, 0.0, 0.0};
+    if (active) {
+        const float4 xf = a.pos[vid], tf = a.xt[vid];
+        const V3d xo = xyzd(xf);
+        if (lig == 0) {
+            const double mdt2 = a.mass[vid] * a.invdt2;
+            r = V3d{-mdt2 * (xo.x - double(tf.x)), -mdt2 * (xo.y - double(tf.y)), -mdt2 * (xo.z - double(tf.z))};
+        }
+        V3d ge;
+        S3d He;
+        if (a.triPtr) {
+            for (int32_t k = a.triPtr[vid] + lig; k < a.triPtr[vid + 1]; k += G) {
+                const int32_t el = a.triElem[k];
+                const int4 v = a.triV[el];
+                const double *p = a.triP + 8 * static_cast<int64_t>(el);
+                // rows of Dinv: g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2)
+                const double g[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
+                const int s = v.x == vid ? 0 : (v.y == vid ? 1 : 2);
+                const int ia = s == 0 ? 1 : 0, ib = s == 2 ? 1 : 2;
+                const int vv[3] = {v.x, v.y, v.z};
+                stvkVertex<double>(xo, xyzd(a.pos[vv[ia]]), xyzd(a.pos[vv[ib]]), g[ia][0], g[ia][1], g[ib][0],
+                                   g[ib][1], p[4], p[5], p[6], ge, He);
+                r = r - ge;
+            }
+        }
+        for (int cls = 0; cls < 2; ++cls) {  // hinge-edge and hinge-wing incidences
+            const int32_t *ptr = cls == 0 ? a.hinEPtr : a.hinWPtr;
+            const int32_t *elem = cls == 0 ? a.hinEElem : a.hinWElem;
+            if (!ptr) continue;
+            for (int32_t k = ptr[vid] + lig; k < ptr[vid + 1]; k += G) {
+                const int32_t el = elem[k];
+                const int4 v = a.hinV[el];
+                const double2 p = a.hinP[el];
+                const int slot = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
+                bendingVertexCompact<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]),
+                                             slot, p.x, p.y, ge, He);
+                r = r - ge;
+            }
+        }
+        if (a.tetPtr) {
+            for (int32_t k = a.tetPtr[vid] + lig; k < a.tetPtr[vid + 1]; k += G) {
+                const int32_t el = a.tetElem[k];
+                const int4 v = a.tetV[el];
+                const double *p = a.tetP + 12 * static_cast<int64_t>(el);
+                V3d gr[4];
+                gr[1] = {p[0], p[1], p[2]};
+                gr[2] = {p[3], p[4], p[5]};
+                gr[3] = {p[6], p[7], p[8]};
+                gr[0] = -(gr[1] + gr[2] + gr[3]);
+                const int vv[4] = {v.x, v.y, v.z, v.w};
+                const int s = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
+                int o[3], m = 0;
+                for (int q = 0; q < 4; ++q)
+                    if (q != s) o[m++] = q;
+                snhVertex<double>(xo, xyzd(a.pos[vv[o[0]]]), xyzd(a.pos[vv[o[1]]]), xyzd(a.pos[vv[o[2]]]), gr[o[0]],
+                                  gr[o[1]], gr[o[2]], p[9], p[10], p[11], ge, He);
+                r = r - ge;
+            }
+        }
+        if (a.sprPtr) {
+            for (int32_t k = a.sprPtr[vid] + lig; k < a.sprPtr[vid + 1]; k += G) {
+                const int32_t el = a.sprElem[k];
+                const int2 v = a.sprV[el];
+                const double2 p = a.sprP[el];
+                const int32_t other = v.x == vid ? v.y : v.x;
+                springVertex<double>(xo, xyzd(a.pos[other]), p.x, p.y, ge, He);
+                r = r - ge;
+            }
+        }
+        if (a.ancPtr) {
+            for (int32_t k = a.ancPtr[vid] + lig; k < a.ancPtr[vid + 1]; k += G) {
+                const int32_t el = a.ancElem[k];
+                const double4 p = a.ancP[el];
+                anchorVertex<double>(xo, V3d{p.x, p.y, p.z}, p.w, ge, He);
+                r = r - ge;
+            }
         }
     }
-    for (int cls = 0; cls < 2; ++cls) {  // hinge-edge and hinge-wing incidences
-        const int32_t *ptr = cls == 0 ? a.hinEPtr : a.hinWPtr;
-        const int32_t *elem = cls == 0 ? a.hinEElem : a.hinWElem;
-        if (!ptr) continue;
-        for (int32_t k = ptr[vid]; k < ptr[vid + 1]; ++k) {
-            const int32_t el = elem[k];
-            const int4 v = a.hinV[el];
-            const double2 p = a.hinP[el];
-            const int slot = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
-            bendingVertexCompact<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), slot,
-                                         p.x, p.y, ge, He);
-            r = r - ge;
-        }
-    }
-    if (a.tetPtr) {
-        for (int32_t k = a.tetPtr[vid]; k < a.tetPtr[vid + 1]; ++k) {
-            const int32_t el = a.tetElem[k];
-            const int4 v = a.tetV[el];
-            const double *p = a.tetP + 12 * static_cast<int64_t>(el);
-            V3d gr[4];
-            gr[1] = {p[0], p[1], p[2]};
-            gr[2] = {p[3], p[4], p[5]};
-            gr[3] = {p[6], p[7], p[8]};
-            gr[0] = -(gr[1] + gr[2] + gr[3]);
-            const int vv[4] = {v.x, v.y, v.z, v.w};
-            const int s = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
-            int o[3], m = 0;
-            for (int q = 0; q < 4; ++q)
-                if (q != s) o[m++] = q;
-            snhVertex<double>(xo, xyzd(a.pos[vv[o[0]]]), xyzd(a.pos[vv[o[1]]]), xyzd(a.pos[vv[o[2]]]), gr[o[0]],
-                              gr[o[1]], gr[o[2]], p[9], p[10], p[11], ge, He);
-            r = r - ge;
-        }
-    }
-    if (a.sprPtr) {
-        for (int32_t k = a.sprPtr[vid]; k < a.sprPtr[vid + 1]; ++k) {
-            const int32_t el = a.sprElem[k];
-            const int2 v = a.sprV[el];
-            const double2 p = a.sprP[el];
-            const int32_t other = v.x == vid ? v.y : v.x;
-            springVertex<double>(xo, xyzd(a.pos[other]), p.x, p.y, ge, He);
-            r = r - ge;
-        }
-    }
-    if (a.ancPtr) {
-        for (int32_t k = a.ancPtr[vid]; k < a.ancPtr[vid + 1]; ++k) {
-            const int32_t el = a.ancElem[k];
-            const double4 p = a.ancP[el];
-            anchorVertex<double>(xo, V3d{p.x, p.y, p.z}, p.w, ge, He);
-            r = r - ge;
-        }
-    }
-    a.r2[vid] = dot(r, r);
+    r.x = groupReduceD<G>(r.x, lane64, lig);
+    r.y = groupReduceD<G>(r.y, lane64, lig);
+    r.z = groupReduceD<G>(r.z, lane64, lig);
+    if (active && lig == 0) a.r2[vid] = dot(r, r);
 }
 
-// Sum of r2 over each object's slots, in a fixed order (one block per object).
 // Per-object |r| in two deterministic stages. Stage 1: block b sums the squared residuals of segment b
 // (at most kReduceSegment entries of one object's slot list) in a fixed tree order. Stage 2: one
 // thread per object sums its segments' partials in segment order and takes the square root.
@@ -529,7 +557,7 @@ size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
 {
     if (a.n <= 0) return hipSuccess;
-    residualKernel<<<blocksFor(a.n, 256), 256, 0, stream>>>(a);
+    residualKernel<<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 

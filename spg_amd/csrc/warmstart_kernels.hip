@@ -64,6 +64,27 @@ __global__ void __launch_bounds__(256) featuresKernel(FeatureArgs a)
     o[21] = es * inv; o[22] = emax; o[23] = emin;
 }
 
+// One smoothing pass of the predicted displacement over the one ring: out_i = (y_i + sum_j y_j) / (1 + deg_i)
+// (first three columns, row stride 4).
+__global__ void __launch_bounds__(256) smoothPredictionKernel(const float *__restrict__ y, float *__restrict__ out,
+                                                              const int32_t *__restrict__ nbPtr,
+                                                              const int32_t *__restrict__ nbIdx, int32_t n)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 yi = *reinterpret_cast<const float4 *>(y + 4 * static_cast<int64_t>(i));
+    float sx = yi.x, sy = yi.y, sz = yi.z;
+    const int32_t b = nbPtr[i], e = nbPtr[i + 1];
+    for (int32_t k = b; k < e; ++k) {
+        const float4 yj = *reinterpret_cast<const float4 *>(y + 4 * static_cast<int64_t>(nbIdx[k]));
+        sx += yj.x;
+        sy += yj.y;
+        sz += yj.z;
+    }
+    const float inv = 1.f / static_cast<float>(1 + e - b);
+    *reinterpret_cast<float4 *>(out + 4 * static_cast<int64_t>(i)) = make_float4(sx * inv, sy * inv, sz * inv, 0.f);
+}
+
 // x = x~ + s * y for every vertex (y: the network output, row stride ldy).
 __global__ void __launch_bounds__(256) applyWarmStartKernel(float4 *__restrict__ pos, const float4 *__restrict__ xt,
                                                             const float *__restrict__ y, int32_t ldy, float s, int32_t n)
@@ -113,6 +134,14 @@ hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float *y, i
 {
     if (n <= 0) return hipSuccess;
     applyWarmStartKernel<<<blocksOf(n), 256, 0, stream>>>(pos, xt, y, ldy, s, n);
+    return hipGetLastError();
+}
+
+hipError_t launchSmoothPrediction(const float *y, float *out, const int32_t *nbPtr, const int32_t *nbIdx, int32_t n,
+                                  hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    smoothPredictionKernel<<<blocksOf(n), 256, 0, stream>>>(y, out, nbPtr, nbIdx, n);
     return hipGetLastError();
 }
 

@@ -61,17 +61,24 @@ def training_scenes(count=256, seed=7, iterations=15):
     return scenes.cloth_batch(count=count, seed=seed, iterations=iterations)
 
 
-def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every=1, gate=True, launch=None,
-            device=0, log=None):
+def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every=1, gate=True, smoothing=2,
+            launch=None, device=0, reference_tol=1e-5, log=None):
     """Per substep of `scene`'s trajectory: iterations and wall time to reach rho <= rel_tol from the
     inertial guess (plain VBD) and from the network's gated prediction (learned), both from the same
-    state (device checkpoint). The trajectory continues from the learned solve."""
+    state (device checkpoint). The learned path keeps plain VBD as the fallback: a rejected prediction
+    starts from x~, and an accepted one whose residual later rises above x~'s is dropped for x~ inside
+    iterate_to_tolerance (counted in fell_back, the wasted iterations included).
+
+    Both paths are measured from the states of one reference trajectory, advanced by plain VBD to the
+    tighter reference_tol, so that neither path's tolerance-sized error steers the states the other is
+    measured on."""
     ctx = native.VbdContext([scene.vbd_object()], device=device)
     if launch:
         ctx.set_launch(**launch)
     ctx.set_launch(use_graph=0)
     h, g = scene.dt, scene.gravity
     rows = []
+    fallbacks_before = ctx.stats()["prediction_fallbacks"]
     for t in range(steps):
         ctx.save_state()
         ctx.synchronize()
@@ -86,21 +93,31 @@ def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every
         t0 = time.perf_counter()
         ctx.begin_substep(h, g)
         t1 = time.perf_counter()
-        acc, rho0 = ctx.warm_start(net, gate=gate)  # returns after the device work (it reads residuals)
+        acc, rho0 = ctx.warm_start(net, gate=gate, smoothing=smoothing)  # returns after the device work
         t_ws = time.perf_counter() - t1
         it_l, rho_l = ctx.iterate_to_tolerance(rel_tol, max_iterations, check_every)
         ctx.end_substep()
         ctx.synchronize()
         t_learned = time.perf_counter() - t0
         x, _ = ctx.state()
+        fallbacks = ctx.stats()["prediction_fallbacks"]
         row = {"step": t, "plain_iterations": it_p, "plain_rho": rho_p, "plain_ms": t_plain * 1e3,
                "learned_iterations": it_l, "learned_rho": rho_l, "learned_ms": t_learned * 1e3,
                "warm_start_ms": t_ws * 1e3,
-               "prediction_rho": float(rho0[0]), "accepted": bool(acc[0]), "finite": bool(np.isfinite(x).all())}
+               "prediction_rho": float(rho0[0]), "accepted": bool(acc[0]),
+               "fell_back": fallbacks > fallbacks_before, "finite": bool(np.isfinite(x).all())}
+        fallbacks_before = fallbacks
+        # advance the reference trajectory
+        ctx.restore_state()
+        ctx.begin_substep(h, g)
+        it_r, rho_r = ctx.iterate_to_tolerance(reference_tol, max(max_iterations, 4000), 10)
+        ctx.end_substep()
+        row["reference_iterations"], row["reference_rho"] = it_r, rho_r
         rows.append(row)
         if log:
-            log(f"step {t}: plain {it_p} it {t_plain * 1e3:.2f} ms | learned {it_l} it {t_learned * 1e3:.2f} ms, "
-                f"prediction rho {rho0[0]:.3e} accepted {bool(acc[0])}")
+            log(f"step {t}: plain {it_p} it {t_plain * 1e3:.2f} ms | learned {it_l} it {t_learned * 1e3:.2f} ms "
+                f"(warm start {t_ws * 1e3:.2f} ms), prediction rho {rho0[0]:.3e} accepted {bool(acc[0])} "
+                f"fell back {row['fell_back']} | reference {it_r} it rho {rho_r:.1e}")
     return rows
 
 
@@ -111,5 +128,6 @@ def summarize(rows, rel_tol=1e-4):
     lm = sum(r["learned_ms"] for r in rows)
     return {"steps": len(rows), "plain_iterations": pi, "learned_iterations": li,
             "plain_ms": pm, "learned_ms": lm, "iteration_ratio": pi / max(li, 1), "time_ratio": pm / max(lm, 1e-9),
-            "accepted": sum(r["accepted"] for r in rows),
-            "all_converged": all(r["plain_rho"] <= rel_tol and r["learned_rho"] <= rel_tol for r in rows)}
+            "accepted": sum(r["accepted"] for r in rows), "fell_back": sum(r["fell_back"] for r in rows),
+            "all_converged": all(r["plain_rho"] <= rel_tol and r["learned_rho"] <= rel_tol and r["finite"]
+                                 for r in rows)}

@@ -49,7 +49,7 @@ class VbdLaunch(ctypes.Structure):
 class VbdStats(ctypes.Structure):
     _fields_ = [("solve_launches", ctypes.c_int64), ("solve_ms", ctypes.c_double), ("vertex_updates", ctypes.c_int64),
                 ("algorithmic_bytes", ctypes.c_double), ("fp64_fallbacks", ctypes.c_int64),
-                ("skipped_blocks", ctypes.c_int64)]
+                ("skipped_blocks", ctypes.c_int64), ("prediction_fallbacks", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -122,7 +122,7 @@ def vbd_lib():
         _proto(L, "spg_mlp_loss_grad", ctypes.c_int, [c, _fp, _fp, i64, _dp, _fp])
         _proto(L, "spg_mlp_train", ctypes.c_int, [c, _fp, _fp, i64, i32, i32, ctypes.c_float, u64, i32, _dp])
         _proto(L, "spg_vbd_features", ctypes.c_int, [c, i32, _fp])
-        _proto(L, "spg_vbd_warm_start", ctypes.c_int, [c, c, i32, _ip, _dp])
+        _proto(L, "spg_vbd_warm_start", ctypes.c_int, [c, c, i32, i32, _ip, _dp])
         _vbd = L
     return _vbd
 
@@ -460,11 +460,12 @@ class VbdContext:
         self._check(self._L.spg_vbd_features(self.h, obj, _fl(out)))
         return out
 
-    def warm_start(self, net, gate=True):
-        """Start the open substep from the network's prediction; returns (accepted, rho) per object."""
+    def warm_start(self, net, gate=True, smoothing=0):
+        """Start the open substep from the network's prediction (after `smoothing` one-ring averaging
+        passes); returns (accepted, rho) per object."""
         acc = np.zeros(self.n_objects, dtype=np.int32)
         rho = np.zeros(self.n_objects)
-        self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(bool(gate)), _i(acc), _d(rho)))
+        self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(smoothing), int(bool(gate)), _i(acc), _d(rho)))
         return acc.astype(bool), rho
 
     def launch(self):

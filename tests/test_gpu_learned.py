@@ -207,3 +207,56 @@ def test_trained_network_needs_fewer_iterations():
     assert sm["all_converged"]
     assert all(r["finite"] for r in rows)
     assert sm["learned_iterations"] < sm["plain_iterations"]
+
+
+def test_iterate_to_tolerance_reports_the_residual_ratio():
+    s = scenes.cloth(24, young=1e4, pin="top_row", seed=5, iterations=5)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.begin_substep(s.dt, s.gravity)
+    it, rho = ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=3)
+    assert rho <= 1e-4 and it % 3 == 0 and it > 0
+    assert rho == pytest.approx(ctx.residual()[0] / ctx.residual_at_inertial()[0], rel=1e-12)
+    ctx.end_substep()
+
+
+def test_iterate_to_tolerance_drops_a_prediction_that_is_worse_than_x_tilde():
+    """An accepted start whose residual is above x~'s goes back to x~ at the first check and the solve
+    continues exactly as plain VBD (bitwise the same state and iteration count)."""
+    s = scenes.cloth(24, young=1e4, pin="top_row", seed=5, iterations=5)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.save_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    it_p, rho_p = ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=1)
+    ctx.end_substep()
+    plain = ctx.state()
+    ctx.restore_state()
+    ctx.reset_stats()
+    ctx.begin_substep(s.dt, s.gravity)
+    acc, rho0 = ctx.warm_start(_constant_net([0.0, 50.0, 0.0]), gate=False)
+    assert acc[0] and rho0[0] > 1.0
+    it_l, rho_l = ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=1)
+    ctx.end_substep()
+    assert ctx.stats()["prediction_fallbacks"] == 1
+    assert (it_l, rho_l) == (it_p, rho_p)
+    x, v = ctx.state()
+    assert np.array_equal(x, plain[0]) and np.array_equal(v, plain[1])
+
+
+def test_warm_start_smoothing_matches_restatement():
+    """x = x~ + s * (one-ring averaged)^k network output, against numpy on the host forward pass."""
+    sc = scenes.cloth(14, young=5e3, pin="top_row", seed=8, iterations=5)
+    ctx = native.VbdContext([sc.vbd_object()])
+    for _ in range(2):
+        ctx.step(sc.dt, 1, 5, sc.gravity)
+    ctx.begin_substep(sc.dt, sc.gravity)
+    xt, _ = ctx.state()
+    net = native.Mlp(seed=7)
+    y = net.forward(ctx.features(0)).astype(np.float64)
+    nb, _, _ = mo.static_tables(sc)
+    for _ in range(2):
+        y = np.stack([(y[i] + y[j].sum(axis=0)) / (1 + len(j)) for i, j in enumerate(nb)])
+    ctx.warm_start(net, gate=False, smoothing=2)
+    x, _ = ctx.state()
+    s = sc.dt ** 2 * np.linalg.norm(sc.gravity)
+    assert np.allclose(x, xt + s * y, rtol=0, atol=2e-6)
+    ctx.end_substep()

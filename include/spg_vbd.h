@@ -165,8 +165,9 @@ int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_i
 /* Launch configuration and autotuning. */
 int spg_vbd_get_launch(const spg_vbd_ctx *ctx, spg_vbd_launch *cfg);
 int spg_vbd_set_launch(spg_vbd_ctx *ctx, const spg_vbd_launch *cfg);
-/* Times the candidate shapes on this context's data (state is restored afterwards) and keeps the
-   fastest. trials: sweeps timed per candidate. */
+/* Times the candidate shapes (lanes_per_vertex x block_size x lds_staging, where the staged rows fit)
+   on this context's data (state is restored afterwards) and keeps the fastest. trials: sweeps per timed
+   run; each candidate is timed twice and its faster run counts. */
 int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best);
 
 int spg_vbd_get_stats(const spg_vbd_ctx *ctx, spg_vbd_stats *stats);

@@ -62,16 +62,16 @@ def training_scenes(count=256, seed=7, iterations=15):
 
 
 def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every=1, gate=True, smoothing=2,
-            launch=None, device=0, reference_tol=1e-5, log=None):
+            launch=None, device=0, reference_tol=1e-5, reference_iterations=2000, log=None):
     """Per substep of `scene`'s trajectory: iterations and wall time to reach rho <= rel_tol from the
     inertial guess (plain VBD) and from the network's gated prediction (learned), both from the same
     state (device checkpoint). The learned path keeps plain VBD as the fallback: a rejected prediction
     starts from x~, and an accepted one whose residual later rises above x~'s is dropped for x~ inside
     iterate_to_tolerance (counted in fell_back, the wasted iterations included).
 
-    Both paths are measured from the states of one reference trajectory, advanced by plain VBD to the
-    tighter reference_tol, so that neither path's tolerance-sized error steers the states the other is
-    measured on."""
+    Both paths are measured from the states of one reference trajectory, advanced by plain VBD towards
+    the tighter reference_tol (at most reference_iterations sweeps), so that neither path's
+    tolerance-sized error steers the states the other is measured on."""
     ctx = native.VbdContext([scene.vbd_object()], device=device)
     if launch:
         ctx.set_launch(**launch)
@@ -110,7 +110,7 @@ def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every
         # advance the reference trajectory
         ctx.restore_state()
         ctx.begin_substep(h, g)
-        it_r, rho_r = ctx.iterate_to_tolerance(reference_tol, max(max_iterations, 4000), 10)
+        it_r, rho_r = ctx.iterate_to_tolerance(reference_tol, reference_iterations, 10)
         ctx.end_substep()
         row["reference_iterations"], row["reference_rho"] = it_r, rho_r
         rows.append(row)

@@ -346,3 +346,59 @@ def test_substep_protocol_errors():
     assert ei.value.code == native.ERR_STATE
     with pytest.raises(native.NativeError):
         ctx.residual()
+
+
+def _anchored_particle(m, k, offset):
+    x = np.array([[0.2, 1.0, -0.3]])
+    a = x[0] + np.asarray(offset)
+    return scenes.Scene(x=x, x0=x.copy(), m=np.array([m]), iterations=1, gravity=(0.0, 0.0, 0.0),
+                        energies=[("anchor", np.array([[0]], np.int32), np.array([[*a, k]]))])
+
+
+@pytest.mark.parametrize("det_h,path", [(1e-3, "fp32"), (1.5e-10, "fp64"), (0.9e-10, "skip")])
+def test_block_solve_threshold_kat(det_h, path):
+    """Host-forced blocks around the reference's det(H) > 1e-10 branch (vbd.cpp:139-146). An anchored
+    particle with no gravity and v = 0 has x~ = x, H = (m/h^2 + k) I and f = k (a - x), so one sweep
+    gives x + k (a - x) / (m/h^2 + k) when det H = (m/h^2 + k)^3 > 1e-10 and leaves x otherwise.
+    Blocks within a factor 2 of the threshold are decided and solved in FP64."""
+    h = scenes.DT
+    lam = det_h ** (1.0 / 3.0)
+    k = 0.4 * lam
+    s = _anchored_particle((lam - k) * h * h, k, [0.3, -0.2, 0.1])
+    ctx = gpu_context(s)
+    ctx.reset_stats()
+    o = oracle_from_gpu_start(s, ctx)
+    ctx.step(h, 1, 1, s.gravity)
+    x = ctx.state()[0][0]
+    st = ctx.stats()
+    x0 = s.x[0].astype(np.float32).astype(np.float64)  # the solver's FP32 start
+    a = s.x[0] + np.array([0.3, -0.2, 0.1])             # the anchor as given (FP64)
+    expect = x0 + k * (a - x0) / lam if path != "skip" else x0
+    o.step(h, 1, 1, s.gravity)
+    assert np.allclose(o.state()[0][0], expect, rtol=0, atol=1e-12)
+    assert np.allclose(x, expect, rtol=0, atol=2e-7)
+    assert st["fp64_fallbacks"] == (1 if path == "fp64" else 0)
+    assert st["skipped_blocks"] == (1 if path == "skip" else 0)
+
+
+def test_ill_conditioned_block_takes_fp64_path(oracle_threads):
+    """A light particle on a stiff spring at rest length: H = (m/h^2) I + k s s^T has condition
+    ~k h^2 / m = 1e5, beyond the FP32 solve's 1e4 limit, so the block is solved in FP64. The step
+    matches the FP64 oracle."""
+    h = scenes.DT
+    x = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]])
+    x1 = x.copy()
+    x1[1] += [2e-3, 1e-3, -5e-4]
+    s = scenes.Scene(x=x1, x0=x, m=np.array([1.0, 1e-2 * h * h]), iterations=3, gravity=(0.0, -9.81, 0.0),
+                     energies=[("spring", np.array([[0, 1]], np.int32), np.array([[1.0, 1e3]])),
+                               ("anchor", np.array([[0]], np.int32), np.array([[0.0, 0.0, 0.0, 1e8]]))])
+    ctx = gpu_context(s)
+    ctx.reset_stats()
+    o = oracle_from_gpu_start(s, ctx)
+    start = ctx.state()[0]
+    ctx.step(h, 1, s.iterations, s.gravity)
+    o.step(h, 1, s.iterations, s.gravity)
+    assert ctx.stats()["fp64_fallbacks"] >= 1
+    e = disp_rel_err(ctx.state()[0], o.state()[0], start)
+    print(f"ill-conditioned spring block: displacement-relative error {e:.2e}", flush=True)
+    assert e < 1e-5

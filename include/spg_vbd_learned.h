@@ -74,10 +74,13 @@ int spg_mlp_train(spg_mlp *net, const float *features, const float *targets, int
 int spg_vbd_features(spg_vbd_ctx *ctx, int32_t object, float *features);
 /* Replaces the open substep's start x~ by the network's prediction x~ + s y. The predicted displacement
  * y is first averaged `smoothing` times over each vertex's one ring (y_i <- (y_i + sum_j y_j) / (1 + deg),
- * 0..64 passes), which removes vertex-to-vertex noise of the prediction and keeps its smooth part. With
- * gate != 0 each object keeps the prediction only if rho = |r(pred)| / |r(x~)| <= 1 (FP64 residual of
- * spg_vbd_residual) and goes back to x~ otherwise. accepted and rho (may be NULL) receive one value per
- * object. */
+ * 0..64 passes), which removes vertex-to-vertex noise of the prediction and keeps its smooth part.
+ * gate 0: the prediction is kept. gate 1: each object keeps it only if rho = |r(pred)| / |r(x~)| <= 1
+ * (FP64 residual of spg_vbd_residual) and goes back to x~ otherwise, before this call returns. gate 2
+ * (deferred): the same test is the first convergence check of the next spg_vbd_iterate_to_tolerance,
+ * which computes it on its check stream while the first sweeps from the prediction run; this call then
+ * returns without synchronising and reports rho as NaN (rejections appear in
+ * spg_vbd_stats.prediction_fallbacks). accepted and rho (may be NULL) receive one value per object. */
 int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t smoothing, int32_t gate, int32_t *accepted,
                        double *rho);
 

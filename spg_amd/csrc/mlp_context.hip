@@ -284,6 +284,20 @@ void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, in
     net->infer(features, n, y, ldy, stream);
 }
 
+void mlpInferAfter(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipEvent_t ready,
+                   hipEvent_t done)
+{
+    if (ldy < kMlpOut) throw VbdError(SPG_VBD_ERR_INVALID, "output stride below 3");
+    if (n > (int64_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many rows");
+    hipCheck(hipSetDevice(net->device), "hipSetDevice");
+    hipCheck(hipStreamWaitEvent(net->stream, ready, 0), "hipStreamWaitEvent");
+    hipCheck(launchMlpInfer(features, static_cast<int>(n), net->mean.p, net->invStd.p, net->frag.p, net->P.p + net->bOff[0],
+                            net->P.p + net->bOff[1], net->P.p + net->bOff[2], net->P.p + net->bOff[3], y, ldy,
+                            net->maxBlocks, net->stream),
+             "inference launch");
+    hipCheck(hipEventRecord(done, net->stream), "hipEventRecord");
+}
+
 int mlpDevice(const spg_mlp *net) { return net->device; }
 
 }  // namespace spgvbd

@@ -87,6 +87,10 @@ constexpr int kMlpInferCols = 16;
 // ldy >= kMlpOut), ordered on `stream`, with the fused kernel. Throws VbdError. Defined with the
 // network object.
 void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipStream_t stream);
+// The same on the network's own stream, after `ready` (recorded by the caller); records `done`, which
+// the caller waits on before it reads y.
+void mlpInferAfter(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipEvent_t ready,
+                   hipEvent_t done);
 int mlpDevice(const spg_mlp *net);
 
 }  // namespace spgvbd

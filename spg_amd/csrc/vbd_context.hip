@@ -97,11 +97,24 @@ struct spg_vbd_ctx {
     DevBuf<float> mass;
     DevBuf<double> massd;  // FP64 masses for the residual
     DevBuf<unsigned long long> counters;
-    DevBuf<double> r2, norms;
+    DevBuf<double> r2, norms, normsAux;
+    // pipelined convergence checks of iterate_to_tolerance: a second stream computes the residual of
+    // position snapshots while the solver's stream keeps sweeping
+    // Two units in flight: the next sweeps are queued while the host reads a check (hiding the host
+    // round trip). Deeper speculation does not pay: the residual and the sweeps each fill the GPU, so
+    // sweeps ahead of a passing check only delay it.
+    static constexpr int kPipe = 2;
+    hipStream_t checkStream = nullptr;
+    DevBuf<float4> snap;                      // kPipe x nv positions
+    DevBuf<double> r2Check, partialCheck, normsCheck;  // normsCheck: kPipe x nObjects
+    double *hostNormsCheck = nullptr;         // pinned, (kPipe + 1) x nObjects; the last row is |r(x~)|
+    hipEvent_t evSnap[kPipe] = {}, evCheck[kPipe] = {}, evInertialCheck = nullptr;
+    hipEvent_t evFeatures = nullptr, evPrediction = nullptr;  // learned start: features ready, prediction ready
     DevBuf<int32_t> objSlots;
     DevBuf<int64_t> objOffsets;
-    // segments of the per-object residual reduction: objSlots[segBegin[k] .. segBegin[k + 1]) belongs to
-    // one object; object o owns segments objSeg[o] .. objSeg[o + 1]
+    // segments of the per-object residual reduction over the particle-ordered squared residuals:
+    // particles segBegin[k] .. segBegin[k + 1] - 1 belong to one object; object o owns segments
+    // objSeg[o] .. objSeg[o + 1] - 1
     DevBuf<int64_t> segBegin;
     DevBuf<int32_t> objSeg;
     DevBuf<double> segPartial;
@@ -111,6 +124,7 @@ struct spg_vbd_ctx {
     std::vector<double> rCurCache;  // |r(x)| per object at the current positions
     bool rCurValid = false;
     bool posIsInertial = false;  // x == x~ (right after begin_substep)
+    bool r0OnCheckStream = false;  // |r(x~)| enqueued on the check stream, not yet read back
 
     // incidence streams
     // row offsets (CSR, used by the residual; anchors also by the solve)
@@ -165,6 +179,16 @@ struct spg_vbd_ctx {
     {
         if (stream) (void)hipStreamSynchronize(stream);
         dropGraph();
+        if (evFeatures) (void)hipEventDestroy(evFeatures);
+        if (evPrediction) (void)hipEventDestroy(evPrediction);
+        if (checkStream) (void)hipStreamSynchronize(checkStream);
+        for (int k = 0; k < kPipe; ++k) {
+            if (evSnap[k]) (void)hipEventDestroy(evSnap[k]);
+            if (evCheck[k]) (void)hipEventDestroy(evCheck[k]);
+        }
+        if (evInertialCheck) (void)hipEventDestroy(evInertialCheck);
+        if (hostNormsCheck) (void)hipHostFree(hostNormsCheck);
+        if (checkStream) (void)hipStreamDestroy(checkStream);
         for (auto &e : events) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
@@ -273,12 +297,23 @@ struct spg_vbd_ctx {
             }
     }
 
+    // A convergence check still in flight from an earlier iterate_to_tolerance reads x~: the next
+    // substep's writes to x~ are ordered after it (on the device, no host wait). Called outside graph
+    // capture, before a new substep is enqueued.
+    void orderAfterChecks()
+    {
+        for (int k = 0; checkStream && k < kPipe; ++k)
+            hipCheck(hipStreamWaitEvent(stream, evCheck[k], 0), "hipStreamWaitEvent");
+        if (evInertialCheck) hipCheck(hipStreamWaitEvent(stream, evInertialCheck, 0), "hipStreamWaitEvent");
+    }
+
     void beginSubstep(double hh, const double *g)
     {
         h = hh;
         r0Valid = false;
         rCurValid = false;
         posIsInertial = true;
+        r0OnCheckStream = false;
         anyPredicted = false;
         predicted.assign(nObjects, 0);
         for (int k = 0; k < 3; ++k) this->g[k] = g[k];
@@ -298,22 +333,18 @@ struct spg_vbd_ctx {
         inSubstep = false;
     }
 
-    // Per-object FP64 |r| at x~ or at the current positions. Both are cached until they can change:
-    // x~ is fixed for the whole substep, x until the next sweep or position write.
-    void residual(bool atInertial, double *out)
+    // Launches the FP64 residual at x~ or at x and its per-object reduction into normsDev (device, one
+    // double per object), on the context's stream, without waiting.
+    void enqueueResidual(bool atInertial, double *normsDev)
     {
-        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
-        if (!atInertial && posIsInertial) atInertial = true;  // x == x~: the same positions
-        if (atInertial && r0Valid) {
-            std::copy(r0Cache.begin(), r0Cache.end(), out);
-            return;
-        }
-        if (!atInertial && rCurValid) {
-            std::copy(rCurCache.begin(), rCurCache.end(), out);
-            return;
-        }
+        enqueueResidualOf(atInertial ? xt.p : pos.p, r2.p, segPartial.p, normsDev, stream);
+    }
+
+    // The same for arbitrary positions, per-vertex and per-segment work buffers and stream.
+    void enqueueResidualOf(const float4 *positions, double *r2Buf, double *partialBuf, double *normsDev, hipStream_t s)
+    {
         ResidualArgs a{};
-        a.pos = atInertial ? xt.p : pos.p;
+        a.pos = positions;
         a.xt = xt.p;
         a.mass = massd.p;
         a.invdt2 = 1.0 / (h * h);
@@ -339,11 +370,34 @@ struct spg_vbd_ctx {
         a.tetElem = tetElem.p;
         a.tetV = tetV.p;
         a.tetP = tetP.p;
-        a.r2 = r2.p;
-        hipCheck(launchResidual(a, stream), "residual launch");
-        hipCheck(launchReduceObjects(r2.p, objSlots.p, segBegin.p, nSegments, objSeg.p, nObjects, segPartial.p, norms.p,
-                                     stream),
+        a.order = objSlots.p;
+        a.r2 = r2Buf;
+        hipCheck(launchResidual(a, s), "residual launch");
+        hipCheck(launchReduceObjects(r2Buf, segBegin.p, nSegments, objSeg.p, nObjects, partialBuf, normsDev, s),
                  "reduce launch");
+    }
+
+    // Per-object FP64 |r| at x~ or at the current positions. Both are cached until they can change:
+    // x~ is fixed for the whole substep, x until the next sweep or position write.
+    void residual(bool atInertial, double *out)
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
+        if (!atInertial && posIsInertial) atInertial = true;  // x == x~: the same positions
+        if (atInertial && !r0Valid && r0OnCheckStream) {
+            hipCheck(hipStreamSynchronize(checkStream), "hipStreamSynchronize");
+            r0Cache.assign(hostNormsCheck + kPipe * nObjects, hostNormsCheck + (kPipe + 1) * nObjects);
+            r0Valid = true;
+            r0OnCheckStream = false;
+        }
+        if (atInertial && r0Valid) {
+            std::copy(r0Cache.begin(), r0Cache.end(), out);
+            return;
+        }
+        if (!atInertial && rCurValid) {
+            std::copy(rCurCache.begin(), rCurCache.end(), out);
+            return;
+        }
+        enqueueResidual(atInertial, norms.p);
         hipCheck(hipMemcpyAsync(out, norms.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
         if (atInertial) {
@@ -411,17 +465,33 @@ struct spg_vbd_ctx {
         return any;
     }
 
-    void warmStart(spg_mlp *net, int32_t smoothing, bool gate, int32_t *accepted, double *rho)
+    // gateMode 0: keep the prediction; 1: gate now (two FP64 residuals); 2: keep it and let the first
+    // check of iterate_to_tolerance apply the same gate (no residual here; rho is reported as NaN).
+    void warmStart(spg_mlp *net, int32_t smoothing, int32_t gateMode, int32_t *accepted, double *rho)
     {
+        const bool gate = gateMode == 1;
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
         if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
         if (smoothing < 0 || smoothing > 64) throw VbdError(SPG_VBD_ERR_INVALID, "smoothing passes must be in 0..64");
-        computeFeatures();
+        if (!evFeatures) {
+            hipCheck(hipEventCreateWithFlags(&evFeatures, hipEventDisableTiming), "hipEventCreate");
+            hipCheck(hipEventCreateWithFlags(&evPrediction, hipEventDisableTiming), "hipEventCreate");
+        }
         pred.reserve(static_cast<size_t>(nv) * 8);  // two ping-pong rows of 4 floats per vertex
         float *y = pred.p, *ySpare = pred.p + static_cast<size_t>(nv) * 4;
         hipCheck(hipMemsetAsync(y, 0, sizeof(float) * 4 * nv, stream), "memset");
-        mlpInferDevice(net, feat.p, nv, y, 4, stream);
+        computeFeatures();
+        hipCheck(hipEventRecord(evFeatures, stream), "hipEventRecord");
+        // the inference runs on the network's stream while this stream computes |r(x~)| for the gate
+        mlpInferAfter(net, feat.p, nv, y, 4, evFeatures, evPrediction);
         hipCheck(hipSetDevice(device), "hipSetDevice");
+        const bool deferGate = gateMode == 2;
+        const bool needResiduals = gateMode == 1 || (gateMode == 0 && rho);
+        const bool needR0 = needResiduals && !r0Valid && !r0OnCheckStream;
+        // the deferred gate needs |r(x~)| in the first check: start it now, beside the inference
+        if (deferGate && !r0Valid && !r0OnCheckStream) enqueueInertialResidualOnCheckStream();
+        if (needR0) enqueueResidual(true, normsAux.p);
+        hipCheck(hipStreamWaitEvent(stream, evPrediction, 0), "hipStreamWaitEvent");
         for (int32_t k = 0; k < smoothing; ++k) {
             hipCheck(launchSmoothPrediction(y, ySpare, nbPtr.p, nbIdx.p, static_cast<int32_t>(nv), stream), "smooth");
             std::swap(y, ySpare);
@@ -432,9 +502,22 @@ struct spg_vbd_ctx {
         positionsChanged();
         std::vector<int32_t> rej(nObjects, 0);
         std::vector<double> r0(nObjects, 0.0), rp(nObjects, 0.0);
-        if (gate || rho) {
-            residual(true, r0.data());
-            residual(false, rp.data());
+        if (needResiduals) {  // |r(pred)| after the inference, one synchronisation for both norms
+            enqueueResidual(false, norms.p);
+            if (needR0)
+                hipCheck(hipMemcpyAsync(r0.data(), normsAux.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream),
+                         "download");
+            hipCheck(hipMemcpyAsync(rp.data(), norms.p, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream),
+                     "download");
+            hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            if (needR0) {
+                r0Cache = r0;
+                r0Valid = true;
+            } else {
+                residual(true, r0.data());  // cached, or waiting for the check stream's copy
+            }
+            rCurCache = rp;
+            rCurValid = true;
         }
         if (gate) {
             bool any = false;
@@ -453,10 +536,145 @@ struct spg_vbd_ctx {
         for (int32_t o = 0; o < nObjects; ++o) {
             predicted[o] = !rej[o];
             if (accepted) accepted[o] = !rej[o];
-            if (rho) rho[o] = r0[o] > 0.0 ? rp[o] / r0[o] : rp[o];
+            if (rho)
+                rho[o] = needResiduals ? (r0[o] > 0.0 ? rp[o] / r0[o] : rp[o]) : std::numeric_limits<double>::quiet_NaN();
         }
         anyPredicted = std::find(predicted.begin(), predicted.end(), 1) != predicted.end();
-        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        // a deferred gate leaves the prediction queued on the stream for the iterations that follow
+        if (!deferGate) hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    // largest per-object |r| / |r(x~)|; NaN when any object's residual is not finite
+    double relMax(const std::vector<double> &r0, const std::vector<double> &r) const
+    {
+        double m = 0.0;
+        for (int32_t o = 0; o < nObjects; ++o) {
+            const double q = r0[o] > 0 ? r[o] / r0[o] : r[o];
+            if (!std::isfinite(q)) return std::numeric_limits<double>::quiet_NaN();
+            m = std::max(m, q);
+        }
+        return m;
+    }
+
+    void ensureCheckPipeline()
+    {
+        if (checkStream) return;
+        hipCheck(hipStreamCreateWithFlags(&checkStream, hipStreamNonBlocking), "hipStreamCreate");
+        for (int k = 0; k < kPipe; ++k) {
+            hipCheck(hipEventCreateWithFlags(&evSnap[k], hipEventDisableTiming), "hipEventCreate");
+            hipCheck(hipEventCreateWithFlags(&evCheck[k], hipEventDisableTiming), "hipEventCreate");
+        }
+        hipCheck(hipEventCreateWithFlags(&evInertialCheck, hipEventDisableTiming), "hipEventCreate");
+        snap.alloc(static_cast<size_t>(kPipe) * nv);
+        r2Check.alloc(nv);
+        partialCheck.alloc(nSegments);
+        normsCheck.alloc(static_cast<size_t>(kPipe) * nObjects);
+        hipCheck(hipHostMalloc(reinterpret_cast<void **>(&hostNormsCheck), sizeof(double) * (kPipe + 1) * nObjects),
+                 "hipHostMalloc");
+    }
+
+    // |r(x~)| on the check stream into the pinned row kPipe of hostNormsCheck (read after the next check).
+    void enqueueInertialResidualOnCheckStream()
+    {
+        hipCheck(hipEventRecord(evSnap[0], stream), "hipEventRecord");  // x~ is written by begin_substep
+        hipCheck(hipStreamWaitEvent(checkStream, evSnap[0], 0), "hipStreamWaitEvent");
+        enqueueResidualOf(xt.p, r2Check.p, partialCheck.p, normsAux.p, checkStream);
+        hipCheck(hipMemcpyAsync(hostNormsCheck + kPipe * nObjects, normsAux.p, sizeof(double) * nObjects,
+                                hipMemcpyDeviceToHost, checkStream),
+                 "download");
+        hipCheck(hipEventRecord(evInertialCheck, checkStream), "hipEventRecord");
+        r0OnCheckStream = true;
+    }
+
+    // Sweeps in units of `every` until max_j |r_j(x)| / |r_j(x~)| <= tol after a unit, or max sweeps.
+    // After each unit the positions are copied to a snapshot whose FP64 residual is computed on the
+    // check stream while the solver's stream goes on with the next units (at most kPipe in flight).
+    // When a check passes, the positions are restored from its snapshot, so the result (state, sweep
+    // count, rho) is that of checking synchronously after every unit. Unless the residual at the
+    // start is already known, the first unit has no sweeps: its check is that of the start (for a
+    // network prediction whose gate was deferred, this check is the gate), and |r(x~)| is computed on
+    // the check stream as well.
+    void iterateToTolerance(double tol, int32_t maxSweeps, int32_t every, int32_t *sweepsOut, double *rhoOut)
+    {
+        std::vector<double> r0(nObjects), r(nObjects);
+        const bool startKnown = rCurValid || (posIsInertial && r0Valid);
+        if (startKnown || maxSweeps == 0) {
+            residual(true, r0.data());
+            residual(false, r.data());
+            if (fallBackFromPredictions(r0, r)) residual(false, r.data());
+            const double cur = relMax(r0, r);
+            if (cur <= tol || maxSweeps == 0) {
+                *sweepsOut = 0;
+                *rhoOut = cur;
+                return;
+            }
+        }
+        ensureCheckPipeline();
+        bool firstUnitEmpty = !startKnown;
+        // |r(x~)|: known, equal to the first check (x == x~), or computed first on the check stream
+        const bool r0FromFirstCheck = !r0Valid && !r0OnCheckStream && firstUnitEmpty && posIsInertial;
+        if (r0Valid) r0 = r0Cache;
+        if (!r0Valid && !r0FromFirstCheck && !r0OnCheckStream) enqueueInertialResidualOnCheckStream();
+        bool r0Pending = !r0Valid;
+        std::vector<int32_t> sweepsAt{0};  // sweeps done after unit u: sweepsAt[u + 1]
+        int32_t issued = 0, checked = 0, done = 0;
+        const size_t nvs = static_cast<size_t>(nv);
+        for (;;) {
+            while (sweepsAt[issued] < maxSweeps && issued - checked < kPipe) {
+                const int32_t n = (issued == 0 && firstUnitEmpty) ? 0 : std::min(every, maxSweeps - sweepsAt[issued]);
+                sweepColors(n, timing);
+                sweepsAt.push_back(sweepsAt[issued] + n);
+                const int k = issued % kPipe;
+                // the last check that read this snapshot slot (this call or an earlier one) must be done
+                hipCheck(hipStreamWaitEvent(stream, evCheck[k], 0), "hipStreamWaitEvent");
+                hipCheck(launchCopy(pos.p, snap.p + k * nvs, nv, stream), "snapshot");
+                hipCheck(hipEventRecord(evSnap[k], stream), "hipEventRecord");
+                hipCheck(hipStreamWaitEvent(checkStream, evSnap[k], 0), "hipStreamWaitEvent");
+                enqueueResidualOf(snap.p + k * nvs, r2Check.p, partialCheck.p, normsCheck.p + k * nObjects, checkStream);
+                hipCheck(hipMemcpyAsync(hostNormsCheck + k * nObjects, normsCheck.p + k * nObjects,
+                                        sizeof(double) * nObjects, hipMemcpyDeviceToHost, checkStream),
+                         "download");
+                hipCheck(hipEventRecord(evCheck[k], checkStream), "hipEventRecord");
+                ++issued;
+            }
+            const int k = checked % kPipe;
+            hipCheck(hipEventSynchronize(evCheck[k]), "hipEventSynchronize");
+            std::copy(hostNormsCheck + k * nObjects, hostNormsCheck + (k + 1) * nObjects, r.begin());
+            if (r0Pending) {  // the check stream computed |r(x~)| before this first check
+                if (r0FromFirstCheck)
+                    r0 = r;  // the first check is at x == x~
+                else
+                    std::copy(hostNormsCheck + kPipe * nObjects, hostNormsCheck + (kPipe + 1) * nObjects, r0.begin());
+                r0Cache = r0;
+                r0Valid = true;
+                r0Pending = false;
+                r0OnCheckStream = false;
+            }
+            ++checked;
+            bool fallBack = false;
+            for (int32_t o = 0; anyPredicted && o < nObjects; ++o) fallBack = fallBack || (predicted[o] && !(r[o] <= r0[o]));
+            const bool stop = fallBack || relMax(r0, r) <= tol || sweepsAt[checked] >= maxSweeps;
+            if (!stop) continue;
+            // the solver's stream may be ahead of this unit: go back to its snapshot (a check still in
+            // flight for a later unit finishes on its own; its slot is guarded by evCheck)
+            if (issued > checked) hipCheck(launchCopy(snap.p + k * nvs, pos.p, nv, stream), "rollback");
+            hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+            positionsChanged();
+            rCurCache = r;
+            rCurValid = true;
+            done = sweepsAt[checked];
+            issued = checked;
+            sweepsAt.resize(checked + 1);
+            firstUnitEmpty = false;
+            if (fallBack) {  // predicted objects above x~ go back to x~; the pipeline refills from there
+                fallBackFromPredictions(r0, r);
+                residual(false, r.data());
+                if (!(relMax(r0, r) <= tol) && done < maxSweeps) continue;
+            }
+            break;
+        }
+        *sweepsOut = done;
+        *rhoOut = relMax(r0, r);
     }
 
     void enqueueStep(double dt, int32_t nsub, int32_t iterations, const double *g, bool timed)
@@ -473,6 +691,7 @@ struct spg_vbd_ctx {
     {
         if (inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "step called inside an open substep");
         if (!(dt > 0) || nsub < 1 || iterations < 0) throw VbdError(SPG_VBD_ERR_INVALID, "invalid dt/substeps/iterations");
+        orderAfterChecks();
         if (launch.use_graph && !timing) {
             const std::array<double, 6> key{dt / nsub, g[0], g[1], g[2], double(iterations), double(nsub)};
             const int lh = launch.lanes_per_vertex * 100000 + launch.block_size * 10 + launch.lds_staging;
@@ -681,6 +900,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     c.counters.upload(std::vector<unsigned long long>(2, 0ull), c.stream);
     c.r2.alloc(c.nv);
     c.norms.alloc(nObjects);
+    c.normsAux.alloc(nObjects);
 
     // incidence streams per type (VBD::computeStencilInfoPerVertex, vbd.cpp:231-268)
     std::array<std::vector<Incidence>, 6> inc;
@@ -902,6 +1122,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                           colorRecordBytes[k];
     hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
     buildFeatureTables(c, objs, nObjects);
+    c.ensureCheckPipeline();
 }
 
 // Static inputs of the learned warm start: one-ring neighbours (vertices sharing a triangle,
@@ -1182,6 +1403,7 @@ int spg_vbd_begin_substep(spg_vbd_ctx *ctx, double h, const double gravity[3])
         if (ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "substep already open");
         if (!(h > 0)) throw VbdError(SPG_VBD_ERR_INVALID, "invalid h");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->orderAfterChecks();
         ctx->beginSubstep(h, gravity);
     });
 }
@@ -1273,32 +1495,7 @@ int spg_vbd_iterate_to_tolerance(spg_vbd_ctx *ctx, double rel_tol, int32_t max_i
         if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "iterate_to_tolerance needs an open substep");
         if (max_iterations < 0 || check_every < 1) throw VbdError(SPG_VBD_ERR_INVALID, "invalid iteration counts");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
-        std::vector<double> r0(ctx->nObjects), r(ctx->nObjects);
-        ctx->residual(true, r0.data());
-        // largest per-object relative residual; NaN when any object's residual is not finite
-        auto relMax = [&] {
-            double m = 0.0;
-            for (int32_t o = 0; o < ctx->nObjects; ++o) {
-                const double q = r0[o] > 0 ? r[o] / r0[o] : r[o];
-                if (!std::isfinite(q)) return std::numeric_limits<double>::quiet_NaN();
-                m = std::max(m, q);
-            }
-            return m;
-        };
-        ctx->residual(false, r.data());
-        ctx->fallBackFromPredictions(r0, r);
-        double cur = relMax();
-        int32_t done = 0;
-        while (!(cur <= rel_tol) && done < max_iterations) {
-            const int32_t n = std::min(check_every, max_iterations - done);
-            ctx->sweepColors(n, ctx->timing);
-            done += n;
-            ctx->residual(false, r.data());
-            if (ctx->fallBackFromPredictions(r0, r)) ctx->residual(false, r.data());
-            cur = relMax();
-        }
-        *iterations = done;
-        *rho = cur;
+        ctx->iterateToTolerance(rel_tol, max_iterations, check_every, iterations, rho);
     });
 }
 
@@ -1326,7 +1523,8 @@ int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t smoothing, int32_
     return guarded(ctx, [&] {
         if (!ctx || !net) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
-        ctx->warmStart(net, smoothing, gate != 0, accepted, rho);
+        if (gate < 0 || gate > 2) throw VbdError(SPG_VBD_ERR_INVALID, "gate must be 0, 1 or 2");
+        ctx->warmStart(net, smoothing, gate, accepted, rho);
     });
 }
 
@@ -1370,6 +1568,7 @@ int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best)
         hipCheck(hipMemcpyAsync(svel.p, ctx->vel.p, sizeof(float4) * ctx->nv, hipMemcpyDeviceToDevice, ctx->stream), "save");
         const spg_vbd_launch keep = ctx->launch;
         const double g[3] = {0.0, -9.81, 0.0};
+        ctx->orderAfterChecks();
         ctx->beginSubstep(1.0 / 60.0, g);
         hipEvent_t e0, e1;
         hipCheck(hipEventCreate(&e0), "hipEventCreate");

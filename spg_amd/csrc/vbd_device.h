@@ -87,7 +87,8 @@ struct ResidualArgs {
     const int32_t *tetElem;
     const int4 *tetV;
     const double *tetP;   // 12 per element: Dinv (9), lambda, mu, alpha
-    double *r2;           // out: |r_v|^2 per slot
+    const int32_t *order; // slot of the i-th particle (objects in order, particles ascending)
+    double *r2;           // out: |r_v|^2 of the i-th particle (particle order)
 };
 
 // kernel launchers (vbd_kernels.hip); all asynchronous on `stream`
@@ -104,9 +105,9 @@ hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSiz
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream);
 // entries of one object's slot list summed by one block of the per-object residual reduction
 constexpr int64_t kReduceSegment = 4096;
-hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *segBegin, int32_t nSegments,
-                               const int32_t *objSeg, int32_t nObjects, double *partial, double *out,
-                               hipStream_t stream);
+// r2 in particle order; segment k sums r2[segBegin[k] .. segBegin[k + 1])
+hipError_t launchReduceObjects(const double *r2, const int64_t *segBegin, int32_t nSegments, const int32_t *objSeg,
+                               int32_t nObjects, double *partial, double *out, hipStream_t stream);
 hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t stream);
 hipError_t launchCopy(const float4 *src, float4 *dst, int64_t n, hipStream_t stream);
 hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,

@@ -332,8 +332,11 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
     const int32_t lane64 = threadIdx.x & 63;
     const int32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int32_t grp = lane64 / G, lig = lane64 - grp * G;
-    const int32_t vid = wave * GPW + grp;
-    const bool active = vid < a.n;
+    // particles are taken in particle order (neighbours, and the elements they share, are processed
+    // close together in time); their rows and positions are addressed by slot
+    const int32_t gi = wave * GPW + grp;
+    const bool active = gi < a.n;
+    const int32_t vid = active ? a.order[gi] : 0;
     V3d r = {0.0, 0.0, 0.0};
     if (active) {
         const float4 xf = a.pos[vid], tf = a.xt[vid];
@@ -415,21 +418,20 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
     r.x = groupReduceD<G>(r.x, lane64, lig);
     r.y = groupReduceD<G>(r.y, lane64, lig);
     r.z = groupReduceD<G>(r.z, lane64, lig);
-    if (active && lig == 0) a.r2[vid] = dot(r, r);
+    if (active && lig == 0) a.r2[gi] = dot(r, r);
 }
 
 // Per-object |r| in two deterministic stages. Stage 1: block b sums the squared residuals of segment b
-// (at most kReduceSegment entries of one object's slot list) in a fixed tree order. Stage 2: one
+// (at most kReduceSegment consecutive particles of one object) in a fixed tree order. Stage 2: one
 // thread per object sums its segments' partials in segment order and takes the square root.
 __global__ void __launch_bounds__(256) reduceSegmentsKernel(const double *__restrict__ r2,
-                                                            const int32_t *__restrict__ objSlots,
                                                             const int64_t *__restrict__ segBegin,
                                                             double *__restrict__ partial)
 {
     __shared__ double part[256];
     const int64_t b = segBegin[blockIdx.x], e = segBegin[blockIdx.x + 1];
     double s = 0.0;
-    for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) s += r2[objSlots[j]];
+    for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) s += r2[j];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -561,12 +563,11 @@ hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launchReduceObjects(const double *r2, const int32_t *objSlots, const int64_t *segBegin, int32_t nSegments,
-                               const int32_t *objSeg, int32_t nObjects, double *partial, double *out,
-                               hipStream_t stream)
+hipError_t launchReduceObjects(const double *r2, const int64_t *segBegin, int32_t nSegments, const int32_t *objSeg,
+                               int32_t nObjects, double *partial, double *out, hipStream_t stream)
 {
     if (nObjects <= 0 || nSegments <= 0) return hipSuccess;
-    reduceSegmentsKernel<<<nSegments, 256, 0, stream>>>(r2, objSlots, segBegin, partial);
+    reduceSegmentsKernel<<<nSegments, 256, 0, stream>>>(r2, segBegin, partial);
     reduceObjectsKernel<<<blocksFor(nObjects, 256), 256, 0, stream>>>(partial, objSeg, nObjects, out);
     return hipGetLastError();
 }

@@ -61,13 +61,14 @@ def training_scenes(count=256, seed=7, iterations=15):
     return scenes.cloth_batch(count=count, seed=seed, iterations=iterations)
 
 
-def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every=1, gate=True, smoothing=2,
+def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every=1, gate="deferred", smoothing=2,
             launch=None, device=0, reference_tol=1e-5, reference_iterations=2000, log=None):
     """Per substep of `scene`'s trajectory: iterations and wall time to reach rho <= rel_tol from the
     inertial guess (plain VBD) and from the network's gated prediction (learned), both from the same
-    state (device checkpoint). The learned path keeps plain VBD as the fallback: a rejected prediction
-    starts from x~, and an accepted one whose residual later rises above x~'s is dropped for x~ inside
-    iterate_to_tolerance (counted in fell_back, the wasted iterations included).
+    state (device checkpoint). The learned path keeps plain VBD as the fallback: the gate (by default
+    deferred to the first convergence check) sends a prediction whose residual is above x~'s back to
+    x~, and an accepted one whose residual later rises above x~'s is dropped for x~ as well (both
+    counted in fell_back, the wasted iterations included).
 
     Both paths are measured from the states of one reference trajectory, advanced by plain VBD towards
     the tighter reference_tol (at most reference_iterations sweeps), so that neither path's
@@ -93,7 +94,7 @@ def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every
         t0 = time.perf_counter()
         ctx.begin_substep(h, g)
         t1 = time.perf_counter()
-        acc, rho0 = ctx.warm_start(net, gate=gate, smoothing=smoothing)  # returns after the device work
+        acc, rho0 = ctx.warm_start(net, gate=gate, smoothing=smoothing)
         t_ws = time.perf_counter() - t1
         it_l, rho_l = ctx.iterate_to_tolerance(rel_tol, max_iterations, check_every)
         ctx.end_substep()
@@ -104,7 +105,7 @@ def compare(scene, net, steps=10, rel_tol=1e-4, max_iterations=3000, check_every
         row = {"step": t, "plain_iterations": it_p, "plain_rho": rho_p, "plain_ms": t_plain * 1e3,
                "learned_iterations": it_l, "learned_rho": rho_l, "learned_ms": t_learned * 1e3,
                "warm_start_ms": t_ws * 1e3,
-               "prediction_rho": float(rho0[0]), "accepted": bool(acc[0]),
+               "prediction_rho": float(rho0[0]) if np.isfinite(rho0[0]) else None, "accepted": bool(acc[0]),
                "fell_back": fallbacks > fallbacks_before, "finite": bool(np.isfinite(x).all())}
         fallbacks_before = fallbacks
         # advance the reference trajectory

@@ -462,10 +462,12 @@ class VbdContext:
 
     def warm_start(self, net, gate=True, smoothing=0):
         """Start the open substep from the network's prediction (after `smoothing` one-ring averaging
-        passes); returns (accepted, rho) per object."""
+        passes); gate: False / True (gate now) / "deferred" (the next iterate_to_tolerance's first check
+        is the gate); returns (accepted, rho) per object (rho NaN when deferred)."""
+        mode = 2 if gate == "deferred" else int(bool(gate))
         acc = np.zeros(self.n_objects, dtype=np.int32)
         rho = np.zeros(self.n_objects)
-        self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(smoothing), int(bool(gate)), _i(acc), _d(rho)))
+        self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(smoothing), mode, _i(acc), _d(rho)))
         return acc.astype(bool), rho
 
     def launch(self):

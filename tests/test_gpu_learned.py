@@ -260,3 +260,59 @@ def test_warm_start_smoothing_matches_restatement():
     s = sc.dt ** 2 * np.linalg.norm(sc.gravity)
     assert np.allclose(x, xt + s * y, rtol=0, atol=2e-6)
     ctx.end_substep()
+
+
+def test_pipelined_tolerance_loop_equals_plain_sweeps():
+    """iterate_to_tolerance checks on a second stream while sweeping ahead and rolls back to the
+    passing snapshot: the result is bitwise that of exactly n sweeps, n is the first sweep count whose
+    residual passes, and the reported rho is the residual ratio of the final state."""
+    sc = scenes.cloth(40, young=2e4, pin="top_row", seed=6, iterations=5)
+    ctx = native.VbdContext([sc.vbd_object()])
+    ctx.step(sc.dt, 1, 5, sc.gravity)
+    ctx.save_state()
+    ctx.begin_substep(sc.dt, sc.gravity)
+    n, rho = ctx.iterate_to_tolerance(1e-4, max_iterations=3000, check_every=1)
+    ctx.end_substep()
+    xa, va = ctx.state()
+    assert rho <= 1e-4 and n > 2
+    ctx.restore_state()
+    ctx.begin_substep(sc.dt, sc.gravity)
+    ctx.iterate(n)
+    assert ctx.residual()[0] / ctx.residual_at_inertial()[0] == pytest.approx(rho, rel=1e-12)
+    ctx.end_substep()
+    xb, vb = ctx.state()
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb)
+    ctx.restore_state()
+    ctx.begin_substep(sc.dt, sc.gravity)
+    ctx.iterate(n - 1)
+    assert ctx.residual()[0] / ctx.residual_at_inertial()[0] > 1e-4
+    ctx.end_substep()
+    # a cap below n stops at the cap with the residual of that state
+    ctx.restore_state()
+    ctx.begin_substep(sc.dt, sc.gravity)
+    m, rho_m = ctx.iterate_to_tolerance(1e-4, max_iterations=n - 2, check_every=1)
+    assert m == n - 2 and rho_m > 1e-4
+    ctx.end_substep()
+
+
+def test_deferred_gate_rejects_in_the_first_check():
+    """gate='deferred': the first check of iterate_to_tolerance is the gate; a worse prediction goes back
+    to x~ and the solve ends bitwise like plain VBD."""
+    s = scenes.cloth(24, young=1e4, pin="top_row", seed=5, iterations=5)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.save_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    it_p, rho_p = ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=1)
+    ctx.end_substep()
+    plain = ctx.state()
+    ctx.restore_state()
+    ctx.reset_stats()
+    ctx.begin_substep(s.dt, s.gravity)
+    acc, rho0 = ctx.warm_start(_constant_net([0.0, 50.0, 0.0]), gate="deferred")
+    assert acc[0] and np.isnan(rho0[0])
+    it_l, rho_l = ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=1)
+    ctx.end_substep()
+    assert ctx.stats()["prediction_fallbacks"] == 1
+    assert (it_l, rho_l) == (it_p, rho_p)
+    x, v = ctx.state()
+    assert np.array_equal(x, plain[0]) and np.array_equal(v, plain[1])

@@ -50,7 +50,7 @@ def main():
     ap.add_argument("--lds", type=int, default=-1, help="lds_staging of the vertex solve (with --lanes/--block)")
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-warm-start comparison")
-    ap.add_argument("--learned-steps", type=int, default=10, help="trajectory steps of the learned comparison")
+    ap.add_argument("--learned-steps", type=int, default=20, help="trajectory steps of the learned comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloth batched measurement")
     ap.add_argument("--no-volumetric", action="store_true", help="skip the 64^3 cube measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",

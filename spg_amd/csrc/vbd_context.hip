@@ -473,6 +473,8 @@ struct spg_vbd_ctx {
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
         if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
         if (smoothing < 0 || smoothing > 64) throw VbdError(SPG_VBD_ERR_INVALID, "smoothing passes must be in 0..64");
+        // the deferred gate needs |r(x~)| in its first check: start it first, beside features and inference
+        if (gateMode == 2 && !r0Valid && !r0OnCheckStream) enqueueInertialResidualOnCheckStream();
         if (!evFeatures) {
             hipCheck(hipEventCreateWithFlags(&evFeatures, hipEventDisableTiming), "hipEventCreate");
             hipCheck(hipEventCreateWithFlags(&evPrediction, hipEventDisableTiming), "hipEventCreate");
@@ -488,8 +490,6 @@ struct spg_vbd_ctx {
         const bool deferGate = gateMode == 2;
         const bool needResiduals = gateMode == 1 || (gateMode == 0 && rho);
         const bool needR0 = needResiduals && !r0Valid && !r0OnCheckStream;
-        // the deferred gate needs |r(x~)| in the first check: start it now, beside the inference
-        if (deferGate && !r0Valid && !r0OnCheckStream) enqueueInertialResidualOnCheckStream();
         if (needR0) enqueueResidual(true, normsAux.p);
         hipCheck(hipStreamWaitEvent(stream, evPrediction, 0), "hipStreamWaitEvent");
         for (int32_t k = 0; k < smoothing; ++k) {

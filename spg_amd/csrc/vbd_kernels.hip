@@ -362,17 +362,23 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
                 r = r - ge;
             }
         }
-        for (int cls = 0; cls < 2; ++cls) {  // hinge-edge and hinge-wing incidences
-            const int32_t *ptr = cls == 0 ? a.hinEPtr : a.hinWPtr;
-            const int32_t *elem = cls == 0 ? a.hinEElem : a.hinWElem;
-            if (!ptr) continue;
-            for (int32_t k = ptr[vid] + lig; k < ptr[vid + 1]; k += G) {
-                const int32_t el = elem[k];
+        if (a.hinEPtr) {  // hinge-edge incidences (slots 0, 1)
+            for (int32_t k = a.hinEPtr[vid] + lig; k < a.hinEPtr[vid + 1]; k += G) {
+                const int32_t el = a.hinEElem[k];
                 const int4 v = a.hinV[el];
                 const double2 p = a.hinP[el];
-                const int slot = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
-                bendingVertexCompact<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]),
-                                             slot, p.x, p.y, ge, He);
+                bendingEdge<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), v.y == vid,
+                                    p.x, p.y, ge, He);
+                r = r - ge;
+            }
+        }
+        if (a.hinWPtr) {  // hinge-wing incidences (slots 2, 3)
+            for (int32_t k = a.hinWPtr[vid] + lig; k < a.hinWPtr[vid + 1]; k += G) {
+                const int32_t el = a.hinWElem[k];
+                const int4 v = a.hinV[el];
+                const double2 p = a.hinP[el];
+                bendingWing<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), v.w == vid,
+                                    p.x, p.y, ge, He);
                 r = r - ge;
             }
         }

@@ -274,17 +274,19 @@ def learned_comparison(scene, args, launch):
 
 def measured_traffic(launch, args):
     """HBM bytes per vertex-solve launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
-    this workload (scaled by the same counters on a copy kernel of known size), when the launch shape
-    and scene match the profiled ones; None otherwise."""
+    this workload (scaled by the same counters on a copy kernel of known size), for the launch shape
+    the bench uses; None when that shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "solveColorKernel_cloth512_traffic.json")
     if args.n != 512 or args.seed != 3 or not os.path.exists(path):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    shape = f"lanes {launch['lanes_per_vertex']}, block {launch['block_size']}, lds {launch['lds_staging']}"
-    if not t["workload"].endswith(shape):
-        return None, None
-    return t["traffic_bytes"], "profiles/solveColorKernel_cloth512_traffic.json (rocprofv3 --pmc, same kernel and launch)"
+    for sh in t["shapes"]:
+        if (sh["lanes"], sh["block"], sh["lds"]) == (launch["lanes_per_vertex"], launch["block_size"],
+                                                     launch["lds_staging"]):
+            return sh["traffic_bytes"], ("profiles/solveColorKernel_cloth512_traffic.json (rocprofv3 --pmc, same "
+                                         "kernel and launch shape)")
+    return None, None
 
 
 def cpu_baseline(scene):

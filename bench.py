@@ -281,7 +281,7 @@ def measured_traffic(launch, args):
         return None, None
     with open(path) as f:
         t = json.load(f)
-    for sh in t["shapes"]:
+    for sh in t.get("shapes", []):
         if (sh["lanes"], sh["block"], sh["lds"]) == (launch["lanes_per_vertex"], launch["block_size"],
                                                      launch["lds_staging"]):
             return sh["traffic_bytes"], ("profiles/solveColorKernel_cloth512_traffic.json (rocprofv3 --pmc, same "

@@ -316,3 +316,46 @@ def test_deferred_gate_rejects_in_the_first_check():
     assert (it_l, rho_l) == (it_p, rho_p)
     x, v = ctx.state()
     assert np.array_equal(x, plain[0]) and np.array_equal(v, plain[1])
+
+
+def _per_object_net(scale):
+    """A network whose output y = (0, scale * silu(silu(silu(f18 - 3))), 0) depends on the log10 Young's
+    feature alone (identity normalisation): exactly 0 for Y = 1e3 (silu(0) = 0 through every layer),
+    an upward lift for stiffer cloths."""
+    p = np.zeros(native.MLP_NUM_PARAMS, dtype=np.float32)
+    (w1, b1), (w2, b2), (w3, b3), (w4, b4) = native.mlp_unflatten(p)  # views into p
+    w1[0, 18] = 1.0
+    b1[0] = -3.0
+    w2[0, 0] = 1.0
+    w3[0, 0] = 1.0
+    w4[1, 0] = scale
+    net = native.Mlp(seed=1)
+    net.set_params(p)
+    return net
+
+
+def test_batched_gate_and_tolerance_are_per_object():
+    """Two cloths in one context. The prediction is exactly x~ for the soft cloth (Y t = 1e3: accepted,
+    rho = 1) and a large lift for the stiff one (Y t = 5e4: rejected), so both start from x~ and the
+    batched learned solve equals the batched plain solve bitwise, with rho the largest per-object ratio."""
+    soft = scenes.cloth(14, young=1e3 / 1e-3, thickness=1e-3, pin="top_row", seed=2, iterations=5)
+    stiff = scenes.cloth(18, young=5e4 / 1e-3, thickness=1e-3, pin="top_row", seed=3, iterations=5)
+    objs = [stiff.vbd_object(), soft.vbd_object()]
+    ctx = native.VbdContext(objs)
+    ctx.begin_substep(soft.dt, soft.gravity)
+    f = ctx.features(-1)
+    assert abs(f[0, 18] - np.log10(5e4)) < 1e-5 and f[-1, 18] == np.float32(3.0)
+    acc, rho = ctx.warm_start(_per_object_net(scale=20.0), gate=True)
+    print(f"accepted {acc.tolist()} rho {rho.tolist()}", flush=True)
+    assert acc.tolist() == [False, True] and rho[0] > 1.0 and rho[1] == 1.0
+    n, r = ctx.iterate_to_tolerance(1e-4, max_iterations=3000, check_every=1)
+    res, res0 = ctx.residual(), ctx.residual_at_inertial()
+    assert r <= 1e-4 and r == pytest.approx(np.max(res / res0), rel=1e-12)
+    ctx.end_substep()
+    ref = native.VbdContext(objs)
+    ref.begin_substep(soft.dt, soft.gravity)
+    n_ref, r_ref = ref.iterate_to_tolerance(1e-4, max_iterations=3000, check_every=1)
+    ref.end_substep()
+    assert (n, r) == (n_ref, r_ref)
+    for o in range(2):
+        assert all(np.array_equal(a, b) for a, b in zip(ctx.state(o), ref.state(o)))

@@ -402,3 +402,28 @@ def test_ill_conditioned_block_takes_fp64_path(oracle_threads):
     e = disp_rel_err(ctx.state()[0], o.state()[0], start)
     print(f"ill-conditioned spring block: displacement-relative error {e:.2e}", flush=True)
     assert e < 1e-5
+
+
+def test_several_substeps_match_oracle(oracle_threads):
+    """VBD::step with nsubsteps > 1 (h = dt / nsubsteps, vbd.cpp:19): two steps of 3 substeps."""
+    s = scenes.cloth(24, young=1e4, pin="top_row", seed=9, iterations=6)
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    for _ in range(2):
+        ctx.step(s.dt, 3, s.iterations, s.gravity)
+        o.step(s.dt, 3, s.iterations, s.gravity)
+    assert rel_err(ctx.state()[0], o.state()[0]) < 1e-5
+    assert rel_err(ctx.state()[1], o.state()[1]) < 1e-4
+
+
+def test_zero_mass_particles_match_oracle(oracle_threads):
+    """Zero mass does not pin in VBD (vbd.cpp:102,123): gravity is still integrated for such particles and
+    their block is the elastic Hessian alone."""
+    s = scenes.cloth(16, young=1e4, pin="top_row", seed=4, iterations=8)
+    s.m = s.m.copy()
+    s.m[[20, 37, 100]] = 0.0
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    assert rel_err(ctx.state()[0], o.state()[0]) < 1e-5

@@ -9,7 +9,8 @@
  *   spg_trimesh_*     spg::TriangleMesh edge/opposing-vertex enumeration (spg/include/spg/geom/triangleMesh.h:16-50)
  *                     and the cloth bending wiring of apps/solverCompare.cpp:441-452.
  *   spg_solver_*      spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41) implemented by
- *                     spg::solver::VBDGpu, the GPU drop-in for spg::solver::VBD (vbd.h:14-41).
+ *                     spg::solver::VBDGpu, the GPU drop-in for spg::solver::VBD (vbd.h:14-41), or by
+ *                     spg::solver::XPBDGpu, the GPU drop-in for spg::solver::XPBD (xpbd.h:15-30).
  *
  * Errors: 0 on success, nonzero on failure with the message (the reference's exception text where
  * the reference throws) in spg_host_last_error().
@@ -20,6 +21,7 @@
 #include <stdint.h>
 
 #include "spg_vbd.h"
+#include "spg_xpbd.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -45,6 +47,10 @@ int spg_pg_prepare_energy(spg_pgroup *pg, int32_t index);
 int spg_pg_num_particles(const spg_pgroup *pg, int64_t *n);
 /* Prepared data of energy `index` in the params layout of spg_vbd_energy (spg_vbd.h). */
 int spg_pg_energy_params(const spg_pgroup *pg, int32_t index, double *params);
+/* Same energy in the params layout of spg_xpbd_energy (spg_xpbd.h): the effectiveCompliance() form. */
+int spg_pg_energy_xpbd_params(const spg_pgroup *pg, int32_t index, double *params);
+/* ParticleGroup::invMasses() (n entries). */
+int spg_pg_inv_masses(const spg_pgroup *pg, double *w);
 int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v);
 int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v);
 
@@ -53,19 +59,22 @@ int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int3
                                  int64_t *count);
 
 spg_solver *spg_solver_create_vbd_gpu(int32_t device);
+/* XPBDGpu(parallel_gauss_seidel != 0) on `device` (XPBD(true) / XPBD(false) of the reference). */
+spg_solver *spg_solver_create_xpbd_gpu(int32_t device, int32_t parallel_gauss_seidel);
 void spg_solver_destroy(spg_solver *s);
 int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg); /* copies the group, shares its energies */
 int spg_solver_set_dt(spg_solver *s, double dt);
 int spg_solver_set_num_substeps(spg_solver *s, int32_t n);
 int spg_solver_set_gravity(spg_solver *s, const double g[3]);
-int spg_solver_set_iterations(spg_solver *s, int32_t iterations);
+int spg_solver_set_iterations(spg_solver *s, int32_t iterations); /* VBD only */
 int spg_solver_set_resident_state(spg_solver *s, int32_t resident);
 int spg_solver_step(spg_solver *s);
 int spg_solver_reset(spg_solver *s);
 int spg_solver_sync_to_host(spg_solver *s);
 int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v);
-/* The C-ABI context behind the solver (NULL before the first step). */
+/* The C-ABI context behind the solver (NULL before the first step, or for the other solver type). */
 spg_vbd_ctx *spg_solver_context(spg_solver *s);
+spg_xpbd_ctx *spg_solver_xpbd_context(spg_solver *s);
 
 #ifdef __cplusplus
 }

@@ -603,6 +603,41 @@ void adVertex(int type, const double *xs, const double *p, int slot, double *g, 
     }
 }
 
+// Constraints per stencil: the TnConstraints of each energy's StencilBlockEnergy base
+// (membraneStvkEnergy.h <3, 3>, stableNeoHookeanEnergy.h <4, 2>, the others 1).
+int numConstraints(int type)
+{
+    switch (type) {
+    case ORC_MEMBRANE_STVK: return 3;
+    case ORC_STABLE_NEO_HOOKEAN: return 2;
+    case ORC_SPRING:
+    case ORC_DISCRETE_BENDING:
+    case ORC_SPRING_ANCHOR: return 1;
+    }
+    return 0;
+}
+
+// Inverse of a 3x3 (row-major) by cofactors: inverse = adj / det.
+M3 inverse3(const M3 &m)
+{
+    const double c00 = m[4] * m[8] - m[5] * m[7];
+    const double c01 = m[5] * m[6] - m[3] * m[8];
+    const double c02 = m[3] * m[7] - m[4] * m[6];
+    const double det = m[0] * c00 + m[1] * c01 + m[2] * c02;
+    const M3 adj = {c00,
+                    m[2] * m[7] - m[1] * m[8],
+                    m[1] * m[5] - m[2] * m[4],
+                    c01,
+                    m[0] * m[8] - m[2] * m[6],
+                    m[2] * m[3] - m[0] * m[5],
+                    c02,
+                    m[1] * m[6] - m[0] * m[7],
+                    m[0] * m[4] - m[1] * m[3]};
+    M3 r;
+    for (int k = 0; k < 9; ++k) r[k] = adj[k] / det;
+    return r;
+}
+
 // ---------------------------------------------------------------------------------------------
 // Object: ParticleGroup + energies + VBD solver state
 // ---------------------------------------------------------------------------------------------
@@ -612,6 +647,7 @@ struct Energy {
     std::vector<int32_t> st;   // flat stencils (Energy::flatStencils, energy.h:102-110)
     std::vector<double> raw;   // addStencil arguments
     std::vector<double> prm;   // prepared parameters
+    std::vector<double> cmp;   // effectiveCompliance() per stencil, nC x nC row-major (XPBD)
     bool prepared = false;
     int64_t n() const { return ssize ? static_cast<int64_t>(st.size()) / ssize : 0; }
 };
@@ -664,22 +700,27 @@ void prepareEnergy(const Object &o, Energy &e)
     if (e.prepared) throw OracleError("Energy was already prepared");  // energy.h:54-61
     const int64_t n = e.n();
     const int np = preparedParams(e.type);
+    const int nc = numConstraints(e.type);
     e.prm.assign(n * np, 0.0);
+    e.cmp.assign(n * nc * nc, 0.0);
     for (int64_t i = 0; i < n; ++i) {
         const int32_t *s = &e.st[i * e.ssize];
         const double *r = &e.raw[i * rawParams(e.type)];
         double *p = &e.prm[i * np];
+        double *cm = &e.cmp[i * nc * nc];
         auto X0 = [&](int k) { return &o.x0[3 * static_cast<int64_t>(s[k])]; };
         switch (e.type) {
-        case ORC_SPRING:
+        case ORC_SPRING:  // springEnergy.cpp:42-45: compliance = StiffnessMat{k}.inverse()
             p[0] = r[0];
             p[1] = r[1];
+            cm[0] = 1.0 / r[1];
             break;
-        case ORC_SPRING_ANCHOR:
+        case ORC_SPRING_ANCHOR:  // springAnchorEnergy.cpp:34-37
             p[0] = r[0];
             p[1] = r[1];
             p[2] = r[2];
             p[3] = r[3];
+            cm[0] = 1.0 / r[3];
             break;
         case ORC_MEMBRANE_STVK: {
             const double young = r[0], poisson = r[1];
@@ -698,6 +739,9 @@ void prepareEnergy(const Object &o, Energy &e)
             const double area = (u0 * v1 - v0 * u1) * 0.5;
             if (area <= 0) throw OracleError("Rest area of membrane StVK triangle is not positive");
             for (int k = 0; k < 9; ++k) p[4 + k] = K[k] * area;
+            // membraneStvkEnergy.cpp:41-42,67: effectiveCompliance = C.inverse() / area
+            const M3 Ci = inverse3({K[0], K[1], K[2], K[3], K[4], K[5], K[6], K[7], K[8]});
+            for (int k = 0; k < 9; ++k) cm[k] = Ci[k] / area;
             break;
         }
         case ORC_DISCRETE_BENDING: {
@@ -706,8 +750,9 @@ void prepareEnergy(const Object &o, Energy &e)
             const A3 c03 = a3cross(e0, e3), c04 = a3cross(e0, e4);
             const double factor =
                 2. * 3. * a3dot(e0, e0) / (0.5 * (std::sqrt(a3dot(c03, c03)) + std::sqrt(a3dot(c04, c04))));
-            const double compliance = (1.0 / r[1]) / factor;
+            const double compliance = (1.0 / r[1]) / factor;  // discreteBendingEnergy.cpp:44-45,63
             p[1] = 1.0 / compliance;
+            cm[0] = compliance;
             const double xsr[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
             p[0] = bendingTheta<double>(reinterpret_cast<const V3<double> *>(xsr));
             break;
@@ -741,6 +786,12 @@ void prepareEnergy(const Object &o, Energy &e)
             p[9] = lambda * volume;
             p[10] = mu * volume;
             p[11] = 1 + mu / lambda;
+            // stableNeoHookeanEnergy.cpp:74-78,104: effectiveCompliance = diag(lambda, mu).inverse() / volume
+            const double det2 = lambda * mu;
+            cm[0] = mu / det2 / volume;
+            cm[1] = 0.0;
+            cm[2] = 0.0;
+            cm[3] = lambda / det2 / volume;
             break;
         }
         }
@@ -932,6 +983,281 @@ void substep(Object &o, double h, int iterations, const double *g, int mode, con
     }
     const double invdt = 1.0 / h;
     for (int64_t i = 0; i < 3 * nv; ++i) o.v[i] = (o.x[i] - xOld[i]) * invdt;  // particleGroup.cpp:65-72
+}
+
+// ---------------------------------------------------------------------------------------------
+// XPBD (spg/source/sim/solver/xpbd.cpp:16-160): one constraint projection per stencil per substep,
+// energies in addEnergy order, stencils in parallel colour groups (computeParallelStencilGroups,
+// xpbd.cpp:134-160) or in index order (the serial Gauss-Seidel path, xpbd.cpp:99-104).
+// ---------------------------------------------------------------------------------------------
+inline double valueOf(double s) { return s; }
+template <int N, bool H>
+inline double valueOf(const Dual<N, H> &s)
+{
+    return s.v;
+}
+
+// The dConstraints lambdas of the five energies over a generic scalar (double or Dual).
+// springEnergy.cpp:10-18 — C = |x0 - x1| - L0
+template <class S>
+void constraintSpring(const V3<S> *x, const double *p, S *C)
+{
+    const V3<S> d = sub(x[0], x[1]);
+    C[0] = sqrt(dot(d, d)) - p[0];
+}
+// springAnchorEnergy.cpp:8-20 — C = |x - a|, replaced by the constant 0 when it is exactly 0
+template <class S>
+void constraintAnchor(const V3<S> *x, const double *p, S *C)
+{
+    const V3<S> d = {x[0].x - p[0], x[0].y - p[1], x[0].z - p[2]};
+    C[0] = sqrt(dot(d, d));
+    if (valueOf(C[0]) == 0) C[0] = S(0.0);
+}
+// membraneStvkEnergy.cpp:9-28 — C = (E00, E11, 2 E01) of the Green strain E = 1/2 (F^T F - I)
+template <class S>
+void constraintStvk(const V3<S> *x, const double *p, S *C)
+{
+    const double *Di = p;
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    const V3<S> f0 = {u.x * Di[0] + v.x * Di[2], u.y * Di[0] + v.y * Di[2], u.z * Di[0] + v.z * Di[2]};
+    const V3<S> f1 = {u.x * Di[1] + v.x * Di[3], u.y * Di[1] + v.y * Di[3], u.z * Di[1] + v.z * Di[3]};
+    C[0] = 0.5 * (dot(f0, f0) - 1.0);
+    C[1] = 0.5 * (dot(f1, f1) - 1.0);
+    C[2] = 2.0 * (0.5 * dot(f0, f1));
+}
+// discreteBendingEnergy.cpp:9-31 — C = theta - theta0
+template <class S>
+void constraintBending(const V3<S> *x, const double *p, S *C)
+{
+    C[0] = bendingTheta(x) - p[0];
+}
+// stableNeoHookeanEnergy.cpp:9-31 — C = (det F - alpha, sqrt(tr F^T F))
+template <class S>
+void constraintSnh(const V3<S> *x, const double *p, S *C)
+{
+    const double *Di = p;
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    const V3<S> w = sub(x[3], x[0]);
+    V3<S> f[3];
+    for (int c = 0; c < 3; ++c)
+        f[c] = {u.x * Di[0 * 3 + c] + v.x * Di[1 * 3 + c] + w.x * Di[2 * 3 + c],
+                u.y * Di[0 * 3 + c] + v.y * Di[1 * 3 + c] + w.y * Di[2 * 3 + c],
+                u.z * Di[0 * 3 + c] + v.z * Di[1 * 3 + c] + w.z * Di[2 * 3 + c]};
+    C[0] = dot(f[0], cross(f[1], f[2])) - p[11];
+    C[1] = sqrt(dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]));
+}
+
+template <int NS, class S>
+void evalConstraintsN(int type, const V3<S> *x, const double *p, S *C)
+{
+    if constexpr (NS == 1) {
+        constraintAnchor(x, p, C);
+    } else if constexpr (NS == 2) {
+        constraintSpring(x, p, C);
+    } else if constexpr (NS == 3) {
+        constraintStvk(x, p, C);
+    } else {
+        if (type == ORC_DISCRETE_BENDING)
+            constraintBending(x, p, C);
+        else
+            constraintSnh(x, p, C);
+    }
+}
+
+// Constraint values and their gradient by forward AD, as StencilBlockEnergy::dConstraintsAndGradient
+// does through TinyAD (energy.h:289-292). grad is (3 NS) x NC row-major.
+template <int NS, int NC>
+void adConstraints(int type, const double *xs, const double *p, double *c, double *grad)
+{
+    constexpr int N = 3 * NS;
+    using D = Dual<N, false>;
+    V3<D> x[NS];
+    for (int s = 0; s < NS; ++s) {
+        x[s].x = D::var(xs[3 * s + 0], 3 * s + 0);
+        x[s].y = D::var(xs[3 * s + 1], 3 * s + 1);
+        x[s].z = D::var(xs[3 * s + 2], 3 * s + 2);
+    }
+    D C[NC];
+    evalConstraintsN<NS>(type, x, p, C);
+    for (int j = 0; j < NC; ++j) {
+        c[j] = C[j].v;
+        for (int i = 0; i < N; ++i) grad[i * NC + j] = C[j].g[i];
+    }
+}
+
+// Fixed-size inverse of the NC x NC constraint matrix (1x1, 2x2 adjugate, 3x3 cofactors).
+template <int NC>
+void smallInverse(const double *A, double *Ai)
+{
+    if constexpr (NC == 1) {
+        Ai[0] = 1.0 / A[0];
+    } else if constexpr (NC == 2) {
+        const double invdet = 1.0 / (A[0] * A[3] - A[1] * A[2]);
+        Ai[0] = A[3] * invdet;
+        Ai[1] = -A[1] * invdet;
+        Ai[2] = -A[2] * invdet;
+        Ai[3] = A[0] * invdet;
+    } else {
+        const M3 r = inverse3({A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], A[8]});
+        for (int k = 0; k < 9; ++k) Ai[k] = r[k];
+    }
+}
+
+// StencilBlockEnergy::projectPosition (energy.h:117-153): W = diag(inverse masses),
+// A = grad^T W grad + compliance / dt^2 + 1e-10 I, dlambda = -A^-1 C, dx = W grad dlambda.
+template <int NS, int NC>
+void projectGeneric(int type, const double *xs, const double *w, const double *p, const double *cmp, double dt,
+                    double *dx)
+{
+    constexpr int N = 3 * NS;
+    double c[NC], grad[N * NC], Wg[N * NC], A[NC * NC], Ai[NC * NC], dl[NC];
+    adConstraints<NS, NC>(type, xs, p, c, grad);
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < NC; ++j) Wg[i * NC + j] = w[i / 3] * grad[i * NC + j];
+    for (int a = 0; a < NC; ++a)
+        for (int b = 0; b < NC; ++b) {
+            double s = 0.0;
+            for (int i = 0; i < N; ++i) s += grad[i * NC + a] * Wg[i * NC + b];
+            A[a * NC + b] = s + cmp[a * NC + b] / (dt * dt) + (a == b ? 1e-10 : 0.0);
+        }
+    smallInverse<NC>(A, Ai);
+    for (int a = 0; a < NC; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < NC; ++b) s += Ai[a * NC + b] * c[b];
+        dl[a] = -s;
+    }
+    for (int i = 0; i < N; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < NC; ++a) s += Wg[i * NC + a] * dl[a];
+        dx[i] = s;
+    }
+}
+
+// SpringEnergy::projectPosition (springEnergy.cpp:48-78), the spring's own closed form.
+void projectSpring(const double *xs, const double *w, const double *p, const double *cmp, double dt, double *dx)
+{
+    constexpr double eps = 1.0e-6;  // springEnergy.cpp:8
+    for (int i = 0; i < 6; ++i) dx[i] = 0.0;
+    const double wSum = w[0] + w[1];
+    if (wSum < eps) return;
+    const double alphaTilde = cmp[0] / (dt * dt);
+    double n[3] = {xs[3] - xs[0], xs[4] - xs[1], xs[5] - xs[2]};
+    const double d = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    if (d < eps) return;
+    for (double &k : n) k = k / d;
+    for (int k = 0; k < 3; ++k) {
+        const double corr = n[k] * (d - p[0]) / (wSum + alphaTilde);
+        dx[k] = w[0] * corr;
+        dx[3 + k] = -(w[1] * corr);
+    }
+}
+
+void projectStencilDx(int type, const double *xs, const double *w, const double *p, const double *cmp, double dt,
+                      double *dx)
+{
+    switch (type) {
+    case ORC_SPRING: projectSpring(xs, w, p, cmp, dt, dx); return;
+    case ORC_SPRING_ANCHOR: projectGeneric<1, 1>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_MEMBRANE_STVK: projectGeneric<3, 3>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_DISCRETE_BENDING: projectGeneric<4, 1>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_STABLE_NEO_HOOKEAN: projectGeneric<4, 2>(type, xs, w, p, cmp, dt, dx); return;
+    }
+    throw OracleError("unknown energy type");
+}
+
+// ParticleGroup::invMasses (particleGroup.cpp:81): 0 for zero mass, 1/m otherwise.
+inline double invMass(double m) { return m == 0 ? 0 : 1 / m; }
+
+// projectPosition(i) on the object's positions, then updateElementPositionFromDx per stencil slot
+// (particleGroup.cpp:43-46), slots in order.
+void projectStencil(Object &o, const Energy &e, int64_t i, double dt)
+{
+    double xs[12], w[4], dx[12];
+    gatherStencil(o, o.x.data(), e, i, xs);
+    for (int k = 0; k < e.ssize; ++k) w[k] = invMass(o.m[e.st[i * e.ssize + k]]);
+    const int nc = numConstraints(e.type);
+    projectStencilDx(e.type, xs, w, &e.prm[i * preparedParams(e.type)], &e.cmp[i * nc * nc], dt, dx);
+    for (int k = 0; k < e.ssize; ++k) {
+        const int64_t vi = e.st[i * e.ssize + k];
+        for (int d = 0; d < 3; ++d) o.x[3 * vi + d] += dx[3 * k + d];
+    }
+}
+
+// coloring::colorStencils (graphColoring.cpp:51-70): a graph node per stencil, an edge between every
+// two stencils that share a vertex, then Graph::greedyColoring (graphColoring.cpp:21-49): node 0 takes
+// colour 0, every later node the smallest colour not used by an already coloured neighbour.
+std::vector<int32_t> colorStencils(const int32_t *flat, int64_t nStencils, int ssize)
+{
+    std::vector<int32_t> col(nStencils, -1);
+    if (nStencils == 0) return col;
+    int32_t maxV = -1;
+    for (int64_t k = 0; k < nStencils * ssize; ++k) maxV = std::max(maxV, flat[k]);
+    std::vector<std::vector<int32_t>> perVertex(maxV + 1);
+    for (int64_t k = 0; k < nStencils * ssize; ++k) perVertex[flat[k]].push_back(static_cast<int32_t>(k / ssize));
+    std::vector<std::vector<int32_t>> adj(nStencils);
+    for (const auto &list : perVertex)
+        for (size_t a = 0; a < list.size(); ++a)
+            for (size_t b = a + 1; b < list.size(); ++b) {
+                adj[list[a]].push_back(list[b]);
+                adj[list[b]].push_back(list[a]);
+            }
+    // the availability flags are reset over the touched colours only (the reference refills an
+    // n-sized array per node): same colours, O(edges) instead of O(n^2)
+    col[0] = 0;
+    std::vector<char> used(nStencils + 1, 0);
+    for (int64_t n = 1; n < nStencils; ++n) {
+        for (int32_t q : adj[n])
+            if (col[q] != -1) used[col[q]] = 1;
+        int32_t c = 0;
+        while (used[c]) ++c;
+        col[n] = c;
+        for (int32_t q : adj[n])
+            if (col[q] != -1) used[col[q]] = 0;
+    }
+    return col;
+}
+
+// One XPBD::step (xpbd.cpp:16-132) of nsub substeps on one object.
+void xpbdStep(Object &o, double dtStep, int nsub, const double *g, bool parallel)
+{
+    const double dt = dtStep / nsub;  // xpbd.cpp:22
+    const double invdt = 1 / dt;
+    // computeParallelStencilGroups (xpbd.cpp:134-160): per energy, stencil ids per colour, ascending
+    std::vector<std::vector<std::vector<int32_t>>> groups(o.energies.size());
+    if (parallel)
+        for (size_t ei = 0; ei < o.energies.size(); ++ei) {
+            const auto &e = o.energies[ei];
+            const auto col = colorStencils(e.st.data(), e.n(), e.ssize);
+            for (int64_t i = 0; i < e.n(); ++i) {
+                if (static_cast<int32_t>(groups[ei].size()) < col[i] + 1) groups[ei].resize(col[i] + 1);
+                groups[ei][col[i]].push_back(static_cast<int32_t>(i));
+            }
+        }
+    const int64_t nv = o.nv;
+    for (int s = 0; s < nsub; ++s) {
+        const std::vector<double> xOld(o.x);  // xpbd.cpp:47-54
+        const double dtg[3] = {dt * g[0], dt * g[1], dt * g[2]};
+        for (int64_t i = 0; i < nv; ++i)
+            for (int d = 0; d < 3; ++d) {
+                o.v[3 * i + d] += dtg[d];              // xpbd.cpp:58-66 (every particle, also m = 0)
+                o.x[3 * i + d] += o.v[3 * i + d] * dt;  // particleGroup.cpp:57-63
+            }
+        for (size_t ei = 0; ei < o.energies.size(); ++ei) {
+            const Energy &e = o.energies[ei];
+            if (parallel) {
+                for (const auto &group : groups[ei]) {
+                    const int64_t gs = static_cast<int64_t>(group.size());
+#pragma omp parallel for schedule(static)
+                    for (int64_t k = 0; k < gs; ++k) projectStencil(o, e, group[k], dt);
+                }
+            } else {
+                for (int64_t i = 0; i < e.n(); ++i) projectStencil(o, e, i, dt);
+            }
+        }
+        for (int64_t i = 0; i < 3 * nv; ++i) o.v[i] = (o.x[i] - xOld[i]) * invdt;  // particleGroup.cpp:65-72
+    }
 }
 
 template <class F>
@@ -1194,6 +1520,71 @@ int orc_coloring_valid(void *h, const int32_t *colors)
                     if (va != vb && colors[va] == colors[vb]) return 0;
                 }
     return 1;
+}
+
+/* ---- XPBD ---- */
+int orc_num_constraints(int type) { return numConstraints(type); }
+
+int orc_color_stencils(int64_t n_stencils, int stencil_size, const int32_t *flat, int32_t *colors, int32_t *ncolors)
+{
+    return guard([&] {
+        if (stencil_size <= 0) throw OracleError("invalid stencil size");
+        for (int64_t k = 0; k < n_stencils * stencil_size; ++k)
+            if (flat[k] < 0) throw OracleError("negative vertex index");
+        const auto col = colorStencils(flat, n_stencils, stencil_size);
+        int32_t nc = 0;
+        for (int64_t i = 0; i < n_stencils; ++i) {
+            if (colors) colors[i] = col[i];
+            nc = std::max(nc, col[i] + 1);
+        }
+        if (ncolors) *ncolors = nc;
+    });
+}
+
+int orc_get_compliance(void *h, int energy, double *out)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        if (energy < 0 || energy >= static_cast<int>(o.energies.size())) throw OracleError("bad energy index");
+        const auto &e = o.energies[energy];
+        if (!e.prepared) throw OracleError("energy not prepared");
+        std::copy(e.cmp.begin(), e.cmp.end(), out);
+    });
+}
+
+int orc_xpbd_step(void *h, double dt, int nsubsteps, const double *gravity, int parallel)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        if (!(dt > 0) || nsubsteps < 1) throw OracleError("invalid dt/substeps");
+        xpbdStep(o, dt, nsubsteps, gravity, parallel != 0);
+    });
+}
+
+int orc_constraints(int type, const double *xs, const double *prepared, double *c, double *grad)
+{
+    return guard([&] {
+        switch (type) {
+        case ORC_SPRING: adConstraints<2, 1>(type, xs, prepared, c, grad); return;
+        case ORC_SPRING_ANCHOR: adConstraints<1, 1>(type, xs, prepared, c, grad); return;
+        case ORC_MEMBRANE_STVK: adConstraints<3, 3>(type, xs, prepared, c, grad); return;
+        case ORC_DISCRETE_BENDING: adConstraints<4, 1>(type, xs, prepared, c, grad); return;
+        case ORC_STABLE_NEO_HOOKEAN: adConstraints<4, 2>(type, xs, prepared, c, grad); return;
+        }
+        throw OracleError("unknown energy type");
+    });
+}
+
+int orc_xpbd_project(int type, const double *xs, const double *inv_masses, const double *prepared,
+                     const double *compliance, double dt, int generic_spring, double *dx)
+{
+    return guard([&] {
+        if (type == ORC_SPRING && generic_spring)
+            projectGeneric<2, 1>(type, xs, inv_masses, prepared, compliance, dt, dx);
+        else
+            projectStencilDx(type, xs, inv_masses, prepared, compliance, dt, dx);
+    });
 }
 
 }  // extern "C"

@@ -1,5 +1,5 @@
 /*
- * FP64 CPU oracle for the spg Vertex Block Descent step — TEST INFRASTRUCTURE ONLY.
+ * FP64 CPU oracle for the spg Vertex Block Descent and XPBD steps — TEST INFRASTRUCTURE ONLY.
  *
  * This library is the checker, not the product. Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it. The product (libspg_vbd.so / libspg_host.so)
@@ -13,6 +13,12 @@
  *   spg/source/utils/graphColoring.cpp:21-49,72-84
  *   spg/include/spg/geom/triangleMesh.h:37-50       (edge map order, opposing vertices)
  *   spg/source/sim/solver/implicitEulerBase.cpp:102-143 (system force, residual convention)
+ * and the XPBD step that shares the same element data:
+ *   spg/source/sim/solver/xpbd.cpp:16-160            (XPBD::step, computeParallelStencilGroups)
+ *   spg/include/spg/sim/energy/energy.h:117-153       (StencilBlockEnergy::projectPosition)
+ *   spg/source/sim/energy/springEnergy.cpp:48-78       (the spring's own projectPosition)
+ *   the dConstraints lambdas of the five energy sources (constraint gradients by the same dual numbers)
+ *   spg/source/utils/graphColoring.cpp:21-70          (colorStencils + greedyColoring)
  * with two derivative back-ends: closed-form per-vertex blocks, and the oracle's own
  * forward-mode second-order dual numbers that evaluate the full element derivative twice per
  * incidence, as the reference does through TinyAD.
@@ -85,6 +91,24 @@ int orc_bending_stencils(int64_t nf, const int32_t *faces, int64_t nv, int32_t *
 
 /* brute force colouring check: 1 if no stencil has two vertices of the same colour */
 int orc_coloring_valid(void *h, const int32_t *colors);
+
+/* ---- XPBD ----
+ * Compliance per stencil (effectiveCompliance(), nC x nC row-major, nC = orc_num_constraints):
+ *   SPRING, ANCHOR 1/k; STVK C^-1 / area; BENDING (1/k) / factor; SNH diag(lambda, mu)^-1 / volume.
+ * Computed by orc_prepare from the raw addStencil arguments. */
+int orc_num_constraints(int type);
+/* coloring::colorStencils on one energy's flat stencils; ncolors may be NULL */
+int orc_color_stencils(int64_t n_stencils, int stencil_size, const int32_t *flat, int32_t *colors, int32_t *ncolors);
+int orc_get_compliance(void *h, int energy, double *out);
+/* XPBD::step: nsubsteps substeps of dt / nsubsteps; parallel != 0 is the colour-group path
+   (XPBD(true)), 0 the serial index-order path (XPBD(false)). Uses and updates the object's state. */
+int orc_xpbd_step(void *h, double dt, int nsubsteps, const double *gravity, int parallel);
+/* constraint values c (nC) and gradient grad ((3 size) x nC row-major) by dual numbers */
+int orc_constraints(int type, const double *xs, const double *prepared, double *c, double *grad);
+/* one projectPosition as a displacement dx (3 size) of the stencil's vertices; generic_spring != 0
+   routes a spring through the generic path instead of its own closed form */
+int orc_xpbd_project(int type, const double *xs, const double *inv_masses, const double *prepared,
+                     const double *compliance, double dt, int generic_spring, double *dx);
 
 #ifdef __cplusplus
 }

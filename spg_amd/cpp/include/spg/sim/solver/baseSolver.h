@@ -1,6 +1,7 @@
 // Mirror of spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41,
-// source/sim/solver/baseSolver.cpp:10-35) for particle groups, and the GPU drop-in for
-// spg::solver::VBD (spg/include/spg/sim/solver/vbd.h:14-41).
+// source/sim/solver/baseSolver.cpp:10-35) for particle groups, and the GPU drop-ins for
+// spg::solver::VBD (spg/include/spg/sim/solver/vbd.h:14-41) and spg::solver::XPBD
+// (spg/include/spg/sim/solver/xpbd.h:15-30).
 #pragma once
 
 #include <memory>
@@ -10,6 +11,7 @@
 #include <spg/types.h>
 
 struct spg_vbd_ctx;
+struct spg_xpbd_ctx;
 
 namespace spg {
 
@@ -90,6 +92,41 @@ protected:
     bool m_resident{false};
     bool m_hostDirty{true};
     spg_vbd_ctx *m_ctx{nullptr};
+};
+
+// XPBD on one MI355X, the GPU drop-in for spg::solver::XPBD: XPBDGpu(true) is the parallel Gauss-Seidel
+// solver over stencil colours, XPBDGpu(false) the serial Gauss-Seidel order (run as dependency levels,
+// which reproduce it). The first step (or the first after reset()/requirePrecomputationUpdate()) colours
+// the stencils and uploads the particle groups (xpbd.cpp:18-21). Supported energies and state ownership
+// are those of VBDGpu; the projection uses each energy's effectiveCompliance() and the particles'
+// invMasses().
+class XPBDGpu : public BaseSolver
+{
+public:
+    explicit XPBDGpu(bool useParallelGaussSeidel = true, int device = 0);
+    ~XPBDGpu() override;
+    XPBDGpu(const XPBDGpu &) = delete;
+    XPBDGpu &operator=(const XPBDGpu &) = delete;
+
+    void step() override;
+    void reset() override;
+    void requirePrecomputationUpdate() { m_parallelGroupsDirty = true; }
+    void setResidentState(bool resident) { m_resident = resident; }
+    void syncToHost();
+    void markHostStateDirty() { m_hostDirty = true; }
+    // The C-ABI context (include/spg_xpbd.h) behind this solver, or nullptr before the first step.
+    spg_xpbd_ctx *context() { return m_ctx; }
+
+protected:
+    void buildContext();
+    void uploadState();
+
+    bool m_useParallelGaussSeidel{true};
+    bool m_parallelGroupsDirty{true};
+    int m_device{0};
+    bool m_resident{false};
+    bool m_hostDirty{true};
+    spg_xpbd_ctx *m_ctx{nullptr};
 };
 
 }  // namespace solver

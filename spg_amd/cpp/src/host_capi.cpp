@@ -1,4 +1,5 @@
 // C binding of the C++ spg mirror (include/spg_host.h).
+#include <algorithm>
 #include <memory>
 #include <string>
 
@@ -6,6 +7,7 @@
 #include <spg/sim/energy/energies.h>
 #include <spg/sim/solver/baseSolver.h>
 
+#include "flatten.h"
 #include "spg_host.h"
 
 struct spg_pgroup {
@@ -13,7 +15,9 @@ struct spg_pgroup {
 };
 
 struct spg_solver {
-    std::unique_ptr<spg::solver::VBDGpu> solver;
+    std::unique_ptr<spg::solver::BaseSolver> solver;
+    spg::solver::VBDGpu *vbd = nullptr;    // the solver, when it is a VBDGpu
+    spg::solver::XPBDGpu *xpbd = nullptr;  // the solver, when it is an XPBDGpu
 };
 
 namespace {
@@ -171,6 +175,23 @@ int spg_pg_energy_params(const spg_pgroup *pg, int32_t index, double *out)
     });
 }
 
+int spg_pg_energy_xpbd_params(const spg_pgroup *pg, int32_t index, double *out)
+{
+    return guarded([&] {
+        checkIndex(pg, index);
+        const auto f = spg::solver::flattenXpbd(pg->group.energies()[index].get());
+        std::copy(f.params.begin(), f.params.end(), out);
+    });
+}
+
+int spg_pg_inv_masses(const spg_pgroup *pg, double *w)
+{
+    return guarded([&] {
+        if (!pg || !w) throw std::runtime_error("invalid arguments");
+        std::copy(pg->group.invMasses().begin(), pg->group.invMasses().end(), w);
+    });
+}
+
 int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v)
 {
     return guarded([&] {
@@ -216,7 +237,16 @@ int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int3
 spg_solver *spg_solver_create_vbd_gpu(int32_t device)
 {
     auto *s = new spg_solver;
-    s->solver = std::make_unique<spg::solver::VBDGpu>(device);
+    s->vbd = new spg::solver::VBDGpu(device);
+    s->solver.reset(s->vbd);
+    return s;
+}
+
+spg_solver *spg_solver_create_xpbd_gpu(int32_t device, int32_t parallel_gauss_seidel)
+{
+    auto *s = new spg_solver;
+    s->xpbd = new spg::solver::XPBDGpu(parallel_gauss_seidel != 0, device);
+    s->solver.reset(s->xpbd);
     return s;
 }
 
@@ -227,7 +257,8 @@ int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg)
     return guarded([&] {
         if (!s || !pg) throw std::runtime_error("invalid arguments");
         s->solver->addObject(pg->group);
-        s->solver->requirePrecomputationUpdate();
+        if (s->vbd) s->vbd->requirePrecomputationUpdate();
+        if (s->xpbd) s->xpbd->requirePrecomputationUpdate();
     });
 }
 
@@ -245,11 +276,17 @@ int spg_solver_set_gravity(spg_solver *s, const double g[3])
 }
 int spg_solver_set_iterations(spg_solver *s, int32_t iterations)
 {
-    return guarded([&] { s->solver->setIterations(iterations); });
+    return guarded([&] {
+        if (!s->vbd) throw std::runtime_error("set_iterations: not a VBD solver");
+        s->vbd->setIterations(iterations);
+    });
 }
 int spg_solver_set_resident_state(spg_solver *s, int32_t resident)
 {
-    return guarded([&] { s->solver->setResidentState(resident != 0); });
+    return guarded([&] {
+        if (s->vbd) s->vbd->setResidentState(resident != 0);
+        if (s->xpbd) s->xpbd->setResidentState(resident != 0);
+    });
 }
 int spg_solver_step(spg_solver *s)
 {
@@ -261,7 +298,10 @@ int spg_solver_reset(spg_solver *s)
 }
 int spg_solver_sync_to_host(spg_solver *s)
 {
-    return guarded([&] { s->solver->syncToHost(); });
+    return guarded([&] {
+        if (s->vbd) s->vbd->syncToHost();
+        if (s->xpbd) s->xpbd->syncToHost();
+    });
 }
 
 int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v)
@@ -277,6 +317,7 @@ int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v)
     });
 }
 
-spg_vbd_ctx *spg_solver_context(spg_solver *s) { return s ? s->solver->context() : nullptr; }
+spg_vbd_ctx *spg_solver_context(spg_solver *s) { return s && s->vbd ? s->vbd->context() : nullptr; }
+spg_xpbd_ctx *spg_solver_xpbd_context(spg_solver *s) { return s && s->xpbd ? s->xpbd->context() : nullptr; }
 
 }  // extern "C"

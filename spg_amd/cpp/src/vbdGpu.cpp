@@ -6,6 +6,7 @@
 #include <spg/sim/energy/energies.h>
 #include <spg/sim/solver/baseSolver.h>
 
+#include "flatten.h"
 #include "spg_vbd.h"
 
 namespace spg::solver {
@@ -30,82 +31,6 @@ void check(int rc, spg_vbd_ctx *ctx)
     if (rc != SPG_VBD_OK) throw std::runtime_error(spg_vbd_last_error(ctx));
 }
 
-// Flattens one energy into the prepared-parameter layout of spg_vbd_energy. Identifies the energy
-// by its concrete type, as the GPU solver cannot call host virtuals; unknown energies throw.
-struct FlatEnergy {
-    int32_t type = 0;
-    std::vector<int32_t> stencils;
-    std::vector<double> params;
-};
-
-template <class E>
-void copyStencils(const E &e, FlatEnergy &f)
-{
-    f.stencils = e.flatStencils();
-}
-
-FlatEnergy flatten(const ParticleGroup::EnergyT *energy)
-{
-    FlatEnergy f;
-    if (auto *e = dynamic_cast<const SpringEnergy *>(energy)) {
-        f.type = SPG_VBD_SPRING;
-        copyStencils(*e, f);
-        for (int i = 0; i < e->nStencils(); ++i) {
-            f.params.push_back(e->restLengths()[i]);
-            f.params.push_back(e->effectiveStiffness()[i](0, 0));
-        }
-    } else if (auto *e = dynamic_cast<const MembraneStvkEnergy *>(energy)) {
-        if (!e->prepared()) throw std::runtime_error("MembraneStvkEnergy used before preparePrecomputations");
-        f.type = SPG_VBD_MEMBRANE_STVK;
-        copyStencils(*e, f);
-        for (int i = 0; i < e->nStencils(); ++i) {
-            const auto &D = e->inverseReferenceMats()[i];
-            const auto &K = e->effectiveStiffness()[i];
-            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
-            f.params.insert(f.params.end(), K.m.begin(), K.m.end());
-        }
-    } else if (auto *e = dynamic_cast<const DiscreteBendingEnergy *>(energy)) {
-        if (!e->prepared()) throw std::runtime_error("DiscreteBendingEnergy used before preparePrecomputations");
-        f.type = SPG_VBD_DISCRETE_BENDING;
-        copyStencils(*e, f);
-        for (int i = 0; i < e->nStencils(); ++i) {
-            f.params.push_back(e->restThetas()[i]);
-            f.params.push_back(e->effectiveStiffness()[i](0, 0));
-        }
-    } else if (auto *e = dynamic_cast<const StableNeoHookeanEnergy *>(energy)) {
-        if (!e->prepared()) throw std::runtime_error("StableNeoHookeanEnergy used before preparePrecomputations");
-        f.type = SPG_VBD_STABLE_NEO_HOOKEAN;
-        copyStencils(*e, f);
-        for (int i = 0; i < e->nStencils(); ++i) {
-            const auto &D = e->inverseReferenceMats()[i];
-            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
-            f.params.push_back(e->effectiveStiffness()[i](0, 0));
-            f.params.push_back(e->effectiveStiffness()[i](1, 1));
-            f.params.push_back(e->hydrostaticCorrectionTerms()[i]);
-        }
-    } else if (auto *e = dynamic_cast<const SpringAnchorEnergy *>(energy)) {
-        f.type = SPG_VBD_SPRING_ANCHOR;
-        copyStencils(*e, f);
-        for (int i = 0; i < e->nStencils(); ++i) {
-            const auto &a = e->anchors()[i];
-            f.params.insert(f.params.end(), {a[0], a[1], a[2], e->modelStiffness()[i](0, 0)});
-        }
-    } else {
-        throw std::runtime_error("VBDGpu: unsupported energy type for the GPU solver");
-    }
-    return f;
-}
-
-int64_t stencilSize(int32_t type)
-{
-    switch (type) {
-    case SPG_VBD_SPRING_ANCHOR: return 1;
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 3;
-    default: return 4;
-    }
-}
-
 std::vector<double> packVec3(const std::vector<Vector3> &v)
 {
     std::vector<double> out(3 * v.size());
@@ -128,9 +53,9 @@ void VBDGpu::buildContext()
     std::vector<spg_vbd_object> objs(m_objects.size());
     for (size_t o = 0; o < m_objects.size(); ++o) {
         const auto &pg = m_objects[o];
-        for (const auto &e : pg.energies()) flat[o].push_back(flatten(e.get()));
+        for (const auto &e : pg.energies()) flat[o].push_back(flattenVbd(e.get()));
         for (const auto &f : flat[o])
-            descs[o].push_back({f.type, static_cast<int64_t>(f.stencils.size()) / stencilSize(f.type),
+            descs[o].push_back({f.type, static_cast<int64_t>(f.stencils.size()) / flatStencilSize(f.type),
                                 f.stencils.data(), f.params.data()});
         xs[o] = packVec3(pg.positions());
         vs[o] = packVec3(pg.velocities());

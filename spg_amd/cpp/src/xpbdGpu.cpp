@@ -1,0 +1,117 @@
+// spg::solver::XPBDGpu: BaseSolver drop-in for spg::solver::XPBD (xpbd.h:15-30, xpbd.cpp:12-160) that
+// hands the particle groups and energies to the GPU solver through the C-ABI of spg_xpbd.h.
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include <spg/sim/energy/energies.h>
+#include <spg/sim/solver/baseSolver.h>
+
+#include "flatten.h"
+#include "spg_xpbd.h"
+
+namespace spg::solver {
+
+namespace {
+
+void check(int rc, spg_xpbd_ctx *ctx)
+{
+    if (rc != SPG_VBD_OK) throw std::runtime_error(spg_xpbd_last_error(ctx));
+}
+
+std::vector<double> packVec3(const std::vector<Vector3> &v)
+{
+    std::vector<double> out(3 * v.size());
+    for (size_t i = 0; i < v.size(); ++i)
+        for (int d = 0; d < 3; ++d) out[3 * i + d] = v[i][d];
+    return out;
+}
+
+}  // namespace
+
+XPBDGpu::XPBDGpu(bool useParallelGaussSeidel, int device)
+    : m_useParallelGaussSeidel(useParallelGaussSeidel), m_device(device)
+{
+}
+
+XPBDGpu::~XPBDGpu()
+{
+    if (m_ctx) spg_xpbd_destroy(m_ctx);
+}
+
+void XPBDGpu::buildContext()
+{
+    if (m_ctx) {
+        spg_xpbd_destroy(m_ctx);
+        m_ctx = nullptr;
+    }
+    const size_t n = m_objects.size();
+    std::vector<std::vector<FlatEnergy>> flat(n);
+    std::vector<std::vector<spg_xpbd_energy>> descs(n);
+    std::vector<std::vector<double>> xs(n), vs(n);
+    std::vector<spg_xpbd_object> objs(n);
+    for (size_t o = 0; o < n; ++o) {
+        const auto &pg = m_objects[o];
+        for (const auto &e : pg.energies()) flat[o].push_back(flattenXpbd(e.get()));
+        for (const auto &f : flat[o])
+            descs[o].push_back({f.type, static_cast<int64_t>(f.stencils.size()) / flatStencilSize(f.type),
+                                f.stencils.data(), f.params.data()});
+        xs[o] = packVec3(pg.positions());
+        vs[o] = packVec3(pg.velocities());
+        objs[o] = {pg.size(), xs[o].data(), vs[o].data(), pg.invMasses().data(),
+                   static_cast<int32_t>(descs[o].size()), descs[o].data()};
+    }
+    spg_xpbd_ctx *ctx = nullptr;
+    check(spg_xpbd_create(&ctx, objs.data(), static_cast<int32_t>(n), m_useParallelGaussSeidel ? 1 : 0, m_device),
+          nullptr);
+    m_ctx = ctx;
+    m_hostDirty = false;
+}
+
+void XPBDGpu::uploadState()
+{
+    for (size_t o = 0; o < m_objects.size(); ++o) {
+        const auto x = packVec3(m_objects[o].positions());
+        const auto v = packVec3(m_objects[o].velocities());
+        check(spg_xpbd_set_state(m_ctx, static_cast<int32_t>(o), x.data(), v.data()), m_ctx);
+    }
+    m_hostDirty = false;
+}
+
+void XPBDGpu::syncToHost()
+{
+    if (!m_ctx) return;
+    for (size_t o = 0; o < m_objects.size(); ++o) {
+        auto &pg = m_objects[o];
+        std::vector<double> x(3 * pg.size()), v(3 * pg.size());
+        check(spg_xpbd_get_state(m_ctx, static_cast<int32_t>(o), x.data(), v.data()), m_ctx);
+        for (int i = 0; i < pg.size(); ++i)
+            for (int d = 0; d < 3; ++d) {
+                pg.positions()[i][d] = x[3 * i + d];
+                pg.velocities()[i][d] = v[3 * i + d];
+            }
+    }
+}
+
+void XPBDGpu::step()
+{
+    // xpbd.cpp:18-21: the stencil groups are (re)computed on the first step after a change
+    if (m_objects.empty()) return;
+    if (m_parallelGroupsDirty || !m_ctx) {
+        buildContext();
+        m_parallelGroupsDirty = false;
+    } else if (!m_resident || m_hostDirty) {
+        uploadState();
+    }
+    const double g[3] = {m_gravity[0], m_gravity[1], m_gravity[2]};
+    check(spg_xpbd_step(m_ctx, m_dtStep, m_nsubsteps, g), m_ctx);
+    if (!m_resident) syncToHost();
+}
+
+void XPBDGpu::reset()
+{
+    BaseSolver::reset();
+    m_parallelGroupsDirty = true;
+}
+
+}  // namespace spg::solver

@@ -21,6 +21,15 @@
 
 namespace spgvbd {
 
+// Workgroups are dealt round-robin over the 8 XCDs of the chip. This renumbering gives XCD x the
+// x-th contiguous share of the logical block range, so blocks that touch neighbouring data share
+// that XCD's L2.
+__device__ __forceinline__ int32_t xcdContiguousBlock(int32_t b, int32_t nb)
+{
+    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
+
 enum EnergyType { kSpring = 1, kStvk = 2, kBending = 3, kSnh = 4, kAnchor = 5 };
 
 // Incidence rows of the solve. Dense classes use fixed-width rows: the j-th incidence of vertex slot v

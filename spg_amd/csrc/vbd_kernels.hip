@@ -127,13 +127,8 @@ __device__ __forceinline__ bool blockSolve(const S3f &H, const V3f &f, V3f &dx, 
 // hinge-wing slots of bending elements, SNH tetrahedra, springs, anchors), then the force and the
 // six Hessian entries are summed over the group in a fixed order with cross-lane shuffles and the
 // group's first lane solves the 3x3 block and writes the position.
-// Blocks are renumbered so that each XCD (blocks are dealt round-robin over the 8 XCDs) works on
-// one contiguous range of the colour, keeping neighbouring positions in that XCD's L2.
-__device__ __forceinline__ int32_t xcdContiguousBlock(int32_t b, int32_t nb)
-{
-    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
-    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
-}
+// Blocks are renumbered so that each XCD works on one contiguous range of the colour
+// (xcdContiguousBlock), keeping neighbouring positions in that XCD's L2.
 
 template <int G>
 __device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t lig)

@@ -1,0 +1,217 @@
+// CDNA4 kernels of the XPBD step (spg/source/sim/solver/xpbd.cpp:16-132): substep boundaries and one
+// projection kernel per energy type. A projection launch runs one stencil group (a colour, or a level of
+// the serial order) of one energy: a thread per stencil gathers its particles, solves the stencil's
+// nC x nC system and writes the displaced particles back. No two stencils of a launch share a
+// particle, so the writes need no atomics and the result does not depend on the thread order.
+#include <hip/hip_ext.h>
+
+#include "element.h"
+#include "vbd_device.h"
+#include "xpbd_device.h"
+#include "xpbd_element.h"
+
+namespace spgvbd {
+
+namespace {
+
+using V3f = V3<float>;
+
+__device__ __forceinline__ V3f xyz(float4 p) { return {p.x, p.y, p.z}; }
+__device__ __forceinline__ float4 moved(float4 p, V3f d) { return make_float4(p.x + d.x, p.y + d.y, p.z + d.z, p.w); }
+
+__device__ __forceinline__ void countFp64(bool fell, unsigned long long *counters)
+{
+    if (fell && counters) atomicAdd(counters, 1ull);
+}
+
+template <int T>
+__global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t begin, int32_t end)
+{
+    const int32_t blk = xcdContiguousBlock(blockIdx.x, gridDim.x);
+    const int32_t i = begin + blk * static_cast<int32_t>(blockDim.x) + static_cast<int32_t>(threadIdx.x);
+    if (i >= end) return;
+    float4 *pos = a.pos;
+    if constexpr (T == kSpring) {
+        const int2 id = a.sprI[i];
+        const float2 p = a.sprP[i];
+        const float4 X0 = pos[id.x], X1 = pos[id.y];
+        V3f d0, d1;
+        xpbdSpring<float>(xyz(X0), xyz(X1), X0.w, X1.w, p.x, p.y * a.invdt2, d0, d1);
+        pos[id.x] = moved(X0, d0);
+        pos[id.y] = moved(X1, d1);
+    } else if constexpr (T == kAnchor) {
+        const int32_t id = a.ancI[i];
+        const float4 p = a.ancP[i];
+        const float4 X = pos[id];
+        pos[id] = moved(X, xpbdAnchor<float>(xyz(X), X.w, V3f{p.x, p.y, p.z}, p.w * a.invdt2));
+    } else if constexpr (T == kStvk) {
+        const int4 id = a.triI[i];
+        const float4 D = a.triD[i], c = a.triC0[i];
+        const float2 c2 = a.triC1[i];
+        const float s = a.invdt2;
+        const float4 X0 = pos[id.x], X1 = pos[id.y], X2 = pos[id.z];
+        V3f o0, o1, o2;
+        countFp64(xpbdStvkF(xyz(X0), xyz(X1), xyz(X2), X0.w, X1.w, X2.w, D.x, D.y, D.z, D.w, c.x * s, c.y * s, c.z * s,
+                            c.w * s, c2.x * s, c2.y * s, o0, o1, o2),
+                  a.counters);
+        pos[id.x] = moved(X0, o0);
+        pos[id.y] = moved(X1, o1);
+        pos[id.z] = moved(X2, o2);
+    } else if constexpr (T == kBending) {
+        const int4 id = a.hinI[i];
+        const float2 p = a.hinP[i];
+        const float4 X0 = pos[id.x], X1 = pos[id.y], X2 = pos[id.z], X3 = pos[id.w];
+        V3f o0, o1, o2, o3;
+        xpbdBending<float>(xyz(X0), xyz(X1), xyz(X2), xyz(X3), X0.w, X1.w, X2.w, X3.w, p.x, p.y * a.invdt2, o0, o1,
+                           o2, o3);
+        pos[id.x] = moved(X0, o0);
+        pos[id.y] = moved(X1, o1);
+        pos[id.z] = moved(X2, o2);
+        pos[id.w] = moved(X3, o3);
+    } else {  // kSnh
+        const int4 id = a.tetI[i];
+        const float4 d0 = a.tetD0[i], d1 = a.tetD1[i], d2 = a.tetD2[i];
+        const float hyd = a.tetH[i];
+        const float D[9] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w, d2.x};
+        const float s = a.invdt2;
+        const float4 X0 = pos[id.x], X1 = pos[id.y], X2 = pos[id.z], X3 = pos[id.w];
+        V3f o0, o1, o2, o3;
+        countFp64(xpbdSnhF(xyz(X0), xyz(X1), xyz(X2), xyz(X3), X0.w, X1.w, X2.w, X3.w, D, d2.y * s, d2.z * s, d2.w * s,
+                           hyd, o0, o1, o2, o3),
+                  a.counters);
+        pos[id.x] = moved(X0, o0);
+        pos[id.y] = moved(X1, o1);
+        pos[id.z] = moved(X2, o2);
+        pos[id.w] = moved(X3, o3);
+    }
+}
+
+// xpbd.cpp:47-70 with particleGroup.cpp:57-63: x_old = x; v += h g; x += h v (every particle).
+__global__ void __launch_bounds__(256) xpbdBeginKernel(float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                       float4 *__restrict__ xold, int32_t n, float h, float hgx,
+                                                       float hgy, float hgz)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = pos[i], v = vel[i];
+    xold[i] = x;
+    v.x += hgx;
+    v.y += hgy;
+    v.z += hgz;
+    x.x += v.x * h;
+    x.y += v.y * h;
+    x.z += v.z * h;
+    vel[i] = v;
+    pos[i] = x;
+}
+
+// End of one substep (particleGroup.cpp:65-72, v = (x - x_old) / h) fused with the start of the next.
+__global__ void __launch_bounds__(256) xpbdBoundaryKernel(float4 *__restrict__ pos, float4 *__restrict__ vel,
+                                                          float4 *__restrict__ xold, int32_t n, float invhPrev,
+                                                          float h, float hgx, float hgy, float hgz)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float4 x = pos[i];
+    const float4 o = xold[i];
+    float4 v = make_float4((x.x - o.x) * invhPrev, (x.y - o.y) * invhPrev, (x.z - o.z) * invhPrev, 0.f);
+    xold[i] = x;
+    v.x += hgx;
+    v.y += hgy;
+    v.z += hgz;
+    x.x += v.x * h;
+    x.y += v.y * h;
+    x.z += v.z * h;
+    vel[i] = v;
+    pos[i] = x;
+}
+
+__global__ void __launch_bounds__(256) xpbdEndKernel(const float4 *__restrict__ pos, const float4 *__restrict__ xold,
+                                                     float4 *__restrict__ vel, int32_t n, float invh)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 x = pos[i], o = xold[i];
+    vel[i] = make_float4((x.x - o.x) * invh, (x.y - o.y) * invh, (x.z - o.z) * invh, 0.f);
+}
+
+// Check hook: one projection per thread from FP64 inputs in the C-ABI parameter layout, by the device
+// functions of the projection kernels (precision 0, with their FP64 fallback) or in double (1).
+__global__ void xpbdCheckKernel(int type, int64_t n, const double *__restrict__ xs, const double *__restrict__ w,
+                                const double *__restrict__ params, double invdt2, int precision,
+                                double *__restrict__ dx)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const int ns = type == kAnchor ? 1 : type == kSpring ? 2 : type == kStvk ? 3 : 4;
+    const int np = type == kStvk ? 13 : type == kSnh ? 14 : type == kAnchor ? 4 : 2;
+    if (precision == 0)
+        xpbdProjectRecord<true>(type, xs + i * ns * 3, w + i * ns, params + i * np, invdt2, dx + i * ns * 3);
+    else
+        xpbdProjectRecord<false>(type, xs + i * ns * 3, w + i * ns, params + i * np, invdt2, dx + i * ns * 3);
+}
+
+inline int blocksFor(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+
+template <int T>
+void projectLaunch(const XpbdArgs &a, int32_t begin, int32_t end, int bs, hipStream_t stream, hipEvent_t start,
+                   hipEvent_t stop)
+{
+    const int grid = blocksFor(end - begin, bs);
+    if (start)
+        hipExtLaunchKernelGGL(xpbdProjectKernel<T>, dim3(grid), dim3(bs), 0, stream, start, stop, 0, a, begin, end);
+    else
+        xpbdProjectKernel<T><<<grid, bs, 0, stream>>>(a, begin, end);
+}
+
+}  // namespace
+
+hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t end, int blockSize,
+                             hipStream_t stream, hipEvent_t start, hipEvent_t stop)
+{
+    if (end <= begin) return hipSuccess;
+    if (blockSize < 64 || blockSize > 256 || blockSize % 64) return hipErrorInvalidValue;
+    switch (type) {
+    case kSpring: projectLaunch<kSpring>(a, begin, end, blockSize, stream, start, stop); break;
+    case kStvk: projectLaunch<kStvk>(a, begin, end, blockSize, stream, start, stop); break;
+    case kBending: projectLaunch<kBending>(a, begin, end, blockSize, stream, start, stop); break;
+    case kSnh: projectLaunch<kSnh>(a, begin, end, blockSize, stream, start, stop); break;
+    case kAnchor: projectLaunch<kAnchor>(a, begin, end, blockSize, stream, start, stop); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchXpbdBegin(float4 *pos, float4 *vel, float4 *xold, int32_t n, float h, float gx, float gy, float gz,
+                           hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    xpbdBeginKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, vel, xold, n, h, gx, gy, gz);
+    return hipGetLastError();
+}
+
+hipError_t launchXpbdBoundary(float4 *pos, float4 *vel, float4 *xold, int32_t n, float invhPrev, float h, float gx,
+                              float gy, float gz, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    xpbdBoundaryKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, vel, xold, n, invhPrev, h, gx, gy, gz);
+    return hipGetLastError();
+}
+
+hipError_t launchXpbdEnd(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
+                         hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    xpbdEndKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, xold, vel, n, invh);
+    return hipGetLastError();
+}
+
+hipError_t launchXpbdProjectCheck(int type, int64_t n, const double *xs, const double *w, const double *params,
+                                  double invdt2, int precision, double *dx, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    xpbdCheckKernel<<<blocksFor(n, 64), 64, 0, stream>>>(type, n, xs, w, params, invdt2, precision, dx);
+    return hipGetLastError();
+}
+
+}  // namespace spgvbd

@@ -1,6 +1,7 @@
 """ctypes bindings of the two native libraries.
 
-  libspg_vbd.so  — include/spg_vbd.h : the GPU VBD solver's C-ABI (the drop-in boundary)
+  libspg_vbd.so  — include/spg_vbd.h, spg_vbd_learned.h, spg_xpbd.h: the GPU VBD and XPBD solvers'
+                   C-ABI (the drop-in boundary)
   libspg_host.so — include/spg_host.h: the C++ mirror of spg's ParticleGroup / energies / BaseSolver
 
 There is no CPU or PyTorch fallback: if the libraries are not built, every entry point raises.
@@ -20,6 +21,7 @@ KIND = {"spring": SPRING, "stvk": MEMBRANE_STVK, "bending": DISCRETE_BENDING, "s
         "anchor": SPRING_ANCHOR}
 STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1}
 PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 12, SPRING_ANCHOR: 4}
+XPBD_PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 14, SPRING_ANCHOR: 4}
 ERR_NONFINITE = 5
 ERR_STATE = 4
 
@@ -50,6 +52,31 @@ class VbdStats(ctypes.Structure):
     _fields_ = [("solve_launches", ctypes.c_int64), ("solve_ms", ctypes.c_double), ("vertex_updates", ctypes.c_int64),
                 ("algorithmic_bytes", ctypes.c_double), ("fp64_fallbacks", ctypes.c_int64),
                 ("skipped_blocks", ctypes.c_int64), ("prediction_fallbacks", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class XpbdEnergy(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("n_stencils", ctypes.c_int64), ("stencils", _ip), ("params", _dp)]
+
+
+class XpbdObject(ctypes.Structure):
+    _fields_ = [("n_particles", ctypes.c_int64), ("x", _dp), ("v", _dp), ("inv_mass", _dp),
+                ("n_energies", ctypes.c_int32), ("energies", ctypes.POINTER(XpbdEnergy))]
+
+
+class XpbdLaunch(ctypes.Structure):
+    _fields_ = [("block_size", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class XpbdStats(ctypes.Structure):
+    _fields_ = [("projection_launches", ctypes.c_int64), ("projection_ms", ctypes.c_double),
+                ("stencil_projections", ctypes.c_int64), ("algorithmic_bytes", ctypes.c_double),
+                ("fp64_solves", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -123,6 +150,28 @@ def vbd_lib():
         _proto(L, "spg_mlp_train", ctypes.c_int, [c, _fp, _fp, i64, i32, i32, ctypes.c_float, u64, i32, _dp])
         _proto(L, "spg_vbd_features", ctypes.c_int, [c, i32, _fp])
         _proto(L, "spg_vbd_warm_start", ctypes.c_int, [c, c, i32, i32, _ip, _dp])
+        # XPBD (include/spg_xpbd.h)
+        _proto(L, "spg_xpbd_create", ctypes.c_int, [ctypes.POINTER(c), ctypes.POINTER(XpbdObject), i32, i32, i32])
+        _proto(L, "spg_xpbd_destroy", None, [c])
+        _proto(L, "spg_xpbd_last_error", ctypes.c_char_p, [c])
+        _proto(L, "spg_xpbd_num_objects", ctypes.c_int, [c, _ip])
+        _proto(L, "spg_xpbd_num_particles", ctypes.c_int, [c, i32, _lp])
+        _proto(L, "spg_xpbd_num_groups", ctypes.c_int, [c, i32, i32, _ip])
+        _proto(L, "spg_xpbd_get_groups", ctypes.c_int, [c, i32, i32, _ip])
+        _proto(L, "spg_xpbd_num_launches", ctypes.c_int, [c, _ip])
+        _proto(L, "spg_xpbd_set_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_xpbd_get_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_xpbd_step", ctypes.c_int, [c, dbl, i32, _dp])
+        _proto(L, "spg_xpbd_save_state", ctypes.c_int, [c])
+        _proto(L, "spg_xpbd_restore_state", ctypes.c_int, [c])
+        _proto(L, "spg_xpbd_get_launch", ctypes.c_int, [c, ctypes.POINTER(XpbdLaunch)])
+        _proto(L, "spg_xpbd_set_launch", ctypes.c_int, [c, ctypes.POINTER(XpbdLaunch)])
+        _proto(L, "spg_xpbd_get_stats", ctypes.c_int, [c, ctypes.POINTER(XpbdStats)])
+        _proto(L, "spg_xpbd_reset_stats", ctypes.c_int, [c])
+        _proto(L, "spg_xpbd_set_timing", ctypes.c_int, [c, i32])
+        _proto(L, "spg_xpbd_synchronize", ctypes.c_int, [c])
+        _proto(L, "spg_xpbd_color_stencils", ctypes.c_int, [i64, i32, _ip, _ip])
+        _proto(L, "spg_xpbd_project_stencils", ctypes.c_int, [i32, i64, _dp, _dp, _dp, dbl, i32, i32, _dp])
         _vbd = L
     return _vbd
 
@@ -144,10 +193,14 @@ def host_lib():
         _proto(L, "spg_pg_prepare_energy", ctypes.c_int, [c, i32])
         _proto(L, "spg_pg_num_particles", ctypes.c_int, [c, _lp])
         _proto(L, "spg_pg_energy_params", ctypes.c_int, [c, i32, _dp])
+        _proto(L, "spg_pg_energy_xpbd_params", ctypes.c_int, [c, i32, _dp])
+        _proto(L, "spg_pg_inv_masses", ctypes.c_int, [c, _dp])
         _proto(L, "spg_pg_get_state", ctypes.c_int, [c, _dp, _dp])
         _proto(L, "spg_pg_set_state", ctypes.c_int, [c, _dp, _dp])
         _proto(L, "spg_trimesh_bending_stencils", ctypes.c_int, [i64, i64, _ip, _ip, _lp])
         _proto(L, "spg_solver_create_vbd_gpu", c, [i32])
+        _proto(L, "spg_solver_create_xpbd_gpu", c, [i32, i32])
+        _proto(L, "spg_solver_xpbd_context", c, [c])
         _proto(L, "spg_solver_destroy", None, [c])
         _proto(L, "spg_solver_add_object", ctypes.c_int, [c, c])
         _proto(L, "spg_solver_set_dt", ctypes.c_int, [c, dbl])
@@ -261,6 +314,18 @@ class ParticleGroup:
         _host_check(self._L.spg_pg_get_state(self.h, _d(x), _d(v)))
         return x, v
 
+    def xpbd_params(self, index):
+        """Energy `index` in the compliance layout of spg_xpbd_energy (spg_xpbd.h)."""
+        t, n, _ = self.energy_types[index]
+        out = np.zeros((n, XPBD_PARAMS[t]))
+        _host_check(self._L.spg_pg_energy_xpbd_params(self.h, index, _d(out)))
+        return out
+
+    def inv_masses(self):
+        w = np.zeros(self.num_particles())
+        _host_check(self._L.spg_pg_inv_masses(self.h, _d(w)))
+        return w
+
     def set_state(self, x=None, v=None):
         _host_check(self._L.spg_pg_set_state(self.h, _d(None if x is None else _f64(x, (-1, 3))),
                                              _d(None if v is None else _f64(v, (-1, 3)))))
@@ -326,6 +391,25 @@ class VBDGpu:
         if not p:
             raise RuntimeError("no context before the first step")
         return VbdContext._borrow(p, self)
+
+
+class XPBDGpu(VBDGpu):
+    """spg::solver::XPBDGpu (the GPU drop-in for spg::solver::XPBD) through the C++ mirror."""
+
+    def __init__(self, parallel_gauss_seidel=True, device=0):
+        self._L = host_lib()
+        self.h = self._L.spg_solver_create_xpbd_gpu(device, int(parallel_gauss_seidel))
+        self._groups = []
+
+    def set_iterations(self, n):
+        raise NotImplementedError("XPBD projects every stencil once per substep (no iteration count)")
+
+    def context(self):
+        """The C-ABI XPBD context behind this solver (valid after the first step), borrowed."""
+        p = self._L.spg_solver_xpbd_context(self.h)
+        if not p:
+            raise RuntimeError("no context before the first step")
+        return XpbdContext._borrow(p, self)
 
 
 # ------------------------------------------------------------------------------------------------
@@ -636,4 +720,168 @@ def mlp_unflatten(p):
         k += o
         out.append((w, b))
     assert k == p.size
+    return out
+
+
+# ------------------------------------------------------------------------------------------------
+# XPBD C-ABI context (include/spg_xpbd.h)
+# ------------------------------------------------------------------------------------------------
+
+def inverse_masses(m):
+    """ParticleGroup::invMasses() of masses m: 0 for zero mass, 1/m otherwise (particleGroup.cpp:81)."""
+    m = _f64(m)
+    w = np.zeros_like(m)
+    nz = m != 0
+    w[nz] = 1.0 / m[nz]
+    return w
+
+
+class XpbdContext:
+    """A spg_xpbd_ctx: one or more particle groups advanced together by XPBD on one GPU."""
+
+    def __init__(self, objects, parallel_gauss_seidel=True, device=0):
+        """objects: dicts with x (n,3), v (n,3) or None, w (n,) inverse masses (or m (n,) masses),
+        energies [(type, stencils, params in the spg_xpbd_energy layout)]."""
+        self._L = vbd_lib()
+        keep = []
+        objs = (XpbdObject * len(objects))()
+        for k, o in enumerate(objects):
+            x = _f64(o["x"], (-1, 3))
+            v = None if o.get("v") is None else _f64(o["v"], (-1, 3))
+            w = _f64(o["w"]) if o.get("w") is not None else inverse_masses(o["m"])
+            ens = (XpbdEnergy * max(1, len(o["energies"])))()
+            for j, (t, st, prm) in enumerate(o["energies"]):
+                t = _kind(t)
+                st = _i32(st, (-1, STENCIL_SIZE[t]))
+                prm = _f64(prm, (st.shape[0], XPBD_PARAMS[t]))
+                ens[j] = XpbdEnergy(t, st.shape[0], _i(st), _d(prm))
+                keep += [st, prm]
+            objs[k] = XpbdObject(x.shape[0], _d(x), _d(v), _d(w), len(o["energies"]), ens)
+            keep += [x, v, w, ens]
+        h = ctypes.c_void_p()
+        rc = self._L.spg_xpbd_create(ctypes.byref(h), objs, len(objects), int(parallel_gauss_seidel), device)
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_xpbd_last_error(None).decode())
+        self.h = h.value
+        self._owned = True
+        self._owner = None
+        self.n_objects = len(objects)
+        self._n = [int(_f64(o["x"]).size // 3) for o in objects]
+        self._ne = [len(o["energies"]) for o in objects]
+
+    @classmethod
+    def _borrow(cls, ptr, owner):
+        self = cls.__new__(cls)
+        self._L = vbd_lib()
+        self.h = ptr
+        self._owned = False
+        self._owner = owner
+        n = ctypes.c_int32(0)
+        self._L.spg_xpbd_num_objects(ptr, ctypes.byref(n))
+        self.n_objects = n.value
+        self._n = []
+        for o in range(self.n_objects):
+            k = ctypes.c_int64(0)
+            self._L.spg_xpbd_num_particles(ptr, o, ctypes.byref(k))
+            self._n.append(k.value)
+        return self
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and getattr(self, "h", None):
+            self._L.spg_xpbd_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_xpbd_last_error(self.h).decode())
+
+    def num_particles(self, obj=0):
+        return self._n[obj]
+
+    def num_groups(self, obj, energy):
+        n = ctypes.c_int32(0)
+        self._check(self._L.spg_xpbd_num_groups(self.h, obj, energy, ctypes.byref(n)))
+        return n.value
+
+    def groups(self, obj, energy, n_stencils):
+        out = np.zeros(n_stencils, dtype=np.int32)
+        self._check(self._L.spg_xpbd_get_groups(self.h, obj, energy, _i(out)))
+        return out
+
+    def num_launches(self):
+        n = ctypes.c_int32(0)
+        self._check(self._L.spg_xpbd_num_launches(self.h, ctypes.byref(n)))
+        return n.value
+
+    def state(self, obj=0):
+        x, v = np.zeros((self._n[obj], 3)), np.zeros((self._n[obj], 3))
+        self._check(self._L.spg_xpbd_get_state(self.h, obj, _d(x), _d(v)))
+        return x, v
+
+    def set_state(self, obj, x=None, v=None):
+        self._check(self._L.spg_xpbd_set_state(self.h, obj, _d(None if x is None else _f64(x, (-1, 3))),
+                                               _d(None if v is None else _f64(v, (-1, 3)))))
+
+    def step(self, dt, nsubsteps=1, gravity=(0.0, -9.8, 0.0)):
+        g = _f64(gravity)
+        self._check(self._L.spg_xpbd_step(self.h, dt, nsubsteps, _d(g)))
+
+    def save_state(self):
+        self._check(self._L.spg_xpbd_save_state(self.h))
+
+    def restore_state(self):
+        self._check(self._L.spg_xpbd_restore_state(self.h))
+
+    def launch(self):
+        cfg = XpbdLaunch()
+        self._check(self._L.spg_xpbd_get_launch(self.h, ctypes.byref(cfg)))
+        return cfg.as_dict()
+
+    def set_launch(self, **kw):
+        cur = self.launch()
+        cur.update(kw)
+        cfg = XpbdLaunch(**cur)
+        self._check(self._L.spg_xpbd_set_launch(self.h, ctypes.byref(cfg)))
+
+    def stats(self):
+        st = XpbdStats()
+        self._check(self._L.spg_xpbd_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        self._check(self._L.spg_xpbd_reset_stats(self.h))
+
+    def set_timing(self, enabled):
+        self._check(self._L.spg_xpbd_set_timing(self.h, int(enabled)))
+
+    def synchronize(self):
+        self._check(self._L.spg_xpbd_synchronize(self.h))
+
+
+def xpbd_color_stencils(stencils):
+    """coloring::colorStencils through the C-ABI (host computation)."""
+    L = vbd_lib()
+    st = _i32(stencils)
+    if st.ndim != 2:
+        raise ValueError("stencils must be (n, size)")
+    out = np.zeros(st.shape[0], dtype=np.int32)
+    rc = L.spg_xpbd_color_stencils(st.shape[0], st.shape[1], _i(st), _i(out))
+    if rc != 0:
+        raise NativeError(rc, L.spg_xpbd_last_error(None).decode())
+    return out
+
+
+def xpbd_project_stencils(kind, xs, inv_masses, params, dt, precision=0, device=0):
+    """n independent projections on the GPU by the projection kernels' device functions: (n, size, 3) dx."""
+    L = vbd_lib()
+    t = _kind(kind)
+    ns = STENCIL_SIZE[t]
+    xs = _f64(xs, (-1, ns * 3))
+    n = xs.shape[0]
+    w = _f64(inv_masses, (n, ns))
+    p = _f64(params, (n, XPBD_PARAMS[t]))
+    out = np.zeros((n, ns, 3))
+    rc = L.spg_xpbd_project_stencils(t, n, _d(xs), _d(w), _d(p), dt, precision, device, _d(out))
+    if rc != 0:
+        raise NativeError(rc, L.spg_xpbd_last_error(None).decode())
     return out

@@ -86,6 +86,13 @@ class Scene:
         return {"x": self.x, "v": None, "m": self.m, "x0": self.x0,
                 "energies": [(kind, st, pg.energy_params(i)) for i, (kind, st, _) in enumerate(self.energies)]}
 
+    def xpbd_object(self):
+        """Prepared arrays (spg_xpbd_object layout: effectiveCompliance() form, invMasses()) from the C++
+        mirror."""
+        pg = self.particle_group()
+        return {"x": self.x, "v": None, "w": pg.inv_masses(),
+                "energies": [(kind, st, pg.xpbd_params(i)) for i, (kind, st, _) in enumerate(self.energies)]}
+
 
 def grid_cloth_mesh(n):
     """n x n vertices, 1 m square in the xy plane (z = 0), row-major ids (row = y), every quad split

@@ -51,6 +51,12 @@ def lib():
         L.orc_coloring_valid.argtypes = [ctypes.c_void_p, _ip]
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_num_prepared_params.argtypes = [ctypes.c_int]
+        L.orc_num_constraints.argtypes = [ctypes.c_int]
+        L.orc_color_stencils.argtypes = [ctypes.c_int64, ctypes.c_int, _ip, _ip, _ip]
+        L.orc_get_compliance.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
+        L.orc_xpbd_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
+        L.orc_constraints.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_xpbd_project.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
         _lib = L
     return _lib
 
@@ -93,6 +99,46 @@ def element(kind, xs, prepared, mode=CLOSED_FORM, full_hess=False):
     return out
 
 
+def num_constraints(kind):
+    t = KIND[kind] if isinstance(kind, str) else kind
+    return lib().orc_num_constraints(t)
+
+
+def constraints(kind, xs, prepared):
+    """XPBD constraint values (nC) and their gradient ((3N, nC)) by the oracle's dual numbers."""
+    t = KIND[kind] if isinstance(kind, str) else kind
+    ns, nc = STENCIL_SIZE[t], num_constraints(t)
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    p = np.ascontiguousarray(prepared, dtype=np.float64)
+    c = np.zeros(nc)
+    g = np.zeros(3 * ns * nc)
+    _check(lib().orc_constraints(t, _d(xs), _d(p), _d(c), _d(g)))
+    return c, g.reshape(3 * ns, nc)
+
+
+def xpbd_project(kind, xs, inv_masses, prepared, compliance, dt, generic_spring=False):
+    """One projectPosition as the displacement (N, 3) of the stencil's vertices."""
+    t = KIND[kind] if isinstance(kind, str) else kind
+    ns = STENCIL_SIZE[t]
+    xs = np.ascontiguousarray(xs, dtype=np.float64)
+    w = np.ascontiguousarray(inv_masses, dtype=np.float64)
+    p = np.ascontiguousarray(prepared, dtype=np.float64)
+    cm = np.ascontiguousarray(compliance, dtype=np.float64)
+    dx = np.zeros(3 * ns)
+    _check(lib().orc_xpbd_project(t, _d(xs), _d(w), _d(p), _d(cm), dt, int(generic_spring), _d(dx)))
+    return dx.reshape(ns, 3)
+
+
+def color_stencils(stencils):
+    """coloring::colorStencils of one energy's stencils (n, size) -> (colours, count)."""
+    st = np.ascontiguousarray(stencils, dtype=np.int32)
+    n, ss = st.shape
+    out = np.zeros(n, dtype=np.int32)
+    nc = ctypes.c_int32(0)
+    _check(lib().orc_color_stencils(n, ss, _i(st), _i(out), ctypes.byref(nc)))
+    return out, nc.value
+
+
 def bending_stencils(faces, nv):
     faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
     n = ctypes.c_int64(0)
@@ -103,7 +149,7 @@ def bending_stencils(faces, nv):
 
 
 class Oracle:
-    """One ParticleGroup with its energies and the VBD solver state, FP64 on the CPU."""
+    """One ParticleGroup with its energies and the VBD / XPBD solver state, FP64 on the CPU."""
 
     def __init__(self, x, x0, m):
         self.x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1, 3)
@@ -186,6 +232,17 @@ class Oracle:
         r = np.zeros((self.nv, 3)) if return_vector else None
         _check(lib().orc_residual(self.h, _d(x), _d(xt), dt, mode, _d(nrm), _d(r) if return_vector else None))
         return (nrm[0], r) if return_vector else nrm[0]
+
+    def compliance(self, e):
+        t, n = self.types[e]
+        nc = num_constraints(t)
+        out = np.zeros((n, nc * nc))
+        _check(lib().orc_get_compliance(self.h, e, _d(out)))
+        return out
+
+    def xpbd_step(self, dt, nsubsteps=1, gravity=(0.0, -9.8, 0.0), parallel=True):
+        g = np.asarray(gravity, dtype=np.float64)
+        _check(lib().orc_xpbd_step(self.h, dt, nsubsteps, _d(g), int(parallel)))
 
     def energy(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

@@ -1,5 +1,8 @@
 """CPU-only checks of the native libraries (no GPU compute calls):
   * both shared libraries load and export every function declared in include/*.h;
+  * XPBD host logic: stencil colouring (bit-exact against the oracle), the compliance-form parameters
+    of the C++ mirror, and the projection closed forms instantiated on the host against the oracle's
+    dual-number projection;
   * host-side logic against the oracle: greedy colouring (bit-exact, incl. the known class sizes of
     the 512^2 cloth and the 64^3 Kuhn cube), TriangleMesh bending stencils, preparePrecomputations;
   * the reference's setup errors surface with the reference's messages.
@@ -24,7 +27,7 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,lib", [("spg_vbd.h", native.VBD_LIB), ("spg_vbd_learned.h", native.VBD_LIB),
-                                        ("spg_host.h", native.HOST_LIB)])
+                                        ("spg_xpbd.h", native.VBD_LIB), ("spg_host.h", native.HOST_LIB)])
 def test_exports(header, lib):
     native.host_lib()
     L = ctypes.CDLL(lib)
@@ -149,3 +152,140 @@ def test_device_closed_forms_on_host(kind, form):
                 assert np.max(np.abs(g[k] - g_ref[k])) <= tol * sg, (kind, k, prec)
             else:
                 assert np.max(np.abs(g[k])) <= 1e-6 * sh
+
+
+# ------------------------------------------------------------------------------------------------
+# XPBD host logic
+# ------------------------------------------------------------------------------------------------
+
+def test_xpbd_stencil_coloring_matches_oracle():
+    for s in (scenes.cloth_config1(), scenes.cloth_config2(seed=1), scenes.cube(8)):
+        for kind, st, _ in s.energies:
+            a = native.xpbd_color_stencils(st)
+            b, nc = ol.color_stencils(st)
+            assert np.array_equal(a, b), kind
+    # 512^2 cloth: class sizes of the greedy stencil colouring
+    s = scenes.cloth_config2(seed=1)
+    sizes = {k: np.bincount(native.xpbd_color_stencils(st)).tolist() for k, st, _ in s.energies}
+    assert sizes["stvk"] == [87296, 87169, 87039, 86955, 86828, 86701, 254]
+    assert sizes["bending"] == [65536] + [65280] * 4 + [65025] * 7 + [255, 255]
+    assert sizes["anchor"] == [512]
+
+
+def test_xpbd_stencil_coloring_random_matches_oracle():
+    rng = np.random.default_rng(11)
+    for ss in (1, 2, 3, 4):
+        st = np.array([rng.choice(300, size=ss, replace=False) for _ in range(900)], dtype=np.int32)
+        assert np.array_equal(native.xpbd_color_stencils(st), ol.color_stencils(st)[0])
+
+
+def test_xpbd_stencil_coloring_bad_index():
+    with pytest.raises(native.NativeError, match="invalid node indices"):
+        native.xpbd_color_stencils(np.array([[0, -1]], dtype=np.int32))
+
+
+@pytest.mark.parametrize("make", [scenes.cloth_config1, lambda: scenes.cube(6)], ids=["cloth", "cube"])
+def test_xpbd_params_match_oracle(make):
+    """effectiveCompliance() form of the C++ mirror against the oracle's compliance (computed from the raw
+    addStencil arguments), and the inverse masses."""
+    s = make()
+    pg = s.particle_group()
+    o = ol.from_scene(s)
+    assert np.array_equal(pg.inv_masses(), native.inverse_masses(s.m))
+    for i, (kind, st, _) in enumerate(s.energies):
+        xp = pg.xpbd_params(i)
+        cm = o.compliance(i)
+        pr = o.prepared(i)
+        t = native.KIND[kind]
+        if t == native.MEMBRANE_STVK:
+            got_c, got_p = xp[:, 4:], xp[:, :4]
+            ref_p = pr[:, :4]
+        elif t == native.STABLE_NEO_HOOKEAN:
+            got_c, got_p = xp[:, 9:13], np.column_stack([xp[:, :9], xp[:, 13]])
+            ref_p = np.column_stack([pr[:, :9], pr[:, 11]])
+        elif t == native.SPRING_ANCHOR:
+            got_c, got_p, ref_p = xp[:, 3:], xp[:, :3], pr[:, :3]
+        else:
+            got_c, got_p, ref_p = xp[:, 1:], xp[:, :1], pr[:, :1]
+        assert np.allclose(got_c, cm, rtol=1e-14, atol=0), kind
+        assert np.array_equal(got_p, ref_p) or np.allclose(got_p, ref_p, rtol=1e-15), kind
+
+
+def _xpbd_layout(kind, p, cm):
+    """Oracle prepared params + compliance -> spg_xpbd_energy params of one stencil."""
+    if kind == "spring":
+        return np.array([p[0], cm[0]])
+    if kind == "anchor":
+        return np.array([*p[:3], cm[0]])
+    if kind == "stvk":
+        return np.concatenate([p[:4], cm])
+    if kind == "bending":
+        return np.array([p[0], cm[0]])
+    return np.concatenate([p[:9], cm, [p[11]]])
+
+
+def _projection_cases():
+    import test_xpbd_oracle as T
+    rng = np.random.default_rng(5)
+    out = []
+    for kind, xs, p in T.CASES:
+        ns = xs.shape[0]
+        w = rng.uniform(0.5, 2.0, size=ns)
+        c, J = ol.constraints(kind, xs.ravel(), p)
+        for scale in (0.0, 1.0, 1e3):  # stiff, balanced and soft compliance
+            base = float(np.mean(np.sum(J * J, axis=0) * np.mean(w))) * (1 / 60) ** 2
+            out.append((kind, xs, w, p, T._compliance(kind, scale * base)))
+    # SNH near rest (F ~ I), where dC0/dx and dC1/dx are nearly parallel and only alpha~ keeps A regular:
+    # a 2 cm tet with Y = 1e5 and 1e7 (det(A) / A00 A11 ~ 2e-2 and 3e-3: FP32 path) and Y = 1e8 (~3e-4:
+    # the kernels' FP64 fallback runs), nu = 0.45
+    x0 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.02
+    D = np.linalg.inv((x0[1:] - x0[0]).T)
+    vol = 0.02 ** 3 / 6
+    for young in (1e5, 1e7, 1e8):
+        mu = young / 2.9
+        lam = mu / (1 - 0.9)
+        p = np.concatenate([D.ravel(), [lam * vol, mu * vol, 1 + mu / lam]])
+        cm = np.array([1 / lam, 0, 0, 1 / mu]) / vol
+        out.append(("snh", x0 + 2e-4 * rng.normal(size=(4, 3)), np.full(4, 1 / (1000 * vol)), p, cm))
+    return out
+
+
+PROJ = _projection_cases()
+
+
+@pytest.mark.parametrize("case", range(len(PROJ)))
+def test_xpbd_closed_forms_on_host_match_oracle(case):
+    """Closed-form projections (xpbd_element.h, host instantiation) against the oracle's generic projection
+    with dual-number constraint gradients: FP64 within 1e-12 of the displacement, FP32 (the kernels' path
+    with its FP64 fallback) within 1e-5."""
+    kind, xs, w, p, cm = PROJ[case]
+    dt = 1 / 60
+    ref = ol.xpbd_project(kind, xs.ravel(), w, p, cm, dt)
+    xp = _xpbd_layout(kind, p, cm)
+    scale = max(np.abs(ref).max(), 1e-300)
+    for prec, tol in ((1, 1e-12), (0, 1e-5)):
+        got = native.xpbd_project_stencils(kind, xs.ravel()[None], w[None], xp[None], dt, precision=prec,
+                                           device=-1)[0]
+        assert np.abs(got - ref).max() <= tol * scale, (kind, prec, got, ref)
+
+
+def test_xpbd_closed_form_guards_on_host():
+    dt = 0.01
+    # spring with both ends immovable, and with coincident ends: no correction
+    z = native.xpbd_project_stencils("spring", [[0, 0, 0, 1, 0, 0]], [[0, 0]], [[0.5, 0.0]], dt, 0, -1)
+    assert not np.any(z)
+    z = native.xpbd_project_stencils("spring", [[0, 0, 0, 0, 0, 0]], [[1, 1]], [[0.5, 0.0]], dt, 0, -1)
+    assert not np.any(z)
+    # anchor exactly at its anchor point
+    z = native.xpbd_project_stencils("anchor", [[1, 2, 3]], [[1.0]], [[1, 2, 3, 1e-3]], dt, 0, -1)
+    assert not np.any(z)
+    # zero-mass vertices are never moved
+    import test_xpbd_oracle as T
+    for kind, xs, p in T.CASES[:5]:
+        ns = xs.shape[0]
+        w = np.ones(ns)
+        w[0] = 0.0
+        nc = ol.num_constraints(kind)
+        xp = _xpbd_layout(kind, p, T._compliance(kind, 1e-3))
+        d = native.xpbd_project_stencils(kind, xs.ravel()[None], w[None], xp[None], dt, 0, -1)[0]
+        assert not np.any(d[0]), kind

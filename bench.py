@@ -20,9 +20,11 @@ learned    = time to a 1e-4 relative FP64 force residual, learned warm start vs 
              cloth (network trained in this run on the GPU from 256 config-4-style cloths)
 batched    = 256-cloth batch (config 4) throughput, one launch per colour for all instances
 volumetric = 64^3-cell Stable Neo-Hookean cube (config 3) throughput
+xpbd       = the XPBD parallel Gauss-Seidel step (spg's XPBD(true)) on the same cloth: stencil projections/s,
+             per-type projection-kernel rooflines, and the FP64 oracle's XPBD step on the host cores
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu] [--no-learned]
-                       [--no-batched] [--no-volumetric]
+                       [--no-batched] [--no-volumetric] [--no-xpbd] [--xpbd-substeps 20]
 """
 import argparse
 import json
@@ -53,6 +55,9 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=20, help="trajectory steps of the learned comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloth batched measurement")
     ap.add_argument("--no-volumetric", action="store_true", help="skip the 64^3 cube measurement")
+    ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD measurement")
+    ap.add_argument("--xpbd-substeps", type=int, default=20, help="XPBD substeps per step")
+    ap.add_argument("--xpbd-steps", type=int, default=30, help="timed XPBD steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -129,6 +134,7 @@ def main():
 
     batched = None if args.no_batched else batched_throughput(args)
     volumetric = None if args.no_volumetric else volumetric_throughput(args)
+    xpbd = None if args.no_xpbd else xpbd_throughput(s, args)
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
 
     cpu = None
@@ -165,6 +171,7 @@ def main():
         "learned": learned_cmp,
         "batched": batched,
         "volumetric": volumetric,
+        "xpbd": xpbd,
     }
     print(json.dumps(out))
 
@@ -234,6 +241,84 @@ def volumetric_throughput(args, steps=20):
                        "anchored, rest scaled 1.05 in y, 20 iterations/step, hipGraph replay"}
     out.update(_throughput([cube], steps))
     return out
+
+
+def xpbd_throughput(scene, args):
+    """XPBD(true) on the bench's cloth: each step is args.xpbd_substeps substeps of one projection of every
+    stencil, in stencil-colour launches (14 bending + 7 membrane + 1 anchor per substep), captured in a
+    hipGraph; every step starts from the initial state. Roofline per projection kernel type from an
+    event-timed step right after the throughput region."""
+    import numpy as np
+    from spg_amd import native
+
+    nsub, steps = args.xpbd_substeps, args.xpbd_steps
+    t0 = time.perf_counter()
+    ctx = native.XpbdContext([scene.xpbd_object()], parallel_gauss_seidel=True)
+    setup = time.perf_counter() - t0
+    ctx.set_launch(use_graph=1, block_size=256)
+    n_st = sum(len(st) for _, st, _ in scene.energies)
+    ctx.save_state()
+    for _ in range(3):
+        ctx.restore_state()
+        ctx.step(scene.dt, nsub, scene.gravity)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.restore_state()
+        ctx.step(scene.dt, nsub, scene.gravity)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.restore_state()
+    ctx.step(scene.dt, nsub, scene.gravity)
+    ctx.synchronize()
+    st = ctx.stats()
+    ctx.set_timing(False)
+    finite = bool(np.isfinite(ctx.state()[0]).all())
+    per_type = {}
+    for k, d in st["per_type"].items():
+        ms = d["ms"] / d["launches"]
+        gbs = d["bytes"] / d["launches"] / (ms * 1e-3) / 1e9
+        per_type[k] = {"launches": d["launches"], "mean_launch_us": ms * 1e3, "achieved_GBs": gbs,
+                       "frac": gbs / HBM_PEAK_GBS, "bytes_per_launch": d["bytes"] / d["launches"]}
+    dom = max(per_type, key=lambda k: st["per_type"][k]["ms"])
+    out = {"workload": f"same cloth, XPBD parallel Gauss-Seidel, dt=1/60, {nsub} substeps/step, "
+                       f"{ctx.num_launches()} projection launches per substep, hipGraph replay",
+           "value": n_st * nsub * steps / wall, "unit": "stencil-projections/s",
+           "ms_per_step": wall * 1e3 / steps, "steps": steps, "substeps": nsub, "stencils": n_st,
+           "projection_ms_per_substep": st["projection_ms"] / nsub,
+           "roofline": {"bound": "hbm", "kernel": f"xpbdProjectKernel<{dom}>", "achieved": per_type[dom]["achieved_GBs"],
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": per_type[dom]["frac"]},
+           "per_type": per_type, "fp64_solves": st["fp64_solves"], "setup_s": setup, "state_finite": finite}
+    if not args.no_cpu:
+        out["cpu_baseline"] = xpbd_cpu_baseline(scene)
+    return out
+
+
+def xpbd_cpu_baseline(scene, budget_s=8.0):
+    """FP64 oracle XPBD step (OpenMP over the stencils of a colour, as xpbd.cpp:91 parallelises) on the host
+    cores: repeated one-substep steps from the initial state for about budget_s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    import numpy as np
+
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ol.set_threads(cores)
+    o = ol.from_scene(scene)
+    v0 = np.zeros_like(scene.x)
+    n_st = sum(len(st) for _, st, _ in scene.energies)
+    o.xpbd_step(scene.dt, 1, scene.gravity, parallel=True)  # first step: stencil colouring (xpbd.cpp:18-21)
+    n, t = 0, 0.0
+    while t < budget_s:
+        o.set_state(scene.x, v0)
+        t0 = time.perf_counter()
+        o.xpbd_step(scene.dt, 1, scene.gravity, parallel=True)
+        t += time.perf_counter() - t0
+        n += 1
+    return {"value": n_st * n / t, "unit": "stencil-projections/s", "cores": cores, "kind": "port",
+            "sample": f"{n} one-substep XPBD steps of the same cloth after a first step that coloured the "
+                      f"stencils, FP64, dual-number constraint gradients (as TinyAD in the reference), {t:.1f} s"}
 
 
 def learned_comparison(scene, args, launch):

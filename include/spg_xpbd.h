@@ -82,6 +82,10 @@ typedef struct {
                                     membrane 56, bending 24, SNH 68, anchor 20 B) + 32 B per particle
                                     (float4 read and written) */
     int64_t fp64_solves;         /* projections recomputed in FP64 because A was ill-conditioned */
+    /* the same launches split by energy type (index = SPG_VBD_* type id, entry 0 unused) */
+    int64_t type_launches[6];
+    double type_ms[6];
+    double type_bytes[6];
 } spg_xpbd_stats;
 
 typedef struct spg_xpbd_ctx spg_xpbd_ctx;

@@ -663,6 +663,9 @@ struct Object {
     // incidence: per vertex, entries (energy, stencil, slot) in energy-major, ascending stencil order
     std::vector<int64_t> incPtr;
     std::vector<std::array<int32_t, 3>> inc;
+    // XPBD stencil groups per energy (computeParallelStencilGroups, xpbd.cpp:18-21: first step only)
+    bool xpbdGroupsDone = false;
+    std::vector<std::vector<std::vector<int32_t>>> xpbdGroups;
 };
 
 struct OracleError : std::runtime_error {
@@ -1224,17 +1227,21 @@ void xpbdStep(Object &o, double dtStep, int nsub, const double *g, bool parallel
 {
     const double dt = dtStep / nsub;  // xpbd.cpp:22
     const double invdt = 1 / dt;
-    // computeParallelStencilGroups (xpbd.cpp:134-160): per energy, stencil ids per colour, ascending
-    std::vector<std::vector<std::vector<int32_t>>> groups(o.energies.size());
-    if (parallel)
+    // computeParallelStencilGroups (xpbd.cpp:134-160): per energy, stencil ids per colour, ascending;
+    // computed on the first step and kept (xpbd.cpp:18-21)
+    if (parallel && !o.xpbdGroupsDone) {
+        o.xpbdGroups.assign(o.energies.size(), {});
         for (size_t ei = 0; ei < o.energies.size(); ++ei) {
             const auto &e = o.energies[ei];
             const auto col = colorStencils(e.st.data(), e.n(), e.ssize);
             for (int64_t i = 0; i < e.n(); ++i) {
-                if (static_cast<int32_t>(groups[ei].size()) < col[i] + 1) groups[ei].resize(col[i] + 1);
-                groups[ei][col[i]].push_back(static_cast<int32_t>(i));
+                if (static_cast<int32_t>(o.xpbdGroups[ei].size()) < col[i] + 1) o.xpbdGroups[ei].resize(col[i] + 1);
+                o.xpbdGroups[ei][col[i]].push_back(static_cast<int32_t>(i));
             }
         }
+        o.xpbdGroupsDone = true;
+    }
+    const auto &groups = o.xpbdGroups;
     const int64_t nv = o.nv;
     for (int s = 0; s < nsub; ++s) {
         const std::vector<double> xOld(o.x);  // xpbd.cpp:47-54
@@ -1303,6 +1310,7 @@ int orc_add_energy(void *h, int type, int64_t n, const int32_t *stencils, const 
     return guard([&] {
         addEnergy(*static_cast<Object *>(h), type, n, stencils, raw);
         static_cast<Object *>(h)->precompDone = false;
+        static_cast<Object *>(h)->xpbdGroupsDone = false;
     });
 }
 
@@ -1418,6 +1426,7 @@ int orc_reset(void *h)
     o.x = o.xInitial;
     std::fill(o.v.begin(), o.v.end(), 0.0);
     o.precompDone = false;
+    o.xpbdGroupsDone = false;
     return 0;
 }
 

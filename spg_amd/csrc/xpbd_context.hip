@@ -180,6 +180,9 @@ struct spg_xpbd_ctx {
             stats.projection_launches += 1;
             stats.stencil_projections += L.end - L.begin;
             stats.algorithmic_bytes += L.bytes;
+            stats.type_launches[L.type] += 1;
+            stats.type_ms[L.type] += ms;
+            stats.type_bytes[L.type] += L.bytes;
             eventPool.push_back(events[i]);
         }
         events.clear();

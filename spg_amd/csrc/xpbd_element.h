@@ -160,11 +160,13 @@ SPG_HD R xpbdSnh(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, R w0, R w1, R w2, R w3,
 
 // ---------------------------------------------------------------------------------------------
 // FP32 projections with the FP64 fallback of the kernels: a membrane or SNH projection whose
-// det(A) / prod(diag A) falls below kXpbdCondMin is recomputed in FP64 from the same FP32 inputs
-// (J W J^T + alpha~ then loses more than about 3 digits to cancellation in FP32). They return true
-// when the fallback ran.
+// det(A) / prod(diag A) falls below kXpbdCondMin is recomputed in FP64 from the same FP32 inputs.
+// The FP32 rounding of J then reaches dlambda amplified by about prod(diag A) / det(A): below 2e-2 that
+// exceeds 5e-6 of the displacement. (Near F = I the two SNH constraint gradients are almost parallel
+// and only alpha~ keeps A regular, so stiff tetrahedra at rest take this path.) They return true when
+// the fallback ran.
 // ---------------------------------------------------------------------------------------------
-constexpr float kXpbdCondMin = 1e-3f;
+constexpr float kXpbdCondMin = 2e-2f;
 
 SPG_HD V3<double> toDouble(V3<float> a) { return {double(a.x), double(a.y), double(a.z)}; }
 SPG_HD V3<float> toFloat(V3<double> a) { return {float(a.x), float(a.y), float(a.z)}; }

@@ -236,8 +236,8 @@ def _projection_cases():
             base = float(np.mean(np.sum(J * J, axis=0) * np.mean(w))) * (1 / 60) ** 2
             out.append((kind, xs, w, p, T._compliance(kind, scale * base)))
     # SNH near rest (F ~ I), where dC0/dx and dC1/dx are nearly parallel and only alpha~ keeps A regular:
-    # a 2 cm tet with Y = 1e5 and 1e7 (det(A) / A00 A11 ~ 2e-2 and 3e-3: FP32 path) and Y = 1e8 (~3e-4:
-    # the kernels' FP64 fallback runs), nu = 0.45
+    # a 2 cm tet with Y = 1e5 (det(A) / A00 A11 ~ 2.2e-2: FP32 path), 1e7 and 1e8 (~3e-3, 5e-4: the kernels'
+    # FP64 fallback runs), nu = 0.45
     x0 = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float) * 0.02
     D = np.linalg.inv((x0[1:] - x0[0]).T)
     vol = 0.02 ** 3 / 6

@@ -200,7 +200,7 @@ struct spg_xpbd_ctx {
             if (s == 0)
                 hipCheck(launchXpbdBegin(pos.p, vel.p, xold.p, n, fh, hg[0], hg[1], hg[2], stream), "begin launch");
             else
-                hipCheck(launchXpbdBoundary(pos.p, vel.p, xold.p, n, invh, fh, hg[0], hg[1], hg[2], stream),
+                hipCheck(launchXpbdBoundary(pos.p, xold.p, n, invh, fh, hg[0], hg[1], hg[2], stream),
                          "boundary launch");
             for (size_t k = 0; k < launches.size(); ++k) {
                 const Launch &L = launches[k];

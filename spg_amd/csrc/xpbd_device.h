@@ -50,11 +50,12 @@ constexpr double kXpbdRecordBytes[6] = {0, 16, 56, 24, 68, 20};
 hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t end, int blockSize,
                              hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Substep boundaries (xpbd.cpp:47-70,110-121): begin: xold = x, v += h g, x += h v;
-// between two substeps: v = (x - xold) / h_prev, then begin; end: v = (x - xold) / h.
+// between two substeps: v = (x - xold) / h_prev, then begin, with v kept in registers (vel is not written);
+// end: v = (x - xold) / h. g arguments: h g.
 hipError_t launchXpbdBegin(float4 *pos, float4 *vel, float4 *xold, int32_t n, float h, float gx, float gy, float gz,
                            hipStream_t stream);
-hipError_t launchXpbdBoundary(float4 *pos, float4 *vel, float4 *xold, int32_t n, float invhPrev, float h, float gx,
-                              float gy, float gz, hipStream_t stream);
+hipError_t launchXpbdBoundary(float4 *pos, float4 *xold, int32_t n, float invhPrev, float h, float gx, float gy,
+                              float gz, hipStream_t stream);
 hipError_t launchXpbdEnd(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
                          hipStream_t stream);
 // Check hook: n independent projections of one type from FP64 inputs (xs: n * size * 3, w: n * size,

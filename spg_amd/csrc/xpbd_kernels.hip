@@ -106,15 +106,17 @@ __global__ void __launch_bounds__(256) xpbdBeginKernel(float4 *__restrict__ pos,
 }
 
 // End of one substep (particleGroup.cpp:65-72, v = (x - x_old) / h) fused with the start of the next.
-__global__ void __launch_bounds__(256) xpbdBoundaryKernel(float4 *__restrict__ pos, float4 *__restrict__ vel,
-                                                          float4 *__restrict__ xold, int32_t n, float invhPrev,
-                                                          float h, float hgx, float hgy, float hgz)
+// The velocity lives in registers only: inside a step nothing reads vel between two substeps, and the
+// end kernel writes the last substep's velocity.
+__global__ void __launch_bounds__(256) xpbdBoundaryKernel(float4 *__restrict__ pos, float4 *__restrict__ xold,
+                                                          int32_t n, float invhPrev, float h, float hgx, float hgy,
+                                                          float hgz)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float4 x = pos[i];
     const float4 o = xold[i];
-    float4 v = make_float4((x.x - o.x) * invhPrev, (x.y - o.y) * invhPrev, (x.z - o.z) * invhPrev, 0.f);
+    float3 v = make_float3((x.x - o.x) * invhPrev, (x.y - o.y) * invhPrev, (x.z - o.z) * invhPrev);
     xold[i] = x;
     v.x += hgx;
     v.y += hgy;
@@ -122,7 +124,6 @@ __global__ void __launch_bounds__(256) xpbdBoundaryKernel(float4 *__restrict__ p
     x.x += v.x * h;
     x.y += v.y * h;
     x.z += v.z * h;
-    vel[i] = v;
     pos[i] = x;
 }
 
@@ -190,11 +191,11 @@ hipError_t launchXpbdBegin(float4 *pos, float4 *vel, float4 *xold, int32_t n, fl
     return hipGetLastError();
 }
 
-hipError_t launchXpbdBoundary(float4 *pos, float4 *vel, float4 *xold, int32_t n, float invhPrev, float h, float gx,
-                              float gy, float gz, hipStream_t stream)
+hipError_t launchXpbdBoundary(float4 *pos, float4 *xold, int32_t n, float invhPrev, float h, float gx, float gy,
+                              float gz, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    xpbdBoundaryKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, vel, xold, n, invhPrev, h, gx, gy, gz);
+    xpbdBoundaryKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, xold, n, invhPrev, h, gx, gy, gz);
     return hipGetLastError();
 }
 

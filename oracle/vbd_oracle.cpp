@@ -8,6 +8,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -1267,6 +1268,366 @@ void xpbdStep(Object &o, double dtStep, int nsub, const double *g, bool parallel
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Implicit Euler family (spg/source/sim/solver/implicitEulerBase.cpp:102-240, implicitEulerBaraffWitkin.cpp,
+// implicitEulerNewtonDv.cpp, implicitEulerNewtonDx.cpp, implicitEulerNewtonRobust.cpp, bdf2.cpp):
+// global force vector, mass and stiffness matrices, and Eigen's Jacobi-preconditioned conjugate gradient.
+// DOF layout: 3 per particle, particle-major (ParticleGroup::getPositions, particleGroup.cpp:7-14).
+// ---------------------------------------------------------------------------------------------
+
+// Full element gradient and Hessian (3N and 3N x 3N) as energyGradient / energyHessian do: the spring by
+// its own closed form (springEnergy.cpp:101-134), the others by dual numbers over the energy (energy.h:268-270,
+// 348-362; no positive-definite projection).
+void elementGradHess(int type, const double *xs, const double *p, double *grad, double *hess)
+{
+    if (type == ORC_SPRING) {
+        const A3 s = a3sub(xs, xs + 3);
+        const double n = std::sqrt(a3dot(s, s));
+        const double k = p[1], L0 = p[0];
+        const A3 sh = {s[0] / n, s[1] / n, s[2] / n};
+        for (int i = 0; i < 3; ++i) {
+            const double g = k * (n - L0) * s[i] / n;
+            if (grad) {
+                grad[i] = g;
+                grad[3 + i] = -g;
+            }
+        }
+        if (hess)
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const double sst = sh[i] * sh[j];
+                    const double b = k * ((1 - L0 / n) * ((i == j ? 1.0 : 0.0) - sst) + sst);
+                    hess[i * 6 + j] = b;
+                    hess[(3 + i) * 6 + 3 + j] = b;
+                    hess[i * 6 + 3 + j] = -b;
+                    hess[(3 + i) * 6 + j] = -b;
+                }
+        return;
+    }
+    switch (stencilSize(type)) {
+    case 1:
+        if (grad) adElement<1, false>(type, xs, p, nullptr, grad, nullptr);
+        if (hess) adElement<1, true>(type, xs, p, nullptr, nullptr, hess);
+        return;
+    case 3:
+        if (grad) adElement<3, false>(type, xs, p, nullptr, grad, nullptr);
+        if (hess) adElement<3, true>(type, xs, p, nullptr, nullptr, hess);
+        return;
+    case 4:
+        if (grad) adElement<4, false>(type, xs, p, nullptr, grad, nullptr);
+        if (hess) adElement<4, true>(type, xs, p, nullptr, nullptr, hess);
+        return;
+    }
+    throw OracleError("unknown energy type");
+}
+
+// Compressed sparse rows with sorted, unique columns (Eigen's setFromTriplets sums duplicates).
+struct Csr {
+    int64_t n = 0;
+    std::vector<int64_t> ptr;
+    std::vector<int64_t> col;
+    std::vector<double> val;
+};
+
+struct Trip {
+    int64_t r, c;
+    double v;
+};
+
+Csr fromTriplets(int64_t n, std::vector<Trip> &t)
+{
+    std::stable_sort(t.begin(), t.end(), [](const Trip &a, const Trip &b) { return a.r != b.r ? a.r < b.r : a.c < b.c; });
+    Csr m;
+    m.n = n;
+    m.ptr.assign(n + 1, 0);
+    for (size_t k = 0; k < t.size();) {
+        size_t j = k;
+        double s = 0.0;
+        while (j < t.size() && t[j].r == t[k].r && t[j].c == t[k].c) s += t[j++].v;
+        m.col.push_back(t[k].c);
+        m.val.push_back(s);
+        ++m.ptr[t[k].r + 1];
+        k = j;
+    }
+    for (int64_t r = 0; r < n; ++r) m.ptr[r + 1] += m.ptr[r];
+    return m;
+}
+
+void spmv(const Csr &A, const std::vector<double> &x, std::vector<double> &y)
+{
+    y.assign(A.n, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < A.n; ++r) {
+        double s = 0.0;
+        for (int64_t k = A.ptr[r]; k < A.ptr[r + 1]; ++k) s += A.val[k] * x[A.col[k]];
+        y[r] = s;
+    }
+}
+
+// ImplicitEulerBase::getSystemForce (implicitEulerBase.cpp:102-143): gravity * mass + sum of -grad E.
+std::vector<double> systemForce(const Object &o, const double *x, const double *g)
+{
+    std::vector<double> f(3 * o.nv, 0.0);
+    for (int64_t i = 0; i < o.nv; ++i)
+        for (int d = 0; d < 3; ++d) f[3 * i + d] += g[d] * o.m[i];
+    for (const auto &e : o.energies) {
+        const int np = preparedParams(e.type);
+        const int64_t n = e.n();
+        std::vector<double> eg(n * 3 * e.ssize);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            double xs[12];
+            gatherStencil(o, x, e, i, xs);
+            elementGradHess(e.type, xs, &e.prm[i * np], &eg[i * 3 * e.ssize], nullptr);
+        }
+        for (int64_t i = 0; i < n; ++i)  // accumulateForces (energy.h:197-215), stencil order
+            for (int s = 0; s < e.ssize; ++s)
+                for (int d = 0; d < 3; ++d) f[3 * e.st[i * e.ssize + s] + d] -= eg[(i * e.ssize + s) * 3 + d];
+    }
+    return f;
+}
+
+// getSystemStiffnessMatrix (implicitEulerBase.cpp:180-222): triplets of -Hessian of every stencil.
+Csr stiffnessMatrix(const Object &o, const double *x)
+{
+    std::vector<Trip> t;
+    for (const auto &e : o.energies) {
+        const int np = preparedParams(e.type);
+        const int N = 3 * e.ssize;
+        const int64_t n = e.n();
+        std::vector<double> eh(n * N * N);
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            double xs[12];
+            gatherStencil(o, x, e, i, xs);
+            elementGradHess(e.type, xs, &e.prm[i * np], nullptr, &eh[i * N * N]);
+        }
+        for (int64_t i = 0; i < n; ++i)  // negativeHessianTriplets (energy.h:218-236)
+            for (int a = 0; a < e.ssize; ++a)
+                for (int b = 0; b < e.ssize; ++b)
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c)
+                            t.push_back({3 * int64_t(e.st[i * e.ssize + a]) + r, 3 * int64_t(e.st[i * e.ssize + b]) + c,
+                                         -eh[i * N * N + (3 * a + r) * N + 3 * b + c]});
+    }
+    return fromTriplets(3 * o.nv, t);
+}
+
+// alpha * M + beta * K with M = diag(masses) (getSystemMassMatrix, implicitEulerBase.cpp:145-178).
+Csr combine(const Object &o, double alpha, const Csr &K, double beta)
+{
+    std::vector<Trip> t;
+    for (int64_t r = 0; r < K.n; ++r)
+        for (int64_t k = K.ptr[r]; k < K.ptr[r + 1]; ++k) t.push_back({r, K.col[k], beta * K.val[k]});
+    for (int64_t i = 0; i < o.nv; ++i)
+        for (int d = 0; d < 3; ++d) t.push_back({3 * i + d, 3 * i + d, alpha * o.m[i]});
+    return fromTriplets(K.n, t);
+}
+
+double dot(const std::vector<double> &a, const std::vector<double> &b)
+{
+    double s = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) s += a[i] * b[i];
+    return s;
+}
+
+// Eigen's ConjugateGradient<SparseMatrix, Lower|Upper> with its default DiagonalPreconditioner, as
+// solveLinearSystem calls it (implicitEulerBase.cpp:224-239; tolerance 1e-5, 300 iterations,
+// implicitEulerBase.h:26-27). Eigen is an empty submodule of the reference; this restates its published
+// algorithm (Eigen 3 IterativeLinearSolvers, conjugate_gradient): zero initial guess, relative residual
+// threshold max(tol^2 |b|^2, smallest normal double), inverse-diagonal preconditioner (1 where the
+// diagonal is 0 or absent), the convergence test right after the residual update.
+std::vector<double> conjugateGradient(const Csr &A, const std::vector<double> &b, double tol, int maxIters, int *iters,
+                                      double *error)
+{
+    const int64_t n = A.n;
+    std::vector<double> x(n, 0.0), r = b, p(n), z(n), tmp(n), invdiag(n, 1.0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = A.ptr[i]; k < A.ptr[i + 1]; ++k)
+            if (A.col[k] == i && A.val[k] != 0.0) invdiag[i] = 1.0 / A.val[k];
+    const double rhsNorm2 = dot(b, b);
+    if (rhsNorm2 == 0) {
+        *iters = 0;
+        *error = 0;
+        return x;
+    }
+    const double threshold = std::max(tol * tol * rhsNorm2, std::numeric_limits<double>::min());
+    double residualNorm2 = dot(r, r);
+    if (residualNorm2 < threshold) {
+        *iters = 0;
+        *error = std::sqrt(residualNorm2 / rhsNorm2);
+        return x;
+    }
+    for (int64_t i = 0; i < n; ++i) p[i] = invdiag[i] * r[i];
+    double absNew = dot(r, p);
+    int it = 0;
+    while (it < maxIters) {
+        spmv(A, p, tmp);
+        const double alpha = absNew / dot(p, tmp);
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * tmp[i];
+        }
+        residualNorm2 = dot(r, r);
+        if (residualNorm2 < threshold) break;
+        for (int64_t i = 0; i < n; ++i) z[i] = invdiag[i] * r[i];
+        const double absOld = absNew;
+        absNew = dot(r, z);
+        const double beta = absNew / absOld;
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        ++it;
+    }
+    *iters = it;
+    *error = std::sqrt(residualNorm2 / rhsNorm2);
+    return x;
+}
+
+enum { IMPL_BARAFF_WITKIN = 0, IMPL_NEWTON_DV = 1, IMPL_NEWTON_DX = 2, IMPL_NEWTON_ROBUST = 3, IMPL_BDF2 = 4 };
+
+struct ImplicitState {
+    // NewtonRobust (implicitEulerNewtonRobust.h:19-26)
+    double currentDt = -1.0;
+    int consecutiveSuccessful = 0;
+    // BDF2 (bdf2.h:18-19)
+    std::vector<double> xMinusOne, vMinusOne;
+    // counters of the last step
+    int cgIterations = 0, linearSolves = 0, newtonIterations = 0, failedSteps = 0;
+};
+
+const double kCgTol = 1e-5;
+const int kCgMaxIters = 300;
+
+std::vector<double> solveCounted(const Csr &A, const std::vector<double> &b, ImplicitState &st)
+{
+    int it = 0;
+    double err = 0;
+    auto x = conjugateGradient(A, b, kCgTol, kCgMaxIters, &it, &err);
+    st.cgIterations += it;
+    st.linearSolves += 1;
+    return x;
+}
+
+// One substep of each solver on one object.
+void implicitSubstep(Object &o, ImplicitState &st, int solver, double dtStep, int nsub, const double *g)
+{
+    const int64_t N = 3 * o.nv;
+    const double dt = dtStep / nsub;
+    auto integrate = [&](double h) {  // particleGroup.cpp:57-63
+        for (int64_t i = 0; i < N; ++i) o.x[i] += o.v[i] * h;
+    };
+    if (solver == IMPL_BARAFF_WITKIN) {  // implicitEulerBaraffWitkin.cpp:31-53
+        const std::vector<double> v0 = o.v;
+        const auto f = systemForce(o, o.x.data(), g);
+        const Csr K = stiffnessMatrix(o, o.x.data());
+        const Csr LHS = combine(o, 1.0, K, -(dt * dt));
+        std::vector<double> Kv;
+        spmv(K, v0, Kv);
+        std::vector<double> rhs(N);
+        for (int64_t i = 0; i < N; ++i) rhs[i] = dt * (f[i] + dt * Kv[i]);
+        const auto dv = solveCounted(LHS, rhs, st);
+        for (int64_t i = 0; i < N; ++i) o.v[i] = v0[i] + dv[i];
+        integrate(dt);
+    } else if (solver == IMPL_NEWTON_DV) {  // implicitEulerNewtonDv.cpp:30-61
+        const std::vector<double> x0 = o.x, v0 = o.v;
+        integrate(dt);
+        const auto f = systemForce(o, o.x.data(), g);
+        const Csr K = stiffnessMatrix(o, o.x.data());
+        const Csr LHS = combine(o, 1.0, K, -(dt * dt));
+        std::vector<double> rhs(N);
+        for (int64_t i = 0; i < N; ++i) rhs[i] = dt * f[i];
+        const auto dv = solveCounted(LHS, rhs, st);
+        o.x = x0;
+        for (int64_t i = 0; i < N; ++i) o.v[i] = v0[i] + dv[i];
+        integrate(dt);
+    } else if (solver == IMPL_NEWTON_DX) {  // implicitEulerNewtonDx.cpp:30-58
+        const std::vector<double> x0 = o.x;
+        integrate(dt);
+        const auto f = systemForce(o, o.x.data(), g);
+        const Csr K = stiffnessMatrix(o, o.x.data());
+        const double invdt = 1. / dt;
+        const Csr LHS = combine(o, invdt * invdt, K, -1.0);
+        const auto dx = solveCounted(LHS, f, st);
+        for (int64_t i = 0; i < N; ++i) {  // integrateObjectsVelocitiesFromDx (implicitEulerBase.cpp:80-92)
+            o.x[i] += dx[i];
+            o.v[i] = (o.x[i] - x0[i]) * invdt;
+        }
+    } else if (solver == IMPL_BDF2) {  // bdf2.cpp:31-73
+        const std::vector<double> x0 = o.x, v0 = o.v;
+        if (st.xMinusOne.size() != x0.size() || st.vMinusOne.size() != v0.size()) {
+            st.xMinusOne = x0;
+            st.vMinusOne = v0;
+        }
+        const auto f = systemForce(o, o.x.data(), g);
+        const Csr K = stiffnessMatrix(o, o.x.data());
+        const Csr LHS = combine(o, 1.0, K, -4.0 / 9.0 * (dt * dt));
+        std::vector<double> w(N), Kw;
+        for (int64_t i = 0; i < N; ++i) w[i] = x0[i] - st.xMinusOne[i] + 2 * dt * v0[i];
+        spmv(K, w, Kw);
+        std::vector<double> rhs(N);
+        for (int64_t i = 0; i < N; ++i)
+            rhs[i] = 1.0 / 3.0 * o.m[i / 3] * (v0[i] - st.vMinusOne[i]) + 2.0 / 3.0 * dt * (f[i] + 1.0 / 3.0 * Kw[i]);
+        const auto dv = solveCounted(LHS, rhs, st);
+        for (int64_t i = 0; i < N; ++i) {
+            o.v[i] = v0[i] + dv[i];
+            o.x[i] = 4.0 / 3.0 * x0[i] - 1.0 / 3.0 * st.xMinusOne[i] + 2.0 / 3.0 * o.v[i] * dt;
+        }
+        st.xMinusOne = x0;
+        st.vMinusOne = v0;
+    } else {  // IMPL_NEWTON_ROBUST, implicitEulerNewtonRobust.cpp:31-122
+        const double newtonThreshold = 1e-5, splitting = 0.75;
+        const int maxNewton = 5, maxLineSearch = 5, increaseAfter = 5;
+        bool ok = true;
+        const double maxDt = dtStep / nsub;
+        if (st.currentDt == -1. || st.currentDt > maxDt) st.currentDt = maxDt;
+        const double h = st.currentDt;
+        const std::vector<double> x0 = o.x, v0 = o.v;
+        int newton = 0;
+        double stepResidual = 0.;
+        integrate(h);
+        std::vector<double> vi = v0;
+        do {
+            auto f = systemForce(o, o.x.data(), g);
+            const Csr K = stiffnessMatrix(o, o.x.data());
+            const Csr LHS = combine(o, 1.0, K, -(h * h));
+            std::vector<double> rhs(N);
+            for (int64_t i = 0; i < N; ++i) rhs[i] = h * f[i] - o.m[i / 3] * (vi[i] - v0[i]);
+            const double initialResidual = std::sqrt(dot(rhs, rhs));
+            const auto dv = solveCounted(LHS, rhs, st);
+            stepResidual = initialResidual;
+            int ls = 0;
+            double alpha = 1.;
+            while (stepResidual >= initialResidual && ls < maxLineSearch) {
+                std::vector<double> v(N);
+                for (int64_t i = 0; i < N; ++i) v[i] = vi[i] + dv[i] * alpha;
+                o.v = v;
+                o.x = x0;
+                integrate(h);
+                f = systemForce(o, o.x.data(), g);
+                std::vector<double> res(N);
+                for (int64_t i = 0; i < N; ++i) res[i] = h * f[i] - o.m[i / 3] * (v[i] - v0[i]);
+                stepResidual = std::sqrt(dot(res, res));
+                if (stepResidual < initialResidual) vi = v;
+                alpha *= 0.5;
+                ++ls;
+            }
+            if (stepResidual >= initialResidual && stepResidual > newtonThreshold) ok = false;
+            ++newton;
+        } while (stepResidual > newtonThreshold && ok && newton < maxNewton);
+        st.newtonIterations += newton;
+        if (stepResidual > newtonThreshold) ok = false;
+        if (!ok) {
+            o.x = x0;
+            o.v = v0;
+            st.currentDt *= splitting;
+            st.consecutiveSuccessful = 0;
+            ++st.failedSteps;
+        } else {
+            ++st.consecutiveSuccessful;
+            if (st.consecutiveSuccessful > increaseAfter && st.currentDt < maxDt) st.currentDt /= splitting;
+        }
+    }
+}
+
 template <class F>
 int guard(F &&f)
 {
@@ -1593,6 +1954,88 @@ int orc_xpbd_project(int type, const double *xs, const double *inv_masses, const
             projectGeneric<2, 1>(type, xs, inv_masses, prepared, compliance, dt, dx);
         else
             projectStencilDx(type, xs, inv_masses, prepared, compliance, dt, dx);
+    });
+}
+
+/* ---- implicit Euler family ---- */
+int orc_system_force(void *h, const double *x, const double *gravity, double *f)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        const auto v = systemForce(o, x, gravity);
+        std::copy(v.begin(), v.end(), f);
+    });
+}
+
+/* -Hessian as CSR (rows 3 nv, sorted unique columns). Call with col/val NULL to get nnz. */
+int orc_stiffness_csr(void *h, const double *x, int64_t *nnz, int64_t *ptr, int64_t *col, double *val)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        const Csr K = stiffnessMatrix(o, x);
+        *nnz = static_cast<int64_t>(K.val.size());
+        if (ptr) std::copy(K.ptr.begin(), K.ptr.end(), ptr);
+        if (col) std::copy(K.col.begin(), K.col.end(), col);
+        if (val) std::copy(K.val.begin(), K.val.end(), val);
+    });
+}
+
+/* Eigen-style Jacobi-preconditioned CG on A = alpha M + beta K(x) (K = -Hessian at x). */
+int orc_cg_solve(void *h, const double *x, double alpha, double beta, const double *b, double tol, int max_iters,
+                 double *out, int *iters, double *error)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        checkPrepared(o);
+        const Csr A = combine(o, alpha, stiffnessMatrix(o, x), beta);
+        const std::vector<double> bb(b, b + A.n);
+        const auto r = conjugateGradient(A, bb, tol, max_iters, iters, error);
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+
+/* Generic CG on a CSR matrix given by the caller (checks of the CG restatement itself). */
+int orc_cg_csr(int64_t n, const int64_t *ptr, const int64_t *col, const double *val, const double *b, double tol,
+               int max_iters, double *out, int *iters, double *error)
+{
+    return guard([&] {
+        Csr A;
+        A.n = n;
+        A.ptr.assign(ptr, ptr + n + 1);
+        A.col.assign(col, col + ptr[n]);
+        A.val.assign(val, val + ptr[n]);
+        const std::vector<double> bb(b, b + n);
+        const auto r = conjugateGradient(A, bb, tol, max_iters, iters, error);
+        std::copy(r.begin(), r.end(), out);
+    });
+}
+
+void *orc_implicit_new(void) { return new ImplicitState; }
+void orc_implicit_free(void *s) { delete static_cast<ImplicitState *>(s); }
+
+/* One step (nsubsteps substeps) of solver 0 BaraffWitkin, 1 NewtonDv, 2 NewtonDx, 3 NewtonRobust, 4 BDF2
+   on the object's state. counters (may be NULL): cg iterations, linear solves, newton iterations, failed
+   substeps of this step; current_dt (may be NULL): NewtonRobust's dt after the step. */
+int orc_implicit_step(void *h, void *state, int solver, double dt, int nsubsteps, const double *gravity,
+                      int64_t *counters, double *current_dt)
+{
+    return guard([&] {
+        auto &o = *static_cast<Object *>(h);
+        auto &st = *static_cast<ImplicitState *>(state);
+        checkPrepared(o);
+        if (solver < 0 || solver > 4) throw OracleError("unknown solver");
+        if (!(dt > 0) || nsubsteps < 1) throw OracleError("invalid dt/substeps");
+        st.cgIterations = st.linearSolves = st.newtonIterations = st.failedSteps = 0;
+        for (int s = 0; s < nsubsteps; ++s) implicitSubstep(o, st, solver, dt, nsubsteps, gravity);
+        if (counters) {
+            counters[0] = st.cgIterations;
+            counters[1] = st.linearSolves;
+            counters[2] = st.newtonIterations;
+            counters[3] = st.failedSteps;
+        }
+        if (current_dt) *current_dt = st.currentDt;
     });
 }
 

@@ -1,5 +1,5 @@
 /*
- * FP64 CPU oracle for the spg Vertex Block Descent and XPBD steps — TEST INFRASTRUCTURE ONLY.
+ * FP64 CPU oracle for the spg VBD, XPBD and implicit Euler steps — TEST INFRASTRUCTURE ONLY.
  *
  * This library is the checker, not the product. Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it. The product (libspg_vbd.so / libspg_host.so)
@@ -19,6 +19,11 @@
  *   spg/source/sim/energy/springEnergy.cpp:48-78       (the spring's own projectPosition)
  *   the dConstraints lambdas of the five energy sources (constraint gradients by the same dual numbers)
  *   spg/source/utils/graphColoring.cpp:21-70          (colorStencils + greedyColoring)
+ * and the implicit Euler family on the same energies:
+ *   spg/source/sim/solver/implicitEulerBase.cpp:102-240 (system force, mass and stiffness matrices, CG)
+ *   spg/source/sim/solver/implicitEuler{BaraffWitkin,NewtonDv,NewtonDx,NewtonRobust}.cpp, bdf2.cpp
+ *   Eigen's ConjugateGradient with DiagonalPreconditioner (an empty submodule of the reference; its
+ *   published algorithm is restated: zero initial guess, threshold max(tol^2 |b|^2, DBL_MIN))
  * with two derivative back-ends: closed-form per-vertex blocks, and the oracle's own
  * forward-mode second-order dual numbers that evaluate the full element derivative twice per
  * incidence, as the reference does through TinyAD.
@@ -109,6 +114,19 @@ int orc_constraints(int type, const double *xs, const double *prepared, double *
    routes a spring through the generic path instead of its own closed form */
 int orc_xpbd_project(int type, const double *xs, const double *inv_masses, const double *prepared,
                      const double *compliance, double dt, int generic_spring, double *dx);
+
+/* ---- implicit Euler family ----
+ * DOFs: 3 per particle, particle-major. K = -Hessian of the energies (getSystemStiffnessMatrix), M = diag(m). */
+int orc_system_force(void *h, const double *x, const double *gravity, double *f);
+int orc_stiffness_csr(void *h, const double *x, int64_t *nnz, int64_t *ptr, int64_t *col, double *val);
+int orc_cg_solve(void *h, const double *x, double alpha, double beta, const double *b, double tol, int max_iters,
+                 double *out, int *iters, double *error);
+int orc_cg_csr(int64_t n, const int64_t *ptr, const int64_t *col, const double *val, const double *b, double tol,
+               int max_iters, double *out, int *iters, double *error);
+void *orc_implicit_new(void);
+void orc_implicit_free(void *s);
+int orc_implicit_step(void *h, void *state, int solver, double dt, int nsubsteps, const double *gravity,
+                      int64_t *counters, double *current_dt);
 
 #ifdef __cplusplus
 }

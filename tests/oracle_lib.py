@@ -57,6 +57,15 @@ def lib():
         L.orc_xpbd_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_int]
         L.orc_constraints.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp]
         L.orc_xpbd_project.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp]
+        L.orc_system_force.argtypes = [ctypes.c_void_p, _dp, _dp, _dp]
+        L.orc_stiffness_csr.argtypes = [ctypes.c_void_p, _dp, _lp, _lp, _lp, _dp]
+        L.orc_cg_solve.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double, ctypes.c_double, _dp, ctypes.c_double,
+                                   ctypes.c_int, _dp, _ip, _dp]
+        L.orc_cg_csr.argtypes = [ctypes.c_int64, _lp, _lp, _dp, _dp, ctypes.c_double, ctypes.c_int, _dp, _ip, _dp]
+        L.orc_implicit_new.restype = ctypes.c_void_p
+        L.orc_implicit_free.argtypes = [ctypes.c_void_p]
+        L.orc_implicit_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        _dp, _lp, _dp]
         _lib = L
     return _lib
 
@@ -139,6 +148,23 @@ def color_stencils(stencils):
     return out, nc.value
 
 
+BARAFF_WITKIN, NEWTON_DV, NEWTON_DX, NEWTON_ROBUST, BDF2 = 0, 1, 2, 3, 4
+
+
+def cg_csr(ptr, col, val, b, tol=1e-5, max_iters=300):
+    """The oracle's Eigen-style Jacobi-preconditioned CG on a caller CSR matrix -> (x, iterations, error)."""
+    ptr = np.ascontiguousarray(ptr, dtype=np.int64)
+    col = np.ascontiguousarray(col, dtype=np.int64)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    x = np.zeros(len(b))
+    it = ctypes.c_int(0)
+    err = ctypes.c_double(0)
+    _check(lib().orc_cg_csr(len(b), ptr.ctypes.data_as(_lp), col.ctypes.data_as(_lp), _d(val), _d(b), tol, max_iters,
+                            _d(x), ctypes.byref(it), ctypes.byref(err)))
+    return x, it.value, err.value
+
+
 def bending_stencils(faces, nv):
     faces = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 3)
     n = ctypes.c_int64(0)
@@ -160,6 +186,9 @@ class Oracle:
         self.types = []
 
     def __del__(self):
+        if getattr(self, "_impl", None):
+            lib().orc_implicit_free(self._impl)
+            self._impl = None
         if getattr(self, "h", None):
             lib().orc_free(self.h)
             self.h = None
@@ -243,6 +272,51 @@ class Oracle:
     def xpbd_step(self, dt, nsubsteps=1, gravity=(0.0, -9.8, 0.0), parallel=True):
         g = np.asarray(gravity, dtype=np.float64)
         _check(lib().orc_xpbd_step(self.h, dt, nsubsteps, _d(g), int(parallel)))
+
+    def system_force(self, x, gravity):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        g = np.asarray(gravity, dtype=np.float64)
+        f = np.zeros(3 * self.nv)
+        _check(lib().orc_system_force(self.h, _d(x), _d(g), _d(f)))
+        return f
+
+    def stiffness(self, x):
+        """K = -Hessian at x as scipy.sparse.csr_matrix (3 nv square)."""
+        import scipy.sparse as sp
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        nnz = ctypes.c_int64(0)
+        _check(lib().orc_stiffness_csr(self.h, _d(x), ctypes.byref(nnz), None, None, None))
+        ptr = np.zeros(3 * self.nv + 1, dtype=np.int64)
+        col = np.zeros(nnz.value, dtype=np.int64)
+        val = np.zeros(nnz.value)
+        _check(lib().orc_stiffness_csr(self.h, _d(x), ctypes.byref(nnz), ptr.ctypes.data_as(_lp),
+                                       col.ctypes.data_as(_lp), _d(val)))
+        return sp.csr_matrix((val, col, ptr), shape=(3 * self.nv, 3 * self.nv))
+
+    def cg_solve(self, x, alpha, beta, b, tol=1e-5, max_iters=300):
+        """CG on alpha M + beta K(x) -> (solution, iterations, error)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        out = np.zeros(3 * self.nv)
+        it = ctypes.c_int(0)
+        err = ctypes.c_double(0)
+        _check(lib().orc_cg_solve(self.h, _d(x), alpha, beta, _d(b), tol, max_iters, _d(out), ctypes.byref(it),
+                                  ctypes.byref(err)))
+        return out, it.value, err.value
+
+    def implicit_step(self, solver, dt, nsubsteps=1, gravity=(0.0, -9.8, 0.0)):
+        """One step of an implicit solver; returns {cg_iterations, linear_solves, newton_iterations,
+        failed_substeps, current_dt}. The solver's own state (BDF2 history, NewtonRobust dt) lives in this
+        oracle object."""
+        if getattr(self, "_impl", None) is None:
+            self._impl = lib().orc_implicit_new()
+        g = np.asarray(gravity, dtype=np.float64)
+        cnt = np.zeros(4, dtype=np.int64)
+        cd = ctypes.c_double(0)
+        _check(lib().orc_implicit_step(self.h, self._impl, solver, dt, nsubsteps, _d(g), cnt.ctypes.data_as(_lp),
+                                       ctypes.byref(cd)))
+        return {"cg_iterations": int(cnt[0]), "linear_solves": int(cnt[1]), "newton_iterations": int(cnt[2]),
+                "failed_substeps": int(cnt[3]), "current_dt": cd.value}
 
     def energy(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)

@@ -1,0 +1,87 @@
+// Device-side layout and launch interface of the implicit Euler family (global force, stiffness matrix,
+// conjugate gradient).
+//
+// Layout in HBM (one context, all objects concatenated, particle-major DOFs as
+// ParticleGroup::getPositions, spg/source/sim/simObject/particleGroup.cpp:7-14):
+//   * x, v: FP64, 3 per particle.
+//   * Per energy type an element table: global particle ids (size per stencil) and the prepared FP64
+//     parameters of spg_vbd_energy.
+//   * Element derivatives: per element its gradient (3N doubles) and Hessian (3N x 3N, column-major:
+//     entry (r, c) at c * 3N + r), element-major.
+//   * Global matrices in 3x3 block-sparse rows (BSR): row i holds the blocks of the particles sharing a
+//     stencil with i (and i itself), columns ascending; a block is 9 doubles, row-major. The Hessian H is
+//     assembled per upper block (i <= j) from a contribution list (element Hessian sub-blocks, in energy
+//     order and ascending stencil) and mirrored, so H is exactly symmetric. K = -H is the reference's
+//     stiffness matrix (implicitEulerBase.cpp:180-222); system matrices are a M + c H.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace spgvbd {
+
+struct ImplicitElems {
+    int type;              // 1 spring, 2 membrane, 3 bending, 4 SNH, 5 anchor
+    int64_t n;
+    const int32_t *idx;    // n * size global particle ids
+    const double *prm;     // n * params
+    double *grad;          // n * 3 size
+    double *hess;          // n * (3 size)^2, column-major per element
+};
+
+struct Bsr {
+    int64_t rows;          // particles
+    const int64_t *rowPtr; // rows + 1
+    const int32_t *col;    // block columns
+    double *val;           // 9 per block
+};
+
+// gradients and Hessians of all elements of one type at x
+hipError_t launchElementDerivatives(const ImplicitElems &e, const double *x, hipStream_t stream);
+// gradients only (e.hess unused)
+hipError_t launchElementGradients(const ImplicitElems &e, const double *x, hipStream_t stream);
+// f_i = m_i g - sum of the incident element gradient slots (incidence: offsets in units of 3 doubles into
+// the gradient buffer, CSR over particles, energy order, ascending stencil)
+hipError_t launchGatherForce(int64_t nParticles, const double *mass, double gx, double gy, double gz,
+                             const int64_t *incPtr, const int64_t *incOff, const double *gradBase, double *f,
+                             hipStream_t stream);
+// H upper blocks (i <= j) from contribution lists (offsets in doubles of element Hessian sub-block (a, b)
+// origins, and the stride 3N of that element's columns packed as (offset << 4 | N3)), mirrored into the
+// lower blocks through `mirror` (index of block (j, i) for every upper block, -1 on the diagonal)
+hipError_t launchAssembleHessian(int64_t nUpper, const int64_t *upperBlock, const int64_t *contribPtr,
+                                 const int64_t *contrib, const double *hessBase, const int64_t *mirror, double *H,
+                                 hipStream_t stream);
+// A = c H + a M (M = diag(mass) on the diagonal blocks, diagBlock[i] = block index of (i, i))
+hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nParticles, const int64_t *diagBlock,
+                         const double *mass, double a, double *A, hipStream_t stream);
+// y = s * (A x), BSR 3x3 FP64; partial (may be null): per-block partial sums of x . y over the block's rows
+hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, int blocks,
+                         hipStream_t stream);
+
+// Conjugate gradient (Eigen's algorithm with the diagonal preconditioner). Scalars live on the device:
+struct CgScalars {
+    double rhsNorm2, threshold, absNew, alpha, residualNorm2, beta;
+    int32_t done;   // 1 once converged (or the right-hand side is 0)
+    int32_t iters;  // completed iterations (Eigen's convention)
+};
+constexpr int kCgBlocks = 1024;  // blocks of every partial-sum kernel (fixed: deterministic reductions)
+hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, double *x, double *r, double *p,
+                        double *partial, CgScalars *s, double tol, hipStream_t stream);
+// one iteration: tmp = A p; alpha; x += alpha p, r -= alpha tmp; convergence test; beta; p = D^-1 r + beta p
+hipError_t launchCgIteration(const Bsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
+                             double *tmp, double *partial, CgScalars *s, int32_t maxIters, hipStream_t stream);
+// invdiag = 1 / diagonal of A (1 where it is 0), from the diagonal blocks
+hipError_t launchInvDiag(int64_t nParticles, const int64_t *diagBlock, const double *A, double *invdiag,
+                         hipStream_t stream);
+
+// out = a x + b y + c z (y, z may be null)
+hipError_t launchLinComb(int64_t n, double *out, double a, const double *x, double b, const double *y, double c,
+                         const double *z, hipStream_t stream);
+// out_d = a * m_{d/3} * x_d + b * y_d (mass per particle; y may be null)
+hipError_t launchMassLinComb(int64_t n, double *out, double a, const double *mass, const double *x, double b,
+                             const double *y, hipStream_t stream);
+// deterministic |x|^2 into *out (device)
+hipError_t launchNorm2(int64_t n, const double *x, double *partial, double *out, hipStream_t stream);
+
+}  // namespace spgvbd

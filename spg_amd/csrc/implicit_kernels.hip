@@ -1,0 +1,534 @@
+// CDNA4 kernels of the implicit Euler family: FP64 element derivatives, force gather, block-sparse
+// Hessian assembly, BSR SpMV and the conjugate gradient, all with fixed summation orders (the results
+// do not depend on scheduling).
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+
+#include "implicit_device.h"
+#include "implicit_element.h"
+
+namespace spgvbd {
+
+namespace {
+
+inline int blocksFor(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
+
+// One element per thread: gradient (tangent-free values of the first pass) and the Hessian one column
+// per forward-mode pass, written straight to the element's column-major slot. One instantiation per
+// energy type keeps each kernel's registers to that energy's closed form.
+template <int TYPE>
+__global__ void __launch_bounds__(128) elementDerivKernel(ImplicitElems e, const double *__restrict__ x)
+{
+    constexpr int NS = fullStencilSize(TYPE);
+    constexpr int N = 3 * NS;
+    constexpr int NP = TYPE == 2 ? 13 : TYPE == 4 ? 12 : TYPE == 5 ? 4 : 2;
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= e.n) return;
+    double p[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) p[k] = e.prm[i * NP + k];
+    double xs[N];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int64_t v = e.idx[i * NS + s];
+        xs[3 * s] = x[3 * v];
+        xs[3 * s + 1] = x[3 * v + 1];
+        xs[3 * s + 2] = x[3 * v + 2];
+    }
+    double *H = e.hess + i * N * N;
+    double *g = e.grad + i * N;
+#pragma unroll 1
+    for (int j = 0; j < N; ++j) {
+        V3<Dual1> xd[NS], gd[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            xd[s] = {Dual1(xs[3 * s], 3 * s == j ? 1.0 : 0.0), Dual1(xs[3 * s + 1], 3 * s + 1 == j ? 1.0 : 0.0),
+                     Dual1(xs[3 * s + 2], 3 * s + 2 == j ? 1.0 : 0.0)};
+        fullGrad<NS, Dual1>(TYPE, xd, p, gd);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            H[j * N + 3 * s] = gd[s].x.d;
+            H[j * N + 3 * s + 1] = gd[s].y.d;
+            H[j * N + 3 * s + 2] = gd[s].z.d;
+        }
+        if (j == 0)
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                g[3 * s] = gd[s].x.v;
+                g[3 * s + 1] = gd[s].y.v;
+                g[3 * s + 2] = gd[s].z.v;
+            }
+    }
+}
+
+// Gradient only (no Hessian), in plain FP64: the force of the line-search trials.
+template <int TYPE>
+__global__ void __launch_bounds__(256) elementGradKernel(ImplicitElems e, const double *__restrict__ x)
+{
+    constexpr int NS = fullStencilSize(TYPE);
+    constexpr int NP = TYPE == 2 ? 13 : TYPE == 4 ? 12 : TYPE == 5 ? 4 : 2;
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= e.n) return;
+    double p[NP];
+#pragma unroll
+    for (int k = 0; k < NP; ++k) p[k] = e.prm[i * NP + k];
+    V3<double> xd[NS], gd[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const int64_t v = e.idx[i * NS + s];
+        xd[s] = {x[3 * v], x[3 * v + 1], x[3 * v + 2]};
+    }
+    fullGrad<NS, double>(TYPE, xd, p, gd);
+    double *g = e.grad + i * 3 * NS;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        g[3 * s] = gd[s].x;
+        g[3 * s + 1] = gd[s].y;
+        g[3 * s + 2] = gd[s].z;
+    }
+}
+
+__global__ void __launch_bounds__(256) gatherForceKernel(int64_t n, const double *__restrict__ mass, double gx,
+                                                         double gy, double gz, const int64_t *__restrict__ incPtr,
+                                                         const int64_t *__restrict__ incOff,
+                                                         const double *__restrict__ grad, double *__restrict__ f)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const double m = mass[i];
+    double fx = gx * m, fy = gy * m, fz = gz * m;
+    for (int64_t k = incPtr[i]; k < incPtr[i + 1]; ++k) {
+        const double *g = grad + 3 * incOff[k];
+        fx -= g[0];
+        fy -= g[1];
+        fz -= g[2];
+    }
+    f[3 * i] = fx;
+    f[3 * i + 1] = fy;
+    f[3 * i + 2] = fz;
+}
+
+__global__ void __launch_bounds__(256) assembleHessianKernel(int64_t nUpper, const int64_t *__restrict__ upperBlock,
+                                                             const int64_t *__restrict__ contribPtr,
+                                                             const int64_t *__restrict__ contrib,
+                                                             const double *__restrict__ hess,
+                                                             const int64_t *__restrict__ mirror, double *__restrict__ H)
+{
+    const int64_t u = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (u >= nUpper) return;
+    double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = contribPtr[u]; k < contribPtr[u + 1]; ++k) {
+        const int64_t c = contrib[k];
+        const double *h = hess + (c >> 4);
+        const int N3 = static_cast<int>(c & 15);
+        for (int cc = 0; cc < 3; ++cc)
+            for (int r = 0; r < 3; ++r) b[r * 3 + cc] += h[cc * N3 + r];
+    }
+    const int64_t blk = upperBlock[u];
+    const int64_t mir = mirror[u];
+    double *out = H + 9 * blk;
+    if (mir < 0) {  // diagonal block: symmetric part
+        for (int r = 0; r < 3; ++r)
+            for (int cc = 0; cc < 3; ++cc) out[r * 3 + cc] = 0.5 * (b[r * 3 + cc] + b[cc * 3 + r]);
+        return;
+    }
+    double *low = H + 9 * mir;
+    for (int r = 0; r < 3; ++r)
+        for (int cc = 0; cc < 3; ++cc) {
+            out[r * 3 + cc] = b[r * 3 + cc];
+            low[cc * 3 + r] = b[r * 3 + cc];
+        }
+}
+
+__global__ void __launch_bounds__(256) combineKernel(int64_t nBlocks, const double *__restrict__ H, double c,
+                                                     double *__restrict__ A)
+{
+    const int64_t k = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (k >= 9 * nBlocks) return;
+    A[k] = c * H[k];
+}
+
+__global__ void __launch_bounds__(256) addMassKernel(int64_t n, const int64_t *__restrict__ diagBlock,
+                                                     const double *__restrict__ mass, double a, double *__restrict__ A)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    double *d = A + 9 * diagBlock[i];
+    const double am = a * mass[i];
+    d[0] += am;
+    d[4] += am;
+    d[8] += am;
+}
+
+// Block-row SpMV: a group of G lanes per block row walks the row's 9 * nb values in order (coalesced),
+// each lane keeping the three row sums it touches; the group then reduces them in a fixed butterfly.
+constexpr int kSpmvG = 16;
+constexpr int kSpmvBlock = 256;
+
+__device__ __forceinline__ double groupSum(double v)
+{
+#pragma unroll
+    for (int o = kSpmvG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kSpmvG);
+    return v;
+}
+
+__device__ __forceinline__ double blockSumFixed(double v, double *sh)
+{
+    // wave-level fixed butterfly, then the waves' sums in order
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < static_cast<int>(blockDim.x >> 6); ++w) s += sh[w];
+    __syncthreads();
+    return s;
+}
+
+__global__ void __launch_bounds__(kSpmvBlock) bsrSpmvKernel(Bsr A, const double *__restrict__ x, double s,
+                                                            double *__restrict__ y, double *__restrict__ partial,
+                                                            const int32_t *__restrict__ done)
+{
+    __shared__ double sh[kSpmvBlock / 64];
+    if (done && *done) return;
+    const int lig = threadIdx.x % kSpmvG;
+    const int64_t rowsPerBlock = kSpmvBlock / kSpmvG;
+    double dotAcc = 0.0;
+    // grid-stride over block rows so the partial sums have a fixed number of blocks
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * rowsPerBlock; base < A.rows;
+         base += static_cast<int64_t>(gridDim.x) * rowsPerBlock) {
+        const int64_t row = base + threadIdx.x / kSpmvG;
+        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+        if (row < A.rows) {
+            const int64_t b0 = A.rowPtr[row], b1 = A.rowPtr[row + 1];
+            const int64_t e0 = 9 * b0, e1 = 9 * b1;
+            for (int64_t e = e0 + lig; e < e1; e += kSpmvG) {
+                const int64_t blk = e / 9;
+                const int w = static_cast<int>(e - 9 * blk);
+                const int r = w / 3, c = w - 3 * r;
+                const double t = A.val[e] * x[3 * static_cast<int64_t>(A.col[blk]) + c];
+                y0 += r == 0 ? t : 0.0;
+                y1 += r == 1 ? t : 0.0;
+                y2 += r == 2 ? t : 0.0;
+            }
+        }
+        y0 = groupSum(y0);
+        y1 = groupSum(y1);
+        y2 = groupSum(y2);
+        if (row < A.rows && lig == 0) {
+            const double z0 = s * y0, z1 = s * y1, z2 = s * y2;
+            y[3 * row] = z0;
+            y[3 * row + 1] = z1;
+            y[3 * row + 2] = z2;
+            if (partial) dotAcc += x[3 * row] * z0 + x[3 * row + 1] * z1 + x[3 * row + 2] * z2;
+        }
+    }
+    if (partial) {
+        const double t = blockSumFixed(dotAcc, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    }
+}
+
+// Fixed-order partial sums: block b owns a contiguous chunk; its threads sum strided, then a fixed
+// block reduction.
+template <class F>
+__device__ __forceinline__ double chunkSum(int64_t n, F term, double *sh)
+{
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b = blockIdx.x * chunk, e = b + chunk < n ? b + chunk : n;
+    double acc = 0.0;
+    for (int64_t i = b + threadIdx.x; i < e; i += blockDim.x) acc += term(i);
+    return blockSumFixed(acc, sh);
+}
+
+__device__ __forceinline__ double finalSum(const double *__restrict__ partial, int n, double *sh)
+{
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    return blockSumFixed(acc, sh);
+}
+
+__global__ void __launch_bounds__(256) cgInitKernel(int64_t n, const double *__restrict__ b,
+                                                    const double *__restrict__ invdiag, double *__restrict__ x,
+                                                    double *__restrict__ r, double *__restrict__ p,
+                                                    double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * chunk, e0 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
+        const double bi = b[i];
+        x[i] = 0.0;
+        r[i] = bi;
+        p[i] = invdiag[i] * bi;
+    }
+    const double bb = chunkSum(n, [&](int64_t i) { return b[i] * b[i]; }, sh);
+    const double rp = chunkSum(n, [&](int64_t i) { return b[i] * (invdiag[i] * b[i]); }, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = bb;
+        partial[gridDim.x + blockIdx.x] = rp;
+    }
+}
+
+__global__ void __launch_bounds__(1024) cgInitFinalKernel(const double *__restrict__ partial, int nb, CgScalars *s,
+                                                          double tol)
+{
+    __shared__ double sh[16];
+    const double bb = finalSum(partial, nb, sh);
+    const double rp = finalSum(partial + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        s->rhsNorm2 = bb;
+        s->absNew = rp;
+        s->iters = 0;
+        s->alpha = 0.0;
+        s->beta = 0.0;
+        s->residualNorm2 = bb;  // r = b - A 0
+        const double th = tol * tol * bb;
+        s->threshold = th > DBL_MIN ? th : DBL_MIN;
+        s->done = (bb == 0.0 || bb < s->threshold) ? 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(1024) cgAlphaKernel(const double *__restrict__ partial, int nb, CgScalars *s)
+{
+    __shared__ double sh[16];
+    if (s->done) return;
+    const double pAp = finalSum(partial, nb, sh);
+    if (threadIdx.x == 0) s->alpha = s->absNew / pAp;
+}
+
+__global__ void __launch_bounds__(256) cgUpdateKernel(int64_t n, const double *__restrict__ invdiag,
+                                                      double *__restrict__ x, double *__restrict__ r,
+                                                      const double *__restrict__ p, const double *__restrict__ tmp,
+                                                      double *__restrict__ partial, const CgScalars *s)
+{
+    __shared__ double sh[4];
+    if (s->done) return;
+    const double alpha = s->alpha;
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * chunk, e0 = b0 + chunk < n ? b0 + chunk : n;
+    double rr = 0.0, rz = 0.0;
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) {
+        x[i] += alpha * p[i];
+        const double ri = r[i] - alpha * tmp[i];
+        r[i] = ri;
+        rr += ri * ri;
+        rz += ri * (invdiag[i] * ri);
+    }
+    const double a = blockSumFixed(rr, sh);
+    const double c = blockSumFixed(rz, sh);
+    if (threadIdx.x == 0) {
+        partial[blockIdx.x] = a;
+        partial[gridDim.x + blockIdx.x] = c;
+    }
+}
+
+__global__ void __launch_bounds__(1024) cgTestKernel(const double *__restrict__ partial, int nb, CgScalars *s,
+                                                     int32_t maxIters)
+{
+    __shared__ double sh[16];
+    if (s->done) return;
+    const double rr = finalSum(partial, nb, sh);
+    const double rz = finalSum(partial + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        s->residualNorm2 = rr;
+        if (rr < s->threshold) {
+            s->done = 1;
+            return;
+        }
+        const double absOld = s->absNew;
+        s->absNew = rz;
+        s->beta = rz / absOld;
+        s->iters += 1;
+        if (s->iters >= maxIters) s->done = 2;  // iteration budget spent (x is the last iterate)
+    }
+}
+
+__global__ void __launch_bounds__(256) cgDirectionKernel(int64_t n, const double *__restrict__ invdiag,
+                                                         const double *__restrict__ r, double *__restrict__ p,
+                                                         const CgScalars *s)
+{
+    if (s->done) return;
+    const double beta = s->beta;
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    p[i] = invdiag[i] * r[i] + beta * p[i];
+}
+
+__global__ void __launch_bounds__(256) invDiagKernel(int64_t n, const int64_t *__restrict__ diagBlock,
+                                                     const double *__restrict__ A, double *__restrict__ invdiag)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    const double *d = A + 9 * diagBlock[i];
+    for (int k = 0; k < 3; ++k) {
+        const double a = d[4 * k];
+        invdiag[3 * i + k] = a != 0.0 ? 1.0 / a : 1.0;
+    }
+}
+
+__global__ void __launch_bounds__(256) linCombKernel(int64_t n, double *__restrict__ out, double a,
+                                                     const double *__restrict__ x, double b,
+                                                     const double *__restrict__ y, double c,
+                                                     const double *__restrict__ z)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    double v = a * x[i];
+    if (y) v += b * y[i];
+    if (z) v += c * z[i];
+    out[i] = v;
+}
+
+__global__ void __launch_bounds__(256) massLinCombKernel(int64_t n, double *__restrict__ out, double a,
+                                                         const double *__restrict__ mass,
+                                                         const double *__restrict__ x, double b,
+                                                         const double *__restrict__ y)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= n) return;
+    double v = a * mass[i / 3] * x[i];
+    if (y) v += b * y[i];
+    out[i] = v;
+}
+
+__global__ void __launch_bounds__(256) norm2PartialKernel(int64_t n, const double *__restrict__ x,
+                                                          double *__restrict__ partial)
+{
+    __shared__ double sh[4];
+    const double t = chunkSum(n, [&](int64_t i) { return x[i] * x[i]; }, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(1024) sumFinalKernel(const double *__restrict__ partial, int nb, double *out)
+{
+    __shared__ double sh[16];
+    const double t = finalSum(partial, nb, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+}  // namespace
+
+hipError_t launchElementDerivatives(const ImplicitElems &e, const double *x, hipStream_t stream)
+{
+    if (e.n <= 0) return hipSuccess;
+    const int grid = blocksFor(e.n, 128);
+    switch (e.type) {
+    case 1: elementDerivKernel<1><<<grid, 128, 0, stream>>>(e, x); break;
+    case 2: elementDerivKernel<2><<<grid, 128, 0, stream>>>(e, x); break;
+    case 3: elementDerivKernel<3><<<grid, 128, 0, stream>>>(e, x); break;
+    case 4: elementDerivKernel<4><<<grid, 128, 0, stream>>>(e, x); break;
+    case 5: elementDerivKernel<5><<<grid, 128, 0, stream>>>(e, x); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchElementGradients(const ImplicitElems &e, const double *x, hipStream_t stream)
+{
+    if (e.n <= 0) return hipSuccess;
+    const int grid = blocksFor(e.n, 256);
+    switch (e.type) {
+    case 1: elementGradKernel<1><<<grid, 256, 0, stream>>>(e, x); break;
+    case 2: elementGradKernel<2><<<grid, 256, 0, stream>>>(e, x); break;
+    case 3: elementGradKernel<3><<<grid, 256, 0, stream>>>(e, x); break;
+    case 4: elementGradKernel<4><<<grid, 256, 0, stream>>>(e, x); break;
+    case 5: elementGradKernel<5><<<grid, 256, 0, stream>>>(e, x); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchGatherForce(int64_t nParticles, const double *mass, double gx, double gy, double gz,
+                             const int64_t *incPtr, const int64_t *incOff, const double *gradBase, double *f,
+                             hipStream_t stream)
+{
+    if (nParticles <= 0) return hipSuccess;
+    gatherForceKernel<<<blocksFor(nParticles, 256), 256, 0, stream>>>(nParticles, mass, gx, gy, gz, incPtr, incOff,
+                                                                       gradBase, f);
+    return hipGetLastError();
+}
+
+hipError_t launchAssembleHessian(int64_t nUpper, const int64_t *upperBlock, const int64_t *contribPtr,
+                                 const int64_t *contrib, const double *hessBase, const int64_t *mirror, double *H,
+                                 hipStream_t stream)
+{
+    if (nUpper <= 0) return hipSuccess;
+    assembleHessianKernel<<<blocksFor(nUpper, 256), 256, 0, stream>>>(nUpper, upperBlock, contribPtr, contrib,
+                                                                       hessBase, mirror, H);
+    return hipGetLastError();
+}
+
+hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nParticles, const int64_t *diagBlock,
+                         const double *mass, double a, double *A, hipStream_t stream)
+{
+    if (nBlocks <= 0) return hipSuccess;
+    combineKernel<<<blocksFor(9 * nBlocks, 256), 256, 0, stream>>>(nBlocks, H, c, A);
+    if (nParticles > 0 && a != 0.0)
+        addMassKernel<<<blocksFor(nParticles, 256), 256, 0, stream>>>(nParticles, diagBlock, mass, a, A);
+    return hipGetLastError();
+}
+
+hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, int blocks,
+                         hipStream_t stream)
+{
+    if (A.rows <= 0) return hipSuccess;
+    const int rowsPerBlock = kSpmvBlock / kSpmvG;
+    int grid = blocks > 0 ? blocks : blocksFor(A.rows, rowsPerBlock);
+    bsrSpmvKernel<<<grid, kSpmvBlock, 0, stream>>>(A, x, s, y, partial, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, double *x, double *r, double *p,
+                        double *partial, CgScalars *s, double tol, hipStream_t stream)
+{
+    cgInitKernel<<<kCgBlocks, 256, 0, stream>>>(n, b, invdiag, x, r, p, partial);
+    cgInitFinalKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s, tol);
+    return hipGetLastError();
+}
+
+hipError_t launchCgIteration(const Bsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
+                             double *tmp, double *partial, CgScalars *s, int32_t maxIters, hipStream_t stream)
+{
+    bsrSpmvKernel<<<kCgBlocks, kSpmvBlock, 0, stream>>>(A, p, 1.0, tmp, partial, &s->done);
+    cgAlphaKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s);
+    cgUpdateKernel<<<kCgBlocks, 256, 0, stream>>>(n, invdiag, x, r, p, tmp, partial, s);
+    cgTestKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s, maxIters);
+    cgDirectionKernel<<<blocksFor(n, 256), 256, 0, stream>>>(n, invdiag, r, p, s);
+    return hipGetLastError();
+}
+
+hipError_t launchInvDiag(int64_t nParticles, const int64_t *diagBlock, const double *A, double *invdiag,
+                         hipStream_t stream)
+{
+    if (nParticles <= 0) return hipSuccess;
+    invDiagKernel<<<blocksFor(nParticles, 256), 256, 0, stream>>>(nParticles, diagBlock, A, invdiag);
+    return hipGetLastError();
+}
+
+hipError_t launchLinComb(int64_t n, double *out, double a, const double *x, double b, const double *y, double c,
+                         const double *z, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    linCombKernel<<<blocksFor(n, 256), 256, 0, stream>>>(n, out, a, x, b, y, c, z);
+    return hipGetLastError();
+}
+
+hipError_t launchMassLinComb(int64_t n, double *out, double a, const double *mass, const double *x, double b,
+                             const double *y, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    massLinCombKernel<<<blocksFor(n, 256), 256, 0, stream>>>(n, out, a, mass, x, b, y);
+    return hipGetLastError();
+}
+
+hipError_t launchNorm2(int64_t n, const double *x, double *partial, double *out, hipStream_t stream)
+{
+    norm2PartialKernel<<<kCgBlocks, 256, 0, stream>>>(n, x, partial);
+    sumFinalKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, out);
+    return hipGetLastError();
+}
+
+}  // namespace spgvbd

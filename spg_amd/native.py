@@ -88,6 +88,16 @@ class XpbdStats(ctypes.Structure):
         return d
 
 
+class ImplicitStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int64), ("linear_solves", ctypes.c_int64), ("cg_iterations", ctypes.c_int64),
+                ("newton_iterations", ctypes.c_int64), ("failed_substeps", ctypes.c_int64),
+                ("current_dt", ctypes.c_double), ("last_cg_error", ctypes.c_double), ("assembly_ms", ctypes.c_double),
+                ("cg_ms", ctypes.c_double), ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class NativeError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -178,6 +188,23 @@ def vbd_lib():
         _proto(L, "spg_xpbd_synchronize", ctypes.c_int, [c])
         _proto(L, "spg_xpbd_color_stencils", ctypes.c_int, [i64, i32, _ip, _ip])
         _proto(L, "spg_xpbd_project_stencils", ctypes.c_int, [i32, i64, _dp, _dp, _dp, dbl, i32, i32, _dp])
+        # implicit Euler family (include/spg_implicit.h)
+        _proto(L, "spg_implicit_create", ctypes.c_int, [ctypes.POINTER(c), ctypes.POINTER(VbdObject), i32, i32, i32])
+        _proto(L, "spg_implicit_destroy", None, [c])
+        _proto(L, "spg_implicit_last_error", ctypes.c_char_p, [c])
+        _proto(L, "spg_implicit_num_dofs", ctypes.c_int, [c, _lp])
+        _proto(L, "spg_implicit_num_blocks", ctypes.c_int, [c, _lp])
+        _proto(L, "spg_implicit_set_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_implicit_get_state", ctypes.c_int, [c, i32, _dp, _dp])
+        _proto(L, "spg_implicit_step", ctypes.c_int, [c, dbl, i32, _dp])
+        _proto(L, "spg_implicit_reset", ctypes.c_int, [c])
+        _proto(L, "spg_implicit_system_force", ctypes.c_int, [c, _dp, _dp])
+        _proto(L, "spg_implicit_stiffness", ctypes.c_int, [c, _lp, _ip, _dp])
+        _proto(L, "spg_implicit_cg_solve", ctypes.c_int, [c, dbl, dbl, _dp, dbl, i32, _dp, _ip, _dp])
+        _proto(L, "spg_implicit_get_stats", ctypes.c_int, [c, ctypes.POINTER(ImplicitStats)])
+        _proto(L, "spg_implicit_reset_stats", ctypes.c_int, [c])
+        _proto(L, "spg_implicit_set_timing", ctypes.c_int, [c, i32])
+        _proto(L, "spg_implicit_synchronize", ctypes.c_int, [c])
         _vbd = L
     return _vbd
 
@@ -891,3 +918,119 @@ def xpbd_project_stencils(kind, xs, inv_masses, params, dt, precision=0, device=
     if rc != 0:
         raise NativeError(rc, L.spg_xpbd_last_error(None).decode())
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Implicit Euler family (include/spg_implicit.h)
+# ------------------------------------------------------------------------------------------------
+
+BARAFF_WITKIN, NEWTON_DV, NEWTON_DX, NEWTON_ROBUST, BDF2 = 0, 1, 2, 3, 4
+IMPLICIT_SOLVERS = {"baraff_witkin": BARAFF_WITKIN, "newton_dv": NEWTON_DV, "newton_dx": NEWTON_DX,
+                    "newton_robust": NEWTON_ROBUST, "bdf2": BDF2}
+
+
+class ImplicitContext:
+    """A spg_implicit_ctx: particle groups advanced together by one implicit solver (one global system)."""
+
+    def __init__(self, objects, solver, device=0):
+        """objects: the dicts of VbdContext (x, v, m, energies in the stiffness-form spg_vbd_energy layout)."""
+        self._L = vbd_lib()
+        solver = IMPLICIT_SOLVERS[solver] if isinstance(solver, str) else int(solver)
+        keep = []
+        objs = (VbdObject * len(objects))()
+        for k, o in enumerate(objects):
+            x = _f64(o["x"], (-1, 3))
+            v = None if o.get("v") is None else _f64(o["v"], (-1, 3))
+            m = _f64(o["m"])
+            ens = (VbdEnergy * max(1, len(o["energies"])))()
+            for j, (t, st, prm) in enumerate(o["energies"]):
+                t = _kind(t)
+                st = _i32(st, (-1, STENCIL_SIZE[t]))
+                prm = _f64(prm, (st.shape[0], PARAMS[t]))
+                ens[j] = VbdEnergy(t, st.shape[0], _i(st), _d(prm))
+                keep += [st, prm]
+            objs[k] = VbdObject(x.shape[0], _d(x), _d(v), _d(m), len(o["energies"]), ens, None)
+            keep += [x, v, m, ens]
+        h = ctypes.c_void_p()
+        rc = self._L.spg_implicit_create(ctypes.byref(h), objs, len(objects), solver, device)
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_implicit_last_error(None).decode())
+        self.h = h.value
+        self.solver = solver
+        self.n_objects = len(objects)
+        self._n = [int(_f64(o["x"]).size // 3) for o in objects]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_implicit_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_implicit_last_error(self.h).decode())
+
+    def num_dofs(self):
+        n = ctypes.c_int64(0)
+        self._check(self._L.spg_implicit_num_dofs(self.h, ctypes.byref(n)))
+        return n.value
+
+    def num_blocks(self):
+        n = ctypes.c_int64(0)
+        self._check(self._L.spg_implicit_num_blocks(self.h, ctypes.byref(n)))
+        return n.value
+
+    def state(self, obj=0):
+        x, v = np.zeros((self._n[obj], 3)), np.zeros((self._n[obj], 3))
+        self._check(self._L.spg_implicit_get_state(self.h, obj, _d(x), _d(v)))
+        return x, v
+
+    def set_state(self, obj, x=None, v=None):
+        self._check(self._L.spg_implicit_set_state(self.h, obj, _d(None if x is None else _f64(x, (-1, 3))),
+                                                   _d(None if v is None else _f64(v, (-1, 3)))))
+
+    def step(self, dt, nsubsteps=1, gravity=(0.0, -9.8, 0.0)):
+        g = _f64(gravity)
+        self._check(self._L.spg_implicit_step(self.h, dt, nsubsteps, _d(g)))
+
+    def reset(self):
+        self._check(self._L.spg_implicit_reset(self.h))
+
+    def system_force(self, gravity):
+        g = _f64(gravity)
+        f = np.zeros(self.num_dofs())
+        self._check(self._L.spg_implicit_system_force(self.h, _d(g), _d(f)))
+        return f
+
+    def stiffness(self):
+        """K = -Hessian at the current positions as scipy.sparse.bsr_matrix (3x3 blocks)."""
+        import scipy.sparse as sp
+        nb = self.num_blocks()
+        nrows = self.num_dofs() // 3
+        ptr = np.zeros(nrows + 1, dtype=np.int64)
+        cols = np.zeros(nb, dtype=np.int32)
+        blocks = np.zeros((nb, 3, 3))
+        self._check(self._L.spg_implicit_stiffness(self.h, ptr.ctypes.data_as(_lp), _i(cols), _d(blocks)))
+        return sp.bsr_matrix((blocks, cols, ptr), shape=(3 * nrows, 3 * nrows))
+
+    def cg_solve(self, alpha, beta, b, tol=1e-5, max_iters=300):
+        b = _f64(b)
+        x = np.zeros(self.num_dofs())
+        it = ctypes.c_int32(0)
+        err = ctypes.c_double(0)
+        self._check(self._L.spg_implicit_cg_solve(self.h, alpha, beta, _d(b), tol, max_iters, _d(x),
+                                                  ctypes.byref(it), ctypes.byref(err)))
+        return x, it.value, err.value
+
+    def stats(self):
+        st = ImplicitStats()
+        self._check(self._L.spg_implicit_get_stats(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        self._check(self._L.spg_implicit_reset_stats(self.h))
+
+    def set_timing(self, enabled):
+        self._check(self._L.spg_implicit_set_timing(self.h, int(enabled)))
+
+    def synchronize(self):
+        self._check(self._L.spg_implicit_synchronize(self.h))

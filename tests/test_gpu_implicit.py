@@ -1,0 +1,180 @@
+"""GPU parity of the implicit Euler family (through the C-ABI of include/spg_implicit.h) against the FP64
+oracle on the same inputs.
+
+Bars (FP64 on both sides; only summation orders differ):
+  * system force: 1e-12 relative (max-norm error / max-norm of the force);
+  * stiffness matrix K = -Hessian: same block pattern as the oracle's CSR and 1e-10 relative entries
+    (forward-mode over the closed-form gradients against the oracle's second-order dual numbers), and
+    exactly symmetric;
+  * conjugate gradient on the same system: the same iteration count (Eigen's convention) within 1, and
+    solutions within 1e-6 of each other relative to the solution (both stop at a relative residual of
+    1e-5; the difference is what the two summation orders leave in the iterates);
+  * one step of each solver: positions within 1e-9 relative (max-norm error / max-norm of the
+    positions), velocities within 1e-6 of the velocity scale, the same solver counters.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle_lib as ol
+from gpu_common import rel_err
+from spg_amd import native, scenes
+
+pytestmark = pytest.mark.gpu
+
+SOLVERS = {"baraff_witkin": ol.BARAFF_WITKIN, "newton_dv": ol.NEWTON_DV, "newton_dx": ol.NEWTON_DX,
+           "newton_robust": ol.NEWTON_ROBUST, "bdf2": ol.BDF2}
+
+
+def _cloth():
+    return scenes.cloth(12, young=1e4, pin="top_row", noise=1e-3)
+
+
+def _cube():
+    return scenes.cube(3, stretch_y=1.02)
+
+
+def _springs():
+    x0, tets = scenes.kuhn_cube_mesh(3)
+    edges = np.unique(np.sort(np.concatenate([tets[:, [a, b]] for a in range(4) for b in range(a + 1, 4)]), axis=1),
+                      axis=0)
+    L0 = np.linalg.norm(x0[edges[:, 0]] - x0[edges[:, 1]], axis=1)
+    pins = np.flatnonzero(x0[:, 0] == 0.0)
+    return scenes.Scene(x=x0 * np.array([1.0, 1.03, 1.0]), x0=x0, m=np.full(len(x0), 1e-2),
+                        energies=[("anchor", pins[:, None], np.column_stack([x0[pins], np.full(len(pins), 1e4)])),
+                                  ("spring", edges, np.column_stack([L0, np.full(len(edges), 1e2)]))])
+
+
+SCENES = {"cloth": _cloth, "cube": _cube, "springs": _springs}
+
+
+def _pair(scene, solver="baraff_witkin", v=None):
+    ctx = native.ImplicitContext([scene.vbd_object()], solver)
+    o = ol.from_scene(scene)
+    x, _ = ctx.state()
+    if v is None:
+        v = np.zeros_like(x)
+    ctx.set_state(0, x, v)
+    o.set_state(x, v)
+    return ctx, o
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_system_force_matches_oracle(name):
+    s = SCENES[name]()
+    ctx, o = _pair(s)
+    f = ctx.system_force(s.gravity)
+    fo = o.system_force(s.x, s.gravity)
+    e = np.abs(f - fo).max() / np.abs(fo).max()
+    print(f"{name}: force rel {e:.2e}", flush=True)
+    assert e < 1e-12
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_stiffness_matches_oracle(name):
+    s = SCENES[name]()
+    ctx, o = _pair(s)
+    K = ctx.stiffness().tocsr()
+    Ko = o.stiffness(s.x)
+    assert (K - K.T).count_nonzero() == 0  # exactly symmetric
+    D = (K - Ko).toarray()
+    e = np.abs(D).max() / np.abs(Ko.toarray()).max()
+    print(f"{name}: stiffness rel {e:.2e}, {ctx.num_blocks()} blocks", flush=True)
+    assert e < 1e-10
+    # same block pattern: every oracle nonzero lies in a GPU block
+    pat = sp.csr_matrix((np.ones(K.nnz), K.indices, K.indptr), shape=K.shape)
+    assert (abs(Ko) > 0).multiply(pat).count_nonzero() == (abs(Ko) > 0).count_nonzero()
+
+
+@pytest.mark.parametrize("name", ["cloth", "cube"])
+def test_cg_matches_oracle(name):
+    s = SCENES[name]()
+    ctx, o = _pair(s)
+    rng = np.random.default_rng(4)
+    b = rng.normal(size=3 * s.n)
+    h = s.dt
+    for alpha, beta in ((1.0, -h * h), (1.0 / (h * h), -1.0)):
+        xg, ig, eg = ctx.cg_solve(alpha, beta, b)
+        xo, io, eo = o.cg_solve(s.x, alpha, beta, b)
+        d = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
+        print(f"{name}: CG iterations gpu {ig} oracle {io}, error {eg:.2e}/{eo:.2e}, solution diff {d:.2e}", flush=True)
+        assert abs(ig - io) <= 1 and d < 1e-6 and eg < 1e-5
+
+
+def test_cg_iteration_budget_and_zero_rhs():
+    s = _cloth()
+    ctx, o = _pair(s)
+    b = np.random.default_rng(1).normal(size=3 * s.n)
+    x, it, err = ctx.cg_solve(1.0, -s.dt ** 2, b, tol=1e-14, max_iters=3)
+    xo, io, eo = o.cg_solve(s.x, 1.0, -s.dt ** 2, b, tol=1e-14, max_iters=3)
+    assert it == io == 3 and abs(err - eo) <= 1e-9 * eo
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+    x, it, err = ctx.cg_solve(1.0, -s.dt ** 2, np.zeros(3 * s.n))
+    assert it == 0 and err == 0 and not np.any(x)
+
+
+@pytest.mark.parametrize("solver", list(SOLVERS))
+@pytest.mark.parametrize("name", ["cloth", "cube"])
+def test_one_step_matches_oracle(solver, name, oracle_threads):
+    s = SCENES[name]()
+    v0 = np.random.default_rng(2).normal(0, 0.02, size=s.x.shape)
+    ctx, o = _pair(s, solver, v=v0)
+    ctx.step(s.dt, 1, s.gravity)
+    info = o.implicit_step(SOLVERS[solver], s.dt, 1, s.gravity)
+    xg, vg = ctx.state()
+    xo, vo = o.state()
+    st = ctx.stats()
+    ex = rel_err(xg, xo)
+    ev = float(np.abs(vg - vo).max() / max(np.abs(vo).max(), 1e-300))
+    print(f"{name} {solver}: position rel {ex:.2e}, velocity rel {ev:.2e}, CG {st['cg_iterations']}/"
+          f"{info['cg_iterations']}, newton {st['newton_iterations']}/{info['newton_iterations']}", flush=True)
+    assert ex < 1e-9 and ev < 1e-6
+    assert abs(st["cg_iterations"] - info["cg_iterations"]) <= st["linear_solves"]
+    assert st["linear_solves"] == info["linear_solves"]
+    assert st["newton_iterations"] == info["newton_iterations"]
+    assert st["failed_substeps"] == info["failed_substeps"]
+
+
+@pytest.mark.parametrize("solver", ["bdf2", "newton_robust"])
+def test_several_steps_keep_solver_state(solver, oracle_threads):
+    """BDF2 keeps x_{-1}, v_{-1}; NewtonRobust keeps its dt: three steps of two substeps."""
+    s = _cloth()
+    ctx, o = _pair(s, solver)
+    for _ in range(3):
+        ctx.step(s.dt, 2, s.gravity)
+        info = o.implicit_step(SOLVERS[solver], s.dt, 2, s.gravity)
+    assert rel_err(ctx.state()[0], o.state()[0]) < 1e-8
+    if solver == "newton_robust":
+        assert ctx.stats()["current_dt"] == pytest.approx(info["current_dt"], rel=1e-15)
+
+
+def test_two_objects_form_one_system(oracle_threads):
+    """Objects are concatenated into one global system, as the reference assembles them: compare with the
+    oracle on a single object holding both cloths."""
+    a = scenes.cloth(8, young=1e4, pin="top_row", seed=3)
+    b = scenes.cloth(10, young=3e3, pin="top_row", seed=4)
+    ctx = native.ImplicitContext([a.vbd_object(), b.vbd_object()], "newton_dv")
+    merged = scenes.Scene(x=np.vstack([a.x, b.x]), x0=np.vstack([a.x0, b.x0]), m=np.concatenate([a.m, b.m]),
+                          energies=a.energies + [(k, st + a.n, raw) for k, st, raw in b.energies])
+    o = ol.from_scene(merged)
+    xa, va = ctx.state(0)
+    xb, vb = ctx.state(1)
+    o.set_state(np.vstack([xa, xb]), np.vstack([va, vb]))
+    ctx.step(a.dt, 1, a.gravity)
+    o.implicit_step(ol.NEWTON_DV, a.dt, 1, a.gravity)
+    xo = o.state()[0]
+    assert rel_err(np.vstack([ctx.state(0)[0], ctx.state(1)[0]]), xo) < 1e-9
+
+
+def test_invalid_inputs_rejected():
+    s = _cloth()
+    with pytest.raises(native.NativeError):
+        native.ImplicitContext([s.vbd_object()], 7)
+    ctx = native.ImplicitContext([s.vbd_object()], "bdf2")
+    x, v = ctx.state()
+    x[0, 0] = np.inf
+    with pytest.raises(native.NativeError) as e:
+        ctx.set_state(0, x, v)
+    assert e.value.code == native.ERR_NONFINITE
+    with pytest.raises(native.NativeError):
+        ctx.step(0.0, 1, s.gravity)

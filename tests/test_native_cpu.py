@@ -27,7 +27,8 @@ def _declared(header):
 
 
 @pytest.mark.parametrize("header,lib", [("spg_vbd.h", native.VBD_LIB), ("spg_vbd_learned.h", native.VBD_LIB),
-                                        ("spg_xpbd.h", native.VBD_LIB), ("spg_host.h", native.HOST_LIB)])
+                                        ("spg_xpbd.h", native.VBD_LIB), ("spg_implicit.h", native.VBD_LIB),
+                                        ("spg_host.h", native.HOST_LIB)])
 def test_exports(header, lib):
     native.host_lib()
     L = ctypes.CDLL(lib)

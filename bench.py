@@ -22,9 +22,12 @@ batched    = 256-cloth batch (config 4) throughput, one launch per colour for al
 volumetric = 64^3-cell Stable Neo-Hookean cube (config 3) throughput
 xpbd       = the XPBD parallel Gauss-Seidel step (spg's XPBD(true)) on the same cloth: stencil projections/s,
              per-type projection-kernel rooflines, and the FP64 oracle's XPBD step on the host cores
+implicit   = the Baraff-Witkin implicit Euler step (FP64: element Hessians, block-sparse assembly, Jacobi CG with
+             the reference's 1e-5 / 300-iteration settings) on the same cloth, the BSR SpMV roofline, and the
+             oracle's step on a 128x128 cloth on the host cores
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu] [--no-learned]
-                       [--no-batched] [--no-volumetric] [--no-xpbd] [--xpbd-substeps 20]
+                       [--no-batched] [--no-volumetric] [--no-xpbd] [--xpbd-substeps 20] [--no-implicit]
 """
 import argparse
 import json
@@ -58,6 +61,8 @@ def main():
     ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD measurement")
     ap.add_argument("--xpbd-substeps", type=int, default=20, help="XPBD substeps per step")
     ap.add_argument("--xpbd-steps", type=int, default=30, help="timed XPBD steps")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit Euler measurement")
+    ap.add_argument("--implicit-steps", type=int, default=4, help="timed Baraff-Witkin steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
     args = ap.parse_args()
@@ -135,6 +140,7 @@ def main():
     batched = None if args.no_batched else batched_throughput(args)
     volumetric = None if args.no_volumetric else volumetric_throughput(args)
     xpbd = None if args.no_xpbd else xpbd_throughput(s, args)
+    implicit = None if args.no_implicit else implicit_throughput(s, args)
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
 
     cpu = None
@@ -172,6 +178,7 @@ def main():
         "batched": batched,
         "volumetric": volumetric,
         "xpbd": xpbd,
+        "implicit": implicit,
     }
     print(json.dumps(out))
 
@@ -294,6 +301,85 @@ def xpbd_throughput(scene, args):
     if not args.no_cpu:
         out["cpu_baseline"] = xpbd_cpu_baseline(scene)
     return out
+
+
+def implicit_throughput(scene, args):
+    """ImplicitEulerBaraffWitkin::step on the bench's cloth: element Hessians and forces (FP64 forward mode),
+    upper-block assembly, A = M - dt^2 K, Jacobi CG (tolerance 1e-5, 300 iterations). Every step starts from
+    the initial state. Roofline: the BSR SpMV of A, timed on its own right after."""
+    import numpy as np
+    from spg_amd import native
+
+    steps = args.implicit_steps
+    t0 = time.perf_counter()
+    ctx = native.ImplicitContext([scene.vbd_object()], "baraff_witkin")
+    setup = time.perf_counter() - t0
+    x0, v0 = ctx.state()
+    ctx.step(scene.dt, 1, scene.gravity)  # warm-up
+    ctx.set_state(0, x0, v0)
+    ctx.reset_stats()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.set_state(0, x0, v0)
+        ctx.step(scene.dt, 1, scene.gravity)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    finite = bool(np.isfinite(ctx.state()[0]).all())
+    ctx.set_state(0, x0, v0)
+    ctx.reset_stats()
+    ctx.set_timing(True)
+    ctx.step(scene.dt, 1, scene.gravity)
+    tst = ctx.stats()
+    ctx.set_timing(False)
+    ctx.set_state(0, x0, v0)
+    spmv_ms = ctx.bench_spmv(1.0, -scene.dt ** 2, repeats=50)
+    bytes_spmv = ctx.spmv_bytes()
+    achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
+    dofs = ctx.num_dofs()
+    iters = st["cg_iterations"] / steps
+    out = {"workload": f"same cloth, ImplicitEulerBaraffWitkin (FP64), dt=1/60, 1 substep, CG tolerance 1e-5 / "
+                       f"300 iterations, {ctx.num_blocks()} 3x3 blocks, every step from the initial state",
+           "value": dofs * iters * steps / wall, "unit": "DOF x CG-iterations/s",
+           "ms_per_step": wall * 1e3 / steps, "steps": steps, "dofs": dofs, "cg_iterations_per_step": iters,
+           "last_cg_error": st["last_cg_error"], "assembly_ms_per_step": tst["assembly_ms"],
+           "cg_ms_per_step": tst["cg_ms"], "cg_us_per_iteration": tst["cg_ms"] * 1e3 / max(1, tst["cg_iterations"]),
+           "roofline": {"bound": "hbm", "kernel": "bsrSpmvKernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "mean_launch_us": spmv_ms * 1e3,
+                        "algorithmic_bytes_per_launch": bytes_spmv},
+           "setup_s": setup, "state_finite": finite}
+    if not args.no_cpu:
+        out["cpu_baseline"] = implicit_cpu_baseline()
+    return out
+
+
+def implicit_cpu_baseline(budget_s=10.0):
+    """The oracle's Baraff-Witkin step (dual-number Hessians as TinyAD, triplet assembly, the Eigen CG
+    restatement with an OpenMP SpMV) on a 128x128 cloth of the same construction, repeated from its initial
+    state for about budget_s; reported per DOF x CG iteration like the GPU value."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+    import numpy as np
+    from spg_amd import scenes
+
+    cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    ol.set_threads(cores)
+    sc = scenes.cloth_config2(seed=3, n=128)
+    o = ol.from_scene(sc)
+    v0 = np.zeros_like(sc.x)
+    n, t, its = 0, 0.0, 0
+    while t < budget_s:
+        o.set_state(sc.x, v0)
+        t0 = time.perf_counter()
+        info = o.implicit_step(ol.BARAFF_WITKIN, sc.dt, 1, sc.gravity)
+        t += time.perf_counter() - t0
+        its += info["cg_iterations"]
+        n += 1
+    dofs = 3 * sc.n
+    return {"value": dofs * its / t, "unit": "DOF x CG-iterations/s", "cores": cores, "kind": "port",
+            "sample": f"{n} Baraff-Witkin steps of a 128x128 cloth ({dofs} DOFs, {its / n:.0f} CG iterations per "
+                      f"step), FP64, {t:.1f} s ({t / n * 1e3:.0f} ms per step)"}
 
 
 def xpbd_cpu_baseline(scene, budget_s=8.0):

@@ -10,7 +10,10 @@
  *                     and the cloth bending wiring of apps/solverCompare.cpp:441-452.
  *   spg_solver_*      spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41) implemented by
  *                     spg::solver::VBDGpu, the GPU drop-in for spg::solver::VBD (vbd.h:14-41), or by
- *                     spg::solver::XPBDGpu, the GPU drop-in for spg::solver::XPBD (xpbd.h:15-30).
+ *                     spg::solver::XPBDGpu, the GPU drop-in for spg::solver::XPBD (xpbd.h:15-30), or by
+ *                     spg::solver::ImplicitEulerGpu, the GPU drop-in for the implicit Euler family
+ *                     (implicitEulerBaraffWitkin.h, implicitEulerNewtonDv.h, implicitEulerNewtonDx.h,
+ *                     implicitEulerNewtonRobust.h, bdf2.h).
  *
  * Errors: 0 on success, nonzero on failure with the message (the reference's exception text where
  * the reference throws) in spg_host_last_error().
@@ -21,6 +24,7 @@
 #include <stdint.h>
 
 #include "spg_vbd.h"
+#include "spg_implicit.h"
 #include "spg_xpbd.h"
 
 #ifdef __cplusplus
@@ -61,6 +65,8 @@ int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int3
 spg_solver *spg_solver_create_vbd_gpu(int32_t device);
 /* XPBDGpu(parallel_gauss_seidel != 0) on `device` (XPBD(true) / XPBD(false) of the reference). */
 spg_solver *spg_solver_create_xpbd_gpu(int32_t device, int32_t parallel_gauss_seidel);
+/* ImplicitEulerGpu of kind `solver` (SPG_IMPLICIT_* of spg_implicit.h); NULL for an unknown kind. */
+spg_solver *spg_solver_create_implicit_gpu(int32_t device, int32_t solver);
 void spg_solver_destroy(spg_solver *s);
 int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg); /* copies the group, shares its energies */
 int spg_solver_set_dt(spg_solver *s, double dt);
@@ -75,6 +81,7 @@ int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v);
 /* The C-ABI context behind the solver (NULL before the first step, or for the other solver type). */
 spg_vbd_ctx *spg_solver_context(spg_solver *s);
 spg_xpbd_ctx *spg_solver_xpbd_context(spg_solver *s);
+spg_implicit_ctx *spg_solver_implicit_context(spg_solver *s);
 
 #ifdef __cplusplus
 }

@@ -87,6 +87,11 @@ int spg_implicit_stiffness(spg_implicit_ctx *ctx, int64_t *row_ptr, int32_t *col
 int spg_implicit_cg_solve(spg_implicit_ctx *ctx, double alpha, double beta, const double *b, double tolerance,
                           int32_t max_iterations, double *x, int32_t *iterations, double *error);
 
+/* Measurement hook (not used by the solvers): assembles alpha M + beta K at the current positions and
+   times `repeats` back-to-back block-sparse products y = A x with HIP events on the context's stream;
+   *ms receives the mean time of one product. */
+int spg_implicit_bench_spmv(spg_implicit_ctx *ctx, double alpha, double beta, int32_t repeats, double *ms);
+
 int spg_implicit_get_stats(const spg_implicit_ctx *ctx, spg_implicit_stats *stats);
 int spg_implicit_reset_stats(spg_implicit_ctx *ctx);
 int spg_implicit_set_timing(spg_implicit_ctx *ctx, int32_t enabled);

@@ -1,7 +1,8 @@
 // Mirror of spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41,
 // source/sim/solver/baseSolver.cpp:10-35) for particle groups, and the GPU drop-ins for
-// spg::solver::VBD (spg/include/spg/sim/solver/vbd.h:14-41) and spg::solver::XPBD
-// (spg/include/spg/sim/solver/xpbd.h:15-30).
+// spg::solver::VBD (spg/include/spg/sim/solver/vbd.h:14-41), spg::solver::XPBD
+// (spg/include/spg/sim/solver/xpbd.h:15-30) and the implicit Euler family
+// (spg/include/spg/sim/solver/implicitEuler*.h, bdf2.h).
 #pragma once
 
 #include <memory>
@@ -12,6 +13,7 @@
 
 struct spg_vbd_ctx;
 struct spg_xpbd_ctx;
+struct spg_implicit_ctx;
 
 namespace spg {
 
@@ -127,6 +129,67 @@ protected:
     bool m_resident{false};
     bool m_hostDirty{true};
     spg_xpbd_ctx *m_ctx{nullptr};
+};
+
+// The implicit Euler family on one MI355X, in FP64: ImplicitEulerBase's force, mass and stiffness
+// assembly and its conjugate gradient (implicitEulerBase.cpp:102-240) on the GPU, driven by the step of
+// the chosen solver. The concrete classes below carry the reference's class names with a Gpu suffix.
+// The first step (or the first after reset() / a new object) builds the block-sparse pattern; the state
+// handling is that of VBDGpu.
+class ImplicitEulerGpu : public BaseSolver
+{
+public:
+    enum class Kind { BaraffWitkin = 0, NewtonDv = 1, NewtonDx = 2, NewtonRobust = 3, BDF2 = 4 };
+    ImplicitEulerGpu(Kind kind, int device);
+    ~ImplicitEulerGpu() override;
+    ImplicitEulerGpu(const ImplicitEulerGpu &) = delete;
+    ImplicitEulerGpu &operator=(const ImplicitEulerGpu &) = delete;
+
+    void step() override;
+    void reset() override;
+    void requirePrecomputationUpdate() { m_rebuild = true; }
+    void setResidentState(bool resident) { m_resident = resident; }
+    void syncToHost();
+    void markHostStateDirty() { m_hostDirty = true; }
+    // The C-ABI context (include/spg_implicit.h) behind this solver, or nullptr before the first step.
+    spg_implicit_ctx *context() { return m_ctx; }
+
+protected:
+    void buildContext();
+    void uploadState();
+
+    Kind m_kind;
+    int m_device{0};
+    bool m_rebuild{true};
+    bool m_resident{false};
+    bool m_hostDirty{true};
+    spg_implicit_ctx *m_ctx{nullptr};
+};
+
+class ImplicitEulerBaraffWitkinGpu : public ImplicitEulerGpu
+{
+public:
+    explicit ImplicitEulerBaraffWitkinGpu(int device = 0) : ImplicitEulerGpu(Kind::BaraffWitkin, device) {}
+};
+class ImplicitEulerNewtonDvGpu : public ImplicitEulerGpu
+{
+public:
+    explicit ImplicitEulerNewtonDvGpu(int device = 0) : ImplicitEulerGpu(Kind::NewtonDv, device) {}
+};
+class ImplicitEulerNewtonDxGpu : public ImplicitEulerGpu
+{
+public:
+    explicit ImplicitEulerNewtonDxGpu(int device = 0) : ImplicitEulerGpu(Kind::NewtonDx, device) {}
+};
+class ImplicitEulerNewtonRobustGpu : public ImplicitEulerGpu
+{
+public:
+    explicit ImplicitEulerNewtonRobustGpu(int device = 0) : ImplicitEulerGpu(Kind::NewtonRobust, device) {}
+};
+class BDF2Gpu : public ImplicitEulerGpu
+{
+public:
+    explicit BDF2Gpu(int device = 0) : ImplicitEulerGpu(Kind::BDF2, device) {}
 };
 
 }  // namespace solver

@@ -18,6 +18,7 @@ struct spg_solver {
     std::unique_ptr<spg::solver::BaseSolver> solver;
     spg::solver::VBDGpu *vbd = nullptr;    // the solver, when it is a VBDGpu
     spg::solver::XPBDGpu *xpbd = nullptr;  // the solver, when it is an XPBDGpu
+    spg::solver::ImplicitEulerGpu *implicit = nullptr;  // the solver, when it is an implicit Euler solver
 };
 
 namespace {
@@ -242,6 +243,18 @@ spg_solver *spg_solver_create_vbd_gpu(int32_t device)
     return s;
 }
 
+spg_solver *spg_solver_create_implicit_gpu(int32_t device, int32_t solver)
+{
+    if (solver < 0 || solver > 4) {
+        g_err = "unknown implicit solver";
+        return nullptr;
+    }
+    auto *s = new spg_solver;
+    s->implicit = new spg::solver::ImplicitEulerGpu(static_cast<spg::solver::ImplicitEulerGpu::Kind>(solver), device);
+    s->solver.reset(s->implicit);
+    return s;
+}
+
 spg_solver *spg_solver_create_xpbd_gpu(int32_t device, int32_t parallel_gauss_seidel)
 {
     auto *s = new spg_solver;
@@ -259,6 +272,7 @@ int spg_solver_add_object(spg_solver *s, const spg_pgroup *pg)
         s->solver->addObject(pg->group);
         if (s->vbd) s->vbd->requirePrecomputationUpdate();
         if (s->xpbd) s->xpbd->requirePrecomputationUpdate();
+        if (s->implicit) s->implicit->requirePrecomputationUpdate();
     });
 }
 
@@ -286,6 +300,7 @@ int spg_solver_set_resident_state(spg_solver *s, int32_t resident)
     return guarded([&] {
         if (s->vbd) s->vbd->setResidentState(resident != 0);
         if (s->xpbd) s->xpbd->setResidentState(resident != 0);
+        if (s->implicit) s->implicit->setResidentState(resident != 0);
     });
 }
 int spg_solver_step(spg_solver *s)
@@ -301,6 +316,7 @@ int spg_solver_sync_to_host(spg_solver *s)
     return guarded([&] {
         if (s->vbd) s->vbd->syncToHost();
         if (s->xpbd) s->xpbd->syncToHost();
+        if (s->implicit) s->implicit->syncToHost();
     });
 }
 
@@ -319,5 +335,9 @@ int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v)
 
 spg_vbd_ctx *spg_solver_context(spg_solver *s) { return s && s->vbd ? s->vbd->context() : nullptr; }
 spg_xpbd_ctx *spg_solver_xpbd_context(spg_solver *s) { return s && s->xpbd ? s->xpbd->context() : nullptr; }
+spg_implicit_ctx *spg_solver_implicit_context(spg_solver *s)
+{
+    return s && s->implicit ? s->implicit->context() : nullptr;
+}
 
 }  // extern "C"

@@ -170,7 +170,7 @@ struct spg_implicit_ctx {
         stats.cg_iterations += hostCg->iters;
         stats.last_cg_error = hostCg->rhsNorm2 > 0 ? std::sqrt(hostCg->residualNorm2 / hostCg->rhsNorm2) : 0.0;
         if (timing) {
-            const int64_t spmvs = hostCg->iters + (hostCg->done == 1 ? 1 : 0);
+            const int64_t spmvs = hostCg->rhsNorm2 == 0 ? 0 : hostCg->iters + (hostCg->done == 1 ? 1 : 0);
             stats.spmv_launches += spmvs;
             stats.spmv_bytes += spmvs * spmvBytes();
         }
@@ -211,7 +211,7 @@ struct spg_implicit_ctx {
             copy(v0.p, v.p);
             assemble(x.p, true);
             // rhs = dt (f + dt K v0) with K = -H
-            hipCheck(launchBsrSpmv(bsr(H.p), v0.p, -1.0, tmp.p, nullptr, 0, stream), "spmv launch");
+            hipCheck(launchBsrSpmv(bsr(H.p), v0.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
             lin(rhs.p, 1.0, f.p, dt, tmp.p);
             lin(rhs.p, dt, rhs.p);
             solve(1.0, dt * dt, rhs.p, dv.p, tol, maxIt);
@@ -251,7 +251,7 @@ struct spg_implicit_ctx {
             assemble(x.p, true);
             // w = x0 - x_{-1} + 2 dt v0; tmp = K w = -H w
             lin(rhs.p, 1.0, x0.p, -1.0, xm1.p, 2 * dt, v0.p);
-            hipCheck(launchBsrSpmv(bsr(H.p), rhs.p, -1.0, tmp.p, nullptr, 0, stream), "spmv launch");
+            hipCheck(launchBsrSpmv(bsr(H.p), rhs.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
             // tmp = 2/3 dt (f + 1/3 K w); rhs = 1/3 M (v0 - v_{-1}) + tmp
             lin(tmp.p, 2.0 / 3.0 * dt, f.p, 2.0 / 3.0 * dt / 3.0, tmp.p);
             lin(dv.p, 1.0, v0.p, -1.0, vm1.p);
@@ -544,7 +544,7 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
     c.invdiag.alloc(c.ndof);
     for (auto *b : {&c.cgr, &c.cgp, &c.cgtmp, &c.x0, &c.v0, &c.vi, &c.dv, &c.rhs, &c.tmp, &c.xm1, &c.vm1})
         b->alloc(c.ndof);
-    c.partial.alloc(2 * kCgBlocks);
+    c.partial.alloc(std::max<int64_t>(2 * kCgBlocks, spmvGrid(c.np)));
     c.cgs.alloc(1);
     c.normOut.alloc(1);
     hipCheck(hipHostMalloc(reinterpret_cast<void **>(&c.hostCg), sizeof(CgScalars)), "hipHostMalloc");
@@ -720,6 +720,29 @@ int spg_implicit_cg_solve(spg_implicit_ctx *ctx, double alpha, double beta, cons
         hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         if (iterations) *iterations = ctx->hostCg->iters;
         if (error) *error = ctx->stats.last_cg_error;
+    });
+}
+
+int spg_implicit_bench_spmv(spg_implicit_ctx *ctx, double alpha, double beta, int32_t repeats, double *ms)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !ms || repeats < 1) throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        const double g[3] = {0, 0, 0};
+        ctx->derivatives(ctx->x.p, true, g);
+        hipCheck(launchCombine(ctx->nBlocks, ctx->H.p, -beta, ctx->np, ctx->diagBlock.p, ctx->mass.p, alpha, ctx->A.p,
+                               ctx->stream),
+                 "combine launch");
+        hipCheck(launchBsrSpmv(ctx->bsr(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream), "spmv launch");
+        hipCheck(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
+        for (int32_t r = 0; r < repeats; ++r)
+            hipCheck(launchBsrSpmv(ctx->bsr(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream),
+                     "spmv launch");
+        hipCheck(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
+        hipCheck(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");
+        float t = 0.f;
+        hipCheck(hipEventElapsedTime(&t, ctx->ev0, ctx->ev1), "hipEventElapsedTime");
+        *ms = t / repeats;
     });
 }
 

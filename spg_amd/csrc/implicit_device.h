@@ -55,9 +55,10 @@ hipError_t launchAssembleHessian(int64_t nUpper, const int64_t *upperBlock, cons
 // A = c H + a M (M = diag(mass) on the diagonal blocks, diagBlock[i] = block index of (i, i))
 hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nParticles, const int64_t *diagBlock,
                          const double *mass, double a, double *A, hipStream_t stream);
-// y = s * (A x), BSR 3x3 FP64; partial (may be null): per-block partial sums of x . y over the block's rows
-hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, int blocks,
-                         hipStream_t stream);
+// y = s * (A x), BSR 3x3 FP64; partial (may be null): per-workgroup partial sums of x . y, spmvGrid(rows)
+// of them
+hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream);
+int spmvGrid(int64_t rows);
 
 // Conjugate gradient (Eigen's algorithm with the diagonal preconditioner). Scalars live on the device:
 struct CgScalars {
@@ -65,7 +66,8 @@ struct CgScalars {
     int32_t done;   // 1 once converged (or the right-hand side is 0)
     int32_t iters;  // completed iterations (Eigen's convention)
 };
-constexpr int kCgBlocks = 1024;  // blocks of every partial-sum kernel (fixed: deterministic reductions)
+constexpr int kCgBlocks = 1024;  // blocks of the vector partial-sum kernels (fixed: deterministic reductions)
+// the partial-sum buffer of a CG needs max(2 kCgBlocks, spmvGrid(rows)) doubles
 hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, double *x, double *r, double *p,
                         double *partial, CgScalars *s, double tol, hipStream_t stream);
 // one iteration: tmp = A p; alpha; x += alpha p, r -= alpha tmp; convergence test; beta; p = D^-1 r + beta p

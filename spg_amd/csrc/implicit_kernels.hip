@@ -161,18 +161,6 @@ __global__ void __launch_bounds__(256) addMassKernel(int64_t n, const int64_t *_
     d[8] += am;
 }
 
-// Block-row SpMV: a group of G lanes per block row walks the row's 9 * nb values in order (coalesced),
-// each lane keeping the three row sums it touches; the group then reduces them in a fixed butterfly.
-constexpr int kSpmvG = 16;
-constexpr int kSpmvBlock = 256;
-
-__device__ __forceinline__ double groupSum(double v)
-{
-#pragma unroll
-    for (int o = kSpmvG / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kSpmvG);
-    return v;
-}
-
 __device__ __forceinline__ double blockSumFixed(double v, double *sh)
 {
     // wave-level fixed butterfly, then the waves' sums in order
@@ -187,47 +175,50 @@ __device__ __forceinline__ double blockSumFixed(double v, double *sh)
     return s;
 }
 
+// Block-row SpMV: kSpmvG lanes per block row, lane l taking the row's blocks l, l + G, ... whole (9
+// values, its column and the 3 x entries), then a fixed butterfly over the G lanes. One row group per
+// kSpmvG threads over the whole grid, so every block row is in flight at once.
+constexpr int kSpmvG = 4;
+constexpr int kSpmvBlock = 256;
+
 __global__ void __launch_bounds__(kSpmvBlock) bsrSpmvKernel(Bsr A, const double *__restrict__ x, double s,
                                                             double *__restrict__ y, double *__restrict__ partial,
                                                             const int32_t *__restrict__ done)
 {
     __shared__ double sh[kSpmvBlock / 64];
     if (done && *done) return;
-    const int lig = threadIdx.x % kSpmvG;
-    const int64_t rowsPerBlock = kSpmvBlock / kSpmvG;
-    double dotAcc = 0.0;
-    // grid-stride over block rows so the partial sums have a fixed number of blocks
-    for (int64_t base = static_cast<int64_t>(blockIdx.x) * rowsPerBlock; base < A.rows;
-         base += static_cast<int64_t>(gridDim.x) * rowsPerBlock) {
-        const int64_t row = base + threadIdx.x / kSpmvG;
-        double y0 = 0.0, y1 = 0.0, y2 = 0.0;
-        if (row < A.rows) {
-            const int64_t b0 = A.rowPtr[row], b1 = A.rowPtr[row + 1];
-            const int64_t e0 = 9 * b0, e1 = 9 * b1;
-            for (int64_t e = e0 + lig; e < e1; e += kSpmvG) {
-                const int64_t blk = e / 9;
-                const int w = static_cast<int>(e - 9 * blk);
-                const int r = w / 3, c = w - 3 * r;
-                const double t = A.val[e] * x[3 * static_cast<int64_t>(A.col[blk]) + c];
-                y0 += r == 0 ? t : 0.0;
-                y1 += r == 1 ? t : 0.0;
-                y2 += r == 2 ? t : 0.0;
-            }
-        }
-        y0 = groupSum(y0);
-        y1 = groupSum(y1);
-        y2 = groupSum(y2);
-        if (row < A.rows && lig == 0) {
-            const double z0 = s * y0, z1 = s * y1, z2 = s * y2;
-            y[3 * row] = z0;
-            y[3 * row + 1] = z1;
-            y[3 * row + 2] = z2;
-            if (partial) dotAcc += x[3 * row] * z0 + x[3 * row + 1] * z1 + x[3 * row + 2] * z2;
+    const int64_t t = blockIdx.x * static_cast<int64_t>(kSpmvBlock) + threadIdx.x;
+    const int64_t row = t / kSpmvG;
+    const int lig = static_cast<int>(t % kSpmvG);
+    double y0 = 0.0, y1 = 0.0, y2 = 0.0;
+    if (row < A.rows) {
+        const int64_t k1 = A.rowPtr[row + 1];
+        for (int64_t k = A.rowPtr[row] + lig; k < k1; k += kSpmvG) {
+            const double *v = A.val + 9 * k;
+            const int64_t c = 3 * static_cast<int64_t>(A.col[k]);
+            const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+            y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
+            y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
+            y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
         }
     }
+#pragma unroll
+    for (int o = kSpmvG / 2; o > 0; o >>= 1) {
+        y0 += __shfl_xor(y0, o, kSpmvG);
+        y1 += __shfl_xor(y1, o, kSpmvG);
+        y2 += __shfl_xor(y2, o, kSpmvG);
+    }
+    double dot = 0.0;
+    if (row < A.rows && lig == 0) {
+        const double z0 = s * y0, z1 = s * y1, z2 = s * y2;
+        y[3 * row] = z0;
+        y[3 * row + 1] = z1;
+        y[3 * row + 2] = z2;
+        dot = x[3 * row] * z0 + x[3 * row + 1] * z1 + x[3 * row + 2] * z2;
+    }
     if (partial) {
-        const double t = blockSumFixed(dotAcc, sh);
-        if (threadIdx.x == 0) partial[blockIdx.x] = t;
+        const double b = blockSumFixed(dot, sh);
+        if (threadIdx.x == 0) partial[blockIdx.x] = b;
     }
 }
 
@@ -471,13 +462,12 @@ hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nPa
     return hipGetLastError();
 }
 
-hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, int blocks,
-                         hipStream_t stream)
+int spmvGrid(int64_t rows) { return blocksFor(rows * kSpmvG, kSpmvBlock); }
+
+hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream)
 {
     if (A.rows <= 0) return hipSuccess;
-    const int rowsPerBlock = kSpmvBlock / kSpmvG;
-    int grid = blocks > 0 ? blocks : blocksFor(A.rows, rowsPerBlock);
-    bsrSpmvKernel<<<grid, kSpmvBlock, 0, stream>>>(A, x, s, y, partial, nullptr);
+    bsrSpmvKernel<<<spmvGrid(A.rows), kSpmvBlock, 0, stream>>>(A, x, s, y, partial, nullptr);
     return hipGetLastError();
 }
 
@@ -492,8 +482,9 @@ hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, doubl
 hipError_t launchCgIteration(const Bsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
                              double *tmp, double *partial, CgScalars *s, int32_t maxIters, hipStream_t stream)
 {
-    bsrSpmvKernel<<<kCgBlocks, kSpmvBlock, 0, stream>>>(A, p, 1.0, tmp, partial, &s->done);
-    cgAlphaKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s);
+    const int grid = spmvGrid(A.rows);
+    bsrSpmvKernel<<<grid, kSpmvBlock, 0, stream>>>(A, p, 1.0, tmp, partial, &s->done);
+    cgAlphaKernel<<<1, 1024, 0, stream>>>(partial, grid, s);
     cgUpdateKernel<<<kCgBlocks, 256, 0, stream>>>(n, invdiag, x, r, p, tmp, partial, s);
     cgTestKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s, maxIters);
     cgDirectionKernel<<<blocksFor(n, 256), 256, 0, stream>>>(n, invdiag, r, p, s);

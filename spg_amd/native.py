@@ -201,6 +201,7 @@ def vbd_lib():
         _proto(L, "spg_implicit_system_force", ctypes.c_int, [c, _dp, _dp])
         _proto(L, "spg_implicit_stiffness", ctypes.c_int, [c, _lp, _ip, _dp])
         _proto(L, "spg_implicit_cg_solve", ctypes.c_int, [c, dbl, dbl, _dp, dbl, i32, _dp, _ip, _dp])
+        _proto(L, "spg_implicit_bench_spmv", ctypes.c_int, [c, dbl, dbl, i32, _dp])
         _proto(L, "spg_implicit_get_stats", ctypes.c_int, [c, ctypes.POINTER(ImplicitStats)])
         _proto(L, "spg_implicit_reset_stats", ctypes.c_int, [c])
         _proto(L, "spg_implicit_set_timing", ctypes.c_int, [c, i32])
@@ -234,6 +235,8 @@ def host_lib():
         _proto(L, "spg_solver_create_vbd_gpu", c, [i32])
         _proto(L, "spg_solver_create_xpbd_gpu", c, [i32, i32])
         _proto(L, "spg_solver_xpbd_context", c, [c])
+        _proto(L, "spg_solver_create_implicit_gpu", c, [i32, i32])
+        _proto(L, "spg_solver_implicit_context", c, [c])
         _proto(L, "spg_solver_destroy", None, [c])
         _proto(L, "spg_solver_add_object", ctypes.c_int, [c, c])
         _proto(L, "spg_solver_set_dt", ctypes.c_int, [c, dbl])
@@ -443,6 +446,21 @@ class XPBDGpu(VBDGpu):
         if not p:
             raise RuntimeError("no context before the first step")
         return XpbdContext._borrow(p, self)
+
+
+class ImplicitEulerGpu(VBDGpu):
+    """spg::solver::ImplicitEulerGpu (GPU drop-in for the implicit Euler family) through the C++ mirror."""
+
+    def __init__(self, solver, device=0):
+        self._L = host_lib()
+        kind = IMPLICIT_SOLVERS[solver] if isinstance(solver, str) else int(solver)
+        self.h = self._L.spg_solver_create_implicit_gpu(device, kind)
+        if not self.h:
+            raise NativeError(1, self._L.spg_host_last_error().decode())
+        self._groups = []
+
+    def set_iterations(self, n):
+        raise NotImplementedError("the implicit solvers have no iteration count")
 
 
 # ------------------------------------------------------------------------------------------------
@@ -1020,6 +1038,18 @@ class ImplicitContext:
         self._check(self._L.spg_implicit_cg_solve(self.h, alpha, beta, _d(b), tol, max_iters, _d(x),
                                                   ctypes.byref(it), ctypes.byref(err)))
         return x, it.value, err.value
+
+    def bench_spmv(self, alpha, beta, repeats=20):
+        """Mean ms of one block-sparse product with alpha M + beta K (measurement hook)."""
+        ms = ctypes.c_double(0)
+        self._check(self._L.spg_implicit_bench_spmv(self.h, alpha, beta, repeats, ctypes.byref(ms)))
+        return ms.value
+
+    def spmv_bytes(self):
+        """Compulsory bytes of one product: 72 B values + 4 B column per block, 24 B of x and 24 B of y and
+        8 B of row pointer per block row."""
+        n = self.num_dofs() // 3
+        return 76.0 * self.num_blocks() + 48.0 * n + 8.0 * (n + 1)
 
     def stats(self):
         st = ImplicitStats()

@@ -178,3 +178,35 @@ def test_invalid_inputs_rejected():
     assert e.value.code == native.ERR_NONFINITE
     with pytest.raises(native.NativeError):
         ctx.step(0.0, 1, s.gravity)
+
+
+@pytest.mark.parametrize("solver", ["baraff_witkin", "bdf2"])
+def test_mirror_matches_capi(solver):
+    """spg::solver::ImplicitEulerGpu (C++ mirror of the BaseSolver API) gives the C-ABI's result bit for bit."""
+    s = _cloth()
+    pg = s.particle_group()
+    m = native.ImplicitEulerGpu(solver)
+    m.add_object(pg)
+    m.set_dt(s.dt)
+    m.set_num_substeps(2)
+    m.set_gravity(s.gravity)
+    ctx = native.ImplicitContext([s.vbd_object()], solver)
+    for _ in range(3):
+        m.step()
+        ctx.step(s.dt, 2, s.gravity)
+    xm, vm = m.state()
+    xc, vc = ctx.state()
+    assert np.array_equal(xm, xc) and np.array_equal(vm, vc)
+
+
+def test_spmv_hook_and_timing_stats():
+    s = _cloth()
+    ctx = native.ImplicitContext([s.vbd_object()], "baraff_witkin")
+    ms = ctx.bench_spmv(1.0, -s.dt ** 2, repeats=5)
+    assert ms > 0
+    ctx.set_timing(True)
+    ctx.step(s.dt, 1, s.gravity)
+    st = ctx.stats()
+    assert st["cg_ms"] > 0 and st["assembly_ms"] > 0
+    assert st["spmv_launches"] == st["cg_iterations"] + (1 if st["last_cg_error"] < 1e-5 else 0)
+    assert st["spmv_bytes"] == st["spmv_launches"] * ctx.spmv_bytes()

@@ -57,8 +57,13 @@ typedef struct {
     double assembly_ms;         /* element derivatives + force + matrix assembly */
     double cg_ms;               /* conjugate-gradient solves */
     int64_t spmv_launches;      /* SpMV launches inside timed solves */
-    double spmv_bytes;          /* compulsory bytes of those SpMVs: 72 B per block + 4 B column + 24 B of x
-                                   per block row and 24 B of y per block row */
+    double spmv_bytes;          /* compulsory bytes of those SpMVs (the system matrices are stored by their
+                                   upper 3x3 blocks): 72 B + 4 B column per upper block, 4 B index + 4 B
+                                   column per lower block, 24 B of x and 24 B of y per block row, 16 B of
+                                   row pointers per block row */
+    double spmv_sampled_ms;     /* HIP-event time of the sampled SpMVs inside timed solves (the first of
+                                   every chunk of 8 iterations) */
+    int64_t spmv_sampled;       /* how many were sampled */
 } spg_implicit_stats;
 
 typedef struct spg_implicit_ctx spg_implicit_ctx;

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -66,15 +67,15 @@ struct spg_implicit_ctx {
     };
     Table tables[6];
     DevBuf<int64_t> incPtr, incOff;
-    // block-sparse pattern
+    // block-sparse pattern: the full one (exports) and the upper storage of the symmetric matrices
     int64_t nBlocks = 0, nUpper = 0;
-    DevBuf<int64_t> rowPtr, diagBlock, upperBlock, contribPtr, contrib, mirror;
-    DevBuf<int32_t> col;
+    DevBuf<int64_t> rowPtr, diagBlock, contribPtr, contrib, upPtr, loPtr;
+    DevBuf<int32_t> col, upRow, upCol, loIdx, loCol;
     std::vector<int64_t> hRowPtr;
     std::vector<int32_t> hCol;
-    DevBuf<double> H, A, invdiag;
+    DevBuf<double> H, A, invdiag;  // upper blocks only
     // conjugate gradient
-    DevBuf<double> cgr, cgp, cgtmp, partial;
+    DevBuf<double> cgr, cgp, cgtmp, partial, partial2;
     DevBuf<CgScalars> cgs;
     CgScalars *hostCg = nullptr;  // pinned
     double *hostNorm = nullptr;   // pinned
@@ -88,6 +89,8 @@ struct spg_implicit_ctx {
     spg_implicit_stats stats{};
     bool timing = false;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evCopy = nullptr;
+    // one (start, end) pair per iteration chunk of a timed solve: the chunk's first product is sampled
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> spmvEvents;
 
     ~spg_implicit_ctx()
     {
@@ -96,10 +99,14 @@ struct spg_implicit_ctx {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (evCopy) (void)hipEventDestroy(evCopy);
+        for (auto &e : spmvEvents) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 
-    Bsr bsr(double *vals) const { return Bsr{np, rowPtr.p, col.p, vals}; }
+    SymBsr sym(double *vals) const { return SymBsr{np, upPtr.p, upCol.p, loPtr.p, loIdx.p, loCol.p, vals}; }
 
     void timedBegin()
     {
@@ -128,26 +135,42 @@ struct spg_implicit_ctx {
         hipCheck(launchGatherForce(np, mass.p, g[0], g[1], g[2], incPtr.p, incOff.p, grad.p, f.p, stream),
                  "force launch");
         if (hessians)
-            hipCheck(launchAssembleHessian(nUpper, upperBlock.p, contribPtr.p, contrib.p, hess.p, mirror.p, H.p, stream),
+            hipCheck(launchAssembleHessian(nUpper, upRow.p, upCol.p, contribPtr.p, contrib.p, hess.p, H.p, stream),
                      "assembly launch");
     }
 
     // A = a M + c H, its inverse diagonal; CG on A dv = b (zero initial guess) into out
     void solve(double a, double c, const double *b, double *out, double tol, int32_t maxIters)
     {
-        hipCheck(launchCombine(nBlocks, H.p, c, np, diagBlock.p, mass.p, a, A.p, stream), "combine launch");
-        hipCheck(launchInvDiag(np, diagBlock.p, A.p, invdiag.p, stream), "invdiag launch");
+        hipCheck(launchCombine(nUpper, H.p, c, np, upPtr.p, mass.p, a, A.p, stream), "combine launch");
+        hipCheck(launchInvDiag(np, upPtr.p, A.p, invdiag.p, stream), "invdiag launch");
         timedBegin();
         hipCheck(launchCgInit(ndof, b, invdiag.p, out, cgr.p, cgp.p, partial.p, cgs.p, tol, stream), "cg launch");
         // Iterations go in chunks; the scalars are read back after each chunk while the next one is
         // already queued, and every kernel of an iteration returns at once after convergence.
         constexpr int kChunk = 8;
         int launched = 0;
+        std::vector<int> sampled;  // iteration whose product each event pair timed
         auto chunk = [&] {
-            for (int k = 0; k < kChunk && launched < maxIters; ++k, ++launched)
-                hipCheck(launchCgIteration(bsr(A.p), ndof, invdiag.p, out, cgr.p, cgp.p, cgtmp.p, partial.p, cgs.p,
-                                           maxIters, stream),
+            for (int k = 0; k < kChunk && launched < maxIters; ++k, ++launched) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (timing && k == 0) {
+                    if (sampled.size() == spmvEvents.size()) {
+                        std::pair<hipEvent_t, hipEvent_t> e{nullptr, nullptr};
+                        hipCheck(hipEventCreate(&e.first), "hipEventCreate");
+                        const hipError_t err = hipEventCreate(&e.second);
+                        if (err != hipSuccess) (void)hipEventDestroy(e.first);
+                        hipCheck(err, "hipEventCreate");
+                        spmvEvents.push_back(e);
+                    }
+                    e0 = spmvEvents[sampled.size()].first;
+                    e1 = spmvEvents[sampled.size()].second;
+                    sampled.push_back(launched);
+                }
+                hipCheck(launchCgIteration(sym(A.p), ndof, invdiag.p, out, cgr.p, cgp.p, cgtmp.p, partial.p,
+                                           partial2.p, cgs.p, maxIters, launched, stream, e0, e1),
                          "cg launch");
+            }
         };
         auto readBack = [&] {
             hipCheck(hipMemcpyAsync(hostCg, cgs.p, sizeof(CgScalars), hipMemcpyDeviceToHost, stream), "download");
@@ -173,10 +196,23 @@ struct spg_implicit_ctx {
             const int64_t spmvs = hostCg->rhsNorm2 == 0 ? 0 : hostCg->iters + (hostCg->done == 1 ? 1 : 0);
             stats.spmv_launches += spmvs;
             stats.spmv_bytes += spmvs * spmvBytes();
+            // sampled products that ran (the later ones return at once after convergence)
+            for (size_t k = 0; k < sampled.size(); ++k) {
+                if (sampled[k] >= spmvs) break;
+                float t = 0.f;
+                hipCheck(hipEventElapsedTime(&t, spmvEvents[k].first, spmvEvents[k].second), "hipEventElapsedTime");
+                stats.spmv_sampled_ms += t;
+                stats.spmv_sampled += 1;
+            }
         }
     }
 
-    double spmvBytes() const { return 76.0 * nBlocks + 48.0 * np + 8.0 * (np + 1); }
+    // compulsory bytes of one product: 72 B + 4 B column per upper block, 4 B index + 4 B column per
+    // lower block, x (24 B) and y (24 B) per block row and the two row-pointer arrays
+    double spmvBytes() const
+    {
+        return 76.0 * nUpper + 8.0 * (nUpper - np) + 48.0 * np + 16.0 * (np + 1);
+    }
 
     double norm(const double *vec)
     {
@@ -211,7 +247,7 @@ struct spg_implicit_ctx {
             copy(v0.p, v.p);
             assemble(x.p, true);
             // rhs = dt (f + dt K v0) with K = -H
-            hipCheck(launchBsrSpmv(bsr(H.p), v0.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
+            hipCheck(launchSymSpmv(sym(H.p), v0.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
             lin(rhs.p, 1.0, f.p, dt, tmp.p);
             lin(rhs.p, dt, rhs.p);
             solve(1.0, dt * dt, rhs.p, dv.p, tol, maxIt);
@@ -251,7 +287,7 @@ struct spg_implicit_ctx {
             assemble(x.p, true);
             // w = x0 - x_{-1} + 2 dt v0; tmp = K w = -H w
             lin(rhs.p, 1.0, x0.p, -1.0, xm1.p, 2 * dt, v0.p);
-            hipCheck(launchBsrSpmv(bsr(H.p), rhs.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
+            hipCheck(launchSymSpmv(sym(H.p), rhs.p, -1.0, tmp.p, nullptr, stream), "spmv launch");
             // tmp = 2/3 dt (f + 1/3 K w); rhs = 1/3 M (v0 - v_{-1}) + tmp
             lin(tmp.p, 2.0 / 3.0 * dt, f.p, 2.0 / 3.0 * dt / 3.0, tmp.p);
             lin(dv.p, 1.0, v0.p, -1.0, vm1.p);
@@ -486,17 +522,36 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
         const auto b = c.hCol.begin() + c.hRowPtr[i], e = c.hCol.begin() + c.hRowPtr[i + 1];
         return static_cast<int64_t>(std::lower_bound(b, e, j) - c.hCol.begin());
     };
-    // upper blocks (i <= j) in block order, their mirrors
-    std::vector<int64_t> upperOfBlock(c.nBlocks, -1), upper, mirror;
-    for (int64_t i = 0; i < c.np; ++i)
+    // upper blocks (i <= j) in block order; per row the upper blocks (j, i), j < i, read transposed
+    std::vector<int64_t> upperOfBlock(c.nBlocks, -1), upPtr(c.np + 1, 0), loPtr(c.np + 1, 0);
+    std::vector<int32_t> upRow, upCol;
+    for (int64_t i = 0; i < c.np; ++i) {
         for (int64_t k = c.hRowPtr[i]; k < c.hRowPtr[i + 1]; ++k) {
             const int32_t j = c.hCol[k];
-            if (j < i) continue;
-            upperOfBlock[k] = static_cast<int64_t>(upper.size());
-            upper.push_back(k);
-            mirror.push_back(j == i ? -1 : blockOf(j, static_cast<int32_t>(i)));
+            if (j < i) {
+                ++loPtr[i + 1];
+                continue;
+            }
+            upperOfBlock[k] = static_cast<int64_t>(upRow.size());
+            upRow.push_back(static_cast<int32_t>(i));
+            upCol.push_back(j);
         }
-    c.nUpper = static_cast<int64_t>(upper.size());
+        upPtr[i + 1] = static_cast<int64_t>(upRow.size());
+    }
+    c.nUpper = static_cast<int64_t>(upRow.size());
+    if (c.nUpper > std::numeric_limits<int32_t>::max())
+        throw VbdError(SPG_VBD_ERR_INVALID, "system matrix has more than 2^31 - 1 stored blocks");
+    for (int64_t i = 0; i < c.np; ++i) loPtr[i + 1] += loPtr[i];
+    std::vector<int32_t> loIdx(loPtr[c.np]), loCol(loPtr[c.np]);
+    {
+        std::vector<int64_t> fillLo(loPtr.begin(), loPtr.end() - 1);
+        for (int64_t u = 0; u < c.nUpper; ++u)  // rows ascending, so every lower list is in ascending j
+            if (upCol[u] != upRow[u]) {
+                const int64_t at = fillLo[upCol[u]]++;
+                loIdx[at] = static_cast<int32_t>(u);
+                loCol[at] = upRow[u];
+            }
+    }
     // contributions: element Hessian sub-block (a, b) with v_a <= v_b to upper block (v_a, v_b), kept in
     // (object, energy, stencil, a, b) order within each block
     std::vector<int64_t> cnt(c.nUpper + 1, 0);
@@ -535,16 +590,21 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
     c.rowPtr.upload(c.hRowPtr, c.stream);
     c.col.upload(c.hCol, c.stream);
     c.diagBlock.upload(diag, c.stream);
-    c.upperBlock.upload(upper, c.stream);
-    c.mirror.upload(mirror, c.stream);
+    c.upPtr.upload(upPtr, c.stream);
+    c.loPtr.upload(loPtr, c.stream);
+    c.upRow.upload(upRow, c.stream);
+    c.upCol.upload(upCol, c.stream);
+    c.loIdx.upload(loIdx, c.stream);
+    c.loCol.upload(loCol, c.stream);
     c.contribPtr.upload(cnt, c.stream);
     c.contrib.upload(contrib.empty() ? std::vector<int64_t>{0} : contrib, c.stream);
-    c.H.alloc(9 * c.nBlocks);
-    c.A.alloc(9 * c.nBlocks);
+    c.H.alloc(9 * c.nUpper);
+    c.A.alloc(9 * c.nUpper);
     c.invdiag.alloc(c.ndof);
     for (auto *b : {&c.cgr, &c.cgp, &c.cgtmp, &c.x0, &c.v0, &c.vi, &c.dv, &c.rhs, &c.tmp, &c.xm1, &c.vm1})
         b->alloc(c.ndof);
-    c.partial.alloc(std::max<int64_t>(2 * kCgBlocks, spmvGrid(c.np)));
+    c.partial.alloc(std::max<int64_t>(2 * kCgBlocks, spmvPartials(c.np)));
+    c.partial2.alloc(2 * kCgBlocks);
     c.cgs.alloc(1);
     c.normOut.alloc(1);
     hipCheck(hipHostMalloc(reinterpret_cast<void **>(&c.hostCg), sizeof(CgScalars)), "hipHostMalloc");
@@ -692,11 +752,13 @@ int spg_implicit_stiffness(spg_implicit_ctx *ctx, int64_t *row_ptr, int32_t *col
         if (!blocks) return;
         const double g[3] = {0, 0, 0};
         ctx->derivatives(ctx->x.p, true, g);
-        // K = -H
-        hipCheck(launchCombine(ctx->nBlocks, ctx->H.p, -1.0, 0, nullptr, nullptr, 0.0, ctx->A.p, ctx->stream),
-                 "combine launch");
-        hipCheck(hipMemcpyAsync(blocks, ctx->A.p, sizeof(double) * 9 * ctx->nBlocks, hipMemcpyDeviceToHost,
-                                ctx->stream),
+        // K = -H, expanded to the full block rows
+        DevBuf<double> full;
+        full.alloc(9 * ctx->nBlocks);
+        hipCheck(launchExpandSym(ctx->sym(ctx->H.p), -1.0, ctx->rowPtr.p, ctx->col.p, ctx->diagBlock.p, full.p,
+                                 ctx->stream),
+                 "expand launch");
+        hipCheck(hipMemcpyAsync(blocks, full.p, sizeof(double) * 9 * ctx->nBlocks, hipMemcpyDeviceToHost, ctx->stream),
                  "download");
         hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
     });
@@ -730,13 +792,13 @@ int spg_implicit_bench_spmv(spg_implicit_ctx *ctx, double alpha, double beta, in
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         const double g[3] = {0, 0, 0};
         ctx->derivatives(ctx->x.p, true, g);
-        hipCheck(launchCombine(ctx->nBlocks, ctx->H.p, -beta, ctx->np, ctx->diagBlock.p, ctx->mass.p, alpha, ctx->A.p,
+        hipCheck(launchCombine(ctx->nUpper, ctx->H.p, -beta, ctx->np, ctx->upPtr.p, ctx->mass.p, alpha, ctx->A.p,
                                ctx->stream),
                  "combine launch");
-        hipCheck(launchBsrSpmv(ctx->bsr(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream), "spmv launch");
+        hipCheck(launchSymSpmv(ctx->sym(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream), "spmv launch");
         hipCheck(hipEventRecord(ctx->ev0, ctx->stream), "hipEventRecord");
         for (int32_t r = 0; r < repeats; ++r)
-            hipCheck(launchBsrSpmv(ctx->bsr(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream),
+            hipCheck(launchSymSpmv(ctx->sym(ctx->A.p), ctx->v.p, 1.0, ctx->tmp.p, nullptr, ctx->stream),
                      "spmv launch");
         hipCheck(hipEventRecord(ctx->ev1, ctx->stream), "hipEventRecord");
         hipCheck(hipEventSynchronize(ctx->ev1), "hipEventSynchronize");

@@ -8,11 +8,15 @@
 //     parameters of spg_vbd_energy.
 //   * Element derivatives: per element its gradient (3N doubles) and Hessian (3N x 3N, column-major:
 //     entry (r, c) at c * 3N + r), element-major.
-//   * Global matrices in 3x3 block-sparse rows (BSR): row i holds the blocks of the particles sharing a
-//     stencil with i (and i itself), columns ascending; a block is 9 doubles, row-major. The Hessian H is
-//     assembled per upper block (i <= j) from a contribution list (element Hessian sub-blocks, in energy
-//     order and ascending stencil) and mirrored, so H is exactly symmetric. K = -H is the reference's
-//     stiffness matrix (implicitEulerBase.cpp:180-222); system matrices are a M + c H.
+//   * Global matrices are symmetric 3x3 block-sparse matrices stored by their upper blocks only: row i
+//     holds the blocks (i, j), j >= i, of the particles sharing a stencil with i, columns ascending, the
+//     diagonal block first; a block is 9 doubles, row-major. A lower block (i, j), j < i, is the
+//     transpose of the upper block (j, i): each row also lists those (upper index and j, ascending j), so
+//     a product reads every stored block twice, the second time from L2 (the rows that hold it were
+//     streamed just before by the same XCD). The Hessian H is assembled per upper block from a
+//     contribution list (element Hessian sub-blocks, in energy order and ascending stencil), the diagonal
+//     blocks symmetrised, so H is exactly symmetric. K = -H is the reference's stiffness matrix
+//     (implicitEulerBase.cpp:180-222); system matrices are a M + c H.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,11 +34,14 @@ struct ImplicitElems {
     double *hess;          // n * (3 size)^2, column-major per element
 };
 
-struct Bsr {
+struct SymBsr {
     int64_t rows;          // particles
-    const int64_t *rowPtr; // rows + 1
-    const int32_t *col;    // block columns
-    double *val;           // 9 per block
+    const int64_t *upPtr;  // rows + 1: upper blocks of row i are [upPtr[i], upPtr[i + 1]), the first is (i, i)
+    const int32_t *upCol;  // column of every upper block
+    const int64_t *loPtr;  // rows + 1: lower blocks of row i are [loPtr[i], loPtr[i + 1])
+    const int32_t *loIdx;  // upper block (j, i) holding lower block (i, j), j ascending
+    const int32_t *loCol;  // that j
+    double *val;           // 9 per upper block
 };
 
 // gradients and Hessians of all elements of one type at x
@@ -46,33 +53,42 @@ hipError_t launchElementGradients(const ImplicitElems &e, const double *x, hipSt
 hipError_t launchGatherForce(int64_t nParticles, const double *mass, double gx, double gy, double gz,
                              const int64_t *incPtr, const int64_t *incOff, const double *gradBase, double *f,
                              hipStream_t stream);
-// H upper blocks (i <= j) from contribution lists (offsets in doubles of element Hessian sub-block (a, b)
-// origins, and the stride 3N of that element's columns packed as (offset << 4 | N3)), mirrored into the
-// lower blocks through `mirror` (index of block (j, i) for every upper block, -1 on the diagonal)
-hipError_t launchAssembleHessian(int64_t nUpper, const int64_t *upperBlock, const int64_t *contribPtr,
-                                 const int64_t *contrib, const double *hessBase, const int64_t *mirror, double *H,
-                                 hipStream_t stream);
-// A = c H + a M (M = diag(mass) on the diagonal blocks, diagBlock[i] = block index of (i, i))
+// H's upper blocks from contribution lists (offsets in doubles of element Hessian sub-block (a, b)
+// origins, and the stride 3N of that element's columns packed as (offset << 4 | N3)); diagonal blocks
+// (upRow[u] == upCol[u]) are symmetrised
+hipError_t launchAssembleHessian(int64_t nUpper, const int32_t *upRow, const int32_t *upCol, const int64_t *contribPtr,
+                                 const int64_t *contrib, const double *hessBase, double *H, hipStream_t stream);
+// A = c H + a M over the stored blocks (M = diag(mass) on the diagonal blocks, diagBlock[i] = block index
+// of (i, i))
 hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nParticles, const int64_t *diagBlock,
                          const double *mass, double a, double *A, hipStream_t stream);
-// y = s * (A x), BSR 3x3 FP64; partial (may be null): per-workgroup partial sums of x . y, spmvGrid(rows)
-// of them
-hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream);
-int spmvGrid(int64_t rows);
+// y = s * (A x) for the symmetric block matrix; partial (may be null): per-workgroup partial sums of
+// x . y, spmvPartials(rows) of them
+hipError_t launchSymSpmv(const SymBsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream);
+int spmvPartials(int64_t rows);
+// the full block rows of s * A (rowPtr/col: the full pattern, diagBlock: index of (i, i) in it) into out
+hipError_t launchExpandSym(const SymBsr &A, double s, const int64_t *rowPtr, const int32_t *col,
+                           const int64_t *diagBlock, double *out, hipStream_t stream);
 
 // Conjugate gradient (Eigen's algorithm with the diagonal preconditioner). Scalars live on the device:
 struct CgScalars {
-    double rhsNorm2, threshold, absNew, alpha, residualNorm2, beta;
+    double rhsNorm2, threshold, residualNorm2;
+    double absNew[2];  // r.D^-1 r, by iteration parity
     int32_t done;   // 1 once converged (or the right-hand side is 0)
     int32_t iters;  // completed iterations (Eigen's convention)
 };
-constexpr int kCgBlocks = 1024;  // blocks of the vector partial-sum kernels (fixed: deterministic reductions)
-// the partial-sum buffer of a CG needs max(2 kCgBlocks, spmvGrid(rows)) doubles
+constexpr int kCgBlocks = 512;  // workgroups of the vector kernels (fixed: deterministic reductions)
+// a CG needs two partial-sum buffers: pPartial of max(2 kCgBlocks, spmvPartials(rows)) doubles (also used
+// by launchCgInit) and uPartial of 2 kCgBlocks
 hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, double *x, double *r, double *p,
                         double *partial, CgScalars *s, double tol, hipStream_t stream);
-// one iteration: tmp = A p; alpha; x += alpha p, r -= alpha tmp; convergence test; beta; p = D^-1 r + beta p
-hipError_t launchCgIteration(const Bsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
-                             double *tmp, double *partial, CgScalars *s, int32_t maxIters, hipStream_t stream);
+// iteration `iteration` (0-based, launch order): tmp = A p; alpha; x += alpha p, r -= alpha tmp;
+// convergence test; beta; p = D^-1 r + beta p. Three kernels. Non-null events are recorded right before and
+// after the product (to time it inside a solve).
+hipError_t launchCgIteration(const SymBsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
+                             double *tmp, double *pPartial, double *uPartial, CgScalars *s, int32_t maxIters,
+                             int32_t iteration, hipStream_t stream, hipEvent_t spmvStart = nullptr,
+                             hipEvent_t spmvEnd = nullptr);
 // invdiag = 1 / diagonal of A (1 where it is 0), from the diagonal blocks
 hipError_t launchInvDiag(int64_t nParticles, const int64_t *diagBlock, const double *A, double *invdiag,
                          hipStream_t stream);

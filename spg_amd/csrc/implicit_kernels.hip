@@ -109,11 +109,11 @@ __global__ void __launch_bounds__(256) gatherForceKernel(int64_t n, const double
     f[3 * i + 2] = fz;
 }
 
-__global__ void __launch_bounds__(256) assembleHessianKernel(int64_t nUpper, const int64_t *__restrict__ upperBlock,
+__global__ void __launch_bounds__(256) assembleHessianKernel(int64_t nUpper, const int32_t *__restrict__ upRow,
+                                                             const int32_t *__restrict__ upCol,
                                                              const int64_t *__restrict__ contribPtr,
                                                              const int64_t *__restrict__ contrib,
-                                                             const double *__restrict__ hess,
-                                                             const int64_t *__restrict__ mirror, double *__restrict__ H)
+                                                             const double *__restrict__ hess, double *__restrict__ H)
 {
     const int64_t u = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (u >= nUpper) return;
@@ -125,20 +125,13 @@ __global__ void __launch_bounds__(256) assembleHessianKernel(int64_t nUpper, con
         for (int cc = 0; cc < 3; ++cc)
             for (int r = 0; r < 3; ++r) b[r * 3 + cc] += h[cc * N3 + r];
     }
-    const int64_t blk = upperBlock[u];
-    const int64_t mir = mirror[u];
-    double *out = H + 9 * blk;
-    if (mir < 0) {  // diagonal block: symmetric part
+    double *out = H + 9 * u;
+    if (upRow[u] == upCol[u]) {  // diagonal block: symmetric part
         for (int r = 0; r < 3; ++r)
             for (int cc = 0; cc < 3; ++cc) out[r * 3 + cc] = 0.5 * (b[r * 3 + cc] + b[cc * 3 + r]);
         return;
     }
-    double *low = H + 9 * mir;
-    for (int r = 0; r < 3; ++r)
-        for (int cc = 0; cc < 3; ++cc) {
-            out[r * 3 + cc] = b[r * 3 + cc];
-            low[cc * 3 + r] = b[r * 3 + cc];
-        }
+    for (int k = 0; k < 9; ++k) out[k] = b[k];
 }
 
 __global__ void __launch_bounds__(256) combineKernel(int64_t nBlocks, const double *__restrict__ H, double c,
@@ -175,31 +168,49 @@ __device__ __forceinline__ double blockSumFixed(double v, double *sh)
     return s;
 }
 
-// Block-row SpMV: kSpmvG lanes per block row, lane l taking the row's blocks l, l + G, ... whole (9
-// values, its column and the 3 x entries), then a fixed butterfly over the G lanes. One row group per
-// kSpmvG threads over the whole grid, so every block row is in flight at once.
+// Symmetric block-row SpMV: kSpmvG lanes per block row; lane l takes the row's upper blocks l, l + G, ...
+// whole (9 values, the column, the 3 x entries), then its lower blocks l, l + G, ... as transposes of the
+// upper blocks that hold them, then a fixed butterfly over the G lanes. Workgroups are renumbered so
+// each XCD streams a contiguous band of rows: the upper blocks a row reads transposed belong to rows a
+// band streamed just before, and are still in that XCD's L2.
 constexpr int kSpmvG = 4;
-constexpr int kSpmvBlock = 256;
+constexpr int kSpmvBlock = 1024;
 
-__global__ void __launch_bounds__(kSpmvBlock) bsrSpmvKernel(Bsr A, const double *__restrict__ x, double s,
-                                                            double *__restrict__ y, double *__restrict__ partial,
-                                                            const int32_t *__restrict__ done)
+__device__ __forceinline__ int32_t xcdContiguous(int32_t b, int32_t nb)
+{
+    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
+
+__global__ void __launch_bounds__(kSpmvBlock) symSpmvKernel(SymBsr A, const double *__restrict__ x, double s,
+                                                           double *__restrict__ y, double *__restrict__ partial,
+                                                           const int32_t *__restrict__ done)
 {
     __shared__ double sh[kSpmvBlock / 64];
-    if (done && *done) return;
-    const int64_t t = blockIdx.x * static_cast<int64_t>(kSpmvBlock) + threadIdx.x;
+    if (done && *done) return;  // no other kernel runs meanwhile: the same value in every thread
+    const int32_t blk = xcdContiguous(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x));
+    const int64_t t = blk * static_cast<int64_t>(kSpmvBlock) + threadIdx.x;
     const int64_t row = t / kSpmvG;
     const int lig = static_cast<int>(t % kSpmvG);
     double y0 = 0.0, y1 = 0.0, y2 = 0.0;
     if (row < A.rows) {
-        const int64_t k1 = A.rowPtr[row + 1];
-        for (int64_t k = A.rowPtr[row] + lig; k < k1; k += kSpmvG) {
+        const int64_t u1 = A.upPtr[row + 1];
+        for (int64_t k = A.upPtr[row] + lig; k < u1; k += kSpmvG) {
             const double *v = A.val + 9 * k;
-            const int64_t c = 3 * static_cast<int64_t>(A.col[k]);
+            const int64_t c = 3 * static_cast<int64_t>(A.upCol[k]);
             const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
             y0 += v[0] * x0 + v[1] * x1 + v[2] * x2;
             y1 += v[3] * x0 + v[4] * x1 + v[5] * x2;
             y2 += v[6] * x0 + v[7] * x1 + v[8] * x2;
+        }
+        const int64_t l1 = A.loPtr[row + 1];
+        for (int64_t k = A.loPtr[row] + lig; k < l1; k += kSpmvG) {
+            const double *v = A.val + 9 * static_cast<int64_t>(A.loIdx[k]);
+            const int64_t c = 3 * static_cast<int64_t>(A.loCol[k]);
+            const double x0 = x[c], x1 = x[c + 1], x2 = x[c + 2];
+            y0 += v[0] * x0 + v[3] * x1 + v[6] * x2;
+            y1 += v[1] * x0 + v[4] * x1 + v[7] * x2;
+            y2 += v[2] * x0 + v[5] * x1 + v[8] * x2;
         }
     }
 #pragma unroll
@@ -216,9 +227,37 @@ __global__ void __launch_bounds__(kSpmvBlock) bsrSpmvKernel(Bsr A, const double 
         y[3 * row + 2] = z2;
         dot = x[3 * row] * z0 + x[3 * row + 1] * z1 + x[3 * row + 2] * z2;
     }
-    if (partial) {
+    if (partial) {  // per-workgroup partial sums, in row order
         const double b = blockSumFixed(dot, sh);
-        if (threadIdx.x == 0) partial[blockIdx.x] = b;
+        if (threadIdx.x == 0) partial[blk] = b;
+    }
+}
+
+// Full block rows of s * A from the upper storage: one thread per block row.
+__global__ void __launch_bounds__(256) expandSymKernel(SymBsr A, double s, const int64_t *__restrict__ rowPtr,
+                                                       const int32_t *__restrict__ col,
+                                                       const int64_t *__restrict__ diagBlock, double *__restrict__ out)
+{
+    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    if (i >= A.rows) return;
+    for (int64_t k = rowPtr[i]; k < rowPtr[i + 1]; ++k) {
+        const int32_t j = col[k];
+        double *o = out + 9 * k;
+        if (j >= i) {
+            const double *v = A.val + 9 * (A.upPtr[i] + (k - diagBlock[i]));
+            for (int e = 0; e < 9; ++e) o[e] = s * v[e];
+            continue;
+        }
+        // upper block (j, i): binary search of column i in row j's upper blocks
+        int64_t lo = A.upPtr[j], hi = A.upPtr[j + 1];
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) / 2;
+            if (A.upCol[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const double *v = A.val + 9 * lo;
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) o[r * 3 + c] = s * v[c * 3 + r];
     }
 }
 
@@ -271,10 +310,9 @@ __global__ void __launch_bounds__(1024) cgInitFinalKernel(const double *__restri
     const double rp = finalSum(partial + nb, nb, sh);
     if (threadIdx.x == 0) {
         s->rhsNorm2 = bb;
-        s->absNew = rp;
+        s->absNew[0] = rp;
+        s->absNew[1] = 0.0;
         s->iters = 0;
-        s->alpha = 0.0;
-        s->beta = 0.0;
         s->residualNorm2 = bb;  // r = b - A 0
         const double th = tol * tol * bb;
         s->threshold = th > DBL_MIN ? th : DBL_MIN;
@@ -282,22 +320,39 @@ __global__ void __launch_bounds__(1024) cgInitFinalKernel(const double *__restri
     }
 }
 
-__global__ void __launch_bounds__(1024) cgAlphaKernel(const double *__restrict__ partial, int nb, CgScalars *s)
+// The sum of partial[0, n) in a fixed order, the same value in every thread (and every workgroup).
+__device__ __forceinline__ double blockTotal(const double *__restrict__ partial, int n, double *sh)
 {
-    __shared__ double sh[16];
-    if (s->done) return;
-    const double pAp = finalSum(partial, nb, sh);
-    if (threadIdx.x == 0) s->alpha = s->absNew / pAp;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    const double t = blockSumFixed(acc, sh);
+    if (threadIdx.x == 0) sh[0] = t;
+    __syncthreads();
+    const double v = sh[0];
+    __syncthreads();
+    return v;
 }
 
+// The flag read once per workgroup, so every thread of the workgroup takes the same branch.
+__device__ __forceinline__ bool blockDone(const CgScalars *s, int32_t *flag)
+{
+    if (threadIdx.x == 0) *flag = s->done;
+    __syncthreads();
+    return *flag != 0;
+}
+
+// alpha = absNew / p.Ap (every workgroup reduces the product's partial sums itself, in the same order);
+// x += alpha p, r -= alpha A p; per-workgroup partial sums of r.r and r.D^-1 r.
 __global__ void __launch_bounds__(256) cgUpdateKernel(int64_t n, const double *__restrict__ invdiag,
                                                       double *__restrict__ x, double *__restrict__ r,
                                                       const double *__restrict__ p, const double *__restrict__ tmp,
-                                                      double *__restrict__ partial, const CgScalars *s)
+                                                      const double *__restrict__ pPartial, int nbP,
+                                                      double *__restrict__ partial, const CgScalars *s, int parity)
 {
     __shared__ double sh[4];
-    if (s->done) return;
-    const double alpha = s->alpha;
+    __shared__ int32_t flag;
+    if (blockDone(s, &flag)) return;
+    const double alpha = s->absNew[parity] / blockTotal(pPartial, nbP, sh);
     const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
     const int64_t b0 = blockIdx.x * chunk, e0 = b0 + chunk < n ? b0 + chunk : n;
     double rr = 0.0, rz = 0.0;
@@ -316,36 +371,38 @@ __global__ void __launch_bounds__(256) cgUpdateKernel(int64_t n, const double *_
     }
 }
 
-__global__ void __launch_bounds__(1024) cgTestKernel(const double *__restrict__ partial, int nb, CgScalars *s,
-                                                     int32_t maxIters)
+// Convergence test and beta (every workgroup reduces the update's partial sums itself; workgroup 0
+// stores the scalars), then p = D^-1 r + beta p. absNew is double-buffered by iteration parity because
+// the other workgroups still read the old value while workgroup 0 stores the new one. A workgroup that
+// already sees the flag set by workgroup 0 skips p, which no later iteration reads.
+__global__ void __launch_bounds__(256) cgDirectionKernel(int64_t n, const double *__restrict__ invdiag,
+                                                         const double *__restrict__ r, double *__restrict__ p,
+                                                         const double *__restrict__ uPartial, int nbU, CgScalars *s,
+                                                         int32_t maxIters, int parity)
 {
-    __shared__ double sh[16];
-    if (s->done) return;
-    const double rr = finalSum(partial, nb, sh);
-    const double rz = finalSum(partial + nb, nb, sh);
-    if (threadIdx.x == 0) {
-        s->residualNorm2 = rr;
-        if (rr < s->threshold) {
+    __shared__ double sh[4];
+    __shared__ int32_t flag;
+    if (blockDone(s, &flag)) return;
+    const double rr = blockTotal(uPartial, nbU, sh);
+    const double rz = blockTotal(uPartial + nbU, nbU, sh);
+    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    if (rr < s->threshold) {
+        if (lead) {
+            s->residualNorm2 = rr;
             s->done = 1;
-            return;
         }
-        const double absOld = s->absNew;
-        s->absNew = rz;
-        s->beta = rz / absOld;
+        return;
+    }
+    const double beta = rz / s->absNew[parity];
+    if (lead) {
+        s->residualNorm2 = rr;
+        s->absNew[parity ^ 1] = rz;
         s->iters += 1;
         if (s->iters >= maxIters) s->done = 2;  // iteration budget spent (x is the last iterate)
     }
-}
-
-__global__ void __launch_bounds__(256) cgDirectionKernel(int64_t n, const double *__restrict__ invdiag,
-                                                         const double *__restrict__ r, double *__restrict__ p,
-                                                         const CgScalars *s)
-{
-    if (s->done) return;
-    const double beta = s->beta;
-    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-    if (i >= n) return;
-    p[i] = invdiag[i] * r[i] + beta * p[i];
+    const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * chunk, e0 = b0 + chunk < n ? b0 + chunk : n;
+    for (int64_t i = b0 + threadIdx.x; i < e0; i += blockDim.x) p[i] = invdiag[i] * r[i] + beta * p[i];
 }
 
 __global__ void __launch_bounds__(256) invDiagKernel(int64_t n, const int64_t *__restrict__ diagBlock,
@@ -442,13 +499,12 @@ hipError_t launchGatherForce(int64_t nParticles, const double *mass, double gx, 
     return hipGetLastError();
 }
 
-hipError_t launchAssembleHessian(int64_t nUpper, const int64_t *upperBlock, const int64_t *contribPtr,
-                                 const int64_t *contrib, const double *hessBase, const int64_t *mirror, double *H,
-                                 hipStream_t stream)
+hipError_t launchAssembleHessian(int64_t nUpper, const int32_t *upRow, const int32_t *upCol, const int64_t *contribPtr,
+                                 const int64_t *contrib, const double *hessBase, double *H, hipStream_t stream)
 {
     if (nUpper <= 0) return hipSuccess;
-    assembleHessianKernel<<<blocksFor(nUpper, 256), 256, 0, stream>>>(nUpper, upperBlock, contribPtr, contrib,
-                                                                       hessBase, mirror, H);
+    assembleHessianKernel<<<blocksFor(nUpper, 256), 256, 0, stream>>>(nUpper, upRow, upCol, contribPtr, contrib,
+                                                                       hessBase, H);
     return hipGetLastError();
 }
 
@@ -462,12 +518,24 @@ hipError_t launchCombine(int64_t nBlocks, const double *H, double c, int64_t nPa
     return hipGetLastError();
 }
 
+namespace {
 int spmvGrid(int64_t rows) { return blocksFor(rows * kSpmvG, kSpmvBlock); }
+}  // namespace
 
-hipError_t launchBsrSpmv(const Bsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream)
+int spmvPartials(int64_t rows) { return spmvGrid(rows); }
+
+hipError_t launchSymSpmv(const SymBsr &A, const double *x, double s, double *y, double *partial, hipStream_t stream)
 {
     if (A.rows <= 0) return hipSuccess;
-    bsrSpmvKernel<<<spmvGrid(A.rows), kSpmvBlock, 0, stream>>>(A, x, s, y, partial, nullptr);
+    symSpmvKernel<<<spmvGrid(A.rows), kSpmvBlock, 0, stream>>>(A, x, s, y, partial, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launchExpandSym(const SymBsr &A, double s, const int64_t *rowPtr, const int32_t *col,
+                           const int64_t *diagBlock, double *out, hipStream_t stream)
+{
+    if (A.rows <= 0) return hipSuccess;
+    expandSymKernel<<<blocksFor(A.rows, 256), 256, 0, stream>>>(A, s, rowPtr, col, diagBlock, out);
     return hipGetLastError();
 }
 
@@ -479,15 +547,23 @@ hipError_t launchCgInit(int64_t n, const double *b, const double *invdiag, doubl
     return hipGetLastError();
 }
 
-hipError_t launchCgIteration(const Bsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
-                             double *tmp, double *partial, CgScalars *s, int32_t maxIters, hipStream_t stream)
+hipError_t launchCgIteration(const SymBsr &A, int64_t n, const double *invdiag, double *x, double *r, double *p,
+                             double *tmp, double *pPartial, double *uPartial, CgScalars *s, int32_t maxIters,
+                             int32_t iteration, hipStream_t stream, hipEvent_t spmvStart, hipEvent_t spmvEnd)
 {
+    if (spmvStart) {
+        const hipError_t e = hipEventRecord(spmvStart, stream);
+        if (e != hipSuccess) return e;
+    }
     const int grid = spmvGrid(A.rows);
-    bsrSpmvKernel<<<grid, kSpmvBlock, 0, stream>>>(A, p, 1.0, tmp, partial, &s->done);
-    cgAlphaKernel<<<1, 1024, 0, stream>>>(partial, grid, s);
-    cgUpdateKernel<<<kCgBlocks, 256, 0, stream>>>(n, invdiag, x, r, p, tmp, partial, s);
-    cgTestKernel<<<1, 1024, 0, stream>>>(partial, kCgBlocks, s, maxIters);
-    cgDirectionKernel<<<blocksFor(n, 256), 256, 0, stream>>>(n, invdiag, r, p, s);
+    symSpmvKernel<<<grid, kSpmvBlock, 0, stream>>>(A, p, 1.0, tmp, pPartial, &s->done);
+    if (spmvEnd) {
+        const hipError_t e = hipEventRecord(spmvEnd, stream);
+        if (e != hipSuccess) return e;
+    }
+    const int parity = iteration & 1;
+    cgUpdateKernel<<<kCgBlocks, 256, 0, stream>>>(n, invdiag, x, r, p, tmp, pPartial, grid, uPartial, s, parity);
+    cgDirectionKernel<<<kCgBlocks, 256, 0, stream>>>(n, invdiag, r, p, uPartial, kCgBlocks, s, maxIters, parity);
     return hipGetLastError();
 }
 

@@ -92,7 +92,8 @@ class ImplicitStats(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int64), ("linear_solves", ctypes.c_int64), ("cg_iterations", ctypes.c_int64),
                 ("newton_iterations", ctypes.c_int64), ("failed_substeps", ctypes.c_int64),
                 ("current_dt", ctypes.c_double), ("last_cg_error", ctypes.c_double), ("assembly_ms", ctypes.c_double),
-                ("cg_ms", ctypes.c_double), ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double)]
+                ("cg_ms", ctypes.c_double), ("spmv_launches", ctypes.c_int64), ("spmv_bytes", ctypes.c_double),
+                ("spmv_sampled_ms", ctypes.c_double), ("spmv_sampled", ctypes.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -1046,10 +1047,12 @@ class ImplicitContext:
         return ms.value
 
     def spmv_bytes(self):
-        """Compulsory bytes of one product: 72 B values + 4 B column per block, 24 B of x and 24 B of y and
-        8 B of row pointer per block row."""
+        """Compulsory bytes of one product of the upper-block storage: 72 B values + 4 B column per upper
+        block, 4 B index + 4 B column per lower block (read transposed from the upper one), 24 B of x and
+        24 B of y and 16 B of row pointers per block row."""
         n = self.num_dofs() // 3
-        return 76.0 * self.num_blocks() + 48.0 * n + 8.0 * (n + 1)
+        n_upper = (self.num_blocks() + n) // 2
+        return 76.0 * n_upper + 8.0 * (n_upper - n) + 48.0 * n + 16.0 * (n + 1)
 
     def stats(self):
         st = ImplicitStats()

@@ -210,3 +210,5 @@ def test_spmv_hook_and_timing_stats():
     assert st["cg_ms"] > 0 and st["assembly_ms"] > 0
     assert st["spmv_launches"] == st["cg_iterations"] + (1 if st["last_cg_error"] < 1e-5 else 0)
     assert st["spmv_bytes"] == st["spmv_launches"] * ctx.spmv_bytes()
+    # the first product of every 8-iteration chunk that ran is timed
+    assert st["spmv_sampled"] == (st["spmv_launches"] + 7) // 8 and st["spmv_sampled_ms"] > 0

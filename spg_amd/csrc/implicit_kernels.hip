@@ -89,12 +89,20 @@ __global__ void __launch_bounds__(256) elementGradKernel(ImplicitElems e, const 
     }
 }
 
+// Workgroups are dealt round-robin over the 8 XCDs; this renumbering gives each XCD a contiguous band of
+// the logical range, so neighbouring particles (which share elements and blocks) share that XCD's L2.
+__device__ __forceinline__ int32_t xcdContiguous(int32_t b, int32_t nb)
+{
+    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
+    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+}
+
 __global__ void __launch_bounds__(256) gatherForceKernel(int64_t n, const double *__restrict__ mass, double gx,
                                                          double gy, double gz, const int64_t *__restrict__ incPtr,
                                                          const int64_t *__restrict__ incOff,
                                                          const double *__restrict__ grad, double *__restrict__ f)
 {
-    const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t i = xcdContiguous(blockIdx.x, gridDim.x) * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= n) return;
     const double m = mass[i];
     double fx = gx * m, fy = gy * m, fz = gz * m;
@@ -115,7 +123,7 @@ __global__ void __launch_bounds__(256) assembleHessianKernel(int64_t nUpper, con
                                                              const int64_t *__restrict__ contrib,
                                                              const double *__restrict__ hess, double *__restrict__ H)
 {
-    const int64_t u = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+    const int64_t u = xcdContiguous(blockIdx.x, gridDim.x) * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (u >= nUpper) return;
     double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int64_t k = contribPtr[u]; k < contribPtr[u + 1]; ++k) {
@@ -175,12 +183,6 @@ __device__ __forceinline__ double blockSumFixed(double v, double *sh)
 // band streamed just before, and are still in that XCD's L2.
 constexpr int kSpmvG = 4;
 constexpr int kSpmvBlock = 1024;
-
-__device__ __forceinline__ int32_t xcdContiguous(int32_t b, int32_t nb)
-{
-    const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
-    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
-}
 
 __global__ void __launch_bounds__(kSpmvBlock) symSpmvKernel(SymBsr A, const double *__restrict__ x, double s,
                                                            double *__restrict__ y, double *__restrict__ partial,

@@ -119,6 +119,16 @@ inline Dual<N, H> operator-(const Dual<N, H> &a, double c)
     return r;
 }
 template <int N, bool H>
+inline Dual<N, H> operator/(const Dual<N, H> &a, double c)
+{
+    Dual<N, H> r;
+    r.v = a.v / c;
+    for (int i = 0; i < N; ++i) r.g[i] = a.g[i] / c;
+    if (H)
+        for (int i = 0; i < N * N; ++i) r.h[i] = a.h[i] / c;
+    return r;
+}
+template <int N, bool H>
 inline Dual<N, H> operator/(const Dual<N, H> &a, const Dual<N, H> &b)
 {
     const double iv = 1.0 / b.v;
@@ -151,6 +161,12 @@ inline Dual<N, H> atan2(const Dual<N, H> &y, const Dual<N, H> &x)
 }
 inline double sqrt(double x) { return std::sqrt(x); }
 inline double atan2(double y, double x) { return std::atan2(y, x); }
+inline double valueOf(double s) { return s; }
+template <int N, bool H>
+inline double valueOf(const Dual<N, H> &s)
+{
+    return s.v;
+}
 
 template <class S>
 struct V3 {
@@ -170,6 +186,16 @@ template <class S>
 V3<S> cross(const V3<S> &a, const V3<S> &b)
 {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+template <class S>
+V3<S> add(const V3<S> &a, const V3<S> &b)
+{
+    return {a.x + b.x, a.y + b.y, a.z + b.z};
+}
+template <class S, class T>
+V3<S> scale(const V3<S> &a, const T &c)
+{
+    return {a.x * c, a.y * c, a.z * c};
 }
 template <class S>
 V3<S> normalized(const V3<S> &a)
@@ -263,6 +289,196 @@ S energySnh(const V3<S> *x, const double *p)
     return 0.5 * p[9] * hyd * hyd + 0.5 * p[10] * dev;
 }
 
+// The eight further particle energies. Each is 1/2 C^T Keff C of its dConstraints (energy.h:336-341) unless
+// noted; Keff is diagonal for all of them, so only its diagonal is kept.
+
+// F = [x1 - x0, x2 - x0] Dinv (3x2) of a triangle; Dinv 2x2 row-major.
+template <class S>
+void triangleF(const V3<S> *x, const double *Di, V3<S> &f0, V3<S> &f1)
+{
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    f0 = {u.x * Di[0] + v.x * Di[2], u.y * Di[0] + v.y * Di[2], u.z * Di[0] + v.z * Di[2]};
+    f1 = {u.x * Di[1] + v.x * Di[3], u.y * Di[1] + v.y * Di[3], u.z * Di[1] + v.z * Di[3]};
+}
+// F = [x1 - x0, x2 - x0, x3 - x0] Dinv (3x3) of a tetrahedron, as columns; Dinv 3x3 row-major.
+template <class S>
+void tetF(const V3<S> *x, const double *Di, V3<S> *f)
+{
+    const V3<S> u = sub(x[1], x[0]);
+    const V3<S> v = sub(x[2], x[0]);
+    const V3<S> w = sub(x[3], x[0]);
+    for (int c = 0; c < 3; ++c)
+        f[c] = {u.x * Di[0 * 3 + c] + v.x * Di[1 * 3 + c] + w.x * Di[2 * 3 + c],
+                u.y * Di[0 * 3 + c] + v.y * Di[1 * 3 + c] + w.y * Di[2 * 3 + c],
+                u.z * Di[0 * 3 + c] + v.z * Di[1 * 3 + c] + w.z * Di[2 * 3 + c]};
+}
+template <class S, int NC>
+S diagQuadratic(const S *C, const double *k)
+{
+    S e = S(0.0);
+    for (int i = 0; i < NC; ++i) e = e + C[i] * (k[i] * C[i]);
+    return 0.5 * e;
+}
+
+// springContinuumEnergy.cpp:9-18 — C = |x0 - x1| / L0 - 1. p: (L0, k L0).
+template <class S>
+void constraintSpringContinuum(const V3<S> *x, const double *p, S *C)
+{
+    const V3<S> d = sub(x[0], x[1]);
+    C[0] = sqrt(dot(d, d)) / p[0] - 1.0;
+}
+// springSquaredContinuumEnergy.cpp:9-19 — C = |x0 - x1|^2 / L0^2 - 1. p: (L0, k L0).
+template <class S>
+void constraintSpringSquaredContinuum(const V3<S> *x, const double *p, S *C)
+{
+    const V3<S> d = sub(x[0], x[1]);
+    C[0] = dot(d, d) / (p[0] * p[0]) - 1.0;
+}
+// membraneBaraffWitkinEnergy.cpp:9-26 — C = (|F0| - 1, |F1| - 1, F0 . F1). p: Dinv (4), Keff diagonal (3).
+template <class S>
+void constraintMembraneBw(const V3<S> *x, const double *p, S *C)
+{
+    V3<S> f0, f1;
+    triangleF(x, p, f0, f1);
+    C[0] = sqrt(dot(f0, f0)) - 1.0;
+    C[1] = sqrt(dot(f1, f1)) - 1.0;
+    C[2] = dot(f0, f1);
+}
+// membraneChoiEnergy.cpp:9-28 — C = (|F0| - 1, |F1| - 1, |(F0 + F1)/sqrt 2| - 1, |(F0 - F1)/sqrt 2| - 1).
+// p: Dinv (4), Keff diagonal (4).
+template <class S>
+void constraintMembraneChoi(const V3<S> *x, const double *p, S *C)
+{
+    V3<S> f0, f1;
+    triangleF(x, p, f0, f1);
+    const double r = 1.0 / std::sqrt(2.0);
+    const V3<S> a = scale(add(f0, f1), r), b = scale(sub(f0, f1), r);
+    C[0] = sqrt(dot(f0, f0)) - 1.0;
+    C[1] = sqrt(dot(f1, f1)) - 1.0;
+    C[2] = sqrt(dot(a, a)) - 1.0;
+    C[3] = sqrt(dot(b, b)) - 1.0;
+}
+// quadraticBendingEnergy.cpp:9-30 — C = L x, L = [c0 I, c1 I, c2 I, c3 I] (3 x 12). p: c0..c3, keff.
+template <class S>
+void constraintQuadraticBending(const V3<S> *x, const double *p, S *C)
+{
+    C[0] = x[0].x * p[0] + x[1].x * p[1] + x[2].x * p[2] + x[3].x * p[3];
+    C[1] = x[0].y * p[0] + x[1].y * p[1] + x[2].y * p[2] + x[3].y * p[3];
+    C[2] = x[0].z * p[0] + x[1].z * p[1] + x[2].z * p[2] + x[3].z * p[3];
+}
+// baraffWitkinBendingEnergy.cpp:9-36 — C = theta - theta0 with the normalisations of e0, n1 and n2 taken
+// as constants (their current values; no derivative flows through them). p: (theta0, k).
+template <class S>
+void constraintBendingBw(const V3<S> *x, const double *p, S *C)
+{
+    const V3<S> e0 = sub(x[1], x[0]);
+    const V3<S> e3 = sub(x[2], x[1]);
+    const V3<S> e4 = sub(x[3], x[1]);
+    const double e0Inv = 1.0 / std::sqrt(valueOf(dot(e0, e0)));
+    const V3<S> n1 = cross(e0, e3);
+    const double n1Inv = 1.0 / std::sqrt(valueOf(dot(n1, n1)));
+    const V3<S> n2 = neg(cross(e0, e4));
+    const double n2Inv = 1.0 / std::sqrt(valueOf(dot(n2, n2)));
+    const V3<S> a = scale(n1, n1Inv), b = scale(n2, n2Inv), e = scale(e0, e0Inv);
+    C[0] = atan2(dot(cross(a, b), e), dot(a, b)) - p[0];
+}
+// Green strain E = 1/2 (F^T F - I) of a tetrahedron (stvkEnergy.cpp:27-29), symmetric, row-major.
+template <class S>
+void tetGreenStrain(const V3<S> *x, const double *Di, S *E)
+{
+    V3<S> f[3];
+    tetF(x, Di, f);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) E[i * 3 + j] = 0.5 * (dot(f[i], f[j]) - (i == j ? 1.0 : 0.0));
+}
+// stvkEnergy.cpp:9-39 — C = (tr E, sqrt(2 tr(E E))), the second replaced by the constant 0 when it is 0.
+template <class S>
+void constraintStvkTet(const V3<S> *x, const double *p, S *C)
+{
+    S E[9];
+    tetGreenStrain(x, p, E);
+    S ee = S(0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) ee = ee + E[i * 3 + j] * E[j * 3 + i];
+    C[0] = E[0] + E[4] + E[8];
+    C[1] = sqrt(2.0 * ee);
+    if (valueOf(C[1]) == 0) C[1] = S(0.0);
+}
+// stableSquaredNeoHookeanEnergy.cpp:9-31 — C = (det F - 1, tr F^T F - 3). p: Dinv (9), lambda, mu.
+template <class S>
+void constraintSsnh(const V3<S> *x, const double *p, S *C)
+{
+    V3<S> f[3];
+    tetF(x, p, f);
+    C[0] = dot(f[0], cross(f[1], f[2])) - 1.0;
+    C[1] = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.0;
+}
+
+template <class S>
+S energySpringContinuum(const V3<S> *x, const double *p)
+{
+    S C[1];
+    constraintSpringContinuum(x, p, C);
+    return diagQuadratic<S, 1>(C, p + 1);
+}
+template <class S>
+S energySpringSquaredContinuum(const V3<S> *x, const double *p)
+{
+    S C[1];
+    constraintSpringSquaredContinuum(x, p, C);
+    return diagQuadratic<S, 1>(C, p + 1);
+}
+template <class S>
+S energyMembraneBw(const V3<S> *x, const double *p)
+{
+    S C[3];
+    constraintMembraneBw(x, p, C);
+    return diagQuadratic<S, 3>(C, p + 4);
+}
+template <class S>
+S energyMembraneChoi(const V3<S> *x, const double *p)
+{
+    S C[4];
+    constraintMembraneChoi(x, p, C);
+    return diagQuadratic<S, 4>(C, p + 4);
+}
+template <class S>
+S energyQuadraticBending(const V3<S> *x, const double *p)
+{
+    S C[3];
+    constraintQuadraticBending(x, p, C);
+    const double k[3] = {p[4], p[4], p[4]};
+    return diagQuadratic<S, 3>(C, k);
+}
+template <class S>
+S energyBendingBw(const V3<S> *x, const double *p)
+{
+    S C[1];
+    constraintBendingBw(x, p, C);
+    return diagQuadratic<S, 1>(C, p + 1);
+}
+// stvkEnergy.cpp:41-63 — the energy form the reference differentiates (dEnergy override):
+// E = 1/2 lambda (tr E)^2 + mu tr(E^T E).
+template <class S>
+S energyStvkTet(const V3<S> *x, const double *p)
+{
+    S E[9];
+    tetGreenStrain(x, p, E);
+    const S tr = E[0] + E[4] + E[8];
+    S ee = S(0.0);
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) ee = ee + E[k * 3 + i] * E[k * 3 + i];
+    return 0.5 * p[9] * tr * tr + p[10] * ee;
+}
+template <class S>
+S energySsnh(const V3<S> *x, const double *p)
+{
+    S C[2];
+    constraintSsnh(x, p, C);
+    return diagQuadratic<S, 2>(C, p + 9);
+}
+
 int stencilSize(int type)
 {
     switch (type) {
@@ -271,6 +487,14 @@ int stencilSize(int type)
     case ORC_DISCRETE_BENDING: return 4;
     case ORC_STABLE_NEO_HOOKEAN: return 4;
     case ORC_SPRING_ANCHOR: return 1;
+    case ORC_SPRING_CONTINUUM: return 2;
+    case ORC_SPRING_SQUARED_CONTINUUM: return 2;
+    case ORC_MEMBRANE_BARAFF_WITKIN: return 3;
+    case ORC_MEMBRANE_CHOI: return 3;
+    case ORC_QUADRATIC_BENDING: return 4;
+    case ORC_BARAFF_WITKIN_BENDING: return 4;
+    case ORC_STVK: return 4;
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: return 4;
     }
     return 0;
 }
@@ -282,6 +506,14 @@ int rawParams(int type)
     case ORC_DISCRETE_BENDING: return 2;
     case ORC_STABLE_NEO_HOOKEAN: return 2;
     case ORC_SPRING_ANCHOR: return 4;
+    case ORC_SPRING_CONTINUUM: return 2;
+    case ORC_SPRING_SQUARED_CONTINUUM: return 2;
+    case ORC_MEMBRANE_BARAFF_WITKIN: return 3;
+    case ORC_MEMBRANE_CHOI: return 3;
+    case ORC_QUADRATIC_BENDING: return 2;
+    case ORC_BARAFF_WITKIN_BENDING: return 2;
+    case ORC_STVK: return 2;
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: return 2;
     }
     return 0;
 }
@@ -293,6 +525,14 @@ int preparedParams(int type)
     case ORC_DISCRETE_BENDING: return 2;
     case ORC_STABLE_NEO_HOOKEAN: return 12;
     case ORC_SPRING_ANCHOR: return 4;
+    case ORC_SPRING_CONTINUUM: return 2;
+    case ORC_SPRING_SQUARED_CONTINUUM: return 2;
+    case ORC_MEMBRANE_BARAFF_WITKIN: return 7;
+    case ORC_MEMBRANE_CHOI: return 8;
+    case ORC_QUADRATIC_BENDING: return 5;
+    case ORC_BARAFF_WITKIN_BENDING: return 2;
+    case ORC_STVK: return 11;
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: return 11;
     }
     return 0;
 }
@@ -306,6 +546,14 @@ S evalEnergy(int type, const V3<S> *x, const double *p)
     case ORC_DISCRETE_BENDING: return energyBending(x, p);
     case ORC_STABLE_NEO_HOOKEAN: return energySnh(x, p);
     case ORC_SPRING_ANCHOR: return energyAnchor(x, p);
+    case ORC_SPRING_CONTINUUM: return energySpringContinuum(x, p);
+    case ORC_SPRING_SQUARED_CONTINUUM: return energySpringSquaredContinuum(x, p);
+    case ORC_MEMBRANE_BARAFF_WITKIN: return energyMembraneBw(x, p);
+    case ORC_MEMBRANE_CHOI: return energyMembraneChoi(x, p);
+    case ORC_QUADRATIC_BENDING: return energyQuadraticBending(x, p);
+    case ORC_BARAFF_WITKIN_BENDING: return energyBendingBw(x, p);
+    case ORC_STVK: return energyStvkTet(x, p);
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: return energySsnh(x, p);
     }
     return S(0.0);
 }
@@ -317,11 +565,22 @@ S evalEnergyN(int type, const V3<S> *x, const double *p)
     if constexpr (NS == 1) {
         return energyAnchor(x, p);
     } else if constexpr (NS == 2) {
+        if (type == ORC_SPRING_CONTINUUM) return energySpringContinuum(x, p);
+        if (type == ORC_SPRING_SQUARED_CONTINUUM) return energySpringSquaredContinuum(x, p);
         return energySpring(x, p);
     } else if constexpr (NS == 3) {
+        if (type == ORC_MEMBRANE_BARAFF_WITKIN) return energyMembraneBw(x, p);
+        if (type == ORC_MEMBRANE_CHOI) return energyMembraneChoi(x, p);
         return energyStvk(x, p);
     } else {
-        return type == ORC_DISCRETE_BENDING ? energyBending(x, p) : energySnh(x, p);
+        switch (type) {
+        case ORC_DISCRETE_BENDING: return energyBending(x, p);
+        case ORC_QUADRATIC_BENDING: return energyQuadraticBending(x, p);
+        case ORC_BARAFF_WITKIN_BENDING: return energyBendingBw(x, p);
+        case ORC_STVK: return energyStvkTet(x, p);
+        case ORC_STABLE_SQUARED_NEO_HOOKEAN: return energySsnh(x, p);
+        default: return energySnh(x, p);
+        }
     }
 }
 
@@ -568,6 +827,8 @@ void cfBending(const double *xs, const double *p, int slot, double *g, double *H
         }
 }
 
+void adVertex(int type, const double *xs, const double *p, int slot, double *g, double *Hb);
+
 void cfVertex(int type, const double *xs, const double *p, int slot, double *g, double *Hb)
 {
     switch (type) {
@@ -577,6 +838,7 @@ void cfVertex(int type, const double *xs, const double *p, int slot, double *g, 
     case ORC_STABLE_NEO_HOOKEAN: cfSnh(xs, p, slot, g, Hb); return;
     case ORC_SPRING_ANCHOR: cfAnchor(xs, p, slot, g, Hb); return;
     }
+    adVertex(type, xs, p, slot, g, Hb);  // the further energies: dual numbers, as the reference (TinyAD)
 }
 
 // AD per-vertex: full element gradient pass + full element Hessian pass, then slice
@@ -602,6 +864,12 @@ void adVertex(int type, const double *xs, const double *p, int slot, double *g, 
     case ORC_STABLE_NEO_HOOKEAN: adVertexN<4>(type, xs, p, slot, g, Hb); return;
     case ORC_SPRING_ANCHOR: adVertexN<1>(type, xs, p, slot, g, Hb); return;
     }
+    switch (stencilSize(type)) {
+    case 2: adVertexN<2>(type, xs, p, slot, g, Hb); return;
+    case 3: adVertexN<3>(type, xs, p, slot, g, Hb); return;
+    case 4: adVertexN<4>(type, xs, p, slot, g, Hb); return;
+    }
+    throw std::runtime_error("unknown energy type");
 }
 
 // Constraints per stencil: the TnConstraints of each energy's StencilBlockEnergy base
@@ -614,6 +882,14 @@ int numConstraints(int type)
     case ORC_SPRING:
     case ORC_DISCRETE_BENDING:
     case ORC_SPRING_ANCHOR: return 1;
+    case ORC_SPRING_CONTINUUM:
+    case ORC_SPRING_SQUARED_CONTINUUM:
+    case ORC_BARAFF_WITKIN_BENDING: return 1;
+    case ORC_MEMBRANE_BARAFF_WITKIN: return 3;
+    case ORC_MEMBRANE_CHOI: return 4;
+    case ORC_QUADRATIC_BENDING: return 3;
+    case ORC_STVK:
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: return 2;
     }
     return 0;
 }
@@ -689,9 +965,14 @@ void addEnergy(Object &o, int type, int64_t n, const int32_t *st, const double *
     for (int64_t i = 0; i < n; ++i) {
         for (int s = 0; s < e.ssize; ++s)
             if (e.st[i * e.ssize + s] < 0 || e.st[i * e.ssize + s] >= o.nv) throw OracleError("stencil index out of range");
-        if (type == ORC_SPRING && e.raw[i * nr] <= 0) throw OracleError("Zero/negative rest length invalid");
-        if (type == ORC_DISCRETE_BENDING && (e.raw[i * nr] < -3.12 || e.raw[i * nr] > 3.12))
-            throw OracleError("Invalid restTheta");
+        const bool springLike =
+            type == ORC_SPRING || type == ORC_SPRING_CONTINUUM || type == ORC_SPRING_SQUARED_CONTINUUM;
+        if (springLike && e.raw[i * nr] <= 0) throw OracleError("Zero/negative rest length invalid");
+        const bool angled = type == ORC_DISCRETE_BENDING || type == ORC_BARAFF_WITKIN_BENDING;
+        if (angled && (e.raw[i * nr] < -3.12 || e.raw[i * nr] > 3.12)) throw OracleError("Invalid restTheta");
+        // quadraticBendingEnergy.cpp:37-39
+        if (type == ORC_QUADRATIC_BENDING && e.raw[i * nr] != 0)
+            throw OracleError("Rest theta != 0 not supported for quadratic bending");
     }
     o.energies.push_back(std::move(e));
 }
@@ -791,6 +1072,93 @@ void prepareEnergy(const Object &o, Energy &e)
             p[10] = mu * volume;
             p[11] = 1 + mu / lambda;
             // stableNeoHookeanEnergy.cpp:74-78,104: effectiveCompliance = diag(lambda, mu).inverse() / volume
+            const double det2 = lambda * mu;
+            cm[0] = mu / det2 / volume;
+            cm[1] = 0.0;
+            cm[2] = 0.0;
+            cm[3] = lambda / det2 / volume;
+            break;
+        }
+        case ORC_SPRING_CONTINUUM:
+        case ORC_SPRING_SQUARED_CONTINUUM:  // springContinuumEnergy.cpp:20-35: the stiffness integrated along L0
+            p[0] = r[0];
+            p[1] = r[1] * r[0];
+            cm[0] = 1.0 / p[1];
+            break;
+        case ORC_MEMBRANE_BARAFF_WITKIN:
+        case ORC_MEMBRANE_CHOI: {  // membraneBaraffWitkinEnergy.cpp:28-68, membraneChoiEnergy.cpp:30-70
+            const bool choi = e.type == ORC_MEMBRANE_CHOI;
+            const double *a = X0(0), *b = X0(1), *c = X0(2);
+            if (a[2] != 0.0 || b[2] != 0.0 || c[2] != 0.0)
+                throw OracleError(choi ? "Rest positions of triangles for membrane Choi energy must have coordinate z=0"
+                                       : "Rest positions of triangles for membrane BaraffWitkin energy must have "
+                                         "coordinate z=0");
+            const double u0 = b[0] - a[0], u1 = b[1] - a[1], v0 = c[0] - a[0], v1 = c[1] - a[1];
+            const double det = u0 * v1 - v0 * u1;
+            p[0] = v1 / det;
+            p[1] = -v0 / det;
+            p[2] = -u1 / det;
+            p[3] = u0 / det;
+            const double area = (u0 * v1 - v0 * u1) * 0.5;
+            if (area <= 0)
+                throw OracleError(choi ? "Rest area of membrane Choi triangle is not positive"
+                                       : "Rest area of membrane BaraffWitkin triangle is not positive");
+            const double k[4] = {r[0], r[1], r[2], r[2]};
+            const int nk = choi ? 4 : 3;
+            for (int q = 0; q < nk; ++q) p[4 + q] = k[q] * area;
+            // compliance: the inverse of the diagonal model stiffness (by cofactors, as the small fixed-size
+            // inverses do), divided by the area
+            double prod = 1.0;
+            for (int q = 0; q < nk; ++q) prod *= k[q];
+            for (int q = 0; q < nk; ++q) cm[q * nk + q] = (prod / k[q]) / prod / area;
+            break;
+        }
+        case ORC_QUADRATIC_BENDING: {  // quadraticBendingEnergy.cpp:49-89
+            const double *a = X0(0), *b = X0(1), *c = X0(2), *d = X0(3);
+            const A3 e0 = a3sub(b, a), e1 = a3sub(c, a), e2 = a3sub(d, a), e3 = a3sub(c, b), e4 = a3sub(d, b);
+            auto cot = [](const A3 &u, const A3 &v) {
+                const A3 w = a3cross(u, v);
+                return a3dot(u, v) / std::sqrt(a3dot(w, w));
+            };
+            const A3 me0 = {-e0[0], -e0[1], -e0[2]};
+            const double c01 = cot(e0, e1), c02 = cot(e0, e2), c03 = cot(me0, e3), c04 = cot(me0, e4);
+            p[0] = c03 + c04;
+            p[1] = c01 + c02;
+            p[2] = -c01 - c03;
+            p[3] = -c02 - c04;
+            const A3 n1 = a3cross(e0, e1), n2 = a3cross(e0, e2);
+            const double keff = r[1] * (3 / (0.5 * (std::sqrt(a3dot(n1, n1)) + std::sqrt(a3dot(n2, n2)))));
+            p[4] = keff;
+            const M3 ci = inverse3({keff, 0, 0, 0, keff, 0, 0, 0, keff});
+            for (int q = 0; q < 9; ++q) cm[q] = ci[q];
+            break;
+        }
+        case ORC_BARAFF_WITKIN_BENDING: {  // baraffWitkinBendingEnergy.cpp:38-75: theta0 from the rest positions
+            const double *a = X0(0), *b = X0(1), *c = X0(2), *d = X0(3);
+            const double xsr[12] = {a[0], a[1], a[2], b[0], b[1], b[2], c[0], c[1], c[2], d[0], d[1], d[2]};
+            p[0] = bendingTheta<double>(reinterpret_cast<const V3<double> *>(xsr));
+            p[1] = r[1];
+            cm[0] = 1.0 / r[1];
+            break;
+        }
+        case ORC_STVK:
+        case ORC_STABLE_SQUARED_NEO_HOOKEAN: {  // stvkEnergy.cpp:66-104, stableSquaredNeoHookeanEnergy.cpp:34-70
+            const double young = r[0], poisson = r[1];
+            const double mu = young / (2 * (1 + poisson));
+            const double lambda = young * poisson / ((1 + poisson) * (1 - 2 * poisson)) + 1e-20;
+            const double *a = X0(0), *b = X0(1), *c = X0(2), *d = X0(3);
+            const A3 u = a3sub(b, a), v = a3sub(c, a), w = a3sub(d, a);
+            const M3 Dm = {u[0], v[0], w[0], u[1], v[1], w[1], u[2], v[2], w[2]};
+            const double det = Dm[0] * (Dm[4] * Dm[8] - Dm[5] * Dm[7]) + Dm[1] * (Dm[5] * Dm[6] - Dm[3] * Dm[8]) +
+                               Dm[2] * (Dm[3] * Dm[7] - Dm[4] * Dm[6]);
+            const double volume = 1. / 6. * det;
+            if (volume <= 0)
+                throw OracleError(e.type == ORC_STVK ? "SNH tetrahedron rest volume is not positive"
+                                                     : "SSNH tetrahedron rest volume is not positive");
+            const M3 Di = inverse3(Dm);
+            for (int q = 0; q < 9; ++q) p[q] = Di[q];
+            p[9] = lambda * volume;
+            p[10] = mu * volume;
             const double det2 = lambda * mu;
             cm[0] = mu / det2 / volume;
             cm[1] = 0.0;
@@ -994,13 +1362,6 @@ void substep(Object &o, double h, int iterations, const double *g, int mode, con
 // energies in addEnergy order, stencils in parallel colour groups (computeParallelStencilGroups,
 // xpbd.cpp:134-160) or in index order (the serial Gauss-Seidel path, xpbd.cpp:99-104).
 // ---------------------------------------------------------------------------------------------
-inline double valueOf(double s) { return s; }
-template <int N, bool H>
-inline double valueOf(const Dual<N, H> &s)
-{
-    return s.v;
-}
-
 // The dConstraints lambdas of the five energies over a generic scalar (double or Dual).
 // springEnergy.cpp:10-18 — C = |x0 - x1| - L0
 template <class S>
@@ -1053,20 +1414,39 @@ void constraintSnh(const V3<S> *x, const double *p, S *C)
     C[1] = sqrt(dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]));
 }
 
-template <int NS, class S>
+template <int NS, int NC, class S>
 void evalConstraintsN(int type, const V3<S> *x, const double *p, S *C)
 {
     if constexpr (NS == 1) {
         constraintAnchor(x, p, C);
     } else if constexpr (NS == 2) {
-        constraintSpring(x, p, C);
-    } else if constexpr (NS == 3) {
-        constraintStvk(x, p, C);
-    } else {
-        if (type == ORC_DISCRETE_BENDING)
+        if (type == ORC_SPRING_CONTINUUM)
+            constraintSpringContinuum(x, p, C);
+        else if (type == ORC_SPRING_SQUARED_CONTINUUM)
+            constraintSpringSquaredContinuum(x, p, C);
+        else
+            constraintSpring(x, p, C);
+    } else if constexpr (NS == 3 && NC == 3) {
+        if (type == ORC_MEMBRANE_BARAFF_WITKIN)
+            constraintMembraneBw(x, p, C);
+        else
+            constraintStvk(x, p, C);
+    } else if constexpr (NS == 3 && NC == 4) {
+        constraintMembraneChoi(x, p, C);
+    } else if constexpr (NS == 4 && NC == 1) {
+        if (type == ORC_BARAFF_WITKIN_BENDING)
+            constraintBendingBw(x, p, C);
+        else
             constraintBending(x, p, C);
+    } else if constexpr (NS == 4 && NC == 2) {
+        if (type == ORC_STVK)
+            constraintStvkTet(x, p, C);
+        else if (type == ORC_STABLE_SQUARED_NEO_HOOKEAN)
+            constraintSsnh(x, p, C);
         else
             constraintSnh(x, p, C);
+    } else if constexpr (NS == 4 && NC == 3) {
+        constraintQuadraticBending(x, p, C);
     }
 }
 
@@ -1084,7 +1464,7 @@ void adConstraints(int type, const double *xs, const double *p, double *c, doubl
         x[s].z = D::var(xs[3 * s + 2], 3 * s + 2);
     }
     D C[NC];
-    evalConstraintsN<NS>(type, x, p, C);
+    evalConstraintsN<NS, NC>(type, x, p, C);
     for (int j = 0; j < NC; ++j) {
         c[j] = C[j].v;
         for (int i = 0; i < N; ++i) grad[i * NC + j] = C[j].g[i];
@@ -1103,9 +1483,30 @@ void smallInverse(const double *A, double *Ai)
         Ai[1] = -A[1] * invdet;
         Ai[2] = -A[2] * invdet;
         Ai[3] = A[0] * invdet;
-    } else {
+    } else if constexpr (NC == 3) {
         const M3 r = inverse3({A[0], A[1], A[2], A[3], A[4], A[5], A[6], A[7], A[8]});
         for (int k = 0; k < 9; ++k) Ai[k] = r[k];
+    } else {
+        // Gauss-Jordan elimination with partial pivoting on [A | I]
+        double m[NC][2 * NC];
+        for (int r = 0; r < NC; ++r)
+            for (int c = 0; c < 2 * NC; ++c) m[r][c] = c < NC ? A[r * NC + c] : (c - NC == r ? 1.0 : 0.0);
+        for (int c = 0; c < NC; ++c) {
+            int piv = c;
+            for (int r = c + 1; r < NC; ++r)
+                if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+            if (piv != c)
+                for (int k = 0; k < 2 * NC; ++k) std::swap(m[c][k], m[piv][k]);
+            const double inv = 1.0 / m[c][c];
+            for (int k = 0; k < 2 * NC; ++k) m[c][k] *= inv;
+            for (int r = 0; r < NC; ++r) {
+                if (r == c) continue;
+                const double f = m[r][c];
+                for (int k = 0; k < 2 * NC; ++k) m[r][k] -= f * m[c][k];
+            }
+        }
+        for (int r = 0; r < NC; ++r)
+            for (int c = 0; c < NC; ++c) Ai[r * NC + c] = m[r][NC + c];
     }
 }
 
@@ -1167,6 +1568,14 @@ void projectStencilDx(int type, const double *xs, const double *w, const double 
     case ORC_MEMBRANE_STVK: projectGeneric<3, 3>(type, xs, w, p, cmp, dt, dx); return;
     case ORC_DISCRETE_BENDING: projectGeneric<4, 1>(type, xs, w, p, cmp, dt, dx); return;
     case ORC_STABLE_NEO_HOOKEAN: projectGeneric<4, 2>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_SPRING_CONTINUUM:
+    case ORC_SPRING_SQUARED_CONTINUUM: projectGeneric<2, 1>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_MEMBRANE_BARAFF_WITKIN: projectGeneric<3, 3>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_MEMBRANE_CHOI: projectGeneric<3, 4>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_QUADRATIC_BENDING: projectGeneric<4, 3>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_BARAFF_WITKIN_BENDING: projectGeneric<4, 1>(type, xs, w, p, cmp, dt, dx); return;
+    case ORC_STVK:
+    case ORC_STABLE_SQUARED_NEO_HOOKEAN: projectGeneric<4, 2>(type, xs, w, p, cmp, dt, dx); return;
     }
     throw OracleError("unknown energy type");
 }
@@ -1308,6 +1717,10 @@ void elementGradHess(int type, const double *xs, const double *p, double *grad, 
     case 1:
         if (grad) adElement<1, false>(type, xs, p, nullptr, grad, nullptr);
         if (hess) adElement<1, true>(type, xs, p, nullptr, nullptr, hess);
+        return;
+    case 2:
+        if (grad) adElement<2, false>(type, xs, p, nullptr, grad, nullptr);
+        if (hess) adElement<2, true>(type, xs, p, nullptr, nullptr, hess);
         return;
     case 3:
         if (grad) adElement<3, false>(type, xs, p, nullptr, grad, nullptr);
@@ -1941,6 +2354,14 @@ int orc_constraints(int type, const double *xs, const double *prepared, double *
         case ORC_MEMBRANE_STVK: adConstraints<3, 3>(type, xs, prepared, c, grad); return;
         case ORC_DISCRETE_BENDING: adConstraints<4, 1>(type, xs, prepared, c, grad); return;
         case ORC_STABLE_NEO_HOOKEAN: adConstraints<4, 2>(type, xs, prepared, c, grad); return;
+        case ORC_SPRING_CONTINUUM:
+        case ORC_SPRING_SQUARED_CONTINUUM: adConstraints<2, 1>(type, xs, prepared, c, grad); return;
+        case ORC_MEMBRANE_BARAFF_WITKIN: adConstraints<3, 3>(type, xs, prepared, c, grad); return;
+        case ORC_MEMBRANE_CHOI: adConstraints<3, 4>(type, xs, prepared, c, grad); return;
+        case ORC_QUADRATIC_BENDING: adConstraints<4, 3>(type, xs, prepared, c, grad); return;
+        case ORC_BARAFF_WITKIN_BENDING: adConstraints<4, 1>(type, xs, prepared, c, grad); return;
+        case ORC_STVK:
+        case ORC_STABLE_SQUARED_NEO_HOOKEAN: adConstraints<4, 2>(type, xs, prepared, c, grad); return;
         }
         throw OracleError("unknown energy type");
     });

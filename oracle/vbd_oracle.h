@@ -34,6 +34,14 @@
  *   ORC_DISCRETE_BENDING [restTheta, stiffness]                  stencil size 4
  *   ORC_STABLE_NEO_HOOKEAN [young, poisson]                      stencil size 4
  *   ORC_SPRING_ANCHOR    [ax, ay, az, stiffness]                 stencil size 1
+ *   ORC_SPRING_CONTINUUM [restLength, stiffness]                 stencil size 2
+ *   ORC_SPRING_SQUARED_CONTINUUM [restLength, stiffness]         stencil size 2
+ *   ORC_MEMBRANE_BARAFF_WITKIN [kWeft, kWarp, kBias]             stencil size 3
+ *   ORC_MEMBRANE_CHOI    [kWeft, kWarp, kBias]                   stencil size 3
+ *   ORC_QUADRATIC_BENDING [restTheta (must be 0), stiffness]     stencil size 4
+ *   ORC_BARAFF_WITKIN_BENDING [restTheta, stiffness]             stencil size 4
+ *   ORC_STVK             [young, poisson]                        stencil size 4
+ *   ORC_STABLE_SQUARED_NEO_HOOKEAN [young, poisson]              stencil size 4
  * Prepared per-stencil parameters (what the energy accessors expose after preparePrecomputations):
  *   SPRING   [restLength, k]                       (restLengths(), effectiveStiffness())
  *   STVK     [Dinv 2x2 row-major (4), Keff 3x3 row-major (9)]  (inverseReferenceMats(), effectiveStiffness())
@@ -41,6 +49,15 @@
  *   SNH      [Dinv 3x3 row-major (9), lambda, mu, alpha]       (inverseReferenceMats(), effectiveStiffness(),
  *                                                   hydrostaticCorrectionTerms())
  *   ANCHOR   [ax, ay, az, k]                       (anchors(), modelStiffness())
+ *   SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM [restLength, k * restLength]  (restLengths(), effectiveStiffness())
+ *   MEMBRANE_BARAFF_WITKIN [Dinv 2x2 (4), Keff diagonal (3)]    (inverseReferenceMats(), effectiveStiffness())
+ *   MEMBRANE_CHOI [Dinv 2x2 (4), Keff diagonal (4)]             (inverseReferenceMats(), effectiveStiffness())
+ *   QUADRATIC_BENDING [c0, c1, c2, c3, keff]      (laplacians() = [c0 I, c1 I, c2 I, c3 I], effectiveStiffness()
+ *                                                   = keff I)
+ *   BARAFF_WITKIN_BENDING [restTheta, k]          (restThetas() recomputed from the rest positions,
+ *                                                   effectiveStiffness())
+ *   STVK, STABLE_SQUARED_NEO_HOOKEAN [Dinv 3x3 (9), lambda, mu]  (inverseReferenceMats(), effectiveStiffness()
+ *                                                   diagonal, premultiplied by the rest volume)
  */
 #pragma once
 #include <stdint.h>
@@ -54,7 +71,15 @@ enum {
     ORC_MEMBRANE_STVK = 2,
     ORC_DISCRETE_BENDING = 3,
     ORC_STABLE_NEO_HOOKEAN = 4,
-    ORC_SPRING_ANCHOR = 5
+    ORC_SPRING_ANCHOR = 5,
+    ORC_SPRING_CONTINUUM = 6,
+    ORC_SPRING_SQUARED_CONTINUUM = 7,
+    ORC_MEMBRANE_BARAFF_WITKIN = 8,
+    ORC_MEMBRANE_CHOI = 9,
+    ORC_QUADRATIC_BENDING = 10,
+    ORC_BARAFF_WITKIN_BENDING = 11,
+    ORC_STVK = 12,
+    ORC_STABLE_SQUARED_NEO_HOOKEAN = 13
 };
 enum { ORC_MODE_CLOSED_FORM = 0, ORC_MODE_AD = 1 };
 
@@ -99,7 +124,9 @@ int orc_coloring_valid(void *h, const int32_t *colors);
 
 /* ---- XPBD ----
  * Compliance per stencil (effectiveCompliance(), nC x nC row-major, nC = orc_num_constraints):
- *   SPRING, ANCHOR 1/k; STVK C^-1 / area; BENDING (1/k) / factor; SNH diag(lambda, mu)^-1 / volume.
+ *   SPRING, ANCHOR 1/k; STVK C^-1 / area; BENDING (1/k) / factor; SNH diag(lambda, mu)^-1 / volume;
+ *   continuum springs 1/(k restLength); Baraff-Witkin and Choi membranes diag(k)^-1 / area; quadratic bending
+ *   I / keff; Baraff-Witkin bending 1/k; StVK tet and stable squared NH diag(lambda, mu)^-1 / volume.
  * Computed by orc_prepare from the raw addStencil arguments. */
 int orc_num_constraints(int type);
 /* coloring::colorStencils on one energy's flat stencils; ncolors may be NULL */

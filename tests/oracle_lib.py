@@ -11,10 +11,17 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 SPRING, MEMBRANE_STVK, DISCRETE_BENDING, STABLE_NEO_HOOKEAN, SPRING_ANCHOR = 1, 2, 3, 4, 5
+(SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM, MEMBRANE_BARAFF_WITKIN, MEMBRANE_CHOI, QUADRATIC_BENDING,
+ BARAFF_WITKIN_BENDING, STVK, STABLE_SQUARED_NEO_HOOKEAN) = range(6, 14)
 CLOSED_FORM, AD = 0, 1
-STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1}
+STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1,
+                SPRING_CONTINUUM: 2, SPRING_SQUARED_CONTINUUM: 2, MEMBRANE_BARAFF_WITKIN: 3, MEMBRANE_CHOI: 3,
+                QUADRATIC_BENDING: 4, BARAFF_WITKIN_BENDING: 4, STVK: 4, STABLE_SQUARED_NEO_HOOKEAN: 4}
 KIND = {"spring": SPRING, "stvk": MEMBRANE_STVK, "bending": DISCRETE_BENDING, "snh": STABLE_NEO_HOOKEAN,
-        "anchor": SPRING_ANCHOR}
+        "anchor": SPRING_ANCHOR, "spring_continuum": SPRING_CONTINUUM,
+        "spring_squared_continuum": SPRING_SQUARED_CONTINUUM, "membrane_bw": MEMBRANE_BARAFF_WITKIN,
+        "membrane_choi": MEMBRANE_CHOI, "bending_quadratic": QUADRATIC_BENDING, "bending_bw": BARAFF_WITKIN_BENDING,
+        "stvk_tet": STVK, "ssnh": STABLE_SQUARED_NEO_HOOKEAN}
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int32)

@@ -97,6 +97,13 @@ int spg_implicit_cg_solve(spg_implicit_ctx *ctx, double alpha, double beta, cons
    *ms receives the mean time of one product. */
 int spg_implicit_bench_spmv(spg_implicit_ctx *ctx, double alpha, double beta, int32_t repeats, double *ms);
 
+/* Test and measurement hook (no context): gradient (n x 3 size) and Hessian (n x (3 size)^2, row-major) of n
+   elements of one energy type at the positions xs (n x 3 size) with their prepared parameters (the layout of
+   spg_vbd_energy), as energyGradient / energyHessian (spg/include/spg/sim/energy/energy.h:268-270) give them.
+   device >= 0 runs the GPU kernel the solvers use; device -1 evaluates the same closed forms on the host. */
+int spg_implicit_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params, double *grad,
+                                     double *hess, int32_t device);
+
 int spg_implicit_get_stats(const spg_implicit_ctx *ctx, spg_implicit_stats *stats);
 int spg_implicit_reset_stats(spg_implicit_ctx *ctx);
 int spg_implicit_set_timing(spg_implicit_ctx *ctx, int32_t enabled);

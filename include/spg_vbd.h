@@ -54,6 +54,14 @@ extern "C" {
 #define SPG_VBD_DISCRETE_BENDING 3    /* DiscreteBendingEnergy     stencil 4 */
 #define SPG_VBD_STABLE_NEO_HOOKEAN 4  /* StableNeoHookeanEnergy    stencil 4 */
 #define SPG_VBD_SPRING_ANCHOR 5       /* SpringAnchorEnergy        stencil 1 */
+#define SPG_VBD_SPRING_CONTINUUM 6            /* SpringContinuumEnergy            stencil 2 */
+#define SPG_VBD_SPRING_SQUARED_CONTINUUM 7    /* SpringSquaredContinuumEnergy     stencil 2 */
+#define SPG_VBD_MEMBRANE_BARAFF_WITKIN 8      /* MembraneBaraffWitkinEnergy       stencil 3 */
+#define SPG_VBD_MEMBRANE_CHOI 9               /* MembraneChoiEnergy               stencil 3 */
+#define SPG_VBD_QUADRATIC_BENDING 10          /* QuadraticBendingEnergy           stencil 4 */
+#define SPG_VBD_BARAFF_WITKIN_BENDING 11      /* BaraffWitkinBendingEnergy        stencil 4 */
+#define SPG_VBD_STVK 12                       /* StvkEnergy (tetrahedron)         stencil 4 */
+#define SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN 13 /* StableSquaredNeoHookeanEnergy    stencil 4 */
 
 /*
  * One energy of a ParticleGroup, as prepared by its preparePrecomputations. Per stencil, `params`
@@ -64,6 +72,16 @@ extern "C" {
  *   STABLE_NEO_HOOKEAN 12: inverseReferenceMats()[i] (3x3), effectiveStiffness()[i](0,0) (lambda),
  *                         effectiveStiffness()[i](1,1) (mu), hydrostaticCorrectionTerms()[i]
  *   SPRING_ANCHOR      4: anchors()[i] (3), modelStiffness()[i](0,0)
+ *   SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM
+ *                      2: restLengths()[i], effectiveStiffness()[i](0,0) (stiffness x rest length)
+ *   MEMBRANE_BARAFF_WITKIN 7: inverseReferenceMats()[i] (2x2), effectiveStiffness()[i] diagonal (3)
+ *   MEMBRANE_CHOI      8: inverseReferenceMats()[i] (2x2), effectiveStiffness()[i] diagonal (4)
+ *   QUADRATIC_BENDING  5: laplacians()[i] as its four coefficients c0..c3 (the 3x12 matrix is
+ *                         [c0 I, c1 I, c2 I, c3 I]), effectiveStiffness()[i](0,0)
+ *   BARAFF_WITKIN_BENDING 2: restThetas()[i], effectiveStiffness()[i](0,0)
+ *   STVK, STABLE_SQUARED_NEO_HOOKEAN
+ *                     11: inverseReferenceMats()[i] (3x3), effectiveStiffness()[i](0,0) (lambda),
+ *                         effectiveStiffness()[i](1,1) (mu)
  * `stencils` is Energy::flatStencils() (n_stencils * stencil size particle indices).
  */
 typedef struct {

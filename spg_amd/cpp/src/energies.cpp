@@ -222,6 +222,277 @@ void SpringAnchorEnergy::addStencil(std::array<int, s_stencilSize> stencil, cons
     m_anchor.push_back(anchor);
 }
 
+namespace {
+// inverse of a diagonal N x N model stiffness, entry by entry
+template <int N>
+Matrix<N, N> inverseDiagonal(const Matrix<N, N> &a)
+{
+    Matrix<N, N> r{};
+    for (int i = 0; i < N; ++i) r(i, i) = 1.0 / a(i, i);
+    return r;
+}
+template <int N>
+Matrix<N, N> diagonal(const std::array<Real, N> &d)
+{
+    Matrix<N, N> r{};
+    for (int i = 0; i < N; ++i) r(i, i) = d[i];
+    return r;
+}
+// F's rest matrix of a triangle in the z = 0 plane: inverse, and the signed area
+Matrix2 restTriangleInverse(const Vector3 &x0, const Vector3 &x1, const Vector3 &x2, Real &area)
+{
+    const Real u0 = x1[0] - x0[0], u1 = x1[1] - x0[1];
+    const Real v0 = x2[0] - x0[0], v1 = x2[1] - x0[1];
+    Matrix2 Dm;
+    Dm(0, 0) = u0;
+    Dm(1, 0) = u1;
+    Dm(0, 1) = v0;
+    Dm(1, 1) = v1;
+    area = (u0 * v1 - v0 * u1) * 0.5;
+    return inverse2(Dm);
+}
+// F's rest matrix of a tetrahedron: inverse, and the signed volume
+Matrix3 restTetInverse(const ParticleGroup &pGroup, const std::array<int, 4> &st, Real &volume)
+{
+    const auto &x0 = pGroup.positions0()[st[0]];
+    const Vector3 u = pGroup.positions0()[st[1]] - x0;
+    const Vector3 v = pGroup.positions0()[st[2]] - x0;
+    const Vector3 w = pGroup.positions0()[st[3]] - x0;
+    Matrix3 Dm;
+    for (int r = 0; r < 3; ++r) {
+        Dm(r, 0) = u[r];
+        Dm(r, 1) = v[r];
+        Dm(r, 2) = w[r];
+    }
+    volume = 1. / 6. * det3(Dm);
+    return inverse3(Dm);
+}
+// Lame parameters of stvkEnergy.cpp:69-73 (lambda + 1e-20 keeps the compliance finite at nu = 0)
+Matrix2 lameStiffness(Real young, Real poisson)
+{
+    const Real mu = young / (2 * (1 + poisson));
+    constexpr Real epsilon = 1e-20;
+    const Real lambda = young * poisson / ((1 + poisson) * (1 - 2 * poisson)) + epsilon;
+    return diagonal<2>({lambda, mu});
+}
+}  // namespace
+
+void SpringContinuumEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real restLength,
+                                       const Real stiffness)
+{
+    m_stencils.push_back(stencil);
+    if (restLength <= 0) {
+        throw std::runtime_error("Zero/negative rest length invalid");
+    }
+    m_restLength.push_back(restLength);
+    m_modelStiffness.push_back(scalarMat<1>(stiffness));
+    // the stiffness integrated along the element
+    m_effectiveStiffness.push_back(scalarMat<1>(stiffness * restLength));
+    m_modelCompliance.push_back(inverse1(m_modelStiffness.back()));
+    m_effectiveCompliance.push_back(inverse1(m_effectiveStiffness.back()));
+}
+
+void SpringSquaredContinuumEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real restLength,
+                                              const Real stiffness)
+{
+    m_stencils.push_back(stencil);
+    if (restLength <= 0) {
+        throw std::runtime_error("Zero/negative rest length invalid");
+    }
+    m_restLength.push_back(restLength);
+    m_modelStiffness.push_back(scalarMat<1>(stiffness));
+    m_effectiveStiffness.push_back(scalarMat<1>(stiffness * restLength));
+    m_modelCompliance.push_back(inverse1(m_modelStiffness.back()));
+    m_effectiveCompliance.push_back(inverse1(m_effectiveStiffness.back()));
+}
+
+void MembraneBaraffWitkinEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real kWeft,
+                                            const Real kWarp, const Real kBias)
+{
+    m_stencils.push_back(stencil);
+    const Matrix3 C = diagonal<3>({kWeft, kWarp, kBias});
+    m_modelStiffness.push_back(C);
+    m_effectiveStiffness.push_back(C);
+    m_modelCompliance.push_back(inverseDiagonal(C));
+    m_effectiveCompliance.push_back(inverseDiagonal(C));
+}
+
+void MembraneBaraffWitkinEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    const int nstencils = static_cast<int>(m_stencils.size());
+    m_inverseReferenceMat.resize(nstencils);
+    for (int i = 0; i < nstencils; ++i) {
+        const auto &x0 = pGroup.positions0()[m_stencils[i][0]];
+        const auto &x1 = pGroup.positions0()[m_stencils[i][1]];
+        const auto &x2 = pGroup.positions0()[m_stencils[i][2]];
+        if (x0.z() != 0.0 || x1.z() != 0.0 || x2.z() != 0.0) {
+            throw std::runtime_error(
+                "Rest positions of triangles for membrane BaraffWitkin energy must have coordinate z=0");
+        }
+        Real area = 0;
+        m_inverseReferenceMat[i] = restTriangleInverse(x0, x1, x2, area);
+        if (area <= 0) {
+            throw std::runtime_error("Rest area of membrane BaraffWitkin triangle is not positive");
+        }
+        m_effectiveCompliance[i] = divided(m_modelCompliance[i], area);
+        m_effectiveStiffness[i] = scaled(m_modelStiffness[i], area);
+    }
+    StencilBlockEnergy<3, 3>::preparePrecomputations(pGroup);
+}
+
+void MembraneChoiEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real kWeft,
+                                    const Real kWarp, const Real kBias)
+{
+    m_stencils.push_back(stencil);
+    const Matrix<4, 4> C = diagonal<4>({kWeft, kWarp, kBias, kBias});
+    m_modelStiffness.push_back(C);
+    m_effectiveStiffness.push_back(C);
+    m_modelCompliance.push_back(inverseDiagonal(C));
+    m_effectiveCompliance.push_back(inverseDiagonal(C));
+}
+
+void MembraneChoiEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    const int nstencils = static_cast<int>(m_stencils.size());
+    m_inverseReferenceMat.resize(nstencils);
+    for (int i = 0; i < nstencils; ++i) {
+        const auto &x0 = pGroup.positions0()[m_stencils[i][0]];
+        const auto &x1 = pGroup.positions0()[m_stencils[i][1]];
+        const auto &x2 = pGroup.positions0()[m_stencils[i][2]];
+        if (x0.z() != 0.0 || x1.z() != 0.0 || x2.z() != 0.0) {
+            throw std::runtime_error("Rest positions of triangles for membrane Choi energy must have coordinate z=0");
+        }
+        Real area = 0;
+        m_inverseReferenceMat[i] = restTriangleInverse(x0, x1, x2, area);
+        if (area <= 0) {
+            throw std::runtime_error("Rest area of membrane Choi triangle is not positive");
+        }
+        m_effectiveCompliance[i] = divided(m_modelCompliance[i], area);
+        m_effectiveStiffness[i] = scaled(m_modelStiffness[i], area);
+    }
+    StencilBlockEnergy<3, 4>::preparePrecomputations(pGroup);
+}
+
+void QuadraticBendingEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real restTheta,
+                                        const Real stiffness)
+{
+    if (restTheta != 0) {
+        throw std::runtime_error("Rest theta != 0 not supported for quadratic bending");
+    }
+    m_stencils.push_back(stencil);
+    m_laplacians.push_back(Matrix<3, 12>{});
+    m_stiffnessFactor.push_back(stiffness);
+    m_modelCompliance.push_back(StiffnessMat{});
+    m_effectiveCompliance.push_back(StiffnessMat{});
+    m_modelStiffness.push_back(StiffnessMat{});
+    m_effectiveStiffness.push_back(StiffnessMat{});
+}
+
+void QuadraticBendingEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    // Bergou et al. 2006: cotangent weights of the hinge's two wings, keff = k * 3 / (wing areas)
+    auto cotangent = [](const Vector3 &a, const Vector3 &b) { return a.dot(b) / a.cross(b).norm(); };
+    const int nstencils = static_cast<int>(m_stencils.size());
+    for (int i = 0; i < nstencils; ++i) {
+        const auto &x0 = pGroup.positions0()[m_stencils[i][0]];
+        const auto &x1 = pGroup.positions0()[m_stencils[i][1]];
+        const auto &x2 = pGroup.positions0()[m_stencils[i][2]];
+        const auto &x3 = pGroup.positions0()[m_stencils[i][3]];
+        const Vector3 e0 = x1 - x0, e1 = x2 - x0, e2 = x3 - x0, e3 = x2 - x1, e4 = x3 - x1;
+        const Vector3 me0 = e0 * -1.0;
+        const Real c01 = cotangent(e0, e1), c02 = cotangent(e0, e2), c03 = cotangent(me0, e3),
+                   c04 = cotangent(me0, e4);
+        const Real c[4] = {c03 + c04, c01 + c02, -c01 - c03, -c02 - c04};
+        Matrix<3, 12> L{};
+        for (int s = 0; s < 4; ++s)
+            for (int d = 0; d < 3; ++d) L(d, 3 * s + d) = c[s];
+        m_laplacians[i] = L;
+        const Real keff = m_stiffnessFactor[i] * (3 / (0.5 * (e0.cross(e1).norm() + e0.cross(e2).norm())));
+        m_effectiveStiffness[i] = diagonal<3>({keff, keff, keff});
+        m_modelStiffness[i] = m_effectiveStiffness[i];
+        m_effectiveCompliance[i] = inverseDiagonal(m_effectiveStiffness[i]);
+        m_modelCompliance[i] = m_effectiveCompliance[i];
+    }
+    StencilBlockEnergy<4, 3>::preparePrecomputations(pGroup);
+}
+
+void BaraffWitkinBendingEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real restTheta,
+                                           const Real stiffness)
+{
+    m_stencils.push_back(stencil);
+    if (restTheta < -3.12 || restTheta > 3.12) {
+        throw std::runtime_error("Invalid restTheta");
+    }
+    m_restTheta.push_back(restTheta);
+    m_modelStiffness.push_back(scalarMat<1>(stiffness));
+    m_effectiveStiffness.push_back(scalarMat<1>(stiffness));
+    m_modelCompliance.push_back(inverse1(m_modelStiffness.back()));
+    m_effectiveCompliance.push_back(inverse1(m_effectiveStiffness.back()));
+}
+
+void BaraffWitkinBendingEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    // no area weighting (the paper bends by curvature); the rest angle comes from the rest positions
+    const int nstencils = static_cast<int>(m_stencils.size());
+    for (int i = 0; i < nstencils; ++i)
+        m_restTheta[i] = hingeAngle(pGroup.positions0()[m_stencils[i][0]], pGroup.positions0()[m_stencils[i][1]],
+                                    pGroup.positions0()[m_stencils[i][2]], pGroup.positions0()[m_stencils[i][3]]);
+    StencilBlockEnergy<4>::preparePrecomputations(pGroup);
+}
+
+void StvkEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real young, const Real poisson)
+{
+    m_stencils.push_back(stencil);
+    const Matrix2 C = lameStiffness(young, poisson);
+    m_modelStiffness.push_back(C);
+    m_effectiveStiffness.push_back(C);
+    m_modelCompliance.push_back(inverse2(C));
+    m_effectiveCompliance.push_back(inverse2(C));
+}
+
+void StvkEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    const int nstencils = static_cast<int>(m_stencils.size());
+    m_inverseReferenceMat.resize(nstencils);
+    for (int i = 0; i < nstencils; ++i) {
+        Real volume = 0;
+        m_inverseReferenceMat[i] = restTetInverse(pGroup, m_stencils[i], volume);
+        if (volume <= 0) {
+            throw std::runtime_error("SNH tetrahedron rest volume is not positive");  // the reference's text
+        }
+        m_effectiveCompliance[i] = divided(m_modelCompliance[i], volume);
+        m_effectiveStiffness[i] = scaled(m_modelStiffness[i], volume);
+    }
+    StencilBlockEnergy<4, 2>::preparePrecomputations(pGroup);
+}
+
+void StableSquaredNeoHookeanEnergy::addStencil(const std::array<int, s_stencilSize> &stencil, const Real young,
+                                               const Real poisson)
+{
+    m_stencils.push_back(stencil);
+    const Matrix2 C = lameStiffness(young, poisson);
+    m_modelStiffness.push_back(C);
+    m_effectiveStiffness.push_back(C);
+    m_modelCompliance.push_back(inverse2(C));
+    m_effectiveCompliance.push_back(inverse2(C));
+}
+
+void StableSquaredNeoHookeanEnergy::preparePrecomputations(const ParticleGroup &pGroup)
+{
+    const int nstencils = static_cast<int>(m_stencils.size());
+    m_inverseReferenceMat.resize(nstencils);
+    for (int i = 0; i < nstencils; ++i) {
+        Real volume = 0;
+        m_inverseReferenceMat[i] = restTetInverse(pGroup, m_stencils[i], volume);
+        if (volume <= 0) {
+            throw std::runtime_error("SSNH tetrahedron rest volume is not positive");
+        }
+        m_effectiveCompliance[i] = divided(m_modelCompliance[i], volume);
+        m_effectiveStiffness[i] = scaled(m_modelStiffness[i], volume);
+    }
+    StencilBlockEnergy<4, 2>::preparePrecomputations(pGroup);
+}
+
 TriangleMesh::TriangleMesh(const std::vector<Vector3> &vertices, const std::vector<Vector3> &vertices0,
                            const std::vector<Int3> &faces)
     : m_vertices(vertices), m_vertices0(vertices0), m_faces(faces)

@@ -22,11 +22,89 @@ int64_t flatStencilSize(int32_t type)
 {
     switch (type) {
     case SPG_VBD_SPRING_ANCHOR: return 1;
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 3;
+    case SPG_VBD_SPRING:
+    case SPG_VBD_SPRING_CONTINUUM:
+    case SPG_VBD_SPRING_SQUARED_CONTINUUM: return 2;
+    case SPG_VBD_MEMBRANE_STVK:
+    case SPG_VBD_MEMBRANE_BARAFF_WITKIN:
+    case SPG_VBD_MEMBRANE_CHOI: return 3;
     default: return 4;
     }
 }
+
+namespace {
+
+// The further energies in either layout: `stiffness` selects effectiveStiffness() (spg_vbd_energy) or
+// effectiveCompliance() (spg_xpbd_energy); both are diagonal for these energies, so only the diagonal goes.
+template <int N>
+void pushDiagonal(std::vector<double> &out, const Matrix<N, N> &m)
+{
+    for (int i = 0; i < N; ++i) out.push_back(m(i, i));
+}
+
+bool flattenFurther(const ParticleGroup::EnergyT *energy, bool stiffness, FlatEnergy &f)
+{
+    auto mats = [&](const auto *e, int i) -> const auto & {
+        return stiffness ? e->effectiveStiffness()[i] : e->effectiveCompliance()[i];
+    };
+    if (auto *e = dynamic_cast<const SpringContinuumEnergy *>(energy)) {
+        f.type = SPG_VBD_SPRING_CONTINUUM;
+        for (int i = 0; i < e->nStencils(); ++i) f.params.insert(f.params.end(), {e->restLengths()[i], mats(e, i)(0, 0)});
+    } else if (auto *e = dynamic_cast<const SpringSquaredContinuumEnergy *>(energy)) {
+        f.type = SPG_VBD_SPRING_SQUARED_CONTINUUM;
+        for (int i = 0; i < e->nStencils(); ++i) f.params.insert(f.params.end(), {e->restLengths()[i], mats(e, i)(0, 0)});
+    } else if (auto *e = dynamic_cast<const MembraneBaraffWitkinEnergy *>(energy)) {
+        requirePrepared(*e, "MembraneBaraffWitkinEnergy");
+        f.type = SPG_VBD_MEMBRANE_BARAFF_WITKIN;
+        for (int i = 0; i < e->nStencils(); ++i) {
+            const auto &D = e->inverseReferenceMats()[i];
+            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
+            pushDiagonal(f.params, mats(e, i));
+        }
+    } else if (auto *e = dynamic_cast<const MembraneChoiEnergy *>(energy)) {
+        requirePrepared(*e, "MembraneChoiEnergy");
+        f.type = SPG_VBD_MEMBRANE_CHOI;
+        for (int i = 0; i < e->nStencils(); ++i) {
+            const auto &D = e->inverseReferenceMats()[i];
+            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
+            pushDiagonal(f.params, mats(e, i));
+        }
+    } else if (auto *e = dynamic_cast<const QuadraticBendingEnergy *>(energy)) {
+        requirePrepared(*e, "QuadraticBendingEnergy");
+        f.type = SPG_VBD_QUADRATIC_BENDING;
+        for (int i = 0; i < e->nStencils(); ++i) {
+            const auto &L = e->laplacians()[i];  // [c0 I, c1 I, c2 I, c3 I]
+            for (int s = 0; s < 4; ++s) f.params.push_back(L(0, 3 * s));
+            f.params.push_back(mats(e, i)(0, 0));
+        }
+    } else if (auto *e = dynamic_cast<const BaraffWitkinBendingEnergy *>(energy)) {
+        requirePrepared(*e, "BaraffWitkinBendingEnergy");
+        f.type = SPG_VBD_BARAFF_WITKIN_BENDING;
+        for (int i = 0; i < e->nStencils(); ++i) f.params.insert(f.params.end(), {e->restThetas()[i], mats(e, i)(0, 0)});
+    } else if (auto *e = dynamic_cast<const StvkEnergy *>(energy)) {
+        requirePrepared(*e, "StvkEnergy");
+        f.type = SPG_VBD_STVK;
+        for (int i = 0; i < e->nStencils(); ++i) {
+            const auto &D = e->inverseReferenceMats()[i];
+            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
+            pushDiagonal(f.params, mats(e, i));
+        }
+    } else if (auto *e = dynamic_cast<const StableSquaredNeoHookeanEnergy *>(energy)) {
+        requirePrepared(*e, "StableSquaredNeoHookeanEnergy");
+        f.type = SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN;
+        for (int i = 0; i < e->nStencils(); ++i) {
+            const auto &D = e->inverseReferenceMats()[i];
+            f.params.insert(f.params.end(), D.m.begin(), D.m.end());
+            pushDiagonal(f.params, mats(e, i));
+        }
+    } else {
+        return false;
+    }
+    f.stencils = energy->flatStencils();
+    return true;
+}
+
+}  // namespace
 
 FlatEnergy flattenVbd(const ParticleGroup::EnergyT *energy)
 {
@@ -74,7 +152,7 @@ FlatEnergy flattenVbd(const ParticleGroup::EnergyT *energy)
             const auto &a = e->anchors()[i];
             f.params.insert(f.params.end(), {a[0], a[1], a[2], e->modelStiffness()[i](0, 0)});
         }
-    } else {
+    } else if (!flattenFurther(energy, true, f)) {
         throw std::runtime_error("VBDGpu: unsupported energy type for the GPU solver");
     }
     return f;
@@ -127,7 +205,7 @@ FlatEnergy flattenXpbd(const ParticleGroup::EnergyT *energy)
             const auto &a = e->anchors()[i];
             f.params.insert(f.params.end(), {a[0], a[1], a[2], e->effectiveCompliance()[i](0, 0)});
         }
-    } else {
+    } else if (!flattenFurther(energy, false, f)) {
         throw std::runtime_error("XPBDGpu: unsupported energy type for the GPU solver");
     }
     return f;

@@ -47,6 +47,22 @@ std::array<int, N> stencilAt(const int32_t *s, int64_t i)
     return a;
 }
 
+// addStencil(stencil, a, b) / addStencil(stencil, a, b, c) over n stencils with their raw arguments
+template <class E, int N>
+std::shared_ptr<spg::ParticleGroup::EnergyT> addStencils2(int64_t n, const int32_t *st, const double *a)
+{
+    auto e = std::make_shared<E>();
+    for (int64_t i = 0; i < n; ++i) e->addStencil(stencilAt<E, N>(st, i), a[2 * i], a[2 * i + 1]);
+    return e;
+}
+template <class E, int N>
+std::shared_ptr<spg::ParticleGroup::EnergyT> addStencils3(int64_t n, const int32_t *st, const double *a)
+{
+    auto e = std::make_shared<E>();
+    for (int64_t i = 0; i < n; ++i) e->addStencil(stencilAt<E, N>(st, i), a[3 * i], a[3 * i + 1], a[3 * i + 2]);
+    return e;
+}
+
 void checkIndex(const spg_pgroup *pg, int32_t index)
 {
     if (!pg || index < 0 || index >= static_cast<int32_t>(pg->group.energies().size()))
@@ -111,6 +127,30 @@ int spg_pg_add_energy(spg_pgroup *pg, int32_t type, int64_t n, const int32_t *st
             energy = e;
             break;
         }
+        case SPG_VBD_SPRING_CONTINUUM:
+            energy = addStencils2<spg::SpringContinuumEnergy, 2>(n, st, a);
+            break;
+        case SPG_VBD_SPRING_SQUARED_CONTINUUM:
+            energy = addStencils2<spg::SpringSquaredContinuumEnergy, 2>(n, st, a);
+            break;
+        case SPG_VBD_MEMBRANE_BARAFF_WITKIN:
+            energy = addStencils3<spg::MembraneBaraffWitkinEnergy, 3>(n, st, a);
+            break;
+        case SPG_VBD_MEMBRANE_CHOI:
+            energy = addStencils3<spg::MembraneChoiEnergy, 3>(n, st, a);
+            break;
+        case SPG_VBD_QUADRATIC_BENDING:
+            energy = addStencils2<spg::QuadraticBendingEnergy, 4>(n, st, a);
+            break;
+        case SPG_VBD_BARAFF_WITKIN_BENDING:
+            energy = addStencils2<spg::BaraffWitkinBendingEnergy, 4>(n, st, a);
+            break;
+        case SPG_VBD_STVK:
+            energy = addStencils2<spg::StvkEnergy, 4>(n, st, a);
+            break;
+        case SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN:
+            energy = addStencils2<spg::StableSquaredNeoHookeanEnergy, 4>(n, st, a);
+            break;
         default: throw std::runtime_error("unsupported energy type");
         }
         for (int64_t i = 0; i < n * energy->stencilSize(); ++i)
@@ -141,38 +181,8 @@ int spg_pg_energy_params(const spg_pgroup *pg, int32_t index, double *out)
 {
     return guarded([&] {
         checkIndex(pg, index);
-        const auto *energy = pg->group.energies()[index].get();
-        size_t k = 0;
-        if (auto *e = dynamic_cast<const spg::SpringEnergy *>(energy)) {
-            for (int i = 0; i < e->nStencils(); ++i) {
-                out[k++] = e->restLengths()[i];
-                out[k++] = e->effectiveStiffness()[i](0, 0);
-            }
-        } else if (auto *e = dynamic_cast<const spg::MembraneStvkEnergy *>(energy)) {
-            for (int i = 0; i < e->nStencils(); ++i) {
-                for (double v : e->inverseReferenceMats()[i].m) out[k++] = v;
-                for (double v : e->effectiveStiffness()[i].m) out[k++] = v;
-            }
-        } else if (auto *e = dynamic_cast<const spg::DiscreteBendingEnergy *>(energy)) {
-            for (int i = 0; i < e->nStencils(); ++i) {
-                out[k++] = e->restThetas()[i];
-                out[k++] = e->effectiveStiffness()[i](0, 0);
-            }
-        } else if (auto *e = dynamic_cast<const spg::StableNeoHookeanEnergy *>(energy)) {
-            for (int i = 0; i < e->nStencils(); ++i) {
-                for (double v : e->inverseReferenceMats()[i].m) out[k++] = v;
-                out[k++] = e->effectiveStiffness()[i](0, 0);
-                out[k++] = e->effectiveStiffness()[i](1, 1);
-                out[k++] = e->hydrostaticCorrectionTerms()[i];
-            }
-        } else if (auto *e = dynamic_cast<const spg::SpringAnchorEnergy *>(energy)) {
-            for (int i = 0; i < e->nStencils(); ++i) {
-                for (int d = 0; d < 3; ++d) out[k++] = e->anchors()[i][d];
-                out[k++] = e->modelStiffness()[i](0, 0);
-            }
-        } else {
-            throw std::runtime_error("unsupported energy type");
-        }
+        const auto f = spg::solver::flattenVbd(pg->group.energies()[index].get());
+        std::copy(f.params.begin(), f.params.end(), out);
     });
 }
 

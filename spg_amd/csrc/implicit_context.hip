@@ -12,6 +12,7 @@
 
 #include "host_util.h"
 #include "implicit_device.h"
+#include "implicit_element.h"
 #include "spg_implicit.h"
 
 using namespace spgvbd;
@@ -20,28 +21,9 @@ namespace {
 
 thread_local std::string g_lastError;
 
-int stencilSizeOf(int type)
-{
-    switch (type) {
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 3;
-    case SPG_VBD_DISCRETE_BENDING: return 4;
-    case SPG_VBD_STABLE_NEO_HOOKEAN: return 4;
-    case SPG_VBD_SPRING_ANCHOR: return 1;
-    }
-    return 0;
-}
-int paramsOf(int type)
-{
-    switch (type) {
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 13;
-    case SPG_VBD_DISCRETE_BENDING: return 2;
-    case SPG_VBD_STABLE_NEO_HOOKEAN: return 12;
-    case SPG_VBD_SPRING_ANCHOR: return 4;
-    }
-    return 0;
-}
+constexpr int kNumTypes = SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN;
+int stencilSizeOf(int type) { return type >= 1 && type <= kNumTypes ? fullStencilSize(type) : 0; }
+int paramsOf(int type) { return type >= 1 && type <= kNumTypes ? fullParams(type) : 0; }
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
@@ -65,7 +47,7 @@ struct spg_implicit_ctx {
         DevBuf<double> prm;
         int64_t gradOff = 0, hessOff = 0;  // in doubles
     };
-    Table tables[6];
+    Table tables[kNumTypes + 1];
     DevBuf<int64_t> incPtr, incOff;
     // block-sparse pattern: the full one (exports) and the upper storage of the symmetric matrices
     int64_t nBlocks = 0, nUpper = 0;
@@ -125,7 +107,7 @@ struct spg_implicit_ctx {
     // element gradients (and Hessians when `hessians`) at positions xp; then the force f at xp
     void derivatives(const double *xp, bool hessians, const double *g)
     {
-        for (int t = 1; t <= 5; ++t) {
+        for (int t = 1; t <= kNumTypes; ++t) {
             Table &tb = tables[t];
             if (!tb.n) continue;
             ImplicitElems e{t, tb.n, tb.idx.p, tb.prm.p, grad.p + tb.gradOff, hessians ? hess.p + tb.hessOff : nullptr};
@@ -391,10 +373,10 @@ void validateEnergy(const spg_vbd_energy &en, int64_t np)
         if (en.stencils[i] < 0 || en.stencils[i] >= np) throw VbdError(SPG_VBD_ERR_INVALID, "stencil index out of range");
     for (int64_t i = 0; i < en.n_stencils * pp; ++i)
         if (!std::isfinite(en.params[i])) throw VbdError(SPG_VBD_ERR_INVALID, "non-finite energy parameter");
-    if (en.type == SPG_VBD_SPRING)
+    if (en.type == SPG_VBD_SPRING || en.type == SPG_VBD_SPRING_CONTINUUM || en.type == SPG_VBD_SPRING_SQUARED_CONTINUUM)
         for (int64_t i = 0; i < en.n_stencils; ++i)
             if (en.params[i * pp] <= 0) throw VbdError(SPG_VBD_ERR_INVALID, "Zero/negative rest length invalid");
-    if (en.type == SPG_VBD_DISCRETE_BENDING)
+    if (en.type == SPG_VBD_DISCRETE_BENDING || en.type == SPG_VBD_BARAFF_WITKIN_BENDING)
         for (int64_t i = 0; i < en.n_stencils; ++i)
             if (en.params[i * pp] < -3.12 || en.params[i * pp] > 3.12)
                 throw VbdError(SPG_VBD_ERR_INVALID, "Invalid restTheta");
@@ -447,7 +429,7 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
     std::vector<int64_t> firstElem(6, 0);  // element index of each (object, energy) in its type table
     std::vector<std::vector<int64_t>> elemBase(nObjects);
     for (int32_t o = 0; o < nObjects; ++o) elemBase[o].assign(objs[o].n_energies, 0);
-    for (int t = 1; t <= 5; ++t) {
+    for (int t = 1; t <= kNumTypes; ++t) {
         auto &tb = c.tables[t];
         tb.type = t;
         tb.n = count[t];
@@ -782,6 +764,55 @@ int spg_implicit_cg_solve(spg_implicit_ctx *ctx, double alpha, double beta, cons
         hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
         if (iterations) *iterations = ctx->hostCg->iters;
         if (error) *error = ctx->stats.last_cg_error;
+    });
+}
+
+int spg_implicit_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params, double *grad,
+                                     double *hess, int32_t device)
+{
+    return guarded(nullptr, [&] {
+        const int ns = stencilSizeOf(type), np = paramsOf(type);
+        if (!ns) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "unknown energy type");
+        if (n < 0 || (n > 0 && (!xs || !params || !grad || !hess)))
+            throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        const int N = 3 * ns;
+        if (device < 0) {  // host instantiation of the same closed forms
+            for (int64_t i = 0; i < n; ++i) {
+                switch (ns) {
+                case 1: elementGradHessFull<1>(type, xs + i * N, params + i * np, grad + i * N, hess + i * N * N); break;
+                case 2: elementGradHessFull<2>(type, xs + i * N, params + i * np, grad + i * N, hess + i * N * N); break;
+                case 3: elementGradHessFull<3>(type, xs + i * N, params + i * np, grad + i * N, hess + i * N * N); break;
+                default: elementGradHessFull<4>(type, xs + i * N, params + i * np, grad + i * N, hess + i * N * N);
+                }
+            }
+            return;
+        }
+        if (n == 0) return;
+        hipCheck(hipSetDevice(device), "hipSetDevice");
+        hipStream_t stream = nullptr;
+        hipCheck(hipStreamCreate(&stream), "hipStreamCreate");
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{stream};
+        std::vector<int32_t> idx(n * ns);
+        for (int64_t k = 0; k < n * ns; ++k) idx[k] = static_cast<int32_t>(k);
+        DevBuf<int32_t> dIdx;
+        DevBuf<double> dX, dP, dG, dH;
+        dIdx.upload(idx, stream);
+        dX.upload(std::vector<double>(xs, xs + n * N), stream);
+        dP.upload(std::vector<double>(params, params + n * np), stream);
+        dG.alloc(n * N);
+        dH.alloc(n * N * N);
+        const ImplicitElems e{type, n, dIdx.p, dP.p, dG.p, dH.p};
+        hipCheck(launchElementDerivatives(e, dX.p, stream), "element derivative launch");
+        std::vector<double> hH(n * N * N);
+        hipCheck(hipMemcpyAsync(grad, dG.p, sizeof(double) * n * N, hipMemcpyDeviceToHost, stream), "download");
+        hipCheck(hipMemcpyAsync(hH.data(), dH.p, sizeof(double) * n * N * N, hipMemcpyDeviceToHost, stream), "download");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        for (int64_t i = 0; i < n; ++i)  // column-major per element on the device
+            for (int r = 0; r < N; ++r)
+                for (int c = 0; c < N; ++c) hess[i * N * N + r * N + c] = hH[i * N * N + c * N + r];
     });
 }
 

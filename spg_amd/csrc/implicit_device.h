@@ -26,7 +26,7 @@
 namespace spgvbd {
 
 struct ImplicitElems {
-    int type;              // 1 spring, 2 membrane, 3 bending, 4 SNH, 5 anchor
+    int type;              // SPG_VBD_* energy type (1 ... 13)
     int64_t n;
     const int32_t *idx;    // n * size global particle ids
     const double *prm;     // n * params

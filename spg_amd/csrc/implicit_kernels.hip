@@ -22,7 +22,7 @@ __global__ void __launch_bounds__(128) elementDerivKernel(ImplicitElems e, const
 {
     constexpr int NS = fullStencilSize(TYPE);
     constexpr int N = 3 * NS;
-    constexpr int NP = TYPE == 2 ? 13 : TYPE == 4 ? 12 : TYPE == 5 ? 4 : 2;
+    constexpr int NP = fullParams(TYPE);
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= e.n) return;
     double p[NP];
@@ -67,7 +67,7 @@ template <int TYPE>
 __global__ void __launch_bounds__(256) elementGradKernel(ImplicitElems e, const double *__restrict__ x)
 {
     constexpr int NS = fullStencilSize(TYPE);
-    constexpr int NP = TYPE == 2 ? 13 : TYPE == 4 ? 12 : TYPE == 5 ? 4 : 2;
+    constexpr int NP = fullParams(TYPE);
     const int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
     if (i >= e.n) return;
     double p[NP];
@@ -461,18 +461,29 @@ __global__ void __launch_bounds__(1024) sumFinalKernel(const double *__restrict_
 
 }  // namespace
 
+#define SPG_ELEMENT_CASES(KERNEL, BS)                                                                              \
+    switch (e.type) {                                                                                          \
+    case 1: KERNEL<1><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 2: KERNEL<2><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 3: KERNEL<3><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 4: KERNEL<4><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 5: KERNEL<5><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 6: KERNEL<6><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 7: KERNEL<7><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 8: KERNEL<8><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 9: KERNEL<9><<<grid, BS, 0, stream>>>(e, x); break;                                                   \
+    case 10: KERNEL<10><<<grid, BS, 0, stream>>>(e, x); break;                                                 \
+    case 11: KERNEL<11><<<grid, BS, 0, stream>>>(e, x); break;                                                 \
+    case 12: KERNEL<12><<<grid, BS, 0, stream>>>(e, x); break;                                                 \
+    case 13: KERNEL<13><<<grid, BS, 0, stream>>>(e, x); break;                                                 \
+    default: return hipErrorInvalidValue;                                                                      \
+    }
+
 hipError_t launchElementDerivatives(const ImplicitElems &e, const double *x, hipStream_t stream)
 {
     if (e.n <= 0) return hipSuccess;
     const int grid = blocksFor(e.n, 128);
-    switch (e.type) {
-    case 1: elementDerivKernel<1><<<grid, 128, 0, stream>>>(e, x); break;
-    case 2: elementDerivKernel<2><<<grid, 128, 0, stream>>>(e, x); break;
-    case 3: elementDerivKernel<3><<<grid, 128, 0, stream>>>(e, x); break;
-    case 4: elementDerivKernel<4><<<grid, 128, 0, stream>>>(e, x); break;
-    case 5: elementDerivKernel<5><<<grid, 128, 0, stream>>>(e, x); break;
-    default: return hipErrorInvalidValue;
-    }
+    SPG_ELEMENT_CASES(elementDerivKernel, 128)
     return hipGetLastError();
 }
 
@@ -480,16 +491,11 @@ hipError_t launchElementGradients(const ImplicitElems &e, const double *x, hipSt
 {
     if (e.n <= 0) return hipSuccess;
     const int grid = blocksFor(e.n, 256);
-    switch (e.type) {
-    case 1: elementGradKernel<1><<<grid, 256, 0, stream>>>(e, x); break;
-    case 2: elementGradKernel<2><<<grid, 256, 0, stream>>>(e, x); break;
-    case 3: elementGradKernel<3><<<grid, 256, 0, stream>>>(e, x); break;
-    case 4: elementGradKernel<4><<<grid, 256, 0, stream>>>(e, x); break;
-    case 5: elementGradKernel<5><<<grid, 256, 0, stream>>>(e, x); break;
-    default: return hipErrorInvalidValue;
-    }
+    SPG_ELEMENT_CASES(elementGradKernel, 256)
     return hipGetLastError();
 }
+
+#undef SPG_ELEMENT_CASES
 
 hipError_t launchGatherForce(int64_t nParticles, const double *mass, double gx, double gy, double gz,
                              const int64_t *incPtr, const int64_t *incOff, const double *gradBase, double *f,

@@ -17,11 +17,22 @@ VBD_LIB = os.path.join(LIBDIR, "libspg_vbd.so")
 HOST_LIB = os.path.join(LIBDIR, "libspg_host.so")
 
 SPRING, MEMBRANE_STVK, DISCRETE_BENDING, STABLE_NEO_HOOKEAN, SPRING_ANCHOR = 1, 2, 3, 4, 5
+(SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM, MEMBRANE_BARAFF_WITKIN, MEMBRANE_CHOI, QUADRATIC_BENDING,
+ BARAFF_WITKIN_BENDING, STVK, STABLE_SQUARED_NEO_HOOKEAN) = range(6, 14)
 KIND = {"spring": SPRING, "stvk": MEMBRANE_STVK, "bending": DISCRETE_BENDING, "snh": STABLE_NEO_HOOKEAN,
-        "anchor": SPRING_ANCHOR}
-STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1}
-PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 12, SPRING_ANCHOR: 4}
-XPBD_PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 14, SPRING_ANCHOR: 4}
+        "anchor": SPRING_ANCHOR, "spring_continuum": SPRING_CONTINUUM,
+        "spring_squared_continuum": SPRING_SQUARED_CONTINUUM, "membrane_bw": MEMBRANE_BARAFF_WITKIN,
+        "membrane_choi": MEMBRANE_CHOI, "bending_quadratic": QUADRATIC_BENDING, "bending_bw": BARAFF_WITKIN_BENDING,
+        "stvk_tet": STVK, "ssnh": STABLE_SQUARED_NEO_HOOKEAN}
+STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1,
+                SPRING_CONTINUUM: 2, SPRING_SQUARED_CONTINUUM: 2, MEMBRANE_BARAFF_WITKIN: 3, MEMBRANE_CHOI: 3,
+                QUADRATIC_BENDING: 4, BARAFF_WITKIN_BENDING: 4, STVK: 4, STABLE_SQUARED_NEO_HOOKEAN: 4}
+PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 12, SPRING_ANCHOR: 4,
+          SPRING_CONTINUUM: 2, SPRING_SQUARED_CONTINUUM: 2, MEMBRANE_BARAFF_WITKIN: 7, MEMBRANE_CHOI: 8,
+          QUADRATIC_BENDING: 5, BARAFF_WITKIN_BENDING: 2, STVK: 11, STABLE_SQUARED_NEO_HOOKEAN: 11}
+XPBD_PARAMS = {SPRING: 2, MEMBRANE_STVK: 13, DISCRETE_BENDING: 2, STABLE_NEO_HOOKEAN: 14, SPRING_ANCHOR: 4,
+               SPRING_CONTINUUM: 2, SPRING_SQUARED_CONTINUUM: 2, MEMBRANE_BARAFF_WITKIN: 7, MEMBRANE_CHOI: 8,
+               QUADRATIC_BENDING: 5, BARAFF_WITKIN_BENDING: 2, STVK: 11, STABLE_SQUARED_NEO_HOOKEAN: 11}
 ERR_NONFINITE = 5
 ERR_STATE = 4
 
@@ -203,6 +214,7 @@ def vbd_lib():
         _proto(L, "spg_implicit_stiffness", ctypes.c_int, [c, _lp, _ip, _dp])
         _proto(L, "spg_implicit_cg_solve", ctypes.c_int, [c, dbl, dbl, _dp, dbl, i32, _dp, _ip, _dp])
         _proto(L, "spg_implicit_bench_spmv", ctypes.c_int, [c, dbl, dbl, i32, _dp])
+        _proto(L, "spg_implicit_element_derivatives", ctypes.c_int, [i32, ctypes.c_int64, _dp, _dp, _dp, _dp, i32])
         _proto(L, "spg_implicit_get_stats", ctypes.c_int, [c, ctypes.POINTER(ImplicitStats)])
         _proto(L, "spg_implicit_reset_stats", ctypes.c_int, [c])
         _proto(L, "spg_implicit_set_timing", ctypes.c_int, [c, i32])
@@ -921,6 +933,23 @@ def xpbd_color_stencils(stencils):
     if rc != 0:
         raise NativeError(rc, L.spg_xpbd_last_error(None).decode())
     return out
+
+
+def implicit_element_derivatives(kind, xs, params, device=0):
+    """Gradients (n, 3 size) and Hessians (n, 3 size, 3 size) of n elements (xs (n, 3 size), params in the
+    spg_vbd_energy layout) by the implicit solvers' closed forms: device -1 on the host, else the GPU kernel."""
+    L = vbd_lib()
+    t = _kind(kind)
+    ns = STENCIL_SIZE[t]
+    xs = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3 * ns)
+    p = np.ascontiguousarray(params, dtype=np.float64).reshape(xs.shape[0], PARAMS[t])
+    n = xs.shape[0]
+    g = np.zeros((n, 3 * ns))
+    h = np.zeros((n, 3 * ns, 3 * ns))
+    rc = L.spg_implicit_element_derivatives(t, n, _d(xs), _d(p), _d(g), _d(h), int(device))
+    if rc:
+        raise NativeError(rc, L.spg_implicit_last_error(None).decode())
+    return g, h
 
 
 def xpbd_project_stencils(kind, xs, inv_masses, params, dt, precision=0, device=0):

@@ -77,7 +77,7 @@ class Scene:
         pg = native.ParticleGroup()
         pg.add_particles(self.x, self.x0, self.m)
         for kind, st, args in self.energies:
-            pg.add_energy(kind, st, args, prepare=prepare and kind in ("stvk", "bending", "snh"))
+            pg.add_energy(kind, st, args, prepare=prepare and kind in PREPARED_KINDS)
         return pg
 
     def vbd_object(self):
@@ -92,6 +92,11 @@ class Scene:
         pg = self.particle_group()
         return {"x": self.x, "v": None, "w": pg.inv_masses(),
                 "energies": [(kind, st, pg.xpbd_params(i)) for i, (kind, st, _) in enumerate(self.energies)]}
+
+
+# energies with a preparePrecomputations of their own (rest-state matrices, rest angles, volumes)
+PREPARED_KINDS = ("stvk", "bending", "snh", "membrane_bw", "membrane_choi", "bending_quadratic", "bending_bw",
+                  "stvk_tet", "ssnh")
 
 
 def grid_cloth_mesh(n):
@@ -109,7 +114,11 @@ def grid_cloth_mesh(n):
 
 
 def cloth(n, young, poisson=0.3, thickness=1e-3, bending=1e-4, mass=1e-3, pin="corners", noise=1e-4, seed=1,
-          anchor_k=1e5, iterations=10):
+          anchor_k=1e5, iterations=10, membrane="stvk", bending_kind="bending", edge_springs=None):
+    """membrane: "stvk" (MembraneStvkEnergy), "membrane_bw" or "membrane_choi" (kWeft = kWarp = Y t,
+    kBias = Y t / (2 (1 + nu))); bending_kind: "bending", "bending_quadratic" or "bending_bw";
+    edge_springs: None, or "spring_continuum" / "spring_squared_continuum" over the mesh edges (stiffness
+    Y t, rest length from the rest positions), added after the membrane."""
     x0, faces = grid_cloth_mesh(n)
     rng = SplitMix64(seed)
     x = x0 + (2.0 * rng.uniforms(3 * n * n).reshape(-1, 3) - 1.0) * noise
@@ -121,11 +130,21 @@ def cloth(n, young, poisson=0.3, thickness=1e-3, bending=1e-4, mass=1e-3, pin="c
         pins = np.arange((n - 1) * n, n * n, dtype=np.int32)
     else:
         raise ValueError(pin)
+    yt = young * thickness
+    if membrane == "stvk":
+        mem = ("stvk", faces, np.tile([yt, poisson], (len(faces), 1)))
+    else:
+        mem = (membrane, faces, np.tile([yt, yt, yt / (2 * (1 + poisson))], (len(faces), 1)))
     energies = [
-        ("bending", hinges, np.column_stack([np.zeros(len(hinges)), np.full(len(hinges), bending)])),
-        ("stvk", faces, np.tile([young * thickness, poisson], (len(faces), 1))),
-        ("anchor", pins[:, None], np.column_stack([x[pins], np.full(len(pins), anchor_k)])),
+        (bending_kind, hinges, np.column_stack([np.zeros(len(hinges)), np.full(len(hinges), bending)])),
+        mem,
     ]
+    if edge_springs:
+        edges = np.unique(np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1),
+                          axis=0).astype(np.int32)
+        L0 = np.linalg.norm(x0[edges[:, 0]] - x0[edges[:, 1]], axis=1)
+        energies.append((edge_springs, edges, np.column_stack([L0, np.full(len(edges), yt)])))
+    energies.append(("anchor", pins[:, None], np.column_stack([x[pins], np.full(len(pins), anchor_k)])))
     return Scene(x=x, x0=x0, m=m, energies=energies, iterations=iterations,
                  meta={"kind": "cloth", "n": n, "young": young, "bending": bending, "seed": seed,
                        "triangles": len(faces), "hinges": len(hinges), "pins": len(pins)})
@@ -171,7 +190,9 @@ def kuhn_cube_mesh(cells, side=1.0):
     return x0, tets
 
 
-def cube(cells, young=1e5, poisson=0.45, density=1000.0, anchor_k=1e4, stretch_y=1.05, iterations=20, side=1.0):
+def cube(cells, young=1e5, poisson=0.45, density=1000.0, anchor_k=1e4, stretch_y=1.05, iterations=20, side=1.0,
+         material="snh"):
+    """material: "snh" (StableNeoHookeanEnergy), "stvk_tet" (StvkEnergy) or "ssnh" (StableSquaredNeoHookeanEnergy)."""
     x0, tets = kuhn_cube_mesh(cells, side)
     p = x0[tets]
     vol = np.einsum("ij,ij->i", p[:, 1] - p[:, 0], np.cross(p[:, 2] - p[:, 0], p[:, 3] - p[:, 0])) / 6.0
@@ -182,7 +203,7 @@ def cube(cells, young=1e5, poisson=0.45, density=1000.0, anchor_k=1e4, stretch_y
     pins = np.flatnonzero(x0[:, 1] == 0.0).astype(np.int32)
     energies = [
         ("anchor", pins[:, None], np.column_stack([x[pins], np.full(len(pins), anchor_k)])),
-        ("snh", tets, np.tile([young, poisson], (len(tets), 1))),
+        (material, tets, np.tile([young, poisson], (len(tets), 1))),
     ]
     return Scene(x=x, x0=x0, m=m, energies=energies, iterations=iterations,
                  meta={"kind": "cube", "cells": cells, "tets": len(tets), "pins": len(pins)})

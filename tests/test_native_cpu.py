@@ -290,3 +290,67 @@ def test_xpbd_closed_form_guards_on_host():
         xp = _xpbd_layout(kind, p, T._compliance(kind, 1e-3))
         d = native.xpbd_project_stencils(kind, xs.ravel()[None], w[None], xp[None], dt, 0, -1)[0]
         assert not np.any(d[0]), kind
+
+
+def _element_cases():
+    import test_energies_oracle as T
+    out = []
+    for kind in T.KINDS:
+        p = T.prepared(kind)
+        for seed, scale in ((0, 0.03), (1, 0.3)):
+            out.append((kind, T.deformed(kind, scale=scale, seed=seed), p))
+    return out
+
+
+ELEMENTS = _element_cases()
+
+
+@pytest.mark.parametrize("case", range(len(ELEMENTS)))
+def test_implicit_closed_forms_on_host_match_oracle(case):
+    """Full element gradients (implicit_element.h closed forms, host instantiation) and their forward-mode
+    Hessians against the oracle's second-order dual numbers, for the further energies: gradient 1e-12 and
+    Hessian 1e-10 relative to their max-norms."""
+    kind, x, p = ELEMENTS[case]
+    g, h = native.implicit_element_derivatives(kind, x.ravel()[None], p[None], device=-1)
+    _, gr, _, hr = ol.element(kind, x.ravel(), p, mode=ol.AD, full_hess=True)
+    assert np.abs(g[0] - gr).max() <= 1e-12 * np.abs(gr).max(), kind
+    assert np.abs(h[0] - hr).max() <= 1e-10 * np.abs(hr).max(), kind
+
+
+@pytest.mark.parametrize("kind", ["spring_continuum", "spring_squared_continuum", "membrane_bw", "membrane_choi",
+                                  "bending_quadratic", "bending_bw", "stvk_tet", "ssnh"])
+def test_mirror_further_energies_prepare_like_oracle(kind):
+    """The C++ mirror's addStencil + preparePrecomputations of the further energies give the oracle's prepared
+    parameters (spg_vbd_energy layout) and its compliance diagonal (spg_xpbd_energy layout) exactly."""
+    import test_energies_oracle as T
+    x0 = T.REST[kind]
+    pg = native.ParticleGroup()
+    pg.add_particles(x0, x0, np.ones(len(x0)))
+    pg.add_energy(kind, np.arange(len(x0))[None], np.asarray(T.RAW[kind])[None], prepare=True)
+    po = T.prepared(kind)
+    assert np.array_equal(pg.energy_params(0)[0], po)
+    o = ol.Oracle(x0, x0, np.ones(len(x0)))
+    o.add_energy(kind, [np.arange(len(x0))], [T.RAW[kind]])
+    o.prepare()
+    nc = ol.num_constraints(kind)
+    cm = o.compliance(0)[0].reshape(nc, nc)
+    xp = pg.xpbd_params(0)[0]
+    t = native.KIND[kind]
+    n_geo = native.XPBD_PARAMS[t] - nc if t not in (native.QUADRATIC_BENDING,) else 4
+    assert np.array_equal(xp[:n_geo], po[:n_geo])
+    diag = xp[n_geo:]
+    assert np.allclose(diag, np.diag(cm)[:len(diag)], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("kind,raw,msg", [
+    ("spring_continuum", [0.0, 1.0], "Zero/negative rest length invalid"),
+    ("bending_quadratic", [0.2, 1.0], "Rest theta != 0 not supported for quadratic bending"),
+    ("bending_bw", [-3.2, 1.0], "Invalid restTheta"),
+])
+def test_mirror_further_energies_validation(kind, raw, msg):
+    import test_energies_oracle as T
+    x0 = T.REST[kind]
+    pg = native.ParticleGroup()
+    pg.add_particles(x0, x0, np.ones(len(x0)))
+    with pytest.raises(native.NativeError, match=msg):
+        pg.add_energy(kind, np.arange(len(x0))[None], np.asarray(raw)[None], prepare=True)

@@ -417,8 +417,8 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
     struct Src {
         int32_t object, energy;
     };
-    std::vector<Src> srcs[6];
-    int64_t count[6] = {0, 0, 0, 0, 0, 0};
+    std::vector<Src> srcs[kNumTypes + 1];
+    int64_t count[kNumTypes + 1] = {};
     for (int32_t o = 0; o < nObjects; ++o)
         for (int32_t e = 0; e < objs[o].n_energies; ++e) {
             const auto &en = objs[o].energies[e];
@@ -426,7 +426,6 @@ void buildContext(spg_implicit_ctx &c, const spg_vbd_object *objs, int32_t nObje
             count[en.type] += en.n_stencils;
         }
     int64_t gradTotal = 0, hessTotal = 0;
-    std::vector<int64_t> firstElem(6, 0);  // element index of each (object, energy) in its type table
     std::vector<std::vector<int64_t>> elemBase(nObjects);
     for (int32_t o = 0; o < nObjects; ++o) elemBase[o].assign(objs[o].n_energies, 0);
     for (int t = 1; t <= kNumTypes; ++t) {

@@ -18,6 +18,7 @@
 #include "spg_vbd_learned.h"
 #include "element_eval.h"
 #include "vbd_device.h"
+#include "implicit_element.h"
 #include "host_util.h"
 #include "mlp_device.h"
 
@@ -27,27 +28,15 @@ namespace {
 
 thread_local std::string g_lastError;
 
+// energy types 1 ... 5 have dedicated incidence classes; 6 ... 13 (the further energies) share one
+bool isFurther(int type) { return type >= SPG_VBD_SPRING_CONTINUUM && type <= SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN; }
 int stencilSizeOf(int type)
 {
-    switch (type) {
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 3;
-    case SPG_VBD_DISCRETE_BENDING: return 4;
-    case SPG_VBD_STABLE_NEO_HOOKEAN: return 4;
-    case SPG_VBD_SPRING_ANCHOR: return 1;
-    }
-    return 0;
+    return type >= SPG_VBD_SPRING && type <= SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN ? fullStencilSize(type) : 0;
 }
 int paramsOf(int type)
 {
-    switch (type) {
-    case SPG_VBD_SPRING: return 2;
-    case SPG_VBD_MEMBRANE_STVK: return 13;
-    case SPG_VBD_DISCRETE_BENDING: return 2;
-    case SPG_VBD_STABLE_NEO_HOOKEAN: return 12;
-    case SPG_VBD_SPRING_ANCHOR: return 4;
-    }
-    return 0;
+    return type >= SPG_VBD_SPRING && type <= SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN ? fullParams(type) : 0;
 }
 // compulsory bytes of one element record read per incidence (roofline model)
 double recordBytes(int type)
@@ -59,7 +48,8 @@ double recordBytes(int type)
     case SPG_VBD_STABLE_NEO_HOOKEAN: return 64;
     case SPG_VBD_SPRING_ANCHOR: return 20;
     }
-    return 0;
+    // further energies: four slots, the padded parameters, and 4 B more per incidence record
+    return isFurther(type) ? 16.0 + 8.0 * kExtParams + 4.0 : 0.0;
 }
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
@@ -138,6 +128,11 @@ struct spg_vbd_ctx {
     DevBuf<float2> hinEP, hinWP;
     DevBuf<int4> tetO;
     DevBuf<float4> tetGa, tetGb, tetGc;
+    // further energies: CSR rows per slot, elements (slots, padded FP64 parameters)
+    DevBuf<int32_t> extPtr;
+    DevBuf<int2> extInc;
+    DevBuf<int4> extV;
+    DevBuf<double> extP;
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int2> sprV;
@@ -232,6 +227,10 @@ struct spg_vbd_ctx {
         a.tetGa = tetGa.p;
         a.tetGb = tetGb.p;
         a.tetGc = tetGc.p;
+        a.extPtr = extPtr.p;
+        a.extInc = extInc.p;
+        a.extV = extV.p;
+        a.extP = extP.p;
         a.counters = counters.p;
         a.begin = colorBegin[color];
         a.end = colorBegin[color + 1];
@@ -370,6 +369,10 @@ struct spg_vbd_ctx {
         a.tetElem = tetElem.p;
         a.tetV = tetV.p;
         a.tetP = tetP.p;
+        a.extPtr = extPtr.p;
+        a.extInc = extInc.p;
+        a.extV = extV.p;
+        a.extP = extP.p;
         a.order = objSlots.p;
         a.r2 = r2Buf;
         hipCheck(launchResidual(a, s), "residual launch");
@@ -819,7 +822,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                                        "membrane stiffness matrix is not of the StVK form [[a,b,0],[b,a,0],[0,0,c]]");
                 }
             }
-            if (en.type == SPG_VBD_SPRING)
+            if (en.type == SPG_VBD_SPRING || en.type == SPG_VBD_SPRING_CONTINUUM ||
+                en.type == SPG_VBD_SPRING_SQUARED_CONTINUUM)
                 for (int64_t i = 0; i < en.n_stencils; ++i)
                     if (en.params[i * np] <= 0) throw VbdError(SPG_VBD_ERR_INVALID, "Zero/negative rest length invalid");
             sets.push_back({en.stencils, en.n_stencils * ss, ss});
@@ -910,6 +914,9 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     std::vector<int4> triV, hinV, tetV;
     std::vector<double> triP, tetP;
     std::vector<double2> hinP;
+    std::vector<Incidence> extIncList;  // local = type << 2 | slot in the stencil
+    std::vector<int4> extVh;
+    std::vector<double> extPh;
     std::vector<double> colorRecordBytes(c.maxColors, 0.0);
     std::vector<int64_t> colorIncidences(c.maxColors, 0);
     for (int32_t o = 0; o < nObjects; ++o) {
@@ -949,9 +956,17 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                     tetV.push_back(make_int4(sl[0], sl[1], sl[2], sl[3]));
                     tetP.insert(tetP.end(), p, p + 12);
                     break;
+                default:  // a further energy
+                    elem = static_cast<int32_t>(extVh.size());
+                    extVh.push_back(make_int4(sl[0], ss > 1 ? sl[1] : -1, ss > 2 ? sl[2] : -1, ss > 3 ? sl[3] : -1));
+                    extPh.insert(extPh.end(), p, p + np);
+                    extPh.insert(extPh.end(), kExtParams - np, 0.0);
                 }
                 for (int k = 0; k < ss; ++k) {
-                    inc[en.type].push_back({sl[k], elem, k});
+                    if (isFurther(en.type))
+                        extIncList.push_back({sl[k], elem, (en.type << 2) | k});
+                    else
+                        inc[en.type].push_back({sl[k], elem, k});
                     const int32_t col = c.localColors[base + st[k]];
                     colorRecordBytes[col] += recordBytes(en.type) + 4.0;
                     colorIncidences[col] += 1;
@@ -1108,6 +1123,21 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.tetElem.upload(el, c.stream);
         c.tetV.upload(tetV, c.stream);
         c.tetP.upload(tetP, c.stream);
+    }
+
+    // further energies: one CSR incidence class (energy order, ascending stencil within each slot)
+    if (!extIncList.empty()) {
+        checkSize(extIncList.size());
+        if (extVh.size() >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many elements");
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(extIncList, c.nv, ptr);
+        std::vector<int2> rec(order.size());
+        for (size_t k = 0; k < order.size(); ++k)
+            rec[k] = make_int2(extIncList[order[k]].elem, extIncList[order[k]].local);
+        c.extPtr.upload(ptr, c.stream);
+        c.extInc.upload(rec, c.stream);
+        c.extV.upload(extVh, c.stream);
+        c.extP.upload(extPh, c.stream);
     }
 
     // compulsory bytes per colour sweep: 12 B per vertex of every object that has this colour (all

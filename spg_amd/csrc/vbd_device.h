@@ -63,10 +63,17 @@ struct SolveArgs {
     const float4 *tetGa;
     const float4 *tetGb;
     const float4 *tetGc;
+    // the further energies (types 6 ... 13): CSR rows per slot of (element, type << 2 | slot in the stencil),
+    // elements as up to four slots (unused -1) and kExtParams prepared FP64 parameters
+    const int32_t *extPtr;  // null: none (the solve kernel's instantiation without this class runs)
+    const int2 *extInc;
+    const int4 *extV;
+    const double *extP;
     unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
     int32_t begin, end;            // slot range of the colour
     float invdt2;
 };
+constexpr int kExtParams = 12;  // parameters stored per further-energy element (the largest needs 11)
 
 struct ResidualArgs {
     const float4 *pos;
@@ -96,6 +103,10 @@ struct ResidualArgs {
     const int32_t *tetElem;
     const int4 *tetV;
     const double *tetP;   // 12 per element: Dinv (9), lambda, mu, alpha
+    const int32_t *extPtr;  // the further energies, as in SolveArgs (null: none)
+    const int2 *extInc;
+    const int4 *extV;
+    const double *extP;
     const int32_t *order; // slot of the i-th particle (objects in order, particles ascending)
     double *r2;           // out: |r_v|^2 of the i-th particle (particle order)
 };

@@ -4,6 +4,7 @@
 
 #include "element.h"
 #include "element_eval.h"
+#include "implicit_element.h"
 #include "vbd_device.h"
 
 namespace spgvbd {
@@ -164,7 +165,70 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
 // LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
 // contiguous span per record array) into LDS with coalesced loads, then every lane reads its records
 // from LDS; positions are gathered from global memory as in the direct variant.
-template <int G, bool LDS>
+// A further energy's gradient w.r.t. its stencil slot `local` and, with HESS, the slot's 3x3 Hessian block:
+// the closed-form full gradient (implicit_element.h) at the element's positions, then one forward-mode pass
+// per coordinate of the slot gives the block's columns (symmetrised). T: float (solve) or double (residual).
+template <int NS, class T, bool HESS>
+__device__ __forceinline__ void extSlot(int type, int local, const float4 *__restrict__ pos, int4 v,
+                                        const double *__restrict__ prm, V3<T> &g, Sym3<T> &H)
+{
+    using D = ad::DualT<T>;
+    double p[kExtParams];
+#pragma unroll
+    for (int k = 0; k < kExtParams; ++k) p[k] = prm[k];
+    const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+    V3<T> xs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        const float4 q = pos[vv[s]];
+        xs[s] = {T(q.x), T(q.y), T(q.z)};
+    }
+    if constexpr (!HESS) {
+        V3<T> gd[NS];
+        fullGrad<NS, T>(type, xs, p, gd);
+        for (int s = 0; s < NS; ++s)
+            if (s == local) g = gd[s];
+        return;
+    } else {
+        T J[3][3];
+#pragma unroll 1
+        for (int d = 0; d < 3; ++d) {
+            V3<D> x[NS], gd[NS];
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const bool own = s == local;
+                x[s] = {D(xs[s].x, own && d == 0 ? T(1) : T(0)), D(xs[s].y, own && d == 1 ? T(1) : T(0)),
+                        D(xs[s].z, own && d == 2 ? T(1) : T(0))};
+            }
+            fullGrad<NS, D>(type, x, p, gd);
+            V3<D> gl = gd[0];
+#pragma unroll
+            for (int s = 1; s < NS; ++s)
+                if (s == local) gl = gd[s];
+            if (d == 0) g = {gl.x.v, gl.y.v, gl.z.v};
+            J[0][d] = gl.x.d;
+            J[1][d] = gl.y.d;
+            J[2][d] = gl.z.d;
+        }
+        const T h = T(0.5);
+        H = {J[0][0], h * (J[0][1] + J[1][0]), h * (J[0][2] + J[2][0]), J[1][1], h * (J[1][2] + J[2][1]), J[2][2]};
+    }
+}
+template <class T, bool HESS>
+__device__ __forceinline__ void extIncidence(int2 in, const float4 *__restrict__ pos, const int4 *__restrict__ extV,
+                                             const double *__restrict__ extP, V3<T> &g, Sym3<T> &H)
+{
+    const int type = in.y >> 2, local = in.y & 3;
+    const int4 v = extV[in.x];
+    const double *p = extP + static_cast<int64_t>(kExtParams) * in.x;
+    switch (fullStencilSize(type)) {
+    case 2: extSlot<2, T, HESS>(type, local, pos, v, p, g, H); break;
+    case 3: extSlot<3, T, HESS>(type, local, pos, v, p, g, H); break;
+    default: extSlot<4, T, HESS>(type, local, pos, v, p, g, H);
+    }
+}
+
+template <int G, bool LDS, bool EXT>
 __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
 {
     constexpr int GPW = 64 / G;  // groups per wavefront
@@ -273,6 +337,14 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             anchorVertex<float>(xo, xyz(s), s.w, ge, He);
             acc(g, H, ge, He);
         }
+        if constexpr (EXT) {
+            for (int32_t k = a.extPtr[vid] + lig; k < a.extPtr[vid + 1]; k += G) {
+                V3f ge;
+                S3f He;
+                extIncidence<float, true>(a.extInc[k], a.pos, a.extV, a.extP, ge, He);
+                acc(g, H, ge, He);
+            }
+        }
     }
     if (G > 1) {
         g.x = groupReduce<G>(g.x, lane64, lig);
@@ -321,6 +393,7 @@ __device__ __forceinline__ double groupReduceD(double v, int32_t lane64, int32_t
 // incidence class, and the group sums its partial forces in a fixed order.
 constexpr int kResidualLanes = 8;
 
+template <bool EXT>
 __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
 {
     constexpr int G = kResidualLanes, GPW = 64 / G;
@@ -412,6 +485,12 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
                 const int32_t el = a.ancElem[k];
                 const double4 p = a.ancP[el];
                 anchorVertex<double>(xo, V3d{p.x, p.y, p.z}, p.w, ge, He);
+                r = r - ge;
+            }
+        }
+        if constexpr (EXT) {
+            for (int32_t k = a.extPtr[vid] + lig; k < a.extPtr[vid + 1]; k += G) {
+                extIncidence<double, false>(a.extInc[k], a.pos, a.extV, a.extP, ge, He);
                 r = r - ge;
             }
         }
@@ -507,14 +586,26 @@ hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t st
     return hipGetLastError();
 }
 
+template <int G, bool LDS, bool EXT>
+void solveColorLaunchE(const SolveArgs &a, int grid, int blockSize, size_t lds, hipStream_t stream, hipEvent_t start,
+                       hipEvent_t stop)
+{
+    if (start)  // timestamps of the dispatch itself (no extra packets between the events and the kernel)
+        hipExtLaunchKernelGGL(solveColorKernel<G, LDS, EXT>, dim3(grid), dim3(blockSize), lds, stream, start, stop, 0,
+                              a);
+    else
+        solveColorKernel<G, LDS, EXT><<<grid, blockSize, lds, stream>>>(a);
+}
+// the kernel with the further energies' incidence class is used only when such energies exist, so the
+// five-energy path keeps its registers
 template <int G, bool LDS>
 void solveColorLaunch(const SolveArgs &a, int grid, int blockSize, size_t lds, hipStream_t stream, hipEvent_t start,
                       hipEvent_t stop)
 {
-    if (start)  // timestamps of the dispatch itself (no extra packets between the events and the kernel)
-        hipExtLaunchKernelGGL(solveColorKernel<G, LDS>, dim3(grid), dim3(blockSize), lds, stream, start, stop, 0, a);
+    if (a.extPtr)
+        solveColorLaunchE<G, LDS, true>(a, grid, blockSize, lds, stream, start, stop);
     else
-        solveColorKernel<G, LDS><<<grid, blockSize, lds, stream>>>(a);
+        solveColorLaunchE<G, LDS, false>(a, grid, blockSize, lds, stream, start, stop);
 }
 
 template <int G>
@@ -560,7 +651,10 @@ size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
 {
     if (a.n <= 0) return hipSuccess;
-    residualKernel<<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
+    if (a.extPtr)
+        residualKernel<true><<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
+    else
+        residualKernel<false><<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
     return hipGetLastError();
 }
 

@@ -114,14 +114,16 @@ def grid_cloth_mesh(n):
 
 
 def cloth(n, young, poisson=0.3, thickness=1e-3, bending=1e-4, mass=1e-3, pin="corners", noise=1e-4, seed=1,
-          anchor_k=1e5, iterations=10, membrane="stvk", bending_kind="bending", edge_springs=None):
+          anchor_k=1e5, iterations=10, membrane="stvk", bending_kind="bending", edge_springs=None, edge_k=None,
+          stretch=1.0):
     """membrane: "stvk" (MembraneStvkEnergy), "membrane_bw" or "membrane_choi" (kWeft = kWarp = Y t,
     kBias = Y t / (2 (1 + nu))); bending_kind: "bending", "bending_quadratic" or "bending_bw";
     edge_springs: None, or "spring_continuum" / "spring_squared_continuum" over the mesh edges (stiffness
-    Y t, rest length from the rest positions), added after the membrane."""
+    edge_k, default Y t; rest length from the rest positions), added after the membrane; stretch: the initial
+    positions are the rest positions scaled by this factor in the cloth's plane (pre-strain), plus the noise."""
     x0, faces = grid_cloth_mesh(n)
     rng = SplitMix64(seed)
-    x = x0 + (2.0 * rng.uniforms(3 * n * n).reshape(-1, 3) - 1.0) * noise
+    x = x0 * np.array([stretch, stretch, 1.0]) + (2.0 * rng.uniforms(3 * n * n).reshape(-1, 3) - 1.0) * noise
     m = np.full(n * n, mass)
     hinges = native.trimesh_bending_stencils(n * n, faces)
     if pin == "corners":
@@ -143,7 +145,7 @@ def cloth(n, young, poisson=0.3, thickness=1e-3, bending=1e-4, mass=1e-3, pin="c
         edges = np.unique(np.sort(np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]]), axis=1),
                           axis=0).astype(np.int32)
         L0 = np.linalg.norm(x0[edges[:, 0]] - x0[edges[:, 1]], axis=1)
-        energies.append((edge_springs, edges, np.column_stack([L0, np.full(len(edges), yt)])))
+        energies.append((edge_springs, edges, np.column_stack([L0, np.full(len(edges), yt if edge_k is None else edge_k)])))
     energies.append(("anchor", pins[:, None], np.column_stack([x[pins], np.full(len(pins), anchor_k)])))
     return Scene(x=x, x0=x0, m=m, energies=energies, iterations=iterations,
                  meta={"kind": "cloth", "n": n, "young": young, "bending": bending, "seed": seed,

@@ -48,6 +48,15 @@ extern "C" {
  *   STABLE_NEO_HOOKEAN 14: inverseReferenceMats()[i] (3x3), effectiveCompliance()[i] (2x2, symmetric),
  *                          hydrostaticCorrectionTerms()[i]
  *   SPRING_ANCHOR       4: anchors()[i] (3), effectiveCompliance()[i](0,0)
+ * and the further energies (their compliance matrices are diagonal; the diagonal is passed):
+ *   SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM 2: restLengths()[i], effectiveCompliance()[i](0,0)
+ *   MEMBRANE_BARAFF_WITKIN 7: inverseReferenceMats()[i] (2x2), effectiveCompliance()[i] diagonal (3)
+ *   MEMBRANE_CHOI      8: inverseReferenceMats()[i] (2x2), effectiveCompliance()[i] diagonal (4)
+ *   QUADRATIC_BENDING  5: laplacians()[i] coefficients c0..c3, effectiveCompliance()[i](0,0) (it is c I)
+ *   BARAFF_WITKIN_BENDING 2: restThetas()[i], effectiveCompliance()[i](0,0)
+ *   STVK, STABLE_SQUARED_NEO_HOOKEAN 11: inverseReferenceMats()[i] (3x3), effectiveCompliance()[i] diagonal (2)
+ * The further energies are projected by the generic coupled projection (closed-form constraint gradients,
+ * a Cholesky solve of the nC x nC system), FP32 with the same FP64 recomputation rule.
  * `stencils` is Energy::flatStencils(). A stencil may not name one particle twice.
  */
 typedef struct {
@@ -79,13 +88,14 @@ typedef struct {
     double projection_ms;        /* summed duration of those launches (HIP events on the ctx stream) */
     int64_t stencil_projections; /* stencils projected by those launches */
     double algorithmic_bytes;    /* compulsory bytes of those launches: per stencil its record (spring 16,
-                                    membrane 56, bending 24, SNH 68, anchor 20 B) + 32 B per particle
+                                    membrane 56, bending 24, SNH 68, anchor 20 B, further energies 64 B:
+                                    4 slots + 12 parameters) + 32 B per particle
                                     (float4 read and written) */
     int64_t fp64_solves;         /* projections recomputed in FP64 because A was ill-conditioned */
     /* the same launches split by energy type (index = SPG_VBD_* type id, entry 0 unused) */
-    int64_t type_launches[6];
-    double type_ms[6];
-    double type_bytes[6];
+    int64_t type_launches[14];
+    double type_ms[14];
+    double type_bytes[14];
 } spg_xpbd_stats;
 
 typedef struct spg_xpbd_ctx spg_xpbd_ctx;

@@ -35,7 +35,7 @@ int stencilSizeOf(int type)
     case SPG_VBD_STABLE_NEO_HOOKEAN: return 4;
     case SPG_VBD_SPRING_ANCHOR: return 1;
     }
-    return 0;
+    return type >= SPG_VBD_SPRING_CONTINUUM && type < kXpbdNumTypes ? xpbdFurtherStencil(type) : 0;
 }
 int paramsOf(int type)
 {
@@ -45,6 +45,14 @@ int paramsOf(int type)
     case SPG_VBD_DISCRETE_BENDING: return 2;
     case SPG_VBD_STABLE_NEO_HOOKEAN: return 14;
     case SPG_VBD_SPRING_ANCHOR: return 4;
+    case SPG_VBD_SPRING_CONTINUUM:
+    case SPG_VBD_SPRING_SQUARED_CONTINUUM:
+    case SPG_VBD_BARAFF_WITKIN_BENDING: return 2;
+    case SPG_VBD_MEMBRANE_BARAFF_WITKIN: return 7;
+    case SPG_VBD_MEMBRANE_CHOI: return 8;
+    case SPG_VBD_QUADRATIC_BENDING: return 5;
+    case SPG_VBD_STVK:
+    case SPG_VBD_STABLE_SQUARED_NEO_HOOKEAN: return 11;
     }
     return 0;
 }
@@ -93,6 +101,8 @@ struct spg_xpbd_ctx {
     DevBuf<int4> tetI;
     DevBuf<float4> tetD0, tetD1, tetD2;
     DevBuf<float> tetH;
+    DevBuf<int4> extI;    // further energies, records of types 6 ... 13 one after the other
+    DevBuf<float4> extP;  // 3 per record
     DevBuf<unsigned long long> counters;
 
     struct Launch {
@@ -150,6 +160,8 @@ struct spg_xpbd_ctx {
         a.tetD1 = tetD1.p;
         a.tetD2 = tetD2.p;
         a.tetH = tetH.p;
+        a.extI = extI.p;
+        a.extP = extP.p;
         a.counters = counters.p;
         a.invdt2 = invdt2;
         return a;
@@ -316,6 +328,17 @@ void validateEnergy(const spg_xpbd_energy &e, int64_t np)
             if (!symmetric(p + 9, 2))
                 throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "non-symmetric SNH compliance is not supported");
             break;
+        default: {  // further energies: finite geometry, compliance diagonal >= 0 (may be +inf)
+            const int nc = e.type == SPG_VBD_QUADRATIC_BENDING ? 1 : xpbdFurtherConstraints(e.type);
+            for (int k = 0; k < pp - nc; ++k)
+                if (!std::isfinite(p[k])) throw VbdError(SPG_VBD_ERR_INVALID, "invalid energy data");
+            for (int k = pp - nc; k < pp; ++k)
+                if (!complianceOk(p[k])) throw VbdError(SPG_VBD_ERR_INVALID, "invalid compliance");
+            if ((e.type == SPG_VBD_SPRING_CONTINUUM || e.type == SPG_VBD_SPRING_SQUARED_CONTINUUM) && p[0] <= 0)
+                throw VbdError(SPG_VBD_ERR_INVALID, "Zero/negative rest length invalid");
+            if (e.type == SPG_VBD_BARAFF_WITKIN_BENDING && (p[0] < -3.12 || p[0] > 3.12))
+                throw VbdError(SPG_VBD_ERR_INVALID, "Invalid restTheta");
+        }
         }
     }
 }
@@ -355,7 +378,7 @@ void buildContext(spg_xpbd_ctx &c, const spg_xpbd_object *objs, int32_t nObjects
     }
 
     // groups per energy (computeParallelStencilGroups, or the serial levels), and the launch order
-    std::vector<Entry> ent[6];
+    std::vector<Entry> ent[kXpbdNumTypes];
     c.groups.assign(nObjects, {});
     c.numGroups.assign(nObjects, {});
     for (int32_t o = 0; o < nObjects; ++o) {
@@ -379,15 +402,20 @@ void buildContext(spg_xpbd_ctx &c, const spg_xpbd_object *objs, int32_t nObjects
         int32_t energy, group, type, begin, end;
     };
     std::vector<Range> ranges;
-    for (int t = 1; t <= 5; ++t) {
+    // the further energies' records follow each other in one table: type t starts at extBase[t]
+    std::array<int32_t, kXpbdNumTypes + 1> extBase{};
+    for (int t = SPG_VBD_SPRING_CONTINUUM; t < kXpbdNumTypes; ++t)
+        extBase[t + 1] = extBase[t] + static_cast<int32_t>(ent[t].size());
+    for (int t = 1; t < kXpbdNumTypes; ++t) {
         auto &v = ent[t];
+        const int32_t off = t >= SPG_VBD_SPRING_CONTINUUM ? extBase[t] : 0;
         std::sort(v.begin(), v.end(), [](const Entry &a, const Entry &b) {
             return std::tie(a.energy, a.group, a.object, a.stencil) < std::tie(b.energy, b.group, b.object, b.stencil);
         });
         for (size_t k = 0; k < v.size();) {
             size_t j = k;
             while (j < v.size() && v[j].energy == v[k].energy && v[j].group == v[k].group) ++j;
-            ranges.push_back({v[k].energy, v[k].group, t, static_cast<int32_t>(k), static_cast<int32_t>(j)});
+            ranges.push_back({v[k].energy, v[k].group, t, off + static_cast<int32_t>(k), off + static_cast<int32_t>(j)});
             k = j;
         }
     }
@@ -482,6 +510,23 @@ void buildContext(spg_xpbd_ctx &c, const spg_xpbd_object *objs, int32_t nObjects
         c.tetD1.upload(D1, c.stream);
         c.tetD2.upload(D2, c.stream);
         c.tetH.upload(H, c.stream);
+    }
+    {
+        std::vector<int4> I;
+        std::vector<float4> P;
+        for (int t = SPG_VBD_SPRING_CONTINUUM; t < kXpbdNumTypes; ++t) {
+            const int ss = stencilSizeOf(t), pp = paramsOf(t);
+            for (const auto &en : ent[t]) {
+                const double *p = prm(en);
+                I.push_back(make_int4(gidx(en, 0), ss > 1 ? gidx(en, 1) : -1, ss > 2 ? gidx(en, 2) : -1,
+                                      ss > 3 ? gidx(en, 3) : -1));
+                float q[12] = {};
+                for (int k = 0; k < pp; ++k) q[k] = float(p[k]);
+                for (int k = 0; k < 3; ++k) P.push_back(make_float4(q[4 * k], q[4 * k + 1], q[4 * k + 2], q[4 * k + 3]));
+            }
+        }
+        c.extI.upload(I, c.stream);
+        c.extP.upload(P, c.stream);
     }
     c.pos.upload(hp, c.stream);
     c.vel.upload(hv, c.stream);

@@ -38,15 +38,19 @@ struct XpbdArgs {
     const float4 *tetD1;  // D11 D12 D20 D21
     const float4 *tetD2;  // D22 c00 c01 c11
     const float *tetH;    // hydrostatic correction term
+    // further energies (types 6 ... 13): slots (unused -1) and 12 parameters in the spg_xpbd_energy layout
+    const int4 *extI;
+    const float4 *extP;  // 3 per record
     unsigned long long *counters;  // [0] projections solved in FP64 (ill-conditioned A)
     float invdt2;
 };
 
+constexpr int kXpbdNumTypes = 14;  // type ids 1 ... 13
 // compulsory bytes of one stencil record (see the layout above)
-constexpr double kXpbdRecordBytes[6] = {0, 16, 56, 24, 68, 20};
+constexpr double kXpbdRecordBytes[kXpbdNumTypes] = {0, 16, 56, 24, 68, 20, 64, 64, 64, 64, 64, 64, 64, 64};
 
 // One projection launch over records [begin, end) of `type` (1 spring, 2 membrane, 3 bending, 4 SNH,
-// 5 anchor). start/stop: optional events stamped with the dispatch's own begin/end.
+// 5 anchor, 6 ... 13 the further energies). start/stop: optional events stamped with the dispatch's own begin/end.
 hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t end, int blockSize,
                              hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 // Substep boundaries (xpbd.cpp:47-70,110-121): begin: xold = x, v += h g, x += h v;

@@ -223,9 +223,235 @@ SPG_HD void xpbdStore3(double *o, V3<R> v)
     o[2] = v.z;
 }
 
+// ---- The further energies: a generic coupled projection from closed-form constraint gradients ----
+// Parameters (spg_xpbd_energy layout): the geometry, then the compliance diagonal:
+//   6, 7 continuum springs (L0, c); 8 Baraff-Witkin membrane (Dinv 2x2, c0..c2); 9 Choi membrane (Dinv 2x2,
+//   c0..c3); 10 quadratic bending (c0..c3 of the Laplacian, compliance); 11 Baraff-Witkin bending (theta0, c);
+//   12 StVK tetrahedron, 13 stable squared Neo-Hookean (Dinv 3x3, c_lambda, c_mu).
+SPG_HD constexpr int xpbdFurtherConstraints(int type)
+{
+    return type == 8 ? 3 : type == 9 ? 4 : type == 10 ? 3 : (type == 12 || type == 13) ? 2 : 1;
+}
+SPG_HD constexpr int xpbdFurtherStencil(int type)
+{
+    return (type == 6 || type == 7) ? 2 : (type == 8 || type == 9) ? 3 : 4;
+}
+
+// Constraint values C[c] and gradients g[c][s] (w.r.t. stencil slot s) of one further-energy stencil.
+template <class R>
+SPG_HD void xpbdFurtherGradients(int type, const V3<R> *x, const R *p, R *C, V3<R> (*g)[4])
+{
+    const V3<R> z = v3<R>(R(0), R(0), R(0));
+    for (int c = 0; c < 4; ++c)
+        for (int s = 0; s < 4; ++s) g[c][s] = z;
+    if (type == 6 || type == 7) {  // springContinuumEnergy.cpp:9-18, springSquaredContinuumEnergy.cpp:9-19
+        const V3<R> d = x[0] - x[1];
+        const R L0 = p[0];
+        if (type == 6) {
+            const R n = sqrt(dot(d, d));
+            C[0] = n / L0 - R(1);
+            g[0][0] = (R(1) / (L0 * n)) * d;
+        } else {
+            C[0] = dot(d, d) / (L0 * L0) - R(1);
+            g[0][0] = (R(2) / (L0 * L0)) * d;
+        }
+        g[0][1] = -g[0][0];
+    } else if (type == 8 || type == 9) {  // membraneBaraffWitkinEnergy.cpp:9-26, membraneChoiEnergy.cpp:9-28
+        const V3<R> u = x[1] - x[0], v = x[2] - x[0];
+        const V3<R> f0 = p[0] * u + p[2] * v, f1 = p[1] * u + p[3] * v;
+        const R a[3] = {-(p[0] + p[2]), p[0], p[2]}, b[3] = {-(p[1] + p[3]), p[1], p[3]};
+        const R n0 = sqrt(dot(f0, f0)), n1 = sqrt(dot(f1, f1));
+        C[0] = n0 - R(1);
+        C[1] = n1 - R(1);
+        const V3<R> h0 = (R(1) / n0) * f0, h1 = (R(1) / n1) * f1;
+        for (int s = 0; s < 3; ++s) {
+            g[0][s] = a[s] * h0;
+            g[1][s] = b[s] * h1;
+        }
+        if (type == 8) {
+            C[2] = dot(f0, f1);
+            for (int s = 0; s < 3; ++s) g[2][s] = a[s] * f1 + b[s] * f0;
+        } else {
+            const R r = R(0.70710678118654752440);
+            const V3<R> gp = r * (f0 + f1), gm = r * (f0 - f1);
+            const R np = sqrt(dot(gp, gp)), nm = sqrt(dot(gm, gm));
+            C[2] = np - R(1);
+            C[3] = nm - R(1);
+            const V3<R> hp = (R(1) / np) * gp, hm = (R(1) / nm) * gm;
+            for (int s = 0; s < 3; ++s) {
+                g[2][s] = (r * (a[s] + b[s])) * hp;
+                g[3][s] = (r * (a[s] - b[s])) * hm;
+            }
+        }
+    } else if (type == 10) {  // quadraticBendingEnergy.cpp:9-30: C = L x, dC_d/dx_s = c_s e_d
+        const R cx = p[0] * x[0].x + p[1] * x[1].x + p[2] * x[2].x + p[3] * x[3].x;
+        const R cy = p[0] * x[0].y + p[1] * x[1].y + p[2] * x[2].y + p[3] * x[3].y;
+        const R cz = p[0] * x[0].z + p[1] * x[1].z + p[2] * x[2].z + p[3] * x[3].z;
+        C[0] = cx;
+        C[1] = cy;
+        C[2] = cz;
+        for (int s = 0; s < 4; ++s) {
+            g[0][s] = v3<R>(p[s], R(0), R(0));
+            g[1][s] = v3<R>(R(0), p[s], R(0));
+            g[2][s] = v3<R>(R(0), R(0), p[s]);
+        }
+    } else if (type == 11) {  // baraffWitkinBendingEnergy.cpp:9-36 with its normalisations held constant
+        const V3<R> e0 = x[1] - x[0], e3 = x[2] - x[1], e4 = x[3] - x[1];
+        const V3<R> n1 = cross(e0, e3), n2 = -cross(e0, e4);
+        const R s0 = R(1) / sqrt(dot(e0, e0));
+        const V3<R> m = cross(n1, n2);
+        const R X = dot(n1, n2), Y = s0 * dot(m, e0);
+        C[0] = atan2(Y, X) - p[0];
+        // grad(v . n1) and grad(v . n2) per slot, as for the energy's gradient
+        auto dN1 = [&](V3<R> vv, V3<R> *o) {
+            const V3<R> q = cross(e3, vv), t = cross(vv, e0);
+            o[0] = -q;
+            o[1] = q - t;
+            o[2] = t;
+            o[3] = z;
+        };
+        auto dN2 = [&](V3<R> vv, V3<R> *o) {
+            const V3<R> q = cross(e4, vv), t = cross(vv, e0);
+            o[0] = q;
+            o[1] = t - q;
+            o[2] = z;
+            o[3] = -t;
+        };
+        V3<R> dx1[4], dx2[4], dy1[4], dy2[4];
+        dN1(n2, dx1);
+        dN2(n1, dx2);
+        dN1(cross(n2, e0), dy1);
+        dN2(cross(e0, n1), dy2);
+        const R w = R(1) / (X * X + Y * Y);
+        for (int s = 0; s < 4; ++s) {
+            V3<R> gy = dy1[s] + dy2[s];
+            if (s == 0) gy = gy - m;
+            if (s == 1) gy = gy + m;
+            g[0][s] = w * (X * (s0 * gy) - Y * (dx1[s] + dx2[s]));
+        }
+    } else {  // 12 stvkEnergy.cpp:9-39, 13 stableSquaredNeoHookeanEnergy.cpp:9-31: F = [x1-x0 x2-x0 x3-x0] Dinv
+        const V3<R> u = x[1] - x[0], v = x[2] - x[0], ww = x[3] - x[0];
+        V3<R> f[3];
+        for (int c = 0; c < 3; ++c) f[c] = p[c] * u + p[3 + c] * v + p[6 + c] * ww;
+        V3<R> P[2][3];
+        if (type == 12) {
+            R E[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) E[i][j] = R(0.5) * (dot(f[i], f[j]) - R(i == j ? 1 : 0));
+            R ee = R(0);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) ee += E[i][j] * E[j][i];
+            C[0] = E[0][0] + E[1][1] + E[2][2];
+            C[1] = sqrt(R(2) * ee);
+            const R k = C[1] != R(0) ? R(2) / C[1] : R(0);  // the reference zeroes a vanishing deviatoric term
+            for (int c = 0; c < 3; ++c) {
+                P[0][c] = f[c];
+                P[1][c] = k * (E[0][c] * f[0] + E[1][c] * f[1] + E[2][c] * f[2]);
+            }
+        } else {
+            const V3<R> k0 = cross(f[1], f[2]), k1 = cross(f[2], f[0]), k2 = cross(f[0], f[1]);
+            C[0] = dot(f[0], k0) - R(1);
+            C[1] = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - R(3);
+            P[0][0] = k0;
+            P[0][1] = k1;
+            P[0][2] = k2;
+            for (int c = 0; c < 3; ++c) P[1][c] = R(2) * f[c];
+        }
+        for (int q = 0; q < 2; ++q) {
+            g[q][1] = p[0] * P[q][0] + p[1] * P[q][1] + p[2] * P[q][2];
+            g[q][2] = p[3] * P[q][0] + p[4] * P[q][1] + p[5] * P[q][2];
+            g[q][3] = p[6] * P[q][0] + p[7] * P[q][1] + p[8] * P[q][2];
+            g[q][0] = -(g[q][1] + g[q][2] + g[q][3]);
+        }
+    }
+}
+
+// A = sum_s w_s g_a . g_b + alpha~ + 1e-10 I by Cholesky, dlambda = -A^-1 C, dx_s = w_s sum_a g_a dlambda_a.
+// Returns det(A) / prod(diag A) (<= 0 when rounding made A indefinite).
+template <class R>
+SPG_HD R xpbdFurtherSolve(int nc, int ns, const R *C, const V3<R> (*g)[4], const R *w, const R *alphaTilde,
+                          V3<R> *dx)
+{
+    R A[4][4], L[4][4];
+    for (int a = 0; a < nc; ++a)
+        for (int b = 0; b < nc; ++b) {
+            R acc = R(0);
+            for (int s = 0; s < ns; ++s) acc += w[s] * dot(g[a][s], g[b][s]);
+            A[a][b] = acc + (a == b ? alphaTilde[a] + R(1e-10) : R(0));
+        }
+    R ratio = R(1);
+    for (int j = 0; j < nc; ++j) {
+        R d = A[j][j];
+        for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+        ratio *= d / A[j][j];
+        if (!(d > R(0))) {
+            for (int s = 0; s < ns; ++s) dx[s] = v3<R>(R(0), R(0), R(0));
+            return R(0);
+        }
+        L[j][j] = sqrt(d);
+        for (int i = j + 1; i < nc; ++i) {
+            R t = A[i][j];
+            for (int k = 0; k < j; ++k) t -= L[i][k] * L[j][k];
+            L[i][j] = t / L[j][j];
+        }
+    }
+    R y[4], dl[4];
+    for (int i = 0; i < nc; ++i) {
+        R t = -C[i];
+        for (int k = 0; k < i; ++k) t -= L[i][k] * y[k];
+        y[i] = t / L[i][i];
+    }
+    for (int i = nc - 1; i >= 0; --i) {
+        R t = y[i];
+        for (int k = i + 1; k < nc; ++k) t -= L[k][i] * dl[k];
+        dl[i] = t / L[i][i];
+    }
+    for (int s = 0; s < ns; ++s) {
+        V3<R> acc = v3<R>(R(0), R(0), R(0));
+        for (int a = 0; a < nc; ++a) acc = acc + dl[a] * g[a][s];
+        dx[s] = w[s] * acc;
+    }
+    return ratio;
+}
+
+template <class R>
+SPG_HD R xpbdFurther(int type, const double *x, const double *w, const double *p, double invdt2, double *dx)
+{
+    const int nc = xpbdFurtherConstraints(type), ns = xpbdFurtherStencil(type);
+    const int ng = type == 12 || type == 13 ? 9 : (type == 8 || type == 9 || type == 10) ? 4 : 1;
+    R pr[9], at[4], wr[4], C[4] = {R(0), R(0), R(0), R(0)};
+    V3<R> xr[4], g[4][4], d[4];
+    for (int k = 0; k < ng; ++k) pr[k] = R(p[k]);
+    for (int a = 0; a < nc; ++a) at[a] = R(p[ng + (type == 10 ? 0 : a)]) * R(invdt2);  // quadratic bending: c I
+    for (int s = 0; s < ns; ++s) {
+        xr[s] = v3<R>(R(x[3 * s]), R(x[3 * s + 1]), R(x[3 * s + 2]));
+        wr[s] = R(w[s]);
+    }
+    xpbdFurtherGradients<R>(type, xr, pr, C, g);
+    const R ratio = xpbdFurtherSolve<R>(nc, ns, C, g, wr, at, d);
+    for (int s = 0; s < ns; ++s) {
+        dx[3 * s] = double(d[s].x);
+        dx[3 * s + 1] = double(d[s].y);
+        dx[3 * s + 2] = double(d[s].z);
+    }
+    return ratio;
+}
+
+// FP32 with the FP64 recomputation when det(A) / prod(diag A) < kXpbdCondMin; returns whether it fell back.
+template <bool FP32>
+SPG_HD bool xpbdFurtherRecord(int type, const double *x, const double *w, const double *p, double invdt2, double *dx)
+{
+    if constexpr (FP32) {
+        if (xpbdFurther<float>(type, x, w, p, invdt2, dx) > kXpbdCondMin) return false;
+    }
+    (void)xpbdFurther<double>(type, x, w, p, invdt2, dx);
+    return FP32;
+}
+
 template <bool FP32>
 SPG_HD bool xpbdProjectRecord(int type, const double *x, const double *w, const double *p, double invdt2, double *dx)
 {
+    if (type >= 6) return xpbdFurtherRecord<FP32>(type, x, w, p, invdt2, dx);
     using R = typename std::conditional<FP32, float, double>::type;
     const R s = R(invdt2);
     bool fell = false;

@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t beg
         pos[id.y] = moved(X1, o1);
         pos[id.z] = moved(X2, o2);
         pos[id.w] = moved(X3, o3);
-    } else {  // kSnh
+    } else if constexpr (T == kSnh) {
         const int4 id = a.tetI[i];
         const float4 d0 = a.tetD0[i], d1 = a.tetD1[i], d2 = a.tetD2[i];
         const float hyd = a.tetH[i];
@@ -83,6 +83,32 @@ __global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t beg
         pos[id.y] = moved(X1, o1);
         pos[id.z] = moved(X2, o2);
         pos[id.w] = moved(X3, o3);
+    } else if constexpr (T >= 6) {
+        constexpr int NS = xpbdFurtherStencil(T);
+        const int4 id = a.extI[i];
+        const int32_t vv[4] = {id.x, id.y, id.z, id.w};
+        float4 X[4];
+        double xs[12], ws[4], ps[12], dx[12];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            X[s] = pos[vv[s]];
+            xs[3 * s] = X[s].x;
+            xs[3 * s + 1] = X[s].y;
+            xs[3 * s + 2] = X[s].z;
+            ws[s] = X[s].w;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float4 c = a.extP[3 * static_cast<int64_t>(i) + q];
+            ps[4 * q] = c.x;
+            ps[4 * q + 1] = c.y;
+            ps[4 * q + 2] = c.z;
+            ps[4 * q + 3] = c.w;
+        }
+        countFp64(xpbdFurtherRecord<true>(T, xs, ws, ps, a.invdt2, dx), a.counters);
+#pragma unroll
+        for (int s = 0; s < NS; ++s)
+            pos[vv[s]] = moved(X[s], V3f{float(dx[3 * s]), float(dx[3 * s + 1]), float(dx[3 * s + 2])});
     }
 }
 
@@ -178,6 +204,14 @@ hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t
     case kBending: projectLaunch<kBending>(a, begin, end, blockSize, stream, start, stop); break;
     case kSnh: projectLaunch<kSnh>(a, begin, end, blockSize, stream, start, stop); break;
     case kAnchor: projectLaunch<kAnchor>(a, begin, end, blockSize, stream, start, stop); break;
+    case 6: projectLaunch<6>(a, begin, end, blockSize, stream, start, stop); break;
+    case 7: projectLaunch<7>(a, begin, end, blockSize, stream, start, stop); break;
+    case 8: projectLaunch<8>(a, begin, end, blockSize, stream, start, stop); break;
+    case 9: projectLaunch<9>(a, begin, end, blockSize, stream, start, stop); break;
+    case 10: projectLaunch<10>(a, begin, end, blockSize, stream, start, stop); break;
+    case 11: projectLaunch<11>(a, begin, end, blockSize, stream, start, stop); break;
+    case 12: projectLaunch<12>(a, begin, end, blockSize, stream, start, stop); break;
+    case 13: projectLaunch<13>(a, begin, end, blockSize, stream, start, stop); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

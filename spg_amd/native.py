@@ -87,13 +87,12 @@ class XpbdLaunch(ctypes.Structure):
 class XpbdStats(ctypes.Structure):
     _fields_ = [("projection_launches", ctypes.c_int64), ("projection_ms", ctypes.c_double),
                 ("stencil_projections", ctypes.c_int64), ("algorithmic_bytes", ctypes.c_double),
-                ("fp64_solves", ctypes.c_int64), ("type_launches", ctypes.c_int64 * 6),
-                ("type_ms", ctypes.c_double * 6), ("type_bytes", ctypes.c_double * 6)]
+                ("fp64_solves", ctypes.c_int64), ("type_launches", ctypes.c_int64 * 14),
+                ("type_ms", ctypes.c_double * 14), ("type_bytes", ctypes.c_double * 14)]
 
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_[:5]}
-        names = {SPRING: "spring", MEMBRANE_STVK: "stvk", DISCRETE_BENDING: "bending", STABLE_NEO_HOOKEAN: "snh",
-                 SPRING_ANCHOR: "anchor"}
+        names = {t: k for k, t in KIND.items()}
         d["per_type"] = {names[t]: {"launches": self.type_launches[t], "ms": self.type_ms[t],
                                     "bytes": self.type_bytes[t]} for t in names if self.type_launches[t]}
         return d

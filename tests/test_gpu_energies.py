@@ -95,3 +95,43 @@ def test_further_energies_mirror_matches_capi():
         m.step()
         ctx.step(s.dt, 1, s.iterations, s.gravity)
     assert np.array_equal(m.state()[0], ctx.state()[0])
+
+
+# ---- XPBD on the further energies ----
+
+def test_xpbd_further_projections_on_gpu_match_oracle():
+    """The generic projection on the GPU (the kernels' device functions): FP64 within 1e-10 of the oracle's
+    displacement; FP32 with its FP64 recomputation within 1e-5 of the displacement plus 1e-6 of the positions."""
+    import test_native_cpu as N
+    dt = 1 / 60
+    for kind, x, p, cm, xp in N.FURTHER_XPBD:
+        w = np.linspace(0.5, 2.0, len(x))
+        ref = ol.xpbd_project(kind, x.ravel(), w, p, cm, dt)
+        scale = max(np.abs(ref).max(), 1e-300)
+        for prec, tol in ((1, 1e-10 * scale), (0, 1e-5 * scale + 1e-6 * np.abs(x).max())):
+            got = native.xpbd_project_stencils(kind, x.ravel()[None], w[None], xp[None], dt, precision=prec,
+                                               device=0)[0]
+            assert np.abs(got - ref).max() <= tol, (kind, prec)
+
+
+@pytest.mark.parametrize("parallel", [True, False])
+@pytest.mark.parametrize("name", ["cloth_bw", "cloth_choi", "cube_stvk", "cube_ssnh"])
+def test_xpbd_further_energies_step(name, parallel, oracle_threads):
+    """XPBD(true) (stencil colours) and XPBD(false) (serial order as dependency levels), 2 steps of 4
+    substeps: positions within 1e-5 relative of the oracle, velocities within 1e-5 of max|x| / h."""
+    s = SCENES[name]()
+    ctx = native.XpbdContext([s.xpbd_object()], parallel_gauss_seidel=parallel)
+    o = ol.from_scene(s)
+    x, v = ctx.state(0)
+    o.set_state(x, v)
+    for _ in range(2):
+        ctx.step(s.dt, 4, s.gravity)
+        o.xpbd_step(s.dt, 4, s.gravity, parallel=parallel)
+    xg, vg = ctx.state(0)
+    xo, vo = o.state()
+    ex = rel_err(xg, xo)
+    ev = float(np.max(np.abs(vg - vo)) / (np.max(np.abs(xo)) * 4 / s.dt))
+    st = ctx.stats()
+    print(f"{name} parallel={parallel}: position rel {ex:.2e}, velocity rel {ev:.2e}, FP64 solves "
+          f"{st['fp64_solves']}", flush=True)
+    assert ex < 1e-5 and ev < 1e-5

@@ -354,3 +354,42 @@ def test_mirror_further_energies_validation(kind, raw, msg):
     pg.add_particles(x0, x0, np.ones(len(x0)))
     with pytest.raises(native.NativeError, match=msg):
         pg.add_energy(kind, np.arange(len(x0))[None], np.asarray(raw)[None], prepare=True)
+
+
+def _further_xpbd_cases():
+    import test_energies_oracle as T
+    out = []
+    for kind in T.KINDS:
+        x0 = T.REST[kind]
+        o = ol.Oracle(x0, x0, np.ones(len(x0)))
+        o.add_energy(kind, [np.arange(len(x0))], [T.RAW[kind]])
+        o.prepare()
+        p = o.prepared(0)[0]
+        nc = ol.num_constraints(kind)
+        cm = o.compliance(0)[0].reshape(nc, nc)
+        t = native.KIND[kind]
+        ngeo = 4 if t == native.QUADRATIC_BENDING else native.XPBD_PARAMS[t] - nc
+        diag = np.diag(cm)[:native.XPBD_PARAMS[t] - ngeo]
+        for seed, (scale, cscale) in enumerate(((0.05, 1.0), (0.3, 1e-3), (0.3, 1e3))):
+            x = T.deformed(kind, scale=scale, seed=seed)
+            out.append((kind, x, p, np.diag(np.diag(cm) * cscale), np.concatenate([p[:ngeo], diag * cscale])))
+    return out
+
+
+FURTHER_XPBD = _further_xpbd_cases()
+
+
+@pytest.mark.parametrize("case", range(len(FURTHER_XPBD)))
+def test_xpbd_further_projection_on_host_matches_oracle(case):
+    """The generic projection of the further energies (xpbd_element.h, host instantiation) against the oracle's
+    projection with dual-number constraint gradients: FP64 within 1e-10 of the displacement; FP32 (with its
+    FP64 recomputation) within 1e-5 of the displacement plus 1e-6 of the positions, since a nearly satisfied
+    constraint such as |d| / L0 - 1 loses digits to cancellation when it is evaluated from FP32 positions."""
+    kind, x, p, cm, xp = FURTHER_XPBD[case]
+    dt = 1 / 60
+    w = np.linspace(0.5, 2.0, len(x))
+    ref = ol.xpbd_project(kind, x.ravel(), w, p, cm, dt)
+    scale = max(np.abs(ref).max(), 1e-300)
+    for prec, tol in ((1, 1e-10 * scale), (0, 1e-5 * scale + 1e-6 * np.abs(x).max())):
+        got = native.xpbd_project_stencils(kind, x.ravel()[None], w[None], xp[None], dt, precision=prec, device=-1)[0]
+        assert np.abs(got - ref).max() <= tol, (kind, prec, np.abs(got - ref).max() / scale)

@@ -22,6 +22,8 @@ batched    = 256-cloth batch (config 4) throughput, one launch per colour for al
 volumetric = 64^3-cell Stable Neo-Hookean cube (config 3) throughput
 xpbd       = the XPBD parallel Gauss-Seidel step (spg's XPBD(true)) on the same cloth: stencil projections/s,
              per-type projection-kernel rooflines, and the FP64 oracle's XPBD step on the host cores
+further_energies = VBD on the same cloth built from the further energies (Baraff-Witkin membrane and bending,
+             continuum edge springs), and the oracle's step on a 64x64 cloth on the host cores
 implicit   = the Baraff-Witkin implicit Euler step (FP64: element Hessians, block-sparse assembly, Jacobi CG with
              the reference's 1e-5 / 300-iteration settings) on the same cloth, the roofline of the symmetric
              block-sparse SpMV sampled inside the CG, and the oracle's step on a 128x128 cloth on the host cores
@@ -58,6 +60,7 @@ def main():
     ap.add_argument("--learned-steps", type=int, default=20, help="trajectory steps of the learned comparison")
     ap.add_argument("--no-batched", action="store_true", help="skip the 256-cloth batched measurement")
     ap.add_argument("--no-volumetric", action="store_true", help="skip the 64^3 cube measurement")
+    ap.add_argument("--no-further", action="store_true", help="skip the further-energies VBD measurement")
     ap.add_argument("--no-xpbd", action="store_true", help="skip the XPBD measurement")
     ap.add_argument("--xpbd-substeps", type=int, default=20, help="XPBD substeps per step")
     ap.add_argument("--xpbd-steps", type=int, default=30, help="timed XPBD steps")
@@ -139,6 +142,7 @@ def main():
 
     batched = None if args.no_batched else batched_throughput(args)
     volumetric = None if args.no_volumetric else volumetric_throughput(args)
+    further = None if args.no_further else further_throughput(args)
     xpbd = None if args.no_xpbd else xpbd_throughput(s, args)
     implicit = None if args.no_implicit else implicit_throughput(s, args)
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
@@ -177,6 +181,7 @@ def main():
         "learned": learned_cmp,
         "batched": batched,
         "volumetric": volumetric,
+        "further_energies": further,
         "xpbd": xpbd,
         "implicit": implicit,
     }
@@ -247,6 +252,47 @@ def volumetric_throughput(args, steps=20):
     out = {"workload": "cube 64^3 cells, 6 tets per cell (V=274625, 1572864 SNH tets, Y 1e5, nu 0.45), bottom face "
                        "anchored, rest scaled 1.05 in y, 20 iterations/step, hipGraph replay"}
     out.update(_throughput([cube], steps))
+    return out
+
+
+def further_throughput(args, steps=20):
+    """VBD on the bench's cloth built from the further energies: Baraff-Witkin membrane and bending, and
+    continuum springs on the mesh edges (one extra incidence class whose vertex blocks come from forward-mode
+    passes over the closed-form element gradients). CPU baseline: the oracle's VBD step (dual numbers, as the
+    reference's TinyAD) on a 64x64 cloth of the same construction, about 5 s on the host cores."""
+    import numpy as np
+    from spg_amd import scenes
+
+    def build(n):
+        base = scenes.cloth_config2(seed=3, n=n)
+        yt = base.meta["young"] * 1e-3
+        return scenes.cloth(n, young=base.meta["young"], poisson=0.3, thickness=1e-3, bending=1e-4,
+                            mass=1e-3 * (32.0 / n) ** 2, pin="top_row", seed=3, iterations=20,
+                            membrane="membrane_bw", bending_kind="bending_bw", edge_springs="spring_continuum",
+                            edge_k=0.1 * yt)
+
+    cloth = build(512)
+    out = {"workload": "cloth 512x512 as the main line (Y from seed 3), Baraff-Witkin membrane (kWeft = kWarp = Y t, "
+                       "kBias = Y t / 2.6) and bending (k 1e-4), continuum springs on the 784,385 edges "
+                       "(k 0.1 Y t), top row anchored, 20 iterations/step, hipGraph replay"}
+    out.update(_throughput([cloth], steps))
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        ol.set_threads(cores)
+        small = build(64)
+        o = ol.from_scene(small)
+        n, t = 0, 0.0
+        while t < 5.0:
+            o.set_state(small.x, np.zeros_like(small.x))
+            t0 = time.perf_counter()
+            o.step(small.dt, 1, small.iterations, small.gravity, mode=ol.AD)
+            t += time.perf_counter() - t0
+            n += 1
+        out["cpu_baseline"] = {"value": small.n * small.iterations * n / t, "unit": "vertex-updates/s", "cores": cores,
+                               "kind": "port", "sample": f"{n} first steps (20 iterations) of the 64x64 cloth of the "
+                                                         f"same construction, FP64 dual numbers, {t:.1f} s"}
     return out
 
 

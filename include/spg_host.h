@@ -58,6 +58,12 @@ int spg_pg_inv_masses(const spg_pgroup *pg, double *w);
 int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v);
 int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v);
 
+/* spg::io::loadObj / loadMsh (source/geom/io/obj.cpp:10-35, msh.cpp:10-41): a mesh file read in two calls,
+   first with NULL outputs for the counts, then with arrays of those sizes (vertices 3 doubles each, faces
+   3 / tetrahedra 4 vertex indices each). */
+int spg_io_load_obj(const char *path, int64_t *n_vertices, int64_t *n_faces, double *vertices, int32_t *faces);
+int spg_io_load_msh(const char *path, int64_t *n_vertices, int64_t *n_tets, double *vertices, int32_t *tets);
+
 /* TriangleMesh(faces).bendingStencils(): out may be NULL to query the count. */
 int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int32_t *faces, int32_t *out,
                                  int64_t *count);

@@ -31,7 +31,8 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
 
 VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "mlp_context.hip", "warmstart_kernels.hip",
                "xpbd_kernels.hip", "xpbd_context.hip", "implicit_kernels.hip", "implicit_context.hip", "coloring.cpp"]
-HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "flatten.cpp", "vbdGpu.cpp", "xpbdGpu.cpp", "implicitEulerGpu.cpp",
+HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "meshIo.cpp", "flatten.cpp", "vbdGpu.cpp", "xpbdGpu.cpp",
+                "implicitEulerGpu.cpp",
                 "host_capi.cpp"]
 
 

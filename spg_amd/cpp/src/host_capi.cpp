@@ -3,6 +3,8 @@
 #include <memory>
 #include <string>
 
+#include <spg/geom/io/msh.h>
+#include <spg/geom/io/obj.h>
 #include <spg/geom/triangleMesh.h>
 #include <spg/sim/energy/energies.h>
 #include <spg/sim/solver/baseSolver.h>
@@ -69,6 +71,25 @@ void checkIndex(const spg_pgroup *pg, int32_t index)
         throw std::runtime_error("energy index out of range");
 }
 
+}  // namespace
+
+namespace {
+template <class Cell, int N, class Loader>
+int loadMesh(Loader load, const char *path, int64_t *nv, int64_t *nc, double *vertices, int32_t *cells)
+{
+    return guarded([&] {
+        if (!path || !nv || !nc) throw std::runtime_error("invalid arguments");
+        const auto [v, c] = load(std::string(path));
+        *nv = static_cast<int64_t>(v.size());
+        *nc = static_cast<int64_t>(c.size());
+        if (vertices)
+            for (size_t i = 0; i < v.size(); ++i)
+                for (int d = 0; d < 3; ++d) vertices[3 * i + d] = v[i][d];
+        if (cells)
+            for (size_t i = 0; i < c.size(); ++i)
+                for (int d = 0; d < N; ++d) cells[N * i + d] = c[i][d];
+    });
+}
 }  // namespace
 
 extern "C" {
@@ -225,6 +246,17 @@ int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v)
                 if (v) pg->group.velocities()[i][d] = v[3 * i + d];
             }
     });
+}
+
+
+int spg_io_load_obj(const char *path, int64_t *n_vertices, int64_t *n_faces, double *vertices, int32_t *faces)
+{
+    return loadMesh<spg::Int3, 3>(spg::io::loadObj, path, n_vertices, n_faces, vertices, faces);
+}
+
+int spg_io_load_msh(const char *path, int64_t *n_vertices, int64_t *n_tets, double *vertices, int32_t *tets)
+{
+    return loadMesh<spg::Int4, 4>(spg::io::loadMsh, path, n_vertices, n_tets, vertices, tets);
 }
 
 int spg_trimesh_bending_stencils(int64_t n_vertices, int64_t n_faces, const int32_t *faces, int32_t *out,

@@ -244,6 +244,8 @@ def host_lib():
         _proto(L, "spg_pg_get_state", ctypes.c_int, [c, _dp, _dp])
         _proto(L, "spg_pg_set_state", ctypes.c_int, [c, _dp, _dp])
         _proto(L, "spg_trimesh_bending_stencils", ctypes.c_int, [i64, i64, _ip, _ip, _lp])
+        _proto(L, "spg_io_load_obj", ctypes.c_int, [ctypes.c_char_p, _lp, _lp, _dp, _ip])
+        _proto(L, "spg_io_load_msh", ctypes.c_int, [ctypes.c_char_p, _lp, _lp, _dp, _ip])
         _proto(L, "spg_solver_create_vbd_gpu", c, [i32])
         _proto(L, "spg_solver_create_xpbd_gpu", c, [i32, i32])
         _proto(L, "spg_solver_xpbd_context", c, [c])
@@ -932,6 +934,27 @@ def xpbd_color_stencils(stencils):
     if rc != 0:
         raise NativeError(rc, L.spg_xpbd_last_error(None).decode())
     return out
+
+
+def _load_mesh(fn, path, per):
+    L = host_lib()
+    nv, nc = ctypes.c_int64(0), ctypes.c_int64(0)
+    p = os.fsencode(path)
+    _host_check(fn(p, ctypes.byref(nv), ctypes.byref(nc), None, None))
+    x = np.zeros((nv.value, 3))
+    c = np.zeros((nc.value, per), dtype=np.int32)
+    _host_check(fn(p, ctypes.byref(nv), ctypes.byref(nc), _d(x), _i(c)))
+    return x, c
+
+
+def load_obj(path):
+    """spg::io::loadObj: (vertices (n, 3), triangles (m, 3)) of an OBJ file's first shape."""
+    return _load_mesh(host_lib().spg_io_load_obj, path, 3)
+
+
+def load_msh(path):
+    """spg::io::loadMsh: (nodes (n, 3), tetrahedra (m, 4)) of an ASCII MSH 4.1 file."""
+    return _load_mesh(host_lib().spg_io_load_msh, path, 4)
 
 
 def implicit_element_derivatives(kind, xs, params, device=0):

@@ -2041,6 +2041,324 @@ void implicitSubstep(Object &o, ImplicitState &st, int solver, double dtStep, in
     }
 }
 
+// ---------------------------------------------------------------------------------------------
+// Rigid-body VBD (vbd.cpp:38-185 with the RigidBodyGroup branches, rigidBodyGroup.cpp:19-243, and the
+// energies rigidBodySpringEnergy.cpp, rigidBodySpringSquaredEnergy.cpp, rigidBodySpringAnchorEnergy.cpp,
+// rigidBodyOrientationAlignmentEnergy.cpp). Each body has 6 DOFs: its centre of mass x and a rotation
+// increment theta applied on the left of its rotation matrix R, (I + [theta]x) R, differentiated at 0.
+// ---------------------------------------------------------------------------------------------
+enum { RB_SPRING = 21, RB_SPRING_SQUARED = 22, RB_SPRING_ANCHOR = 23, RB_ORIENTATION_ALIGNMENT = 24 };
+
+int rbStencilSize(int type) { return type == RB_SPRING_ANCHOR ? 1 : (type >= RB_SPRING && type <= 24) ? 2 : 0; }
+int rbRawParams(int type)
+{
+    switch (type) {
+    case RB_SPRING:
+    case RB_SPRING_SQUARED: return 8;  // local points (2 x 3), rest length, stiffness
+    case RB_SPRING_ANCHOR: return 7;   // local point, anchor, stiffness
+    case RB_ORIENTATION_ALIGNMENT: return 1;
+    }
+    return 0;
+}
+
+// Eigen::AngleAxis(angle, axis).toRotationMatrix(): Rodrigues' formula; a zero vector is the identity
+// (rigidBodyGroup.cpp:6-10).
+M3 angleAxisToR(double angle, const A3 &a)
+{
+    const double c = std::cos(angle), sn = std::sin(angle);
+    const A3 s = {sn * a[0], sn * a[1], sn * a[2]};
+    const A3 c1 = {(1 - c) * a[0], (1 - c) * a[1], (1 - c) * a[2]};
+    M3 r;
+    for (int i = 0; i < 3; ++i) r[4 * i] = c1[i] * a[i] + c;
+    double t = c1[0] * a[1];
+    r[1] = t - s[2];
+    r[3] = t + s[2];
+    t = c1[0] * a[2];
+    r[2] = t + s[1];
+    r[6] = t - s[1];
+    t = c1[1] * a[2];
+    r[5] = t - s[0];
+    r[7] = t + s[0];
+    return r;
+}
+
+M3 aaToR(const A3 &t)
+{
+    const double angle = std::sqrt(a3dot(t, t));
+    if (angle == 0) return {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    return angleAxisToR(angle, {t[0] / angle, t[1] / angle, t[2] / angle});
+}
+
+// Eigen::AngleAxis(R) as angle * axis (rigidBodyGroup.cpp:12-16): the unit quaternion of R by Shoemake's
+// method (the largest of trace and diagonal picks the stable formula), then angle = 2 atan2(|v|, |w|) and
+// axis = v / |v|, negated when w < 0; the identity maps to 0.
+A3 rToAa(const M3 &R)
+{
+    double q[4];  // w, x, y, z
+    const double tr = R[0] + R[4] + R[8];
+    if (tr > 0) {
+        double t = std::sqrt(tr + 1.0);
+        q[0] = 0.5 * t;
+        t = 0.5 / t;
+        q[1] = (R[7] - R[5]) * t;
+        q[2] = (R[2] - R[6]) * t;
+        q[3] = (R[3] - R[1]) * t;
+    } else {
+        int i = 0;
+        if (R[4] > R[0]) i = 1;
+        if (R[8] > R[4 * i]) i = 2;
+        const int j = (i + 1) % 3, k = (j + 1) % 3;
+        double t = std::sqrt(R[4 * i] - R[4 * j] - R[4 * k] + 1.0);
+        q[1 + i] = 0.5 * t;
+        t = 0.5 / t;
+        q[0] = (R[3 * k + j] - R[3 * j + k]) * t;
+        q[1 + j] = (R[3 * j + i] + R[3 * i + j]) * t;
+        q[1 + k] = (R[3 * k + i] + R[3 * i + k]) * t;
+    }
+    double n = std::sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+    if (n == 0) return {0, 0, 0};
+    const double angle = 2.0 * std::atan2(n, std::fabs(q[0]));
+    if (q[0] < 0) n = -n;
+    return {q[1] / n * angle, q[2] / n * angle, q[3] / n * angle};
+}
+
+M3 m3T(const M3 &a) { return {a[0], a[3], a[6], a[1], a[4], a[7], a[2], a[5], a[8]}; }
+M3 m3mul(const M3 &a, const M3 &b)
+{
+    M3 r{};
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    return r;
+}
+A3 m3vec(const M3 &a, const A3 &v)
+{
+    return {a[0] * v[0] + a[1] * v[1] + a[2] * v[2], a[3] * v[0] + a[4] * v[1] + a[5] * v[2],
+            a[6] * v[0] + a[7] * v[1] + a[8] * v[2]};
+}
+
+struct RbObject {
+    int64_t nb = 0;
+    std::vector<A3> x, theta, v, omega;
+    std::vector<M3> R, Iloc, IlocInv, I, Iinv;
+    std::vector<double> m;
+    Object shape;  // stencils of the energies, for the colouring (colorVertices)
+    std::vector<std::vector<double>> params;  // per energy, raw addStencil arguments
+    bool precompDone = false;
+    std::vector<std::vector<int32_t>> groups;
+    std::vector<std::vector<std::array<int32_t, 3>>> inc;  // per body: (energy, stencil, slot)
+};
+
+void rbUpdateInertia(RbObject &o, int64_t b)  // rigidBodyGroup.cpp:195-199
+{
+    o.I[b] = m3mul(m3mul(o.R[b], o.Iloc[b]), m3T(o.R[b]));
+    o.Iinv[b] = m3mul(m3mul(o.R[b], o.IlocInv[b]), m3T(o.R[b]));
+}
+
+// (I + [theta]x) R l with theta a dual vector at 0: R l + theta x (R l)
+template <class S>
+V3<S> rbPoint(const V3<S> &com, const V3<S> &th, const M3 &R, const double *l)
+{
+    const A3 r = m3vec(R, {l[0], l[1], l[2]});
+    const V3<S> rs = {S(r[0]), S(r[1]), S(r[2])};
+    const V3<S> t = cross(th, rs);
+    return {com.x + rs.x + t.x, com.y + rs.y + t.y, com.z + rs.z + t.z};
+}
+
+// The element energy over the stencil's 6 NS DOFs, as the reference differentiates it through TinyAD.
+template <int NS, class S>
+S rbEnergy(int type, const V3<S> *com, const V3<S> *th, const M3 *R, const double *p)
+{
+    if constexpr (NS == 1) {  // rigidBodySpringAnchorEnergy.cpp:35-45: 1/2 k |p - a|^2
+        const V3<S> q = rbPoint(com[0], th[0], R[0], p);
+        const V3<S> d = {q.x - p[3], q.y - p[4], q.z - p[5]};
+        return 0.5 * p[6] * dot(d, d);
+    } else {
+        if (type == RB_ORIENTATION_ALIGNMENT) {  // C = sum_k |col_k(R0') x col_k(R1')|^2
+            S c = S(0.0);
+            for (int k = 0; k < 3; ++k) {
+                const V3<S> u = {S(R[0][k]), S(R[0][3 + k]), S(R[0][6 + k])};
+                const V3<S> w = {S(R[1][k]), S(R[1][3 + k]), S(R[1][6 + k])};
+                const V3<S> a = {u.x + cross(th[0], u).x, u.y + cross(th[0], u).y, u.z + cross(th[0], u).z};
+                const V3<S> b = {w.x + cross(th[1], w).x, w.y + cross(th[1], w).y, w.z + cross(th[1], w).z};
+                const V3<S> ab = cross(a, b);
+                c = c + dot(ab, ab);
+            }
+            return 0.5 * c * (p[0] * c);
+        }
+        const V3<S> q0 = rbPoint(com[0], th[0], R[0], p), q1 = rbPoint(com[1], th[1], R[1], p + 3);
+        const V3<S> d = sub(q0, q1);
+        const S c = type == RB_SPRING ? sqrt(dot(d, d)) - p[6] : dot(d, d) - p[6];
+        return 0.5 * c * (p[7] * c);
+    }
+}
+
+// Gradient (6 NS) and Hessian of one element at theta = 0 (6 NS x 6 NS, row-major).
+template <int NS>
+void rbElementAD(int type, const double *com, const M3 *R, const double *p, double *grad, double *hess)
+{
+    constexpr int N = 6 * NS;
+    using D = Dual<N, true>;
+    V3<D> c[NS], t[NS];
+    for (int s = 0; s < NS; ++s) {
+        c[s] = {D::var(com[3 * s], 6 * s), D::var(com[3 * s + 1], 6 * s + 1), D::var(com[3 * s + 2], 6 * s + 2)};
+        t[s] = {D::var(0.0, 6 * s + 3), D::var(0.0, 6 * s + 4), D::var(0.0, 6 * s + 5)};
+    }
+    const D e = rbEnergy<NS>(type, c, t, R, p);
+    for (int i = 0; i < N; ++i) grad[i] = e.g[i];
+    if (hess)
+        for (int i = 0; i < N * N; ++i) hess[i] = e.h[i];
+}
+
+// prepared parameters: the squared spring keeps L0^2 (rigidBodySpringSquaredEnergy.cpp:35-48)
+void rbPrepareParams(int type, const double *raw, double *p)
+{
+    for (int k = 0; k < rbRawParams(type); ++k) p[k] = raw[k];
+    if (type == RB_SPRING_SQUARED) p[6] = raw[6] * raw[6];
+}
+
+// Solve H dx = f by LU with partial pivoting (H.inverse() * f); returns det(H).
+double solve6(double H[6][6], const double *f, double *dx)
+{
+    double a[6][7];
+    for (int i = 0; i < 6; ++i) {
+        for (int j = 0; j < 6; ++j) a[i][j] = H[i][j];
+        a[i][6] = f[i];
+    }
+    double det = 1.0;
+    for (int c = 0; c < 6; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 6; ++r)
+            if (std::fabs(a[r][c]) > std::fabs(a[piv][c])) piv = r;
+        if (piv != c) {
+            for (int k = 0; k < 7; ++k) std::swap(a[c][k], a[piv][k]);
+            det = -det;
+        }
+        det *= a[c][c];
+        if (a[c][c] == 0) return 0.0;
+        for (int r = c + 1; r < 6; ++r) {
+            const double fct = a[r][c] / a[c][c];
+            for (int k = c; k < 7; ++k) a[r][k] -= fct * a[c][k];
+        }
+    }
+    for (int i = 5; i >= 0; --i) {
+        double t = a[i][6];
+        for (int k = i + 1; k < 6; ++k) t -= a[i][k] * dx[k];
+        dx[i] = t / a[i][i];
+    }
+    return det;
+}
+
+void rbPrecompute(RbObject &o)
+{
+    const auto col = colorVertices(o.shape);
+    int32_t nc = 0;
+    for (auto c : col) nc = std::max(nc, c + 1);
+    o.groups.assign(nc, {});
+    for (int64_t b = 0; b < o.nb; ++b) o.groups[col[b]].push_back(static_cast<int32_t>(b));
+    o.inc.assign(o.nb, {});
+    for (size_t e = 0; e < o.shape.energies.size(); ++e) {
+        const auto &en = o.shape.energies[e];
+        for (int64_t i = 0; i < en.n(); ++i)
+            for (int s = 0; s < en.ssize; ++s)
+                o.inc[en.st[i * en.ssize + s]].push_back({static_cast<int32_t>(e), static_cast<int32_t>(i), s});
+    }
+    o.precompDone = true;
+}
+
+// omega from the rotation R R(thetaOld)^T over h (RigidBodyGroup::computeIntegratedOmega)
+A3 rbIntegratedOmega(const RbObject &o, int64_t b, const A3 &thetaOld, double invdt)
+{
+    const A3 w = rToAa(m3mul(o.R[b], m3T(aaToR(thetaOld))));
+    return {w[0] * invdt, w[1] * invdt, w[2] * invdt};
+}
+
+void rbVertexDescent(RbObject &o, int64_t b, const std::vector<A3> &xT, const std::vector<A3> &thOld, double h)
+{
+    const double invdt = 1 / h, invdt2 = invdt * invdt;
+    const double mdt2 = o.m[b] * invdt2;
+    double f[6], H[6][6] = {};
+    for (int d = 0; d < 3; ++d) f[d] = -mdt2 * (o.x[b][d] - xT[b][d]);
+    const A3 wi = rbIntegratedOmega(o, b, thOld[b], invdt);
+    const A3 dw = {wi[0] - o.omega[b][0], wi[1] - o.omega[b][1], wi[2] - o.omega[b][2]};
+    const A3 Idw = m3vec(o.I[b], dw);
+    for (int d = 0; d < 3; ++d) f[3 + d] = -invdt * Idw[d];
+    for (int d = 0; d < 3; ++d) {
+        H[d][d] = mdt2;
+        for (int q = 0; q < 3; ++q) H[3 + d][3 + q] = invdt2 * o.I[b][3 * d + q];
+    }
+    for (const auto &in : o.inc[b]) {
+        const auto &en = o.shape.energies[in[0]];
+        const int ns = en.ssize, N = 6 * ns;
+        double com[6];
+        M3 R[2];
+        for (int s = 0; s < ns; ++s) {
+            const int64_t bb = en.st[in[1] * ns + s];
+            for (int d = 0; d < 3; ++d) com[3 * s + d] = o.x[bb][d];
+            R[s] = o.R[bb];
+        }
+        double p[8];
+        rbPrepareParams(en.type, &o.params[in[0]][in[1] * rbRawParams(en.type)], p);
+        double g[12], hs[144];
+        if (ns == 1)
+            rbElementAD<1>(en.type, com, R, p, g, hs);
+        else
+            rbElementAD<2>(en.type, com, R, p, g, hs);
+        const int sl = in[2];
+        for (int i = 0; i < 6; ++i) {
+            f[i] -= g[6 * sl + i];
+            for (int j = 0; j < 6; ++j) H[i][j] += hs[(6 * sl + i) * N + 6 * sl + j];
+        }
+    }
+    double dx[6];
+    double Hc[6][6];
+    std::memcpy(Hc, H, sizeof(H));
+    if (solve6(Hc, f, dx) > 1e-10) {  // vbd.cpp:139-145, updateElementPositionFromDx (rigidBodyGroup.cpp:64-77)
+        for (int d = 0; d < 3; ++d) o.x[b][d] += dx[d];
+        const A3 dt = {dx[3], dx[4], dx[5]};
+        if (std::sqrt(a3dot(dt, dt)) != 0) o.R[b] = m3mul(aaToR(dt), o.R[b]);
+        o.theta[b] = rToAa(o.R[b]);
+        rbUpdateInertia(o, b);
+    }
+}
+
+void rbStep(RbObject &o, double dtStep, int nsub, int iterations, const double *g)
+{
+    if (!o.precompDone) rbPrecompute(o);
+    const double h = dtStep / nsub, invdt = 1 / h;
+    for (int sub = 0; sub < nsub; ++sub) {
+        const std::vector<A3> xOld = o.x, thOld = o.theta;
+        for (int64_t b = 0; b < o.nb; ++b) {  // vbd.cpp:62-72
+            for (int d = 0; d < 3; ++d) o.v[b][d] += h * g[d];
+            const A3 Iw = m3vec(o.I[b], o.omega[b]);
+            const A3 c = a3cross(o.omega[b], Iw);
+            const A3 acc = m3vec(o.Iinv[b], {-c[0], -c[1], -c[2]});
+            for (int d = 0; d < 3; ++d) o.omega[b][d] += h * acc[d];
+        }
+        for (int64_t b = 0; b < o.nb; ++b) {  // integrateVelocities (rigidBodyGroup.cpp:97-113)
+            for (int d = 0; d < 3; ++d) o.x[b][d] += o.v[b][d] * h;
+            const double wn = std::sqrt(a3dot(o.omega[b], o.omega[b]));
+            if (wn != 0)
+                o.R[b] = m3mul(angleAxisToR(wn * h, {o.omega[b][0] / wn, o.omega[b][1] / wn, o.omega[b][2] / wn}),
+                               o.R[b]);
+        }
+        for (int64_t b = 0; b < o.nb; ++b) {
+            o.theta[b] = rToAa(o.R[b]);
+            rbUpdateInertia(o, b);
+        }
+        const std::vector<A3> xT = o.x;
+        for (int it = 0; it < iterations; ++it)
+            for (const auto &grp : o.groups) {
+                const int64_t gs = static_cast<int64_t>(grp.size());
+#pragma omp parallel for schedule(static)
+                for (int64_t k = 0; k < gs; ++k) rbVertexDescent(o, grp[k], xT, thOld, h);
+            }
+        for (int64_t b = 0; b < o.nb; ++b) {  // computeIntegratedVelocities (rigidBodyGroup.cpp:115-128)
+            for (int d = 0; d < 3; ++d) o.v[b][d] = (o.x[b][d] - xOld[b][d]) * invdt;
+            o.omega[b] = rbIntegratedOmega(o, b, thOld[b], invdt);
+        }
+    }
+}
+
 template <class F>
 int guard(F &&f)
 {
@@ -2458,6 +2776,138 @@ int orc_implicit_step(void *h, void *state, int solver, double dt, int nsubsteps
         }
         if (current_dt) *current_dt = st.currentDt;
     });
+}
+
+/* ---- rigid-body VBD ---- */
+void *orc_rb_new(int64_t nb, const double *x, const double *theta, const double *m, const double *inertia)
+{
+    auto *o = new RbObject;
+    o->nb = nb;
+    o->shape.nv = nb;
+    for (int64_t b = 0; b < nb; ++b) {
+        o->x.push_back({x[3 * b], x[3 * b + 1], x[3 * b + 2]});
+        o->theta.push_back({theta[3 * b], theta[3 * b + 1], theta[3 * b + 2]});
+        o->v.push_back({0, 0, 0});
+        o->omega.push_back({0, 0, 0});
+        o->m.push_back(m[b]);
+        M3 Il;
+        for (int k = 0; k < 9; ++k) Il[k] = inertia[9 * b + k];
+        o->Iloc.push_back(Il);
+        o->IlocInv.push_back(inverse3(Il));
+        o->R.push_back(aaToR(o->theta.back()));
+        o->I.push_back(Il);
+        o->Iinv.push_back(Il);
+        rbUpdateInertia(*o, b);
+    }
+    return o;
+}
+void orc_rb_free(void *h) { delete static_cast<RbObject *>(h); }
+
+int orc_rb_add_energy(void *h, int type, int64_t n, const int32_t *stencils, const double *raw)
+{
+    return guard([&] {
+        auto &o = *static_cast<RbObject *>(h);
+        const int ss = rbStencilSize(type), np = rbRawParams(type);
+        if (!ss) throw OracleError("unknown rigid-body energy type");
+        Energy e;
+        e.type = type;
+        e.ssize = ss;
+        e.st.assign(stencils, stencils + n * ss);
+        for (auto b : e.st)
+            if (b < 0 || b >= o.nb) throw OracleError("stencil index out of range");
+        for (int64_t i = 0; i < n; ++i) {
+            if (type == RB_SPRING && raw[i * np + 6] <= 0) throw OracleError("Zero/negative rest length invalid");
+            if (type == RB_SPRING_SQUARED && raw[i * np + 6] < 0) throw OracleError("Negative rest length invalid");
+        }
+        o.shape.energies.push_back(e);
+        o.params.emplace_back(raw, raw + n * np);
+        o.precompDone = false;
+    });
+}
+
+int orc_rb_step(void *h, double dt, int nsubsteps, int iterations, const double *gravity)
+{
+    return guard([&] {
+        if (!(dt > 0) || nsubsteps < 1 || iterations < 0) throw OracleError("invalid step arguments");
+        rbStep(*static_cast<RbObject *>(h), dt, nsubsteps, iterations, gravity);
+    });
+}
+
+int orc_rb_state(void *h, double *x, double *theta, double *v, double *omega)
+{
+    return guard([&] {
+        const auto &o = *static_cast<RbObject *>(h);
+        for (int64_t b = 0; b < o.nb; ++b)
+            for (int d = 0; d < 3; ++d) {
+                if (x) x[3 * b + d] = o.x[b][d];
+                if (theta) theta[3 * b + d] = o.theta[b][d];
+                if (v) v[3 * b + d] = o.v[b][d];
+                if (omega) omega[3 * b + d] = o.omega[b][d];
+            }
+    });
+}
+
+int orc_rb_set_state(void *h, const double *x, const double *theta, const double *v, const double *omega)
+{
+    return guard([&] {
+        auto &o = *static_cast<RbObject *>(h);
+        for (int64_t b = 0; b < o.nb; ++b) {
+            for (int d = 0; d < 3; ++d) {
+                if (x) o.x[b][d] = x[3 * b + d];
+                if (theta) o.theta[b][d] = theta[3 * b + d];
+                if (v) o.v[b][d] = v[3 * b + d];
+                if (omega) o.omega[b][d] = omega[3 * b + d];
+            }
+            o.R[b] = aaToR(o.theta[b]);  // setPositions: rotation and inertia from theta (rigidBodyGroup.cpp:41-51)
+            rbUpdateInertia(o, b);
+        }
+    });
+}
+
+int orc_rb_colors(void *h, int32_t *colors, int32_t *ncolors)
+{
+    return guard([&] {
+        auto &o = *static_cast<RbObject *>(h);
+        if (!o.precompDone) rbPrecompute(o);
+        for (size_t c = 0; c < o.groups.size(); ++c)
+            for (auto b : o.groups[c]) colors[b] = static_cast<int32_t>(c);
+        if (ncolors) *ncolors = static_cast<int32_t>(o.groups.size());
+    });
+}
+
+/* gradient (6 size) and Hessian (6 size)^2 of one rigid-body element at theta = 0; com: 3 per body,
+   rot: 9 per body (row-major), raw: the addStencil arguments */
+int orc_rb_element(int type, const double *com, const double *rot, const double *raw, double *grad, double *hess)
+{
+    return guard([&] {
+        const int ss = rbStencilSize(type);
+        if (!ss) throw OracleError("unknown rigid-body energy type");
+        M3 R[2];
+        for (int s = 0; s < ss; ++s)
+            for (int k = 0; k < 9; ++k) R[s][k] = rot[9 * s + k];
+        double p[8];
+        rbPrepareParams(type, raw, p);
+        if (ss == 1)
+            rbElementAD<1>(type, com, R, p, grad, hess);
+        else
+            rbElementAD<2>(type, com, R, p, grad, hess);
+    });
+}
+
+/* Eigen's AngleAxis conversions as restated here (for tests) */
+int orc_rb_aa_to_r(const double *theta, double *R)
+{
+    const M3 r = aaToR({theta[0], theta[1], theta[2]});
+    std::copy(r.begin(), r.end(), R);
+    return 0;
+}
+int orc_rb_r_to_aa(const double *R, double *theta)
+{
+    M3 r;
+    std::copy(R, R + 9, r.begin());
+    const A3 t = rToAa(r);
+    std::copy(t.begin(), t.end(), theta);
+    return 0;
 }
 
 }  // extern "C"

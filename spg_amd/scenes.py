@@ -228,3 +228,104 @@ def cloth_batch(count=256, seed=7, lo=16, hi=128, iterations=15):
                             mass=1e-3 * (32.0 / side) ** 2, pin="top_row", seed=seed * 1000 + k,
                             iterations=iterations))
     return scenes
+
+
+# ---- rigid bodies ----
+
+@dataclass
+class RigidScene:
+    """A RigidBodyGroup: bodies (centre of mass, axis-angle orientation, mass, local inertia) and its
+    energies [(kind, stencils, addStencil args)] in addEnergy order."""
+    x: np.ndarray          # (nb, 3)
+    theta: np.ndarray      # (nb, 3)
+    m: np.ndarray          # (nb,)
+    inertia: np.ndarray    # (nb, 3, 3) local inertia tensors
+    energies: list = field(default_factory=list)
+    iterations: int = 10
+    nsubsteps: int = 1
+    dt: float = DT
+    gravity: tuple = GRAVITY
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nb(self):
+        return self.x.shape[0]
+
+
+def box_inertia(mass, width, height, depth):
+    """Solid box about its centre (apps/solverCompare.cpp:540-543)."""
+    return np.diag([mass * (height ** 2 + depth ** 2), mass * (width ** 2 + depth ** 2),
+                    mass * (width ** 2 + height ** 2)]) / 12.0
+
+
+def rb_anchored(mass=1.0, w=1.0, h=1.0, d=1.0, iterations=10):
+    """createAnchoredRigidBody (apps/solverCompare.cpp:524-555): one box held by three anchor springs."""
+    raw = np.array([[0, h * 0.5, 0, 0, h * 0.5, -1, 100], [w * 0.5, -h * 0.5, 0, w, -h * 0.5, 1, 100],
+                    [-w * 0.5, -h * 0.5, 0, -w, -h * 0.5, 1, 100]], dtype=np.float64)
+    return RigidScene(x=np.zeros((1, 3)), theta=np.zeros((1, 3)), m=np.array([mass]),
+                      inertia=box_inertia(mass, w, h, d)[None], iterations=iterations,
+                      energies=[("rb_anchor", np.zeros((3, 1), np.int32), raw)], meta={"kind": "rb_anchored"})
+
+
+def rb_articulated(n, mass=1.0, w=1.0, h=1.0, d=1.0, iterations=10):
+    """createArticulatedRigidBody (apps/solverCompare.cpp:557-603): a row of boxes, the first held by three
+    stiff anchors, each next one tied to its predecessor by a zero-length squared spring between facing
+    face centres and an orientation alignment."""
+    x = np.column_stack([w * np.arange(n), np.zeros(n), np.zeros(n)]).astype(np.float64)
+    anchors = np.array([[0, h * 0.5, 0, 0, h * 0.5, 0, 1e10], [0, -h * 0.5, 0, 0, -h * 0.5, 0, 1e10],
+                        [-w * 0.5, 0, 0, -w * 0.5, 0, 0, 1e10]], dtype=np.float64)
+    energies = [("rb_anchor", np.zeros((3, 1), np.int32), anchors)]
+    if n > 1:
+        pairs = np.column_stack([np.arange(n - 1), np.arange(1, n)]).astype(np.int32)
+        sp = np.tile([w * 0.5, 0, 0, -w * 0.5, 0, 0, 0.0, 1e10], (n - 1, 1))
+        energies += [("rb_spring_squared", pairs, sp), ("rb_alignment", pairs, np.full((n - 1, 1), 1e5))]
+    return RigidScene(x=x, theta=np.zeros((n, 3)), m=np.full(n, mass),
+                      inertia=np.repeat(box_inertia(mass, w, h, d)[None], n, 0), energies=energies,
+                      iterations=iterations, meta={"kind": "rb_articulated", "bodies": n})
+
+
+def rb_chain(n, mass=1.0, w=1.0, h=1.0, d=1.0, iterations=10):
+    """createRigidBodyChain (apps/solverCompare.cpp:605-654): boxes on a diagonal, the first anchored at
+    a corner, consecutive ones joined corner to corner by springs of rest length sqrt(2 w)."""
+    i = np.arange(n, dtype=np.float64)
+    x = np.column_stack([-(h + 1) * i, np.zeros(n), -(h + 1) * i])
+    energies = [("rb_anchor", np.zeros((1, 1), np.int32),
+                 np.array([[w * 0.5, h * 0.5, 0, w * 0.5, h * 0.5, 0, 100]], dtype=np.float64))]
+    if n > 1:
+        pairs = np.column_stack([np.arange(n - 1), np.arange(1, n)]).astype(np.int32)
+        sp = np.tile([-w * 0.5, -h * 0.5, 0, w * 0.5, h * 0.5, 0, np.sqrt(w * 2), 100], (n - 1, 1))
+        energies.append(("rb_spring", pairs, sp))
+    return RigidScene(x=x, theta=np.zeros((n, 3)), m=np.full(n, mass),
+                      inertia=np.repeat(box_inertia(mass, w, h, d)[None], n, 0), energies=energies,
+                      iterations=iterations, meta={"kind": "rb_chain", "bodies": n})
+
+
+def rb_lattice(n, spacing=1.5, size=1.0, mass=1.0, k=1e3, k_align=10.0, tilt=0.05, seed=3, iterations=10):
+    """An n x n horizontal lattice of boxes (the rigid-body throughput workload): neighbours in x and z
+    joined by springs between facing face centres (rest length spacing - size) and by orientation
+    alignments, the row z = 0 held by anchor springs at its top face centres, every body tilted by a
+    random rotation of angle up to `tilt`. Energy order: anchors, springs, alignments."""
+    rng = SplitMix64(seed)
+    ix, iz = np.meshgrid(np.arange(n), np.arange(n), indexing="xy")
+    ix, iz = ix.ravel(), iz.ravel()
+    nb = n * n
+    x = np.column_stack([spacing * ix, np.zeros(nb), spacing * iz]).astype(np.float64)
+    u = rng.uniforms(3 * nb).reshape(nb, 3) * 2.0 - 1.0
+    theta = u * (tilt / np.sqrt(3.0))
+    hs = size * 0.5
+    row = np.flatnonzero(iz == 0).astype(np.int32)
+    anchors = np.column_stack([np.tile([0, hs, 0], (len(row), 1)), x[row] + [0, hs, 0], np.full(len(row), k * 10)])
+    idx = np.arange(nb).reshape(n, n)
+    px = np.column_stack([idx[:, :-1].ravel(), idx[:, 1:].ravel()]).astype(np.int32)
+    pz = np.column_stack([idx[:-1, :].ravel(), idx[1:, :].ravel()]).astype(np.int32)
+    rest = spacing - size
+    spx = np.tile([hs, 0, 0, -hs, 0, 0, rest, k], (len(px), 1))
+    spz = np.tile([0, 0, hs, 0, 0, -hs, rest, k], (len(pz), 1))
+    pairs = np.concatenate([px, pz])
+    energies = [("rb_anchor", row[:, None], anchors), ("rb_spring", pairs, np.concatenate([spx, spz])),
+                ("rb_alignment", pairs, np.full((len(pairs), 1), k_align))]
+    return RigidScene(x=x, theta=theta, m=np.full(nb, mass),
+                      inertia=np.repeat(box_inertia(mass, size, size, size)[None], nb, 0), energies=energies,
+                      iterations=iterations,
+                      meta={"kind": "rb_lattice", "n": n, "bodies": nb, "springs": len(pairs),
+                            "alignments": len(pairs), "anchors": len(row)})

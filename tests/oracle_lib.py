@@ -14,6 +14,11 @@ SPRING, MEMBRANE_STVK, DISCRETE_BENDING, STABLE_NEO_HOOKEAN, SPRING_ANCHOR = 1, 
 (SPRING_CONTINUUM, SPRING_SQUARED_CONTINUUM, MEMBRANE_BARAFF_WITKIN, MEMBRANE_CHOI, QUADRATIC_BENDING,
  BARAFF_WITKIN_BENDING, STVK, STABLE_SQUARED_NEO_HOOKEAN) = range(6, 14)
 CLOSED_FORM, AD = 0, 1
+RB_SPRING, RB_SPRING_SQUARED, RB_SPRING_ANCHOR, RB_ORIENTATION_ALIGNMENT = 21, 22, 23, 24
+RB_KIND = {"rb_spring": RB_SPRING, "rb_spring_squared": RB_SPRING_SQUARED, "rb_anchor": RB_SPRING_ANCHOR,
+           "rb_alignment": RB_ORIENTATION_ALIGNMENT}
+RB_STENCIL_SIZE = {RB_SPRING: 2, RB_SPRING_SQUARED: 2, RB_SPRING_ANCHOR: 1, RB_ORIENTATION_ALIGNMENT: 2}
+RB_RAW_PARAMS = {RB_SPRING: 8, RB_SPRING_SQUARED: 8, RB_SPRING_ANCHOR: 7, RB_ORIENTATION_ALIGNMENT: 1}
 STENCIL_SIZE = {SPRING: 2, MEMBRANE_STVK: 3, DISCRETE_BENDING: 4, STABLE_NEO_HOOKEAN: 4, SPRING_ANCHOR: 1,
                 SPRING_CONTINUUM: 2, SPRING_SQUARED_CONTINUUM: 2, MEMBRANE_BARAFF_WITKIN: 3, MEMBRANE_CHOI: 3,
                 QUADRATIC_BENDING: 4, BARAFF_WITKIN_BENDING: 4, STVK: 4, STABLE_SQUARED_NEO_HOOKEAN: 4}
@@ -73,6 +78,17 @@ def lib():
         L.orc_implicit_free.argtypes = [ctypes.c_void_p]
         L.orc_implicit_step.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         _dp, _lp, _dp]
+        L.orc_rb_new.restype = ctypes.c_void_p
+        L.orc_rb_new.argtypes = [ctypes.c_int64, _dp, _dp, _dp, _dp]
+        L.orc_rb_free.argtypes = [ctypes.c_void_p]
+        L.orc_rb_add_energy.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _ip, _dp]
+        L.orc_rb_step.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
+        L.orc_rb_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_rb_set_state.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.orc_rb_colors.argtypes = [ctypes.c_void_p, _ip, _ip]
+        L.orc_rb_element.argtypes = [ctypes.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.orc_rb_aa_to_r.argtypes = [_dp, _dp]
+        L.orc_rb_r_to_aa.argtypes = [_dp, _dp]
         _lib = L
     return _lib
 
@@ -338,4 +354,85 @@ def from_scene(scene):
     for kind, st, raw in scene.energies:
         o.add_energy(kind, st, raw)
     o.prepare()
+    return o
+
+
+# ---- rigid bodies ----
+
+def rb_aa_to_r(theta):
+    t = np.ascontiguousarray(theta, dtype=np.float64)
+    r = np.zeros(9)
+    lib().orc_rb_aa_to_r(_d(t), _d(r))
+    return r.reshape(3, 3)
+
+
+def rb_r_to_aa(R):
+    r = np.ascontiguousarray(R, dtype=np.float64).reshape(9)
+    t = np.zeros(3)
+    lib().orc_rb_r_to_aa(_d(r), _d(t))
+    return t
+
+
+def rb_element(kind, com, rot, raw):
+    """Gradient (6 s) and Hessian (6 s x 6 s) of one rigid-body element at theta = 0."""
+    kind = RB_KIND.get(kind, kind)
+    ns = RB_STENCIL_SIZE[kind]
+    com = np.ascontiguousarray(com, dtype=np.float64).reshape(ns * 3)
+    rot = np.ascontiguousarray(rot, dtype=np.float64).reshape(ns * 9)
+    raw = np.ascontiguousarray(raw, dtype=np.float64).reshape(RB_RAW_PARAMS[kind])
+    g = np.zeros(6 * ns)
+    h = np.zeros((6 * ns, 6 * ns))
+    _check(lib().orc_rb_element(kind, _d(com), _d(rot), _d(raw), _d(g), _d(h)))
+    return g, h
+
+
+class RigidOracle:
+    """A RigidBodyGroup with its energies and the VBD step on it (FP64)."""
+
+    def __init__(self, x, theta, m, inertia):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+        self.nb = len(x)
+        theta = np.ascontiguousarray(theta, dtype=np.float64).reshape(self.nb, 3)
+        m = np.ascontiguousarray(m, dtype=np.float64).reshape(self.nb)
+        inertia = np.ascontiguousarray(inertia, dtype=np.float64).reshape(self.nb, 9)
+        self.h = lib().orc_rb_new(self.nb, _d(x), _d(theta), _d(m), _d(inertia))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_rb_free(self.h)
+            self.h = None
+
+    def add_energy(self, kind, stencils, raw):
+        kind = RB_KIND.get(kind, kind)
+        st = np.ascontiguousarray(stencils, dtype=np.int32).reshape(-1, RB_STENCIL_SIZE[kind])
+        raw = np.ascontiguousarray(raw, dtype=np.float64).reshape(len(st), RB_RAW_PARAMS[kind])
+        _check(lib().orc_rb_add_energy(self.h, kind, len(st), _i(st), _d(raw)))
+
+    def step(self, dt, nsubsteps=1, iterations=10, gravity=(0.0, -9.8, 0.0)):
+        g = np.asarray(gravity, dtype=np.float64)
+        _check(lib().orc_rb_step(self.h, dt, nsubsteps, iterations, _d(g)))
+
+    def state(self):
+        """(x, theta, v, omega), each (nb, 3)."""
+        out = [np.zeros((self.nb, 3)) for _ in range(4)]
+        _check(lib().orc_rb_state(self.h, *[_d(a) for a in out]))
+        return tuple(out)
+
+    def set_state(self, x=None, theta=None, v=None, omega=None):
+        cur = self.state()
+        arrs = [np.ascontiguousarray(a if a is not None else c, dtype=np.float64).reshape(self.nb, 3)
+                for a, c in zip((x, theta, v, omega), cur)]
+        _check(lib().orc_rb_set_state(self.h, *[_d(a) for a in arrs]))
+
+    def colors(self):
+        c = np.zeros(self.nb, dtype=np.int32)
+        n = np.zeros(1, dtype=np.int32)
+        _check(lib().orc_rb_colors(self.h, _i(c), _i(n)))
+        return c, int(n[0])
+
+
+def rb_from_scene(scene):
+    o = RigidOracle(scene.x, scene.theta, scene.m, scene.inertia)
+    for kind, st, raw in scene.energies:
+        o.add_energy(kind, st, raw)
     return o

@@ -24,12 +24,15 @@ xpbd       = the XPBD parallel Gauss-Seidel step (spg's XPBD(true)) on the same 
              per-type projection-kernel rooflines, and the FP64 oracle's XPBD step on the host cores
 further_energies = VBD on the same cloth built from the further energies (Baraff-Witkin membrane and bending,
              continuum edge springs), and the oracle's step on a 64x64 cloth on the host cores
+rigid_bodies = VBD on rigid bodies (FP64 6x6 blocks) on a 512x512 lattice of boxes joined by springs and
+             orientation alignments, with the colour-solve roofline and the oracle's step on a 64x64 lattice
 implicit   = the Baraff-Witkin implicit Euler step (FP64: element Hessians, block-sparse assembly, Jacobi CG with
              the reference's 1e-5 / 300-iteration settings) on the same cloth, the roofline of the symmetric
              block-sparse SpMV sampled inside the CG, and the oracle's step on a 128x128 cloth on the host cores
 
 Usage: python bench.py [--gpus 1] [--steps 200] [--warmup 10] [--seed 3] [--no-cpu] [--no-learned]
                        [--no-batched] [--no-volumetric] [--no-xpbd] [--xpbd-substeps 20] [--no-implicit]
+                       [--no-rigid] [--rigid-n 512]
 """
 import argparse
 import json
@@ -65,6 +68,8 @@ def main():
     ap.add_argument("--xpbd-substeps", type=int, default=20, help="XPBD substeps per step")
     ap.add_argument("--xpbd-steps", type=int, default=30, help="timed XPBD steps")
     ap.add_argument("--no-implicit", action="store_true", help="skip the implicit Euler measurement")
+    ap.add_argument("--no-rigid", action="store_true", help="skip the rigid-body VBD measurement")
+    ap.add_argument("--rigid-n", type=int, default=512, help="side of the rigid-body lattice")
     ap.add_argument("--implicit-steps", type=int, default=4, help="timed Baraff-Witkin steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
@@ -145,6 +150,7 @@ def main():
     further = None if args.no_further else further_throughput(args)
     xpbd = None if args.no_xpbd else xpbd_throughput(s, args)
     implicit = None if args.no_implicit else implicit_throughput(s, args)
+    rigid = None if args.no_rigid else rigid_throughput(args)
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
 
     cpu = None
@@ -184,6 +190,7 @@ def main():
         "further_energies": further,
         "xpbd": xpbd,
         "implicit": implicit,
+        "rigid_bodies": rigid,
     }
     print(json.dumps(out))
 
@@ -293,6 +300,61 @@ def further_throughput(args, steps=20):
         out["cpu_baseline"] = {"value": small.n * small.iterations * n / t, "unit": "vertex-updates/s", "cores": cores,
                                "kind": "port", "sample": f"{n} first steps (20 iterations) of the 64x64 cloth of the "
                                                          f"same construction, FP64 dual numbers, {t:.1f} s"}
+    return out
+
+
+def rigid_throughput(args, steps=20):
+    """VBD on rigid bodies (FP64, 6x6 blocks): an n x n lattice of unit boxes (n = --rigid-n, default 512:
+    262,144 bodies, 523,264 springs and as many orientation alignments, one anchored row, random tilts up to
+    0.05 rad), dt 1/60, 10 iterations per step over 2 colours, hipGraph replay. Roofline: the colour-solve kernel,
+    algorithmic bytes of one sweep (the body's state read and written, and per incident element its record and
+    the other body's centre and rotation) over the summed HIP-event durations of the sweep's launches.
+    CPU baseline: the oracle's step (dual numbers, as the reference's TinyAD) on a 64x64 lattice."""
+    import numpy as np
+    from spg_amd import native, scenes
+
+    sc = scenes.rb_lattice(args.rigid_n)
+    ctx = native.RigidContext([sc])
+    for _ in range(3):
+        ctx.step(sc.dt, 1, sc.iterations, sc.gravity)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.step(sc.dt, 1, sc.iterations, sc.gravity)
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    samples = [ctx.profile_substep(sc.dt, sc.iterations, sc.gravity) for _ in range(3)]
+    ms = min(m for m, _ in samples)
+    launches = samples[0][1]
+    sweep_ms = ms / sc.iterations
+    achieved = st["solve_bytes"] / (sweep_ms * 1e-3) / 1e9
+    x = ctx.state()[0]
+    out = {"workload": f"rigid-body lattice {args.rigid_n}x{args.rigid_n} ({sc.nb} boxes, {sc.meta['springs']} "
+                       f"springs, {sc.meta['alignments']} orientation alignments, {sc.meta['anchors']} anchors), "
+                       f"dt=1/60, {sc.iterations} iterations/step, FP64, hipGraph replay",
+           "value": sc.nb * sc.iterations * steps / wall, "unit": "body-updates/s", "steps": steps,
+           "ms_per_step": wall * 1e3 / steps, "colors": st["n_colors"],
+           "skipped_blocks": st["skipped_blocks"], "state_finite": bool(np.all(np.isfinite(x))),
+           "roofline": {"bound": "hbm", "kernel": "rbSolveKernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_sweep": st["solve_bytes"], "launches_per_sweep": launches // sc.iterations,
+                        "sweep_us": sweep_ms * 1e3, "mean_launch_us": ms * 1e3 / launches}}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as ol
+        cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+        ol.set_threads(cores)
+        small = scenes.rb_lattice(64)
+        n, t = 0, 0.0
+        while t < 5.0:
+            o = ol.rb_from_scene(small)
+            t0 = time.perf_counter()
+            o.step(small.dt, 1, small.iterations, small.gravity)
+            t += time.perf_counter() - t0
+            n += 1
+        out["cpu_baseline"] = {"value": small.nb * small.iterations * n / t, "unit": "body-updates/s",
+                               "cores": cores, "kind": "port",
+                               "sample": f"{n} first steps (10 iterations) of the 64x64 lattice of the same "
+                                         f"construction, FP64 dual numbers, {t:.1f} s"}
     return out
 
 

@@ -1,6 +1,6 @@
 """Build the native libraries in-tree.
 
-  spg_amd/lib/libspg_vbd.so   HIP kernels + C-ABI (include/spg_vbd.h, spg_vbd_learned.h, spg_xpbd.h, spg_implicit.h), hipcc for gfx950
+  spg_amd/lib/libspg_vbd.so   HIP kernels + C-ABI (include/spg_vbd.h, spg_vbd_learned.h, spg_xpbd.h, spg_implicit.h, spg_rigid.h), hipcc for gfx950
   spg_amd/lib/libspg_host.so  C++ mirror of spg's BaseSolver/ParticleGroup/Energy API (g++), over libspg_vbd.so
 
 Run:  python -m spg_amd.build     (or spg_amd.build.build() from Python)
@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-Wall", "-Wno-unused-result"]
 
 VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "mlp_context.hip", "warmstart_kernels.hip",
-               "xpbd_kernels.hip", "xpbd_context.hip", "implicit_kernels.hip", "implicit_context.hip", "coloring.cpp"]
+               "xpbd_kernels.hip", "xpbd_context.hip", "implicit_kernels.hip", "implicit_context.hip", "rigid_vbd.hip", "coloring.cpp"]
 HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "meshIo.cpp", "flatten.cpp", "vbdGpu.cpp", "xpbdGpu.cpp",
                 "implicitEulerGpu.cpp",
                 "host_capi.cpp"]

@@ -109,6 +109,27 @@ class ImplicitStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class RigidEnergy(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int32), ("n_stencils", ctypes.c_int64), ("stencils", _ip), ("params", _dp)]
+
+
+class RigidGroup(ctypes.Structure):
+    _fields_ = [("n_bodies", ctypes.c_int64), ("x", _dp), ("theta", _dp), ("v", _dp), ("omega", _dp),
+                ("mass", _dp), ("inertia", _dp), ("n_energies", ctypes.c_int32),
+                ("energies", ctypes.POINTER(RigidEnergy))]
+
+
+class RigidStats(ctypes.Structure):
+    _fields_ = [("n_colors", ctypes.c_int32), ("n_bodies", ctypes.c_int64), ("n_elements", ctypes.c_int64),
+                ("incidence_entries", ctypes.c_int64), ("skipped_blocks", ctypes.c_int64),
+                ("solve_bytes", ctypes.c_int64)]
+
+
+RIGID_KIND = {"rb_spring": 21, "rb_spring_squared": 22, "rb_anchor": 23, "rb_alignment": 24}
+RIGID_STENCIL = {21: 2, 22: 2, 23: 1, 24: 2}
+RIGID_PARAMS = {21: 8, 22: 8, 23: 7, 24: 1}
+
+
 class NativeError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"[{code}] {msg}")
@@ -218,6 +239,18 @@ def vbd_lib():
         _proto(L, "spg_implicit_reset_stats", ctypes.c_int, [c])
         _proto(L, "spg_implicit_set_timing", ctypes.c_int, [c, i32])
         _proto(L, "spg_implicit_synchronize", ctypes.c_int, [c])
+        # rigid bodies (include/spg_rigid.h)
+        _proto(L, "spg_rigid_create", ctypes.c_int, [ctypes.POINTER(c), ctypes.POINTER(RigidGroup), i32, i32])
+        _proto(L, "spg_rigid_destroy", None, [c])
+        _proto(L, "spg_rigid_last_error", ctypes.c_char_p, [c])
+        _proto(L, "spg_rigid_step", ctypes.c_int, [c, dbl, i32, i32, _dp])
+        _proto(L, "spg_rigid_get_state", ctypes.c_int, [c, i32, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_rigid_set_state", ctypes.c_int, [c, i32, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_rigid_get_colors", ctypes.c_int, [c, i32, _ip, _ip])
+        _proto(L, "spg_rigid_get_stats", ctypes.c_int, [c, ctypes.POINTER(RigidStats)])
+        _proto(L, "spg_rigid_set_graph", ctypes.c_int, [c, i32])
+        _proto(L, "spg_rigid_profile_substep", ctypes.c_int, [c, dbl, i32, _dp, _dp, _ip])
+        _proto(L, "spg_rigid_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, _dp, _dp, _dp, i32])
         _vbd = L
     return _vbd
 
@@ -1118,3 +1151,111 @@ class ImplicitContext:
 
     def synchronize(self):
         self._check(self._L.spg_implicit_synchronize(self.h))
+
+
+def _rigid_kind(k):
+    return RIGID_KIND.get(k, k)
+
+
+def rigid_element_derivatives(kind, com, rot, params, device=0):
+    """Closed-form gradient (n, 6 s) and per-body diagonal Hessian blocks (n, s, 6, 6) of rigid-body elements at
+    theta = 0; device -1 evaluates the same code on the host."""
+    t = _rigid_kind(kind)
+    ns, np_ = RIGID_STENCIL[t], RIGID_PARAMS[t]
+    com = _f64(com, (-1, 3 * ns))
+    n = com.shape[0]
+    rot = _f64(rot, (n, 9 * ns))
+    prm = _f64(params, (n, np_))
+    g = np.zeros((n, 6 * ns))
+    h = np.zeros((n, ns, 6, 6))
+    L = vbd_lib()
+    rc = L.spg_rigid_element_derivatives(t, n, _d(com), _d(rot), _d(prm), _d(g), _d(h), device)
+    if rc != 0:
+        raise NativeError(rc, L.spg_rigid_last_error(None).decode())
+    return g, h
+
+
+class RigidContext:
+    """A spg_rigid_ctx: RigidBodyGroups advanced together by VBD on one GPU (FP64)."""
+
+    def __init__(self, groups, device=0, graph=True):
+        """groups: objects with x, theta (nb, 3), m (nb,), inertia (nb, 3, 3), energies [(kind, stencils, args)]
+        and optionally v, omega (a scenes.RigidScene), or dicts with the same keys."""
+        self._L = vbd_lib()
+        get = (lambda o, k: o.get(k)) if isinstance(groups[0], dict) else (lambda o, k: getattr(o, k, None))
+        keep = []
+        gs = (RigidGroup * len(groups))()
+        self._nb = []
+        for k, o in enumerate(groups):
+            x = _f64(get(o, "x"), (-1, 3))
+            nb = x.shape[0]
+            th = _f64(get(o, "theta"), (nb, 3))
+            m = _f64(get(o, "m"), (nb,))
+            inertia = _f64(get(o, "inertia"), (nb, 9))
+            v = get(o, "v")
+            w = get(o, "omega")
+            v = None if v is None else _f64(v, (nb, 3))
+            w = None if w is None else _f64(w, (nb, 3))
+            energies = get(o, "energies")
+            ens = (RigidEnergy * max(1, len(energies)))()
+            for j, (kind, st, prm) in enumerate(energies):
+                t = _rigid_kind(kind)
+                st = _i32(st, (-1, RIGID_STENCIL[t]))
+                prm = _f64(prm, (st.shape[0], RIGID_PARAMS[t]))
+                ens[j] = RigidEnergy(t, st.shape[0], _i(st), _d(prm))
+                keep += [st, prm]
+            gs[k] = RigidGroup(nb, _d(x), _d(th), _d(v) if v is not None else None,
+                               _d(w) if w is not None else None, _d(m), _d(inertia), len(energies), ens)
+            keep += [x, th, m, inertia, v, w, ens]
+            self._nb.append(nb)
+        h = ctypes.c_void_p()
+        rc = self._L.spg_rigid_create(ctypes.byref(h), gs, len(groups), device)
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_rigid_last_error(None).decode())
+        self.h = h.value
+        if not graph:
+            self._L.spg_rigid_set_graph(self.h, 0)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_rigid_destroy(self.h)
+            self.h = None
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NativeError(rc, self._L.spg_rigid_last_error(self.h).decode())
+
+    def step(self, dt, nsubsteps=1, iterations=10, gravity=(0.0, -9.81, 0.0)):
+        g = np.asarray(gravity, dtype=np.float64)
+        self._check(self._L.spg_rigid_step(self.h, dt, nsubsteps, iterations, _d(g)))
+
+    def state(self, group=0):
+        """(x, theta, v, omega), each (nb, 3)."""
+        out = [np.zeros((self._nb[group], 3)) for _ in range(4)]
+        self._check(self._L.spg_rigid_get_state(self.h, group, *[_d(a) for a in out]))
+        return tuple(out)
+
+    def set_state(self, group=0, x=None, theta=None, v=None, omega=None):
+        nb = self._nb[group]
+        arrs = [None if a is None else _f64(a, (nb, 3)) for a in (x, theta, v, omega)]
+        self._check(self._L.spg_rigid_set_state(self.h, group, *[None if a is None else _d(a) for a in arrs]))
+
+    def colors(self, group=0):
+        c = np.zeros(self._nb[group], dtype=np.int32)
+        n = ctypes.c_int32(0)
+        self._check(self._L.spg_rigid_get_colors(self.h, group, _i(c), ctypes.byref(n)))
+        return c, n.value
+
+    def stats(self):
+        st = RigidStats()
+        self._check(self._L.spg_rigid_get_stats(self.h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in RigidStats._fields_}
+
+    def profile_substep(self, h, iterations, gravity=(0.0, -9.81, 0.0)):
+        """One eager substep with HIP events around each colour launch: (summed ms, launches)."""
+        g = np.asarray(gravity, dtype=np.float64)
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int32(0)
+        self._check(self._L.spg_rigid_profile_substep(self.h, h, iterations, _d(g), ctypes.byref(ms),
+                                                      ctypes.byref(n)))
+        return ms.value, n.value

@@ -28,6 +28,7 @@ def _declared(header):
 
 @pytest.mark.parametrize("header,lib", [("spg_vbd.h", native.VBD_LIB), ("spg_vbd_learned.h", native.VBD_LIB),
                                         ("spg_xpbd.h", native.VBD_LIB), ("spg_implicit.h", native.VBD_LIB),
+                                        ("spg_rigid.h", native.VBD_LIB),
                                         ("spg_host.h", native.HOST_LIB)])
 def test_exports(header, lib):
     native.host_lib()

@@ -98,6 +98,19 @@ RB_HD void aaToR(const double *t, double *R)
 
 // rotMatrixToAxisAngle (rigidBodyGroup.cpp:12-16): the unit quaternion of R (the trace, or the largest diagonal
 // entry, selects the numerically stable formula), then angle 2 atan2(|v|, |w|) about v / |v| with the sign of w.
+// Quaternion (w, x, y, z) of R when its largest diagonal entry R_II is used as the pivot; J, K follow I cyclically.
+template <int I>
+RB_HD void quatFromDiagonal(const double *R, double *q)
+{
+    constexpr int J = (I + 1) % 3, K = (J + 1) % 3;
+    double s = sqrt(R[4 * I] - R[4 * J] - R[4 * K] + 1.0);
+    q[1 + I] = 0.5 * s;
+    s = 0.5 / s;
+    q[0] = (R[3 * K + J] - R[3 * J + K]) * s;
+    q[1 + J] = (R[3 * J + I] + R[3 * I + J]) * s;
+    q[1 + K] = (R[3 * K + I] + R[3 * I + K]) * s;
+}
+
 RB_HD void rToAa(const double *R, double *t)
 {
     double q[4];  // w, x, y, z
@@ -110,16 +123,14 @@ RB_HD void rToAa(const double *R, double *t)
         q[2] = (R[2] - R[6]) * s;
         q[3] = (R[3] - R[1]) * s;
     } else {
-        int i = 0;
-        if (R[4] > R[0]) i = 1;
-        if (R[8] > R[4 * i]) i = 2;
-        const int j = (i + 1) % 3, k = (j + 1) % 3;
-        double s = sqrt(R[4 * i] - R[4 * j] - R[4 * k] + 1.0);
-        q[1 + i] = 0.5 * s;
-        s = 0.5 / s;
-        q[0] = (R[3 * k + j] - R[3 * j + k]) * s;
-        q[1 + j] = (R[3 * j + i] + R[3 * i + j]) * s;
-        q[1 + k] = (R[3 * k + i] + R[3 * i + k]) * s;
+        const int i0 = R[4] > R[0] ? 1 : 0;
+        const int i = R[8] > (i0 ? R[4] : R[0]) ? 2 : i0;
+        if (i == 0)
+            quatFromDiagonal<0>(R, q);
+        else if (i == 1)
+            quatFromDiagonal<1>(R, q);
+        else
+            quatFromDiagonal<2>(R, q);
     }
     double n = sqrt(q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
     if (n == 0) {
@@ -133,12 +144,26 @@ RB_HD void rToAa(const double *R, double *t)
     t[2] = q[3] / n * angle;
 }
 
+// Block storage of the accumulated 6x6 Hessian: the full row-major matrix (36 entries), or its upper triangle
+// (21 entries, row-major) when the Hessian is known to be symmetric; entries below the diagonal are dropped then.
+template <bool UPPER>
+RB_HD void addB(double *B, int i, int j, double v)
+{
+    if constexpr (UPPER) {
+        if (i <= j) B[i * 6 - i * (i - 1) / 2 + (j - i)] += v;
+    } else {
+        B[6 * i + j] += v;
+    }
+}
+
 // Adds J^T H_p J (J = [I, -[r]x]) into the 6x6 block B and J^T g into f with the given sign (the solve
 // accumulates forces, -gradient).
+template <bool UPPER>
 RB_HD void addPointTerms(const double *r, const double *gp, const double *Hp, double gsign, double *f, double *B)
 {
     double rg[3];
     cross3(r, gp, rg);
+#pragma unroll
     for (int d = 0; d < 3; ++d) {
         f[d] -= gsign * gp[d];
         f[3 + d] -= gsign * rg[d];
@@ -146,23 +171,28 @@ RB_HD void addPointTerms(const double *r, const double *gp, const double *Hp, do
     double S[9], HS[9], SH[9];
     skew3(r, S);
     // H (-S), S H, S H (-S)
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
             HS[3 * i + j] = -(Hp[3 * i] * S[j] + Hp[3 * i + 1] * S[3 + j] + Hp[3 * i + 2] * S[6 + j]);
             SH[3 * i + j] = S[3 * i] * Hp[j] + S[3 * i + 1] * Hp[3 + j] + S[3 * i + 2] * Hp[6 + j];
         }
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) {
-            B[6 * i + j] += Hp[3 * i + j];
-            B[6 * i + 3 + j] += HS[3 * i + j];
-            B[6 * (3 + i) + j] += SH[3 * i + j];
-            B[6 * (3 + i) + 3 + j] += -(SH[3 * i] * S[j] + SH[3 * i + 1] * S[3 + j] + SH[3 * i + 2] * S[6 + j]);
+            addB<UPPER>(B, i, j, Hp[3 * i + j]);
+            addB<UPPER>(B, i, 3 + j, HS[3 * i + j]);
+            addB<UPPER>(B, 3 + i, j, SH[3 * i + j]);
+            addB<UPPER>(B, 3 + i, 3 + j, -(SH[3 * i] * S[j] + SH[3 * i + 1] * S[3 + j] + SH[3 * i + 2] * S[6 + j]));
         }
 }
 
 // One element's force (-gradient) and diagonal Hessian block for the body in `slot`, accumulated into f and B.
 // x0, R0 (x1, R1): centre and rotation of the stencil's first (second) body; p: the element's parameters
 // (springs: l0, l1, L0 (squared spring: L0^2), k; anchor: l, a, k; alignment: k).
+template <bool UPPER = false>
 RB_HD void rbAccumulate(int type, int slot, const double *x0, const double *R0, const double *x1, const double *R1,
                         const double *p, double *f, double *B)
 {
@@ -172,7 +202,7 @@ RB_HD void rbAccumulate(int type, int slot, const double *x0, const double *R0, 
         const double k = p[6];
         const double gp[3] = {k * (x0[0] + r[0] - p[3]), k * (x0[1] + r[1] - p[4]), k * (x0[2] + r[2] - p[5])};
         const double Hp[9] = {k, 0, 0, 0, k, 0, 0, 0, k};
-        addPointTerms(r, gp, Hp, 1.0, f, B);
+        addPointTerms<UPPER>(r, gp, Hp, 1.0, f, B);
         return;
     }
     if (type == kRbAlignment) {
@@ -180,6 +210,7 @@ RB_HD void rbAccumulate(int type, int slot, const double *x0, const double *R0, 
         // (rigidBodyOrientationAlignmentEnergy.cpp:15-31). With c_k = u_k x w_k: dc_k/dt0 = [w_k]x [u_k]x,
         // dc_k/dt1 = -[u_k]x [w_k]x, and c_k is bilinear in (t0, t1), so d2C/dts2 = 2 sum_k J_k^T J_k.
         double C = 0, g[3] = {0, 0, 0}, M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
         for (int k = 0; k < 3; ++k) {
             const double u[3] = {R0[k], R0[3 + k], R0[6 + k]}, w[3] = {R1[k], R1[3 + k], R1[6 + k]};
             double c[3], Su[9], Sw[9], J[9];
@@ -191,17 +222,22 @@ RB_HD void rbAccumulate(int type, int slot, const double *x0, const double *R0, 
                 matMul3(Sw, Su, J);
             else {
                 matMul3(Su, Sw, J);
+#pragma unroll
                 for (int q = 0; q < 9; ++q) J[q] = -J[q];
             }
+#pragma unroll
             for (int i = 0; i < 3; ++i) {
                 g[i] += 2.0 * (J[i] * c[0] + J[3 + i] * c[1] + J[6 + i] * c[2]);
+#pragma unroll
                 for (int j = 0; j < 3; ++j) M[3 * i + j] += J[i] * J[j] + J[3 + i] * J[3 + j] + J[6 + i] * J[6 + j];
             }
         }
         const double k = p[0];
+#pragma unroll
         for (int i = 0; i < 3; ++i) {
             f[3 + i] -= k * C * g[i];
-            for (int j = 0; j < 3; ++j) B[6 * (3 + i) + 3 + j] += k * (g[i] * g[j] + 2.0 * C * M[3 * i + j]);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) addB<UPPER>(B, 3 + i, 3 + j, k * (g[i] * g[j] + 2.0 * C * M[3 * i + j]));
         }
         return;
     }
@@ -217,56 +253,66 @@ RB_HD void rbAccumulate(int type, int slot, const double *x0, const double *R0, 
         const double L = sqrt(dot3(d, d));
         const double n[3] = {d[0] / L, d[1] / L, d[2] / L};
         const double C = L - p[6], a = C / L;
+#pragma unroll
         for (int i = 0; i < 3; ++i) {
             gp[i] = k * C * n[i];
+#pragma unroll
             for (int j = 0; j < 3; ++j) Hp[3 * i + j] = k * ((1.0 - a) * n[i] * n[j] + (i == j ? a : 0.0));
         }
     } else {  // C = |d|^2 - L0^2
         const double C = dot3(d, d) - p[6];
+#pragma unroll
         for (int i = 0; i < 3; ++i) {
             gp[i] = 2.0 * k * C * d[i];
+#pragma unroll
             for (int j = 0; j < 3; ++j) Hp[3 * i + j] = k * (4.0 * d[i] * d[j] + (i == j ? 2.0 * C : 0.0));
         }
     }
-    addPointTerms(slot == 0 ? r0 : r1, gp, Hp, slot == 0 ? 1.0 : -1.0, f, B);
+    const double r[3] = {slot == 0 ? r0[0] : r1[0], slot == 0 ? r0[1] : r1[1], slot == 0 ? r0[2] : r1[2]};
+    addPointTerms<UPPER>(r, gp, Hp, slot == 0 ? 1.0 : -1.0, f, B);
 }
 
 // H dx = f by LU with partial pivoting; returns det(H) (0 when a pivot vanishes, dx then unset). H is destroyed.
+// The pivot search compares each lower row with the current pivot row and swaps on a strictly larger magnitude,
+// so every index is a compile-time constant once the loops unroll and the matrix stays in registers.
 RB_HD double solve6(double *H, const double *f, double *dx)
 {
     double b[6];
+#pragma unroll
     for (int i = 0; i < 6; ++i) b[i] = f[i];
     double det = 1.0;
+#pragma unroll
     for (int c = 0; c < 6; ++c) {
-        int piv = c;
-        double best = fabs(H[6 * c + c]);
-        for (int r = c + 1; r < 6; ++r)
-            if (fabs(H[6 * r + c]) > best) {
-                best = fabs(H[6 * r + c]);
-                piv = r;
+#pragma unroll
+        for (int r = c + 1; r < 6; ++r) {
+            const bool sw = fabs(H[6 * r + c]) > fabs(H[6 * c + c]);
+#pragma unroll
+            for (int q = c; q < 6; ++q) {
+                const double hc = H[6 * c + q], hr = H[6 * r + q];
+                H[6 * c + q] = sw ? hr : hc;
+                H[6 * r + q] = sw ? hc : hr;
             }
-        if (piv != c) {
-            for (int q = 0; q < 6; ++q) {
-                const double t = H[6 * c + q];
-                H[6 * c + q] = H[6 * piv + q];
-                H[6 * piv + q] = t;
-            }
-            const double t = b[c];
-            b[c] = b[piv];
-            b[piv] = t;
-            det = -det;
+            const double bc = b[c], br = b[r];
+            b[c] = sw ? br : bc;
+            b[r] = sw ? bc : br;
+            det = sw ? -det : det;
         }
         const double d = H[6 * c + c];
         det *= d;
         if (d == 0) return 0.0;
+        const double inv = 1.0 / d;
+#pragma unroll
         for (int r = c + 1; r < 6; ++r) {
-            const double fct = H[6 * r + c] / d;
-            for (int q = c; q < 6; ++q) H[6 * r + q] -= fct * H[6 * c + q];
+            const double fct = H[6 * r + c] * inv;
+#pragma unroll
+            for (int q = c + 1; q < 6; ++q) H[6 * r + q] -= fct * H[6 * c + q];
             b[r] -= fct * b[c];
         }
     }
+#pragma unroll
     for (int i = 5; i >= 0; --i) {
         double t = b[i];
+#pragma unroll
         for (int q = i + 1; q < 6; ++q) t -= H[6 * i + q] * dx[q];
         dx[i] = t / H[6 * i + i];
     }

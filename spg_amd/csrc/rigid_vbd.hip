@@ -2,12 +2,14 @@
 // built from spg RigidBodyGroups, and the C-ABI.
 //
 // Layout in HBM. Bodies are renumbered colour-major ("slots"): the bodies of colour 0 of every group first, then
-// colour 1, ..., each colour a contiguous slot range that one launch covers with one thread per body. Per-slot
-// state is stored as arrays of 3 or 9 doubles per body (x, v, omega, theta, xt (inertial prediction), xold,
-// thetaold in 3; R, world inertia, local inertia and its inverse, the rotation at the substep start in 9).
-// Elements keep their two body slots (int2) and 8 parameter doubles; a slot's incidence is a CSR row of
-// (element << 1 | stencil slot) entries in energy-major, ascending-stencil order, the order in which the reference
-// sums them (vbd.cpp:105-135 over computeStencilInfoPerVertex, vbd.cpp:231-268).
+// colour 1, ..., each colour a contiguous slot range that one launch covers (kRbLanes lanes per body). A slot's
+// pose (centre and rotation matrix, 12 doubles = 96 bytes, read with 16-byte loads) is the record its neighbours
+// gather; the rest of its state is stored as arrays of 3 or 9 doubles per body (v, omega, theta, xt (inertial
+// prediction), xold, thetaold in 3; local inertia, its inverse and the rotation at the substep start in 9). World
+// inertias R Il R^T are recomputed where used instead of stored. Elements keep 8 parameter doubles each; a slot's
+// incidence is a CSR row of (other body's slot, element << 3 | type << 1 | stencil slot) entries in energy-major,
+// ascending-stencil order, the order in which the reference sums them (vbd.cpp:105-135 over
+// computeStencilInfoPerVertex, vbd.cpp:231-268).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,13 +30,52 @@ using namespace spgrigid;
 
 namespace {
 
-constexpr int kRbBlock = 128;
+constexpr int kRbBlock = 256;
+// Lanes cooperating on one body's incidence row in the colour solve. One lane per body measured fastest on a
+// 512 x 512 lattice (8 incidence entries per body): 78.7 us per sweep, against 82 with 2 lanes and 115 with 4; the
+// shuffle reduction stays for rows long enough to want more.
+constexpr int kRbLanes = 1;
+constexpr int kPose = 12;    // per-slot pose record: centre (3) then rotation matrix (9, row-major), 96 bytes
 
 struct RbState {
-    double *x, *v, *w, *th, *xt, *xold, *thold;  // 3 per slot
-    double *R, *I, *Il, *IlInv, *Rold;           // 9 per slot
+    double *pose;                                // kPose per slot
+    double *v, *w, *th, *xt, *xold, *thold;      // 3 per slot
+    const double *Il, *IlInv;                    // 9 per slot: local inertia and its inverse
+    double *Rold;                                // 9 per slot: rotation at the substep start
     const double *m;
 };
+
+__device__ __forceinline__ void loadPose(const double *pose, int64_t b, double *x, double *R)
+{
+    const double2 *q = reinterpret_cast<const double2 *>(pose + kPose * b);
+    double v[12];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const double2 t = q[k];
+        v[2 * k] = t.x;
+        v[2 * k + 1] = t.y;
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = v[d];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = v[3 + k];
+}
+
+__device__ __forceinline__ void storePose(double *pose, int64_t b, const double *x, const double *R)
+{
+    double2 *q = reinterpret_cast<double2 *>(pose + kPose * b);
+    const double v[12] = {x[0], x[1], x[2], R[0], R[1], R[2], R[3], R[4], R[5], R[6], R[7], R[8]};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) q[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+
+// world inertia R Il R^T (RigidBodyGroup::updateInertia, rigidBodyGroup.cpp:195-199)
+__device__ __forceinline__ void worldInertia(const double *R, const double *Il, double *I)
+{
+    double T[9];
+    matMul3(R, Il, T);
+    matMulT3(T, R, I);
+}
 
 // Gyroscopic prediction (vbd.cpp:62-72), RigidBodyGroup::integrateVelocities with updateThetas/updateInertias
 // (rigidBodyGroup.cpp:97-113), and the substep-start copies (getPositions into xOld, vbd.cpp:47-55).
@@ -43,19 +84,17 @@ __global__ void __launch_bounds__(kRbBlock) rbBeginKernel(RbState s, int64_t nb,
 {
     const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    double x[3], v[3], w[3], R[9], I[9], Iinv[9], th[3];
+    double x[3], v[3], w[3], R[9], I[9], Iinv[9], th[3], Il[9];
+    loadPose(s.pose, b, x, R);
     for (int d = 0; d < 3; ++d) {
-        x[d] = s.x[3 * b + d];
         v[d] = s.v[3 * b + d];
         w[d] = s.w[3 * b + d];
         th[d] = s.th[3 * b + d];
         s.xold[3 * b + d] = x[d];
         s.thold[3 * b + d] = th[d];
     }
-    for (int k = 0; k < 9; ++k) {
-        R[k] = s.R[9 * b + k];
-        I[k] = s.I[9 * b + k];
-    }
+    for (int k = 0; k < 9; ++k) Il[k] = s.Il[9 * b + k];
+    worldInertia(R, Il, I);
     {
         double T[9];
         matMul3(R, s.IlInv + 9 * b, T);
@@ -81,23 +120,17 @@ __global__ void __launch_bounds__(kRbBlock) rbBeginKernel(RbState s, int64_t nb,
         matMul3(dR, R, Rn);
         for (int k = 0; k < 9; ++k) R[k] = Rn[k];
     }
-    double thn[3], Rold[9], T[9];
+    double thn[3], Rold[9];
     rToAa(R, thn);
     aaToR(th, Rold);
-    matMul3(R, s.Il + 9 * b, T);
-    matMulT3(T, R, I);
+    storePose(s.pose, b, x, R);
     for (int d = 0; d < 3; ++d) {
-        s.x[3 * b + d] = x[d];
         s.xt[3 * b + d] = x[d];
         s.v[3 * b + d] = v[d];
         s.w[3 * b + d] = w[d];
         s.th[3 * b + d] = thn[d];
     }
-    for (int k = 0; k < 9; ++k) {
-        s.R[9 * b + k] = R[k];
-        s.I[9 * b + k] = I[k];
-        s.Rold[9 * b + k] = Rold[k];
-    }
+    for (int k = 0; k < 9; ++k) s.Rold[9 * b + k] = Rold[k];
 }
 
 // omega from the rotation increment since the substep start: log(R Rold^T) / h
@@ -115,84 +148,119 @@ __device__ __forceinline__ void integratedOmega(const double *R, const double *R
 struct RbSolveArgs {
     RbState s;
     const int64_t *incPtr;
-    const int64_t *inc;       // element << 1 | slot
-    const int2 *elemBodies;   // body slots (second -1 for anchors)
-    const int32_t *elemType;
+    const int2 *inc;          // (other body's slot or -1, element << 3 | (type - 21) << 1 | stencil slot)
     const double *elemP;      // kRbParamStride per element
     unsigned long long *skipped;
     double h;
 };
 
 // One colour: the 6x6 block descent of every body in [begin, end) (vbd.cpp:101-145 with the RigidBodyGroup
-// branches) and updateElementPositionFromDx (rigidBodyGroup.cpp:64-77).
+// branches) and updateElementPositionFromDx (rigidBodyGroup.cpp:64-77). kRbLanes consecutive lanes share a body:
+// each sums every kRbLanes-th entry of its incidence row (lane 0 also the inertia terms) into a force and the
+// upper triangle of the block, the partial sums are combined by lane shuffles, and every lane of the group then
+// solves the same block; lane 0 writes the body.
 __global__ void __launch_bounds__(kRbBlock) rbSolveKernel(RbSolveArgs a, int64_t begin, int64_t end)
 {
     const int32_t blk = xcdContiguousBlock(int32_t(blockIdx.x), int32_t(gridDim.x));
-    const int64_t b = begin + int64_t(blk) * blockDim.x + threadIdx.x;
-    if (b >= end) return;
+    const int64_t t = int64_t(blk) * blockDim.x + threadIdx.x;
+    const int64_t b = begin + t / kRbLanes;
+    const int lane = int(t % kRbLanes);
+    if (b >= end) return;  // whole groups: kRbLanes divides the wavefront and the block
     const RbState &s = a.s;
     const double invh = 1.0 / a.h, invh2 = invh * invh;
-    double x[3], R[9], f[6], H[36];
-    for (int d = 0; d < 3; ++d) x[d] = s.x[3 * b + d];
-    for (int k = 0; k < 9; ++k) R[k] = s.R[9 * b + k];
-    {
+    double x[3], R[9], f[6], Hu[21];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) f[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 21; ++k) Hu[k] = 0.0;
+    loadPose(s.pose, b, x, R);
+    if (lane == 0) {
         const double mh2 = s.m[b] * invh2;
         for (int d = 0; d < 3; ++d) f[d] = -mh2 * (x[d] - s.xt[3 * b + d]);
-        double wi[3], I[9];
-        for (int k = 0; k < 9; ++k) I[k] = s.I[9 * b + k];
-        integratedOmega(R, s.Rold + 9 * b, invh, wi);
+        double wi[3], I[9], Il[9], Ro[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            Il[k] = s.Il[9 * b + k];
+            Ro[k] = s.Rold[9 * b + k];
+        }
+        worldInertia(R, Il, I);
+        integratedOmega(R, Ro, invh, wi);
         const double dw[3] = {wi[0] - s.w[3 * b], wi[1] - s.w[3 * b + 1], wi[2] - s.w[3 * b + 2]};
         double Idw[3];
         matVec3(I, dw, Idw);
-        for (int d = 0; d < 3; ++d) f[3 + d] = -invh * Idw[d];
-        for (int k = 0; k < 36; ++k) H[k] = 0.0;
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-            H[7 * d] = mh2;
-            for (int q = 0; q < 3; ++q) H[6 * (3 + d) + 3 + q] = invh2 * I[3 * d + q];
+            f[3 + d] = -invh * Idw[d];
+            addB<true>(Hu, d, d, mh2);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) addB<true>(Hu, 3 + d, 3 + q, invh2 * I[3 * d + q]);
         }
     }
     const int64_t e0 = a.incPtr[b], e1 = a.incPtr[b + 1];
-    for (int64_t j = e0; j < e1; ++j) {
-        const int64_t code = a.inc[j];
-        const int64_t e = code >> 1;
-        const int slot = int(code & 1);
-        const int type = a.elemType[e];
-        const int2 eb = a.elemBodies[e];
+    for (int64_t j = e0 + lane; j < e1; j += kRbLanes) {
+        const int2 en = a.inc[j];
+        const int64_t e = int64_t(uint32_t(en.y) >> 3);
+        const int type = kRbSpring + ((en.y >> 1) & 3);
+        const int slot = en.y & 1;
+        const double *pp = a.elemP + kRbParamStride * e;
         double p[kRbParamStride];
-        for (int q = 0; q < kRbParamStride; ++q) p[q] = a.elemP[kRbParamStride * e + q];
+        if (type == kRbAlignment) {
+            p[0] = pp[0];
+        } else {
+            const double2 *p2 = reinterpret_cast<const double2 *>(pp);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double2 v = p2[q];
+                p[2 * q] = v.x;
+                p[2 * q + 1] = v.y;
+            }
+        }
         if (type == kRbAnchor) {
-            rbAccumulate(type, 0, x, R, x, R, p, f, H);
+            rbAccumulate<true>(type, 0, x, R, x, R, p, f, Hu);
             continue;
         }
-        const int64_t o = slot == 0 ? eb.y : eb.x;
         double xo[3], Ro[9];
-        for (int d = 0; d < 3; ++d) xo[d] = s.x[3 * o + d];
-        for (int k = 0; k < 9; ++k) Ro[k] = s.R[9 * o + k];
+        loadPose(s.pose, en.x, xo, Ro);
         if (slot == 0)
-            rbAccumulate(type, 0, x, R, xo, Ro, p, f, H);
+            rbAccumulate<true>(type, 0, x, R, xo, Ro, p, f, Hu);
         else
-            rbAccumulate(type, 1, xo, Ro, x, R, p, f, H);
+            rbAccumulate<true>(type, 1, xo, Ro, x, R, p, f, Hu);
     }
+#pragma unroll
+    for (int m = 1; m < kRbLanes; m <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) f[k] += __shfl_xor(f[k], m, kRbLanes);
+#pragma unroll
+        for (int k = 0; k < 21; ++k) Hu[k] += __shfl_xor(Hu[k], m, kRbLanes);
+    }
+    double H[36];
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+            const double v = Hu[i * 6 - i * (i - 1) / 2 + (j - i)];
+            H[6 * i + j] = v;
+            H[6 * j + i] = v;
+        }
     double dx[6];
-    if (solve6(H, f, dx) > 1e-10) {
-        for (int d = 0; d < 3; ++d) s.x[3 * b + d] = x[d] + dx[d];
+    const double det = solve6(H, f, dx);
+    if (lane != 0) return;
+    if (det > 1e-10) {
+        double xn[3];
+        for (int d = 0; d < 3; ++d) xn[d] = x[d] + dx[d];
         const double dn = sqrt(dx[3] * dx[3] + dx[4] * dx[4] + dx[5] * dx[5]);
         if (dn != 0) {
             const double ax[3] = {dx[3] / dn, dx[4] / dn, dx[5] / dn};
             double dR[9], Rn[9];
             angleAxisToR(dn, ax, dR);
             matMul3(dR, R, Rn);
+#pragma unroll
             for (int k = 0; k < 9; ++k) R[k] = Rn[k];
         }
-        double th[3], T[9], I[9];
+        double th[3];
         rToAa(R, th);
-        matMul3(R, s.Il + 9 * b, T);
-        matMulT3(T, R, I);
+        storePose(s.pose, b, xn, R);
         for (int d = 0; d < 3; ++d) s.th[3 * b + d] = th[d];
-        for (int k = 0; k < 9; ++k) {
-            s.R[9 * b + k] = R[k];
-            s.I[9 * b + k] = I[k];
-        }
     } else {
         atomicAdd(a.skipped, 1ull);
     }
@@ -203,9 +271,11 @@ __global__ void __launch_bounds__(kRbBlock) rbEndKernel(RbState s, int64_t nb, d
 {
     const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    for (int d = 0; d < 3; ++d) s.v[3 * b + d] = (s.x[3 * b + d] - s.xold[3 * b + d]) * invh;
+    double x[3], R[9];
+    loadPose(s.pose, b, x, R);
+    for (int d = 0; d < 3; ++d) s.v[3 * b + d] = (x[d] - s.xold[3 * b + d]) * invh;
     double w[3];
-    integratedOmega(s.R + 9 * b, s.Rold + 9 * b, invh, w);
+    integratedOmega(R, s.Rold + 9 * b, invh, w);
     for (int d = 0; d < 3; ++d) s.w[3 * b + d] = w[d];
 }
 
@@ -242,10 +312,9 @@ struct spg_rigid_ctx {
     std::vector<std::vector<int32_t>> colors;  // per group, per body
     std::vector<int64_t> colorStart;         // slot range of colour c: [colorStart[c], colorStart[c + 1])
     int64_t incEntries = 0, solveBytes = 0;
-    DevBuf<double> x, v, w, th, xt, xold, thold, R, I, Il, IlInv, Rold, m, elemP;
-    DevBuf<int64_t> incPtr, inc;
-    DevBuf<int2> elemBodies;
-    DevBuf<int32_t> elemType;
+    DevBuf<double> pose, v, w, th, xt, xold, thold, Il, IlInv, Rold, m, elemP;
+    DevBuf<int64_t> incPtr;
+    DevBuf<int2> inc;
     DevBuf<unsigned long long> skipped;
     bool useGraph = true;
     hipGraphExec_t graphExec = nullptr;
@@ -259,11 +328,11 @@ struct spg_rigid_ctx {
     }
     RbState state() const
     {
-        return RbState{x.p, v.p, w.p, th.p, xt.p, xold.p, thold.p, R.p, I.p, Il.p, IlInv.p, Rold.p, m.p};
+        return RbState{pose.p, v.p, w.p, th.p, xt.p, xold.p, thold.p, Il.p, IlInv.p, Rold.p, m.p};
     }
     RbSolveArgs solveArgs(double h) const
     {
-        return RbSolveArgs{state(), incPtr.p, inc.p, elemBodies.p, elemType.p, elemP.p, skipped.p, h};
+        return RbSolveArgs{state(), incPtr.p, inc.p, elemP.p, skipped.p, h};
     }
 };
 
@@ -307,16 +376,6 @@ void inverse3(const double *m, double *r)
     r[6] = c02 * id;
     r[7] = (m[1] * m[6] - m[0] * m[7]) * id;
     r[8] = (m[0] * m[4] - m[1] * m[3]) * id;
-}
-
-// Host mirror of the slot state: rotation and world inertia from theta (setPositions, rigidBodyGroup.cpp:41-51).
-void bodyFrames(const double *th, const double *Il, const double *IlInv, double *R, double *I)
-{
-    aaToR(th, R);
-    double T[9];
-    matMul3(R, Il, T);
-    matMulT3(T, R, I);
-    (void)IlInv;
 }
 
 void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGroups)
@@ -401,8 +460,9 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
     }
     // per-slot host arrays
     const int64_t nb = c.nb;
-    std::vector<double> hx(3 * nb), hv(3 * nb, 0.0), hw(3 * nb, 0.0), hth(3 * nb), hm(nb), hR(9 * nb), hI(9 * nb),
-        hIl(9 * nb), hIli(9 * nb);
+    if (c.nElem >= (int64_t(1) << 29)) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many rigid-body elements");
+    std::vector<double> hpose(kPose * nb), hv(3 * nb, 0.0), hw(3 * nb, 0.0), hth(3 * nb), hm(nb), hIl(9 * nb),
+        hIli(9 * nb);
     for (int32_t g = 0; g < nGroups; ++g) {
         const auto &G = groups[g];
         for (int64_t i = 0; i < G.n_bodies; ++i) {
@@ -412,7 +472,7 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
                 (G.omega && !finiteN(G.omega + 3 * i, 3)))
                 throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite body data");
             for (int d = 0; d < 3; ++d) {
-                hx[3 * s + d] = G.x[3 * i + d];
+                hpose[kPose * s + d] = G.x[3 * i + d];
                 hth[3 * s + d] = G.theta[3 * i + d];
                 if (G.v) hv[3 * s + d] = G.v[3 * i + d];
                 if (G.omega) hw[3 * s + d] = G.omega[3 * i + d];
@@ -420,53 +480,44 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
             hm[s] = G.mass[i];
             for (int k = 0; k < 9; ++k) hIl[9 * s + k] = G.inertia[9 * i + k];
             inverse3(G.inertia + 9 * i, hIli.data() + 9 * s);
-            bodyFrames(hth.data() + 3 * s, hIl.data() + 9 * s, hIli.data() + 9 * s, hR.data() + 9 * s,
-                       hI.data() + 9 * s);
+            aaToR(hth.data() + 3 * s, hpose.data() + kPose * s + 3);  // setPositions: R from theta
         }
     }
-    // elements and incidence rows in slot order
-    std::vector<int2> eb(c.nElem);
-    std::vector<int32_t> et(c.nElem);
+    // element parameters and incidence rows in slot order
     std::vector<double> ep(kRbParamStride * c.nElem);
-    for (int64_t e = 0; e < c.nElem; ++e) {
-        const auto &el = elems[e];
-        eb[e] = make_int2(int32_t(c.slotOf[el.b0]), el.b1 >= 0 ? int32_t(c.slotOf[el.b1]) : -1);
-        et[e] = el.type;
-        std::copy(el.p, el.p + kRbParamStride, ep.begin() + kRbParamStride * e);
-    }
-    std::vector<int64_t> ptr(nb + 1, 0), inc;
+    for (int64_t e = 0; e < c.nElem; ++e) std::copy(elems[e].p, elems[e].p + kRbParamStride, ep.begin() + kRbParamStride * e);
+    std::vector<int64_t> ptr(nb + 1, 0);
+    std::vector<int2> inc;
     std::vector<int64_t> bodyOfSlot(nb);
     for (int64_t b = 0; b < nb; ++b) bodyOfSlot[c.slotOf[b]] = b;
     int64_t bytes = 0;
     for (int64_t s = 0; s < nb; ++s) {
         const auto &row = bodyInc[bodyOfSlot[s]];
         for (const auto &pr : row) {
-            inc.push_back(pr.first << 1 | pr.second);
-            const int t = elems[pr.first].type;
-            bytes += 8 + 8 + 4 + 8 * rbParams(t) + (t == kRbAnchor ? 0 : 96);
+            const auto &el = elems[pr.first];
+            const int64_t other = el.b1 < 0 ? -1 : c.slotOf[pr.second == 0 ? el.b1 : el.b0];
+            inc.push_back(make_int2(int32_t(other), int32_t((pr.first << 3) | ((el.type - kRbSpring) << 1) | pr.second)));
+            // the entry, the parameters it uses and the other body's pose
+            bytes += 8 + 8 * rbParams(el.type) + (el.type == kRbAnchor ? 0 : 8 * kPose);
         }
         ptr[s + 1] = static_cast<int64_t>(inc.size());
-        // own state read (x, R, xt, Rold, omega, I, m, row pointers) and written (x, R, theta, I)
-        bytes += 24 + 72 + 24 + 72 + 24 + 72 + 8 + 16 + 24 + 72 + 24 + 72 + 72;
+        // read: pose, row pointers, mass, xt, omega, local inertia, Rold; written: pose, theta
+        bytes += 8 * kPose + 16 + 8 + 24 + 24 + 72 + 72 + 8 * kPose + 24;
     }
     c.incEntries = static_cast<int64_t>(inc.size());
     c.solveBytes = bytes;
     auto up = [&](DevBuf<double> &d, const std::vector<double> &h) { d.upload(h, c.stream); };
-    up(c.x, hx);
+    up(c.pose, hpose);
     up(c.v, hv);
     up(c.w, hw);
     up(c.th, hth);
     up(c.m, hm);
-    up(c.R, hR);
-    up(c.I, hI);
     up(c.Il, hIl);
     up(c.IlInv, hIli);
     c.xt.alloc(3 * nb);
     c.xold.alloc(3 * nb);
     c.thold.alloc(3 * nb);
     c.Rold.alloc(9 * nb);
-    c.elemBodies.upload(eb, c.stream);
-    c.elemType.upload(et, c.stream);
     c.elemP.upload(ep, c.stream);
     c.incPtr.upload(ptr, c.stream);
     c.inc.upload(inc, c.stream);
@@ -493,7 +544,8 @@ void enqueueSubstep(spg_rigid_ctx &c, double h, int32_t iterations, const double
             const int64_t b0 = c.colorStart[col], b1 = c.colorStart[col + 1];
             if (b1 == b0) continue;
             if (events) hipCheck(hipEventRecord((*events)[ev].first, c.stream), "hipEventRecord");
-            hipLaunchKernelGGL(rbSolveKernel, dim3(blocksFor(b1 - b0)), dim3(kRbBlock), 0, c.stream, a, b0, b1);
+            hipLaunchKernelGGL(rbSolveKernel, dim3(blocksFor(kRbLanes * (b1 - b0))), dim3(kRbBlock), 0, c.stream, a, b0,
+                               b1);
             hipCheck(hipGetLastError(), "solve launch");
             if (events) hipCheck(hipEventRecord((*events)[ev++].second, c.stream), "hipEventRecord");
         }
@@ -638,19 +690,20 @@ int spg_rigid_get_state(spg_rigid_ctx *ctx, int32_t group, double *x, double *th
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         const int64_t b0 = ctx->groupOffset[group], n = ctx->groupOffset[group + 1] - b0;
         if (n == 0) return;
-        std::vector<double> buf(3 * ctx->nb);
-        auto fetch = [&](const DevBuf<double> &src, double *dst) {
+        // fetch(src, width, offset, dst): 3 doubles at `offset` of each width-double slot record
+        auto fetch = [&](const DevBuf<double> &src, int width, int offset, double *dst) {
             if (!dst) return;
+            std::vector<double> buf(width * ctx->nb);
             hipCheck(hipMemcpyAsync(buf.data(), src.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
                      "download");
             hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
             for (int64_t i = 0; i < n; ++i)
-                for (int d = 0; d < 3; ++d) dst[3 * i + d] = buf[3 * ctx->slotOf[b0 + i] + d];
+                for (int d = 0; d < 3; ++d) dst[3 * i + d] = buf[width * ctx->slotOf[b0 + i] + offset + d];
         };
-        fetch(ctx->x, x);
-        fetch(ctx->th, theta);
-        fetch(ctx->v, v);
-        fetch(ctx->w, omega);
+        fetch(ctx->pose, kPose, 0, x);
+        fetch(ctx->th, 3, 0, theta);
+        fetch(ctx->v, 3, 0, v);
+        fetch(ctx->w, 3, 0, omega);
     });
 }
 
@@ -666,35 +719,28 @@ int spg_rigid_set_state(spg_rigid_ctx *ctx, int32_t group, const double *x, cons
         if (n == 0) return;
         const size_t nb = static_cast<size_t>(ctx->nb);
         auto sync = [&] { hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); };
-        auto scatter = [&](DevBuf<double> &dst, const double *src, int width, const std::vector<double> *vals) {
+        // edit(dst, width, fill): read-modify-write of the slot records of this group
+        auto edit = [&](DevBuf<double> &dst, int width, auto &&fill) {
             std::vector<double> buf(width * nb);
             hipCheck(hipMemcpyAsync(buf.data(), dst.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
                      "download");
             sync();
-            for (int64_t i = 0; i < n; ++i)
-                for (int d = 0; d < width; ++d)
-                    buf[width * ctx->slotOf[b0 + i] + d] = vals ? (*vals)[width * i + d] : src[width * i + d];
+            for (int64_t i = 0; i < n; ++i) fill(i, buf.data() + width * ctx->slotOf[b0 + i]);
             hipCheck(hipMemcpyAsync(dst.p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream),
                      "upload");
             sync();
         };
-        if (x) scatter(ctx->x, x, 3, nullptr);
-        if (v) scatter(ctx->v, v, 3, nullptr);
-        if (omega) scatter(ctx->w, omega, 3, nullptr);
-        if (theta) {
-            // setPositions: rotation matrices and world inertias follow theta
-            std::vector<double> Il(9 * nb);
-            hipCheck(hipMemcpyAsync(Il.data(), ctx->Il.p, Il.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream),
-                     "download");
-            sync();
-            std::vector<double> Rs(9 * n), Is(9 * n);
-            for (int64_t i = 0; i < n; ++i)
-                bodyFrames(theta + 3 * i, Il.data() + 9 * ctx->slotOf[b0 + i], nullptr, Rs.data() + 9 * i,
-                           Is.data() + 9 * i);
-            scatter(ctx->th, theta, 3, nullptr);
-            scatter(ctx->R, nullptr, 9, &Rs);
-            scatter(ctx->I, nullptr, 9, &Is);
-        }
+        auto copy3 = [](const double *src) {
+            return [src](int64_t i, double *rec) { std::copy(src + 3 * i, src + 3 * i + 3, rec); };
+        };
+        if (v) edit(ctx->v, 3, copy3(v));
+        if (omega) edit(ctx->w, 3, copy3(omega));
+        if (theta) edit(ctx->th, 3, copy3(theta));
+        if (x || theta)  // setPositions: the rotation matrix follows theta (world inertias derive from it)
+            edit(ctx->pose, kPose, [&](int64_t i, double *rec) {
+                if (x) std::copy(x + 3 * i, x + 3 * i + 3, rec);
+                if (theta) aaToR(theta + 3 * i, rec + 3);
+            });
     });
 }
 

@@ -6,6 +6,8 @@
  *   spg_pg_*          spg::ParticleGroup (spg/include/spg/sim/simObject/particleGroup.h:14-60) with
  *                     addParticle (particleGroup.cpp:74-82), addEnergy and each energy's addStencil +
  *                     preparePrecomputations (the five energy headers under spg/include/spg/sim/energy/).
+ *   spg_rb_*          spg::RigidBodyGroup (spg/include/spg/sim/simObject/rigidBodyGroup.h:11-86) with addBody
+ *                     (rigidBodyGroup.cpp:136-161), addEnergy and the four rigid-body energies' addStencil.
  *   spg_trimesh_*     spg::TriangleMesh edge/opposing-vertex enumeration (spg/include/spg/geom/triangleMesh.h:16-50)
  *                     and the cloth bending wiring of apps/solverCompare.cpp:441-452.
  *   spg_solver_*      spg::solver::BaseSolver (spg/include/spg/sim/solver/baseSolver.h:14-41) implemented by
@@ -25,6 +27,7 @@
 
 #include "spg_vbd.h"
 #include "spg_implicit.h"
+#include "spg_rigid.h"
 #include "spg_xpbd.h"
 
 #ifdef __cplusplus
@@ -32,6 +35,7 @@ extern "C" {
 #endif
 
 typedef struct spg_pgroup spg_pgroup;
+typedef struct spg_rbgroup spg_rbgroup;
 typedef struct spg_solver spg_solver;
 
 const char *spg_host_last_error(void);
@@ -57,6 +61,23 @@ int spg_pg_energy_xpbd_params(const spg_pgroup *pg, int32_t index, double *param
 int spg_pg_inv_masses(const spg_pgroup *pg, double *w);
 int spg_pg_get_state(const spg_pgroup *pg, double *x, double *v);
 int spg_pg_set_state(spg_pgroup *pg, const double *x, const double *v);
+
+/* spg::RigidBodyGroup (spg/include/spg/sim/simObject/rigidBodyGroup.h:11-86): addBody(x[i], x0[i], mass[i],
+   theta[i], theta0[i], inertia[i], empty mesh) for i < n (x0 / theta0 may be NULL: the same as x / theta;
+   inertia: local tensors, 9 per body, row-major). */
+spg_rbgroup *spg_rb_create(void);
+void spg_rb_destroy(spg_rbgroup *rb);
+int spg_rb_add_bodies(spg_rbgroup *rb, int64_t n, const double *x, const double *x0, const double *mass,
+                      const double *theta, const double *theta0, const double *inertia);
+/* One rigid-body energy of `type` (SPG_RIGID_* of spg_rigid.h), addStencil per stencil with the arguments of
+   spg_rigid_energy, then addEnergy; *index (may be NULL) receives its position. */
+int spg_rb_add_energy(spg_rbgroup *rb, int32_t type, int64_t n, const int32_t *stencils, const double *args,
+                      int32_t *index);
+int spg_rb_num_bodies(const spg_rbgroup *rb, int64_t *n);
+/* positions(), thetas(), velocities(), omegas(), rotationMatrices() (9 per body); any pointer may be NULL */
+int spg_rb_get_state(const spg_rbgroup *rb, double *x, double *theta, double *v, double *omega, double *rot);
+/* setPositions / setVelocities: the given quantities (NULL keeps the current ones); rotations follow theta */
+int spg_rb_set_state(spg_rbgroup *rb, const double *x, const double *theta, const double *v, const double *omega);
 
 /* spg::io::loadObj / loadMsh (source/geom/io/obj.cpp:10-35, msh.cpp:10-41): a mesh file read in two calls,
    first with NULL outputs for the counts, then with arrays of those sizes (vertices 3 doubles each, faces
@@ -84,6 +105,14 @@ int spg_solver_step(spg_solver *s);
 int spg_solver_reset(spg_solver *s);
 int spg_solver_sync_to_host(spg_solver *s);
 int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v);
+/* BaseSolver::addObject(const RigidBodyGroup &) (baseSolver.cpp:10-14): copies the group, shares its energies.
+   Only VBDGpu steps rigid-body groups; the other solvers throw at step(). */
+int spg_solver_add_rb_object(spg_solver *s, const spg_rbgroup *rb);
+/* rbGroups()[object]: positions, thetas, velocities, omegas, rotation matrices (any pointer may be NULL) */
+int spg_solver_get_rb_state(spg_solver *s, int32_t object, double *x, double *theta, double *v, double *omega,
+                            double *rot);
+/* VBDGpu's rigid-body context (include/spg_rigid.h), NULL before the first step or without rigid bodies. */
+spg_rigid_ctx *spg_solver_rigid_context(spg_solver *s);
 /* The C-ABI context behind the solver (NULL before the first step, or for the other solver type). */
 spg_vbd_ctx *spg_solver_context(spg_solver *s);
 spg_xpbd_ctx *spg_solver_xpbd_context(spg_solver *s);

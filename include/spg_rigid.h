@@ -21,6 +21,7 @@
  *   spg_rigid_get_state / set_state  RigidBodyGroup::getPositions/getVelocities and setPositions/setVelocities
  *                         (rigidBodyGroup.cpp:21-62): positions are (x, theta) with theta the axis-angle vector of
  *                         the rotation; setting them rebuilds rotation matrices and world inertias.
+ *   spg_rigid_get_rotations / set_rotations  RigidBodyGroup::rotationMatrices() (rigidBodyGroup.h:45-46).
  *   spg_rigid_get_colors  the colour groups of VBD::computeParallelVertexGroups (vbd.cpp:203-229).
  *   spg_rigid_element_derivatives  StencilBlockEnergy::energyGradient / energyHessian of one rigid-body element
  *                         (spg/include/spg/sim/energy/energy.h:268-270; the anchor's own overrides,
@@ -92,6 +93,13 @@ int spg_rigid_step(spg_rigid_ctx *ctx, double dt, int32_t nsubsteps, int32_t ite
 int spg_rigid_get_state(spg_rigid_ctx *ctx, int32_t group, double *x, double *theta, double *v, double *omega);
 int spg_rigid_set_state(spg_rigid_ctx *ctx, int32_t group, const double *x, const double *theta, const double *v,
                         const double *omega);
+
+/* RigidBodyGroup::rotationMatrices() (rigidBodyGroup.h:45-46): 9 doubles per body, row-major. The rotation is
+   state of its own: the solver composes it with each increment and derives theta from it, so a caller that keeps
+   the state on the host reads and writes it beside theta to continue bit for bit. set_rotations leaves theta as it
+   is (like assigning rotationMatrices() without updateThetas()). */
+int spg_rigid_get_rotations(spg_rigid_ctx *ctx, int32_t group, double *R);
+int spg_rigid_set_rotations(spg_rigid_ctx *ctx, int32_t group, const double *R);
 
 int spg_rigid_get_colors(const spg_rigid_ctx *ctx, int32_t group, int32_t *colors, int32_t *n_colors);
 int spg_rigid_get_stats(spg_rigid_ctx *ctx, spg_rigid_stats *stats);

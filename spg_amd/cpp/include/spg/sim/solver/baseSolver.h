@@ -9,9 +9,11 @@
 #include <vector>
 
 #include <spg/sim/simObject/particleGroup.h>
+#include <spg/sim/simObject/rigidBodyGroup.h>
 #include <spg/types.h>
 
 struct spg_vbd_ctx;
+struct spg_rigid_ctx;
 struct spg_xpbd_ctx;
 struct spg_implicit_ctx;
 
@@ -28,9 +30,12 @@ public:
     virtual void step() = 0;
     // copies the group; its energies stay shared (baseSolver.cpp:10-14)
     void addObject(const ParticleGroup &object) { m_objects.push_back(object); }
+    void addObject(const RigidBodyGroup &object) { m_rbObjects.push_back(object); }
     std::vector<ParticleGroup> &particleGroups() { return m_objects; }
     const std::vector<ParticleGroup> &particleGroups() const { return m_objects; }
-    int numSimObjects() const { return static_cast<int>(m_objects.size()); }
+    std::vector<RigidBodyGroup> &rbGroups() { return m_rbObjects; }
+    const std::vector<RigidBodyGroup> &rbGroups() const { return m_rbObjects; }
+    int numSimObjects() const { return static_cast<int>(m_objects.size() + m_rbObjects.size()); }
     void setDt(Real dt) { m_dtStep = dt; }
     Real dt() const { return m_dtStep; }
     void setNumSubsteps(int nsubsteps) { m_nsubsteps = nsubsteps; }
@@ -40,7 +45,11 @@ public:
     void setGravity(const Vector3 &gravity) { m_gravity = gravity; }
 
 protected:
+    // rejects rigid-body groups in the solvers that do not handle them
+    void requireNoRigidBodies(const char *solverName) const;
+
     std::vector<ParticleGroup> m_objects;
+    std::vector<RigidBodyGroup> m_rbObjects;
     Vector3 m_gravity{0, -9.8, 0};
     Real m_time{0};
     Real m_dtStep{0.01};
@@ -52,9 +61,11 @@ protected:
 // call step() in a loop. The first step (or the first after reset()/requirePrecomputationUpdate())
 // colours the graph, builds the incidence streams and uploads them (vbd.cpp:14-18).
 //
-// Every object's energies must be among SpringEnergy, MembraneStvkEnergy, DiscreteBendingEnergy,
-// StableNeoHookeanEnergy and SpringAnchorEnergy; any other energy type throws std::runtime_error at
-// the first step (there is no CPU fallback).
+// Every particle group's energies must be among the thirteen particle energies of spg_vbd.h, and every
+// RigidBodyGroup's among the four rigid-body energies of spg_rigid.h; any other energy type throws
+// std::runtime_error at the first step (there is no CPU fallback). Rigid-body groups run in their own FP64
+// context (include/spg_rigid.h), stepped with the same dt, substeps, iterations and gravity; no energy couples
+// the two kinds of object, so the order between them does not change the result.
 //
 // State ownership: by default the host ParticleGroups stay the source of truth, as with the CPU
 // solver: step() uploads their positions and velocities, advances on the GPU and writes the result
@@ -82,10 +93,15 @@ public:
     void markHostStateDirty() { m_hostDirty = true; }
     // The C-ABI context (include/spg_vbd.h) behind this solver, or nullptr before the first step.
     spg_vbd_ctx *context() { return m_ctx; }
+    // The rigid-body context (include/spg_rigid.h), or nullptr before the first step / without rigid bodies.
+    spg_rigid_ctx *rigidContext() { return m_rbCtx; }
 
 protected:
     void buildContext();
     void uploadState();
+    void buildRigidContext();
+    void uploadRigidState();
+    void syncRigidToHost();
 
     InitialGuessType m_initialGuessType{InitialGuessType::InertialWithAcceleration};
     int m_iterations{1};
@@ -94,6 +110,7 @@ protected:
     bool m_resident{false};
     bool m_hostDirty{true};
     spg_vbd_ctx *m_ctx{nullptr};
+    spg_rigid_ctx *m_rbCtx{nullptr};
 };
 
 // XPBD on one MI355X, the GPU drop-in for spg::solver::XPBD: XPBDGpu(true) is the parallel Gauss-Seidel

@@ -1,4 +1,6 @@
 // C binding of the C++ spg mirror (include/spg_host.h).
+#include <spg/sim/energy/rigidBodyEnergies.h>
+#include <spg/sim/simObject/rigidBodyGroup.h>
 #include <algorithm>
 #include <memory>
 #include <string>
@@ -14,6 +16,10 @@
 
 struct spg_pgroup {
     spg::ParticleGroup group;
+};
+
+struct spg_rbgroup {
+    spg::RigidBodyGroup group;
 };
 
 struct spg_solver {
@@ -365,7 +371,8 @@ int spg_solver_sync_to_host(spg_solver *s)
 int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v)
 {
     return guarded([&] {
-        if (!s || object < 0 || object >= s->solver->numSimObjects()) throw std::runtime_error("invalid object");
+        if (!s || object < 0 || object >= static_cast<int32_t>(s->solver->particleGroups().size()))
+            throw std::runtime_error("invalid object");
         const auto &pg = s->solver->particleGroups()[object];
         for (int i = 0; i < pg.size(); ++i)
             for (int d = 0; d < 3; ++d) {
@@ -381,5 +388,148 @@ spg_implicit_ctx *spg_solver_implicit_context(spg_solver *s)
 {
     return s && s->implicit ? s->implicit->context() : nullptr;
 }
+
+
+/* ---- rigid bodies ---- */
+
+spg_rbgroup *spg_rb_create(void) { return new spg_rbgroup; }
+void spg_rb_destroy(spg_rbgroup *rb) { delete rb; }
+
+int spg_rb_add_bodies(spg_rbgroup *rb, int64_t n, const double *x, const double *x0, const double *mass,
+                      const double *theta, const double *theta0, const double *inertia)
+{
+    return guarded([&] {
+        if (!rb || n < 0 || (n > 0 && (!x || !mass || !theta || !inertia))) throw std::runtime_error("invalid arguments");
+        const spg::TriangleMesh mesh({}, {}, {});
+        for (int64_t i = 0; i < n; ++i) {
+            const spg::Vector3 p(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+            const spg::Vector3 p0 = x0 ? spg::Vector3(x0[3 * i], x0[3 * i + 1], x0[3 * i + 2]) : p;
+            const spg::Vector3 t(theta[3 * i], theta[3 * i + 1], theta[3 * i + 2]);
+            const spg::Vector3 t0 = theta0 ? spg::Vector3(theta0[3 * i], theta0[3 * i + 1], theta0[3 * i + 2]) : t;
+            spg::Matrix3 I;
+            for (int k = 0; k < 9; ++k) I.m[k] = inertia[9 * i + k];
+            rb->group.addBody(p, p0, mass[i], t, t0, I, mesh);
+        }
+    });
+}
+
+int spg_rb_add_energy(spg_rbgroup *rb, int32_t type, int64_t n, const int32_t *stencils, const double *args,
+                      int32_t *index)
+{
+    return guarded([&] {
+        if (!rb || n < 0 || (n > 0 && (!stencils || !args))) throw std::runtime_error("invalid arguments");
+        auto v3 = [](const double *p) { return spg::Vector3(p[0], p[1], p[2]); };
+        std::shared_ptr<spg::RigidBodyGroup::EnergyT> e;
+        switch (type) {
+        case SPG_RIGID_SPRING: {
+            auto s = std::make_shared<spg::RBSpringEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                s->addStencil({stencils[2 * i], stencils[2 * i + 1]}, {v3(args + 8 * i), v3(args + 8 * i + 3)},
+                              args[8 * i + 6], args[8 * i + 7]);
+            e = s;
+            break;
+        }
+        case SPG_RIGID_SPRING_SQUARED: {
+            auto s = std::make_shared<spg::RBSpringSquaredEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                s->addStencil({stencils[2 * i], stencils[2 * i + 1]}, {v3(args + 8 * i), v3(args + 8 * i + 3)},
+                              args[8 * i + 6], args[8 * i + 7]);
+            e = s;
+            break;
+        }
+        case SPG_RIGID_SPRING_ANCHOR: {
+            auto s = std::make_shared<spg::RBSpringAnchorEnergy>();
+            for (int64_t i = 0; i < n; ++i)
+                s->addStencil({stencils[i]}, v3(args + 7 * i), v3(args + 7 * i + 3), args[7 * i + 6]);
+            e = s;
+            break;
+        }
+        case SPG_RIGID_ORIENTATION_ALIGNMENT: {
+            auto s = std::make_shared<spg::RBOrientationAlignmentEnergy>();
+            for (int64_t i = 0; i < n; ++i) s->addStencil({stencils[2 * i], stencils[2 * i + 1]}, args[i]);
+            e = s;
+            break;
+        }
+        default: throw std::runtime_error("unknown rigid-body energy type");
+        }
+        rb->group.addEnergy(e);
+        if (index) *index = static_cast<int32_t>(rb->group.energies().size()) - 1;
+    });
+}
+
+namespace {
+void rbState(const spg::RigidBodyGroup &g, double *x, double *theta, double *v, double *omega, double *rot)
+{
+    for (int i = 0; i < g.size(); ++i) {
+        for (int d = 0; d < 3; ++d) {
+            if (x) x[3 * i + d] = g.positions()[i][d];
+            if (theta) theta[3 * i + d] = g.thetas()[i][d];
+            if (v) v[3 * i + d] = g.velocities()[i][d];
+            if (omega) omega[3 * i + d] = g.omegas()[i][d];
+        }
+        if (rot) std::copy(g.rotationMatrices()[i].m.begin(), g.rotationMatrices()[i].m.end(), rot + 9 * i);
+    }
+}
+}  // namespace
+
+int spg_rb_num_bodies(const spg_rbgroup *rb, int64_t *n)
+{
+    return guarded([&] {
+        if (!rb || !n) throw std::runtime_error("invalid arguments");
+        *n = rb->group.size();
+    });
+}
+
+int spg_rb_get_state(const spg_rbgroup *rb, double *x, double *theta, double *v, double *omega, double *rot)
+{
+    return guarded([&] {
+        if (!rb) throw std::runtime_error("invalid arguments");
+        rbState(rb->group, x, theta, v, omega, rot);
+    });
+}
+
+int spg_rb_set_state(spg_rbgroup *rb, const double *x, const double *theta, const double *v, const double *omega)
+{
+    return guarded([&] {
+        if (!rb) throw std::runtime_error("invalid arguments");
+        auto &g = rb->group;
+        const size_t nb = static_cast<size_t>(g.size());
+        spg::VectorX pos(6 * nb), vel(6 * nb);
+        g.getPositions(pos, 0);
+        g.getVelocities(vel, 0);
+        for (size_t b = 0; b < nb; ++b)
+            for (int d = 0; d < 3; ++d) {
+                if (x) pos[6 * b + d] = x[3 * b + d];
+                if (theta) pos[6 * b + 3 + d] = theta[3 * b + d];
+                if (v) vel[6 * b + d] = v[3 * b + d];
+                if (omega) vel[6 * b + 3 + d] = omega[3 * b + d];
+            }
+        g.setPositions(pos, 0);
+        g.setVelocities(vel, 0);
+    });
+}
+
+int spg_solver_add_rb_object(spg_solver *s, const spg_rbgroup *rb)
+{
+    return guarded([&] {
+        if (!s || !rb) throw std::runtime_error("invalid arguments");
+        s->solver->addObject(rb->group);
+        if (s->vbd) s->vbd->requirePrecomputationUpdate();
+        if (s->xpbd) s->xpbd->requirePrecomputationUpdate();
+        if (s->implicit) s->implicit->requirePrecomputationUpdate();
+    });
+}
+
+int spg_solver_get_rb_state(spg_solver *s, int32_t object, double *x, double *theta, double *v, double *omega,
+                            double *rot)
+{
+    return guarded([&] {
+        if (!s || object < 0 || object >= static_cast<int32_t>(s->solver->rbGroups().size()))
+            throw std::runtime_error("invalid object");
+        rbState(s->solver->rbGroups()[object], x, theta, v, omega, rot);
+    });
+}
+
+spg_rigid_ctx *spg_solver_rigid_context(spg_solver *s) { return s && s->vbd ? s->vbd->rigidContext() : nullptr; }
 
 }  // extern "C"

@@ -93,6 +93,7 @@ void ImplicitEulerGpu::syncToHost()
 
 void ImplicitEulerGpu::step()
 {
+    requireNoRigidBodies("ImplicitEulerGpu");
     if (m_objects.empty()) return;
     if (m_rebuild || !m_ctx) {
         buildContext();

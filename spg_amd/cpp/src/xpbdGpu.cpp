@@ -96,6 +96,7 @@ void XPBDGpu::syncToHost()
 void XPBDGpu::step()
 {
     // xpbd.cpp:18-21: the stencil groups are (re)computed on the first step after a change
+    requireNoRigidBodies("XPBDGpu");
     if (m_objects.empty()) return;
     if (m_parallelGroupsDirty || !m_ctx) {
         buildContext();

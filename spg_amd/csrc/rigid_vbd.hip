@@ -744,6 +744,48 @@ int spg_rigid_set_state(spg_rigid_ctx *ctx, int32_t group, const double *x, cons
     });
 }
 
+int spg_rigid_get_rotations(spg_rigid_ctx *ctx, int32_t group, double *R)
+{
+    return guarded(ctx, [&] {
+        checkGroup(ctx, group);
+        const int64_t b0 = ctx->groupOffset[group], n = ctx->groupOffset[group + 1] - b0;
+        if (n == 0) return;
+        if (!R) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<double> buf(kPose * ctx->nb);
+        hipCheck(hipMemcpyAsync(buf.data(), ctx->pose.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream),
+                 "download");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        for (int64_t i = 0; i < n; ++i) {
+            const double *rec = buf.data() + kPose * ctx->slotOf[b0 + i];
+            std::copy(rec + 3, rec + kPose, R + 9 * i);
+        }
+    });
+}
+
+int spg_rigid_set_rotations(spg_rigid_ctx *ctx, int32_t group, const double *R)
+{
+    return guarded(ctx, [&] {
+        checkGroup(ctx, group);
+        const int64_t b0 = ctx->groupOffset[group], n = ctx->groupOffset[group + 1] - b0;
+        if (n == 0) return;
+        if (!R) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        if (!finiteN(R, static_cast<int>(9 * n))) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        std::vector<double> buf(kPose * ctx->nb);
+        hipCheck(hipMemcpyAsync(buf.data(), ctx->pose.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream),
+                 "download");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+        for (int64_t i = 0; i < n; ++i) std::copy(R + 9 * i, R + 9 * i + 9, buf.data() + kPose * ctx->slotOf[b0 + i] + 3);
+        hipCheck(hipMemcpyAsync(ctx->pose.p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream),
+                 "upload");
+        hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize");
+    });
+}
+
 int spg_rigid_get_colors(const spg_rigid_ctx *ctx, int32_t group, int32_t *colors, int32_t *n_colors)
 {
     return guarded(const_cast<spg_rigid_ctx *>(ctx), [&] {

@@ -247,6 +247,8 @@ def vbd_lib():
         _proto(L, "spg_rigid_get_state", ctypes.c_int, [c, i32, _dp, _dp, _dp, _dp])
         _proto(L, "spg_rigid_set_state", ctypes.c_int, [c, i32, _dp, _dp, _dp, _dp])
         _proto(L, "spg_rigid_get_colors", ctypes.c_int, [c, i32, _ip, _ip])
+        _proto(L, "spg_rigid_get_rotations", ctypes.c_int, [c, i32, _dp])
+        _proto(L, "spg_rigid_set_rotations", ctypes.c_int, [c, i32, _dp])
         _proto(L, "spg_rigid_get_stats", ctypes.c_int, [c, ctypes.POINTER(RigidStats)])
         _proto(L, "spg_rigid_set_graph", ctypes.c_int, [c, i32])
         _proto(L, "spg_rigid_profile_substep", ctypes.c_int, [c, dbl, i32, _dp, _dp, _ip])
@@ -286,6 +288,16 @@ def host_lib():
         _proto(L, "spg_solver_implicit_context", c, [c])
         _proto(L, "spg_solver_destroy", None, [c])
         _proto(L, "spg_solver_add_object", ctypes.c_int, [c, c])
+        _proto(L, "spg_rb_create", c, [])
+        _proto(L, "spg_rb_destroy", None, [c])
+        _proto(L, "spg_rb_add_bodies", ctypes.c_int, [c, i64, _dp, _dp, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_rb_add_energy", ctypes.c_int, [c, i32, i64, _ip, _dp, _ip])
+        _proto(L, "spg_rb_num_bodies", ctypes.c_int, [c, _lp])
+        _proto(L, "spg_rb_get_state", ctypes.c_int, [c, _dp, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_rb_set_state", ctypes.c_int, [c, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_solver_add_rb_object", ctypes.c_int, [c, c])
+        _proto(L, "spg_solver_get_rb_state", ctypes.c_int, [c, i32, _dp, _dp, _dp, _dp, _dp])
+        _proto(L, "spg_solver_rigid_context", c, [c])
         _proto(L, "spg_solver_set_dt", ctypes.c_int, [c, dbl])
         _proto(L, "spg_solver_set_num_substeps", ctypes.c_int, [c, i32])
         _proto(L, "spg_solver_set_gravity", ctypes.c_int, [c, _dp])
@@ -420,6 +432,57 @@ class ParticleGroup:
                 "energies": [(t, st, self.energy_params(i)) for i, (t, n, st) in enumerate(self.energy_types)]}
 
 
+class RigidBodyGroup:
+    """spg::RigidBodyGroup built through the C++ mirror (addBody, the rigid-body energies' addStencil)."""
+
+    def __init__(self, scene=None):
+        self._L = host_lib()
+        self.h = self._L.spg_rb_create()
+        if scene is not None:
+            self.add_bodies(scene.x, scene.m, scene.theta, scene.inertia)
+            for kind, st, args in scene.energies:
+                self.add_energy(kind, st, args)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._L.spg_rb_destroy(self.h)
+            self.h = None
+
+    def add_bodies(self, x, m, theta, inertia, x0=None, theta0=None):
+        x = _f64(x, (-1, 3))
+        n = x.shape[0]
+        m, th, inertia = _f64(m, (n,)), _f64(theta, (n, 3)), _f64(inertia, (n, 9))
+        x0 = None if x0 is None else _f64(x0, (n, 3))
+        th0 = None if theta0 is None else _f64(theta0, (n, 3))
+        _host_check(self._L.spg_rb_add_bodies(self.h, n, _d(x), None if x0 is None else _d(x0), _d(m), _d(th),
+                                              None if th0 is None else _d(th0), _d(inertia)))
+
+    def add_energy(self, kind, stencils, args):
+        t = RIGID_KIND.get(kind, kind)
+        st = _i32(stencils, (-1, RIGID_STENCIL[t]))
+        a = _f64(args, (st.shape[0], RIGID_PARAMS[t]))
+        idx = ctypes.c_int32(0)
+        _host_check(self._L.spg_rb_add_energy(self.h, t, st.shape[0], _i(st), _d(a), ctypes.byref(idx)))
+        return idx.value
+
+    def num_bodies(self):
+        n = ctypes.c_int64(0)
+        _host_check(self._L.spg_rb_num_bodies(self.h, ctypes.byref(n)))
+        return n.value
+
+    def state(self):
+        """(x, theta, v, omega, rotation matrices)."""
+        n = self.num_bodies()
+        out = [np.zeros((n, 3)) for _ in range(4)] + [np.zeros((n, 3, 3))]
+        _host_check(self._L.spg_rb_get_state(self.h, *[_d(a) for a in out]))
+        return tuple(out)
+
+    def set_state(self, x=None, theta=None, v=None, omega=None):
+        n = self.num_bodies()
+        arrs = [None if a is None else _f64(a, (n, 3)) for a in (x, theta, v, omega)]
+        _host_check(self._L.spg_rb_set_state(self.h, *[None if a is None else _d(a) for a in arrs]))
+
+
 class VBDGpu:
     """spg::solver::VBDGpu (the GPU drop-in for spg::solver::VBD) through the C++ mirror."""
 
@@ -427,6 +490,18 @@ class VBDGpu:
         self._L = host_lib()
         self.h = self._L.spg_solver_create_vbd_gpu(device)
         self._groups = []
+        self._rb_groups = []
+
+    def add_rb_object(self, rb):
+        _host_check(self._L.spg_solver_add_rb_object(self.h, rb.h))
+        self._rb_groups.append(rb)
+
+    def rb_state(self, obj=0):
+        """rbGroups()[obj]: (x, theta, v, omega, rotation matrices)."""
+        n = self._rb_groups[obj].num_bodies()
+        out = [np.zeros((n, 3)) for _ in range(4)] + [np.zeros((n, 3, 3))]
+        _host_check(self._L.spg_solver_get_rb_state(self.h, obj, *[_d(a) for a in out]))
+        return tuple(out)
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -483,6 +558,7 @@ class XPBDGpu(VBDGpu):
         self._L = host_lib()
         self.h = self._L.spg_solver_create_xpbd_gpu(device, int(parallel_gauss_seidel))
         self._groups = []
+        self._rb_groups = []
 
     def set_iterations(self, n):
         raise NotImplementedError("XPBD projects every stencil once per substep (no iteration count)")
@@ -505,6 +581,7 @@ class ImplicitEulerGpu(VBDGpu):
         if not self.h:
             raise NativeError(1, self._L.spg_host_last_error().decode())
         self._groups = []
+        self._rb_groups = []
 
     def set_iterations(self, n):
         raise NotImplementedError("the implicit solvers have no iteration count")
@@ -1239,6 +1316,16 @@ class RigidContext:
         nb = self._nb[group]
         arrs = [None if a is None else _f64(a, (nb, 3)) for a in (x, theta, v, omega)]
         self._check(self._L.spg_rigid_set_state(self.h, group, *[None if a is None else _d(a) for a in arrs]))
+
+    def rotations(self, group=0):
+        """rotationMatrices() of the group, (nb, 9) row-major."""
+        R = np.zeros((self._nb[group], 9))
+        self._check(self._L.spg_rigid_get_rotations(self.h, group, _d(R)))
+        return R
+
+    def set_rotations(self, R, group=0):
+        R = _f64(R, (self._nb[group], 9))
+        self._check(self._L.spg_rigid_set_rotations(self.h, group, _d(R)))
 
     def colors(self, group=0):
         c = np.zeros(self._nb[group], dtype=np.int32)

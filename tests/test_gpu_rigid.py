@@ -140,3 +140,38 @@ def test_rigid_validation_messages():
     with pytest.raises(native.NativeError, match="invalid node indices"):
         native.RigidContext([with_energy(("rb_alignment", [[0, 3]], [[1.0]]))])
     native.RigidContext([with_energy(("rb_spring_squared", [[0, 1]], [[0, 0, 0, 0, 0, 0, 0.0, 1.0]]))])
+
+
+def test_vbd_gpu_mirror_with_rigid_groups(oracle_threads):
+    """spg::solver::VBDGpu of the C++ mirror holding a cloth and two RigidBodyGroups: the rigid groups equal the
+    spg_rigid C-ABI bitwise and the oracle to the bars above, the cloth equals a VBDGpu without rigid bodies
+    bitwise, and the host groups receive the state including the composed rotation matrices."""
+    cloth = scenes.cloth(16, young=1e4, pin="top_row", iterations=10)
+    rbs = [scenes.rb_chain(5, iterations=10), scenes.rb_lattice(6, iterations=10)]
+    solver = native.VBDGpu()
+    solver.add_object(cloth.particle_group())
+    for s in rbs:
+        solver.add_rb_object(native.RigidBodyGroup(s))
+    solver.set_dt(cloth.dt)
+    solver.set_gravity(cloth.gravity)
+    solver.set_iterations(10)
+    plain = native.VBDGpu()
+    plain.add_object(cloth.particle_group())
+    plain.set_dt(cloth.dt)
+    plain.set_gravity(cloth.gravity)
+    plain.set_iterations(10)
+    ctx = native.RigidContext(rbs)
+    os_ = [ol.rb_from_scene(s) for s in rbs]
+    for _ in range(3):
+        solver.step()
+        plain.step()
+        ctx.step(cloth.dt, 1, 10, cloth.gravity)
+        for s, o in zip(rbs, os_):
+            o.step(s.dt, 1, 10, cloth.gravity)
+    assert np.array_equal(solver.state(0)[0], plain.state(0)[0])
+    for g, (s, o) in enumerate(zip(rbs, os_)):
+        x, th, v, w, R = solver.rb_state(g)
+        for a, b in zip((x, th, v, w), ctx.state(g)):
+            assert np.array_equal(a, b)
+        assert np.array_equal(R.reshape(-1, 9), ctx.rotations(g))
+        compare(f"mirror group {g}", ctx, o, s.x, group=g)

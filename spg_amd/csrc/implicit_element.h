@@ -72,6 +72,60 @@ SPG_HD R valueOf(DualT<R> a)
 {
     return a.v;
 }
+// Forward mode with three tangents at once: one pass over an element's gradient gives the three columns of a
+// vertex's 3x3 block (the tangents seeded with the unit vectors of that vertex's coordinates).
+template <class R>
+struct Dual3T {
+    R v, d[3];
+    SPG_HD Dual3T() : v(0), d{0, 0, 0} {}
+    SPG_HD Dual3T(R a) : v(a), d{0, 0, 0} {}  // NOLINT: constants inside expressions
+    SPG_HD Dual3T(R a, R d0, R d1, R d2) : v(a), d{d0, d1, d2} {}
+};
+template <class R>
+SPG_HD Dual3T<R> operator+(Dual3T<R> a, Dual3T<R> b)
+{
+    return {a.v + b.v, a.d[0] + b.d[0], a.d[1] + b.d[1], a.d[2] + b.d[2]};
+}
+template <class R>
+SPG_HD Dual3T<R> operator-(Dual3T<R> a, Dual3T<R> b)
+{
+    return {a.v - b.v, a.d[0] - b.d[0], a.d[1] - b.d[1], a.d[2] - b.d[2]};
+}
+template <class R>
+SPG_HD Dual3T<R> operator-(Dual3T<R> a)
+{
+    return {-a.v, -a.d[0], -a.d[1], -a.d[2]};
+}
+template <class R>
+SPG_HD Dual3T<R> operator*(Dual3T<R> a, Dual3T<R> b)
+{
+    return {a.v * b.v, a.v * b.d[0] + a.d[0] * b.v, a.v * b.d[1] + a.d[1] * b.v, a.v * b.d[2] + a.d[2] * b.v};
+}
+template <class R>
+SPG_HD Dual3T<R> operator/(Dual3T<R> a, Dual3T<R> b)
+{
+    const R q = a.v / b.v, ib = R(1) / b.v;
+    return {q, (a.d[0] - q * b.d[0]) * ib, (a.d[1] - q * b.d[1]) * ib, (a.d[2] - q * b.d[2]) * ib};
+}
+template <class R>
+SPG_HD Dual3T<R> sqrt(Dual3T<R> a)
+{
+    const R s = realSqrt(a.v);
+    const R h = R(1) / (R(2) * s);
+    return {s, a.d[0] * h, a.d[1] * h, a.d[2] * h};
+}
+template <class R>
+SPG_HD Dual3T<R> atan2(Dual3T<R> y, Dual3T<R> x)
+{
+    const R ir2 = R(1) / (x.v * x.v + y.v * y.v);
+    return {realAtan2(y.v, x.v), (x.v * y.d[0] - y.v * x.d[0]) * ir2, (x.v * y.d[1] - y.v * x.d[1]) * ir2,
+            (x.v * y.d[2] - y.v * x.d[2]) * ir2};
+}
+template <class R>
+SPG_HD R valueOf(Dual3T<R> a)
+{
+    return a.v;
+}
 }  // namespace ad
 using Dual1 = ad::DualT<double>;
 using Dual1f = ad::DualT<float>;
@@ -113,10 +167,12 @@ SPG_HD void fullGradStvk(const V3<T> *x, const double *p, V3<T> *g)
     const V3<T> f1 = T(p[1]) * u + T(p[3]) * v;
     const T C[3] = {T(0.5) * (dot(f0, f0) - T(1)), T(0.5) * (dot(f1, f1) - T(1)), dot(f0, f1)};
     T s[3];
+#pragma unroll
     for (int i = 0; i < 3; ++i) s[i] = T(p[4 + 3 * i]) * C[0] + T(p[5 + 3 * i]) * C[1] + T(p[6 + 3 * i]) * C[2];
     // F S = [f0 s0 + f1 s2, f0 s2 + f1 s1]
     const V3<T> c0 = s[0] * f0 + s[2] * f1, c1 = s[2] * f0 + s[1] * f1;
     const double ga[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
+#pragma unroll
     for (int a = 0; a < 3; ++a) g[a] = T(ga[a][0]) * c0 + T(ga[a][1]) * c1;
 }
 
@@ -175,6 +231,7 @@ template <class T>
 SPG_HD void triangleScatter(const double *p, V3<T> dEdf0, V3<T> dEdf1, V3<T> *g)
 {
     const double ga[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
+#pragma unroll
     for (int a = 0; a < 3; ++a) g[a] = T(ga[a][0]) * dEdf0 + T(ga[a][1]) * dEdf1;
 }
 // d/dv of 1/2 k (|v| - 1)^2 = k (|v| - 1) v / |v|
@@ -231,6 +288,7 @@ template <class T>
 SPG_HD void fullGradBendingQuadratic(const V3<T> *x, const double *p, V3<T> *g)
 {
     const V3<T> C = T(p[0]) * x[0] + T(p[1]) * x[1] + T(p[2]) * x[2] + T(p[3]) * x[3];
+#pragma unroll
     for (int s = 0; s < 4; ++s) g[s] = (T(p[4]) * T(p[s])) * C;
 }
 // baraffWitkinBendingEnergy.cpp:9-36: C = theta - theta0, theta = atan2(s0 Y, X) with X = n1 . n2,
@@ -270,6 +328,7 @@ SPG_HD void fullGradBendingBw(const V3<T> *x, const double *p, V3<T> *g)
     gradDotN2(cross(e0, n1), e0, e4, dy2);
     const T theta = atan2(Y, X);
     const T w = (T(p[1]) * (theta - T(p[0]))) / (X * X + Y * Y);
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         V3<T> gy = dy1[s] + dy2[s];
         if (s == 0) gy = gy - m;
@@ -283,6 +342,7 @@ template <class T>
 SPG_HD void tetF(const V3<T> *x, const double *p, V3<T> *f)
 {
     const V3<T> u = x[1] - x[0], v = x[2] - x[0], w = x[3] - x[0];
+#pragma unroll
     for (int c = 0; c < 3; ++c) f[c] = T(p[c]) * u + T(p[3 + c]) * v + T(p[6 + c]) * w;
 }
 template <class T>
@@ -301,11 +361,14 @@ SPG_HD void fullGradStvkTet(const V3<T> *x, const double *p, V3<T> *g)
     V3<T> f[3];
     tetF(x, p, f);
     T E[3][3];
+#pragma unroll
     for (int i = 0; i < 3; ++i)
+#pragma unroll
         for (int j = 0; j < 3; ++j) E[i][j] = T(0.5) * (dot(f[i], f[j]) - T(i == j ? 1 : 0));
     const T lt = T(p[9]) * (E[0][0] + E[1][1] + E[2][2]);
     const T m2 = T(2) * T(p[10]);
     V3<T> P[3];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         const T s0 = m2 * E[0][c] + (c == 0 ? lt : T(0));
         const T s1 = m2 * E[1][c] + (c == 1 ? lt : T(0));

@@ -165,17 +165,15 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
 // LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
 // contiguous span per record array) into LDS with coalesced loads, then every lane reads its records
 // from LDS; positions are gathered from global memory as in the direct variant.
-// A further energy's gradient w.r.t. its stencil slot `local` and, with HESS, the slot's 3x3 Hessian block:
-// the closed-form full gradient (implicit_element.h) at the element's positions, then one forward-mode pass
-// per coordinate of the slot gives the block's columns (symmetrised). T: float (solve) or double (residual).
-template <int NS, class T, bool HESS>
-__device__ __forceinline__ void extSlot(int type, int local, const float4 *__restrict__ pos, int4 v,
-                                        const double *__restrict__ prm, V3<T> &g, Sym3<T> &H)
+// A further energy's gradient w.r.t. its stencil slot LOCAL and, with HESS, the slot's 3x3 Hessian block: the
+// closed-form full gradient (implicit_element.h) at the element's positions in one forward-mode pass whose three
+// tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template parameter so
+// the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in scratch memory)
+// and the other slots' outputs are dropped. T: float (solve) or double (residual).
+template <int NS, int LOCAL, class T, bool HESS>
+__device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
+                                        const double *__restrict__ p, V3<T> &g, Sym3<T> &H)
 {
-    using D = ad::DualT<T>;
-    double p[kExtParams];
-#pragma unroll
-    for (int k = 0; k < kExtParams; ++k) p[k] = prm[k];
     const int32_t vv[4] = {v.x, v.y, v.z, v.w};
     V3<T> xs[NS];
 #pragma unroll
@@ -186,32 +184,46 @@ __device__ __forceinline__ void extSlot(int type, int local, const float4 *__res
     if constexpr (!HESS) {
         V3<T> gd[NS];
         fullGrad<NS, T>(type, xs, p, gd);
-        for (int s = 0; s < NS; ++s)
-            if (s == local) g = gd[s];
-        return;
+        g = gd[LOCAL];
     } else {
-        T J[3][3];
-#pragma unroll 1
-        for (int d = 0; d < 3; ++d) {
-            V3<D> x[NS], gd[NS];
+        using D3 = ad::Dual3T<T>;
+        V3<D3> x[NS], gd[NS];
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const bool own = s == local;
-                x[s] = {D(xs[s].x, own && d == 0 ? T(1) : T(0)), D(xs[s].y, own && d == 1 ? T(1) : T(0)),
-                        D(xs[s].z, own && d == 2 ? T(1) : T(0))};
-            }
-            fullGrad<NS, D>(type, x, p, gd);
-            V3<D> gl = gd[0];
-#pragma unroll
-            for (int s = 1; s < NS; ++s)
-                if (s == local) gl = gd[s];
-            if (d == 0) g = {gl.x.v, gl.y.v, gl.z.v};
-            J[0][d] = gl.x.d;
-            J[1][d] = gl.y.d;
-            J[2][d] = gl.z.d;
+        for (int s = 0; s < NS; ++s) {
+            const T o = s == LOCAL ? T(1) : T(0);
+            x[s] = {D3(xs[s].x, o, T(0), T(0)), D3(xs[s].y, T(0), o, T(0)), D3(xs[s].z, T(0), T(0), o)};
         }
+        fullGrad<NS, D3>(type, x, p, gd);
+        const V3<D3> &gl = gd[LOCAL];
+        g = {gl.x.v, gl.y.v, gl.z.v};
         const T h = T(0.5);
-        H = {J[0][0], h * (J[0][1] + J[1][0]), h * (J[0][2] + J[2][0]), J[1][1], h * (J[1][2] + J[2][1]), J[2][2]};
+        H = {gl.x.d[0], h * (gl.x.d[1] + gl.y.d[0]), h * (gl.x.d[2] + gl.z.d[0]), gl.y.d[1],
+             h * (gl.y.d[2] + gl.z.d[1]), gl.z.d[2]};
+    }
+}
+template <int NS, class T, bool HESS>
+__device__ __forceinline__ void extSlotAny(int type, int local, const float4 *__restrict__ pos, int4 v,
+                                           const double *__restrict__ p, V3<T> &g, Sym3<T> &H)
+{
+    if constexpr (NS == 2) {
+        if (local == 0)
+            extSlot<2, 0, T, HESS>(type, pos, v, p, g, H);
+        else
+            extSlot<2, 1, T, HESS>(type, pos, v, p, g, H);
+    } else if constexpr (NS == 3) {
+        if (local == 0)
+            extSlot<3, 0, T, HESS>(type, pos, v, p, g, H);
+        else if (local == 1)
+            extSlot<3, 1, T, HESS>(type, pos, v, p, g, H);
+        else
+            extSlot<3, 2, T, HESS>(type, pos, v, p, g, H);
+    } else {
+        switch (local) {
+        case 0: extSlot<4, 0, T, HESS>(type, pos, v, p, g, H); break;
+        case 1: extSlot<4, 1, T, HESS>(type, pos, v, p, g, H); break;
+        case 2: extSlot<4, 2, T, HESS>(type, pos, v, p, g, H); break;
+        default: extSlot<4, 3, T, HESS>(type, pos, v, p, g, H);
+        }
     }
 }
 template <class T, bool HESS>
@@ -222,9 +234,9 @@ __device__ __forceinline__ void extIncidence(int2 in, const float4 *__restrict__
     const int4 v = extV[in.x];
     const double *p = extP + static_cast<int64_t>(kExtParams) * in.x;
     switch (fullStencilSize(type)) {
-    case 2: extSlot<2, T, HESS>(type, local, pos, v, p, g, H); break;
-    case 3: extSlot<3, T, HESS>(type, local, pos, v, p, g, H); break;
-    default: extSlot<4, T, HESS>(type, local, pos, v, p, g, H);
+    case 2: extSlotAny<2, T, HESS>(type, local, pos, v, p, g, H); break;
+    case 3: extSlotAny<3, T, HESS>(type, local, pos, v, p, g, H); break;
+    default: extSlotAny<4, T, HESS>(type, local, pos, v, p, g, H);
     }
 }
 

@@ -30,8 +30,9 @@
  * Error behaviour and codes are those of spg_vbd.h: SPG_VBD_OK (0) or a nonzero code, with the message from
  * spg_rigid_last_error (per context, or the last context-free failure on this thread when ctx is NULL).
  * Setup checks reproduce the reference's exceptions ("Zero/negative rest length invalid", "Negative rest length
- * invalid", "Trying to add edge to graph with invalid node indices"). A context is not thread-safe; its work
- * goes to one HIP stream in call order.
+ * invalid"); a stencil index outside the group ("stencil index out of range", which the reference leaves
+ * unchecked for single-body stencils) and a singular local inertia are rejected as SPG_VBD_ERR_INVALID. A context
+ * is not thread-safe; its work goes to one HIP stream in call order.
  */
 #ifndef SPG_RIGID_H
 #define SPG_RIGID_H

@@ -354,9 +354,9 @@ int guarded(spg_rigid_ctx *ctx, F &&f)
     }
 }
 
-bool finiteN(const double *p, int n)
+bool finiteN(const double *p, int64_t n)
 {
-    for (int k = 0; k < n; ++k)
+    for (int64_t k = 0; k < n; ++k)
         if (!std::isfinite(p[k])) return false;
     return true;
 }
@@ -421,6 +421,9 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
                 if (E.type == kRbSpringSquared && q[6] < 0)
                     throw VbdError(SPG_VBD_ERR_INVALID, "Negative rest length invalid");
             }
+            for (int64_t k = 0; k < E.n_stencils * ss; ++k)  // anchors add no graph edge, so check here
+                if (E.stencils[k] < 0 || E.stencils[k] >= G.n_bodies)
+                    throw VbdError(SPG_VBD_ERR_INVALID, "stencil index out of range");
             sets.push_back(FlatStencilSet{E.stencils, E.n_stencils * ss, ss});
         }
         c.colors[g] = colorVertices(G.n_bodies, sets);  // throws on out-of-range indices like Graph::addEdge
@@ -600,6 +603,7 @@ int spg_rigid_step(spg_rigid_ctx *ctx, double dt, int32_t nsubsteps, int32_t ite
         if (!(dt > 0) || !std::isfinite(dt) || nsubsteps < 1 || iterations < 0 || !gravity || !finiteN(gravity, 3))
             throw VbdError(SPG_VBD_ERR_INVALID, "invalid step arguments");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        if (ctx->nb == 0) return;
         const double h = dt / nsubsteps;
         if (!ctx->useGraph) {
             for (int32_t sub = 0; sub < nsubsteps; ++sub) enqueueSubstep(*ctx, h, iterations, gravity, nullptr);
@@ -715,7 +719,7 @@ int spg_rigid_set_state(spg_rigid_ctx *ctx, int32_t group, const double *x, cons
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         const int64_t b0 = ctx->groupOffset[group], n = ctx->groupOffset[group + 1] - b0;
         for (const double *p : {x, theta, v, omega})
-            if (p && !finiteN(p, static_cast<int>(3 * n))) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
+            if (p && !finiteN(p, 3 * n)) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
         if (n == 0) return;
         const size_t nb = static_cast<size_t>(ctx->nb);
         auto sync = [&] { hipCheck(hipStreamSynchronize(ctx->stream), "hipStreamSynchronize"); };
@@ -771,7 +775,7 @@ int spg_rigid_set_rotations(spg_rigid_ctx *ctx, int32_t group, const double *R)
         const int64_t b0 = ctx->groupOffset[group], n = ctx->groupOffset[group + 1] - b0;
         if (n == 0) return;
         if (!R) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
-        if (!finiteN(R, static_cast<int>(9 * n))) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
+        if (!finiteN(R, 9 * n)) throw VbdError(SPG_VBD_ERR_NONFINITE, "non-finite state");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         std::vector<double> buf(kPose * ctx->nb);
         hipCheck(hipMemcpyAsync(buf.data(), ctx->pose.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost,

@@ -137,8 +137,10 @@ def test_rigid_validation_messages():
         native.RigidContext([with_energy(("rb_spring", [[0, 1]], [[0, 0, 0, 0, 0, 0, 0.0, 1.0]]))])
     with pytest.raises(native.NativeError, match="Negative rest length invalid"):
         native.RigidContext([with_energy(("rb_spring_squared", [[0, 1]], [[0, 0, 0, 0, 0, 0, -1.0, 1.0]]))])
-    with pytest.raises(native.NativeError, match="invalid node indices"):
+    with pytest.raises(native.NativeError, match="stencil index out of range"):
         native.RigidContext([with_energy(("rb_alignment", [[0, 3]], [[1.0]]))])
+    with pytest.raises(native.NativeError, match="stencil index out of range"):
+        native.RigidContext([with_energy(("rb_anchor", [[5]], [[0, 0, 0, 0, 0, 0, 1.0]]))])
     native.RigidContext([with_energy(("rb_spring_squared", [[0, 1]], [[0, 0, 0, 0, 0, 0, 0.0, 1.0]]))])
 
 
@@ -175,3 +177,29 @@ def test_vbd_gpu_mirror_with_rigid_groups(oracle_threads):
             assert np.array_equal(a, b)
         assert np.array_equal(R.reshape(-1, 9), ctx.rotations(g))
         compare(f"mirror group {g}", ctx, o, s.x, group=g)
+
+
+def test_rigid_edge_cases(oracle_threads):
+    """An empty group beside a free body with no energies (ballistic centre, gyroscopic spin of an anisotropic body),
+    a body tied to itself by a squared spring (both stencil slots on one body, as addStencil allows), and a step of
+    zero iterations (prediction and velocity update only): each equals the oracle."""
+    empty = scenes.RigidScene(x=np.zeros((0, 3)), theta=np.zeros((0, 3)), m=np.zeros(0), inertia=np.zeros((0, 3, 3)))
+    free = scenes.RigidScene(x=np.array([[0.3, 1.0, -0.2]]), theta=np.array([[0.2, -0.4, 0.1]]), m=np.array([2.0]),
+                             inertia=np.diag([0.2, 0.5, 0.9])[None])
+    selftied = scenes.rb_chain(3)
+    selftied.energies.append(("rb_spring_squared", np.array([[1, 1]], np.int32),
+                              np.array([[0.5, 0.0, 0.0, -0.5, 0.0, 0.0, 0.2, 50.0]])))
+    groups = [empty, free, selftied]
+    ctx = native.RigidContext(groups)
+    w0 = np.array([[1.5, -0.7, 2.0]])
+    ctx.set_state(1, v=np.array([[1.0, 0.0, 0.5]]), omega=w0)
+    assert ctx.state(0)[0].shape == (0, 3)
+    os_ = [None, ol.rb_from_scene(free), ol.rb_from_scene(selftied)]
+    os_[1].set_state(v=np.array([[1.0, 0.0, 0.5]]), omega=w0)
+    for it in (10, 0, 10):
+        ctx.step(1 / 60, 2, it, (0.0, -9.81, 0.0))
+        for o in os_[1:]:
+            o.step(1 / 60, 2, it, (0.0, -9.81, 0.0))
+    for g in (1, 2):
+        compare(f"edge group {g}", ctx, os_[g], groups[g].x, group=g)
+    assert ctx.stats()["skipped_blocks"] == 0

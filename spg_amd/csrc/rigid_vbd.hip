@@ -6,8 +6,9 @@
 // pose (centre and rotation matrix, 12 doubles = 96 bytes, read with 16-byte loads) is the record its neighbours
 // gather; the rest of its state is stored as arrays of 3 or 9 doubles per body (v, omega, theta, xt (inertial
 // prediction), xold, thetaold in 3; local inertia, its inverse and the rotation at the substep start in 9). World
-// inertias R Il R^T are recomputed where used instead of stored. Elements keep 8 parameter doubles each; a slot's
-// incidence is a CSR row of (other body's slot, element << 3 | type << 1 | stencil slot) entries in energy-major,
+// inertias R Il R^T are recomputed where used instead of stored. Element parameters are packed 16-byte aligned
+// (8 doubles for springs and anchors, 2 for the alignment); a slot's incidence is a CSR row of (other body's slot,
+// parameter offset << 3 | type << 1 | stencil slot) entries in energy-major,
 // ascending-stencil order, the order in which the reference sums them (vbd.cpp:105-135 over
 // computeStencilInfoPerVertex, vbd.cpp:231-268).
 #include <hip/hip_runtime.h>
@@ -148,8 +149,8 @@ __device__ __forceinline__ void integratedOmega(const double *R, const double *R
 struct RbSolveArgs {
     RbState s;
     const int64_t *incPtr;
-    const int2 *inc;          // (other body's slot or -1, element << 3 | (type - 21) << 1 | stencil slot)
-    const double *elemP;      // kRbParamStride per element
+    const int2 *inc;          // (other body's slot or -1, parameter offset / 2 << 3 | (type - 21) << 1 | stencil slot)
+    const double *elemP;      // per element its parameters, 16-byte aligned: 8 doubles (springs, anchor), 2 (alignment)
     unsigned long long *skipped;
     double h;
 };
@@ -199,10 +200,9 @@ __global__ void __launch_bounds__(kRbBlock) rbSolveKernel(RbSolveArgs a, int64_t
     const int64_t e0 = a.incPtr[b], e1 = a.incPtr[b + 1];
     for (int64_t j = e0 + lane; j < e1; j += kRbLanes) {
         const int2 en = a.inc[j];
-        const int64_t e = int64_t(uint32_t(en.y) >> 3);
         const int type = kRbSpring + ((en.y >> 1) & 3);
         const int slot = en.y & 1;
-        const double *pp = a.elemP + kRbParamStride * e;
+        const double *pp = a.elemP + 2 * int64_t(uint32_t(en.y) >> 3);
         double p[kRbParamStride];
         if (type == kRbAlignment) {
             p[0] = pp[0];
@@ -463,7 +463,6 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
     }
     // per-slot host arrays
     const int64_t nb = c.nb;
-    if (c.nElem >= (int64_t(1) << 29)) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many rigid-body elements");
     std::vector<double> hpose(kPose * nb), hv(3 * nb, 0.0), hw(3 * nb, 0.0), hth(3 * nb), hm(nb), hIl(9 * nb),
         hIli(9 * nb);
     for (int32_t g = 0; g < nGroups; ++g) {
@@ -486,9 +485,17 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
             aaToR(hth.data() + 3 * s, hpose.data() + kPose * s + 3);  // setPositions: R from theta
         }
     }
-    // element parameters and incidence rows in slot order
-    std::vector<double> ep(kRbParamStride * c.nElem);
-    for (int64_t e = 0; e < c.nElem; ++e) std::copy(elems[e].p, elems[e].p + kRbParamStride, ep.begin() + kRbParamStride * e);
+    // element parameters, packed with 16-byte alignment (the alignment energy's single stiffness takes 16 bytes
+    // instead of a full 64-byte record), and incidence rows in slot order
+    std::vector<int64_t> poff(c.nElem);
+    std::vector<double> ep;
+    for (int64_t e = 0; e < c.nElem; ++e) {
+        poff[e] = static_cast<int64_t>(ep.size());
+        const int width = elems[e].type == kRbAlignment ? 2 : kRbParamStride;
+        ep.insert(ep.end(), elems[e].p, elems[e].p + width);
+    }
+    if (static_cast<int64_t>(ep.size()) / 2 >= (int64_t(1) << 28))
+        throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many rigid-body elements");
     std::vector<int64_t> ptr(nb + 1, 0);
     std::vector<int2> inc;
     std::vector<int64_t> bodyOfSlot(nb);
@@ -499,7 +506,8 @@ void buildContext(spg_rigid_ctx &c, const spg_rigid_group *groups, int32_t nGrou
         for (const auto &pr : row) {
             const auto &el = elems[pr.first];
             const int64_t other = el.b1 < 0 ? -1 : c.slotOf[pr.second == 0 ? el.b1 : el.b0];
-            inc.push_back(make_int2(int32_t(other), int32_t((pr.first << 3) | ((el.type - kRbSpring) << 1) | pr.second)));
+            inc.push_back(make_int2(int32_t(other), int32_t(((poff[pr.first] / 2) << 3) | ((el.type - kRbSpring) << 1) |
+                                                            pr.second)));
             // the entry, the parameters it uses and the other body's pose
             bytes += 8 + 8 * rbParams(el.type) + (el.type == kRbAnchor ? 0 : 8 * kPose);
         }

@@ -419,6 +419,28 @@ SPG_HD constexpr int fullParams(int type)
     return 0;
 }
 
+// One further energy's gradient with the type fixed at compile time (types 6 ... 13).
+template <int TYPE, class T>
+SPG_HD void fullGradOf(const V3<T> *x, const double *p, V3<T> *g)
+{
+    if constexpr (TYPE == 6)
+        fullGradSpringContinuum(x, p, g);
+    else if constexpr (TYPE == 7)
+        fullGradSpringSquaredContinuum(x, p, g);
+    else if constexpr (TYPE == 8)
+        fullGradMembraneBw(x, p, g);
+    else if constexpr (TYPE == 9)
+        fullGradMembraneChoi(x, p, g);
+    else if constexpr (TYPE == 10)
+        fullGradBendingQuadratic(x, p, g);
+    else if constexpr (TYPE == 11)
+        fullGradBendingBw(x, p, g);
+    else if constexpr (TYPE == 12)
+        fullGradStvkTet(x, p, g);
+    else
+        fullGradSsnh(x, p, g);
+}
+
 // The energies of stencil size NS (selected at compile time, so no path indexes past x[NS - 1]).
 template <int NS, class T>
 SPG_HD void fullGrad(int type, const V3<T> *x, const double *p, V3<T> *g)

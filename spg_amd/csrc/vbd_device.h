@@ -66,9 +66,11 @@ struct SolveArgs {
     // the further energies (types 6 ... 13): CSR rows per slot of (element, type << 2 | slot in the stencil),
     // elements as up to four slots (unused -1) and kExtParams prepared FP64 parameters
     const int32_t *extPtr;  // null: none (the solve kernel's instantiation without this class runs)
-    const int2 *extInc;
+    const int2 *extInc;     // each row sorted by type (element order kept within a type)
     const int4 *extV;
     const double *extP;
+    const uint2 *extCnt;    // per slot, the row's entry count of each type 6 ... 13, 8 bits each (x: 6-9, y: 10-13)
+    uint32_t extTypes;      // bit t set: type t occurs in some row
     unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
     int32_t begin, end;            // slot range of the colour
     float invdt2;

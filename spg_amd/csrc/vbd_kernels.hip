@@ -170,7 +170,8 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
 // tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template parameter so
 // the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in scratch memory)
 // and the other slots' outputs are dropped. T: float (solve) or double (residual).
-template <int NS, int LOCAL, class T, bool HESS>
+// TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
+template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
                                         const double *__restrict__ p, V3<T> &g, Sym3<T> &H)
 {
@@ -183,7 +184,10 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
     }
     if constexpr (!HESS) {
         V3<T> gd[NS];
-        fullGrad<NS, T>(type, xs, p, gd);
+        if constexpr (TYPE == 0)
+            fullGrad<NS, T>(type, xs, p, gd);
+        else
+            fullGradOf<TYPE, T>(xs, p, gd);
         g = gd[LOCAL];
     } else {
         using D3 = ad::Dual3T<T>;
@@ -193,7 +197,10 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
             const T o = s == LOCAL ? T(1) : T(0);
             x[s] = {D3(xs[s].x, o, T(0), T(0)), D3(xs[s].y, T(0), o, T(0)), D3(xs[s].z, T(0), T(0), o)};
         }
-        fullGrad<NS, D3>(type, x, p, gd);
+        if constexpr (TYPE == 0)
+            fullGrad<NS, D3>(type, x, p, gd);
+        else
+            fullGradOf<TYPE, D3>(x, p, gd);
         const V3<D3> &gl = gd[LOCAL];
         g = {gl.x.v, gl.y.v, gl.z.v};
         const T h = T(0.5);
@@ -201,30 +208,51 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
              h * (gl.y.d[2] + gl.z.d[1]), gl.z.d[2]};
     }
 }
-template <int NS, class T, bool HESS>
+template <int NS, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlotAny(int type, int local, const float4 *__restrict__ pos, int4 v,
                                            const double *__restrict__ p, V3<T> &g, Sym3<T> &H)
 {
     if constexpr (NS == 2) {
         if (local == 0)
-            extSlot<2, 0, T, HESS>(type, pos, v, p, g, H);
+            extSlot<2, 0, T, HESS, TYPE>(type, pos, v, p, g, H);
         else
-            extSlot<2, 1, T, HESS>(type, pos, v, p, g, H);
+            extSlot<2, 1, T, HESS, TYPE>(type, pos, v, p, g, H);
     } else if constexpr (NS == 3) {
         if (local == 0)
-            extSlot<3, 0, T, HESS>(type, pos, v, p, g, H);
+            extSlot<3, 0, T, HESS, TYPE>(type, pos, v, p, g, H);
         else if (local == 1)
-            extSlot<3, 1, T, HESS>(type, pos, v, p, g, H);
+            extSlot<3, 1, T, HESS, TYPE>(type, pos, v, p, g, H);
         else
-            extSlot<3, 2, T, HESS>(type, pos, v, p, g, H);
+            extSlot<3, 2, T, HESS, TYPE>(type, pos, v, p, g, H);
     } else {
         switch (local) {
-        case 0: extSlot<4, 0, T, HESS>(type, pos, v, p, g, H); break;
-        case 1: extSlot<4, 1, T, HESS>(type, pos, v, p, g, H); break;
-        case 2: extSlot<4, 2, T, HESS>(type, pos, v, p, g, H); break;
-        default: extSlot<4, 3, T, HESS>(type, pos, v, p, g, H);
+        case 0: extSlot<4, 0, T, HESS, TYPE>(type, pos, v, p, g, H); break;
+        case 1: extSlot<4, 1, T, HESS, TYPE>(type, pos, v, p, g, H); break;
+        case 2: extSlot<4, 2, T, HESS, TYPE>(type, pos, v, p, g, H); break;
+        default: extSlot<4, 3, T, HESS, TYPE>(type, pos, v, p, g, H);
         }
     }
+}
+
+// The entries of one further type in a slot's row (rows are sorted by type; `word` packs four 8-bit counts):
+// G lanes stride over them with the type's own code, and `base` moves past them.
+template <int TYPE, int G>
+__device__ __forceinline__ void extTypeRun(const SolveArgs &a, int32_t &base, uint32_t word, int32_t lig, V3f &g,
+                                           S3f &H)
+{
+    const int32_t n = static_cast<int32_t>((word >> (8 * ((TYPE - 6) & 3))) & 255u);
+    if (a.extTypes & (1u << TYPE)) {
+        for (int32_t k = lig; k < n; k += G) {
+            const int2 in = a.extInc[base + k];
+            V3f ge;
+            S3f He;
+            extSlotAny<fullStencilSize(TYPE), float, true, TYPE>(TYPE, in.y & 3, a.pos, a.extV[in.x],
+                                                                  a.extP + static_cast<int64_t>(kExtParams) * in.x,
+                                                                  ge, He);
+            acc(g, H, ge, He);
+        }
+    }
+    base += n;
 }
 template <class T, bool HESS>
 __device__ __forceinline__ void extIncidence(int2 in, const float4 *__restrict__ pos, const int4 *__restrict__ extV,
@@ -350,12 +378,16 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             acc(g, H, ge, He);
         }
         if constexpr (EXT) {
-            for (int32_t k = a.extPtr[vid] + lig; k < a.extPtr[vid + 1]; k += G) {
-                V3f ge;
-                S3f He;
-                extIncidence<float, true>(a.extInc[k], a.pos, a.extV, a.extP, ge, He);
-                acc(g, H, ge, He);
-            }
+            int32_t base = a.extPtr[vid];
+            const uint2 cnt = a.extCnt[vid];
+            extTypeRun<6, G>(a, base, cnt.x, lig, g, H);
+            extTypeRun<7, G>(a, base, cnt.x, lig, g, H);
+            extTypeRun<8, G>(a, base, cnt.x, lig, g, H);
+            extTypeRun<9, G>(a, base, cnt.x, lig, g, H);
+            extTypeRun<10, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<11, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<12, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<13, G>(a, base, cnt.y, lig, g, H);
         }
     }
     if (G > 1) {

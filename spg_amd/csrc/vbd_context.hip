@@ -935,11 +935,14 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                 int32_t sl[4] = {0, 0, 0, 0};
                 for (int k = 0; k < ss; ++k) sl[k] = c.slotOfGid[base + st[k]];
                 int32_t elem = 0;
-                switch (en.type) {
+                // SpringContinuumEnergy is a spring: 1/2 Keff (|d| / L0 - 1)^2 = 1/2 (Keff / L0^2) (|d| - L0)^2, so it
+                // joins the spring class (closed-form blocks) with stiffness Keff / L0^2
+                const int cls = en.type == SPG_VBD_SPRING_CONTINUUM ? SPG_VBD_SPRING : en.type;
+                switch (cls) {
                 case SPG_VBD_SPRING:
                     elem = static_cast<int32_t>(sprV.size());
                     sprV.push_back(make_int2(sl[0], sl[1]));
-                    sprP.push_back(make_double2(p[0], p[1]));
+                    sprP.push_back(cls == en.type ? make_double2(p[0], p[1]) : make_double2(p[0], p[1] / (p[0] * p[0])));
                     break;
                 case SPG_VBD_SPRING_ANCHOR:
                     elem = static_cast<int32_t>(ancP.size());
@@ -967,12 +970,12 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                     extPh.insert(extPh.end(), kExtParams - np, 0.0);
                 }
                 for (int k = 0; k < ss; ++k) {
-                    if (isFurther(en.type))
+                    if (isFurther(cls))
                         extIncList.push_back({sl[k], elem, (en.type << 2) | k});
                     else
-                        inc[en.type].push_back({sl[k], elem, k});
+                        inc[cls].push_back({sl[k], elem, k});
                     const int32_t col = c.localColors[base + st[k]];
-                    colorRecordBytes[col] += recordBytes(en.type) + 4.0;
+                    colorRecordBytes[col] += recordBytes(cls) + 4.0;
                     colorIncidences[col] += 1;
                 }
             }

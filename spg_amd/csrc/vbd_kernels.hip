@@ -2,6 +2,8 @@
 // velocity update (K3), FP64 residual (K4) and the per-element derivative hook.
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "element.h"
 #include "element_eval.h"
 #include "implicit_element.h"
@@ -170,6 +172,65 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
 // tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template parameter so
 // the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in scratch memory)
 // and the other slots' outputs are dropped. T: float (solve) or double (residual).
+// Baraff-Witkin bending (baraffWitkinBendingEnergy.cpp:9-36), the gradient and 3x3 block of one hinge slot in
+// closed form. With e0 = x1 - x0, e3 = x2 - x1, e4 = x3 - x1 and s0 = 1/|e0| held constant:
+//   X = n1 . n2 = -[(e0.e0)(e3.e4) - (e0.e4)(e0.e3)]          (Binet-Cauchy identity)
+//   Y = s0 (n1 x n2) . e0 = -s0 (e0.e0) D,  D = e0 . (e3 x e4)  (n1 x n2 = -D e0)
+// both polynomials in (e0, e3, e4); theta = atan2(Y, X), E = 1/2 k (theta - theta0)^2. A slot's block sums the
+// second derivatives of X and Y over the edge vectors it moves (x0: -e0; x1: e0 - e3 - e4; x2: e3; x3: e4); in the
+// symmetric sum the skew parts cancel and the wing slots (x2, x3), on which X and Y are linear, get none.
+template <int LOCAL>
+__device__ __forceinline__ void bendingBwSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    const V3f e0 = x[1] - x[0], e3 = x[2] - x[1], e4 = x[3] - x[1];
+    const float q = dot(e0, e0), s0 = 1.f / sqrtf(q);
+    const float d34 = dot(e3, e4), d03 = dot(e0, e3), d04 = dot(e0, e4);
+    const V3f g0 = cross(e3, e4), g3 = cross(e4, e0), g4 = cross(e0, e3);
+    const float D = dot(e0, g0);
+    const float X = -(q * d34 - d04 * d03), Y = -s0 * q * D;
+    V3f gX, gY;
+    S3f HX = zeroSym<float>(), HY = zeroSym<float>();
+    if constexpr (LOCAL == 0 || LOCAL == 1) {
+        const V3f X0 = (-2.f * d34) * e0 + d03 * e4 + d04 * e3;
+        const V3f Y0 = (-s0) * ((2.f * D) * e0 + q * g0);
+        addSymOuter(HX, 1.f, e4, e3);
+        if constexpr (LOCAL == 0) {
+            gX = -X0;
+            gY = -Y0;
+            addDiag(HX, -2.f * d34);
+            addDiag(HY, -2.f * s0 * D);
+            addSymOuter(HY, -2.f * s0, e0, g0);
+        } else {
+            gX = X0 - (d04 * e0 - q * e4) - (d03 * e0 - q * e3);
+            gY = Y0 + (s0 * q) * (g3 + g4);
+            addDiag(HX, -2.f * (d34 + d04 + d03 + q));
+            addSymOuter(HX, 1.f, e0, e4);
+            addSymOuter(HX, 1.f, e0, e3);
+            addOuter(HX, 2.f, e0);
+            addDiag(HY, -2.f * s0 * D);
+            addSymOuter(HY, -2.f * s0, e0, g0 - g3 - g4);
+        }
+    } else if constexpr (LOCAL == 2) {
+        gX = d04 * e0 - q * e4;
+        gY = (-s0 * q) * g3;
+    } else {
+        gX = d03 * e0 - q * e3;
+        gY = (-s0 * q) * g4;
+    }
+    const float ir2 = 1.f / (X * X + Y * Y);
+    const V3f gt = ir2 * (X * gY - Y * gX);  // grad theta
+    const V3f u = X * gX + Y * gY;
+    const float k = static_cast<float>(p[1]);
+    const float kd = k * (atan2f(Y, X) - static_cast<float>(p[0]));
+    g = kd * gt;
+    // k grad theta grad theta^T + k (theta - theta0) [(X HY - Y HX) / r2 - sym(grad theta, u) / r2]
+    const float a = kd * ir2;
+    H = {a * (X * HY.xx - Y * HX.xx), a * (X * HY.xy - Y * HX.xy), a * (X * HY.xz - Y * HX.xz),
+         a * (X * HY.yy - Y * HX.yy), a * (X * HY.yz - Y * HX.yz), a * (X * HY.zz - Y * HX.zz)};
+    addSymOuter(H, -a, gt, u);
+    addOuter(H, k, gt);
+}
+
 // TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
@@ -189,6 +250,8 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
         else
             fullGradOf<TYPE, T>(xs, p, gd);
         g = gd[LOCAL];
+    } else if constexpr (TYPE == 11 && std::is_same_v<T, float>) {
+        bendingBwSlot<LOCAL>(xs, p, g, H);
     } else {
         using D3 = ad::Dual3T<T>;
         V3<D3> x[NS], gd[NS];

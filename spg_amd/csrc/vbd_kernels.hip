@@ -231,6 +231,32 @@ __device__ __forceinline__ void bendingBwSlot(const V3f *x, const double *p, V3f
     addOuter(H, k, gt);
 }
 
+// Baraff-Witkin membrane (membraneBaraffWitkinEnergy.cpp:9-68), one triangle slot in closed form. F = [f0 f1] =
+// [x1 - x0, x2 - x0] Dinv, C = (|f0| - 1, |f1| - 1, f0 . f1), E = 1/2 sum k_i C_i^2 (k_i the effective stiffness
+// diagonal, area included); slot a moves f0 by gamma_a0 and f1 by gamma_a1 (the rows of Dinv, gamma_0 their
+// negated sum), so its block is gamma_a0^2 (A0 + k2 f1 f1^T) + gamma_a1^2 (A1 + k2 f0 f0^T)
+// + gamma_a0 gamma_a1 k2 (2 C2 I + f0 f1^T + f1 f0^T) with Ai = k_i (fi fi^T / |fi|^3 + C_i / |fi| I).
+template <int LOCAL>
+__device__ __forceinline__ void membraneBwSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    const float d0 = float(p[0]), d1 = float(p[1]), d2 = float(p[2]), d3 = float(p[3]);
+    const V3f u = x[1] - x[0], v = x[2] - x[0];
+    const V3f f0 = d0 * u + d2 * v, f1 = d1 * u + d3 * v;
+    const float k0 = float(p[4]), k1 = float(p[5]), k2 = float(p[6]);
+    const float n0 = sqrtf(dot(f0, f0)), n1 = sqrtf(dot(f1, f1));
+    const float c0 = n0 - 1.f, c1 = n1 - 1.f, c2 = dot(f0, f1);
+    const float ga = LOCAL == 0 ? -(d0 + d2) : LOCAL == 1 ? d0 : d2;
+    const float gb = LOCAL == 0 ? -(d1 + d3) : LOCAL == 1 ? d1 : d3;
+    const float i0 = 1.f / n0, i1 = 1.f / n1;
+    const V3f dEdf0 = (k0 * c0 * i0) * f0 + (k2 * c2) * f1, dEdf1 = (k1 * c1 * i1) * f1 + (k2 * c2) * f0;
+    g = ga * dEdf0 + gb * dEdf1;
+    H = zeroSym<float>();
+    addDiag(H, ga * ga * k0 * c0 * i0 + gb * gb * k1 * c1 * i1 + 2.f * ga * gb * k2 * c2);
+    addOuter(H, ga * ga * k0 * i0 * i0 * i0 + gb * gb * k2, f0);
+    addOuter(H, gb * gb * k1 * i1 * i1 * i1 + ga * ga * k2, f1);
+    addSymOuter(H, ga * gb * k2, f0, f1);
+}
+
 // TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
@@ -252,6 +278,8 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
         g = gd[LOCAL];
     } else if constexpr (TYPE == 11 && std::is_same_v<T, float>) {
         bendingBwSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 8 && std::is_same_v<T, float>) {
+        membraneBwSlot<LOCAL>(xs, p, g, H);
     } else {
         using D3 = ad::Dual3T<T>;
         V3<D3> x[NS], gd[NS];

@@ -48,8 +48,9 @@ double recordBytes(int type)
     case SPG_VBD_STABLE_NEO_HOOKEAN: return 64;
     case SPG_VBD_SPRING_ANCHOR: return 20;
     }
-    // further energies: four slots, the padded parameters, and 4 B more per incidence record
-    return isFurther(type) ? 16.0 + 8.0 * kExtParams + 4.0 : 0.0;
+    // further energies: the four slots, the parameters the energy reads (not the padding), and 4 B more for the
+    // 8-byte row entry (the caller adds 4 B per incidence)
+    return isFurther(type) ? 16.0 + 8.0 * fullParams(type) + 4.0 : 0.0;
 }
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }

@@ -210,7 +210,8 @@ int spg_vbd_bench_copy(int64_t bytes, int32_t repeats, int32_t device, double *m
    FP32 (precision 0) or FP64 (1) by the same device functions the solve uses, on GPU `device`, or
    on the host when device is -1. form 0 selects the closed forms of the solve; forms 1 (general
    (a, b, sigma) hinge form) and 2 (single-expression compact hinge form) are independent derivations
-   of the hinge derivatives, for cross-checking. */
+   of the hinge derivatives, for cross-checking. The further energies (types 6 ... 13) exist in the FP32
+   solve only: precision 0, form 0, device >= 0 (SPG_VBD_ERR_UNSUPPORTED otherwise). */
 int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const double *params,
                                 int32_t precision, int32_t form, int32_t device, double *grad,
                                 double *hess_blocks);

@@ -1767,6 +1767,30 @@ int spg_vbd_element_derivatives(int32_t type, int64_t n, const double *xs, const
             throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
         if (form < 0 || form > 2) throw VbdError(SPG_VBD_ERR_INVALID, "form must be 0, 1 or 2");
         if (n == 0) return;
+        if (isFurther(type)) {
+            // the further energies' slot blocks exist in the FP32 solve only (device code)
+            if (device < 0 || precision != 0 || form != 0)
+                throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "further energies: FP32 solve forms on a GPU only");
+            hipCheck(hipSetDevice(device), "hipSetDevice");
+            hipStream_t s;
+            hipCheck(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+            std::vector<float4> hp(n * ss);
+            for (int64_t k = 0; k < n * ss; ++k)
+                hp[k] = make_float4(float(xs[3 * k]), float(xs[3 * k + 1]), float(xs[3 * k + 2]), 0.f);
+            DevBuf<float4> dx;
+            DevBuf<double> dp, dg, dh;
+            dx.upload(hp, s);
+            dp.upload(std::vector<double>(params, params + n * np), s);
+            dg.alloc(n * ss * 3);
+            dh.alloc(n * ss * 9);
+            hipCheck(launchExtElementDerivatives(type, n, dx.p, dp.p, np, dg.p, dh.p, s), "element launch");
+            hipCheck(hipMemcpyAsync(grad, dg.p, sizeof(double) * n * ss * 3, hipMemcpyDeviceToHost, s), "download");
+            hipCheck(hipMemcpyAsync(hess_blocks, dh.p, sizeof(double) * n * ss * 9, hipMemcpyDeviceToHost, s),
+                     "download");
+            hipCheck(hipStreamSynchronize(s), "hipStreamSynchronize");
+            (void)hipStreamDestroy(s);
+            return;
+        }
         if (device == -1) {
             // host instantiation of the same closed forms (derivative checks only)
             for (int64_t i = 0; i < n; ++i) {

@@ -132,6 +132,8 @@ hipError_t launchReduceObjects(const double *r2, const int64_t *segBegin, int32_
                                int32_t nObjects, double *partial, double *out, hipStream_t stream);
 hipError_t launchCopyXt(float4 *pos, const float4 *xt, int32_t n, hipStream_t stream);
 hipError_t launchCopy(const float4 *src, float4 *dst, int64_t n, hipStream_t stream);
+hipError_t launchExtElementDerivatives(int type, int64_t n, const float4 *pos, const double *prm, int np,
+                                       double *grad, double *hess, hipStream_t stream);
 hipError_t launchElementDerivatives(int type, int64_t n, const double *xs, const double *params, int precision,
                                     int form, double *grad, double *hess, hipStream_t stream);
 

@@ -257,6 +257,61 @@ __device__ __forceinline__ void membraneBwSlot(const V3f *x, const double *p, V3
     addSymOuter(H, ga * gb * k2, f0, f1);
 }
 
+// k (f f^T / |f|^3 + (|f| - 1) / |f| I) and k (|f| - 1) f / |f|, scaled by c^2 and c: one unit-stretch term
+// 1/2 k (|f| - 1)^2 of a membrane whose f moves by c times the slot's displacement.
+__device__ __forceinline__ void unitStretchSlot(V3f f, float k, float c, V3f &g, S3f &H)
+{
+    const float n = sqrtf(dot(f, f)), in = 1.f / n;
+    g = g + (c * k * (n - 1.f) * in) * f;
+    addDiag(H, c * c * k * (n - 1.f) * in);
+    addOuter(H, c * c * k * in * in * in, f);
+}
+
+// Choi membrane (membraneChoiEnergy.cpp:9-28): four unit stretches, of f0, f1, gp = (f0 + f1)/sqrt 2 and
+// gm = (f0 - f1)/sqrt 2; slot a moves them by gamma_a0, gamma_a1, (gamma_a0 +- gamma_a1)/sqrt 2.
+template <int LOCAL>
+__device__ __forceinline__ void membraneChoiSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    const float d0 = float(p[0]), d1 = float(p[1]), d2 = float(p[2]), d3 = float(p[3]);
+    const V3f u = x[1] - x[0], v = x[2] - x[0];
+    const V3f f0 = d0 * u + d2 * v, f1 = d1 * u + d3 * v;
+    const float ga = LOCAL == 0 ? -(d0 + d2) : LOCAL == 1 ? d0 : d2;
+    const float gb = LOCAL == 0 ? -(d1 + d3) : LOCAL == 1 ? d1 : d3;
+    const float r = 0.70710678118654752440f;
+    g = {0.f, 0.f, 0.f};
+    H = zeroSym<float>();
+    unitStretchSlot(f0, float(p[4]), ga, g, H);
+    unitStretchSlot(f1, float(p[5]), gb, g, H);
+    unitStretchSlot(r * (f0 + f1), float(p[6]), r * (ga + gb), g, H);
+    unitStretchSlot(r * (f0 - f1), float(p[7]), r * (ga - gb), g, H);
+}
+
+// Quadratic bending (quadraticBendingEnergy.cpp:9-30): E = 1/2 keff |C|^2 with C = sum c_s x_s, so slot a has
+// gradient keff c_a C and block keff c_a^2 I.
+template <int LOCAL>
+__device__ __forceinline__ void bendingQuadraticSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    const V3f C = float(p[0]) * x[0] + float(p[1]) * x[1] + float(p[2]) * x[2] + float(p[3]) * x[3];
+    const float kc = float(p[4]) * float(p[LOCAL]);
+    g = kc * C;
+    H = zeroSym<float>();
+    addDiag(H, kc * float(p[LOCAL]));
+}
+
+// Squared continuum spring (springSquaredContinuumEnergy.cpp:9-19): C = |d|^2 / L0^2 - 1, E = 1/2 Keff C^2,
+// d = x0 - x1: both slots have the block Keff ((2/L0^2)^2 d d^T + C (2/L0^2) I).
+template <int LOCAL>
+__device__ __forceinline__ void springSquaredContinuumSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    const V3f d = x[0] - x[1];
+    const float L2 = float(p[0]) * float(p[0]), k = float(p[1]);
+    const float C = dot(d, d) / L2 - 1.f, s = 2.f / L2;
+    g = ((LOCAL == 0 ? 1.f : -1.f) * k * C * s) * d;
+    H = zeroSym<float>();
+    addDiag(H, k * C * s);
+    addOuter(H, k * s * s, d);
+}
+
 // TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
@@ -280,6 +335,12 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
         bendingBwSlot<LOCAL>(xs, p, g, H);
     } else if constexpr (TYPE == 8 && std::is_same_v<T, float>) {
         membraneBwSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 9 && std::is_same_v<T, float>) {
+        membraneChoiSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 10 && std::is_same_v<T, float>) {
+        bendingQuadraticSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 7 && std::is_same_v<T, float>) {
+        springSquaredContinuumSlot<LOCAL>(xs, p, g, H);
     } else {
         using D3 = ad::Dual3T<T>;
         V3<D3> x[NS], gd[NS];
@@ -799,6 +860,52 @@ hipError_t launchReduceObjects(const double *r2, const int64_t *segBegin, int32_
     if (nObjects <= 0 || nSegments <= 0) return hipSuccess;
     reduceSegmentsKernel<<<nSegments, 256, 0, stream>>>(r2, segBegin, partial);
     reduceObjectsKernel<<<blocksFor(nObjects, 256), 256, 0, stream>>>(partial, objSeg, nObjects, out);
+    return hipGetLastError();
+}
+
+// The solve's slot gradients and blocks of further-energy elements (types 6 ... 13), for derivative checks:
+// element i has its stencil's positions at pos[NS i ... NS i + NS - 1] and its prepared parameters at prm + np i.
+template <int TYPE>
+__global__ void __launch_bounds__(64) extElementKernel(int64_t n, const float4 *__restrict__ pos,
+                                                       const double *__restrict__ prm, int np, double *grad,
+                                                       double *hess)
+{
+    constexpr int NS = fullStencilSize(TYPE);
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t b = static_cast<int32_t>(i * NS);
+    const int4 v = make_int4(b, b + 1, NS > 2 ? b + 2 : -1, NS > 3 ? b + 3 : -1);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        V3f g;
+        S3f H;
+        extSlotAny<NS, float, true, TYPE>(TYPE, s, pos, v, prm + i * np, g, H);
+        double *gd = grad + (i * NS + s) * 3, *hd = hess + (i * NS + s) * 9;
+        gd[0] = g.x;
+        gd[1] = g.y;
+        gd[2] = g.z;
+        const float hv[9] = {H.xx, H.xy, H.xz, H.xy, H.yy, H.yz, H.xz, H.yz, H.zz};
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hd[k] = hv[k];
+    }
+}
+
+hipError_t launchExtElementDerivatives(int type, int64_t n, const float4 *pos, const double *prm, int np,
+                                       double *grad, double *hess, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = blocksFor(n, 64);
+    switch (type) {
+    case 6: extElementKernel<6><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 7: extElementKernel<7><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 8: extElementKernel<8><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 9: extElementKernel<9><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 10: extElementKernel<10><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 11: extElementKernel<11><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 12: extElementKernel<12><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    case 13: extElementKernel<13><<<blocks, 64, 0, stream>>>(n, pos, prm, np, grad, hess); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -135,3 +135,21 @@ def test_xpbd_further_energies_step(name, parallel, oracle_threads):
     print(f"{name} parallel={parallel}: position rel {ex:.2e}, velocity rel {ev:.2e}, FP64 solves "
           f"{st['fp64_solves']}", flush=True)
     assert ex < 1e-5 and ev < 1e-5
+
+
+def test_further_slot_blocks_on_gpu_match_oracle():
+    """The solve's per-slot gradients and 3x3 blocks of the further energies (closed forms for the squared continuum
+    spring, both membranes and both bendings; the three-tangent dual pass for the rest), evaluated by the device code
+    in FP32, against the oracle's second-order dual numbers. Blocks within 1e-4 of their max-norm; gradients within
+    1e-4 of theirs plus 1e-6 of (block max-norm x element size): near rest a gradient is a small difference of FP32
+    terms of that size (e.g. |d|/L0 - 1), the FP32 rounding of the constraint."""
+    import test_native_cpu as N
+    for kind, x, p in N.ELEMENTS:
+        g, h = native.element_derivatives(kind, x.ravel()[None], p[None], precision=0, device=0)
+        _, gr, hr = ol.element(kind, x.ravel(), p, mode=ol.AD)
+        size = np.abs(x - x.mean(axis=0)).max()
+        dg = np.abs(g[0].ravel() - gr).max()
+        eh = np.abs(h[0] - hr).max() / np.abs(hr).max()
+        bar = 1e-4 * np.abs(gr).max() + 1e-6 * np.abs(hr).max() * size
+        print(f"{kind}: fp32 grad error {dg:.2e} (bar {bar:.2e}), rel blocks {eh:.2e}", flush=True)
+        assert dg <= bar and eh < 1e-4, kind

@@ -312,6 +312,75 @@ __device__ __forceinline__ void springSquaredContinuumSlot(const V3f *x, const d
     addOuter(H, k * s * s, d);
 }
 
+// Tetrahedra: F = [f0 f1 f2] = [x1 - x0, x2 - x0, x3 - x0] Dinv and slot a moves column c by gamma_ac (row a - 1
+// of Dinv, gamma_0 the negated column sums), so it moves F by u gamma^T and w = F gamma is F's rate.
+template <int LOCAL>
+__device__ __forceinline__ void tetFrame(const V3f *x, const double *p, V3f *f, float *gm)
+{
+    const V3f u = x[1] - x[0], v = x[2] - x[0], w = x[3] - x[0];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f[c] = float(p[c]) * u + float(p[3 + c]) * v + float(p[6 + c]) * w;
+        gm[c] = LOCAL == 0 ? -float(p[c] + p[3 + c] + p[6 + c]) : float(p[3 * (LOCAL - 1) + c]);
+    }
+}
+
+// StVK tetrahedron (stvkEnergy.cpp:41-63, energy form): P = F S, S = lambda tr(E) I + 2 mu E; the slot block is
+// (gamma^T S gamma) I + (lambda + mu) w w^T + mu |gamma|^2 F F^T.
+template <int LOCAL>
+__device__ __forceinline__ void tetStvkSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    V3f f[3];
+    float gm[3];
+    tetFrame<LOCAL>(x, p, f, gm);
+    const float lam = float(p[9]), mu = float(p[10]);
+    float S[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) S[i][j] = mu * (dot(f[i], f[j]) - (i == j ? 1.f : 0.f));
+    const float lt = 0.5f * lam * (dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.f);
+    S[0][0] += lt;
+    S[1][1] += lt;
+    S[2][2] += lt;
+    float Sg[3], gSg = 0.f;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        Sg[i] = S[i][0] * gm[0] + S[i][1] * gm[1] + S[i][2] * gm[2];
+        gSg += gm[i] * Sg[i];
+    }
+    g = Sg[0] * f[0] + Sg[1] * f[1] + Sg[2] * f[2];
+    const V3f w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
+    const float g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
+    H = zeroSym<float>();
+    addDiag(H, gSg);
+    addOuter(H, lam + mu, w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) addOuter(H, mu * g2, f[c]);
+}
+
+// Stable squared Neo-Hookean (stableSquaredNeoHookeanEnergy.cpp:9-31): E = 1/2 lambda (J - 1)^2
+// + 1/2 mu (|F|^2 - 3)^2; J is affine along the slot's rank-one update, so the block is lambda c c^T
+// + mu (4 w w^T + 2 (|F|^2 - 3) |gamma|^2 I) with c = cof(F) gamma.
+template <int LOCAL>
+__device__ __forceinline__ void tetSsnhSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+{
+    V3f f[3];
+    float gm[3];
+    tetFrame<LOCAL>(x, p, f, gm);
+    const float lam = float(p[9]), mu = float(p[10]);
+    const V3f k0 = cross(f[1], f[2]), k1 = cross(f[2], f[0]), k2 = cross(f[0], f[1]);
+    const float J = dot(f[0], k0), s = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.f;
+    const V3f c = gm[0] * k0 + gm[1] * k1 + gm[2] * k2;
+    const V3f w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
+    const float g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
+    g = (lam * (J - 1.f)) * c + (2.f * mu * s) * w;
+    H = zeroSym<float>();
+    addOuter(H, lam, c);
+    addOuter(H, 4.f * mu, w);
+    addDiag(H, 2.f * mu * s * g2);
+}
+
 // TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
@@ -341,6 +410,10 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
         bendingQuadraticSlot<LOCAL>(xs, p, g, H);
     } else if constexpr (TYPE == 7 && std::is_same_v<T, float>) {
         springSquaredContinuumSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 12 && std::is_same_v<T, float>) {
+        tetStvkSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE == 13 && std::is_same_v<T, float>) {
+        tetSsnhSlot<LOCAL>(xs, p, g, H);
     } else {
         using D3 = ad::Dual3T<T>;
         V3<D3> x[NS], gd[NS];

@@ -24,6 +24,12 @@
  *   spg/source/sim/solver/implicitEuler{BaraffWitkin,NewtonDv,NewtonDx,NewtonRobust}.cpp, bdf2.cpp
  *   Eigen's ConjugateGradient with DiagonalPreconditioner (an empty submodule of the reference; its
  *   published algorithm is restated: zero initial guess, threshold max(tol^2 |b|^2, DBL_MIN))
+ * and VBD on rigid bodies (6-DOF blocks):
+ *   spg/source/sim/solver/vbd.cpp:62-72,105-145,173-185 (the RigidBodyGroup branches)
+ *   spg/source/sim/simObject/rigidBodyGroup.cpp:6-243   (axis-angle maps, integration, velocities, inertia)
+ *   spg/source/sim/energy/rigidBody{Spring,SpringSquared,SpringAnchor,OrientationAlignment}Energy.cpp
+ *   (the rotation maps restate Eigen's AngleAxis arithmetic: Rodrigues, and the quaternion of R by the
+ *   largest-of-trace-and-diagonal rule, Eigen being an empty submodule of the reference)
  * with two derivative back-ends: closed-form per-vertex blocks, and the oracle's own
  * forward-mode second-order dual numbers that evaluate the full element derivative twice per
  * incidence, as the reference does through TinyAD.
@@ -154,6 +160,19 @@ void *orc_implicit_new(void);
 void orc_implicit_free(void *s);
 int orc_implicit_step(void *h, void *state, int solver, double dt, int nsubsteps, const double *gravity,
                       int64_t *counters, double *current_dt);
+
+/* ---- rigid-body VBD (types 21 spring, 22 squared spring, 23 anchor, 24 orientation alignment; raw addStencil
+   arguments: 8, 8, 7 and 1 doubles) ---- */
+void *orc_rb_new(int64_t nb, const double *x, const double *theta, const double *m, const double *inertia);
+void orc_rb_free(void *h);
+int orc_rb_add_energy(void *h, int type, int64_t n, const int32_t *stencils, const double *raw);
+int orc_rb_step(void *h, double dt, int nsubsteps, int iterations, const double *gravity);
+int orc_rb_state(void *h, double *x, double *theta, double *v, double *omega);
+int orc_rb_set_state(void *h, const double *x, const double *theta, const double *v, const double *omega);
+int orc_rb_colors(void *h, int32_t *colors, int32_t *ncolors);
+int orc_rb_element(int type, const double *com, const double *rot, const double *raw, double *grad, double *hess);
+int orc_rb_aa_to_r(const double *theta, double *R);
+int orc_rb_r_to_aa(const double *R, double *theta);
 
 #ifdef __cplusplus
 }

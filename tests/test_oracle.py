@@ -296,3 +296,96 @@ def test_error_messages():
     o3.add_energy("stvk", [[0, 1, 2]], [[1.0, 0.3]])
     with pytest.raises(ol.OracleError, match="not positive"):
         o3.prepare()
+
+
+# ---- the FP64 force residual r(x) = -grad E(x) - (M/h^2)(x - x~) (implicitEulerBase.cpp:102-143, x-form of
+# implicitEulerNewtonRobust.cpp:65-66), pinned against the oracle's own energy by central differences, against
+# a residual written by hand, and against the exact one-iteration solution of an anchored particle ----
+
+def _fd_residual(o, x, xt, h, m, eps):
+    g = np.zeros_like(x)
+    for i in range(x.shape[0]):
+        for c in range(3):
+            xp = x.copy()
+            xp[i, c] += eps
+            xm = x.copy()
+            xm[i, c] -= eps
+            g[i, c] = (o.energy(xp) - o.energy(xm)) / (2.0 * eps)
+    return -g - (m[:, None] / h ** 2) * (x - xt)
+
+
+def _residual_scene(name):
+    from spg_amd import scenes
+    if name == "cloth":
+        # StVK membrane + discrete bending + anchors (the energies of the 512x512 cloth), soft anchors so no
+        # single term dominates the comparison
+        s = scenes.cloth(6, young=1e4, bending=1e-3, pin="top_row", anchor_k=10.0)
+    elif name == "cube":
+        s = scenes.cube(2, young=1e4, anchor_k=10.0)
+    else:  # every energy type of the survey's list that a spring mesh adds
+        s = scenes.cloth(5, young=1e3, pin="corners", anchor_k=10.0, edge_springs="spring_continuum")
+    return s
+
+
+@pytest.mark.parametrize("name", ["cloth", "cube", "springs"])
+@pytest.mark.parametrize("mode", [ol.CLOSED_FORM, ol.AD], ids=["closed", "ad"])
+def test_residual_matches_energy_differences(name, mode):
+    s = _residual_scene(name)
+    o = ol.from_scene(s)
+    rng = np.random.default_rng(17)
+    x = s.x + rng.uniform(-0.02, 0.02, s.x.shape)
+    xt = x + rng.uniform(-0.01, 0.01, s.x.shape)
+    h = 1.0 / 60.0
+    nrm, r = o.residual(x, xt, h, mode=mode, return_vector=True)
+    r_fd = _fd_residual(o, x, xt, h, s.m, 1e-6)
+    scale = np.max(np.abs(r_fd))
+    assert scale > 0
+    assert np.max(np.abs(r - r_fd)) <= 1e-8 * scale
+    # the norm is the 2-norm of that vector
+    assert abs(nrm - np.linalg.norm(r)) <= 1e-13 * nrm
+    assert abs(nrm - np.linalg.norm(r_fd)) <= 1e-8 * nrm
+    # each part carries weight here: dropping the inertia term or the elastic forces breaks the match
+    inertia = -(s.m[:, None] / h ** 2) * (x - xt)
+    assert np.max(np.abs((r - inertia) - r_fd)) > 1e-3 * scale
+    assert np.max(np.abs(inertia - r_fd)) > 1e-3 * scale
+
+
+def test_residual_two_particle_spring_by_hand():
+    x = np.array([[0.0, 0.0, 0.0], [1.2, 0.3, -0.1]])
+    xt = np.array([[0.01, -0.02, 0.0], [1.1, 0.35, -0.05]])
+    m = np.array([2.0, 0.5])
+    o = ol.Oracle(x, x, m)
+    o.add_energy("spring", [[0, 1]], [[1.0, 40.0]])
+    o.add_energy("anchor", [[0]], [[0.05, 0.0, 0.02, 7.0]])
+    o.prepare()
+    h = 0.02
+    nrm, r = o.residual(x, xt, h, return_vector=True)
+    s = x[0] - x[1]
+    L = np.linalg.norm(s)
+    fs = 40.0 * (L - 1.0) * s / L  # dE/dx0 of 1/2 k (|x0 - x1| - L0)^2
+    ref = -(m[:, None] / h ** 2) * (x - xt)
+    ref[0] -= fs + 7.0 * (x[0] - np.array([0.05, 0.0, 0.02]))
+    ref[1] += fs
+    assert np.max(np.abs(r - ref)) <= 1e-13 * np.max(np.abs(ref))
+    assert abs(nrm - np.linalg.norm(ref)) <= 1e-13 * nrm
+
+
+def test_residual_zero_at_anchored_particle_solution():
+    """One VBD iteration on a single anchored particle lands on the exact minimiser
+    (m/h^2 x~ + k a)/(m/h^2 + k), where r = -k (x - a) - (m/h^2)(x - x~) vanishes."""
+    m, k, h = 3e-3, 50.0, 1.0 / 60.0
+    a = np.array([0.3, -0.2, 0.7])
+    x = np.array([[0.1, 0.4, 0.5]])
+    o = ol.Oracle(x, x, np.array([m]))
+    o.add_energy("anchor", [[0]], [[*a, k]])
+    o.prepare()
+    g = np.array([0.0, -9.81, 0.0])
+    o.step(h, 1, 1, g)
+    xt = x + h * h * g  # v = 0: x~ = x + h (h g)
+    x1, _ = o.state()
+    md = m / h ** 2
+    assert np.max(np.abs(x1 - (md * xt + k * a) / (md + k))) < 1e-14
+    r0 = o.residual(xt, xt, h)
+    r1 = o.residual(x1, xt, h)
+    assert r0 > 1.0
+    assert r1 <= 1e-12 * r0

@@ -103,6 +103,9 @@ int spg_solver_set_iterations(spg_solver *s, int32_t iterations); /* VBD only */
 int spg_solver_set_resident_state(spg_solver *s, int32_t resident);
 int spg_solver_step(spg_solver *s);
 int spg_solver_reset(spg_solver *s);
+/* VBD::requirePrecomputationUpdate (spg/include/spg/sim/solver/vbd.h:26; XPBD and the implicit solvers alike):
+   the next step rebuilds colouring, incidence and device data from the groups' current state. */
+int spg_solver_require_precomputation_update(spg_solver *s);
 int spg_solver_sync_to_host(spg_solver *s);
 int spg_solver_get_state(spg_solver *s, int32_t object, double *x, double *v);
 /* BaseSolver::addObject(const RigidBodyGroup &) (baseSolver.cpp:10-14): copies the group, shares its energies.

@@ -359,6 +359,14 @@ int spg_solver_reset(spg_solver *s)
 {
     return guarded([&] { s->solver->reset(); });
 }
+int spg_solver_require_precomputation_update(spg_solver *s)
+{
+    return guarded([&] {
+        if (s->vbd) s->vbd->requirePrecomputationUpdate();
+        if (s->xpbd) s->xpbd->requirePrecomputationUpdate();
+        if (s->implicit) s->implicit->requirePrecomputationUpdate();
+    });
+}
 int spg_solver_sync_to_host(spg_solver *s)
 {
     return guarded([&] {

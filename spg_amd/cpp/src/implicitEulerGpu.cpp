@@ -96,6 +96,8 @@ void ImplicitEulerGpu::step()
     requireNoRigidBodies("ImplicitEulerGpu");
     if (m_objects.empty()) return;
     if (m_rebuild || !m_ctx) {
+        // resident state: the device holds the newest positions unless the host was edited since
+        if (m_ctx && m_resident && !m_hostDirty) syncToHost();
         buildContext();
         m_rebuild = false;
     } else if (!m_resident || m_hostDirty) {

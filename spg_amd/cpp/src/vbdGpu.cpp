@@ -111,6 +111,9 @@ void VBDGpu::step()
     const bool rebuild = m_precomputationUpdateRequired || (!m_objects.empty() && !m_ctx) ||
                          (!m_rbObjects.empty() && !m_rbCtx);
     if (rebuild) {
+        // with resident state the device holds the newest positions unless the host was edited since
+        // (markHostStateDirty): bring them back first, so the rebuilt contexts continue from them
+        if (m_resident && !m_hostDirty) syncToHost();
         if (!m_objects.empty()) buildContext();
         if (!m_rbObjects.empty()) buildRigidContext();
         m_hostDirty = false;
@@ -222,6 +225,12 @@ void VBDGpu::buildRigidContext()
     spg_rigid_ctx *ctx = nullptr;
     rigidCheck(spg_rigid_create(&ctx, groups.data(), static_cast<int32_t>(n), m_device), nullptr);
     m_rbCtx = ctx;
+    // create derives each rotation from theta; the group's rotation matrices are composed increments and
+    // may differ from aaToR(theta) after steps, so the context starts from the matrices the group holds
+    for (size_t g = 0; g < n; ++g) {
+        const auto R = packMat3(m_rbObjects[g].rotationMatrices());
+        rigidCheck(spg_rigid_set_rotations(m_rbCtx, static_cast<int32_t>(g), R.data()), m_rbCtx);
+    }
 }
 
 void VBDGpu::uploadRigidState()

@@ -99,6 +99,8 @@ void XPBDGpu::step()
     requireNoRigidBodies("XPBDGpu");
     if (m_objects.empty()) return;
     if (m_parallelGroupsDirty || !m_ctx) {
+        // resident state: the device holds the newest positions unless the host was edited since
+        if (m_ctx && m_resident && !m_hostDirty) syncToHost();
         buildContext();
         m_parallelGroupsDirty = false;
     } else if (!m_resident || m_hostDirty) {

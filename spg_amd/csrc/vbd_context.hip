@@ -134,7 +134,7 @@ struct spg_vbd_ctx {
     DevBuf<int2> extInc;
     DevBuf<int4> extV;
     DevBuf<double> extP;
-    DevBuf<uint2> extCnt;
+    DevBuf<uint4> extCnt;
     uint32_t extTypes = 0;
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
@@ -1141,7 +1141,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         auto order = sortBySlot(extIncList, c.nv, ptr);
         // within each row, entries of one type together (stable: element order kept), so the solve runs one
         // type-specialised loop per type; per slot the count of each type
-        std::vector<uint2> cnt(c.nv, make_uint2(0u, 0u));
+        std::vector<uint4> cnt(c.nv, make_uint4(0u, 0u, 0u, 0u));
         for (int64_t v = 0; v < c.nv; ++v) {
             std::stable_sort(order.begin() + ptr[v], order.begin() + ptr[v + 1], [&](int32_t i, int32_t j) {
                 return (extIncList[i].local >> 2) < (extIncList[j].local >> 2);
@@ -1149,11 +1149,11 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             uint32_t n[8] = {};
             for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
                 const int t = extIncList[order[k]].local >> 2;
-                if (++n[t - SPG_VBD_SPRING_CONTINUUM] > 255)
-                    throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "more than 255 incidences of one energy type at a particle");
+                if (++n[t - SPG_VBD_SPRING_CONTINUUM] > 0xffffu)
+                    throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "more than 65535 incidences of one energy type at a particle");
                 c.extTypes |= 1u << t;
             }
-            cnt[v] = make_uint2(n[0] | n[1] << 8 | n[2] << 16 | n[3] << 24, n[4] | n[5] << 8 | n[6] << 16 | n[7] << 24);
+            cnt[v] = make_uint4(n[0] | n[1] << 16, n[2] | n[3] << 16, n[4] | n[5] << 16, n[6] | n[7] << 16);
         }
         std::vector<int2> rec(order.size());
         for (size_t k = 0; k < order.size(); ++k)

@@ -69,7 +69,8 @@ struct SolveArgs {
     const int2 *extInc;     // each row sorted by type (element order kept within a type)
     const int4 *extV;
     const double *extP;
-    const uint2 *extCnt;    // per slot, the row's entry count of each type 6 ... 13, 8 bits each (x: 6-9, y: 10-13)
+    const uint4 *extCnt;    // per slot, the row's entry count of each type 6 ... 13, 16 bits each (x: 6-7, y: 8-9,
+                            // z: 10-11, w: 12-13)
     uint32_t extTypes;      // bit t set: type t occurs in some row
     unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
     int32_t begin, end;            // slot range of the colour

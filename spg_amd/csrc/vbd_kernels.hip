@@ -459,13 +459,13 @@ __device__ __forceinline__ void extSlotAny(int type, int local, const float4 *__
     }
 }
 
-// The entries of one further type in a slot's row (rows are sorted by type; `word` packs four 8-bit counts):
+// The entries of one further type in a slot's row (rows are sorted by type; `word` packs two 16-bit counts):
 // G lanes stride over them with the type's own code, and `base` moves past them.
 template <int TYPE, int G>
 __device__ __forceinline__ void extTypeRun(const SolveArgs &a, int32_t &base, uint32_t word, int32_t lig, V3f &g,
                                            S3f &H)
 {
-    const int32_t n = static_cast<int32_t>((word >> (8 * ((TYPE - 6) & 3))) & 255u);
+    const int32_t n = static_cast<int32_t>((word >> (16 * ((TYPE - 6) & 1))) & 0xffffu);
     if (a.extTypes & (1u << TYPE)) {
         for (int32_t k = lig; k < n; k += G) {
             const int2 in = a.extInc[base + k];
@@ -604,15 +604,15 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
         }
         if constexpr (EXT) {
             int32_t base = a.extPtr[vid];
-            const uint2 cnt = a.extCnt[vid];
+            const uint4 cnt = a.extCnt[vid];
             extTypeRun<6, G>(a, base, cnt.x, lig, g, H);
             extTypeRun<7, G>(a, base, cnt.x, lig, g, H);
-            extTypeRun<8, G>(a, base, cnt.x, lig, g, H);
-            extTypeRun<9, G>(a, base, cnt.x, lig, g, H);
-            extTypeRun<10, G>(a, base, cnt.y, lig, g, H);
-            extTypeRun<11, G>(a, base, cnt.y, lig, g, H);
-            extTypeRun<12, G>(a, base, cnt.y, lig, g, H);
-            extTypeRun<13, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<8, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<9, G>(a, base, cnt.y, lig, g, H);
+            extTypeRun<10, G>(a, base, cnt.z, lig, g, H);
+            extTypeRun<11, G>(a, base, cnt.z, lig, g, H);
+            extTypeRun<12, G>(a, base, cnt.w, lig, g, H);
+            extTypeRun<13, G>(a, base, cnt.w, lig, g, H);
         }
     }
     if (G > 1) {

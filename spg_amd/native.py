@@ -305,6 +305,7 @@ def host_lib():
         _proto(L, "spg_solver_set_resident_state", ctypes.c_int, [c, i32])
         _proto(L, "spg_solver_step", ctypes.c_int, [c])
         _proto(L, "spg_solver_reset", ctypes.c_int, [c])
+        _proto(L, "spg_solver_require_precomputation_update", ctypes.c_int, [c])
         _proto(L, "spg_solver_sync_to_host", ctypes.c_int, [c])
         _proto(L, "spg_solver_get_state", ctypes.c_int, [c, i32, _dp, _dp])
         _proto(L, "spg_solver_context", c, [c])
@@ -533,6 +534,9 @@ class VBDGpu:
 
     def reset(self):
         _host_check(self._L.spg_solver_reset(self.h))
+
+    def require_precomputation_update(self):
+        _host_check(self._L.spg_solver_require_precomputation_update(self.h))
 
     def sync_to_host(self):
         _host_check(self._L.spg_solver_sync_to_host(self.h))

@@ -153,3 +153,32 @@ def test_further_slot_blocks_on_gpu_match_oracle():
         bar = 1e-4 * np.abs(gr).max() + 1e-6 * np.abs(hr).max() * size
         print(f"{kind}: fp32 grad error {dg:.2e} (bar {bar:.2e}), rel blocks {eh:.2e}", flush=True)
         assert dg <= bar and eh < 1e-4, kind
+
+
+def test_further_energy_fan_over_255_incidences(oracle_threads):
+    """A particle with 300 incidences of one further type (the centre of a closed fan of Baraff-Witkin membrane
+    triangles): its row's per-type count does not fit 8 bits, and the step still equals the oracle's."""
+    n = 300
+    ang = 2.0 * np.pi * np.arange(n) / n
+    rim = np.column_stack([0.5 * np.cos(ang), 0.5 * np.sin(ang), np.zeros(n)])
+    x0 = np.vstack([[0.0, 0.0, 0.0], rim])
+    faces = np.array([[0, 1 + k, 1 + (k + 1) % n] for k in range(n)], dtype=np.int32)
+    x = x0 * np.array([1.05, 1.05, 1.0])
+    x[0, 2] = 0.01
+    yt = 10.0
+    rim_ids = np.arange(1, n + 1, dtype=np.int32)
+    s = scenes.Scene(x=x, x0=x0, m=np.full(n + 1, 1e-3),
+                     energies=[("membrane_bw", faces, np.tile([yt, yt, yt / 2.6], (n, 1))),
+                               ("anchor", rim_ids[:, None], np.column_stack([x[rim_ids], np.full(n, 1e3)]))],
+                     iterations=10)
+    ctx = gpu_context(s)
+    o = oracle_from_gpu_start(s, ctx)
+    assert np.array_equal(ctx.colors(), o.colors()[0])
+    x_start = o.state()[0]
+    ctx.step(s.dt, 1, s.iterations, s.gravity)
+    o.step(s.dt, 1, s.iterations, s.gravity)
+    xg, xo = ctx.state()[0], o.state()[0]
+    e, ed = rel_err(xg, xo), disp_rel_err(xg, xo, x_start)
+    print(f"fan of {n}: rel {e:.2e} disp-rel {ed:.2e}", flush=True)
+    assert np.all(np.isfinite(xg))
+    assert e < 1e-5

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -55,6 +56,35 @@ double recordBytes(int type)
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
+// Work memory of spg_vbd_residual_noalloc: the per-particle squared residuals, the per-segment partial sums and
+// the per-object norms of the residual's two-stage reduction. One copy per GPU, held until the process ends and
+// shared by every call on that GPU. In host code `scratch` is only a stand-in for the symbol; its device address
+// on each GPU comes from scratchOnDevice.
+constexpr int64_t kScratchParticles = int64_t(1) << 21;  // 2,097,152 particles
+constexpr int64_t kScratchSegments = 4096;               // residual segments (at most 4096 particles of one object)
+constexpr int64_t kScratchObjects = 4096;
+constexpr int64_t kScratchDoubles = kScratchParticles + kScratchSegments + kScratchObjects;  // 16.06 MiB
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kScratchMaxDevices = 64;
+std::mutex g_scratchMutex;
+double *g_scratchPtr[kScratchMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be current: resolved once per device and kept.
+double *scratchOnDevice(int device)
+{
+    if (device < 0 || device >= kScratchMaxDevices)
+        throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "device index beyond the allocation-free residual's device table");
+    std::lock_guard<std::mutex> lock(g_scratchMutex);
+    if (!g_scratchPtr[device]) {
+        void *p = nullptr;
+        hipCheck(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw VbdError(SPG_VBD_ERR_HIP, "hipGetSymbolAddress(scratch) returned a null address");
+        g_scratchPtr[device] = static_cast<double *>(p);
+    }
+    return g_scratchPtr[device];
+}
+
 float intBitsAsFloat(int32_t i)
 {
     float f;
@@ -68,6 +98,7 @@ struct spg_vbd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
+    double *scratchDev = nullptr;  // this device's `scratch` (spg_vbd_residual_noalloc)
 
     // objects and vertex numbering
     int32_t nObjects = 0;
@@ -383,6 +414,32 @@ struct spg_vbd_ctx {
         hipCheck(launchResidual(a, s), "residual launch");
         hipCheck(launchReduceObjects(r2Buf, segBegin.p, nSegments, objSeg.p, nObjects, partialBuf, normsDev, s),
                  "reduce launch");
+    }
+
+    // Per-object FP64 |r| at the current positions with the device's `scratch` as the work memory of the residual
+    // and its reduction (every entry a call reads is written first by that call: no clearing). The same kernels
+    // in the same order as residual(), so the norms are bitwise those of spg_vbd_residual; nothing is cached.
+    void residualNoAlloc(double *out)
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
+        if (nv > kScratchParticles)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 2097152 particles (" +
+                                                        std::to_string(nv) + ")");
+        if (nObjects > kScratchObjects)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 4096 objects (" +
+                                                        std::to_string(nObjects) + ")");
+        if (nSegments > kScratchSegments)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 4096 reduction segments (" +
+                                                        std::to_string(nSegments) + ")");
+        if (!scratchDev) throw VbdError(SPG_VBD_ERR_STATE, "allocation-free residual: scratch address not resolved");
+        if (nSegments == 0) {  // every object is empty
+            std::fill(out, out + nObjects, 0.0);
+            return;
+        }
+        double *r2s = scratchDev, *partial = scratchDev + kScratchParticles, *nrm = partial + kScratchSegments;
+        enqueueResidualOf(posIsInertial ? xt.p : pos.p, r2s, partial, nrm, stream);
+        hipCheck(hipMemcpyAsync(out, nrm, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
+        hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
 
     // Per-object FP64 |r| at x~ or at the current positions. Both are cached until they can change:
@@ -1349,6 +1406,7 @@ int spg_vbd_create(spg_vbd_ctx **out, const spg_vbd_object *objects, int32_t n_o
         if (device < 0 || device >= ndev) throw VbdError(SPG_VBD_ERR_INVALID, "invalid HIP device");
         c->device = device;
         hipCheck(hipSetDevice(device), "hipSetDevice");
+        c->scratchDev = scratchOnDevice(device);
         hipCheck(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
         buildContext(*c, objects, n_objects);
     });
@@ -1530,6 +1588,15 @@ int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms)
         if (!ctx || !norms) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         ctx->residual(false, norms);
+    });
+}
+
+int spg_vbd_residual_noalloc(spg_vbd_ctx *ctx, double *norms)
+{
+    return guarded(ctx, [&] {
+        if (!ctx || !norms) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->residualNoAlloc(norms);
     });
 }
 

@@ -171,6 +171,7 @@ def vbd_lib():
         _proto(L, "spg_vbd_save_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_restore_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_residual", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_vbd_residual_noalloc", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_residual_at_inertial", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_iterate_to_tolerance", ctypes.c_int, [c, dbl, i32, i32, _ip, _dp])
         _proto(L, "spg_vbd_get_launch", ctypes.c_int, [c, ctypes.POINTER(VbdLaunch)])
@@ -701,6 +702,12 @@ class VbdContext:
     def residual(self):
         out = np.zeros(self.n_objects)
         self._check(self._L.spg_vbd_residual(self.h, _d(out)))
+        return out
+
+    def residual_noalloc(self):
+        """spg_vbd_residual_noalloc: the same norms with the library's static work memory (no allocation)."""
+        out = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_residual_noalloc(self.h, _d(out)))
         return out
 
     def residual_at_inertial(self):

@@ -111,7 +111,11 @@ typedef struct {
     int32_t use_graph;           /* 1: capture a whole step in a hipGraph and replay it */
     int32_t lds_staging;         /* 1: each workgroup first copies the element-record rows of its vertices
                                     into LDS (coalesced), then reads them from there; available while
-                                    they fit 64 KB */
+                                    they fit 64 KB.
+                                    2: each vertex first gathers its own and its distinct neighbours'
+                                    positions into LDS once, and its element records index them there;
+                                    available when no particle has more than 15 distinct neighbours and
+                                    the objects have no further energies (types 6 ... 13) */
 } spg_vbd_launch;
 
 /* Counters and HIP-event timings kept by a context since creation (or the last reset_stats). */

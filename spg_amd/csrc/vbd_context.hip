@@ -160,6 +160,10 @@ struct spg_vbd_ctx {
     DevBuf<float2> hinEP, hinWP;
     DevBuf<int4> tetO;
     DevBuf<float4> tetGa, tetGb, tetGc;
+    // neighbourhood layout of the LDS-staged solve (lds_staging 2; nbW 0: not built)
+    int32_t nbW = 0;
+    DevBuf<int32_t> nbRow, tetL;
+    DevBuf<float4> triL, hinEL, hinWL, sprL;
     // further energies: CSR rows per slot, elements (slots, padded FP64 parameters)
     DevBuf<int32_t> extPtr;
     DevBuf<int2> extInc;
@@ -261,6 +265,13 @@ struct spg_vbd_ctx {
         a.tetGa = tetGa.p;
         a.tetGb = tetGb.p;
         a.tetGc = tetGc.p;
+        a.nbW = nbW;
+        a.nbRow = nbRow.p;
+        a.triL = triL.p;
+        a.hinEL = hinEL.p;
+        a.hinWL = hinWL.p;
+        a.tetL = tetL.p;
+        a.sprL = sprL.p;
         a.extPtr = extPtr.p;
         a.extInc = extInc.p;
         a.extV = extV.p;
@@ -842,6 +853,121 @@ std::vector<Rec> toRows(const std::vector<int32_t> &ptr, int32_t width, Rec pad,
 
 void buildFeatureTables(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects);
 
+// The neighbourhood layout of the LDS-staged solve (lds_staging 2), from the fixed-width rows of the direct
+// solve: per vertex slot v a row [v, then each distinct vertex its records name, in order of first use over
+// the classes in solve order], and per record the 5-bit indices of its vertices in that row. Not built
+// (c.nbW = 0) when some vertex names more than kNbLayoutMax - 1 other vertices.
+constexpr int32_t kNbLayoutMax = 16;
+
+void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows, const std::vector<int2> &triO,
+                              const std::vector<float4> &triM, const std::vector<int4> (&hinV)[2],
+                              const std::vector<float2> (&hinP)[2], const std::vector<int4> &tetO)
+{
+    const int64_t nv = c.nv;
+    std::vector<int32_t> ent;  // row of the vertex being built
+    auto localOf = [&](int32_t s) -> int32_t {
+        for (size_t k = 0; k < ent.size(); ++k)
+            if (ent[k] == s) return static_cast<int32_t>(k);
+        ent.push_back(s);
+        return static_cast<int32_t>(ent.size() - 1);
+    };
+    // pass 1: row widths
+    int32_t width = 1;
+    std::vector<std::vector<int32_t>> rows(nv);
+    for (int64_t v = 0; v < nv; ++v) {
+        ent.assign(1, static_cast<int32_t>(v));
+        for (int32_t j = 0; j < c.triW; ++j) {
+            const int2 o = triO[v * c.triW + j];
+            if (o.x >= 0) {
+                localOf(o.x);
+                localOf(o.y);
+            }
+        }
+        for (int cls = 0; cls < 2; ++cls) {
+            const int32_t w = cls == 0 ? c.hinEW : c.hinWW;
+            for (int32_t j = 0; j < w; ++j) {
+                const int4 h = hinV[cls][v * w + j];
+                if (h.x >= 0) {
+                    localOf(h.x);
+                    localOf(h.y);
+                    localOf(h.z);
+                    localOf(h.w);
+                }
+            }
+        }
+        for (int32_t j = 0; j < c.tetW; ++j) {
+            const int4 o = tetO[v * c.tetW + j];
+            if (o.x >= 0) {
+                localOf(o.x);
+                localOf(o.y);
+                localOf(o.z);
+            }
+        }
+        for (int32_t j = 0; j < c.sprW; ++j) {
+            int32_t other;
+            std::memcpy(&other, &sprRows[v * c.sprW + j].x, sizeof(other));
+            if (other >= 0) localOf(other);
+        }
+        if (static_cast<int32_t>(ent.size()) > kNbLayoutMax) {
+            c.nbW = 0;
+            return;
+        }
+        width = std::max(width, static_cast<int32_t>(ent.size()));
+        rows[v] = ent;
+    }
+    // pass 2: the row table and the records with local indices
+    std::vector<int32_t> nb(static_cast<size_t>(nv) * width, -1);
+    for (int64_t v = 0; v < nv; ++v) std::copy(rows[v].begin(), rows[v].end(), nb.begin() + v * width);
+    auto loc = [&](int64_t v, int32_t s) {
+        const auto &r = rows[v];
+        for (size_t k = 0; k < r.size(); ++k)
+            if (r[k] == s) return static_cast<int32_t>(k);
+        throw VbdError(SPG_VBD_ERR_INVALID, "neighbourhood layout: slot missing from its row");
+    };
+    std::vector<float4> triL(triM.size()), hinL[2], sprL(sprRows.size());
+    std::vector<int32_t> tetL(tetO.size(), -1);
+    for (int64_t v = 0; v < nv; ++v) {
+        for (int32_t j = 0; j < c.triW; ++j) {
+            const size_t r = v * c.triW + j;
+            const int2 o = triO[r];
+            const int32_t pk = o.x >= 0 ? loc(v, o.x) | loc(v, o.y) << 5 : -1;
+            triL[r] = make_float4(triM[r].x, triM[r].y, triM[r].z, intBitsAsFloat(pk));
+        }
+        for (int32_t j = 0; j < c.tetW; ++j) {
+            const size_t r = v * c.tetW + j;
+            const int4 o = tetO[r];
+            tetL[r] = o.x >= 0 ? loc(v, o.x) | loc(v, o.y) << 5 | loc(v, o.z) << 10 : -1;
+        }
+        for (int32_t j = 0; j < c.sprW; ++j) {
+            const size_t r = v * c.sprW + j;
+            int32_t other;
+            std::memcpy(&other, &sprRows[r].x, sizeof(other));
+            sprL[r] = make_float4(intBitsAsFloat(other >= 0 ? loc(v, other) : -1), sprRows[r].y, sprRows[r].z, 0.f);
+        }
+    }
+    for (int cls = 0; cls < 2; ++cls) {
+        const int32_t w = cls == 0 ? c.hinEW : c.hinWW;
+        hinL[cls].resize(hinV[cls].size());
+        for (int64_t v = 0; v < nv; ++v)
+            for (int32_t j = 0; j < w; ++j) {
+                const size_t r = v * w + j;
+                const int4 h = hinV[cls][r];
+                const int32_t pk =
+                    h.x >= 0 ? loc(v, h.x) | loc(v, h.y) << 5 | loc(v, h.z) << 10 | loc(v, h.w) << 15 : -1;
+                hinL[cls][r] = make_float4(hinP[cls][r].x, hinP[cls][r].y, intBitsAsFloat(pk), 0.f);
+            }
+    }
+    c.nbW = width;
+    c.nbRow.upload(nb, c.stream);
+    if (!triL.empty()) c.triL.upload(triL, c.stream);
+    if (!hinL[0].empty()) c.hinEL.upload(hinL[0], c.stream);
+    if (!hinL[1].empty()) c.hinWL.upload(hinL[1], c.stream);
+    if (!tetL.empty()) c.tetL.upload(tetL, c.stream);
+    if (!sprL.empty()) c.sprL.upload(sprL, c.stream);
+    // the host vectors are released when this returns; the uploads must have read them
+    hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
+}
+
 void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
 {
     if (nObjects < 1 || !objs) throw VbdError(SPG_VBD_ERR_INVALID, "at least one object required");
@@ -1042,6 +1168,11 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     auto checkSize = [](size_t n) {
         if (n >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many incidences");
     };
+    // the solve's fixed-width rows, kept on the host for the neighbourhood layout
+    std::vector<float4> sprRowsH, triMH;
+    std::vector<int2> triOH;
+    std::vector<int4> hinVH[2], tetOH;
+    std::vector<float2> hinPH[2];
     // springs
     if (!inc[SPG_VBD_SPRING].empty()) {
         checkSize(inc[SPG_VBD_SPRING].size());
@@ -1050,15 +1181,13 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         std::vector<int32_t> el(order.size());
         for (size_t k = 0; k < order.size(); ++k) el[k] = inc[SPG_VBD_SPRING][order[k]].elem;
         c.sprW = rowWidth(ptr);
-        c.spr.upload(toRows(ptr, c.sprW, make_float4(intBitsAsFloat(-1), 0.f, 0.f, 0.f),
-                            [&](int32_t k) {
-                                const auto &in = inc[SPG_VBD_SPRING][order[k]];
-                                const int2 v = sprV[in.elem];
-                                const int32_t other = in.local == 0 ? v.y : v.x;
-                                return make_float4(intBitsAsFloat(other), float(sprP[in.elem].x),
-                                                   float(sprP[in.elem].y), 0.f);
-                            }),
-                     c.stream);
+        sprRowsH = toRows(ptr, c.sprW, make_float4(intBitsAsFloat(-1), 0.f, 0.f, 0.f), [&](int32_t k) {
+            const auto &in = inc[SPG_VBD_SPRING][order[k]];
+            const int2 v = sprV[in.elem];
+            const int32_t other = in.local == 0 ? v.y : v.x;
+            return make_float4(intBitsAsFloat(other), float(sprP[in.elem].x), float(sprP[in.elem].y), 0.f);
+        });
+        c.spr.upload(sprRowsH, c.stream);
         c.sprPtr.upload(ptr, c.stream);
         c.sprElem.upload(el, c.stream);
         c.sprV.upload(sprV, c.stream);
@@ -1104,9 +1233,11 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         }
         c.triW = rowWidth(ptr);
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-        c.triO.upload(toRows(ptr, c.triW, make_int2(-1, -1), [&](int32_t k) { return ro[k]; }), c.stream);
+        triOH = toRows(ptr, c.triW, make_int2(-1, -1), [&](int32_t k) { return ro[k]; });
+        triMH = toRows(ptr, c.triW, z4, [&](int32_t k) { return rk[k]; });
+        c.triO.upload(triOH, c.stream);
         c.triG.upload(toRows(ptr, c.triW, z4, [&](int32_t k) { return rg[k]; }), c.stream);
-        c.triM.upload(toRows(ptr, c.triW, z4, [&](int32_t k) { return rk[k]; }), c.stream);
+        c.triM.upload(triMH, c.stream);
         c.triPtr.upload(ptr, c.stream);
         c.triElem.upload(el, c.stream);
         c.triV.upload(triV, c.stream);
@@ -1139,10 +1270,10 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             const int32_t w = rowWidth(ptr);
             (cls == 0 ? c.hinEW : c.hinWW) = w;
             (cls == 0 ? c.hinEPtr : c.hinWPtr).upload(ptr, c.stream);
-            (cls == 0 ? c.hinEV : c.hinWV)
-                .upload(toRows(ptr, w, make_int4(-1, -1, -1, -1), [&](int32_t k) { return rv[k]; }), c.stream);
-            (cls == 0 ? c.hinEP : c.hinWP)
-                .upload(toRows(ptr, w, make_float2(0.f, 0.f), [&](int32_t k) { return rp[k]; }), c.stream);
+            hinVH[cls] = toRows(ptr, w, make_int4(-1, -1, -1, -1), [&](int32_t k) { return rv[k]; });
+            hinPH[cls] = toRows(ptr, w, make_float2(0.f, 0.f), [&](int32_t k) { return rp[k]; });
+            (cls == 0 ? c.hinEV : c.hinWV).upload(hinVH[cls], c.stream);
+            (cls == 0 ? c.hinEP : c.hinWP).upload(hinPH[cls], c.stream);
             (cls == 0 ? c.hinEElem : c.hinWElem).upload(el, c.stream);
         }
         c.hinVe.upload(hinV, c.stream);
@@ -1181,7 +1312,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.tetW = rowWidth(ptr);
         const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
         c.tetPtr.upload(ptr, c.stream);
-        c.tetO.upload(toRows(ptr, c.tetW, make_int4(-1, -1, -1, -1), [&](int32_t k) { return ro[k]; }), c.stream);
+        tetOH = toRows(ptr, c.tetW, make_int4(-1, -1, -1, -1), [&](int32_t k) { return ro[k]; });
+        c.tetO.upload(tetOH, c.stream);
         c.tetGa.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return ga[k]; }), c.stream);
         c.tetGb.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gb[k]; }), c.stream);
         c.tetGc.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gc[k]; }), c.stream);
@@ -1221,6 +1353,9 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.extV.upload(extVh, c.stream);
         c.extP.upload(extPh, c.stream);
     }
+
+    if (extIncList.empty())
+        buildNeighbourhoodLayout(c, sprRowsH, triOH, triMH, hinVH, hinPH, tetOH);
 
     // compulsory bytes per colour sweep: 12 B per vertex of every object that has this colour (all
     // positions read once), 32 B per updated vertex (x write, x_tilde read, mass, row offset), and
@@ -1666,10 +1801,14 @@ int spg_vbd_set_launch(spg_vbd_ctx *ctx, const spg_vbd_launch *cfg)
             throw VbdError(SPG_VBD_ERR_INVALID, "lanes_per_vertex must be 1, 2, 3, 4, 6, 8 or 16");
         if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
             throw VbdError(SPG_VBD_ERR_INVALID, "block_size must be 64, 128 or 256");
-        if (cfg->lds_staging != 0 && cfg->lds_staging != 1)
-            throw VbdError(SPG_VBD_ERR_INVALID, "lds_staging must be 0 or 1");
-        if (cfg->lds_staging && solveStagedLdsBytes(ctx->solveArgs(0, 1.f), g, cfg->block_size) > kSolveMaxLds)
-            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "the element-record rows of one workgroup exceed 64 KB of LDS");
+        if (cfg->lds_staging < 0 || cfg->lds_staging > 2)
+            throw VbdError(SPG_VBD_ERR_INVALID, "lds_staging must be 0, 1 or 2");
+        if (cfg->lds_staging == 2 && ctx->nbW == 0)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED,
+                           "neighbourhood staging needs at most 15 distinct neighbours per particle and no further "
+                           "energies");
+        if (solveStagedLdsBytes(ctx->solveArgs(0, 1.f), g, cfg->block_size, cfg->lds_staging) > kSolveMaxLds)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "the staged data of one workgroup exceed 64 KB of LDS");
         ctx->launch = *cfg;
         ctx->dropGraph();
     });
@@ -1699,10 +1838,11 @@ int spg_vbd_autotune(spg_vbd_ctx *ctx, int32_t trials, spg_vbd_launch *best)
         spg_vbd_launch bestCfg = keep;
         const int lanes[] = {1, 2, 3, 4, 6, 8, 16};
         const int blocks[] = {64, 128, 256};
-        for (int staged = 0; staged < 2; ++staged)
+        for (int staged = 0; staged < 3; ++staged)
         for (int l : lanes)
             for (int b : blocks) {
-                if (staged && solveStagedLdsBytes(ctx->solveArgs(0, 1.f), l, b) > kSolveMaxLds) continue;
+                if (staged == 2 && ctx->nbW == 0) continue;
+                if (solveStagedLdsBytes(ctx->solveArgs(0, 1.f), l, b, staged) > kSolveMaxLds) continue;
                 ctx->launch = keep;
                 ctx->launch.lanes_per_vertex = l;
                 ctx->launch.block_size = b;

@@ -26,8 +26,10 @@ namespace spgvbd {
 // that XCD's L2.
 __device__ __forceinline__ int32_t xcdContiguousBlock(int32_t b, int32_t nb)
 {
+    // XCDs below r = nb mod 8 hold q + 1 blocks, the others q: XCD x starts at x q + min(x, r) (branch-free,
+    // so the kernel's argument loads are not held behind a branch)
     const int32_t xcd = b & 7, i = b >> 3, q = nb >> 3, r = nb & 7;
-    return xcd < r ? xcd * (q + 1) + i : r * (q + 1) + (xcd - r) * q + i;
+    return xcd * q + min(xcd, r) + i;
 }
 
 enum EnergyType { kSpring = 1, kStvk = 2, kBending = 3, kSnh = 4, kAnchor = 5 };
@@ -75,6 +77,21 @@ struct SolveArgs {
     unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
     int32_t begin, end;            // slot range of the colour
     float invdt2;
+    // neighbourhood layout (lds_staging 2): per vertex slot a row of nbW slots (entry 0 the vertex itself, then
+    // its distinct neighbours in order of first use, -1 padding), and records that name their vertices by 5-bit
+    // index into that row: StVK (kd, ko, ks, a | b << 5), hinges (theta0, keff, i0 | i1 << 5 | i2 << 10 |
+    // i3 << 15, 0), SNH a | b << 5 | c << 10 (with tetGa..c), spring (index, L0, k, 0); padding records -1.
+    // nbW is 0 when the layout does not exist (a vertex with more than 31 neighbours, or further energies).
+    int32_t nbW;
+    const int32_t *nbRow;
+    const float4 *triL;
+    const float4 *hinEL;
+    const float4 *hinWL;
+    const int32_t *tetL;
+    const float4 *sprL;
+    // launch shape, set by the launcher: the kernel reads it with its other arguments instead of from the
+    // dispatch packet, so its prologue waits for one round of scalar loads
+    int32_t gridBlocks, wavesPerBlock;
 };
 constexpr int kExtParams = 12;  // parameters stored per further-energy element (the largest needs 11)
 
@@ -121,7 +138,8 @@ hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, 
                             hipStream_t stream);
 // LDS staging of the element-record rows is available while one workgroup's rows fit this many bytes
 constexpr size_t kSolveMaxLds = 65536;
-size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize);
+// LDS bytes per workgroup of the staged variants (lds_staging 1: element rows, 2: neighbourhood slabs; 0: none)
+size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging);
 // start/stop: optional events stamped with the dispatch's own begin/end (nullptr: untimed launch)
 hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
                             hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);

@@ -89,7 +89,9 @@ __global__ void __launch_bounds__(256) copyXtKernel(float4 *__restrict__ pos, co
 // 3x3 block solve of l_vertexDescent (vbd.cpp:139-146): if det(H) > 1e-10 then x += H^-1 f.
 // FP32 adjugate solve; when det is near the 1e-10 threshold or small relative to |H|^3 (condition
 // worse than ~1e4) the decision and the solve are redone in FP64 from the same accumulated H, f.
-__device__ __forceinline__ bool blockSolve(const S3f &H, const V3f &f, V3f &dx, unsigned long long *counters)
+// Returns kSolvedFp32, kSolvedFp64 (dx set) or kSkipped (det(H) <= 1e-10: x stays).
+constexpr int kSolvedFp32 = 0, kSolvedFp64 = 1, kSkipped = 2;
+__device__ __forceinline__ int blockSolve(const S3f &H, const V3f &f, V3f &dx)
 {
     const float c00 = H.yy * H.zz - H.yz * H.yz;
     const float c01 = H.xz * H.yz - H.xy * H.zz;
@@ -106,7 +108,7 @@ __device__ __forceinline__ bool blockSolve(const S3f &H, const V3f &f, V3f &dx, 
         dx.x = (c00 * f.x + c01 * f.y + c02 * f.z) * id;
         dx.y = (c01 * f.x + c11 * f.y + c12 * f.z) * id;
         dx.z = (c02 * f.x + c12 * f.y + c22 * f.z) * id;
-        return true;
+        return kSolvedFp32;
     }
     const double xx = H.xx, xy = H.xy, xz = H.xz, yy = H.yy, yz = H.yz, zz = H.zz;
     const double d00 = yy * zz - yz * yz, d01 = xz * yz - xy * zz, d02 = xy * yz - xz * yy;
@@ -117,11 +119,49 @@ __device__ __forceinline__ bool blockSolve(const S3f &H, const V3f &f, V3f &dx, 
         dx.x = static_cast<float>((d00 * fx + d01 * fy + d02 * fz) / det64);
         dx.y = static_cast<float>((d01 * fx + d11 * fy + d12 * fz) / det64);
         dx.z = static_cast<float>((d02 * fx + d12 * fy + d22 * fz) / det64);
-        atomicAdd(&counters[0], 1ull);
-        return true;
+        return kSolvedFp64;
     }
-    atomicAdd(&counters[1], 1ull);
-    return false;
+    return kSkipped;
+}
+
+template <int G>
+__device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t lig);
+
+// The end of a vertex's update, shared by the solve kernels: the group's partial forces and Hessians are
+// summed in a fixed shuffle order, the group's first lane adds the inertia term, solves the 3x3 block and
+// writes the position; the FP64-fallback and skip counts are summed per wavefront (one atomic each at most).
+template <int G>
+__device__ __forceinline__ void finishVertex(const SolveArgs &a, bool active, int32_t vid, int32_t lane64,
+                                             int32_t lig, const V3f &xo, const float4 &xt, V3f g, S3f H)
+{
+    if (G > 1) {
+        g.x = groupReduce<G>(g.x, lane64, lig);
+        g.y = groupReduce<G>(g.y, lane64, lig);
+        g.z = groupReduce<G>(g.z, lane64, lig);
+        H.xx = groupReduce<G>(H.xx, lane64, lig);
+        H.xy = groupReduce<G>(H.xy, lane64, lig);
+        H.xz = groupReduce<G>(H.xz, lane64, lig);
+        H.yy = groupReduce<G>(H.yy, lane64, lig);
+        H.yz = groupReduce<G>(H.yz, lane64, lig);
+        H.zz = groupReduce<G>(H.zz, lane64, lig);
+    }
+    int path = -1;
+    if (active && lig == 0) {
+        const float mdt2 = xt.w * a.invdt2;
+        // f = -(m/h^2)(x - x_tilde) - sum grad; H = (m/h^2) I + sum H_e (vbd.cpp:102-138)
+        const V3f f = {-mdt2 * (xo.x - xt.x) - g.x, -mdt2 * (xo.y - xt.y) - g.y, -mdt2 * (xo.z - xt.z) - g.z};
+        H.xx += mdt2;
+        H.yy += mdt2;
+        H.zz += mdt2;
+        V3f dx;
+        path = blockSolve(H, f, dx);
+        if (path != kSkipped) a.pos[vid] = make_float4(xo.x + dx.x, xo.y + dx.y, xo.z + dx.z, 0.f);
+    }
+    const unsigned long long fb = __ballot(path == kSolvedFp64), sk = __ballot(path == kSkipped);
+    if (lane64 == 0) {
+        if (fb) atomicAdd(&a.counters[0], static_cast<unsigned long long>(__popcll(fb)));
+        if (sk) atomicAdd(&a.counters[1], static_cast<unsigned long long>(__popcll(sk)));
+    }
 }
 
 // K2: one colour sweep. A group of G lanes of one wavefront cooperates on one vertex; each
@@ -147,10 +187,11 @@ __device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t li
 // Copies count records starting at src[first] into LDS at *cursor (16-byte aligned), with the whole
 // workgroup, and advances the cursor. T is 8 or 16 bytes.
 template <class T>
-__device__ __forceinline__ const T *stageRows(const T *src, int64_t first, int32_t count, unsigned char *&cursor)
+__device__ __forceinline__ const T *stageRows(const T *src, int64_t first, int32_t count, int32_t threads,
+                                              unsigned char *&cursor)
 {
     T *dst = reinterpret_cast<T *>(cursor);
-    for (int32_t i = threadIdx.x; i < count; i += blockDim.x) dst[i] = src[first + i];
+    for (int32_t i = threadIdx.x; i < count; i += threads) dst[i] = src[first + i];
     cursor += (static_cast<size_t>(count) * sizeof(T) + 15) & ~size_t(15);
     return dst;
 }
@@ -162,6 +203,16 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
     auto seg = [&](int32_t w, size_t bytes) { return (static_cast<size_t>(nvb * w) * bytes + 15) & ~size_t(15); };
     return seg(a.triW, 8) + 2 * seg(a.triW, 16) + seg(a.hinEW, 16) + seg(a.hinEW, 8) + seg(a.hinWW, 16) +
            seg(a.hinWW, 8) + 4 * seg(a.tetW, 16) + seg(a.sprW, 16);
+}
+
+// LDS bytes of the neighbourhood-staged variant: per vertex group of the workgroup (and one spare per
+// wavefront for its idle lanes when G does not divide 64) a slab of ceil(16 / G) G float4 (the row's at most
+// 16 entries, rounded up to whole lane strides).
+__host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int blockSize)
+{
+    const size_t slab = static_cast<size_t>((16 + G - 1) / G) * G;
+    const size_t slabs = static_cast<size_t>(64 / G + (64 % G ? 1 : 0));
+    return static_cast<size_t>(blockSize / 64) * slabs * slab * 16;
 }
 
 // LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
@@ -499,32 +550,36 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
     constexpr int GPW = 64 / G;  // groups per wavefront
     extern __shared__ float4 ldsRows[];
     const int32_t lane64 = threadIdx.x & 63;
-    const int32_t blk = xcdContiguousBlock(blockIdx.x, gridDim.x);
-    const int32_t wave = blk * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int32_t blk = xcdContiguousBlock(blockIdx.x, a.gridBlocks);
+    const int32_t wave = blk * a.wavesPerBlock + (threadIdx.x >> 6);
     const int32_t grp = lane64 / G;
     const int32_t lig = lane64 - grp * G;
     const int32_t vid = a.begin + wave * GPW + grp;
     const bool active = grp < GPW && vid < a.end;
     int64_t rowBase = 0;  // record row of vertex vid is vid - rowBase
     if constexpr (LDS) {
-        const int32_t v0 = a.begin + blk * (blockDim.x >> 6) * GPW;
-        const int32_t nvb = max(0, min(static_cast<int32_t>(blockDim.x >> 6) * GPW, a.end - v0));
+        const int32_t v0 = a.begin + blk * a.wavesPerBlock * GPW;
+        const int32_t nvb = max(0, min(a.wavesPerBlock * GPW, a.end - v0));
+        const int32_t nt = a.wavesPerBlock * 64;
         unsigned char *cur = reinterpret_cast<unsigned char *>(ldsRows);
-        a.triO = stageRows(a.triO, int64_t(v0) * a.triW, nvb * a.triW, cur);
-        a.triG = stageRows(a.triG, int64_t(v0) * a.triW, nvb * a.triW, cur);
-        a.triM = stageRows(a.triM, int64_t(v0) * a.triW, nvb * a.triW, cur);
-        a.hinEV = stageRows(a.hinEV, int64_t(v0) * a.hinEW, nvb * a.hinEW, cur);
-        a.hinEP = stageRows(a.hinEP, int64_t(v0) * a.hinEW, nvb * a.hinEW, cur);
-        a.hinWV = stageRows(a.hinWV, int64_t(v0) * a.hinWW, nvb * a.hinWW, cur);
-        a.hinWP = stageRows(a.hinWP, int64_t(v0) * a.hinWW, nvb * a.hinWW, cur);
-        a.tetO = stageRows(a.tetO, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
-        a.tetGa = stageRows(a.tetGa, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
-        a.tetGb = stageRows(a.tetGb, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
-        a.tetGc = stageRows(a.tetGc, int64_t(v0) * a.tetW, nvb * a.tetW, cur);
-        a.spr = stageRows(a.spr, int64_t(v0) * a.sprW, nvb * a.sprW, cur);
+        a.triO = stageRows(a.triO, int64_t(v0) * a.triW, nvb * a.triW, nt, cur);
+        a.triG = stageRows(a.triG, int64_t(v0) * a.triW, nvb * a.triW, nt, cur);
+        a.triM = stageRows(a.triM, int64_t(v0) * a.triW, nvb * a.triW, nt, cur);
+        a.hinEV = stageRows(a.hinEV, int64_t(v0) * a.hinEW, nvb * a.hinEW, nt, cur);
+        a.hinEP = stageRows(a.hinEP, int64_t(v0) * a.hinEW, nvb * a.hinEW, nt, cur);
+        a.hinWV = stageRows(a.hinWV, int64_t(v0) * a.hinWW, nvb * a.hinWW, nt, cur);
+        a.hinWP = stageRows(a.hinWP, int64_t(v0) * a.hinWW, nvb * a.hinWW, nt, cur);
+        a.tetO = stageRows(a.tetO, int64_t(v0) * a.tetW, nvb * a.tetW, nt, cur);
+        a.tetGa = stageRows(a.tetGa, int64_t(v0) * a.tetW, nvb * a.tetW, nt, cur);
+        a.tetGb = stageRows(a.tetGb, int64_t(v0) * a.tetW, nvb * a.tetW, nt, cur);
+        a.tetGc = stageRows(a.tetGc, int64_t(v0) * a.tetW, nvb * a.tetW, nt, cur);
+        a.spr = stageRows(a.spr, int64_t(v0) * a.sprW, nvb * a.sprW, nt, cur);
         rowBase = v0;
         __syncthreads();
     }
+    // the arguments of the first loads, taken here (an empty asm that reads them), so the compiler issues all
+    // their scalar loads in the prologue's one round instead of a second round inside the branch below
+    asm volatile("" ::"s"(a.xt), "s"(a.ancPtr), "s"(a.triO), "s"(a.triG), "s"(a.triM), "s"(a.triW));
     V3f g = {0.f, 0.f, 0.f};
     S3f H = zeroSym<float>();
     V3f xo = {0.f, 0.f, 0.f};
@@ -556,11 +611,13 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const int4 v = a.hinEV[r];
             const float2 p = a.hinEP[r];
             const bool ok = v.x >= 0;
+            // the vertex is edge end x0 or x1: its own position is xo, only the other end is gathered
+            const bool second = v.y == vid;
+            const V3f pe = xyz(a.pos[ok ? (second ? v.x : v.y) : vid]);
             V3f ge;
             S3f He;
-            bendingEdge<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]),
-                               xyz(a.pos[ok ? v.z : vid]), xyz(a.pos[ok ? v.w : vid]), v.y == vid, p.x, p.y, ge,
-                               He);
+            bendingEdge<float>(second ? pe : xo, second ? xo : pe, xyz(a.pos[ok ? v.z : vid]),
+                               xyz(a.pos[ok ? v.w : vid]), second, p.x, p.y, ge, He);
             accIf(ok, g, H, ge, He);
         }
         for (int32_t j = lig; j < a.hinWW; j += G) {
@@ -568,11 +625,13 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const int4 v = a.hinWV[r];
             const float2 p = a.hinWP[r];
             const bool ok = v.x >= 0;
+            // the vertex is wing x2 or x3: its own position is xo, only the other wing is gathered
+            const bool second = v.w == vid;
+            const V3f pw = xyz(a.pos[ok ? (second ? v.z : v.w) : vid]);
             V3f ge;
             S3f He;
-            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]),
-                               xyz(a.pos[ok ? v.z : vid]), xyz(a.pos[ok ? v.w : vid]), v.w == vid, p.x, p.y, ge,
-                               He);
+            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), second ? pw : xo,
+                               second ? xo : pw, second, p.x, p.y, ge, He);
             accIf(ok, g, H, ge, He);
         }
         for (int32_t j = lig; j < a.tetW; j += G) {
@@ -615,28 +674,184 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             extTypeRun<13, G>(a, base, cnt.w, lig, g, H);
         }
     }
-    if (G > 1) {
-        g.x = groupReduce<G>(g.x, lane64, lig);
-        g.y = groupReduce<G>(g.y, lane64, lig);
-        g.z = groupReduce<G>(g.z, lane64, lig);
-        H.xx = groupReduce<G>(H.xx, lane64, lig);
-        H.xy = groupReduce<G>(H.xy, lane64, lig);
-        H.xz = groupReduce<G>(H.xz, lane64, lig);
-        H.yy = groupReduce<G>(H.yy, lane64, lig);
-        H.yz = groupReduce<G>(H.yz, lane64, lig);
-        H.zz = groupReduce<G>(H.zz, lane64, lig);
+    finishVertex<G>(a, active, vid, lane64, lig, xo, xt4, g, H);
+}
+
+// K2, neighbourhood-staged variant (lds_staging 2). A vertex needs the positions of few distinct vertices (on
+// the cloth grid 12: its one-ring and the far wing of each hinge it is a wing of) but reads them once per
+// incident element (54 times). Here the vertex's group first gathers its own position and those of its
+// distinct neighbours (entry e of its neighbour row, lanes striding by G) into an LDS slab, and the element
+// records name their vertices by 5-bit index into the slab (0: the vertex itself), so each position is read
+// from memory once per vertex and the texture data path, busy most of the launch in solveColorKernel, carries
+// a quarter of the position loads. The first record of each class does not depend on positions and is loaded
+// before the gather, so a vertex waits for two rounds of memory loads. The element functions see the same
+// values in the same order as in solveColorKernel: the result is bitwise the same.
+constexpr int kNbMax = 16;  // neighbour-row entries per vertex at most
+// incidence classes compiled into a neighbourhood-staged kernel (the launcher picks the instantiation that
+// holds the context's classes, so a cloth kernel keeps no registers for tetrahedra)
+constexpr int kClsTri = 1, kClsHinge = 2, kClsTet = 4, kClsSpring = 8, kClsAll = 15;
+
+template <int G, int CLS>
+__global__ void __launch_bounds__(256) solveColorNbKernel(SolveArgs a)
+{
+    constexpr int GPW = 64 / G;
+    constexpr int R = (kNbMax + G - 1) / G;  // slab entries per lane at most
+    extern __shared__ float4 nbSlab[];
+    const int32_t lane64 = threadIdx.x & 63;
+    const int32_t blk = xcdContiguousBlock(blockIdx.x, a.gridBlocks);
+    const int32_t wib = threadIdx.x >> 6;
+    const int32_t wave = blk * a.wavesPerBlock + wib;
+    const int32_t grp = lane64 / G;
+    const int32_t lig = lane64 - grp * G;
+    const int32_t vid = a.begin + wave * GPW + grp;
+    const bool active = grp < GPW && vid < a.end;
+    const int32_t vsafe = active ? vid : a.begin;  // a slot inside the colour, for the loads of idle lanes
+    // R G entries per group: every lane writes all R of its entries (those past the row are never read), so
+    // the position loads are issued together instead of one per branch. When G does not divide 64 the idle
+    // lanes past the last group (grp == GPW) write into one spare slab per wavefront.
+    constexpr int SLABS = GPW + (64 % G ? 1 : 0);
+    float4 *slab = nbSlab + static_cast<int64_t>(wib * SLABS + grp) * (R * G);
+    asm volatile("" ::"s"(a.xt), "s"(a.nbRow), "s"(a.triG), "s"(a.triL), "s"(a.hinEL), "s"(a.hinWL), "s"(a.triW),
+                 "s"(a.hinEW), "s"(a.hinWW), "s"(a.nbW));
+    const int64_t row = vid;
+    // records that do not depend on positions: the first of each class, and the neighbour row
+    const bool t0 = (CLS & kClsTri) && active && lig < a.triW;
+    const bool e0 = (CLS & kClsHinge) && active && lig < a.hinEW;
+    const bool w0 = (CLS & kClsHinge) && active && lig < a.hinWW;
+    const bool q0 = (CLS & kClsTet) && active && lig < a.tetW;
+    const bool s0 = (CLS & kClsSpring) && active && lig < a.sprW;
+    float4 tg = {}, tk = {}, he = {}, hw = {}, ga = {}, gb = {}, gc = {}, sp = {};
+    int32_t tl = -1;
+    if constexpr ((CLS & kClsTri) != 0) {
+        if (t0) {
+            tg = a.triG[row * a.triW + lig];
+            tk = a.triL[row * a.triW + lig];
+        }
     }
-    if (active && lig == 0) {
-        const float4 xt = xt4;
-        const float mdt2 = xt.w * a.invdt2;
-        // f = -(m/h^2)(x - x_tilde) - sum grad; H = (m/h^2) I + sum H_e (vbd.cpp:102-138)
-        const V3f f = {-mdt2 * (xo.x - xt.x) - g.x, -mdt2 * (xo.y - xt.y) - g.y, -mdt2 * (xo.z - xt.z) - g.z};
-        H.xx += mdt2;
-        H.yy += mdt2;
-        H.zz += mdt2;
-        V3f dx;
-        if (blockSolve(H, f, dx, a.counters)) a.pos[vid] = make_float4(xo.x + dx.x, xo.y + dx.y, xo.z + dx.z, 0.f);
+    if constexpr ((CLS & kClsHinge) != 0) {
+        if (e0) he = a.hinEL[row * a.hinEW + lig];
+        if (w0) hw = a.hinWL[row * a.hinWW + lig];
     }
+    if constexpr ((CLS & kClsTet) != 0) {
+        if (q0) {
+            tl = a.tetL[row * a.tetW + lig];
+            ga = a.tetGa[row * a.tetW + lig];
+            gb = a.tetGb[row * a.tetW + lig];
+            gc = a.tetGc[row * a.tetW + lig];
+        }
+    }
+    if constexpr ((CLS & kClsSpring) != 0) {
+        if (s0) sp = a.sprL[row * a.sprW + lig];
+    }
+    int32_t ids[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int32_t e = lig + k * G;
+        ids[k] = (active && e < a.nbW) ? a.nbRow[row * a.nbW + e] : -1;
+    }
+    const float4 xt4 = a.xt[vsafe];
+    int32_t ab = 0, ae = 0;
+    if (active && a.ancPtr) {
+        ab = a.ancPtr[vid];
+        ae = a.ancPtr[vid + 1];
+    }
+    // the neighbourhood into LDS: every lane issues its loads (padding entries read the vertex's own
+    // position), then writes them
+    float px[R], py[R], pz[R];  // (an array of float4 here is not kept in registers)
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const float4 p = a.pos[ids[k] >= 0 ? ids[k] : vsafe];
+        px[k] = p.x;
+        py[k] = p.y;
+        pz[k] = p.z;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) slab[lig + k * G] = make_float4(px[k], py[k], pz[k], 0.f);
+    __syncthreads();
+    V3f g = {0.f, 0.f, 0.f};
+    S3f H = zeroSym<float>();
+    V3f xo = {0.f, 0.f, 0.f};
+    if (active) {
+        xo = xyz(slab[0]);
+        if constexpr ((CLS & kClsTri) != 0) {
+            auto tri = [&](const float4 &gg, const float4 &kk) {
+                const int32_t pk = __float_as_int(kk.w);
+                const bool ok = pk >= 0;
+                const int32_t q = ok ? pk : 0;
+                V3f ge;
+                S3f He;
+                stvkVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), gg.x, gg.y, gg.z, gg.w, kk.x,
+                                  kk.y, kk.z, ge, He);
+                accIf(ok, g, H, ge, He);
+            };
+            if (t0) tri(tg, tk);
+            for (int32_t j = lig + G; j < a.triW; j += G) tri(a.triG[row * a.triW + j], a.triL[row * a.triW + j]);
+        }
+        if constexpr ((CLS & kClsHinge) != 0) {
+            auto hinE = [&](const float4 &r) {
+                const int32_t pk = __float_as_int(r.z);
+                const bool ok = pk >= 0;
+                const int32_t q = ok ? pk : 0;
+                const int32_t l1 = (q >> 5) & 31;
+                V3f ge;
+                S3f He;
+                bendingEdge<float>(xyz(slab[q & 31]), xyz(slab[l1]), xyz(slab[(q >> 10) & 31]),
+                                   xyz(slab[(q >> 15) & 31]), l1 == 0, r.x, r.y, ge, He);
+                accIf(ok, g, H, ge, He);
+            };
+            if (e0) hinE(he);
+            for (int32_t j = lig + G; j < a.hinEW; j += G) hinE(a.hinEL[row * a.hinEW + j]);
+            auto hinW = [&](const float4 &r) {
+                const int32_t pk = __float_as_int(r.z);
+                const bool ok = pk >= 0;
+                const int32_t q = ok ? pk : 0;
+                const int32_t l3 = (q >> 15) & 31;
+                V3f ge;
+                S3f He;
+                bendingWing<float>(xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
+                                   xyz(slab[l3]), l3 == 0, r.x, r.y, ge, He);
+                accIf(ok, g, H, ge, He);
+            };
+            if (w0) hinW(hw);
+            for (int32_t j = lig + G; j < a.hinWW; j += G) hinW(a.hinWL[row * a.hinWW + j]);
+        }
+        if constexpr ((CLS & kClsTet) != 0) {
+            auto tet = [&](int32_t pk, const float4 &ra, const float4 &rb, const float4 &rc) {
+                const bool ok = pk >= 0;
+                const int32_t q = ok ? pk : 0;
+                V3f ge;
+                S3f He;
+                snhVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
+                                 xyz(ra), xyz(rb), xyz(rc), ra.w, rb.w, rc.w, ge, He);
+                accIf(ok, g, H, ge, He);
+            };
+            if (q0) tet(tl, ga, gb, gc);
+            for (int32_t j = lig + G; j < a.tetW; j += G) {
+                const int64_t r = row * a.tetW + j;
+                tet(a.tetL[r], a.tetGa[r], a.tetGb[r], a.tetGc[r]);
+            }
+        }
+        if constexpr ((CLS & kClsSpring) != 0) {
+            auto spring = [&](const float4 &r) {
+                const int32_t pk = __float_as_int(r.x);
+                const bool ok = pk >= 0;
+                V3f ge;
+                S3f He;
+                springVertex<float>(xo, xyz(slab[ok ? pk : 0]), r.y, r.z, ge, He);
+                accIf(ok, g, H, ge, He);
+            };
+            if (s0) spring(sp);
+            for (int32_t j = lig + G; j < a.sprW; j += G) spring(a.sprL[row * a.sprW + j]);
+        }
+        for (int32_t k = ab + lig; k < ae; k += G) {
+            const float4 r = a.anc[k];
+            V3f ge;
+            S3f He;
+            anchorVertex<float>(xo, xyz(r), r.w, ge, He);
+            acc(g, H, ge, He);
+        }
+    }
+    finishVertex<G>(a, active, vid, lane64, lig, xo, xt4, g, H);
 }
 
 using V3d = V3<double>;
@@ -887,18 +1102,65 @@ void solveColorLaunchG(const SolveArgs &a, int grid, int blockSize, bool staged,
         solveColorLaunch<G, false>(a, grid, blockSize, 0, stream, start, stop);
 }
 
-hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging,
+template <int G, int CLS>
+void solveColorNbLaunchC(const SolveArgs &a, int grid, int blockSize, hipStream_t stream, hipEvent_t start,
+                         hipEvent_t stop)
+{
+    const size_t lds = solveNbLdsBytes(a, G, blockSize);
+    if (start)
+        hipExtLaunchKernelGGL(solveColorNbKernel<G, CLS>, dim3(grid), dim3(blockSize), lds, stream, start, stop, 0,
+                              a);
+    else
+        solveColorNbKernel<G, CLS><<<grid, blockSize, lds, stream>>>(a);
+}
+
+// the instantiation for the classes present: surface (StVK + hinges), solid (SNH), or all
+template <int G>
+void solveColorNbLaunch(const SolveArgs &a, int grid, int blockSize, hipStream_t stream, hipEvent_t start,
+                        hipEvent_t stop)
+{
+    const int cls = (a.triW ? kClsTri : 0) | (a.hinEW || a.hinWW ? kClsHinge : 0) | (a.tetW ? kClsTet : 0) |
+                    (a.sprW ? kClsSpring : 0);
+    if ((cls & ~(kClsTri | kClsHinge)) == 0)
+        solveColorNbLaunchC<G, kClsTri | kClsHinge>(a, grid, blockSize, stream, start, stop);
+    else if (cls == kClsTet)
+        solveColorNbLaunchC<G, kClsTet>(a, grid, blockSize, stream, start, stop);
+    else
+        solveColorNbLaunchC<G, kClsAll>(a, grid, blockSize, stream, start, stop);
+}
+
+hipError_t launchSolveColor(const SolveArgs &args, int lanesPerVertex, int blockSize, int ldsStaging,
                             hipStream_t stream, hipEvent_t start, hipEvent_t stop)
 {
-    const int64_t nv = a.end - a.begin;
+    const int64_t nv = args.end - args.begin;
     if (nv <= 0) return hipSuccess;
     if (blockSize > 256 || blockSize < 64 || blockSize % 64) return hipErrorInvalidValue;
     if (lanesPerVertex < 1 || lanesPerVertex > 16) return hipErrorInvalidValue;
-    if (ldsStaging && solveLdsBytes(a, lanesPerVertex, blockSize) > kSolveMaxLds) return hipErrorInvalidValue;
+    if (ldsStaging < 0 || ldsStaging > 2) return hipErrorInvalidValue;
+    if (ldsStaging == 1 && solveLdsBytes(args, lanesPerVertex, blockSize) > kSolveMaxLds) return hipErrorInvalidValue;
+    if (ldsStaging == 2 && (args.nbW < 1 || args.nbW > kNbMax || args.extPtr ||
+                            solveNbLdsBytes(args, lanesPerVertex, blockSize) > kSolveMaxLds))
+        return hipErrorInvalidValue;
     // wavefronts of 64 / G vertex groups, blockSize / 64 wavefronts per block
     const int64_t waves = (nv + 64 / lanesPerVertex - 1) / (64 / lanesPerVertex);
     const int grid = blocksFor(waves, blockSize / 64);
     const bool st = ldsStaging != 0;
+    SolveArgs a = args;
+    a.gridBlocks = grid;
+    a.wavesPerBlock = blockSize / 64;
+    if (ldsStaging == 2) {
+        switch (lanesPerVertex) {
+        case 1: solveColorNbLaunch<1>(a, grid, blockSize, stream, start, stop); break;
+        case 2: solveColorNbLaunch<2>(a, grid, blockSize, stream, start, stop); break;
+        case 3: solveColorNbLaunch<3>(a, grid, blockSize, stream, start, stop); break;
+        case 4: solveColorNbLaunch<4>(a, grid, blockSize, stream, start, stop); break;
+        case 6: solveColorNbLaunch<6>(a, grid, blockSize, stream, start, stop); break;
+        case 8: solveColorNbLaunch<8>(a, grid, blockSize, stream, start, stop); break;
+        case 16: solveColorNbLaunch<16>(a, grid, blockSize, stream, start, stop); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (lanesPerVertex) {
     case 1: solveColorLaunchG<1>(a, grid, blockSize, st, stream, start, stop); break;
     case 2: solveColorLaunchG<2>(a, grid, blockSize, st, stream, start, stop); break;
@@ -912,9 +1174,10 @@ hipError_t launchSolveColor(const SolveArgs &a, int lanesPerVertex, int blockSiz
     return hipGetLastError();
 }
 
-size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize)
+size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize, int ldsStaging)
 {
-    return solveLdsBytes(a, lanesPerVertex, blockSize);
+    if (ldsStaging == 2) return solveNbLdsBytes(a, lanesPerVertex, blockSize);
+    return ldsStaging == 1 ? solveLdsBytes(a, lanesPerVertex, blockSize) : 0;
 }
 
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)

@@ -93,7 +93,10 @@ def main():
     from spg_amd import native, scenes
 
     s = scenes.cloth_config2(seed=args.seed, n=args.n)
-    ctx = native.VbdContext([s.vbd_object()], device=0)
+    obj = s.vbd_object()
+    t0 = time.perf_counter()
+    ctx = native.VbdContext([obj], device=0)
+    create_s = time.perf_counter() - t0  # colouring + incidence streams + upload (vbd.cpp:14-18 on the first step)
     tuned = None
     if args.lanes or args.block:
         kw = {}
@@ -144,6 +147,19 @@ def main():
     x, v = ctx.state()
     finite = bool(np.all(np.isfinite(x)))
     traffic, traffic_src = measured_traffic(ctx.launch(), args)
+    # the force residual of the headline step, FP64 on the GPU: one substep of `iters` sweeps from the
+    # workload's initial state, |r(x)| / |r(x~)| (the oracle's value for the same x, x~ is in the CPU leg)
+    ctx.restore_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    x_tilde, _ = ctx.state()
+    ctx.iterate(iters)
+    r_x, r_xt = float(ctx.residual()[0]), float(ctx.residual_at_inertial()[0])
+    x_iter, _ = ctx.state()
+    ctx.end_substep()
+    step_residual = {"gpu_fp64_rho": r_x / r_xt, "gpu_fp64_residual": r_x, "gpu_fp64_residual_at_inertial": r_xt,
+                     "iterations": iters, "state": "one substep from the workload's initial state"}
+    kernel = {0: "solveColorKernel", 1: "solveColorKernel (element rows staged in LDS)",
+              2: "solveColorNbKernel"}[ctx.launch()["lds_staging"]]
 
     batched = None if args.no_batched else batched_throughput(args)
     volumetric = None if args.no_volumetric else volumetric_throughput(args)
@@ -154,8 +170,14 @@ def main():
     learned_cmp = None if args.no_learned else learned_comparison(s, args, ctx.launch())
 
     cpu = None
+    setup = {"gpu_create_s": create_s,
+             "what": "spg_vbd_create: greedy colouring, incidence streams and neighbourhood rows built on the host, "
+                     "uploaded (the reference does this inside its first step, vbd.cpp:14-18); not in the timed "
+                     "region"}
     if not args.no_cpu:
         cpu = cpu_baseline(s)
+        setup["oracle_precompute_s"] = cpu.pop("precompute_s")
+        step_residual.update(oracle_residual(s, x_iter, x_tilde))
 
     out = {
         "metric": "VBD vertex-updates/s on 512² cloth; time to 1e-4 residual, learned vs plain",
@@ -178,11 +200,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "traffic_source": traffic_src,
-                     "kernel": "solveColorKernel", "mean_launch_us": mean_ms * 1e3,
+                     "traffic_note": "2.0 FETCH_SIZE + 1.0 WRITE_SIZE per launch (rocprofv3 PMC, scaled by the same "
+                                     "counters on a copy kernel): bytes between the L2 caches and the memory fabric. "
+                                     "The 512x512 working set (~170 MB of records and state) stays in the 256 MB "
+                                     "Infinity Cache between sweeps, so this is an upper bound on DRAM bytes",
+                     "kernel": kernel, "mean_launch_us": mean_ms * 1e3,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "launches_timed": st["solve_launches"]},
         "solver_stats": {"fp64_fallbacks": st["fp64_fallbacks"], "skipped_blocks": st["skipped_blocks"],
                          "state_finite": finite},
+        "step_residual": step_residual,
+        "setup": setup,
         "cpu_baseline": cpu,
         "learned": learned_cmp,
         "batched": batched,
@@ -557,6 +585,18 @@ def learned_comparison(scene, args, launch):
                          "target_variance": float(np.var(t, axis=0).sum())}}
 
 
+def oracle_residual(scene, x, x_tilde):
+    """The FP64 oracle's relative force residual |r(x)| / |r(x~)| for the GPU's state (the checker, in the CPU
+    leg of the bench)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as ol
+
+    o = ol.from_scene(scene)
+    r = o.residual(x, x_tilde, scene.dt)
+    r0 = o.residual(x_tilde, x_tilde, scene.dt)
+    return {"oracle_fp64_rho": r / r0, "oracle_fp64_residual": r, "oracle_fp64_residual_at_inertial": r0}
+
+
 def measured_traffic(launch, args):
     """HBM bytes per vertex-solve launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes of
     this workload (scaled by the same counters on a copy kernel of known size), for the launch shape
@@ -584,7 +624,10 @@ def cpu_baseline(scene):
 
     cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     o = ol.from_scene(scene)
-    o.colors()  # first-call precomputation outside the timed region (vbd.cpp:14-18)
+    ol.set_threads(cores)
+    t0 = time.perf_counter()
+    o.colors()  # first-call precomputation outside the timed region (vbd.cpp:14-18), timed on its own
+    precompute_s = time.perf_counter() - t0
     v0 = np.zeros_like(scene.x)
 
     def timed(threads, iterations, mode, budget_s):
@@ -603,7 +646,7 @@ def cpu_baseline(scene):
     cf1, n_cf1, t_cf1 = timed(1, 2, ol.CLOSED_FORM, 5.0)
     # AD mode (TinyAD-like cost: two full-element derivative passes per incidence)
     ad, n_ad, t_ad = timed(cores, 1, ol.AD, 8.0)
-    return {"value": cf, "unit": "vertex-updates/s", "cores": cores, "kind": "port",
+    return {"value": cf, "unit": "vertex-updates/s", "cores": cores, "kind": "port", "precompute_s": precompute_s,
             "sample": f"{n_cf} first steps ({scene.iterations} iterations each) of the same {scene.n}-vertex cloth, "
                       f"FP64 closed-form derivatives, {t_cf:.1f} s",
             "one_core_value": cf1,

@@ -172,9 +172,10 @@ int spg_vbd_restore_state(spg_vbd_ctx *ctx);
 int spg_vbd_residual(spg_vbd_ctx *ctx, double *norms);
 /* spg_vbd_residual without device allocation, for callers that cannot allocate (serving loops, harnesses that
    forbid hipMalloc). No counterpart in the reference. The residual's work memory (per-particle squared
-   residuals, per-segment partials, per-object norms) is a static device array of 16.06 MiB per GPU, held until
-   the process ends, instead of buffers of the context; the call allocates nothing. Limits: at most 2,097,152
-   particles, 4096 objects and 4096 reduction segments per context (SPG_VBD_ERR_UNSUPPORTED beyond). Same
+   residuals, per-hinge angle terms, per-segment partials, per-object norms) is a static device array of
+   24.06 MiB per GPU, held until the process ends, instead of buffers of the context; the call allocates
+   nothing. Limits: at most 1,048,576 particles, 2,097,152 bending hinges, 4096 objects and 4096 reduction
+   segments per context (SPG_VBD_ERR_UNSUPPORTED beyond). Same
    norms as spg_vbd_residual, bit for bit. Every call on one GPU shares that array: calls that overlap on one
    GPU (from several threads) race; calls in sequence do not, each call synchronises its stream. */
 int spg_vbd_residual_noalloc(spg_vbd_ctx *ctx, double *norms);

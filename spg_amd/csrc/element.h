@@ -254,6 +254,45 @@ SPG_HD void bendingWing(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot3, R th
     addSymOuter(H, -kC * sw * L * iw2 * iw2, w, cross(w, e));
 }
 
+// The hinge angle alone, as bendingEdge computes it: theta = atan2(-(a x b).e^, -a.b).
+template <class R>
+SPG_HD R bendingTheta(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> a = cross(e, x2 - x1), b = cross(e, x3 - x1);
+    const R iL = R(1) / sqrt(dot(e, e));
+    return atan2(-dot(cross(a, b), iL * e), -dot(a, b));
+}
+
+// The gradient alone of the hinge energy at an edge slot (bendingEdge's g) for a given kC = k (theta - theta0),
+// so that the angle, shared by the hinge's four slots, is evaluated once per hinge (the FP64 residual).
+template <class R>
+SPG_HD V3<R> bendingEdgeGrad(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot1, R kC)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> p = x2 - x1, q = x3 - x1;
+    const V3<R> a = cross(e, p), b = cross(e, q);
+    const R L = sqrt(dot(e, e));
+    const R iL = R(1) / L;
+    const V3<R> eh = iL * e;
+    const R pe = dot(p, eh), qe = dot(q, eh);
+    const R rae = slot1 ? pe + L : pe, rbe = slot1 ? qe + L : qe;
+    const R sg = slot1 ? R(1) : R(-1);
+    const R ia2 = R(1) / dot(a, a), ib2 = R(1) / dot(b, b);
+    return kC * ((sg * rae * ia2) * a + (-sg * rbe * ib2) * b);
+}
+
+// The same at a wing slot (bendingWing's g): kC s_w L w / |w|^2.
+template <class R>
+SPG_HD V3<R> bendingWingGrad(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot3, R kC)
+{
+    const V3<R> e = x1 - x0;
+    const V3<R> w = cross(e, (slot3 ? x3 : x2) - x1);
+    const R L = sqrt(dot(e, e));
+    const R sw = slot3 ? R(1) : R(-1);
+    return (kC * (sw * L / dot(w, w))) * w;
+}
+
 template <class R>
 SPG_HD void bendingVertexCompact(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, int slot, R theta0, R k, V3<R> &g,
                                  Sym3<R> &H)

@@ -56,14 +56,15 @@ double recordBytes(int type)
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
 
-// Work memory of spg_vbd_residual_noalloc: the per-particle squared residuals, the per-segment partial sums and
-// the per-object norms of the residual's two-stage reduction. One copy per GPU, held until the process ends and
-// shared by every call on that GPU. In host code `scratch` is only a stand-in for the symbol; its device address
-// on each GPU comes from scratchOnDevice.
-constexpr int64_t kScratchParticles = int64_t(1) << 21;  // 2,097,152 particles
+// Work memory of spg_vbd_residual_noalloc: the per-particle squared residuals, the per-hinge k (theta - theta0),
+// the per-segment partial sums and the per-object norms of the residual's two-stage reduction. One copy per GPU,
+// held until the process ends and shared by every call on that GPU. In host code `scratch` is only a stand-in
+// for the symbol; its device address on each GPU comes from scratchOnDevice.
+constexpr int64_t kScratchParticles = int64_t(1) << 20;  // 1,048,576 particles
+constexpr int64_t kScratchHinges = int64_t(1) << 21;     // 2,097,152 bending hinges
 constexpr int64_t kScratchSegments = 4096;               // residual segments (at most 4096 particles of one object)
 constexpr int64_t kScratchObjects = 4096;
-constexpr int64_t kScratchDoubles = kScratchParticles + kScratchSegments + kScratchObjects;  // 16.06 MiB
+constexpr int64_t kScratchDoubles = kScratchParticles + kScratchHinges + kScratchSegments + kScratchObjects;  // 24.06 MiB
 static __device__ __align__(256) double scratch[kScratchDoubles];
 
 constexpr int kScratchMaxDevices = 64;
@@ -120,6 +121,7 @@ struct spg_vbd_ctx {
     DevBuf<double> massd;  // FP64 masses for the residual
     DevBuf<unsigned long long> counters;
     DevBuf<double> r2, norms, normsAux;
+    DevBuf<double> kc, kcCheck;  // per hinge k (theta - theta0) of the residual (solver's stream, check stream)
     // pipelined convergence checks of iterate_to_tolerance: a second stream computes the residual of
     // position snapshots while the solver's stream keeps sweeping
     // Two units in flight: the next sweeps are queued while the host reads a check (hiding the host
@@ -173,6 +175,7 @@ struct spg_vbd_ctx {
     uint32_t extTypes = 0;
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
+    DevBuf<int32_t> triRowR, hinERowR, hinWRowR, tetRowR, sprRowR;  // element ids in the solve's row widths
     DevBuf<int2> sprV;
     DevBuf<double2> sprP;
     DevBuf<double4> ancP;
@@ -383,11 +386,12 @@ struct spg_vbd_ctx {
     // double per object), on the context's stream, without waiting.
     void enqueueResidual(bool atInertial, double *normsDev)
     {
-        enqueueResidualOf(atInertial ? xt.p : pos.p, r2.p, segPartial.p, normsDev, stream);
+        enqueueResidualOf(atInertial ? xt.p : pos.p, r2.p, kc.p, segPartial.p, normsDev, stream);
     }
 
-    // The same for arbitrary positions, per-vertex and per-segment work buffers and stream.
-    void enqueueResidualOf(const float4 *positions, double *r2Buf, double *partialBuf, double *normsDev, hipStream_t s)
+    // The same for arbitrary positions, per-vertex, per-hinge and per-segment work buffers and stream.
+    void enqueueResidualOf(const float4 *positions, double *r2Buf, double *kcBuf, double *partialBuf, double *normsDev,
+                           hipStream_t s)
     {
         ResidualArgs a{};
         a.pos = positions;
@@ -422,6 +426,18 @@ struct spg_vbd_ctx {
         a.extP = extP.p;
         a.order = objSlots.p;
         a.r2 = r2Buf;
+        a.triW = triW;
+        a.hinEW = hinEW;
+        a.hinWW = hinWW;
+        a.tetW = tetW;
+        a.sprW = sprW;
+        a.triRow = triRowR.p;
+        a.hinERow = hinERowR.p;
+        a.hinWRow = hinWRowR.p;
+        a.tetRow = tetRowR.p;
+        a.sprRow = sprRowR.p;
+        a.hinKc = kcBuf;
+        a.nHinges = static_cast<int32_t>(hinVe.n);
         hipCheck(launchResidual(a, s), "residual launch");
         hipCheck(launchReduceObjects(r2Buf, segBegin.p, nSegments, objSeg.p, nObjects, partialBuf, normsDev, s),
                  "reduce launch");
@@ -434,8 +450,11 @@ struct spg_vbd_ctx {
     {
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
         if (nv > kScratchParticles)
-            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 2097152 particles (" +
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 1048576 particles (" +
                                                         std::to_string(nv) + ")");
+        if (static_cast<int64_t>(hinVe.n) > kScratchHinges)
+            throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 2097152 hinges (" +
+                                                        std::to_string(hinVe.n) + ")");
         if (nObjects > kScratchObjects)
             throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "allocation-free residual: more than 4096 objects (" +
                                                         std::to_string(nObjects) + ")");
@@ -447,8 +466,9 @@ struct spg_vbd_ctx {
             std::fill(out, out + nObjects, 0.0);
             return;
         }
-        double *r2s = scratchDev, *partial = scratchDev + kScratchParticles, *nrm = partial + kScratchSegments;
-        enqueueResidualOf(posIsInertial ? xt.p : pos.p, r2s, partial, nrm, stream);
+        double *r2s = scratchDev, *kcs = scratchDev + kScratchParticles, *partial = kcs + kScratchHinges,
+               *nrm = partial + kScratchSegments;
+        enqueueResidualOf(posIsInertial ? xt.p : pos.p, r2s, kcs, partial, nrm, stream);
         hipCheck(hipMemcpyAsync(out, nrm, sizeof(double) * nObjects, hipMemcpyDeviceToHost, stream), "download");
         hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     }
@@ -643,6 +663,7 @@ struct spg_vbd_ctx {
         hipCheck(hipEventCreateWithFlags(&evInertialCheck, hipEventDisableTiming), "hipEventCreate");
         snap.alloc(static_cast<size_t>(kPipe) * nv);
         r2Check.alloc(nv);
+        kcCheck.alloc(hinVe.n);
         partialCheck.alloc(nSegments);
         normsCheck.alloc(static_cast<size_t>(kPipe) * nObjects);
         hipCheck(hipHostMalloc(reinterpret_cast<void **>(&hostNormsCheck), sizeof(double) * (kPipe + 1) * nObjects),
@@ -654,7 +675,7 @@ struct spg_vbd_ctx {
     {
         hipCheck(hipEventRecord(evSnap[0], stream), "hipEventRecord");  // x~ is written by begin_substep
         hipCheck(hipStreamWaitEvent(checkStream, evSnap[0], 0), "hipStreamWaitEvent");
-        enqueueResidualOf(xt.p, r2Check.p, partialCheck.p, normsAux.p, checkStream);
+        enqueueResidualOf(xt.p, r2Check.p, kcCheck.p, partialCheck.p, normsAux.p, checkStream);
         hipCheck(hipMemcpyAsync(hostNormsCheck + kPipe * nObjects, normsAux.p, sizeof(double) * nObjects,
                                 hipMemcpyDeviceToHost, checkStream),
                  "download");
@@ -706,7 +727,8 @@ struct spg_vbd_ctx {
                 hipCheck(launchCopy(pos.p, snap.p + k * nvs, nv, stream), "snapshot");
                 hipCheck(hipEventRecord(evSnap[k], stream), "hipEventRecord");
                 hipCheck(hipStreamWaitEvent(checkStream, evSnap[k], 0), "hipStreamWaitEvent");
-                enqueueResidualOf(snap.p + k * nvs, r2Check.p, partialCheck.p, normsCheck.p + k * nObjects, checkStream);
+                enqueueResidualOf(snap.p + k * nvs, r2Check.p, kcCheck.p, partialCheck.p, normsCheck.p + k * nObjects,
+                                  checkStream);
                 hipCheck(hipMemcpyAsync(hostNormsCheck + k * nObjects, normsCheck.p + k * nObjects,
                                         sizeof(double) * nObjects, hipMemcpyDeviceToHost, checkStream),
                          "download");
@@ -1190,6 +1212,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.spr.upload(sprRowsH, c.stream);
         c.sprPtr.upload(ptr, c.stream);
         c.sprElem.upload(el, c.stream);
+        c.sprRowR.upload(toRows(ptr, c.sprW, -1, [&](int32_t k) { return el[k]; }), c.stream);
         c.sprV.upload(sprV, c.stream);
         c.sprP.upload(sprP, c.stream);
     }
@@ -1240,6 +1263,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.triM.upload(triMH, c.stream);
         c.triPtr.upload(ptr, c.stream);
         c.triElem.upload(el, c.stream);
+        c.triRowR.upload(toRows(ptr, c.triW, -1, [&](int32_t k) { return el[k]; }), c.stream);
         c.triV.upload(triV, c.stream);
         c.triP.upload(triP, c.stream);
     }
@@ -1275,9 +1299,11 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             (cls == 0 ? c.hinEV : c.hinWV).upload(hinVH[cls], c.stream);
             (cls == 0 ? c.hinEP : c.hinWP).upload(hinPH[cls], c.stream);
             (cls == 0 ? c.hinEElem : c.hinWElem).upload(el, c.stream);
+            (cls == 0 ? c.hinERowR : c.hinWRowR).upload(toRows(ptr, w, -1, [&](int32_t k) { return el[k]; }), c.stream);
         }
         c.hinVe.upload(hinV, c.stream);
         c.hinPe.upload(hinP, c.stream);
+        c.kc.alloc(hinV.size());
     }
     // Stable Neo-Hookean: rows of Dinv are g_1..g_3; g_0 = -(g_1 + g_2 + g_3)
     if (!inc[SPG_VBD_STABLE_NEO_HOOKEAN].empty()) {
@@ -1318,6 +1344,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.tetGb.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gb[k]; }), c.stream);
         c.tetGc.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gc[k]; }), c.stream);
         c.tetElem.upload(el, c.stream);
+        c.tetRowR.upload(toRows(ptr, c.tetW, -1, [&](int32_t k) { return el[k]; }), c.stream);
         c.tetV.upload(tetV, c.stream);
         c.tetP.upload(tetP, c.stream);
     }

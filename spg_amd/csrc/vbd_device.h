@@ -129,6 +129,15 @@ struct ResidualArgs {
     const double *extP;
     const int32_t *order; // slot of the i-th particle (objects in order, particles ascending)
     double *r2;           // out: |r_v|^2 of the i-th particle (particle order)
+    // fixed-width rows of element ids per vertex slot and class, in the solve's widths (-1: padding)
+    int32_t triW, hinEW, hinWW, tetW, sprW;
+    const int32_t *triRow;
+    const int32_t *hinERow;
+    const int32_t *hinWRow;
+    const int32_t *tetRow;
+    const int32_t *sprRow;
+    double *hinKc;  // work: per hinge k (theta - theta0) at `pos` (hingeKcKernel, launched first)
+    int32_t nHinges;
 };
 
 // kernel launchers (vbd_kernels.hip); all asynchronous on `stream`

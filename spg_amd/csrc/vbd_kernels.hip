@@ -872,97 +872,89 @@ __device__ __forceinline__ double groupReduceD(double v, int32_t lane64, int32_t
 
 // K4: r_v = -(m/h^2)(x_v - x_tilde_v) - sum_e grad_v E_e(x), all in FP64 from the FP32 state and the
 // exact prepared parameters (implicitEulerBase.cpp:102-143 force, x-form of
-// implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per vertex slot.
-// A group of kResidualLanes lanes per vertex: lane l of the group takes entries l, l + G, ... of each
-// incidence class, and the group sums its partial forces in a fixed order.
-constexpr int kResidualLanes = 8;
-
-template <bool EXT>
+// implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per particle (particle order).
+// A group of G lanes per vertex, vertices in particle order (neighbours and the elements they share are
+// processed close together in time). Lane l takes entries l, l + G, ... of the vertex's fixed-width row of
+// element ids in each class, reads the element's slots and exact parameters from its table and the stencil's
+// positions, and the group sums its partial forces in a fixed shuffle order. The vertex's slot in an element
+// is found by comparing ids, and the other slots and their shape vectors are picked by selects, so nothing is
+// indexed at run time in private memory.
+template <int G, bool EXT>
 __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
 {
-    constexpr int G = kResidualLanes, GPW = 64 / G;
+    constexpr int GPW = 64 / G;
     const int32_t lane64 = threadIdx.x & 63;
     const int32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int32_t grp = lane64 / G, lig = lane64 - grp * G;
-    // particles are taken in particle order (neighbours, and the elements they share, are processed
-    // close together in time); their rows and positions are addressed by slot
     const int32_t gi = wave * GPW + grp;
-    const bool active = gi < a.n;
+    const bool active = grp < GPW && gi < a.n;
     const int32_t vid = active ? a.order[gi] : 0;
+    const int64_t row = vid;
     V3d r = {0.0, 0.0, 0.0};
     if (active) {
-        const float4 xf = a.pos[vid], tf = a.xt[vid];
+        const float4 xf = a.pos[vid];
         const V3d xo = xyzd(xf);
         if (lig == 0) {
+            const float4 tf = a.xt[vid];
             const double mdt2 = a.mass[vid] * a.invdt2;
             r = V3d{-mdt2 * (xo.x - double(tf.x)), -mdt2 * (xo.y - double(tf.y)), -mdt2 * (xo.z - double(tf.z))};
         }
         V3d ge;
         S3d He;
-        if (a.triPtr) {
-            for (int32_t k = a.triPtr[vid] + lig; k < a.triPtr[vid + 1]; k += G) {
-                const int32_t el = a.triElem[k];
-                const int4 v = a.triV[el];
-                const double *p = a.triP + 8 * static_cast<int64_t>(el);
-                // rows of Dinv: g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2)
-                const double g[3][2] = {{-(p[0] + p[2]), -(p[1] + p[3])}, {p[0], p[1]}, {p[2], p[3]}};
-                const int s = v.x == vid ? 0 : (v.y == vid ? 1 : 2);
-                const int ia = s == 0 ? 1 : 0, ib = s == 2 ? 1 : 2;
-                const int vv[3] = {v.x, v.y, v.z};
-                stvkVertex<double>(xo, xyzd(a.pos[vv[ia]]), xyzd(a.pos[vv[ib]]), g[ia][0], g[ia][1], g[ib][0],
-                                   g[ib][1], p[4], p[5], p[6], ge, He);
-                r = r - ge;
-            }
+        for (int32_t j = lig; j < a.triW; j += G) {
+            const int32_t el = a.triRow[row * a.triW + j];
+            if (el < 0) continue;
+            const int4 v = a.triV[el];
+            const double2 *p = reinterpret_cast<const double2 *>(a.triP + 8 * static_cast<int64_t>(el));
+            const double2 p01 = p[0], p23 = p[1], p45 = p[2], p67 = p[3];
+            // rows of Dinv: g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2); the other two slots in stencil order
+            const bool s0 = v.x == vid, s2 = v.z == vid;
+            const double g00 = -(p01.x + p23.x), g01 = -(p01.y + p23.y);
+            const int32_t ia = s0 ? v.y : v.x, ib = s2 ? v.y : v.z;
+            const double ga0 = s0 ? p01.x : g00, ga1 = s0 ? p01.y : g01;
+            const double gb0 = s2 ? p01.x : p23.x, gb1 = s2 ? p01.y : p23.y;
+            stvkVertex<double>(xo, xyzd(a.pos[ia]), xyzd(a.pos[ib]), ga0, ga1, gb0, gb1, p45.x, p45.y, p67.x, ge, He);
+            r = r - ge;
         }
-        if (a.hinEPtr) {  // hinge-edge incidences (slots 0, 1)
-            for (int32_t k = a.hinEPtr[vid] + lig; k < a.hinEPtr[vid + 1]; k += G) {
-                const int32_t el = a.hinEElem[k];
-                const int4 v = a.hinV[el];
-                const double2 p = a.hinP[el];
-                bendingEdge<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), v.y == vid,
-                                    p.x, p.y, ge, He);
-                r = r - ge;
-            }
+        // hinges: k (theta - theta0) once per hinge (hingeKcKernel), the slot's gradient of theta here
+        for (int32_t j = lig; j < a.hinEW; j += G) {  // hinge-edge incidences (slots 0, 1)
+            const int32_t el = a.hinERow[row * a.hinEW + j];
+            if (el < 0) continue;
+            const int4 v = a.hinV[el];
+            r = r - bendingEdgeGrad<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]),
+                                            v.y == vid, a.hinKc[el]);
         }
-        if (a.hinWPtr) {  // hinge-wing incidences (slots 2, 3)
-            for (int32_t k = a.hinWPtr[vid] + lig; k < a.hinWPtr[vid + 1]; k += G) {
-                const int32_t el = a.hinWElem[k];
-                const int4 v = a.hinV[el];
-                const double2 p = a.hinP[el];
-                bendingWing<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]), v.w == vid,
-                                    p.x, p.y, ge, He);
-                r = r - ge;
-            }
+        for (int32_t j = lig; j < a.hinWW; j += G) {  // hinge-wing incidences (slots 2, 3)
+            const int32_t el = a.hinWRow[row * a.hinWW + j];
+            if (el < 0) continue;
+            const int4 v = a.hinV[el];
+            r = r - bendingWingGrad<double>(xyzd(a.pos[v.x]), xyzd(a.pos[v.y]), xyzd(a.pos[v.z]), xyzd(a.pos[v.w]),
+                                            v.w == vid, a.hinKc[el]);
         }
-        if (a.tetPtr) {
-            for (int32_t k = a.tetPtr[vid] + lig; k < a.tetPtr[vid + 1]; k += G) {
-                const int32_t el = a.tetElem[k];
-                const int4 v = a.tetV[el];
-                const double *p = a.tetP + 12 * static_cast<int64_t>(el);
-                V3d gr[4];
-                gr[1] = {p[0], p[1], p[2]};
-                gr[2] = {p[3], p[4], p[5]};
-                gr[3] = {p[6], p[7], p[8]};
-                gr[0] = -(gr[1] + gr[2] + gr[3]);
-                const int vv[4] = {v.x, v.y, v.z, v.w};
-                const int s = v.x == vid ? 0 : (v.y == vid ? 1 : (v.z == vid ? 2 : 3));
-                int o[3], m = 0;
-                for (int q = 0; q < 4; ++q)
-                    if (q != s) o[m++] = q;
-                snhVertex<double>(xo, xyzd(a.pos[vv[o[0]]]), xyzd(a.pos[vv[o[1]]]), xyzd(a.pos[vv[o[2]]]), gr[o[0]],
-                                  gr[o[1]], gr[o[2]], p[9], p[10], p[11], ge, He);
-                r = r - ge;
-            }
+        for (int32_t j = lig; j < a.tetW; j += G) {
+            const int32_t el = a.tetRow[row * a.tetW + j];
+            if (el < 0) continue;
+            const int4 v = a.tetV[el];
+            const double2 *p = reinterpret_cast<const double2 *>(a.tetP + 12 * static_cast<int64_t>(el));
+            const double2 p01 = p[0], p23 = p[1], p45 = p[2], p67 = p[3], p89 = p[4], pab = p[5];
+            // rows of Dinv: g1 = p0..2, g2 = p3..5, g3 = p6..8, g0 = -(g1 + g2 + g3); the other three slots in
+            // stencil order: s = 0: (1, 2, 3), 1: (0, 2, 3), 2: (0, 1, 3), 3: (0, 1, 2)
+            const V3d g1 = {p01.x, p01.y, p23.x}, g2 = {p23.y, p45.x, p45.y}, g3 = {p67.x, p67.y, p89.x};
+            const V3d g0 = -(g1 + g2 + g3);
+            const bool s0 = v.x == vid, s01 = s0 || v.y == vid, s012 = s01 || v.z == vid;
+            const int32_t i0 = s0 ? v.y : v.x, i1 = s01 ? v.z : v.y, i2 = s012 ? v.w : v.z;
+            const V3d ga = s0 ? g1 : g0, gb = s01 ? g2 : g1, gc = s012 ? g3 : g2;
+            snhVertex<double>(xo, xyzd(a.pos[i0]), xyzd(a.pos[i1]), xyzd(a.pos[i2]), ga, gb, gc, p89.y, pab.x, pab.y,
+                              ge, He);
+            r = r - ge;
         }
-        if (a.sprPtr) {
-            for (int32_t k = a.sprPtr[vid] + lig; k < a.sprPtr[vid + 1]; k += G) {
-                const int32_t el = a.sprElem[k];
-                const int2 v = a.sprV[el];
-                const double2 p = a.sprP[el];
-                const int32_t other = v.x == vid ? v.y : v.x;
-                springVertex<double>(xo, xyzd(a.pos[other]), p.x, p.y, ge, He);
-                r = r - ge;
-            }
+        for (int32_t j = lig; j < a.sprW; j += G) {
+            const int32_t el = a.sprRow[row * a.sprW + j];
+            if (el < 0) continue;
+            const int2 v = a.sprV[el];
+            const double2 p = a.sprP[el];
+            springVertex<double>(xo, xyzd(a.pos[v.x == vid ? v.y : v.x]), p.x, p.y, ge, He);
+            r = r - ge;
         }
         if (a.ancPtr) {
             for (int32_t k = a.ancPtr[vid] + lig; k < a.ancPtr[vid + 1]; k += G) {
@@ -1180,13 +1172,46 @@ size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize
     return ldsStaging == 1 ? solveLdsBytes(a, lanesPerVertex, blockSize) : 0;
 }
 
+// The residual's first pass over the hinges: kC = k (theta - theta0) per hinge in FP64 (discreteBendingEnergy.cpp
+// :20-27 angle), so that the four slots of a hinge share one angle evaluation.
+__global__ void __launch_bounds__(256) hingeKcKernel(const float4 *__restrict__ pos, const int4 *__restrict__ hinV,
+                                                    const double2 *__restrict__ hinP, int32_t n,
+                                                    double *__restrict__ kc)
+{
+    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int4 v = hinV[i];
+    const double2 p = hinP[i];
+    const double th = bendingTheta<double>(xyzd(pos[v.x]), xyzd(pos[v.y]), xyzd(pos[v.z]), xyzd(pos[v.w]));
+    kc[i] = p.y * (th - p.x);
+}
+
+template <int G>
+void residualLaunchG(const ResidualArgs &a, hipStream_t stream)
+{
+    const int blocks = blocksFor(a.n, 256 / 64 * (64 / G));
+    if (a.extPtr)
+        residualKernel<G, true><<<blocks, 256, 0, stream>>>(a);
+    else
+        residualKernel<G, false><<<blocks, 256, 0, stream>>>(a);
+}
+
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)
 {
     if (a.n <= 0) return hipSuccess;
-    if (a.extPtr)
-        residualKernel<true><<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
+    if (a.nHinges > 0) {
+        if (!a.hinKc) return hipErrorInvalidValue;
+        hingeKcKernel<<<blocksFor(a.nHinges, 256), 256, 0, stream>>>(a.pos, a.hinV, a.hinP, a.nHinges, a.hinKc);
+    }
+    // lanes per vertex: the widest row's entries in one pass where they fit 6 lanes (the cloth's 6 / 6 / 6),
+    // otherwise 8 lanes striding over longer rows (the cube's 24 tetrahedra)
+    const int32_t w = max(max(max(a.triW, a.hinEW), max(a.hinWW, a.tetW)), a.sprW);
+    if (w <= 4)
+        residualLaunchG<4>(a, stream);
+    else if (w <= 6)
+        residualLaunchG<6>(a, stream);
     else
-        residualKernel<false><<<blocksFor(a.n, 256 / kResidualLanes), 256, 0, stream>>>(a);
+        residualLaunchG<8>(a, stream);
     return hipGetLastError();
 }
 

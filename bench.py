@@ -552,10 +552,12 @@ def xpbd_cpu_baseline(scene, budget_s=8.0):
 def learned_comparison(scene, args, launch):
     """Learned warm start against plain VBD on the bench's own cloth. The network is trained here, on
     the GPU, from the solver's own converged trajectories of 256 config-4-style cloths (2M vertex
-    samples); then for each state of a reference trajectory, from the same device checkpoint, plain VBD
-    iterates from x~ and the learned path from the gated prediction until the FP64 relative force
-    residual is at most 1e-4 (checked every iteration), timing both with the host clock around the
-    whole substep."""
+    samples). Then, for each state of a reference trajectory and from the same device checkpoint: plain VBD
+    from x~ with the FP64 convergence check every 1, 2, 4 or 8 sweeps; the history extrapolation x~ + (x* -
+    x~) of the previous substep (the non-learned start); and the network's prediction, the last two behind
+    the deferred gate; each iterates until the FP64 relative force residual is at most 1e-4, timed with the
+    host clock around the whole substep. Finally a deployment run, in which each of the three paths continues
+    from its own 1e-4 solves."""
     import numpy as np
     from spg_amd import learned, scenes
 
@@ -563,25 +565,63 @@ def learned_comparison(scene, args, launch):
     train_set = scenes.cloth_batch(count=256, seed=7)
     f, t = learned.collect(train_set, steps=10, samples=2_000_000, rel_tol=1e-5, max_iterations=1500, check_every=10)
     collect_s = time.perf_counter() - t0
+    # the learned start predicts x* - x~; a second network, trained on the same data, predicts the change of the
+    # previous substep's offset on top of the history start (x~ + c + s MLP), for comparison
     t0 = time.perf_counter()
-    net, losses = learned.train(f, t, epochs=12, batch=4096, lr=2e-3, seed=2)
+    net, losses = learned.train(f, t, epochs=12, batch=4096, lr=2e-3, seed=2, history_base=False)
     train_s = time.perf_counter() - t0
-    rows = learned.compare(scene, net, steps=args.learned_steps, rel_tol=1e-4, max_iterations=3000, check_every=1,
-                           smoothing=2, reference_tol=1e-5, reference_iterations=2000,
-                           launch={k: launch[k] for k in ("lanes_per_vertex", "block_size", "lds_staging")})
+    net_hist, losses_hist = learned.train(f, t, epochs=12, batch=4096, lr=2e-3, seed=2, history_base=True)
+    nets = {"learned": net, "learned_on_history": net_hist}
+    lk = {k: launch[k] for k in ("lanes_per_vertex", "block_size", "lds_staging")}
+    rows = learned.compare(scene, nets, steps=args.learned_steps, rel_tol=1e-4, max_iterations=3000, check_every=1,
+                           smoothing=2, reference_tol=1e-5, reference_iterations=2000, cadences=(1, 2, 4, 8),
+                           launch=lk)
     sm = learned.summarize(rows)
+    drows = learned.deploy(scene, nets, steps=args.learned_steps, rel_tol=1e-4, check_every=1, smoothing=2, launch=lk)
+    dsm = learned.summarize_deploy(drows)
+
+    def clean(r):
+        out = {}
+        for k, v in r.items():
+            if k == "plain_by_cadence":
+                out[k] = {str(ce): {"iterations": c["iterations"], "rho": c["rho"], "ms": round(c["ms"], 4)}
+                          for ce, c in v.items()}
+            elif isinstance(v, float) and k.endswith("_ms"):
+                out[k] = round(v, 4)
+            elif isinstance(v, float) and not np.isfinite(v):
+                out[k] = None
+            else:
+                out[k] = v  # residual ratios unrounded
+        return out
+
     return {"metric": "wall time to a 1e-4 relative FP64 force residual per substep, learned warm start vs plain VBD",
             "workload": f"same cloth as above, {args.learned_steps} consecutive steps of a reference trajectory "
-                        f"(plain VBD, 2000 iterations per step); from each state plain VBD starts at x~ and the "
-                        f"learned path at the gated prediction (2 one-ring smoothing passes); FP64 residual checked "
-                        f"after every sweep; warm start (features, inference, gate residual) included",
+                        f"(plain VBD, 2000 iterations per step); from each state: plain VBD from x~ (checks every "
+                        f"1/2/4/8 sweeps), the history start x~ + c (c = (x* - x~) of the previous substep), the "
+                        f"learned start x~ + s MLP (the network predicts x* - x~) and, for comparison, x~ + c + s MLP' "
+                        f"(a second network predicts the change of c), the starts behind the deferred gate, 2 "
+                        f"one-ring smoothing passes of the network output, checks after every sweep; warm start "
+                        f"(features, inference, gate residual) included",
             "plain_ms": sm["plain_ms"], "learned_ms": sm["learned_ms"], "time_ratio": sm["time_ratio"],
             "plain_iterations": sm["plain_iterations"], "learned_iterations": sm["learned_iterations"],
-            "iteration_ratio": sm["iteration_ratio"], "accepted": sm["accepted"], "fell_back": sm["fell_back"],
-            "all_converged": sm["all_converged"],
-            "per_step": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+            "iteration_ratio": sm["iteration_ratio"],
+            "plain_ms_by_check_cadence": {str(k): v for k, v in sm["plain_ms_by_cadence"].items()},
+            "plain_iterations_by_check_cadence": {str(k): v for k, v in sm["plain_iterations_by_cadence"].items()},
+            "best_plain_check_cadence": sm["best_plain_cadence"], "best_plain_ms": sm["best_plain_ms"],
+            "time_ratio_vs_best_plain": sm["time_ratio_vs_best_plain"],
+            "history_ms": sm["history_ms"], "history_iterations": sm["history_iterations"],
+            "history_all_converged": sm["history_all_converged"],
+            "time_ratio_vs_history": sm["time_ratio_vs_history"],
+            "iteration_ratio_vs_history": sm["iteration_ratio_vs_history"],
+            "learned_on_history": sm["learned_on_history"],
+            "accepted": sm["accepted"], "fell_back": sm["fell_back"], "all_converged": sm["all_converged"],
+            "deployment": {"what": "three contexts from the initial state, each continuing from its own 1e-4 solves "
+                                   f"for {args.learned_steps} steps (checks after every sweep)", **dsm,
+                           "per_step": [clean(r) for r in drows]},
+            "per_step": [clean(r) for r in rows],
             "training": {"samples": int(f.shape[0]), "scenes": len(train_set), "collect_s": collect_s,
                          "train_s": train_s, "epochs": len(losses), "final_loss": float(losses[-1]),
+                         "final_loss_change_target": float(losses_hist[-1]),
                          "target_variance": float(np.var(t, axis=0).sum())}}
 
 

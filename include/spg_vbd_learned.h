@@ -52,6 +52,12 @@ const char *spg_mlp_last_error(const spg_mlp *net);
  * b3, W4 (3x128), b4 (3). */
 int spg_mlp_get_params(spg_mlp *net, float *params);
 int spg_mlp_set_params(spg_mlp *net, const float *params);
+/* enabled 1: the network's output y is the change of the previous substep's offset, (x* - x~ - c) / s with c
+ * the previous substep's x* - x~ (feature columns 0-2 are c / s), and spg_vbd_warm_start starts from
+ * x~ + c + s y, on top of the history start of spg_vbd_history_start. 0 (the default): y is (x* - x~) / s and
+ * the start is x~ + s y. A property of how the network was trained; kept with the network. */
+int spg_mlp_set_history_base(spg_mlp *net, int32_t enabled);
+int spg_mlp_get_history_base(spg_mlp *net, int32_t *enabled);
 /* Per-feature standardisation, SPG_MLP_FEATURES floats each; std must be > 0. */
 int spg_mlp_get_normalization(spg_mlp *net, float *mean, float *std_dev);
 int spg_mlp_set_normalization(spg_mlp *net, const float *mean, const float *std_dev);
@@ -72,7 +78,8 @@ int spg_mlp_train(spg_mlp *net, const float *features, const float *targets, int
 
 /* Feature rows of `object`'s particles (n_particles x 24, particle order) for the open substep. */
 int spg_vbd_features(spg_vbd_ctx *ctx, int32_t object, float *features);
-/* Replaces the open substep's start x~ by the network's prediction x~ + s y. The predicted displacement
+/* Replaces the open substep's start x~ by the network's prediction x~ + s y (x~ + c + s y for a network with
+ * spg_mlp_set_history_base). The predicted displacement
  * y is first averaged `smoothing` times over each vertex's one ring (y_i <- (y_i + sum_j y_j) / (1 + deg),
  * 0..64 passes), which removes vertex-to-vertex noise of the prediction and keeps its smooth part.
  * gate 0: the prediction is kept. gate 1: each object keeps it only if rho = |r(pred)| / |r(x~)| <= 1
@@ -83,6 +90,14 @@ int spg_vbd_features(spg_vbd_ctx *ctx, int32_t object, float *features);
  * spg_vbd_stats.prediction_fallbacks). accepted and rho (may be NULL) receive one value per object. */
 int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t smoothing, int32_t gate, int32_t *accepted,
                        double *rho);
+/* The non-learned start that the network is measured against: replaces the open substep's start x~ by
+ * x~ + (x* - x~) of the previous substep (the solution's offset from its inertial guess carried over, the
+ * quantity of feature columns 0-2), with the gates of spg_vbd_warm_start. No reference counterpart. */
+int spg_vbd_history_start(spg_vbd_ctx *ctx, int32_t gate, int32_t *accepted, double *rho);
+/* Per object, rho = |r(start)| / |r(x~)| of the open (or last) substep's start as its gate measured it: by
+ * spg_vbd_warm_start / spg_vbd_history_start with gate 1, or by the first check of
+ * spg_vbd_iterate_to_tolerance for a deferred gate; NaN when not measured (plain starts, gate 0). */
+int spg_vbd_start_rho(const spg_vbd_ctx *ctx, double *rho);
 
 #ifdef __cplusplus
 }

@@ -50,6 +50,7 @@ struct spg_mlp {
     int maxBlocks = 512;         // persistent workgroups of the fused inference kernel
     std::vector<float> hMean, hStd;
     int64_t adamT = 0;
+    int32_t historyBase = 0;  // 1: the output is the change of the previous substep's offset (see spg_mlp_set_history_base)
 
     // workspaces
     int64_t trainRows = 0;
@@ -299,6 +300,7 @@ void mlpInferAfter(spg_mlp *net, const float *features, int64_t n, float *y, int
 }
 
 int mlpDevice(const spg_mlp *net) { return net->device; }
+int mlpHistoryBase(const spg_mlp *net) { return net->historyBase; }
 
 }  // namespace spgvbd
 
@@ -390,6 +392,22 @@ int spg_mlp_get_normalization(spg_mlp *net, float *mean, float *std_dev)
         if (!net || !mean || !std_dev) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         std::copy(net->hMean.begin(), net->hMean.end(), mean);
         std::copy(net->hStd.begin(), net->hStd.end(), std_dev);
+    });
+}
+
+int spg_mlp_set_history_base(spg_mlp *net, int32_t enabled)
+{
+    return guardedMlp(net, [&] {
+        if (!net) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        net->historyBase = enabled ? 1 : 0;
+    });
+}
+
+int spg_mlp_get_history_base(spg_mlp *net, int32_t *enabled)
+{
+    return guardedMlp(net, [&] {
+        if (!net || !enabled) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        *enabled = net->historyBase;
     });
 }
 

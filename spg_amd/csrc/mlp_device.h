@@ -72,7 +72,9 @@ struct FeatureArgs {
 };
 
 hipError_t launchFeatures(const FeatureArgs &a, hipStream_t stream);
-hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float *y, int32_t ldy, float s, int32_t n,
+// x = x~ + base + s y (base may be null: x~ + s y)
+hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float4 *base, const float *y, int32_t ldy, float s,
+                                int32_t n,
                                 hipStream_t stream);
 // y, out: n rows of 4 floats (the first three are the displacement)
 hipError_t launchSmoothPrediction(const float *y, float *out, const int32_t *nbPtr, const int32_t *nbIdx, int32_t n,
@@ -92,5 +94,6 @@ void mlpInferDevice(spg_mlp *net, const float *features, int64_t n, float *y, in
 void mlpInferAfter(spg_mlp *net, const float *features, int64_t n, float *y, int32_t ldy, hipEvent_t ready,
                    hipEvent_t done);
 int mlpDevice(const spg_mlp *net);
+int mlpHistoryBase(const spg_mlp *net);
 
 }  // namespace spgvbd

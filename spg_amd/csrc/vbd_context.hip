@@ -195,6 +195,8 @@ struct spg_vbd_ctx {
     std::vector<int32_t> predicted;  // per object: the open substep started from a network prediction
     bool anyPredicted = false;
     int64_t predictionFallbacks = 0;
+    // per object |r(start)| / |r(x~)| of the open substep's start as its gate measured it (NaN: not measured)
+    std::vector<double> startRho;
 
     spg_vbd_launch launch{4, 256, 0, 0};
     bool timing = false;
@@ -365,6 +367,7 @@ struct spg_vbd_ctx {
         r0OnCheckStream = false;
         anyPredicted = false;
         predicted.assign(nObjects, 0);
+        startRho.assign(nObjects, std::numeric_limits<double>::quiet_NaN());
         for (int k = 0; k < 3; ++k) this->g[k] = g[k];
         hipCheck(launchBeginSubstep(pos.p, xold.p, vel.p, xt.p, mass.p, static_cast<int32_t>(nv), static_cast<float>(hh),
                                     static_cast<float>(g[0]), static_cast<float>(g[1]), static_cast<float>(g[2]),
@@ -565,7 +568,6 @@ struct spg_vbd_ctx {
     // check of iterate_to_tolerance apply the same gate (no residual here; rho is reported as NaN).
     void warmStart(spg_mlp *net, int32_t smoothing, int32_t gateMode, int32_t *accepted, double *rho)
     {
-        const bool gate = gateMode == 1;
         if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "warm_start needs an open substep (begin_substep)");
         if (mlpDevice(net) != device) throw VbdError(SPG_VBD_ERR_INVALID, "network and context on different devices");
         if (smoothing < 0 || smoothing > 64) throw VbdError(SPG_VBD_ERR_INVALID, "smoothing passes must be in 0..64");
@@ -583,7 +585,6 @@ struct spg_vbd_ctx {
         // the inference runs on the network's stream while this stream computes |r(x~)| for the gate
         mlpInferAfter(net, feat.p, nv, y, 4, evFeatures, evPrediction);
         hipCheck(hipSetDevice(device), "hipSetDevice");
-        const bool deferGate = gateMode == 2;
         const bool needResiduals = gateMode == 1 || (gateMode == 0 && rho);
         const bool needR0 = needResiduals && !r0Valid && !r0OnCheckStream;
         if (needR0) enqueueResidual(true, normsAux.p);
@@ -592,10 +593,21 @@ struct spg_vbd_ctx {
             hipCheck(launchSmoothPrediction(y, ySpare, nbPtr.p, nbIdx.p, static_cast<int32_t>(nv), stream), "smooth");
             std::swap(y, ySpare);
         }
-        hipCheck(launchApplyWarmStart(pos.p, xt.p, y, 4, static_cast<float>(h * h * refAccel()),
-                                      static_cast<int32_t>(nv), stream),
+        // a network trained on the change of the previous substep's offset predicts on top of that offset
+        hipCheck(launchApplyWarmStart(pos.p, xt.p, mlpHistoryBase(net) ? corr.p : nullptr, y, 4,
+                                      static_cast<float>(h * h * refAccel()), static_cast<int32_t>(nv), stream),
                  "warm start launch");
         positionsChanged();
+        finishStart(gateMode, needResiduals, needR0, accepted, rho);
+    }
+
+    // The end of a start (network prediction or history extrapolation) already written to the positions:
+    // gate 1 compares |r(start)| with |r(x~)| per object now and sends rejected objects back to x~; gate 2
+    // leaves that test to the first check of iterate_to_tolerance; gate 0 keeps the start. Marks the kept
+    // objects as started from a prediction and reports accepted / rho.
+    void finishStart(int32_t gateMode, bool needResiduals, bool needR0, int32_t *accepted, double *rho)
+    {
+        const bool gate = gateMode == 1, deferGate = gateMode == 2;
         std::vector<int32_t> rej(nObjects, 0);
         std::vector<double> r0(nObjects, 0.0), rp(nObjects, 0.0);
         if (needResiduals) {  // |r(pred)| after the inference, one synchronisation for both norms
@@ -636,8 +648,28 @@ struct spg_vbd_ctx {
                 rho[o] = needResiduals ? (r0[o] > 0.0 ? rp[o] / r0[o] : rp[o]) : std::numeric_limits<double>::quiet_NaN();
         }
         anyPredicted = std::find(predicted.begin(), predicted.end(), 1) != predicted.end();
+        startRho.assign(nObjects, std::numeric_limits<double>::quiet_NaN());
+        if (needResiduals)
+            for (int32_t o = 0; o < nObjects; ++o) startRho[o] = r0[o] > 0.0 ? rp[o] / r0[o] : rp[o];
         // a deferred gate leaves the prediction queued on the stream for the iterations that follow
         if (!deferGate) hipCheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+
+    // Start from the history: x = x~ + (x* - x~) of the previous substep (the solution's offset from its own
+    // inertial guess carried over, the quantity the network's first three inputs hold), with the gate of
+    // warmStart.
+    void historyStart(int32_t gateMode, int32_t *accepted, double *rho)
+    {
+        if (!inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "history_start needs an open substep (begin_substep)");
+        if (gateMode == 2 && !r0Valid && !r0OnCheckStream) enqueueInertialResidualOnCheckStream();
+        const bool needResiduals = gateMode == 1 || (gateMode == 0 && rho);
+        const bool needR0 = needResiduals && !r0Valid && !r0OnCheckStream;
+        if (needR0) enqueueResidual(true, normsAux.p);
+        hipCheck(launchApplyWarmStart(pos.p, xt.p, nullptr, reinterpret_cast<const float *>(corr.p), 4, 1.0f,
+                                      static_cast<int32_t>(nv), stream),
+                 "history start launch");
+        positionsChanged();
+        finishStart(gateMode, needResiduals, needR0, accepted, rho);
     }
 
     // largest per-object |r| / |r(x~)|; NaN when any object's residual is not finite
@@ -749,6 +781,11 @@ struct spg_vbd_ctx {
                 r0OnCheckStream = false;
             }
             ++checked;
+            if (checked == 1 && firstUnitEmpty && anyPredicted) {  // the deferred gate's check
+                startRho.assign(nObjects, std::numeric_limits<double>::quiet_NaN());
+                for (int32_t o = 0; o < nObjects; ++o)
+                    if (predicted[o]) startRho[o] = r0[o] > 0.0 ? r[o] / r0[o] : r[o];
+            }
             bool fallBack = false;
             for (int32_t o = 0; anyPredicted && o < nObjects; ++o) fallBack = fallBack || (predicted[o] && !(r[o] <= r0[o]));
             const bool stop = fallBack || relMax(r0, r) <= tol || sweepsAt[checked] >= maxSweeps;
@@ -1809,6 +1846,26 @@ int spg_vbd_warm_start(spg_vbd_ctx *ctx, spg_mlp *net, int32_t smoothing, int32_
         hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
         if (gate < 0 || gate > 2) throw VbdError(SPG_VBD_ERR_INVALID, "gate must be 0, 1 or 2");
         ctx->warmStart(net, smoothing, gate, accepted, rho);
+    });
+}
+
+int spg_vbd_history_start(spg_vbd_ctx *ctx, int32_t gate, int32_t *accepted, double *rho)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        if (gate < 0 || gate > 2) throw VbdError(SPG_VBD_ERR_INVALID, "gate must be 0, 1 or 2");
+        ctx->historyStart(gate, accepted, rho);
+    });
+}
+
+int spg_vbd_start_rho(const spg_vbd_ctx *ctx, double *rho)
+{
+    return guarded(const_cast<spg_vbd_ctx *>(ctx), [&] {
+        if (!ctx || !rho) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
+        for (int32_t o = 0; o < ctx->nObjects; ++o)
+            rho[o] = o < static_cast<int32_t>(ctx->startRho.size()) ? ctx->startRho[o]
+                                                                    : std::numeric_limits<double>::quiet_NaN();
     });
 }
 

@@ -87,11 +87,18 @@ __global__ void __launch_bounds__(256) smoothPredictionKernel(const float *__res
 
 // x = x~ + s * y for every vertex (y: the network output, row stride ldy).
 __global__ void __launch_bounds__(256) applyWarmStartKernel(float4 *__restrict__ pos, const float4 *__restrict__ xt,
+                                                            const float4 *__restrict__ base,
                                                             const float *__restrict__ y, int32_t ldy, float s, int32_t n)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 t = xt[i];
+    float4 t = xt[i];
+    if (base) {
+        const float4 b = base[i];
+        t.x += b.x;
+        t.y += b.y;
+        t.z += b.z;
+    }
     const float *yi = y + static_cast<int64_t>(i) * ldy;
     pos[i] = make_float4(t.x + s * yi[0], t.y + s * yi[1], t.z + s * yi[2], 0.f);
 }
@@ -129,11 +136,11 @@ hipError_t launchFeatures(const FeatureArgs &a, hipStream_t stream)
     return hipGetLastError();
 }
 
-hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float *y, int32_t ldy, float s, int32_t n,
-                                hipStream_t stream)
+hipError_t launchApplyWarmStart(float4 *pos, const float4 *xt, const float4 *base, const float *y, int32_t ldy, float s,
+                                int32_t n, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    applyWarmStartKernel<<<blocksOf(n), 256, 0, stream>>>(pos, xt, y, ldy, s, n);
+    applyWarmStartKernel<<<blocksOf(n), 256, 0, stream>>>(pos, xt, base, y, ldy, s, n);
     return hipGetLastError();
 }
 

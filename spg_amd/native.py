@@ -194,11 +194,15 @@ def vbd_lib():
         _proto(L, "spg_mlp_set_params", ctypes.c_int, [c, _fp])
         _proto(L, "spg_mlp_get_normalization", ctypes.c_int, [c, _fp, _fp])
         _proto(L, "spg_mlp_set_normalization", ctypes.c_int, [c, _fp, _fp])
+        _proto(L, "spg_mlp_set_history_base", ctypes.c_int, [c, i32])
+        _proto(L, "spg_mlp_get_history_base", ctypes.c_int, [c, _ip])
         _proto(L, "spg_mlp_forward", ctypes.c_int, [c, _fp, i64, _fp])
         _proto(L, "spg_mlp_loss_grad", ctypes.c_int, [c, _fp, _fp, i64, _dp, _fp])
         _proto(L, "spg_mlp_train", ctypes.c_int, [c, _fp, _fp, i64, i32, i32, ctypes.c_float, u64, i32, _dp])
         _proto(L, "spg_vbd_features", ctypes.c_int, [c, i32, _fp])
         _proto(L, "spg_vbd_warm_start", ctypes.c_int, [c, c, i32, i32, _ip, _dp])
+        _proto(L, "spg_vbd_history_start", ctypes.c_int, [c, i32, _ip, _dp])
+        _proto(L, "spg_vbd_start_rho", ctypes.c_int, [c, _dp])
         # XPBD (include/spg_xpbd.h)
         _proto(L, "spg_xpbd_create", ctypes.c_int, [ctypes.POINTER(c), ctypes.POINTER(XpbdObject), i32, i32, i32])
         _proto(L, "spg_xpbd_destroy", None, [c])
@@ -740,6 +744,21 @@ class VbdContext:
         self._check(self._L.spg_vbd_warm_start(self.h, net.h, int(smoothing), mode, _i(acc), _d(rho)))
         return acc.astype(bool), rho
 
+    def history_start(self, gate=True):
+        """Start the open substep from x~ + (x* - x~) of the previous substep, with the gates of warm_start;
+        returns (accepted, rho) per object (rho NaN when deferred)."""
+        mode = 2 if gate == "deferred" else int(bool(gate))
+        acc = np.zeros(self.n_objects, dtype=np.int32)
+        rho = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_history_start(self.h, mode, _i(acc), _d(rho)))
+        return acc.astype(bool), rho
+
+    def start_rho(self):
+        """Per object |r(start)| / |r(x~)| of the substep's start as its gate measured it (NaN: not measured)."""
+        rho = np.zeros(self.n_objects)
+        self._check(self._L.spg_vbd_start_rho(self.h, _d(rho)))
+        return rho
+
     def launch(self):
         c = VbdLaunch()
         self._check(self._L.spg_vbd_get_launch(self.h, ctypes.byref(c)))
@@ -862,6 +881,16 @@ class Mlp:
     def set_normalization(self, mean, std):
         m, s = _f32(mean, (MLP_FEATURES,)), _f32(std, (MLP_FEATURES,))
         self._check(self._L.spg_mlp_set_normalization(self.h, _fl(m), _fl(s)))
+
+    def set_history_base(self, enabled):
+        """True: the output is the change of the previous substep's offset; the warm start adds it to the
+        history start x~ + (x* - x~)_previous."""
+        self._check(self._L.spg_mlp_set_history_base(self.h, int(bool(enabled))))
+
+    def history_base(self):
+        v = np.zeros(1, dtype=np.int32)
+        self._check(self._L.spg_mlp_get_history_base(self.h, _i(v)))
+        return bool(v[0])
 
     def forward(self, features):
         f = _f32(features, (-1, MLP_FEATURES))

@@ -359,3 +359,40 @@ def test_batched_gate_and_tolerance_are_per_object():
     assert (n, r) == (n_ref, r_ref)
     for o in range(2):
         assert all(np.array_equal(a, b) for a, b in zip(ctx.state(o), ref.state(o)))
+
+
+def test_history_start_is_the_previous_offset_and_gated():
+    """spg_vbd_history_start: without history the start is x~ exactly (rho 1); after a substep it is
+    x~ + (x* - x~)_previous in FP32; the deferred gate's measurement equals the immediate gate's."""
+    s = scenes.cloth(20, young=3e3, pin="top_row", seed=4, iterations=6)
+    ctx = native.VbdContext([s.vbd_object()])
+    ctx.begin_substep(s.dt, s.gravity)
+    xt0, _ = ctx.state()
+    acc, rho = ctx.history_start(gate=True)
+    assert acc[0] and rho[0] == 1.0
+    assert np.array_equal(ctx.state()[0], xt0)
+    ctx.iterate(6)
+    x1, _ = ctx.state()
+    ctx.end_substep()
+    ctx.save_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    xt1, _ = ctx.state()
+    ctx.history_start(gate=False)
+    xh, _ = ctx.state()
+    f32 = np.float32
+    expect = xt1.astype(f32) + (x1.astype(f32) - xt0.astype(f32))
+    assert np.array_equal(xh, expect.astype(np.float64))
+    assert np.abs(xh - xt1).max() > 0
+    ctx.end_substep()
+    ctx.restore_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    _, rho_now = ctx.history_start(gate=True)
+    ctx.end_substep()
+    ctx.restore_state()
+    ctx.begin_substep(s.dt, s.gravity)
+    acc_d, rho_d = ctx.history_start(gate="deferred")
+    assert acc_d[0] and np.isnan(rho_d[0])
+    ctx.iterate_to_tolerance(1e-4, max_iterations=2000, check_every=1)
+    ctx.end_substep()
+    print(f"history start rho: immediate gate {rho_now[0]:.6e}, deferred gate {ctx.start_rho()[0]:.6e}", flush=True)
+    assert ctx.start_rho()[0] == pytest.approx(rho_now[0], rel=1e-12)

@@ -73,7 +73,20 @@ def main():
     ap.add_argument("--implicit-steps", type=int, default=4, help="timed Baraff-Witkin steps")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run the plain copy kernel of BYTES (counter calibration) and exit")
+    ap.add_argument("--barrier-calibration", action="store_true",
+                    help="only measure a barrier inside one cooperative launch against back-to-back launches, and exit")
     args = ap.parse_args()
+
+    if args.barrier_calibration:
+        from spg_amd import native
+        rows = []
+        for grid in (256, 512, 1024, 2048):
+            for group in (grid, 8):
+                b, l = native.bench_barrier(grid, block=256, group=group, rounds=500)
+                rows.append({"grid": grid, "block": 256, "group": group, "us_per_barrier": b, "us_per_launch": l})
+                print(json.dumps(rows[-1]), file=sys.stderr, flush=True)
+        print(json.dumps({"barrier_calibration": rows}))
+        return
 
     if args.copy_calibration:
         from spg_amd import native

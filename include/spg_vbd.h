@@ -217,6 +217,17 @@ int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *co
    launch). *ms receives the mean launch time from HIP events. Not part of the solver. */
 int spg_vbd_bench_copy(int64_t bytes, int32_t repeats, int32_t device, double *ms);
 
+/* Cost of a barrier between colours inside one persistent (cooperative) launch against back-to-back launches,
+   the measurement behind one launch per colour. `rounds` barriers of `grid` blocks of `block` threads, counted in
+   groups of `group` consecutive blocks (group == grid: one grid-wide barrier; a smaller group: independent
+   barriers per group, as per-instance barriers of a batch would be), each arrival a release add and each wait a
+   bounded spin on relaxed loads followed by an acquire fence. *us_per_barrier: mean per round (-1 when a spin hit
+   its bound); *us_per_launch: mean of `rounds` back-to-back launches of an empty kernel of the same shape.
+   SPG_VBD_ERR_HIP when the runtime refuses the cooperative launch (blocks not co-resident). Not part of the
+   solver. */
+int spg_vbd_bench_barrier(int32_t grid, int32_t block, int32_t group, int32_t rounds, int32_t device,
+                          double *us_per_barrier, double *us_per_launch);
+
 /* Per-element derivatives (check hook for the element closed forms; not used by the solver): for each
    of n stencils of one type (positions xs: n * size * 3, prepared params: n * nparams), the gradient
    w.r.t. each stencil vertex (n*size*3) and its diagonal 3x3 Hessian block (n*size*9), evaluated in

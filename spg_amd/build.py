@@ -30,7 +30,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", 
              "-Wall", "-Wno-unused-result"]
 
 VBD_SOURCES = ["vbd_kernels.hip", "vbd_context.hip", "mlp_kernels.hip", "mlp_context.hip", "warmstart_kernels.hip",
-               "xpbd_kernels.hip", "xpbd_context.hip", "implicit_kernels.hip", "implicit_context.hip", "rigid_vbd.hip", "coloring.cpp"]
+               "xpbd_kernels.hip", "xpbd_context.hip", "implicit_kernels.hip", "implicit_context.hip", "rigid_vbd.hip", "barrier_bench.hip", "coloring.cpp"]
 HOST_SOURCES = ["particleGroup.cpp", "energies.cpp", "rigidBodyGroup.cpp", "meshIo.cpp", "flatten.cpp", "vbdGpu.cpp", "xpbdGpu.cpp",
                 "implicitEulerGpu.cpp",
                 "host_capi.cpp"]

@@ -171,6 +171,7 @@ def vbd_lib():
         _proto(L, "spg_vbd_save_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_restore_state", ctypes.c_int, [c])
         _proto(L, "spg_vbd_residual", ctypes.c_int, [c, _dp])
+        _proto(L, "spg_vbd_bench_barrier", ctypes.c_int, [i32, i32, i32, i32, i32, _dp, _dp])
         _proto(L, "spg_vbd_residual_noalloc", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_residual_at_inertial", ctypes.c_int, [c, _dp])
         _proto(L, "spg_vbd_iterate_to_tolerance", ctypes.c_int, [c, dbl, i32, i32, _ip, _dp])
@@ -813,6 +814,21 @@ def bench_copy(nbytes, repeats=10, device=0):
     if rc != 0:
         raise NativeError(rc, L.spg_vbd_last_error(None).decode())
     return ms.value
+
+
+def bench_barrier(grid, block=256, group=None, rounds=500, device=0):
+    """(us per barrier round inside one cooperative launch of grid x block threads, counted in groups of `group`
+    blocks (default: the whole grid), us per back-to-back empty launch of the same shape); None for the barrier
+    when the runtime refuses the cooperative launch."""
+    L = vbd_lib()
+    b, l = ctypes.c_double(0.0), ctypes.c_double(0.0)
+    rc = L.spg_vbd_bench_barrier(int(grid), int(block), int(group or grid), int(rounds), device, ctypes.byref(b),
+                                 ctypes.byref(l))
+    if rc == 2:
+        return None, None
+    if rc != 0:
+        raise NativeError(rc, "spg_vbd_bench_barrier: invalid arguments")
+    return (b.value if b.value >= 0 else None), l.value
 
 
 def element_derivatives(kind, xs, params, precision=0, device=0, form=0):

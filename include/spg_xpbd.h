@@ -81,6 +81,9 @@ typedef struct {
 typedef struct {
     int32_t block_size; /* threads per workgroup of the projection kernels: 64, 128 or 256 */
     int32_t use_graph;  /* 1: capture a whole step in a hipGraph and replay it */
+    int32_t group_runs; /* 1 (default): consecutive stencil groups of at most 1024 stencils (the tail colours)
+                           run in one workgroup, in order with a barrier between them, as one launch; 0: one
+                           launch per group. Same result either way. */
 } spg_xpbd_launch;
 
 typedef struct {
@@ -92,7 +95,8 @@ typedef struct {
                                     4 slots + 12 parameters) + 32 B per particle
                                     (float4 read and written) */
     int64_t fp64_solves;         /* projections recomputed in FP64 because A was ill-conditioned */
-    /* the same launches split by energy type (index = SPG_VBD_* type id, entry 0 unused) */
+    /* the same launches split by energy type (index = SPG_VBD_* type id); entry 0: launches that run several
+       consecutive small stencil groups (at most 1024 stencils each, of any type) in one workgroup */
     int64_t type_launches[14];
     double type_ms[14];
     double type_bytes[14];
@@ -111,7 +115,7 @@ int spg_xpbd_num_particles(const spg_xpbd_ctx *ctx, int32_t object, int64_t *n);
 /* Stencil groups of one energy: colours (parallel) or serial levels; group of every stencil. */
 int spg_xpbd_num_groups(const spg_xpbd_ctx *ctx, int32_t object, int32_t energy, int32_t *n);
 int spg_xpbd_get_groups(const spg_xpbd_ctx *ctx, int32_t object, int32_t energy, int32_t *group_of_stencil);
-/* Projection kernel launches per substep (all objects together). */
+/* Projection kernel launches per substep (all objects together), with the launch shape's group runs. */
 int spg_xpbd_num_launches(const spg_xpbd_ctx *ctx, int32_t *n);
 
 int spg_xpbd_set_state(spg_xpbd_ctx *ctx, int32_t object, const double *x, const double *v);

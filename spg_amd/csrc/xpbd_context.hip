@@ -111,8 +111,14 @@ struct spg_xpbd_ctx {
         double bytes;  // compulsory bytes of the launch
     };
     std::vector<Launch> launches;  // per substep, in order
+    // the launches as enqueued: a group alone, or a run of 2 ... kXpbdRunMax consecutive small groups in one
+    // workgroup (the tail colours of the greedy colouring have a few hundred stencils each)
+    struct Item {
+        int32_t first, count;  // launches[first .. first + count - 1]
+    };
+    std::vector<Item> plan, single;  // with group runs / one launch per group
 
-    spg_xpbd_launch launch{256, 1};
+    spg_xpbd_launch launch{256, 1, 1};
     bool timing = false;
     spg_xpbd_stats stats{};
     std::vector<std::pair<hipEvent_t, hipEvent_t>> eventPool, events;
@@ -187,14 +193,21 @@ struct spg_xpbd_ctx {
         for (size_t i = 0; i < events.size(); ++i) {
             float ms = 0.f;
             hipCheck(hipEventElapsedTime(&ms, events[i].first, events[i].second), "hipEventElapsedTime");
-            const Launch &L = launches[eventLaunch[i]];
+            const Item &it = (launch.group_runs ? plan : single)[eventLaunch[i]];
+            int64_t stencils = 0;
+            double bytes = 0.0;
+            for (int32_t k = it.first; k < it.first + it.count; ++k) {
+                stencils += launches[k].end - launches[k].begin;
+                bytes += launches[k].bytes;
+            }
+            const int t = it.count == 1 ? launches[it.first].type : 0;  // 0: a run of small groups
             stats.projection_ms += ms;
             stats.projection_launches += 1;
-            stats.stencil_projections += L.end - L.begin;
-            stats.algorithmic_bytes += L.bytes;
-            stats.type_launches[L.type] += 1;
-            stats.type_ms[L.type] += ms;
-            stats.type_bytes[L.type] += L.bytes;
+            stats.stencil_projections += stencils;
+            stats.algorithmic_bytes += bytes;
+            stats.type_launches[t] += 1;
+            stats.type_ms[t] += ms;
+            stats.type_bytes[t] += bytes;
             eventPool.push_back(events[i]);
         }
         events.clear();
@@ -214,12 +227,25 @@ struct spg_xpbd_ctx {
             else
                 hipCheck(launchXpbdBoundary(pos.p, xold.p, n, invh, fh, hg[0], hg[1], hg[2], stream),
                          "boundary launch");
-            for (size_t k = 0; k < launches.size(); ++k) {
-                const Launch &L = launches[k];
+            const std::vector<Item> &items = launch.group_runs ? plan : single;
+            for (size_t k = 0; k < items.size(); ++k) {
+                const Item &it = items[k];
                 std::pair<hipEvent_t, hipEvent_t> ev{};
                 if (timed) ev = takeEvents();
-                hipCheck(launchXpbdProject(L.type, a, L.begin, L.end, launch.block_size, stream, ev.first, ev.second),
-                         "projection launch");
+                if (it.count == 1) {
+                    const Launch &L = launches[it.first];
+                    hipCheck(launchXpbdProject(L.type, a, L.begin, L.end, launch.block_size, stream, ev.first, ev.second),
+                             "projection launch");
+                } else {
+                    XpbdRun run{};
+                    run.count = it.count;
+                    for (int32_t r = 0; r < it.count; ++r) {
+                        run.type[r] = launches[it.first + r].type;
+                        run.begin[r] = launches[it.first + r].begin;
+                        run.end[r] = launches[it.first + r].end;
+                    }
+                    hipCheck(launchXpbdProjectRun(a, run, stream, ev.first, ev.second), "projection run launch");
+                }
                 if (timed) {
                     events.push_back(ev);
                     eventLaunch.push_back(static_cast<int32_t>(k));
@@ -428,6 +454,18 @@ void buildContext(spg_xpbd_ctx &c, const spg_xpbd_object *objs, int32_t nObjects
         const double bytes = (r.end - r.begin) * (kXpbdRecordBytes[r.type] + 32.0 * ss);
         c.launches.push_back({r.type, r.begin, r.end, bytes});
     }
+    // consecutive small groups (whatever their energy) run in one workgroup, in the same order
+    c.plan.clear();
+    c.single.clear();
+    for (int32_t k = 0; k < static_cast<int32_t>(c.launches.size()); ++k) c.single.push_back({k, 1});
+    for (int32_t k = 0; k < static_cast<int32_t>(c.launches.size());) {
+        int32_t n = 1;
+        auto small = [&](int32_t q) { return c.launches[q].end - c.launches[q].begin <= kXpbdSmallGroup; };
+        if (small(k))
+            while (n < kXpbdRunMax && k + n < static_cast<int32_t>(c.launches.size()) && small(k + n)) ++n;
+        c.plan.push_back({k, n});
+        k += n;
+    }
 
     // stencil records in launch order
     auto gidx = [&](const Entry &en, int k) {
@@ -621,7 +659,7 @@ int spg_xpbd_get_groups(const spg_xpbd_ctx *ctx, int32_t object, int32_t energy,
 int spg_xpbd_num_launches(const spg_xpbd_ctx *ctx, int32_t *n)
 {
     if (!ctx || !n) return SPG_VBD_ERR_INVALID;
-    *n = static_cast<int32_t>(ctx->launches.size());
+    *n = static_cast<int32_t>(ctx->launch.group_runs ? ctx->plan.size() : ctx->single.size());
     return SPG_VBD_OK;
 }
 
@@ -739,6 +777,8 @@ int spg_xpbd_set_launch(spg_xpbd_ctx *ctx, const spg_xpbd_launch *cfg)
         if (!ctx || !cfg) throw VbdError(SPG_VBD_ERR_INVALID, "null argument");
         if (cfg->block_size != 64 && cfg->block_size != 128 && cfg->block_size != 256)
             throw VbdError(SPG_VBD_ERR_INVALID, "block_size must be 64, 128 or 256");
+        if (cfg->group_runs != 0 && cfg->group_runs != 1) throw VbdError(SPG_VBD_ERR_INVALID, "group_runs must be 0 or 1");
+        ctx->drainEvents();  // timed launches of the previous shape are accounted with its plan
         ctx->launch = *cfg;
         ctx->dropGraph();
     });

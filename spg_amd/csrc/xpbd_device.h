@@ -43,6 +43,18 @@ struct XpbdArgs {
     const float4 *extP;  // 3 per record
     unsigned long long *counters;  // [0] projections solved in FP64 (ill-conditioned A)
     float invdt2;
+    int32_t gridBlocks, blockThreads;  // launch shape, set by the launcher
+};
+
+// Consecutive small stencil groups run by one workgroup (xpbdProjectRunKernel), in order.
+constexpr int kXpbdRunMax = 8;          // groups per run
+constexpr int kXpbdRunThreads = 256;    // threads of the run's workgroup
+constexpr int32_t kXpbdSmallGroup = 1024;  // groups of at most this many stencils join runs
+struct XpbdRun {
+    int32_t count;
+    int32_t type[kXpbdRunMax];
+    int32_t begin[kXpbdRunMax];
+    int32_t end[kXpbdRunMax];
 };
 
 constexpr int kXpbdNumTypes = 14;  // type ids 1 ... 13
@@ -53,6 +65,9 @@ constexpr double kXpbdRecordBytes[kXpbdNumTypes] = {0, 16, 56, 24, 68, 20, 64, 6
 // 5 anchor, 6 ... 13 the further energies). start/stop: optional events stamped with the dispatch's own begin/end.
 hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t end, int blockSize,
                              hipStream_t stream, hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+// One workgroup running the groups of `run` in order with a workgroup barrier between them.
+hipError_t launchXpbdProjectRun(const XpbdArgs &a, const XpbdRun &run, hipStream_t stream, hipEvent_t start = nullptr,
+                                hipEvent_t stop = nullptr);
 // Substep boundaries (xpbd.cpp:47-70,110-121): begin: xold = x, v += h g, x += h v;
 // between two substeps: v = (x - xold) / h_prev, then begin, with v kept in registers (vel is not written);
 // end: v = (x - xold) / h. g arguments: h g.

@@ -24,12 +24,10 @@ __device__ __forceinline__ void countFp64(bool fell, unsigned long long *counter
     if (fell && counters) atomicAdd(counters, 1ull);
 }
 
+// The projection of stencil record i of type T (xpbd.cpp:84-95 for one stencil).
 template <int T>
-__global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t begin, int32_t end)
+__device__ __forceinline__ void projectStencil(const XpbdArgs &a, int32_t i)
 {
-    const int32_t blk = xcdContiguousBlock(blockIdx.x, gridDim.x);
-    const int32_t i = begin + blk * static_cast<int32_t>(blockDim.x) + static_cast<int32_t>(threadIdx.x);
-    if (i >= end) return;
     float4 *pos = a.pos;
     if constexpr (T == kSpring) {
         const int2 id = a.sprI[i];
@@ -112,6 +110,48 @@ __global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t beg
     }
 }
 
+// One stencil group (a colour of one energy) over the whole chip: a thread per record. The grid size and
+// threads per block come with the arguments, so the prologue waits for one round of scalar loads.
+template <int T>
+__global__ void __launch_bounds__(256) xpbdProjectKernel(XpbdArgs a, int32_t begin, int32_t end)
+{
+    asm volatile("" ::"s"(a.pos), "s"(a.hinI), "s"(a.hinP), "s"(a.triI), "s"(a.triD), "s"(a.triC0), "s"(a.triC1));
+    const int32_t blk = xcdContiguousBlock(blockIdx.x, a.gridBlocks);
+    const int32_t i = begin + blk * a.blockThreads + static_cast<int32_t>(threadIdx.x);
+    if (i >= end) return;
+    projectStencil<T>(a, i);
+}
+
+// A run of consecutive small stencil groups in one workgroup: the groups in order, each spread over the
+// workgroup's threads, a workgroup barrier between two groups (the writes of one group are visible to the
+// next: the whole run stays on one CU). Stencils of one group share no particle, so the result equals one
+// launch per group.
+__global__ void __launch_bounds__(256) xpbdProjectRunKernel(XpbdArgs a, XpbdRun run)
+{
+    for (int32_t r = 0; r < run.count; ++r) {
+        const int32_t t = run.type[r];
+        for (int32_t i = run.begin[r] + static_cast<int32_t>(threadIdx.x); i < run.end[r];
+             i += static_cast<int32_t>(blockDim.x)) {
+            switch (t) {
+            case kSpring: projectStencil<kSpring>(a, i); break;
+            case kStvk: projectStencil<kStvk>(a, i); break;
+            case kBending: projectStencil<kBending>(a, i); break;
+            case kSnh: projectStencil<kSnh>(a, i); break;
+            case kAnchor: projectStencil<kAnchor>(a, i); break;
+            case 6: projectStencil<6>(a, i); break;
+            case 7: projectStencil<7>(a, i); break;
+            case 8: projectStencil<8>(a, i); break;
+            case 9: projectStencil<9>(a, i); break;
+            case 10: projectStencil<10>(a, i); break;
+            case 11: projectStencil<11>(a, i); break;
+            case 12: projectStencil<12>(a, i); break;
+            default: projectStencil<13>(a, i);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // xpbd.cpp:47-70 with particleGroup.cpp:57-63: x_old = x; v += h g; x += h v (every particle).
 __global__ void __launch_bounds__(256) xpbdBeginKernel(float4 *__restrict__ pos, float4 *__restrict__ vel,
                                                        float4 *__restrict__ xold, int32_t n, float h, float hgx,
@@ -181,10 +221,13 @@ __global__ void xpbdCheckKernel(int type, int64_t n, const double *__restrict__ 
 inline int blocksFor(int64_t n, int bs) { return static_cast<int>((n + bs - 1) / bs); }
 
 template <int T>
-void projectLaunch(const XpbdArgs &a, int32_t begin, int32_t end, int bs, hipStream_t stream, hipEvent_t start,
+void projectLaunch(const XpbdArgs &args, int32_t begin, int32_t end, int bs, hipStream_t stream, hipEvent_t start,
                    hipEvent_t stop)
 {
     const int grid = blocksFor(end - begin, bs);
+    XpbdArgs a = args;
+    a.gridBlocks = grid;
+    a.blockThreads = bs;
     if (start)
         hipExtLaunchKernelGGL(xpbdProjectKernel<T>, dim3(grid), dim3(bs), 0, stream, start, stop, 0, a, begin, end);
     else
@@ -214,6 +257,19 @@ hipError_t launchXpbdProject(int type, const XpbdArgs &a, int32_t begin, int32_t
     case 13: projectLaunch<13>(a, begin, end, blockSize, stream, start, stop); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launchXpbdProjectRun(const XpbdArgs &a, const XpbdRun &run, hipStream_t stream, hipEvent_t start,
+                                hipEvent_t stop)
+{
+    if (run.count < 1 || run.count > kXpbdRunMax) return hipErrorInvalidValue;
+    for (int32_t r = 0; r < run.count; ++r)
+        if (run.type[r] < 1 || run.type[r] > 13 || run.end[r] < run.begin[r]) return hipErrorInvalidValue;
+    if (start)
+        hipExtLaunchKernelGGL(xpbdProjectRunKernel, dim3(1), dim3(kXpbdRunThreads), 0, stream, start, stop, 0, a, run);
+    else
+        xpbdProjectRunKernel<<<1, kXpbdRunThreads, 0, stream>>>(a, run);
     return hipGetLastError();
 }
 

@@ -78,7 +78,7 @@ class XpbdObject(ctypes.Structure):
 
 
 class XpbdLaunch(ctypes.Structure):
-    _fields_ = [("block_size", ctypes.c_int32), ("use_graph", ctypes.c_int32)]
+    _fields_ = [("block_size", ctypes.c_int32), ("use_graph", ctypes.c_int32), ("group_runs", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -93,6 +93,7 @@ class XpbdStats(ctypes.Structure):
     def as_dict(self):
         d = {k: getattr(self, k) for k, _ in self._fields_[:5]}
         names = {t: k for k, t in KIND.items()}
+        names[0] = "small_group_runs"
         d["per_type"] = {names[t]: {"launches": self.type_launches[t], "ms": self.type_ms[t],
                                     "bytes": self.type_bytes[t]} for t in names if self.type_launches[t]}
         return d

@@ -53,7 +53,28 @@ def test_stencil_colors_bit_exact_config1_and_512():
             ref, nc = ol.color_stencils(st)
             assert ctx.num_groups(0, e) == nc
             assert np.array_equal(ctx.groups(0, e, len(st)), ref), kind
+    # 14 bending colours, 7 membrane colours and 1 anchor group; the two 255-hinge tail colours run as one
+    # launch, and so do the 254-triangle tail colour and the 512 anchors
+    ctx.set_launch(group_runs=0)
     assert ctx.num_launches() == 14 + 7 + 1
+    ctx.set_launch(group_runs=1)
+    assert ctx.num_launches() == 12 + 1 + 6 + 1
+
+
+@pytest.mark.parametrize("parallel", [True, False])
+def test_group_runs_bitwise_equal_single_launches(parallel):
+    """Consecutive small stencil groups run in one workgroup give bitwise the steps of one launch per group
+    (512x512 cloth: two tail runs per substep; serial levels of a 48x48 cloth: many small levels)."""
+    for s in (scenes.cloth_config2(seed=3), scenes.cloth(48, young=1e4, pin="top_row")):
+        if not parallel and s.n > 10000:
+            continue
+        out = []
+        for runs in (0, 1):
+            ctx = xpbd_context(s, parallel=parallel)
+            ctx.set_launch(group_runs=runs)
+            ctx.step(s.dt, 4, s.gravity)
+            out.append(ctx.state())
+        assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
 def test_serial_levels_order_property():

@@ -23,9 +23,9 @@ volumetric = 64^3-cell Stable Neo-Hookean cube (config 3) throughput
 xpbd       = the XPBD parallel Gauss-Seidel step (spg's XPBD(true)) on the same cloth: stencil projections/s,
              per-type projection-kernel rooflines, and the FP64 oracle's XPBD step on the host cores
 further_energies = VBD on the same cloth built from the further energies (Baraff-Witkin membrane and bending,
-             continuum edge springs), and the oracle's step on a 64x64 cloth on the host cores
+             continuum edge springs), and the oracle's step (one iteration) on the same cloth on the host cores
 rigid_bodies = VBD on rigid bodies (FP64 6x6 blocks) on a 512x512 lattice of boxes joined by springs and
-             orientation alignments, with the colour-solve roofline and the oracle's step on a 64x64 lattice
+             orientation alignments, with the colour-solve roofline and the oracle's step on the same lattice
 implicit   = the Baraff-Witkin implicit Euler step (FP64: element Hessians, block-sparse assembly, Jacobi CG with
              the reference's 1e-5 / 300-iteration settings) on the same cloth, the roofline of the symmetric
              block-sparse SpMV sampled inside the CG, and the oracle's step on a 128x128 cloth on the host cores
@@ -307,7 +307,7 @@ def further_throughput(args, steps=20):
     """VBD on the bench's cloth built from the further energies: Baraff-Witkin membrane and bending, and
     continuum springs on the mesh edges (one extra incidence class whose vertex blocks come from forward-mode
     passes over the closed-form element gradients). CPU baseline: the oracle's VBD step (dual numbers, as the
-    reference's TinyAD) on a 64x64 cloth of the same construction, about 5 s on the host cores."""
+    reference's TinyAD) on the same 512x512 cloth, one iteration per step, about 5 s on the host cores."""
     import numpy as np
     from spg_amd import scenes
 
@@ -329,18 +329,18 @@ def further_throughput(args, steps=20):
         import oracle_lib as ol
         cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         ol.set_threads(cores)
-        small = build(64)
-        o = ol.from_scene(small)
+        o = ol.from_scene(cloth)
+        o.colors()  # first-step precomputation outside the sample
         n, t = 0, 0.0
         while t < 5.0:
-            o.set_state(small.x, np.zeros_like(small.x))
+            o.set_state(cloth.x, np.zeros_like(cloth.x))
             t0 = time.perf_counter()
-            o.step(small.dt, 1, small.iterations, small.gravity, mode=ol.AD)
+            o.step(cloth.dt, 1, 1, cloth.gravity, mode=ol.AD)
             t += time.perf_counter() - t0
             n += 1
-        out["cpu_baseline"] = {"value": small.n * small.iterations * n / t, "unit": "vertex-updates/s", "cores": cores,
-                               "kind": "port", "sample": f"{n} first steps (20 iterations) of the 64x64 cloth of the "
-                                                         f"same construction, FP64 dual numbers, {t:.1f} s"}
+        out["cpu_baseline"] = {"value": cloth.n * n / t, "unit": "vertex-updates/s", "cores": cores,
+                               "kind": "port", "sample": f"{n} first steps of 1 iteration of the same 512x512 cloth, "
+                                                         f"FP64 dual numbers, {t:.1f} s"}
     return out
 
 
@@ -350,7 +350,8 @@ def rigid_throughput(args, steps=20):
     0.05 rad), dt 1/60, 10 iterations per step over 2 colours, hipGraph replay. Roofline: the colour-solve kernel,
     algorithmic bytes of one sweep (the body's state read and written, and per incident element its record and
     the other body's centre and rotation) over the summed HIP-event durations of the sweep's launches.
-    CPU baseline: the oracle's step (dual numbers, as the reference's TinyAD) on a 64x64 lattice."""
+    CPU baseline: the oracle's step (dual numbers, as the reference's TinyAD) on the same lattice, one iteration
+    per step."""
     import numpy as np
     from spg_amd import native, scenes
 
@@ -384,18 +385,18 @@ def rigid_throughput(args, steps=20):
         import oracle_lib as ol
         cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
         ol.set_threads(cores)
-        small = scenes.rb_lattice(64)
         n, t = 0, 0.0
+        o = ol.rb_from_scene(sc)
+        o.colors()  # first-step precomputation outside the sample
         while t < 5.0:
-            o = ol.rb_from_scene(small)
+            o.set_state(x=sc.x, theta=sc.theta, v=np.zeros_like(sc.x), omega=np.zeros_like(sc.x))
             t0 = time.perf_counter()
-            o.step(small.dt, 1, small.iterations, small.gravity)
+            o.step(sc.dt, 1, 1, sc.gravity)
             t += time.perf_counter() - t0
             n += 1
-        out["cpu_baseline"] = {"value": small.nb * small.iterations * n / t, "unit": "body-updates/s",
-                               "cores": cores, "kind": "port",
-                               "sample": f"{n} first steps (10 iterations) of the 64x64 lattice of the same "
-                                         f"construction, FP64 dual numbers, {t:.1f} s"}
+        out["cpu_baseline"] = {"value": sc.nb * n / t, "unit": "body-updates/s", "cores": cores, "kind": "port",
+                               "sample": f"{n} first steps of 1 iteration of the same {args.rigid_n}x{args.rigid_n} "
+                                         f"lattice, FP64 dual numbers, {t:.1f} s"}
     return out
 
 
@@ -492,8 +493,13 @@ def implicit_throughput(scene, args):
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9
     dofs = ctx.num_dofs()
     iters = st["cg_iterations"] / steps
+    converged = st["last_cg_error"] <= 1e-5
+    status = ("CG converged" if converged else
+              f"CG not converged: capped at 300 iterations as the reference (implicitEulerBase.h:26-27), relative "
+              f"error {st['last_cg_error']:.2e}")
     out = {"workload": f"same cloth, ImplicitEulerBaraffWitkin (FP64), dt=1/60, 1 substep, CG tolerance 1e-5 / "
-                       f"300 iterations, {ctx.num_blocks()} 3x3 blocks, every step from the initial state",
+                       f"300 iterations, {ctx.num_blocks()} 3x3 blocks, every step from the initial state; {status}",
+           "cg_converged": converged,
            "value": dofs * iters * steps / wall, "unit": "DOF x CG-iterations/s",
            "ms_per_step": wall * 1e3 / steps, "steps": steps, "dofs": dofs, "cg_iterations_per_step": iters,
            "last_cg_error": st["last_cg_error"], "assembly_ms_per_step": tst["assembly_ms"] / timing_steps,
@@ -512,7 +518,9 @@ def implicit_throughput(scene, args):
 def implicit_cpu_baseline(budget_s=10.0):
     """The oracle's Baraff-Witkin step (dual-number Hessians as TinyAD, triplet assembly, the Eigen CG
     restatement with an OpenMP SpMV) on a 128x128 cloth of the same construction, repeated from its initial
-    state for about budget_s; reported per DOF x CG iteration like the GPU value."""
+    state for about budget_s; reported per DOF x CG iteration like the GPU value. One oracle step of the 512x512
+    cloth itself takes about 80 s on 8 host cores (dual-number assembly plus 300 CG iterations), above what a
+    bench run can spend, so the sample is the smaller mesh and says so."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as ol
     import numpy as np
@@ -533,8 +541,9 @@ def implicit_cpu_baseline(budget_s=10.0):
         n += 1
     dofs = 3 * sc.n
     return {"value": dofs * its / t, "unit": "DOF x CG-iterations/s", "cores": cores, "kind": "port",
-            "sample": f"{n} Baraff-Witkin steps of a 128x128 cloth ({dofs} DOFs, {its / n:.0f} CG iterations per "
-                      f"step), FP64, {t:.1f} s ({t / n * 1e3:.0f} ms per step)"}
+            "mesh": "128x128 (the GPU line runs 512x512)",
+            "sample": f"{n} Baraff-Witkin steps of a 128x128 cloth of the same construction ({dofs} DOFs, "
+                      f"{its / n:.0f} CG iterations per step), FP64, {t:.1f} s ({t / n * 1e3:.0f} ms per step)"}
 
 
 def xpbd_cpu_baseline(scene, budget_s=8.0):

@@ -324,6 +324,9 @@ def further_throughput(args, steps=20):
                        "kBias = Y t / 2.6) and bending (k 1e-4), continuum springs on the 784,385 edges "
                        "(k 0.1 Y t), top row anchored, 20 iterations/step, hipGraph replay"}
     out.update(_throughput([cloth], steps))
+    out["roofline"] = {"bound": "hbm", "kernel": "solveColorKernel (further-energy classes)",
+                       "achieved": out["solve_achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": out["solve_achieved_GBs"] / HBM_PEAK_GBS, "mean_launch_us": out["solve_launch_us"]}
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as ol

@@ -49,9 +49,14 @@ double recordBytes(int type)
     case SPG_VBD_STABLE_NEO_HOOKEAN: return 64;
     case SPG_VBD_SPRING_ANCHOR: return 20;
     }
-    // further energies: the four slots, the parameters the energy reads (not the padding), and 4 B more for the
-    // 8-byte row entry (the caller adds 4 B per incidence)
-    return isFurther(type) ? 16.0 + 8.0 * fullParams(type) + 4.0 : 0.0;
+    return 0.0;
+}
+// the same for a further-energy incidence of solve class cls: the element's four slots and the FP32
+// coefficients the class reads (not the padding of its planes)
+double furtherRecordBytes(int cls)
+{
+    static constexpr int kFloats[kXCls] = {3, 8, 10, 6, 3, 3, 14, 14};
+    return 16.0 + 4.0 * kFloats[cls];
 }
 
 bool finite3(const double *p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
@@ -166,13 +171,15 @@ struct spg_vbd_ctx {
     int32_t nbW = 0;
     DevBuf<int32_t> nbRow, tetL;
     DevBuf<float4> triL, hinEL, hinWL, sprL;
-    // further energies: CSR rows per slot, elements (slots, padded FP64 parameters)
+    // further energies: CSR rows per slot, elements (slots, padded FP64 parameters) for the residual, and the
+    // solve's per-class fixed-width record rows
     DevBuf<int32_t> extPtr;
     DevBuf<int2> extInc;
     DevBuf<int4> extV;
     DevBuf<double> extP;
-    DevBuf<uint4> extCnt;
-    uint32_t extTypes = 0;
+    int32_t xrW[kXCls] = {};
+    DevBuf<int4> xrV[kXCls];
+    DevBuf<float4> xrP[kXCls];
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int32_t> triRowR, hinERowR, hinWRowR, tetRowR, sprRowR;  // element ids in the solve's row widths
@@ -281,8 +288,12 @@ struct spg_vbd_ctx {
         a.extInc = extInc.p;
         a.extV = extV.p;
         a.extP = extP.p;
-        a.extCnt = extCnt.p;
-        a.extTypes = extTypes;
+        for (int k = 0; k < kXCls; ++k) {
+            a.xrW[k] = xrW[k];
+            a.xrV[k] = xrV[k].p;
+            a.xrP[k] = xrP[k].p;
+            a.xrPlane[k] = static_cast<int64_t>(nv) * xrW[k];
+        }
         a.counters = counters.p;
         a.begin = colorBegin[color];
         a.end = colorBegin[color + 1];
@@ -1027,6 +1038,77 @@ void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows
     hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
 }
 
+// FP32 coefficients of one further-energy incidence (element parameters p, the incidence's slot in the stencil),
+// in the planes of its solve class (the table beside xclsOf).
+void furtherCoefficients(int cls, int slot, const double *p, float4 *q)
+{
+    auto f = [](double x) { return static_cast<float>(x); };
+    switch (cls) {
+    case 0: q[0] = make_float4(f(p[0]), f(p[1]), slot == 0 ? 1.f : -1.f, 0.f); break;
+    case 1:
+        q[0] = make_float4(f(p[0]), f(p[1]), f(p[2]), f(p[3]));
+        q[1] = make_float4(f(p[4]), f(p[5]), f(p[6]), static_cast<float>(slot));
+        break;
+    case 2: {
+        const float4 d = make_float4(f(p[0]), f(p[1]), f(p[2]), f(p[3]));
+        q[0] = d;
+        q[1] = make_float4(f(p[4]), f(p[5]), f(p[6]), f(p[7]));
+        q[2] = make_float4(slot == 0 ? -(d.x + d.z) : slot == 1 ? d.x : d.z,
+                           slot == 0 ? -(d.y + d.w) : slot == 1 ? d.y : d.w, 0.f, 0.f);
+        break;
+    }
+    case 3:
+        q[0] = make_float4(f(p[0]), f(p[1]), f(p[2]), f(p[3]));
+        q[1] = make_float4(f(p[4]), f(p[slot]), 0.f, 0.f);
+        break;
+    case 4:
+    case 5: q[0] = make_float4(f(p[0]), f(p[1]), (slot == 1 || slot == 3) ? 1.f : 0.f, 0.f); break;
+    default: {
+        float gm[3];
+        for (int k = 0; k < 3; ++k) gm[k] = slot == 0 ? -f(p[k] + p[3 + k] + p[6 + k]) : f(p[3 * (slot - 1) + k]);
+        q[0] = make_float4(f(p[0]), f(p[1]), f(p[2]), f(p[3]));
+        q[1] = make_float4(f(p[4]), f(p[5]), f(p[6]), f(p[7]));
+        q[2] = make_float4(f(p[8]), gm[0], gm[1], gm[2]);
+        q[3] = make_float4(f(p[9]), f(p[10]), 0.f, 0.f);
+    }
+    }
+}
+
+// The solve's records of the further energies: per incidence class a fixed-width row per vertex slot (in each row
+// the incidences in the reference's order, energy then ascending stencil), each record the element's slots and
+// its coefficient planes.
+void buildFurtherRecords(spg_vbd_ctx &c, const std::vector<Incidence> &extIncList, const std::vector<int4> &extVh,
+                         const std::vector<double> &extPh)
+{
+    std::vector<Incidence> cls[kXCls];
+    for (const auto &in : extIncList) cls[xclsOf(in.local >> 2, in.local & 3)].push_back(in);
+    for (int k = 0; k < kXCls; ++k) {
+        c.xrW[k] = 0;
+        c.xrV[k].release();
+        c.xrP[k].release();
+        if (cls[k].empty()) continue;
+        std::vector<int32_t> ptr;
+        const auto order = sortBySlot(cls[k], c.nv, ptr);
+        const int32_t w = rowWidth(ptr);
+        const int np = xclsPlanes(k);
+        const size_t plane = static_cast<size_t>(c.nv) * w;
+        std::vector<int4> rv(plane, make_int4(-1, -1, -1, -1));
+        std::vector<float4> rp(plane * np, make_float4(0.f, 0.f, 0.f, 0.f));
+        for (int64_t v = 0; v < c.nv; ++v)
+            for (int32_t i = ptr[v]; i < ptr[v + 1]; ++i) {
+                const Incidence &in = cls[k][order[i]];
+                const size_t r = static_cast<size_t>(v) * w + (i - ptr[v]);
+                float4 q[4];
+                furtherCoefficients(k, in.local & 3, &extPh[static_cast<size_t>(kExtParams) * in.elem], q);
+                rv[r] = extVh[in.elem];
+                for (int j = 0; j < np; ++j) rp[j * plane + r] = q[j];
+            }
+        c.xrW[k] = w;
+        c.xrV[k].upload(rv, c.stream);
+        c.xrP[k].upload(rp, c.stream);
+    }
+}
+
 void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
 {
     if (nObjects < 1 || !objs) throw VbdError(SPG_VBD_ERR_INVALID, "at least one object required");
@@ -1218,7 +1300,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                     else
                         inc[cls].push_back({sl[k], elem, k});
                     const int32_t col = c.localColors[base + st[k]];
-                    colorRecordBytes[col] += recordBytes(cls) + 4.0;
+                    colorRecordBytes[col] += (isFurther(cls) ? furtherRecordBytes(xclsOf(cls, k)) : recordBytes(cls)) + 4.0;
                     colorIncidences[col] += 1;
                 }
             }
@@ -1391,29 +1473,13 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         checkSize(extIncList.size());
         if (extVh.size() >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many elements");
         std::vector<int32_t> ptr;
-        auto order = sortBySlot(extIncList, c.nv, ptr);
-        // within each row, entries of one type together (stable: element order kept), so the solve runs one
-        // type-specialised loop per type; per slot the count of each type
-        std::vector<uint4> cnt(c.nv, make_uint4(0u, 0u, 0u, 0u));
-        for (int64_t v = 0; v < c.nv; ++v) {
-            std::stable_sort(order.begin() + ptr[v], order.begin() + ptr[v + 1], [&](int32_t i, int32_t j) {
-                return (extIncList[i].local >> 2) < (extIncList[j].local >> 2);
-            });
-            uint32_t n[8] = {};
-            for (int32_t k = ptr[v]; k < ptr[v + 1]; ++k) {
-                const int t = extIncList[order[k]].local >> 2;
-                if (++n[t - SPG_VBD_SPRING_CONTINUUM] > 0xffffu)
-                    throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "more than 65535 incidences of one energy type at a particle");
-                c.extTypes |= 1u << t;
-            }
-            cnt[v] = make_uint4(n[0] | n[1] << 16, n[2] | n[3] << 16, n[4] | n[5] << 16, n[6] | n[7] << 16);
-        }
+        const auto order = sortBySlot(extIncList, c.nv, ptr);
         std::vector<int2> rec(order.size());
         for (size_t k = 0; k < order.size(); ++k)
             rec[k] = make_int2(extIncList[order[k]].elem, extIncList[order[k]].local);
+        buildFurtherRecords(c, extIncList, extVh, extPh);
         c.extPtr.upload(ptr, c.stream);
         c.extInc.upload(rec, c.stream);
-        c.extCnt.upload(cnt, c.stream);
         c.extV.upload(extVh, c.stream);
         c.extP.upload(extPh, c.stream);
     }

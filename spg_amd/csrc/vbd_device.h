@@ -37,6 +37,35 @@ enum EnergyType { kSpring = 1, kStvk = 2, kBending = 3, kSnh = 4, kAnchor = 5 };
 // Incidence rows of the solve. Dense classes use fixed-width rows: the j-th incidence of vertex slot v
 // is record v * width + j, and rows shorter than the width are padded with records whose first
 // index is -1 (their contribution is dropped). Anchors, which few vertices have, use row offsets.
+// Incidence classes of the further energies in the solve: 0 squared continuum spring (type 7), 1 Baraff-Witkin
+// membrane (8), 2 Choi membrane (9), 3 quadratic bending (10), 4 Baraff-Witkin bending at a hinge end (type 11,
+// slot 0 or 1), 5 the same at a wing (slot 2 or 3), 6 StVK tetrahedron (12), 7 stable squared Neo-Hookean (13).
+// Coefficient planes per record (float4 each):
+//   0: (L0, k, +1 for slot 0 / -1 for slot 1, 0)
+//   1: (Dinv 00, 01, 10, 11), (k0, k1, k2, slot)
+//   2: (Dinv 00, 01, 10, 11), (k0, k1, k2, k3), (gamma_a0, gamma_a1, 0, 0)
+//   3: (c0, c1, c2, c3), (keff, c of the slot, 0, 0)
+//   4, 5: (theta0, keff, 1 for slot 1 or 3 else 0, 0)
+//   6, 7: (Dinv 0 ... 3), (Dinv 4 ... 7), (Dinv 8, gamma of the slot), (lambda, mu, 0, 0)
+constexpr int kXCls = 8;
+__host__ __device__ constexpr int xclsOf(int type, int slot)
+{
+    switch (type) {
+    case 7: return 0;
+    case 8: return 1;
+    case 9: return 2;
+    case 10: return 3;
+    case 11: return slot < 2 ? 4 : 5;
+    case 12: return 6;
+    default: return 7;
+    }
+}
+__host__ __device__ constexpr int xclsPlanes(int c)
+{
+    return (c == 0 || c == 4 || c == 5) ? 1 : (c == 1 || c == 3) ? 2 : c == 2 ? 3 : 4;
+}
+__host__ __device__ constexpr int xclsStencil(int c) { return c == 0 ? 2 : c <= 2 ? 3 : 4; }
+
 struct SolveArgs {
     float4 *pos;
     const float4 *xt;
@@ -65,15 +94,22 @@ struct SolveArgs {
     const float4 *tetGa;
     const float4 *tetGb;
     const float4 *tetGc;
-    // the further energies (types 6 ... 13): CSR rows per slot of (element, type << 2 | slot in the stencil),
-    // elements as up to four slots (unused -1) and kExtParams prepared FP64 parameters
-    const int32_t *extPtr;  // null: none (the solve kernel's instantiation without this class runs)
-    const int2 *extInc;     // each row sorted by type (element order kept within a type)
+    // the further energies (types 7 ... 13; continuum springs join the spring class): CSR rows per slot of
+    // (element, type << 2 | slot in the stencil) over element tables (up to four slots, unused -1, and kExtParams
+    // FP64 parameters), read by the FP64 residual; extPtr null: none (the solve kernel's instantiation without
+    // these classes runs)
+    const int32_t *extPtr;
+    const int2 *extInc;
     const int4 *extV;
     const double *extP;
-    const uint4 *extCnt;    // per slot, the row's entry count of each type 6 ... 13, 16 bits each (x: 6-7, y: 8-9,
-                            // z: 10-11, w: 12-13)
-    uint32_t extTypes;      // bit t set: type t occurs in some row
+    // the solve's records of the further energies, one fixed-width row per vertex slot and incidence class
+    // (xcls*): record j of slot v at v * xrW[c] + j holds the element's slots in stencil order (xrV; padding
+    // records -1) and xclsPlanes(c) float4 of FP32 coefficients (plane i at xrP[c] + i * xrPlane[c]) that include
+    // what depends on the incidence's own slot in the element
+    int32_t xrW[kXCls];  // 0: class absent
+    const int4 *xrV[kXCls];
+    const float4 *xrP[kXCls];
+    int64_t xrPlane[kXCls];
     unsigned long long *counters;  // [0] FP64 fallbacks, [1] skipped blocks
     int32_t begin, end;            // slot range of the colour
     float invdt2;

@@ -167,9 +167,13 @@ __device__ __forceinline__ void finishVertex(const SolveArgs &a, bool active, in
 // K2: one colour sweep. A group of G lanes of one wavefront cooperates on one vertex; each
 // wavefront holds floor(64 / G) groups (the remaining lanes idle when G does not divide 64). Lane l
 // of a group takes incidences l, l+G, ... of each incidence class (StVK triangles, hinge-edge and
-// hinge-wing slots of bending elements, SNH tetrahedra, springs, anchors), then the force and the
-// six Hessian entries are summed over the group in a fixed order with cross-lane shuffles and the
-// group's first lane solves the 3x3 block and writes the position.
+// hinge-wing slots of bending elements, SNH tetrahedra, springs, anchors and, with EXT, the further
+// energies' classes), then the force and the six Hessian entries are summed over the group in a fixed
+// order with cross-lane shuffles and the group's first lane solves the 3x3 block and writes the position.
+// LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
+// contiguous span per record array) into LDS with coalesced loads, then every lane reads its records
+// from LDS (the further energies' records excepted); positions are gathered from global memory as in the
+// direct variant.
 // Blocks are renumbered so that each XCD works on one contiguous range of the colour
 // (xcdContiguousBlock), keeping neighbouring positions in that XCD's L2.
 
@@ -215,14 +219,12 @@ __host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int 
     return static_cast<size_t>(blockSize / 64) * slabs * slab * 16;
 }
 
-// LDS = true: the workgroup first copies the element-record rows of its contiguous vertex range (one
-// contiguous span per record array) into LDS with coalesced loads, then every lane reads its records
-// from LDS; positions are gathered from global memory as in the direct variant.
-// A further energy's gradient w.r.t. its stencil slot LOCAL and, with HESS, the slot's 3x3 Hessian block: the
-// closed-form full gradient (implicit_element.h) at the element's positions in one forward-mode pass whose three
-// tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template parameter so
-// the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in scratch memory)
-// and the other slots' outputs are dropped. T: float (solve) or double (residual).
+// The further energies' closed-form slot blocks. Each takes the element's positions in stencil order and FP32
+// coefficients that already hold what depends on the slot (the gamma row of a membrane or tetrahedron slot, the
+// spring's sign, the quadratic bending's own coefficient, which end or wing of a hinge), so the lanes of a
+// wavefront run one instruction stream whatever slots they hold. The solve reads these coefficients from its
+// per-incidence records; the per-element hook (extSlot) derives them from the FP64 parameters and the slot.
+
 // Baraff-Witkin bending (baraffWitkinBendingEnergy.cpp:9-36), the gradient and 3x3 block of one hinge slot in
 // closed form. With e0 = x1 - x0, e3 = x2 - x1, e4 = x3 - x1 and s0 = 1/|e0| held constant:
 //   X = n1 . n2 = -[(e0.e0)(e3.e4) - (e0.e4)(e0.e3)]          (Binet-Cauchy identity)
@@ -230,8 +232,10 @@ __host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int 
 // both polynomials in (e0, e3, e4); theta = atan2(Y, X), E = 1/2 k (theta - theta0)^2. A slot's block sums the
 // second derivatives of X and Y over the edge vectors it moves (x0: -e0; x1: e0 - e3 - e4; x2: e3; x3: e4); in the
 // symmetric sum the skew parts cancel and the wing slots (x2, x3), on which X and Y are linear, get none.
-template <int LOCAL>
-__device__ __forceinline__ void bendingBwSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+// WING false: slot 0 (second false) or 1 (second true); WING true: slot 2 or 3. Slot 1's terms are slot 0's with
+// the opposite sign plus the terms of the edges e3, e4 that x1 also moves (weight m = 1, 0 for slot 0).
+template <bool WING>
+__device__ __forceinline__ void bendingBwRec(const V3f *x, float theta0, float k, bool second, V3f &g, S3f &H)
 {
     const V3f e0 = x[1] - x[0], e3 = x[2] - x[1], e4 = x[3] - x[1];
     const float q = dot(e0, e0), s0 = 1.f / sqrtf(q);
@@ -241,38 +245,28 @@ __device__ __forceinline__ void bendingBwSlot(const V3f *x, const double *p, V3f
     const float X = -(q * d34 - d04 * d03), Y = -s0 * q * D;
     V3f gX, gY;
     S3f HX = zeroSym<float>(), HY = zeroSym<float>();
-    if constexpr (LOCAL == 0 || LOCAL == 1) {
+    if constexpr (!WING) {
+        const float m = second ? 1.f : 0.f, sg = second ? 1.f : -1.f;
         const V3f X0 = (-2.f * d34) * e0 + d03 * e4 + d04 * e3;
         const V3f Y0 = (-s0) * ((2.f * D) * e0 + q * g0);
+        const V3f g34 = g3 + g4;
+        gX = sg * X0 - m * ((d04 * e0 - q * e4) + (d03 * e0 - q * e3));
+        gY = sg * Y0 + (m * s0 * q) * g34;
         addSymOuter(HX, 1.f, e4, e3);
-        if constexpr (LOCAL == 0) {
-            gX = -X0;
-            gY = -Y0;
-            addDiag(HX, -2.f * d34);
-            addDiag(HY, -2.f * s0 * D);
-            addSymOuter(HY, -2.f * s0, e0, g0);
-        } else {
-            gX = X0 - (d04 * e0 - q * e4) - (d03 * e0 - q * e3);
-            gY = Y0 + (s0 * q) * (g3 + g4);
-            addDiag(HX, -2.f * (d34 + d04 + d03 + q));
-            addSymOuter(HX, 1.f, e0, e4);
-            addSymOuter(HX, 1.f, e0, e3);
-            addOuter(HX, 2.f, e0);
-            addDiag(HY, -2.f * s0 * D);
-            addSymOuter(HY, -2.f * s0, e0, g0 - g3 - g4);
-        }
-    } else if constexpr (LOCAL == 2) {
-        gX = d04 * e0 - q * e4;
-        gY = (-s0 * q) * g3;
+        addDiag(HX, -2.f * (d34 + m * (d04 + d03 + q)));
+        addSymOuter(HX, m, e0, e3 + e4);
+        addOuter(HX, 2.f * m, e0);
+        addDiag(HY, -2.f * s0 * D);
+        addSymOuter(HY, -2.f * s0, e0, g0 - m * g34);
     } else {
-        gX = d03 * e0 - q * e3;
-        gY = (-s0 * q) * g4;
+        const V3f ew = second ? e3 : e4, gw = second ? g4 : g3;
+        gX = (second ? d03 : d04) * e0 - q * ew;
+        gY = (-s0 * q) * gw;
     }
     const float ir2 = 1.f / (X * X + Y * Y);
     const V3f gt = ir2 * (X * gY - Y * gX);  // grad theta
     const V3f u = X * gX + Y * gY;
-    const float k = static_cast<float>(p[1]);
-    const float kd = k * (atan2f(Y, X) - static_cast<float>(p[0]));
+    const float kd = k * (atan2f(Y, X) - theta0);
     g = kd * gt;
     // k grad theta grad theta^T + k (theta - theta0) [(X HY - Y HX) / r2 - sym(grad theta, u) / r2]
     const float a = kd * ir2;
@@ -284,20 +278,16 @@ __device__ __forceinline__ void bendingBwSlot(const V3f *x, const double *p, V3f
 
 // Baraff-Witkin membrane (membraneBaraffWitkinEnergy.cpp:9-68), one triangle slot in closed form. F = [f0 f1] =
 // [x1 - x0, x2 - x0] Dinv, C = (|f0| - 1, |f1| - 1, f0 . f1), E = 1/2 sum k_i C_i^2 (k_i the effective stiffness
-// diagonal, area included); slot a moves f0 by gamma_a0 and f1 by gamma_a1 (the rows of Dinv, gamma_0 their
-// negated sum), so its block is gamma_a0^2 (A0 + k2 f1 f1^T) + gamma_a1^2 (A1 + k2 f0 f0^T)
+// diagonal, area included); slot a moves f0 by gamma_a0 (ga) and f1 by gamma_a1 (gb) (the rows of Dinv, gamma_0
+// their negated sum), so its block is gamma_a0^2 (A0 + k2 f1 f1^T) + gamma_a1^2 (A1 + k2 f0 f0^T)
 // + gamma_a0 gamma_a1 k2 (2 C2 I + f0 f1^T + f1 f0^T) with Ai = k_i (fi fi^T / |fi|^3 + C_i / |fi| I).
-template <int LOCAL>
-__device__ __forceinline__ void membraneBwSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+__device__ __forceinline__ void membraneBwRec(const V3f *x, float4 d, float k0, float k1, float k2, float ga, float gb,
+                                              V3f &g, S3f &H)
 {
-    const float d0 = float(p[0]), d1 = float(p[1]), d2 = float(p[2]), d3 = float(p[3]);
     const V3f u = x[1] - x[0], v = x[2] - x[0];
-    const V3f f0 = d0 * u + d2 * v, f1 = d1 * u + d3 * v;
-    const float k0 = float(p[4]), k1 = float(p[5]), k2 = float(p[6]);
+    const V3f f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
     const float n0 = sqrtf(dot(f0, f0)), n1 = sqrtf(dot(f1, f1));
     const float c0 = n0 - 1.f, c1 = n1 - 1.f, c2 = dot(f0, f1);
-    const float ga = LOCAL == 0 ? -(d0 + d2) : LOCAL == 1 ? d0 : d2;
-    const float gb = LOCAL == 0 ? -(d1 + d3) : LOCAL == 1 ? d1 : d3;
     const float i0 = 1.f / n0, i1 = 1.f / n1;
     const V3f dEdf0 = (k0 * c0 * i0) * f0 + (k2 * c2) * f1, dEdf1 = (k1 * c1 * i1) * f1 + (k2 * c2) * f0;
     g = ga * dEdf0 + gb * dEdf1;
@@ -319,72 +309,62 @@ __device__ __forceinline__ void unitStretchSlot(V3f f, float k, float c, V3f &g,
 }
 
 // Choi membrane (membraneChoiEnergy.cpp:9-28): four unit stretches, of f0, f1, gp = (f0 + f1)/sqrt 2 and
-// gm = (f0 - f1)/sqrt 2; slot a moves them by gamma_a0, gamma_a1, (gamma_a0 +- gamma_a1)/sqrt 2.
-template <int LOCAL>
-__device__ __forceinline__ void membraneChoiSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+// gm = (f0 - f1)/sqrt 2; the slot moves them by ga, gb, (ga +- gb)/sqrt 2.
+__device__ __forceinline__ void membraneChoiRec(const V3f *x, float4 d, float4 k, float ga, float gb, V3f &g, S3f &H)
 {
-    const float d0 = float(p[0]), d1 = float(p[1]), d2 = float(p[2]), d3 = float(p[3]);
     const V3f u = x[1] - x[0], v = x[2] - x[0];
-    const V3f f0 = d0 * u + d2 * v, f1 = d1 * u + d3 * v;
-    const float ga = LOCAL == 0 ? -(d0 + d2) : LOCAL == 1 ? d0 : d2;
-    const float gb = LOCAL == 0 ? -(d1 + d3) : LOCAL == 1 ? d1 : d3;
+    const V3f f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
     const float r = 0.70710678118654752440f;
     g = {0.f, 0.f, 0.f};
     H = zeroSym<float>();
-    unitStretchSlot(f0, float(p[4]), ga, g, H);
-    unitStretchSlot(f1, float(p[5]), gb, g, H);
-    unitStretchSlot(r * (f0 + f1), float(p[6]), r * (ga + gb), g, H);
-    unitStretchSlot(r * (f0 - f1), float(p[7]), r * (ga - gb), g, H);
+    unitStretchSlot(f0, k.x, ga, g, H);
+    unitStretchSlot(f1, k.y, gb, g, H);
+    unitStretchSlot(r * (f0 + f1), k.z, r * (ga + gb), g, H);
+    unitStretchSlot(r * (f0 - f1), k.w, r * (ga - gb), g, H);
 }
 
 // Quadratic bending (quadraticBendingEnergy.cpp:9-30): E = 1/2 keff |C|^2 with C = sum c_s x_s, so slot a has
-// gradient keff c_a C and block keff c_a^2 I.
-template <int LOCAL>
-__device__ __forceinline__ void bendingQuadraticSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+// gradient keff c_a C and block keff c_a^2 I (cs = c_a).
+__device__ __forceinline__ void bendingQuadraticRec(const V3f *x, float4 c, float keff, float cs, V3f &g, S3f &H)
 {
-    const V3f C = float(p[0]) * x[0] + float(p[1]) * x[1] + float(p[2]) * x[2] + float(p[3]) * x[3];
-    const float kc = float(p[4]) * float(p[LOCAL]);
+    const V3f C = c.x * x[0] + c.y * x[1] + c.z * x[2] + c.w * x[3];
+    const float kc = keff * cs;
     g = kc * C;
     H = zeroSym<float>();
-    addDiag(H, kc * float(p[LOCAL]));
+    addDiag(H, kc * cs);
 }
 
 // Squared continuum spring (springSquaredContinuumEnergy.cpp:9-19): C = |d|^2 / L0^2 - 1, E = 1/2 Keff C^2,
-// d = x0 - x1: both slots have the block Keff ((2/L0^2)^2 d d^T + C (2/L0^2) I).
-template <int LOCAL>
-__device__ __forceinline__ void springSquaredContinuumSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+// d = x0 - x1: both slots have the block Keff ((2/L0^2)^2 d d^T + C (2/L0^2) I); the gradient has sign sg
+// (+1 for slot 0, -1 for slot 1).
+__device__ __forceinline__ void springSquaredContinuumRec(const V3f *x, float L0, float k, float sg, V3f &g, S3f &H)
 {
     const V3f d = x[0] - x[1];
-    const float L2 = float(p[0]) * float(p[0]), k = float(p[1]);
+    const float L2 = L0 * L0;
     const float C = dot(d, d) / L2 - 1.f, s = 2.f / L2;
-    g = ((LOCAL == 0 ? 1.f : -1.f) * k * C * s) * d;
+    g = (sg * k * C * s) * d;
     H = zeroSym<float>();
     addDiag(H, k * C * s);
     addOuter(H, k * s * s, d);
 }
 
-// Tetrahedra: F = [f0 f1 f2] = [x1 - x0, x2 - x0, x3 - x0] Dinv and slot a moves column c by gamma_ac (row a - 1
-// of Dinv, gamma_0 the negated column sums), so it moves F by u gamma^T and w = F gamma is F's rate.
-template <int LOCAL>
-__device__ __forceinline__ void tetFrame(const V3f *x, const double *p, V3f *f, float *gm)
+// Tetrahedra: F = [f0 f1 f2] = [x1 - x0, x2 - x0, x3 - x0] Dinv (Dinv row-major in dv[0..8]) and the slot moves
+// column c by gm[c] (row a - 1 of Dinv, gamma_0 the negated column sums), so it moves F by u gamma^T and
+// w = F gamma is F's rate.
+__device__ __forceinline__ void tetFrameRec(const V3f *x, const float *dv, V3f *f)
 {
     const V3f u = x[1] - x[0], v = x[2] - x[0], w = x[3] - x[0];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        f[c] = float(p[c]) * u + float(p[3 + c]) * v + float(p[6 + c]) * w;
-        gm[c] = LOCAL == 0 ? -float(p[c] + p[3 + c] + p[6 + c]) : float(p[3 * (LOCAL - 1) + c]);
-    }
+    for (int c = 0; c < 3; ++c) f[c] = dv[c] * u + dv[3 + c] * v + dv[6 + c] * w;
 }
 
 // StVK tetrahedron (stvkEnergy.cpp:41-63, energy form): P = F S, S = lambda tr(E) I + 2 mu E; the slot block is
 // (gamma^T S gamma) I + (lambda + mu) w w^T + mu |gamma|^2 F F^T.
-template <int LOCAL>
-__device__ __forceinline__ void tetStvkSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+__device__ __forceinline__ void tetStvkRec(const V3f *x, const float *dv, const float *gm, float lam, float mu,
+                                           V3f &g, S3f &H)
 {
     V3f f[3];
-    float gm[3];
-    tetFrame<LOCAL>(x, p, f, gm);
-    const float lam = float(p[9]), mu = float(p[10]);
+    tetFrameRec(x, dv, f);
     float S[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -413,13 +393,11 @@ __device__ __forceinline__ void tetStvkSlot(const V3f *x, const double *p, V3f &
 // Stable squared Neo-Hookean (stableSquaredNeoHookeanEnergy.cpp:9-31): E = 1/2 lambda (J - 1)^2
 // + 1/2 mu (|F|^2 - 3)^2; J is affine along the slot's rank-one update, so the block is lambda c c^T
 // + mu (4 w w^T + 2 (|F|^2 - 3) |gamma|^2 I) with c = cof(F) gamma.
-template <int LOCAL>
-__device__ __forceinline__ void tetSsnhSlot(const V3f *x, const double *p, V3f &g, S3f &H)
+__device__ __forceinline__ void tetSsnhRec(const V3f *x, const float *dv, const float *gm, float lam, float mu,
+                                           V3f &g, S3f &H)
 {
     V3f f[3];
-    float gm[3];
-    tetFrame<LOCAL>(x, p, f, gm);
-    const float lam = float(p[9]), mu = float(p[10]);
+    tetFrameRec(x, dv, f);
     const V3f k0 = cross(f[1], f[2]), k1 = cross(f[2], f[0]), k2 = cross(f[0], f[1]);
     const float J = dot(f[0], k0), s = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.f;
     const V3f c = gm[0] * k0 + gm[1] * k1 + gm[2] * k2;
@@ -432,6 +410,59 @@ __device__ __forceinline__ void tetSsnhSlot(const V3f *x, const double *p, V3f &
     addDiag(H, 2.f * mu * s * g2);
 }
 
+// The slot-dependent coefficients from an element's FP64 parameters (the same values the host writes into the
+// solve's records): membrane gamma rows, tetrahedron gamma rows (the negated column sums summed in FP64).
+template <int LOCAL>
+__device__ __forceinline__ void triGammaOf(float4 d, float &ga, float &gb)
+{
+    ga = LOCAL == 0 ? -(d.x + d.z) : LOCAL == 1 ? d.x : d.z;
+    gb = LOCAL == 0 ? -(d.y + d.w) : LOCAL == 1 ? d.y : d.w;
+}
+template <int LOCAL>
+__device__ __forceinline__ void tetRecOf(const double *p, float *dv, float *gm)
+{
+#pragma unroll
+    for (int i = 0; i < 9; ++i) dv[i] = float(p[i]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+        gm[c] = LOCAL == 0 ? -float(p[c] + p[3 + c] + p[6 + c]) : float(p[3 * (LOCAL - 1) + c]);
+}
+
+// The closed-form slot block of further type TYPE (7 ... 13) from the element's FP64 parameters.
+template <int TYPE, int LOCAL>
+__device__ __forceinline__ void extRecOf(const V3f *xs, const double *p, V3f &g, S3f &H)
+{
+    if constexpr (TYPE == 11) {
+        bendingBwRec<(LOCAL >= 2)>(xs, float(p[0]), float(p[1]), LOCAL == 1 || LOCAL == 3, g, H);
+    } else if constexpr (TYPE == 8 || TYPE == 9) {
+        const float4 d = make_float4(float(p[0]), float(p[1]), float(p[2]), float(p[3]));
+        float ga, gb;
+        triGammaOf<LOCAL>(d, ga, gb);
+        if constexpr (TYPE == 8)
+            membraneBwRec(xs, d, float(p[4]), float(p[5]), float(p[6]), ga, gb, g, H);
+        else
+            membraneChoiRec(xs, d, make_float4(float(p[4]), float(p[5]), float(p[6]), float(p[7])), ga, gb, g, H);
+    } else if constexpr (TYPE == 10) {
+        bendingQuadraticRec(xs, make_float4(float(p[0]), float(p[1]), float(p[2]), float(p[3])), float(p[4]),
+                            float(p[LOCAL]), g, H);
+    } else if constexpr (TYPE == 7) {
+        springSquaredContinuumRec(xs, float(p[0]), float(p[1]), LOCAL == 0 ? 1.f : -1.f, g, H);
+    } else {
+        float dv[9], gm[3];
+        tetRecOf<LOCAL>(p, dv, gm);
+        if constexpr (TYPE == 12)
+            tetStvkRec(xs, dv, gm, float(p[9]), float(p[10]), g, H);
+        else
+            tetSsnhRec(xs, dv, gm, float(p[9]), float(p[10]), g, H);
+    }
+}
+
+// A further energy's gradient w.r.t. its stencil slot LOCAL and, with HESS, the slot's 3x3 Hessian block. In
+// FP32 with the type known at compile time (7 ... 13) the block is the closed form above; otherwise the
+// closed-form full gradient (implicit_element.h) at the element's positions is taken in one forward-mode pass
+// whose three tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template
+// parameter so the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in
+// scratch memory) and the other slots' outputs are dropped. T: float (solve) or double (residual).
 // TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
@@ -451,20 +482,8 @@ __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos
         else
             fullGradOf<TYPE, T>(xs, p, gd);
         g = gd[LOCAL];
-    } else if constexpr (TYPE == 11 && std::is_same_v<T, float>) {
-        bendingBwSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 8 && std::is_same_v<T, float>) {
-        membraneBwSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 9 && std::is_same_v<T, float>) {
-        membraneChoiSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 10 && std::is_same_v<T, float>) {
-        bendingQuadraticSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 7 && std::is_same_v<T, float>) {
-        springSquaredContinuumSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 12 && std::is_same_v<T, float>) {
-        tetStvkSlot<LOCAL>(xs, p, g, H);
-    } else if constexpr (TYPE == 13 && std::is_same_v<T, float>) {
-        tetSsnhSlot<LOCAL>(xs, p, g, H);
+    } else if constexpr (TYPE >= 7 && TYPE <= 13 && std::is_same_v<T, float>) {
+        extRecOf<TYPE, LOCAL>(xs, p, g, H);
     } else {
         using D3 = ad::Dual3T<T>;
         V3<D3> x[NS], gd[NS];
@@ -510,25 +529,51 @@ __device__ __forceinline__ void extSlotAny(int type, int local, const float4 *__
     }
 }
 
-// The entries of one further type in a slot's row (rows are sorted by type; `word` packs two 16-bit counts):
-// G lanes stride over them with the type's own code, and `base` moves past them.
-template <int TYPE, int G>
-__device__ __forceinline__ void extTypeRun(const SolveArgs &a, int32_t &base, uint32_t word, int32_t lig, V3f &g,
-                                           S3f &H)
+// One further-energy incidence class of the vertex's records (fixed-width row, G lanes striding over it): the
+// record's coefficients already hold what depends on the vertex's own slot, so every lane runs the class's one
+// closed form.
+template <int C, int G>
+__device__ __forceinline__ void extClassRun(const SolveArgs &a, int32_t vid, int32_t lig, V3f &g, S3f &H)
 {
-    const int32_t n = static_cast<int32_t>((word >> (16 * ((TYPE - 6) & 1))) & 0xffffu);
-    if (a.extTypes & (1u << TYPE)) {
-        for (int32_t k = lig; k < n; k += G) {
-            const int2 in = a.extInc[base + k];
-            V3f ge;
-            S3f He;
-            extSlotAny<fullStencilSize(TYPE), float, true, TYPE>(TYPE, in.y & 3, a.pos, a.extV[in.x],
-                                                                  a.extP + static_cast<int64_t>(kExtParams) * in.x,
-                                                                  ge, He);
-            acc(g, H, ge, He);
+    constexpr int NS = xclsStencil(C), NP = xclsPlanes(C);
+    const int32_t w = a.xrW[C];
+    for (int32_t j = lig; j < w; j += G) {
+        const int64_t r = static_cast<int64_t>(vid) * w + j;
+        const int4 v = a.xrV[C][r];
+        float4 q[NP];
+#pragma unroll
+        for (int i = 0; i < NP; ++i) q[i] = a.xrP[C][i * a.xrPlane[C] + r];
+        const bool ok = v.x >= 0;
+        const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+        V3f xs[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) xs[s] = xyz(a.pos[ok ? vv[s] : vid]);
+        V3f ge;
+        S3f He;
+        if constexpr (C == 0) {
+            springSquaredContinuumRec(xs, q[0].x, q[0].y, q[0].z, ge, He);
+        } else if constexpr (C == 1) {
+            const float4 d = q[0];
+            const float sl = q[1].w;
+            const float ga = sl == 0.f ? -(d.x + d.z) : sl == 1.f ? d.x : d.z;
+            const float gb = sl == 0.f ? -(d.y + d.w) : sl == 1.f ? d.y : d.w;
+            membraneBwRec(xs, d, q[1].x, q[1].y, q[1].z, ga, gb, ge, He);
+        } else if constexpr (C == 2) {
+            membraneChoiRec(xs, q[0], q[1], q[2].x, q[2].y, ge, He);
+        } else if constexpr (C == 3) {
+            bendingQuadraticRec(xs, q[0], q[1].x, q[1].y, ge, He);
+        } else if constexpr (C == 4 || C == 5) {
+            bendingBwRec<C == 5>(xs, q[0].x, q[0].y, q[0].z != 0.f, ge, He);
+        } else {
+            const float dv[9] = {q[0].x, q[0].y, q[0].z, q[0].w, q[1].x, q[1].y, q[1].z, q[1].w, q[2].x};
+            const float gm[3] = {q[2].y, q[2].z, q[2].w};
+            if constexpr (C == 6)
+                tetStvkRec(xs, dv, gm, q[3].x, q[3].y, ge, He);
+            else
+                tetSsnhRec(xs, dv, gm, q[3].x, q[3].y, ge, He);
         }
+        accIf(ok, g, H, ge, He);
     }
-    base += n;
 }
 template <class T, bool HESS>
 __device__ __forceinline__ void extIncidence(int2 in, const float4 *__restrict__ pos, const int4 *__restrict__ extV,
@@ -662,16 +707,14 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             acc(g, H, ge, He);
         }
         if constexpr (EXT) {
-            int32_t base = a.extPtr[vid];
-            const uint4 cnt = a.extCnt[vid];
-            extTypeRun<6, G>(a, base, cnt.x, lig, g, H);
-            extTypeRun<7, G>(a, base, cnt.x, lig, g, H);
-            extTypeRun<8, G>(a, base, cnt.y, lig, g, H);
-            extTypeRun<9, G>(a, base, cnt.y, lig, g, H);
-            extTypeRun<10, G>(a, base, cnt.z, lig, g, H);
-            extTypeRun<11, G>(a, base, cnt.z, lig, g, H);
-            extTypeRun<12, G>(a, base, cnt.w, lig, g, H);
-            extTypeRun<13, G>(a, base, cnt.w, lig, g, H);
+            extClassRun<0, G>(a, vid, lig, g, H);
+            extClassRun<1, G>(a, vid, lig, g, H);
+            extClassRun<2, G>(a, vid, lig, g, H);
+            extClassRun<3, G>(a, vid, lig, g, H);
+            extClassRun<4, G>(a, vid, lig, g, H);
+            extClassRun<5, G>(a, vid, lig, g, H);
+            extClassRun<6, G>(a, vid, lig, g, H);
+            extClassRun<7, G>(a, vid, lig, g, H);
         }
     }
     finishVertex<G>(a, active, vid, lane64, lig, xo, xt4, g, H);

@@ -183,6 +183,10 @@ struct spg_vbd_ctx {
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int32_t> triRowR, hinERowR, hinWRowR, tetRowR, sprRowR;  // element ids in the solve's row widths
+    // the neighbourhood-staged residual's row entries (element id, slot indices into the neighbour row) and the
+    // particle index of each slot
+    DevBuf<int2> triRL, hinERL, hinWRL, tetRL, sprRL;
+    DevBuf<int32_t> gidOfSlotD;
     DevBuf<int2> sprV;
     DevBuf<double2> sprP;
     DevBuf<double4> ancP;
@@ -450,6 +454,14 @@ struct spg_vbd_ctx {
         a.hinWRow = hinWRowR.p;
         a.tetRow = tetRowR.p;
         a.sprRow = sprRowR.p;
+        a.nbW = nbW;
+        a.nbRow = nbRow.p;
+        a.triRL = triRL.p;
+        a.hinERL = hinERL.p;
+        a.hinWRL = hinWRL.p;
+        a.tetRL = tetRL.p;
+        a.sprRL = sprRL.p;
+        a.gidOf = gidOfSlotD.p;
         a.hinKc = kcBuf;
         a.nHinges = static_cast<int32_t>(hinVe.n);
         hipCheck(launchResidual(a, s), "residual launch");
@@ -929,9 +941,16 @@ void buildFeatureTables(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObj
 // (c.nbW = 0) when some vertex names more than kNbLayoutMax - 1 other vertices.
 constexpr int32_t kNbLayoutMax = 16;
 
+// element-id rows of the residual in the solve's widths (host copies)
+struct ResidualRowsH {
+    std::vector<int32_t> tri, hin[2], tet, spr;
+};
+
 void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows, const std::vector<int2> &triO,
                               const std::vector<float4> &triM, const std::vector<int4> (&hinV)[2],
-                              const std::vector<float2> (&hinP)[2], const std::vector<int4> &tetO)
+                              const std::vector<float2> (&hinP)[2], const std::vector<int4> &tetO,
+                              const ResidualRowsH &res, const std::vector<int4> &triE, const std::vector<int4> &hinE,
+                              const std::vector<int4> &tetE, const std::vector<int2> &sprE)
 {
     const int64_t nv = c.nv;
     std::vector<int32_t> ent;  // row of the vertex being built
@@ -1027,6 +1046,34 @@ void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows
                 hinL[cls][r] = make_float4(hinP[cls][r].x, hinP[cls][r].y, intBitsAsFloat(pk), 0.f);
             }
     }
+    // the residual's row entries: element id and the slab indices of its slots in stencil order
+    auto resEntries = [&](const std::vector<int32_t> &ids, int32_t w, auto slotsOf) {
+        std::vector<int2> out(ids.size(), make_int2(-1, -1));
+        for (int64_t v = 0; v < nv; ++v)
+            for (int32_t j = 0; j < w; ++j) {
+                const size_t r = v * w + j;
+                if (ids[r] >= 0) out[r] = make_int2(ids[r], slotsOf(v, ids[r]));
+            }
+        return out;
+    };
+    auto pack4 = [&](int64_t v, const int4 &e, int n) {
+        const int32_t s[4] = {e.x, e.y, e.z, e.w};
+        int32_t pk = 0;
+        for (int k = 0; k < n; ++k) pk |= loc(v, s[k]) << (5 * k);
+        return pk;
+    };
+    const auto triRL = resEntries(res.tri, c.triW, [&](int64_t v, int32_t e) { return pack4(v, triE[e], 3); });
+    const auto hinERL = resEntries(res.hin[0], c.hinEW, [&](int64_t v, int32_t e) { return pack4(v, hinE[e], 4); });
+    const auto hinWRL = resEntries(res.hin[1], c.hinWW, [&](int64_t v, int32_t e) { return pack4(v, hinE[e], 4); });
+    const auto tetRL = resEntries(res.tet, c.tetW, [&](int64_t v, int32_t e) { return pack4(v, tetE[e], 4); });
+    const auto sprRL = resEntries(res.spr, c.sprW, [&](int64_t v, int32_t e) {
+        return loc(v, sprE[e].x == v ? sprE[e].y : sprE[e].x);
+    });
+    if (!triRL.empty()) c.triRL.upload(triRL, c.stream);
+    if (!hinERL.empty()) c.hinERL.upload(hinERL, c.stream);
+    if (!hinWRL.empty()) c.hinWRL.upload(hinWRL, c.stream);
+    if (!tetRL.empty()) c.tetRL.upload(tetRL, c.stream);
+    if (!sprRL.empty()) c.sprRL.upload(sprRL, c.stream);
     c.nbW = width;
     c.nbRow.upload(nb, c.stream);
     if (!triL.empty()) c.triL.upload(triL, c.stream);
@@ -1106,6 +1153,8 @@ void buildFurtherRecords(spg_vbd_ctx &c, const std::vector<Incidence> &extIncLis
         c.xrW[k] = w;
         c.xrV[k].upload(rv, c.stream);
         c.xrP[k].upload(rp, c.stream);
+        // the host vectors are released at the end of this iteration; the uploads must have read them
+        hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
     }
 }
 
@@ -1193,6 +1242,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             for (int64_t g = c.objOffset[o]; g < c.objOffset[o + 1]; ++g) objSlots[g] = c.slotOfGid[g];
         }
         c.objSlots.upload(objSlots, c.stream);
+        c.gidOfSlotD.upload(c.gidOfSlot, c.stream);
         c.objOffsets.upload(c.objSlotsOffsets, c.stream);
         std::vector<int64_t> seg{0};
         std::vector<int32_t> objSeg{0};
@@ -1309,7 +1359,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     auto checkSize = [](size_t n) {
         if (n >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many incidences");
     };
-    // the solve's fixed-width rows, kept on the host for the neighbourhood layout
+    // the solve's fixed-width rows and the residual's element-id rows, kept on the host for the neighbourhood layout
+    ResidualRowsH resRows;
     std::vector<float4> sprRowsH, triMH;
     std::vector<int2> triOH;
     std::vector<int4> hinVH[2], tetOH;
@@ -1331,7 +1382,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.spr.upload(sprRowsH, c.stream);
         c.sprPtr.upload(ptr, c.stream);
         c.sprElem.upload(el, c.stream);
-        c.sprRowR.upload(toRows(ptr, c.sprW, -1, [&](int32_t k) { return el[k]; }), c.stream);
+        resRows.spr = toRows(ptr, c.sprW, -1, [&](int32_t k) { return el[k]; });
+        c.sprRowR.upload(resRows.spr, c.stream);
         c.sprV.upload(sprV, c.stream);
         c.sprP.upload(sprP, c.stream);
     }
@@ -1382,7 +1434,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.triM.upload(triMH, c.stream);
         c.triPtr.upload(ptr, c.stream);
         c.triElem.upload(el, c.stream);
-        c.triRowR.upload(toRows(ptr, c.triW, -1, [&](int32_t k) { return el[k]; }), c.stream);
+        resRows.tri = toRows(ptr, c.triW, -1, [&](int32_t k) { return el[k]; });
+        c.triRowR.upload(resRows.tri, c.stream);
         c.triV.upload(triV, c.stream);
         c.triP.upload(triP, c.stream);
     }
@@ -1418,7 +1471,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             (cls == 0 ? c.hinEV : c.hinWV).upload(hinVH[cls], c.stream);
             (cls == 0 ? c.hinEP : c.hinWP).upload(hinPH[cls], c.stream);
             (cls == 0 ? c.hinEElem : c.hinWElem).upload(el, c.stream);
-            (cls == 0 ? c.hinERowR : c.hinWRowR).upload(toRows(ptr, w, -1, [&](int32_t k) { return el[k]; }), c.stream);
+            resRows.hin[cls] = toRows(ptr, w, -1, [&](int32_t k) { return el[k]; });
+            (cls == 0 ? c.hinERowR : c.hinWRowR).upload(resRows.hin[cls], c.stream);
         }
         c.hinVe.upload(hinV, c.stream);
         c.hinPe.upload(hinP, c.stream);
@@ -1463,7 +1517,8 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.tetGb.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gb[k]; }), c.stream);
         c.tetGc.upload(toRows(ptr, c.tetW, z4, [&](int32_t k) { return gc[k]; }), c.stream);
         c.tetElem.upload(el, c.stream);
-        c.tetRowR.upload(toRows(ptr, c.tetW, -1, [&](int32_t k) { return el[k]; }), c.stream);
+        resRows.tet = toRows(ptr, c.tetW, -1, [&](int32_t k) { return el[k]; });
+        c.tetRowR.upload(resRows.tet, c.stream);
         c.tetV.upload(tetV, c.stream);
         c.tetP.upload(tetP, c.stream);
     }
@@ -1485,7 +1540,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     }
 
     if (extIncList.empty())
-        buildNeighbourhoodLayout(c, sprRowsH, triOH, triMH, hinVH, hinPH, tetOH);
+        buildNeighbourhoodLayout(c, sprRowsH, triOH, triMH, hinVH, hinPH, tetOH, resRows, triV, hinV, tetV, sprV);
 
     // compulsory bytes per colour sweep: 12 B per vertex of every object that has this colour (all
     // positions read once), 32 B per updated vertex (x write, x_tilde read, mass, row offset), and

@@ -212,11 +212,15 @@ __host__ __device__ inline size_t solveLdsBytes(const SolveArgs &a, int G, int b
 // LDS bytes of the neighbourhood-staged variant: per vertex group of the workgroup (and one spare per
 // wavefront for its idle lanes when G does not divide 64) a slab of ceil(16 / G) G float4 (the row's at most
 // 16 entries, rounded up to whole lane strides).
-__host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int blockSize)
+__host__ __device__ inline size_t nbSlabBytes(int G, int blockSize)
 {
     const size_t slab = static_cast<size_t>((16 + G - 1) / G) * G;
     const size_t slabs = static_cast<size_t>(64 / G + (64 % G ? 1 : 0));
     return static_cast<size_t>(blockSize / 64) * slabs * slab * 16;
+}
+__host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int blockSize)
+{
+    return nbSlabBytes(G, blockSize);
 }
 
 // The further energies' closed-form slot blocks. Each takes the element's positions in stencil order and FP32
@@ -913,6 +917,37 @@ __device__ __forceinline__ double groupReduceD(double v, int32_t lane64, int32_t
     return v;
 }
 
+// The FP64 StVK gradient at a triangle slot: p01 .. p67 the element's prepared parameters (rows of Dinv
+// g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2); kd, ko, ks), s0 / s2 whether the vertex is slot 0 / slot 2, xa, xb
+// the other two slots' positions in stencil order.
+__device__ __forceinline__ V3d triSlotGradD(double2 p01, double2 p23, double2 p45, double2 p67, bool s0, bool s2,
+                                            const V3d &xo, const V3d &xa, const V3d &xb)
+{
+    const double g00 = -(p01.x + p23.x), g01 = -(p01.y + p23.y);
+    const double ga0 = s0 ? p01.x : g00, ga1 = s0 ? p01.y : g01;
+    const double gb0 = s2 ? p01.x : p23.x, gb1 = s2 ? p01.y : p23.y;
+    V3d ge;
+    S3d He;
+    stvkVertex<double>(xo, xa, xb, ga0, ga1, gb0, gb1, p45.x, p45.y, p67.x, ge, He);
+    return ge;
+}
+
+// The FP64 Stable Neo-Hookean gradient at a tetrahedron slot: p the element's twelve prepared parameters (rows
+// of Dinv g1 = p0..2, g2 = p3..5, g3 = p6..8, g0 = -(g1 + g2 + g3); lambda, mu, alpha), s0 / s01 / s012 whether
+// the vertex's slot is 0 / at most 1 / at most 2, x0 .. x2 the other three slots' positions in stencil order.
+__device__ __forceinline__ V3d tetSlotGradD(const double2 *p, bool s0, bool s01, bool s012, const V3d &xo,
+                                            const V3d &x0, const V3d &x1, const V3d &x2)
+{
+    const double2 p01 = p[0], p23 = p[1], p45 = p[2], p67 = p[3], p89 = p[4], pab = p[5];
+    const V3d g1 = {p01.x, p01.y, p23.x}, g2 = {p23.y, p45.x, p45.y}, g3 = {p67.x, p67.y, p89.x};
+    const V3d g0 = -(g1 + g2 + g3);
+    const V3d ga = s0 ? g1 : g0, gb = s01 ? g2 : g1, gc = s012 ? g3 : g2;
+    V3d ge;
+    S3d He;
+    snhVertex<double>(xo, x0, x1, x2, ga, gb, gc, p89.y, pab.x, pab.y, ge, He);
+    return ge;
+}
+
 // K4: r_v = -(m/h^2)(x_v - x_tilde_v) - sum_e grad_v E_e(x), all in FP64 from the FP32 state and the
 // exact prepared parameters (implicitEulerBase.cpp:102-143 force, x-form of
 // implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per particle (particle order).
@@ -949,15 +984,10 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
             if (el < 0) continue;
             const int4 v = a.triV[el];
             const double2 *p = reinterpret_cast<const double2 *>(a.triP + 8 * static_cast<int64_t>(el));
-            const double2 p01 = p[0], p23 = p[1], p45 = p[2], p67 = p[3];
-            // rows of Dinv: g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2); the other two slots in stencil order
+            // the other two slots in stencil order
             const bool s0 = v.x == vid, s2 = v.z == vid;
-            const double g00 = -(p01.x + p23.x), g01 = -(p01.y + p23.y);
             const int32_t ia = s0 ? v.y : v.x, ib = s2 ? v.y : v.z;
-            const double ga0 = s0 ? p01.x : g00, ga1 = s0 ? p01.y : g01;
-            const double gb0 = s2 ? p01.x : p23.x, gb1 = s2 ? p01.y : p23.y;
-            stvkVertex<double>(xo, xyzd(a.pos[ia]), xyzd(a.pos[ib]), ga0, ga1, gb0, gb1, p45.x, p45.y, p67.x, ge, He);
-            r = r - ge;
+            r = r - triSlotGradD(p[0], p[1], p[2], p[3], s0, s2, xo, xyzd(a.pos[ia]), xyzd(a.pos[ib]));
         }
         // hinges: k (theta - theta0) once per hinge (hingeKcKernel), the slot's gradient of theta here
         for (int32_t j = lig; j < a.hinEW; j += G) {  // hinge-edge incidences (slots 0, 1)
@@ -979,17 +1009,10 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
             if (el < 0) continue;
             const int4 v = a.tetV[el];
             const double2 *p = reinterpret_cast<const double2 *>(a.tetP + 12 * static_cast<int64_t>(el));
-            const double2 p01 = p[0], p23 = p[1], p45 = p[2], p67 = p[3], p89 = p[4], pab = p[5];
-            // rows of Dinv: g1 = p0..2, g2 = p3..5, g3 = p6..8, g0 = -(g1 + g2 + g3); the other three slots in
-            // stencil order: s = 0: (1, 2, 3), 1: (0, 2, 3), 2: (0, 1, 3), 3: (0, 1, 2)
-            const V3d g1 = {p01.x, p01.y, p23.x}, g2 = {p23.y, p45.x, p45.y}, g3 = {p67.x, p67.y, p89.x};
-            const V3d g0 = -(g1 + g2 + g3);
+            // the other three slots in stencil order: s = 0: (1, 2, 3), 1: (0, 2, 3), 2: (0, 1, 3), 3: (0, 1, 2)
             const bool s0 = v.x == vid, s01 = s0 || v.y == vid, s012 = s01 || v.z == vid;
             const int32_t i0 = s0 ? v.y : v.x, i1 = s01 ? v.z : v.y, i2 = s012 ? v.w : v.z;
-            const V3d ga = s0 ? g1 : g0, gb = s01 ? g2 : g1, gc = s012 ? g3 : g2;
-            snhVertex<double>(xo, xyzd(a.pos[i0]), xyzd(a.pos[i1]), xyzd(a.pos[i2]), ga, gb, gc, p89.y, pab.x, pab.y,
-                              ge, He);
-            r = r - ge;
+            r = r - tetSlotGradD(p, s0, s01, s012, xo, xyzd(a.pos[i0]), xyzd(a.pos[i1]), xyzd(a.pos[i2]));
         }
         for (int32_t j = lig; j < a.sprW; j += G) {
             const int32_t el = a.sprRow[row * a.sprW + j];
@@ -1215,6 +1238,168 @@ size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize
     return ldsStaging == 1 ? solveLdsBytes(a, lanesPerVertex, blockSize) : 0;
 }
 
+// K4, neighbourhood-staged variant: the same residual in the same order of operations (bitwise the same result),
+// with the vertex's group first gathering its own and its distinct neighbours' positions into an LDS slab (the
+// neighbour rows of the staged solve), so each position is read from memory once per vertex, and row entries
+// that carry the element id and the 5-bit slab indices of its slots in stencil order (the vertex itself is
+// index 0). Vertices run in slot order (the output index comes with the row), and the first entry of every class
+// and its element's parameters are loaded before the gather: a vertex waits for two rounds of memory loads.
+template <int G, int CLS>
+__global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
+{
+    constexpr int GPW = 64 / G;
+    constexpr int R = (kNbMax + G - 1) / G;
+    constexpr int SLABS = GPW + (64 % G ? 1 : 0);
+    extern __shared__ float4 nbSlab[];
+    const int32_t lane64 = threadIdx.x & 63;
+    const int32_t wib = threadIdx.x >> 6;
+    const int32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const int32_t grp = lane64 / G, lig = lane64 - grp * G;
+    const int32_t vs = wave * GPW + grp;
+    const bool active = grp < GPW && vs < a.n;
+    const int32_t vid = active ? vs : 0;
+    const int64_t row = vid;
+    float4 *slab = nbSlab + static_cast<int64_t>(wib * SLABS + grp) * (R * G);
+    // first round: row entries, neighbour ids, the vertex's own data
+    const int2 none = make_int2(-1, -1);
+    int2 et = none, ee = none, ew = none, eq = none, es = none;
+    if constexpr ((CLS & kClsTri) != 0)
+        if (active && lig < a.triW) et = a.triRL[row * a.triW + lig];
+    if constexpr ((CLS & kClsHinge) != 0) {
+        if (active && lig < a.hinEW) ee = a.hinERL[row * a.hinEW + lig];
+        if (active && lig < a.hinWW) ew = a.hinWRL[row * a.hinWW + lig];
+    }
+    if constexpr ((CLS & kClsTet) != 0)
+        if (active && lig < a.tetW) eq = a.tetRL[row * a.tetW + lig];
+    if constexpr ((CLS & kClsSpring) != 0)
+        if (active && lig < a.sprW) es = a.sprRL[row * a.sprW + lig];
+    int32_t ids[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const int32_t e = lig + k * G;
+        ids[k] = (active && e < a.nbW) ? a.nbRow[row * a.nbW + e] : -1;
+    }
+    const float4 tf = a.xt[vid];
+    const double mass = a.mass[vid];
+    const int32_t out = a.gidOf[vid];
+    int32_t ab = 0, ae = 0;
+    if (active && a.ancPtr) {
+        ab = a.ancPtr[vid];
+        ae = a.ancPtr[vid + 1];
+    }
+    // second round: the first entries' element parameters and hinge angles, and the neighbourhood's positions
+    double2 t01 = {}, t23 = {}, t45 = {}, t67 = {}, sp = {};
+    double ke = 0.0, kw = 0.0;
+    if constexpr ((CLS & kClsTri) != 0) {
+        if (et.x >= 0) {
+            const double2 *p = reinterpret_cast<const double2 *>(a.triP + 8 * static_cast<int64_t>(et.x));
+            t01 = p[0];
+            t23 = p[1];
+            t45 = p[2];
+            t67 = p[3];
+        }
+    }
+    if constexpr ((CLS & kClsHinge) != 0) {
+        if (ee.x >= 0) ke = a.hinKc[ee.x];
+        if (ew.x >= 0) kw = a.hinKc[ew.x];
+    }
+    if constexpr ((CLS & kClsSpring) != 0)
+        if (es.x >= 0) sp = a.sprP[es.x];
+    float px[R], py[R], pz[R];
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const float4 p = a.pos[ids[k] >= 0 ? ids[k] : vid];
+        px[k] = p.x;
+        py[k] = p.y;
+        pz[k] = p.z;
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) slab[lig + k * G] = make_float4(px[k], py[k], pz[k], 0.f);
+    __syncthreads();
+    auto at = [&](int32_t pk, int s) { return xyzd(slab[(pk >> (5 * s)) & 31]); };
+    V3d r = {0.0, 0.0, 0.0};
+    if (active) {
+        const V3d xo = at(0, 0);
+        if (lig == 0) {
+            const double mdt2 = mass * a.invdt2;
+            r = V3d{-mdt2 * (xo.x - double(tf.x)), -mdt2 * (xo.y - double(tf.y)), -mdt2 * (xo.z - double(tf.z))};
+        }
+        if constexpr ((CLS & kClsTri) != 0) {
+            // the slots' indices: the vertex is slot 0 / slot 2 when that index is 0
+            auto tri = [&](int32_t pk, double2 p01, double2 p23, double2 p45, double2 p67) {
+                const int32_t l0 = pk & 31, l1 = (pk >> 5) & 31, l2 = (pk >> 10) & 31;
+                const bool s0 = l0 == 0, s2 = l2 == 0;
+                const int32_t ia = s0 ? l1 : l0, ib = s2 ? l1 : l2;
+                r = r - triSlotGradD(p01, p23, p45, p67, s0, s2, xo, xyzd(slab[ia]), xyzd(slab[ib]));
+            };
+            if (et.x >= 0) tri(et.y, t01, t23, t45, t67);
+            for (int32_t j = lig + G; j < a.triW; j += G) {
+                const int2 e = a.triRL[row * a.triW + j];
+                if (e.x < 0) continue;
+                const double2 *p = reinterpret_cast<const double2 *>(a.triP + 8 * static_cast<int64_t>(e.x));
+                tri(e.y, p[0], p[1], p[2], p[3]);
+            }
+        }
+        if constexpr ((CLS & kClsHinge) != 0) {
+            auto edge = [&](int32_t pk, double kc) {
+                r = r - bendingEdgeGrad<double>(at(pk, 0), at(pk, 1), at(pk, 2), at(pk, 3), ((pk >> 5) & 31) == 0, kc);
+            };
+            if (ee.x >= 0) edge(ee.y, ke);
+            for (int32_t j = lig + G; j < a.hinEW; j += G) {
+                const int2 e = a.hinERL[row * a.hinEW + j];
+                if (e.x >= 0) edge(e.y, a.hinKc[e.x]);
+            }
+            auto wing = [&](int32_t pk, double kc) {
+                r = r - bendingWingGrad<double>(at(pk, 0), at(pk, 1), at(pk, 2), at(pk, 3), ((pk >> 15) & 31) == 0, kc);
+            };
+            if (ew.x >= 0) wing(ew.y, kw);
+            for (int32_t j = lig + G; j < a.hinWW; j += G) {
+                const int2 e = a.hinWRL[row * a.hinWW + j];
+                if (e.x >= 0) wing(e.y, a.hinKc[e.x]);
+            }
+        }
+        if constexpr ((CLS & kClsTet) != 0) {
+            auto tet = [&](int2 e) {
+                const int32_t pk = e.y;
+                const int32_t l0 = pk & 31, l1 = (pk >> 5) & 31, l2 = (pk >> 10) & 31, l3 = (pk >> 15) & 31;
+                const bool s0 = l0 == 0, s01 = s0 || l1 == 0, s012 = s01 || l2 == 0;
+                const int32_t i0 = s0 ? l1 : l0, i1 = s01 ? l2 : l1, i2 = s012 ? l3 : l2;
+                const double2 *p = reinterpret_cast<const double2 *>(a.tetP + 12 * static_cast<int64_t>(e.x));
+                r = r - tetSlotGradD(p, s0, s01, s012, xo, xyzd(slab[i0]), xyzd(slab[i1]), xyzd(slab[i2]));
+            };
+            if (eq.x >= 0) tet(eq);
+            for (int32_t j = lig + G; j < a.tetW; j += G) {
+                const int2 e = a.tetRL[row * a.tetW + j];
+                if (e.x >= 0) tet(e);
+            }
+        }
+        if constexpr ((CLS & kClsSpring) != 0) {
+            auto spring = [&](int32_t pk, double2 p) {
+                V3d ge;
+                S3d He;
+                springVertex<double>(xo, at(pk, 0), p.x, p.y, ge, He);
+                r = r - ge;
+            };
+            if (es.x >= 0) spring(es.y, sp);
+            for (int32_t j = lig + G; j < a.sprW; j += G) {
+                const int2 e = a.sprRL[row * a.sprW + j];
+                if (e.x >= 0) spring(e.y, a.sprP[e.x]);
+            }
+        }
+        for (int32_t k = ab + lig; k < ae; k += G) {
+            const double4 p = a.ancP[a.ancElem[k]];
+            V3d ge;
+            S3d He;
+            anchorVertex<double>(xo, V3d{p.x, p.y, p.z}, p.w, ge, He);
+            r = r - ge;
+        }
+    }
+    r.x = groupReduceD<G>(r.x, lane64, lig);
+    r.y = groupReduceD<G>(r.y, lane64, lig);
+    r.z = groupReduceD<G>(r.z, lane64, lig);
+    if (active && lig == 0) a.r2[out] = dot(r, r);
+}
+
 // The residual's first pass over the hinges: kC = k (theta - theta0) per hinge in FP64 (discreteBendingEnergy.cpp
 // :20-27 angle), so that the four slots of a hinge share one angle evaluation.
 __global__ void __launch_bounds__(256) hingeKcKernel(const float4 *__restrict__ pos, const int4 *__restrict__ hinV,
@@ -1233,10 +1418,21 @@ template <int G>
 void residualLaunchG(const ResidualArgs &a, hipStream_t stream)
 {
     const int blocks = blocksFor(a.n, 256 / 64 * (64 / G));
-    if (a.extPtr)
+    if (a.extPtr) {
         residualKernel<G, true><<<blocks, 256, 0, stream>>>(a);
-    else
+    } else if (a.nbW > 0 && a.nbW <= kNbMax) {
+        const size_t lds = nbSlabBytes(G, 256);
+        const int cls = (a.triW ? kClsTri : 0) | (a.hinEW || a.hinWW ? kClsHinge : 0) | (a.tetW ? kClsTet : 0) |
+                        (a.sprW ? kClsSpring : 0);
+        if ((cls & ~(kClsTri | kClsHinge)) == 0)
+            residualNbKernel<G, kClsTri | kClsHinge><<<blocks, 256, lds, stream>>>(a);
+        else if (cls == kClsTet)
+            residualNbKernel<G, kClsTet><<<blocks, 256, lds, stream>>>(a);
+        else
+            residualNbKernel<G, kClsAll><<<blocks, 256, lds, stream>>>(a);
+    } else {
         residualKernel<G, false><<<blocks, 256, 0, stream>>>(a);
+    }
 }
 
 hipError_t launchResidual(const ResidualArgs &a, hipStream_t stream)

@@ -168,9 +168,17 @@ def main():
     ctx.iterate(iters)
     r_x, r_xt = float(ctx.residual()[0]), float(ctx.residual_at_inertial()[0])
     x_iter, _ = ctx.state()
+    res_us, res_bytes = ctx.bench_residual(20)
     ctx.end_substep()
     step_residual = {"gpu_fp64_rho": r_x / r_xt, "gpu_fp64_residual": r_x, "gpu_fp64_residual_at_inertial": r_xt,
-                     "iterations": iters, "state": "one substep from the workload's initial state"}
+                     "iterations": iters, "state": "one substep from the workload's initial state",
+                     "residual_us": res_us, "residual_in_solve_launches": res_us / (mean_ms * 1e3),
+                     "roofline": {"bound": "hbm", "kernel": "hingeKcKernel + residualNbKernel + reduceSegmentsKernel + "
+                                                           "reduceObjectsKernel (one FP64 residual evaluation)",
+                                  "achieved": res_bytes / (res_us * 1e-6) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": res_bytes / (res_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+                                  "algorithmic_bytes": res_bytes, "timing": "HIP events over 20 back-to-back "
+                                                                             "evaluations on the context's stream"}}
     kernel = {0: "solveColorKernel", 1: "solveColorKernel (element rows staged in LDS)",
               2: "solveColorNbKernel"}[ctx.launch()["lds_staging"]]
 

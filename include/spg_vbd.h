@@ -125,7 +125,8 @@ typedef struct {
     int64_t vertex_updates;      /* vertex block updates performed by those launches */
     double algorithmic_bytes;    /* compulsory bytes of those launches: 12 B x every vertex of the objects
                                     with that colour + 32 B x updated vertices + per incidence 4 B plus the
-                                    element record (triangle 40, hinge 24, tet 64, spring 16, anchor 20 B) */
+                                    element record (triangle 40, hinge 24, tet 64, spring 16, anchor 20 B;
+                                    a further-energy record 16 B of slots and 4 B per FP32 coefficient read) */
     int64_t fp64_fallbacks;      /* block solves that took the FP64 near-singular path */
     int64_t skipped_blocks;      /* block solves skipped by det(H) <= 1e-10 */
     int64_t prediction_fallbacks; /* objects whose learned start (spg_vbd_learned.h) was dropped during
@@ -211,6 +212,15 @@ int spg_vbd_synchronize(spg_vbd_ctx *ctx);
    a lower-index neighbour). n_sets flat-stencil arrays of lengths[i] entries with sizes[i] per stencil. */
 int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *const *flat_stencils,
                            const int64_t *lengths, const int32_t *stencil_sizes, int32_t *colors);
+
+/* Time of one FP64 force residual evaluation at the current positions (all its kernels: the per-hinge angle
+   pass, the per-vertex residual and the two reduction stages), the mean over `repeats` evaluations queued back to
+   back on the context's stream, from HIP events, in *us; *bytes (may be NULL) receives its compulsory bytes: per
+   particle position, x~, FP64 mass and the squared residual written and read back (56 B), each element's table
+   entry once (spring 24, anchor 36, triangle 80, hinge 48 with its angle term written and read, tetrahedron
+   112, further element 16 + 8 per parameter) and 4 B per incidence. Needs an open substep; changes no state.
+   Measurement only, no counterpart in the reference. */
+int spg_vbd_bench_residual(spg_vbd_ctx *ctx, int32_t repeats, double *us, double *bytes);
 
 /* Plain float4 device-to-device copy of `bytes` (multiple of 16) repeated `repeats` times, for
    calibrating HBM traffic counters against a kernel of known traffic (bytes read + bytes written per

@@ -117,6 +117,7 @@ struct spg_vbd_ctx {
     std::vector<int32_t> colorBegin;    // slot range of each colour, size maxColors + 1
     std::vector<int64_t> colorActiveV;  // vertices of objects that have this colour (roofline model)
     std::vector<double> colorBytes;     // compulsory bytes of one sweep of this colour
+    double residualBytes = 0.0;         // compulsory bytes of one FP64 residual evaluation
     std::vector<int64_t> objSlotsOffsets;
 
     // device state
@@ -183,10 +184,8 @@ struct spg_vbd_ctx {
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int32_t> triRowR, hinERowR, hinWRowR, tetRowR, sprRowR;  // element ids in the solve's row widths
-    // the neighbourhood-staged residual's row entries (element id, slot indices into the neighbour row) and the
-    // particle index of each slot
+    // the neighbourhood-staged residual's row entries (element id, slot indices into the neighbour row)
     DevBuf<int2> triRL, hinERL, hinWRL, tetRL, sprRL;
-    DevBuf<int32_t> gidOfSlotD;
     DevBuf<int2> sprV;
     DevBuf<double2> sprP;
     DevBuf<double4> ancP;
@@ -461,7 +460,6 @@ struct spg_vbd_ctx {
         a.hinWRL = hinWRL.p;
         a.tetRL = tetRL.p;
         a.sprRL = sprRL.p;
-        a.gidOf = gidOfSlotD.p;
         a.hinKc = kcBuf;
         a.nHinges = static_cast<int32_t>(hinVe.n);
         hipCheck(launchResidual(a, s), "residual launch");
@@ -1242,7 +1240,6 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
             for (int64_t g = c.objOffset[o]; g < c.objOffset[o + 1]; ++g) objSlots[g] = c.slotOfGid[g];
         }
         c.objSlots.upload(objSlots, c.stream);
-        c.gidOfSlotD.upload(c.gidOfSlot, c.stream);
         c.objOffsets.upload(c.objSlotsOffsets, c.stream);
         std::vector<int64_t> seg{0};
         std::vector<int32_t> objSeg{0};
@@ -1296,6 +1293,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     std::vector<Incidence> extIncList;  // local = type << 2 | slot in the stencil
     std::vector<int4> extVh;
     std::vector<double> extPh;
+    double extResidualBytes = 0.0;
     std::vector<double> colorRecordBytes(c.maxColors, 0.0);
     std::vector<int64_t> colorIncidences(c.maxColors, 0);
     for (int32_t o = 0; o < nObjects; ++o) {
@@ -1339,6 +1337,7 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
                     tetP.insert(tetP.end(), p, p + 12);
                     break;
                 default:  // a further energy
+                    extResidualBytes += 16.0 + 8.0 * np;
                     elem = static_cast<int32_t>(extVh.size());
                     extVh.push_back(make_int4(sl[0], ss > 1 ? sl[1] : -1, ss > 2 ? sl[2] : -1, ss > 3 ? sl[3] : -1));
                     extPh.insert(extPh.end(), p, p + np);
@@ -1542,6 +1541,18 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
     if (extIncList.empty())
         buildNeighbourhoodLayout(c, sprRowsH, triOH, triMH, hinVH, hinPH, tetOH, resRows, triV, hinV, tetV, sprV);
 
+    // compulsory bytes of one FP64 residual: per particle its position, x~ and FP64 mass read and its squared
+    // residual written and read back by the reduction (56 B); every element's table entry read once (spring 24,
+    // anchor 36, triangle 80, hinge 32 plus its k (theta - theta0) written and read, 48, tetrahedron 112, further
+    // element 16 + 8 per parameter); 4 B per incidence for the element reference
+    {
+        double b = 56.0 * double(c.nv) + 24.0 * double(sprV.size()) + 36.0 * double(ancP.size()) +
+                   80.0 * double(triV.size()) + 48.0 * double(hinV.size()) + 112.0 * double(tetV.size());
+        int64_t incidences = 0;
+        for (int32_t k = 0; k < c.maxColors; ++k) incidences += colorIncidences[k];
+        b += 4.0 * double(incidences);
+        c.residualBytes = b + extResidualBytes;
+    }
     // compulsory bytes per colour sweep: 12 B per vertex of every object that has this colour (all
     // positions read once), 32 B per updated vertex (x write, x_tilde read, mass, row offset), and
     // per incidence a 4 B CSR entry plus the element record.
@@ -2137,6 +2148,30 @@ int spg_vbd_color_vertices(int64_t n_vertices, int32_t n_sets, const int32_t *co
         for (int32_t i = 0; i < n_sets; ++i) sets.push_back({flat_stencils[i], lengths[i], stencil_sizes[i]});
         const auto col = colorVertices(n_vertices, sets);
         std::copy(col.begin(), col.end(), colors);
+    });
+}
+
+int spg_vbd_bench_residual(spg_vbd_ctx *ctx, int32_t repeats, double *us, double *bytes)
+{
+    return guarded(ctx, [&] {
+        if (!ctx) throw VbdError(SPG_VBD_ERR_INVALID, "null context");
+        if (repeats < 1 || !us) throw VbdError(SPG_VBD_ERR_INVALID, "invalid arguments");
+        if (!ctx->inSubstep) throw VbdError(SPG_VBD_ERR_STATE, "residual needs an open substep (begin_substep)");
+        hipCheck(hipSetDevice(ctx->device), "hipSetDevice");
+        ctx->enqueueResidual(ctx->posIsInertial, ctx->norms.p);  // warm
+        hipEvent_t e0, e1;
+        hipCheck(hipEventCreate(&e0), "hipEventCreate");
+        hipCheck(hipEventCreate(&e1), "hipEventCreate");
+        hipCheck(hipEventRecord(e0, ctx->stream), "hipEventRecord");
+        for (int32_t r = 0; r < repeats; ++r) ctx->enqueueResidual(ctx->posIsInertial, ctx->norms.p);
+        hipCheck(hipEventRecord(e1, ctx->stream), "hipEventRecord");
+        hipCheck(hipEventSynchronize(e1), "hipEventSynchronize");
+        float t = 0.f;
+        hipCheck(hipEventElapsedTime(&t, e0, e1), "hipEventElapsedTime");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *us = 1e3 * t / repeats;
+        if (bytes) *bytes = ctx->residualBytes;
     });
 }
 

@@ -176,7 +176,7 @@ struct ResidualArgs {
     int32_t nHinges;
     // neighbourhood-staged variant (nbW > 0, no further energies): the solve's neighbour rows, and per class
     // row entries (element id, 5-bit indices into the vertex's neighbour row of the element's slots in stencil
-    // order; springs: of the other end) in the same widths (padding -1); the particle index of each vertex slot
+    // order; springs: of the other end) in the same widths (padding -1)
     int32_t nbW;
     const int32_t *nbRow;
     const int2 *triRL;
@@ -184,7 +184,6 @@ struct ResidualArgs {
     const int2 *hinWRL;
     const int2 *tetRL;
     const int2 *sprRL;
-    const int32_t *gidOf;
 };
 
 // kernel launchers (vbd_kernels.hip); all asynchronous on `stream`

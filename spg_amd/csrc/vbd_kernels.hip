@@ -1043,17 +1043,27 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
     if (active && lig == 0) a.r2[gi] = dot(r, r);
 }
 
-// Per-object |r| in two deterministic stages. Stage 1: block b sums the squared residuals of segment b
-// (at most kReduceSegment consecutive particles of one object) in a fixed tree order. Stage 2: one
-// thread per object sums its segments' partials in segment order and takes the square root.
+// Per-object |r| in two deterministic stages. Stage 1: block b sums the squared residuals of segment b (at most
+// kReduceSegment consecutive particles of one object): thread t adds entries t, t + 256, ... in that order, all its
+// loads issued at once, then the block's 256 sums are added in a fixed tree order. Stage 2: one wavefront per
+// object: lane l adds the object's segment partials l, l + 64, ... in that order, the lanes' sums are added in a
+// fixed shuffle tree, and lane 0 takes the square root.
 __global__ void __launch_bounds__(256) reduceSegmentsKernel(const double *__restrict__ r2,
                                                             const int64_t *__restrict__ segBegin,
                                                             double *__restrict__ partial)
 {
+    constexpr int K = kReduceSegment / 256;
     __shared__ double part[256];
     const int64_t b = segBegin[blockIdx.x], e = segBegin[blockIdx.x + 1];
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t j = b + threadIdx.x + 256 * k;
+        v[k] = j < e ? r2[j] : 0.0;
+    }
     double s = 0.0;
-    for (int64_t j = b + threadIdx.x; j < e; j += blockDim.x) s += r2[j];
+#pragma unroll
+    for (int k = 0; k < K; ++k) s += v[k];
     part[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -1062,16 +1072,18 @@ __global__ void __launch_bounds__(256) reduceSegmentsKernel(const double *__rest
     }
     if (threadIdx.x == 0) partial[blockIdx.x] = part[0];
 }
-
 __global__ void __launch_bounds__(256) reduceObjectsKernel(const double *__restrict__ partial,
                                                            const int32_t *__restrict__ objSeg, int32_t nObjects,
                                                            double *__restrict__ out)
 {
-    const int32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t o = blockIdx.x * 4 + (threadIdx.x >> 6);  // uniform over the wavefront
+    const int32_t lane = threadIdx.x & 63;
     if (o >= nObjects) return;
     double s = 0.0;
-    for (int32_t k = objSeg[o]; k < objSeg[o + 1]; ++k) s += partial[k];
-    out[o] = sqrt(s);
+    for (int32_t k = objSeg[o] + lane; k < objSeg[o + 1]; k += 64) s += partial[k];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) out[o] = sqrt(s);
 }
 
 // Per-element derivative hook: evaluates the same device functions the solve uses, for every
@@ -1242,8 +1254,9 @@ size_t solveStagedLdsBytes(const SolveArgs &a, int lanesPerVertex, int blockSize
 // with the vertex's group first gathering its own and its distinct neighbours' positions into an LDS slab (the
 // neighbour rows of the staged solve), so each position is read from memory once per vertex, and row entries
 // that carry the element id and the 5-bit slab indices of its slots in stencil order (the vertex itself is
-// index 0). Vertices run in slot order (the output index comes with the row), and the first entry of every class
-// and its element's parameters are loaded before the gather: a vertex waits for two rounds of memory loads.
+// index 0). Vertices run in particle order, as in residualKernel (neighbouring vertices share elements, so a
+// wavefront's element loads touch fewer cache lines than in colour-major slot order), and the first entry of every
+// class and its element's parameters are loaded before the gather.
 template <int G, int CLS>
 __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
 {
@@ -1255,9 +1268,9 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
     const int32_t wib = threadIdx.x >> 6;
     const int32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
     const int32_t grp = lane64 / G, lig = lane64 - grp * G;
-    const int32_t vs = wave * GPW + grp;
-    const bool active = grp < GPW && vs < a.n;
-    const int32_t vid = active ? vs : 0;
+    const int32_t gi = wave * GPW + grp;
+    const bool active = grp < GPW && gi < a.n;
+    const int32_t vid = active ? a.order[gi] : 0;
     const int64_t row = vid;
     float4 *slab = nbSlab + static_cast<int64_t>(wib * SLABS + grp) * (R * G);
     // first round: row entries, neighbour ids, the vertex's own data
@@ -1281,7 +1294,6 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
     }
     const float4 tf = a.xt[vid];
     const double mass = a.mass[vid];
-    const int32_t out = a.gidOf[vid];
     int32_t ab = 0, ae = 0;
     if (active && a.ancPtr) {
         ab = a.ancPtr[vid];
@@ -1397,7 +1409,7 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
     r.x = groupReduceD<G>(r.x, lane64, lig);
     r.y = groupReduceD<G>(r.y, lane64, lig);
     r.z = groupReduceD<G>(r.z, lane64, lig);
-    if (active && lig == 0) a.r2[out] = dot(r, r);
+    if (active && lig == 0) a.r2[gi] = dot(r, r);
 }
 
 // The residual's first pass over the hinges: kC = k (theta - theta0) per hinge in FP64 (discreteBendingEnergy.cpp
@@ -1459,7 +1471,7 @@ hipError_t launchReduceObjects(const double *r2, const int64_t *segBegin, int32_
 {
     if (nObjects <= 0 || nSegments <= 0) return hipSuccess;
     reduceSegmentsKernel<<<nSegments, 256, 0, stream>>>(r2, segBegin, partial);
-    reduceObjectsKernel<<<blocksFor(nObjects, 256), 256, 0, stream>>>(partial, objSeg, nObjects, out);
+    reduceObjectsKernel<<<blocksFor(nObjects, 4), 256, 0, stream>>>(partial, objSeg, nObjects, out);
     return hipGetLastError();
 }
 

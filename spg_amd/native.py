@@ -187,6 +187,7 @@ def vbd_lib():
                [i64, i32, ctypes.POINTER(_ip), _lp, _ip, _ip])
         _proto(L, "spg_vbd_element_derivatives", ctypes.c_int, [i32, i64, _dp, _dp, i32, i32, i32, _dp, _dp])
         _proto(L, "spg_vbd_bench_copy", ctypes.c_int, [i64, i32, i32, _dp])
+        _proto(L, "spg_vbd_bench_residual", ctypes.c_int, [c, i32, _dp, _dp])
         # learned warm start (include/spg_vbd_learned.h)
         u64 = ctypes.c_uint64
         _proto(L, "spg_mlp_create", ctypes.c_int, [ctypes.POINTER(c), i32, u64])
@@ -720,6 +721,12 @@ class VbdContext:
         out = np.zeros(self.n_objects)
         self._check(self._L.spg_vbd_residual_at_inertial(self.h, _d(out)))
         return out
+
+    def bench_residual(self, repeats=20):
+        """spg_vbd_bench_residual: (mean us of one FP64 residual evaluation, its compulsory bytes)."""
+        us, nbytes = ctypes.c_double(0), ctypes.c_double(0)
+        self._check(self._L.spg_vbd_bench_residual(self.h, int(repeats), ctypes.byref(us), ctypes.byref(nbytes)))
+        return us.value, nbytes.value
 
     def iterate_to_tolerance(self, rel_tol, max_iterations=1000, check_every=1):
         it = ctypes.c_int32(0)

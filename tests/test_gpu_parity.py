@@ -227,6 +227,28 @@ def test_residual_matches_oracle(oracle_threads):
     assert abs(r0 - ro0) <= 1e-9 * ro0
 
 
+def test_bench_residual_time_bytes_and_no_state_change():
+    """spg_vbd_bench_residual: a positive time, the compulsory byte count from the scene's element counts, and the
+    state and residual unchanged by the timed evaluations."""
+    s = scenes.cloth_config1()
+    ctx = gpu_context(s)
+    ctx.begin_substep(s.dt, s.gravity)
+    ctx.iterate(3)
+    r = ctx.residual()[0]
+    x, _ = ctx.state()
+    us, nbytes = ctx.bench_residual(5)
+    per_element = {"stvk": 80, "bending": 48, "spring": 24, "anchor": 36, "snh": 112}
+    expect = 56.0 * s.n
+    for kind, st, _ in s.energies:
+        expect += per_element[kind] * len(st) + 4.0 * st.size
+    x2, _ = ctx.state()
+    assert us > 0.0
+    assert nbytes == expect
+    assert np.array_equal(x, x2)
+    assert ctx.residual_noalloc()[0] == r
+    ctx.end_substep()
+
+
 def test_graph_replay_bitwise():
     s = scenes.cloth_config1()
     a, b = gpu_context(s), gpu_context(s, use_graph=1)

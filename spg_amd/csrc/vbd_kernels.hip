@@ -952,7 +952,8 @@ __device__ __forceinline__ V3d tetSlotGradD(const double2 *p, bool s0, bool s01,
 // exact prepared parameters (implicitEulerBase.cpp:102-143 force, x-form of
 // implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per particle (particle order).
 // A group of G lanes per vertex, vertices in particle order (neighbours and the elements they share are
-// processed close together in time). Lane l takes entries l, l + G, ... of the vertex's fixed-width row of
+// processed close together in time, and with XCD-contiguous block numbering on one XCD, whose L2 then serves
+// the elements and positions they share). Lane l takes entries l, l + G, ... of the vertex's fixed-width row of
 // element ids in each class, reads the element's slots and exact parameters from its table and the stencil's
 // positions, and the group sums its partial forces in a fixed shuffle order. The vertex's slot in an element
 // is found by comparing ids, and the other slots and their shape vectors are picked by selects, so nothing is
@@ -962,7 +963,7 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
 {
     constexpr int GPW = 64 / G;
     const int32_t lane64 = threadIdx.x & 63;
-    const int32_t wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int32_t wave = xcdContiguousBlock(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const int32_t grp = lane64 / G, lig = lane64 - grp * G;
     const int32_t gi = wave * GPW + grp;
     const bool active = grp < GPW && gi < a.n;
@@ -1266,7 +1267,7 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
     extern __shared__ float4 nbSlab[];
     const int32_t lane64 = threadIdx.x & 63;
     const int32_t wib = threadIdx.x >> 6;
-    const int32_t wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const int32_t wave = xcdContiguousBlock(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wib;
     const int32_t grp = lane64 / G, lig = lane64 - grp * G;
     const int32_t gi = wave * GPW + grp;
     const bool active = grp < GPW && gi < a.n;
@@ -1413,12 +1414,13 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
 }
 
 // The residual's first pass over the hinges: kC = k (theta - theta0) per hinge in FP64 (discreteBendingEnergy.cpp
-// :20-27 angle), so that the four slots of a hinge share one angle evaluation.
+// :20-27 angle), so that the four slots of a hinge share one angle evaluation. Blocks are numbered XCD-contiguous
+// (xcdContiguousBlock), so consecutive hinges, which share particles, gather them through one XCD's L2.
 __global__ void __launch_bounds__(256) hingeKcKernel(const float4 *__restrict__ pos, const int4 *__restrict__ hinV,
                                                     const double2 *__restrict__ hinP, int32_t n,
                                                     double *__restrict__ kc)
 {
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int32_t i = xcdContiguousBlock(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int4 v = hinV[i];
     const double2 p = hinP[i];

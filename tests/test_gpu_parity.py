@@ -304,9 +304,18 @@ def test_lds_staged_variants_bitwise_equal_direct(lanes, block):
     values in the same order as the direct solve: bitwise the same steps (neighbourhood staging at one lane per
     vertex: the same to FP32 rounding), on cloth (triangles + hinges + anchors), a
     tetrahedral cube (tets + anchors), a sparse spring net (springs + anchor, every class compiled in) and a dense
-    one (more than 15 neighbours per particle: neighbourhood staging is refused with UNSUPPORTED)."""
+    one (more than 15 neighbours per particle: neighbourhood staging is refused with UNSUPPORTED), and element-row
+    staging with the further energies' classes (Baraff-Witkin membrane and bending with continuum edge springs,
+    Choi membrane with quadratic bending and squared continuum springs, StVK and stable squared Neo-Hookean
+    tetrahedra; neighbourhood staging is not built for them)."""
+    further = dict(young=1e4, pin="top_row", iterations=6, edge_k=5.0)
     cases = [scenes.cloth(40, young=1e4, pin="top_row", iterations=6), scenes.cube(5, iterations=6),
-             _spring_net(60, 0.35, 0), _spring_net(60, 0.5, 0)]
+             _spring_net(60, 0.35, 0), _spring_net(60, 0.5, 0),
+             scenes.cloth(40, membrane="membrane_bw", bending_kind="bending_bw", edge_springs="spring_continuum",
+                          **further),
+             scenes.cloth(40, membrane="membrane_choi", bending_kind="bending_quadratic",
+                          edge_springs="spring_squared_continuum", **further),
+             scenes.cube(4, iterations=6, material="stvk_tet"), scenes.cube(4, iterations=6, material="ssnh")]
     offered = []
     for s in cases:
         out, modes = [], []
@@ -321,6 +330,7 @@ def test_lds_staged_variants_bitwise_equal_direct(lanes, block):
                 ctx.step(s.dt, 1, s.iterations, s.gravity)
             out.append(ctx.state())
             modes.append(staged)
+            assert np.isfinite(out[-1][0]).all()
         offered.append(2 in modes)
         for m, o in zip(modes[1:], out[1:]):
             if lanes == 1 and m == 2:
@@ -330,9 +340,9 @@ def test_lds_staged_variants_bitwise_equal_direct(lanes, block):
                 continue
             assert np.array_equal(out[0][0], o[0]) and np.array_equal(out[0][1], o[1]), \
                 (s.meta.get("kind"), m, float(np.max(np.abs(out[0][0] - o[0]))))
-    # neighbourhood staging exists for the cloth, the cube and the sparse net (at most 13 neighbours) and is
-    # refused for the dense net (up to 26)
-    assert offered == [True, True, True, False]
+    # neighbourhood staging exists for the five-energy cloth, the cube and the sparse net (at most 13 neighbours)
+    # and is refused for the dense net (up to 26) and with further energies
+    assert offered == [True, True, True, False, False, False, False, False]
 
 
 def test_neighbourhood_staged_512_cloth_bitwise():

@@ -203,6 +203,50 @@ SPG_HD void snhVertex(V3<R> xo, V3<R> xa, V3<R> xb, V3<R> xc, V3<R> ga, V3<R> gb
 //   slot 0: ra = x2 - x1, rb = x3 - x1, sg = -1     slot 1: ra = x2 - x0, rb = x3 - x0, sg = +1
 //   slot 2: ra = e, rb = 0, sg = -1                 slot 3: ra = 0, rb = e, sg = -1
 // ---------------------------------------------------------------------------------------------
+// The hinge angle's atan2 and the hinge edge's length and inverse length, by scalar type. FP64 takes the
+// library functions. FP32 (the vertex solve) takes:
+//   * atan2 by octant reduction to t = min(|x|, |y|) / max(|x|, |y|) in [0, 1], then atan t = t P(t^2) with P of
+//     degree 8 (coefficients fitted for this code by weighted least squares on Chebyshev nodes of [0, 1]; largest
+//     error 1.1e-7 in FP32 Horner evaluation), then the octant's reflections; the arguments are finite (a
+//     degenerate (0, 0) gives 0), so none of the library's special cases are needed;
+//   * the length as |e|^2 times the reciprocal square root (one transcendental instead of a square root and a
+//     reciprocal).
+SPG_HD float hingeAtan2(float y, float x)
+{
+    const float ax = fabsf(x), ay = fabsf(y);
+    const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+    const float t = mx > 0.f ? mn * (1.f / mx) : 0.f;
+    const float s = t * t;
+    float p = 0.0028429856f;
+    p = p * s - 0.016038457f;
+    p = p * s + 0.042637214f;
+    p = p * s - 0.07499113f;
+    p = p * s + 0.10638127f;
+    p = p * s - 0.14202788f;
+    p = p * s + 0.19992484f;
+    p = p * s - 0.33333063f;
+    p = p * s + 1.0f;
+    float r = t * p;
+    r = ay > ax ? 1.57079632679489662f - r : r;
+    r = x < 0.f ? 3.14159265358979324f - r : r;
+    return copysignf(r, y);
+}
+SPG_HD double hingeAtan2(double y, double x) { return atan2(y, x); }
+SPG_HD void lengthAndInverse(float ee, float &L, float &iL)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    iL = rsqrtf(ee);
+#else
+    iL = 1.f / std::sqrt(ee);
+#endif
+    L = ee * iL;
+}
+SPG_HD void lengthAndInverse(double ee, double &L, double &iL)
+{
+    L = sqrt(ee);
+    iL = 1.0 / L;
+}
+
 // Hinge slots 0 and 1 (the hinge edge's vertices): ra, rb of the compact form. Both slots share
 // ra_perp = (x2 - x1)_perp and rb_perp = (x3 - x1)_perp since e_perp = 0.
 template <class R>
@@ -211,8 +255,8 @@ SPG_HD void bendingEdge(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot1, R th
     const V3<R> e = x1 - x0;
     const V3<R> p = x2 - x1, q = x3 - x1;
     const V3<R> a = cross(e, p), b = cross(e, q);
-    const R L = sqrt(dot(e, e));
-    const R iL = R(1) / L;
+    R L, iL;
+    lengthAndInverse(dot(e, e), L, iL);
     const V3<R> eh = iL * e;
     const R pe = dot(p, eh), qe = dot(q, eh);
     const V3<R> pp = p - pe * eh, qp = q - qe * eh;  // perpendicular parts
@@ -223,7 +267,7 @@ SPG_HD void bendingEdge(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot1, R th
     const V3<R> gth = (sg * rae * ia2) * a + (-sg * rbe * ib2) * b;
     const V3<R> ua = (iL * ia2) * pp + (R(2) * rae * ia2 * ia2) * cross(a, ra);
     const V3<R> ub = (-iL * ib2) * qp + (R(-2) * rbe * ib2 * ib2) * cross(b, rb);
-    const R theta = atan2(-dot(cross(a, b), eh), -dot(a, b));
+    const R theta = hingeAtan2(-dot(cross(a, b), eh), -dot(a, b));
     const R kC = k * (theta - theta0);
     g = kC * gth;
     H = zeroSym<R>();
@@ -240,9 +284,9 @@ SPG_HD void bendingWing(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot3, R th
 {
     const V3<R> e = x1 - x0;
     const V3<R> a = cross(e, x2 - x1), b = cross(e, x3 - x1);
-    const R L = sqrt(dot(e, e));
-    const R iL = R(1) / L;
-    const R theta = atan2(-dot(cross(a, b), e) * iL, -dot(a, b));
+    R L, iL;
+    lengthAndInverse(dot(e, e), L, iL);
+    const R theta = hingeAtan2(-dot(cross(a, b), e) * iL, -dot(a, b));
     const V3<R> w = slot3 ? b : a;
     const R sw = slot3 ? R(1) : R(-1);
     const R iw2 = R(1) / dot(w, w);

@@ -270,7 +270,7 @@ __device__ __forceinline__ void bendingBwRec(const V3f *x, float theta0, float k
     const float ir2 = 1.f / (X * X + Y * Y);
     const V3f gt = ir2 * (X * gY - Y * gX);  // grad theta
     const V3f u = X * gX + Y * gY;
-    const float kd = k * (atan2f(Y, X) - theta0);
+    const float kd = k * (hingeAtan2(Y, X) - theta0);
     g = kd * gt;
     // k grad theta grad theta^T + k (theta - theta0) [(X HY - Y HX) / r2 - sym(grad theta, u) / r2]
     const float a = kd * ir2;

@@ -124,8 +124,8 @@ __device__ __forceinline__ int blockSolve(const S3f &H, const V3f &f, V3f &dx)
     return kSkipped;
 }
 
-template <int G>
-__device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t lig);
+template <int G, class T>
+__device__ __forceinline__ T groupSum(T v, int32_t lane64);
 
 // The end of a vertex's update, shared by the solve kernels: the group's partial forces and Hessians are
 // summed in a fixed shuffle order, the group's first lane adds the inertia term, solves the 3x3 block and
@@ -135,15 +135,15 @@ __device__ __forceinline__ void finishVertex(const SolveArgs &a, bool active, in
                                              int32_t lig, const V3f &xo, const float4 &xt, V3f g, S3f H)
 {
     if (G > 1) {
-        g.x = groupReduce<G>(g.x, lane64, lig);
-        g.y = groupReduce<G>(g.y, lane64, lig);
-        g.z = groupReduce<G>(g.z, lane64, lig);
-        H.xx = groupReduce<G>(H.xx, lane64, lig);
-        H.xy = groupReduce<G>(H.xy, lane64, lig);
-        H.xz = groupReduce<G>(H.xz, lane64, lig);
-        H.yy = groupReduce<G>(H.yy, lane64, lig);
-        H.yz = groupReduce<G>(H.yz, lane64, lig);
-        H.zz = groupReduce<G>(H.zz, lane64, lig);
+        g.x = groupSum<G>(g.x, lane64);
+        g.y = groupSum<G>(g.y, lane64);
+        g.z = groupSum<G>(g.z, lane64);
+        H.xx = groupSum<G>(H.xx, lane64);
+        H.xy = groupSum<G>(H.xy, lane64);
+        H.xz = groupSum<G>(H.xz, lane64);
+        H.yy = groupSum<G>(H.yy, lane64);
+        H.yz = groupSum<G>(H.yz, lane64);
+        H.zz = groupSum<G>(H.zz, lane64);
     }
     int path = -1;
     if (active && lig == 0) {
@@ -177,13 +177,24 @@ __device__ __forceinline__ void finishVertex(const SolveArgs &a, bool active, in
 // Blocks are renumbered so that each XCD works on one contiguous range of the colour
 // (xcdContiguousBlock), keeping neighbouring positions in that XCD's L2.
 
-template <int G>
-__device__ __forceinline__ float groupReduce(float v, int32_t lane64, int32_t lig)
+// The sum of v over the G lanes of a group, valid in the group's first lane (the other lanes get partial sums
+// that reach into the next group; nobody reads them). acc_k at a lane is the sum of the 2^k lanes from it, and
+// step `off` adds to the first lane the sum of its lanes off .. min(2 off, G) - 1, which is acc_j at lane
+// off with 2^j = min(off, G - off) (a power of two for every G here: 1, 2, 3, 4, 6, 8, 16). These are the
+// additions, in the same order, of a butterfly whose steps are masked to the group's lanes, without the masks.
+template <int G, class T>
+__device__ __forceinline__ T groupSum(T v, int32_t lane64)
 {
+    static_assert(G == 1 || G == 2 || G == 3 || G == 4 || G == 6 || G == 8 || G == 16, "lanes per vertex");
+    T acc[5];
+    acc[0] = v;
+    int k = 0;
 #pragma unroll
     for (int off = 1; off < G; off <<= 1) {
-        const float o = __shfl(v, lane64 + off, 64);
-        if (lig + off < G) v += o;
+        const int len = off < G - off ? off : G - off;
+        const int j = len == 1 ? 0 : len == 2 ? 1 : len == 4 ? 2 : 3;
+        v = v + __shfl(acc[j], lane64 + off, 64);
+        acc[++k] = v;
     }
     return v;
 }
@@ -906,17 +917,6 @@ using S3d = Sym3<double>;
 
 __device__ __forceinline__ V3d xyzd(float4 p) { return {double(p.x), double(p.y), double(p.z)}; }
 
-template <int G>
-__device__ __forceinline__ double groupReduceD(double v, int32_t lane64, int32_t lig)
-{
-#pragma unroll
-    for (int off = 1; off < G; off <<= 1) {
-        const double o = __shfl(v, lane64 + off, 64);
-        if (lig + off < G) v += o;
-    }
-    return v;
-}
-
 // The FP64 StVK gradient at a triangle slot: p01 .. p67 the element's prepared parameters (rows of Dinv
 // g1 = (p0, p1), g2 = (p2, p3), g0 = -(g1 + g2); kd, ko, ks), s0 / s2 whether the vertex is slot 0 / slot 2, xa, xb
 // the other two slots' positions in stencil order.
@@ -1038,9 +1038,9 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
             }
         }
     }
-    r.x = groupReduceD<G>(r.x, lane64, lig);
-    r.y = groupReduceD<G>(r.y, lane64, lig);
-    r.z = groupReduceD<G>(r.z, lane64, lig);
+    r.x = groupSum<G>(r.x, lane64);
+    r.y = groupSum<G>(r.y, lane64);
+    r.z = groupSum<G>(r.z, lane64);
     if (active && lig == 0) a.r2[gi] = dot(r, r);
 }
 
@@ -1407,9 +1407,9 @@ __global__ void __launch_bounds__(256) residualNbKernel(ResidualArgs a)
             r = r - ge;
         }
     }
-    r.x = groupReduceD<G>(r.x, lane64, lig);
-    r.y = groupReduceD<G>(r.y, lane64, lig);
-    r.z = groupReduceD<G>(r.z, lane64, lig);
+    r.x = groupSum<G>(r.x, lane64);
+    r.y = groupSum<G>(r.y, lane64);
+    r.z = groupSum<G>(r.z, lane64);
     if (active && lig == 0) a.r2[gi] = dot(r, r);
 }
 

@@ -390,6 +390,25 @@ def test_batched_objects_match_independent_oracles(oracle_threads):
         assert e < 1e-5, (k, e)
 
 
+def test_batched_residual_per_object_matches_oracle(oracle_threads):
+    """Per-object FP64 residual norms of a batch (uneven cloths and a tetrahedral cube, so the staged residual's
+    all-class kernel runs over workgroups that span objects) against each object's oracle residual."""
+    batch = scenes.cloth_batch(count=4, seed=5, lo=8, hi=20, iterations=15) + [scenes.cube(3, iterations=15)]
+    ctx = gpu_context(batch)
+    ctx.begin_substep(scenes.DT, scenes.GRAVITY)
+    xt = [ctx.state(k)[0] for k in range(len(batch))]
+    ctx.iterate(3)
+    r, r0 = ctx.residual(), ctx.residual_at_inertial()
+    for k, sc in enumerate(batch):
+        o = ol.from_scene(sc)
+        ro = o.residual(ctx.state(k)[0], xt[k], scenes.DT)
+        ro0 = o.residual(xt[k], xt[k], scenes.DT)
+        print(f"object {k}: gpu {r[k]:.10e} oracle {ro:.10e}")
+        assert abs(r[k] - ro) <= 1e-9 * ro, k
+        assert abs(r0[k] - ro0) <= 1e-9 * ro0, k
+    ctx.end_substep()
+
+
 def test_nonfinite_state_rejected():
     s = scenes.cloth_config1()
     ctx = gpu_context(s)

@@ -648,7 +648,8 @@ def learned_comparison(scene, args, launch):
             "iteration_ratio_vs_history": sm["iteration_ratio_vs_history"],
             "learned_on_history": sm["learned_on_history"],
             "accepted": sm["accepted"], "fell_back": sm["fell_back"], "all_converged": sm["all_converged"],
-            "deployment": {"what": "three contexts from the initial state, each continuing from its own 1e-4 solves "
+            "deployment": {"what": "one context per path (plain, history, learned, learned on history) from the "
+                                   "initial state, each continuing from its own 1e-4 solves "
                                    f"for {args.learned_steps} steps (checks after every sweep)", **dsm,
                            "per_step": [clean(r) for r in drows]},
             "per_step": [clean(r) for r in rows],

@@ -175,12 +175,12 @@ struct spg_vbd_ctx {
     // further energies: CSR rows per slot, elements (slots, padded FP64 parameters) for the residual, and the
     // solve's per-class fixed-width record rows
     DevBuf<int32_t> extPtr;
-    DevBuf<int2> extInc;
     DevBuf<int4> extV;
     DevBuf<double> extP;
     int32_t xrW[kXCls] = {};
     DevBuf<int4> xrV[kXCls];
     DevBuf<float4> xrP[kXCls];
+    DevBuf<int2> xrR[kXCls];  // the residual's rows in the same layout: (element, slot in the element)
     // FP64 tables for the residual
     DevBuf<int32_t> sprElem, ancElem, triElem, hinEElem, hinWElem, tetElem;
     DevBuf<int32_t> triRowR, hinERowR, hinWRowR, tetRowR, sprRowR;  // element ids in the solve's row widths
@@ -288,7 +288,6 @@ struct spg_vbd_ctx {
         a.tetL = tetL.p;
         a.sprL = sprL.p;
         a.extPtr = extPtr.p;
-        a.extInc = extInc.p;
         a.extV = extV.p;
         a.extP = extP.p;
         for (int k = 0; k < kXCls; ++k) {
@@ -438,9 +437,12 @@ struct spg_vbd_ctx {
         a.tetV = tetV.p;
         a.tetP = tetP.p;
         a.extPtr = extPtr.p;
-        a.extInc = extInc.p;
         a.extV = extV.p;
         a.extP = extP.p;
+        for (int k = 0; k < kXCls; ++k) {
+            a.xrW[k] = xrW[k];
+            a.xrR[k] = xrR[k].p;
+        }
         a.order = objSlots.p;
         a.r2 = r2Buf;
         a.triW = triW;
@@ -1130,6 +1132,7 @@ void buildFurtherRecords(spg_vbd_ctx &c, const std::vector<Incidence> &extIncLis
     for (int k = 0; k < kXCls; ++k) {
         c.xrW[k] = 0;
         c.xrV[k].release();
+        c.xrR[k].release();
         c.xrP[k].release();
         if (cls[k].empty()) continue;
         std::vector<int32_t> ptr;
@@ -1138,6 +1141,7 @@ void buildFurtherRecords(spg_vbd_ctx &c, const std::vector<Incidence> &extIncLis
         const int np = xclsPlanes(k);
         const size_t plane = static_cast<size_t>(c.nv) * w;
         std::vector<int4> rv(plane, make_int4(-1, -1, -1, -1));
+        std::vector<int2> rr(plane, make_int2(-1, -1));
         std::vector<float4> rp(plane * np, make_float4(0.f, 0.f, 0.f, 0.f));
         for (int64_t v = 0; v < c.nv; ++v)
             for (int32_t i = ptr[v]; i < ptr[v + 1]; ++i) {
@@ -1146,10 +1150,12 @@ void buildFurtherRecords(spg_vbd_ctx &c, const std::vector<Incidence> &extIncLis
                 float4 q[4];
                 furtherCoefficients(k, in.local & 3, &extPh[static_cast<size_t>(kExtParams) * in.elem], q);
                 rv[r] = extVh[in.elem];
+                rr[r] = make_int2(in.elem, in.local & 3);
                 for (int j = 0; j < np; ++j) rp[j * plane + r] = q[j];
             }
         c.xrW[k] = w;
         c.xrV[k].upload(rv, c.stream);
+        c.xrR[k].upload(rr, c.stream);
         c.xrP[k].upload(rp, c.stream);
         // the host vectors are released at the end of this iteration; the uploads must have read them
         hipCheck(hipStreamSynchronize(c.stream), "hipStreamSynchronize");
@@ -1522,18 +1528,15 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         c.tetP.upload(tetP, c.stream);
     }
 
-    // further energies: one CSR incidence class (energy order, ascending stencil within each slot)
+    // further energies: the per-slot incidence offsets, the element tables and the per-class rows of the solve and
+    // the residual
     if (!extIncList.empty()) {
         checkSize(extIncList.size());
         if (extVh.size() >= (size_t(1) << 31) - 1) throw VbdError(SPG_VBD_ERR_UNSUPPORTED, "too many elements");
         std::vector<int32_t> ptr;
-        const auto order = sortBySlot(extIncList, c.nv, ptr);
-        std::vector<int2> rec(order.size());
-        for (size_t k = 0; k < order.size(); ++k)
-            rec[k] = make_int2(extIncList[order[k]].elem, extIncList[order[k]].local);
+        (void)sortBySlot(extIncList, c.nv, ptr);
         buildFurtherRecords(c, extIncList, extVh, extPh);
         c.extPtr.upload(ptr, c.stream);
-        c.extInc.upload(rec, c.stream);
         c.extV.upload(extVh, c.stream);
         c.extP.upload(extPh, c.stream);
     }

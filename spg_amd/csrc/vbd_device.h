@@ -65,6 +65,7 @@ __host__ __device__ constexpr int xclsPlanes(int c)
     return (c == 0 || c == 4 || c == 5) ? 1 : (c == 1 || c == 3) ? 2 : c == 2 ? 3 : 4;
 }
 __host__ __device__ constexpr int xclsStencil(int c) { return c == 0 ? 2 : c <= 2 ? 3 : 4; }
+__host__ __device__ constexpr int xclsType(int c) { return c <= 4 ? c + 7 : c + 6; }  // the energy type of class c
 
 struct SolveArgs {
     float4 *pos;
@@ -94,12 +95,11 @@ struct SolveArgs {
     const float4 *tetGa;
     const float4 *tetGb;
     const float4 *tetGc;
-    // the further energies (types 7 ... 13; continuum springs join the spring class): CSR rows per slot of
-    // (element, type << 2 | slot in the stencil) over element tables (up to four slots, unused -1, and kExtParams
-    // FP64 parameters), read by the FP64 residual; extPtr null: none (the solve kernel's instantiation without
-    // these classes runs)
+    // the further energies (types 7 ... 13; continuum springs join the spring class): per-slot incidence counts
+    // as CSR offsets (extPtr; null: no further energies, and the kernels' instantiations without these classes
+    // run), and element tables (up to four slots, unused -1, and kExtParams FP64 parameters) read by the FP64
+    // residual
     const int32_t *extPtr;
-    const int2 *extInc;
     const int4 *extV;
     const double *extP;
     // the solve's records of the further energies, one fixed-width row per vertex slot and incidence class
@@ -160,9 +160,12 @@ struct ResidualArgs {
     const int4 *tetV;
     const double *tetP;   // 12 per element: Dinv (9), lambda, mu, alpha
     const int32_t *extPtr;  // the further energies, as in SolveArgs (null: none)
-    const int2 *extInc;
     const int4 *extV;
     const double *extP;
+    // their rows in the solve's class layout: per class, vertex slot and entry (element id, the vertex's slot in
+    // the element); -1 padding
+    int32_t xrW[kXCls];
+    const int2 *xrR[kXCls];
     const int32_t *order; // slot of the i-th particle (objects in order, particles ascending)
     double *r2;           // out: |r_v|^2 of the i-th particle (particle order)
     // fixed-width rows of element ids per vertex slot and class, in the solve's widths (-1: padding)

@@ -249,180 +249,208 @@ __host__ __device__ inline size_t solveNbLdsBytes(const SolveArgs &, int G, int 
 // symmetric sum the skew parts cancel and the wing slots (x2, x3), on which X and Y are linear, get none.
 // WING false: slot 0 (second false) or 1 (second true); WING true: slot 2 or 3. Slot 1's terms are slot 0's with
 // the opposite sign plus the terms of the edges e3, e4 that x1 also moves (weight m = 1, 0 for slot 0).
-template <bool WING>
-__device__ __forceinline__ void bendingBwRec(const V3f *x, float theta0, float k, bool second, V3f &g, S3f &H)
+// T: float (the solve, with the block) or double (the residual, HESS false: the gradient only).
+template <bool WING, class T = float, bool HESS = true>
+__device__ __forceinline__ void bendingBwRec(const V3<T> *x, T theta0, T k, bool second, V3<T> &g, Sym3<T> &H)
 {
-    const V3f e0 = x[1] - x[0], e3 = x[2] - x[1], e4 = x[3] - x[1];
-    const float q = dot(e0, e0), s0 = 1.f / sqrtf(q);
-    const float d34 = dot(e3, e4), d03 = dot(e0, e3), d04 = dot(e0, e4);
-    const V3f g0 = cross(e3, e4), g3 = cross(e4, e0), g4 = cross(e0, e3);
-    const float D = dot(e0, g0);
-    const float X = -(q * d34 - d04 * d03), Y = -s0 * q * D;
-    V3f gX, gY;
-    S3f HX = zeroSym<float>(), HY = zeroSym<float>();
+    const V3<T> e0 = x[1] - x[0], e3 = x[2] - x[1], e4 = x[3] - x[1];
+    const T q = dot(e0, e0), s0 = T(1) / sqrt(q);
+    const T d34 = dot(e3, e4), d03 = dot(e0, e3), d04 = dot(e0, e4);
+    const V3<T> g0 = cross(e3, e4), g3 = cross(e4, e0), g4 = cross(e0, e3);
+    const T D = dot(e0, g0);
+    const T X = -(q * d34 - d04 * d03), Y = -s0 * q * D;
+    V3<T> gX, gY;
+    Sym3<T> HX = zeroSym<T>(), HY = zeroSym<T>();
     if constexpr (!WING) {
-        const float m = second ? 1.f : 0.f, sg = second ? 1.f : -1.f;
-        const V3f X0 = (-2.f * d34) * e0 + d03 * e4 + d04 * e3;
-        const V3f Y0 = (-s0) * ((2.f * D) * e0 + q * g0);
-        const V3f g34 = g3 + g4;
+        const T m = second ? T(1) : T(0), sg = second ? T(1) : T(-1);
+        const V3<T> X0 = (T(-2) * d34) * e0 + d03 * e4 + d04 * e3;
+        const V3<T> Y0 = (-s0) * ((T(2) * D) * e0 + q * g0);
+        const V3<T> g34 = g3 + g4;
         gX = sg * X0 - m * ((d04 * e0 - q * e4) + (d03 * e0 - q * e3));
         gY = sg * Y0 + (m * s0 * q) * g34;
-        addSymOuter(HX, 1.f, e4, e3);
-        addDiag(HX, -2.f * (d34 + m * (d04 + d03 + q)));
-        addSymOuter(HX, m, e0, e3 + e4);
-        addOuter(HX, 2.f * m, e0);
-        addDiag(HY, -2.f * s0 * D);
-        addSymOuter(HY, -2.f * s0, e0, g0 - m * g34);
+        if constexpr (HESS) {
+            addSymOuter(HX, T(1), e4, e3);
+            addDiag(HX, T(-2) * (d34 + m * (d04 + d03 + q)));
+            addSymOuter(HX, m, e0, e3 + e4);
+            addOuter(HX, T(2) * m, e0);
+            addDiag(HY, T(-2) * s0 * D);
+            addSymOuter(HY, T(-2) * s0, e0, g0 - m * g34);
+        }
     } else {
-        const V3f ew = second ? e3 : e4, gw = second ? g4 : g3;
+        const V3<T> ew = second ? e3 : e4, gw = second ? g4 : g3;
         gX = (second ? d03 : d04) * e0 - q * ew;
         gY = (-s0 * q) * gw;
     }
-    const float ir2 = 1.f / (X * X + Y * Y);
-    const V3f gt = ir2 * (X * gY - Y * gX);  // grad theta
-    const V3f u = X * gX + Y * gY;
-    const float kd = k * (hingeAtan2(Y, X) - theta0);
+    const T ir2 = T(1) / (X * X + Y * Y);
+    const V3<T> gt = ir2 * (X * gY - Y * gX);  // grad theta
+    const T kd = k * (hingeAtan2(Y, X) - theta0);
     g = kd * gt;
-    // k grad theta grad theta^T + k (theta - theta0) [(X HY - Y HX) / r2 - sym(grad theta, u) / r2]
-    const float a = kd * ir2;
-    H = {a * (X * HY.xx - Y * HX.xx), a * (X * HY.xy - Y * HX.xy), a * (X * HY.xz - Y * HX.xz),
-         a * (X * HY.yy - Y * HX.yy), a * (X * HY.yz - Y * HX.yz), a * (X * HY.zz - Y * HX.zz)};
-    addSymOuter(H, -a, gt, u);
-    addOuter(H, k, gt);
+    if constexpr (HESS) {
+        // k grad theta grad theta^T + k (theta - theta0) [(X HY - Y HX) / r2 - sym(grad theta, u) / r2]
+        const V3<T> u = X * gX + Y * gY;
+        const T a = kd * ir2;
+        H = {a * (X * HY.xx - Y * HX.xx), a * (X * HY.xy - Y * HX.xy), a * (X * HY.xz - Y * HX.xz),
+             a * (X * HY.yy - Y * HX.yy), a * (X * HY.yz - Y * HX.yz), a * (X * HY.zz - Y * HX.zz)};
+        addSymOuter(H, -a, gt, u);
+        addOuter(H, k, gt);
+    }
 }
+
+// four coefficients of a record: float4 in FP32, double4 in FP64
+template <class T>
+using Coef4 = std::conditional_t<std::is_same_v<T, float>, float4, double4>;
 
 // Baraff-Witkin membrane (membraneBaraffWitkinEnergy.cpp:9-68), one triangle slot in closed form. F = [f0 f1] =
 // [x1 - x0, x2 - x0] Dinv, C = (|f0| - 1, |f1| - 1, f0 . f1), E = 1/2 sum k_i C_i^2 (k_i the effective stiffness
 // diagonal, area included); slot a moves f0 by gamma_a0 (ga) and f1 by gamma_a1 (gb) (the rows of Dinv, gamma_0
 // their negated sum), so its block is gamma_a0^2 (A0 + k2 f1 f1^T) + gamma_a1^2 (A1 + k2 f0 f0^T)
 // + gamma_a0 gamma_a1 k2 (2 C2 I + f0 f1^T + f1 f0^T) with Ai = k_i (fi fi^T / |fi|^3 + C_i / |fi| I).
-__device__ __forceinline__ void membraneBwRec(const V3f *x, float4 d, float k0, float k1, float k2, float ga, float gb,
-                                              V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void membraneBwRec(const V3<T> *x, Coef4<T> d, T k0, T k1, T k2, T ga, T gb, V3<T> &g,
+                                              Sym3<T> &H)
 {
-    const V3f u = x[1] - x[0], v = x[2] - x[0];
-    const V3f f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
-    const float n0 = sqrtf(dot(f0, f0)), n1 = sqrtf(dot(f1, f1));
-    const float c0 = n0 - 1.f, c1 = n1 - 1.f, c2 = dot(f0, f1);
-    const float i0 = 1.f / n0, i1 = 1.f / n1;
-    const V3f dEdf0 = (k0 * c0 * i0) * f0 + (k2 * c2) * f1, dEdf1 = (k1 * c1 * i1) * f1 + (k2 * c2) * f0;
+    const V3<T> u = x[1] - x[0], v = x[2] - x[0];
+    const V3<T> f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
+    const T n0 = sqrt(dot(f0, f0)), n1 = sqrt(dot(f1, f1));
+    const T c0 = n0 - T(1), c1 = n1 - T(1), c2 = dot(f0, f1);
+    const T i0 = T(1) / n0, i1 = T(1) / n1;
+    const V3<T> dEdf0 = (k0 * c0 * i0) * f0 + (k2 * c2) * f1, dEdf1 = (k1 * c1 * i1) * f1 + (k2 * c2) * f0;
     g = ga * dEdf0 + gb * dEdf1;
-    H = zeroSym<float>();
-    addDiag(H, ga * ga * k0 * c0 * i0 + gb * gb * k1 * c1 * i1 + 2.f * ga * gb * k2 * c2);
-    addOuter(H, ga * ga * k0 * i0 * i0 * i0 + gb * gb * k2, f0);
-    addOuter(H, gb * gb * k1 * i1 * i1 * i1 + ga * ga * k2, f1);
-    addSymOuter(H, ga * gb * k2, f0, f1);
+    if constexpr (HESS) {
+        H = zeroSym<T>();
+        addDiag(H, ga * ga * k0 * c0 * i0 + gb * gb * k1 * c1 * i1 + T(2) * ga * gb * k2 * c2);
+        addOuter(H, ga * ga * k0 * i0 * i0 * i0 + gb * gb * k2, f0);
+        addOuter(H, gb * gb * k1 * i1 * i1 * i1 + ga * ga * k2, f1);
+        addSymOuter(H, ga * gb * k2, f0, f1);
+    }
 }
 
 // k (f f^T / |f|^3 + (|f| - 1) / |f| I) and k (|f| - 1) f / |f|, scaled by c^2 and c: one unit-stretch term
 // 1/2 k (|f| - 1)^2 of a membrane whose f moves by c times the slot's displacement.
-__device__ __forceinline__ void unitStretchSlot(V3f f, float k, float c, V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void unitStretchSlot(V3<T> f, T k, T c, V3<T> &g, Sym3<T> &H)
 {
-    const float n = sqrtf(dot(f, f)), in = 1.f / n;
-    g = g + (c * k * (n - 1.f) * in) * f;
-    addDiag(H, c * c * k * (n - 1.f) * in);
-    addOuter(H, c * c * k * in * in * in, f);
+    const T n = sqrt(dot(f, f)), in = T(1) / n;
+    g = g + (c * k * (n - T(1)) * in) * f;
+    if constexpr (HESS) {
+        addDiag(H, c * c * k * (n - T(1)) * in);
+        addOuter(H, c * c * k * in * in * in, f);
+    }
 }
 
 // Choi membrane (membraneChoiEnergy.cpp:9-28): four unit stretches, of f0, f1, gp = (f0 + f1)/sqrt 2 and
 // gm = (f0 - f1)/sqrt 2; the slot moves them by ga, gb, (ga +- gb)/sqrt 2.
-__device__ __forceinline__ void membraneChoiRec(const V3f *x, float4 d, float4 k, float ga, float gb, V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void membraneChoiRec(const V3<T> *x, Coef4<T> d, Coef4<T> k, T ga, T gb, V3<T> &g,
+                                                Sym3<T> &H)
 {
-    const V3f u = x[1] - x[0], v = x[2] - x[0];
-    const V3f f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
-    const float r = 0.70710678118654752440f;
-    g = {0.f, 0.f, 0.f};
-    H = zeroSym<float>();
-    unitStretchSlot(f0, k.x, ga, g, H);
-    unitStretchSlot(f1, k.y, gb, g, H);
-    unitStretchSlot(r * (f0 + f1), k.z, r * (ga + gb), g, H);
-    unitStretchSlot(r * (f0 - f1), k.w, r * (ga - gb), g, H);
+    const V3<T> u = x[1] - x[0], v = x[2] - x[0];
+    const V3<T> f0 = d.x * u + d.z * v, f1 = d.y * u + d.w * v;
+    const T r = T(0.70710678118654752440);
+    g = {T(0), T(0), T(0)};
+    H = zeroSym<T>();
+    unitStretchSlot<T, HESS>(f0, k.x, ga, g, H);
+    unitStretchSlot<T, HESS>(f1, k.y, gb, g, H);
+    unitStretchSlot<T, HESS>(r * (f0 + f1), k.z, r * (ga + gb), g, H);
+    unitStretchSlot<T, HESS>(r * (f0 - f1), k.w, r * (ga - gb), g, H);
 }
 
 // Quadratic bending (quadraticBendingEnergy.cpp:9-30): E = 1/2 keff |C|^2 with C = sum c_s x_s, so slot a has
 // gradient keff c_a C and block keff c_a^2 I (cs = c_a).
-__device__ __forceinline__ void bendingQuadraticRec(const V3f *x, float4 c, float keff, float cs, V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void bendingQuadraticRec(const V3<T> *x, Coef4<T> c, T keff, T cs, V3<T> &g, Sym3<T> &H)
 {
-    const V3f C = c.x * x[0] + c.y * x[1] + c.z * x[2] + c.w * x[3];
-    const float kc = keff * cs;
+    const V3<T> C = c.x * x[0] + c.y * x[1] + c.z * x[2] + c.w * x[3];
+    const T kc = keff * cs;
     g = kc * C;
-    H = zeroSym<float>();
-    addDiag(H, kc * cs);
+    if constexpr (HESS) {
+        H = zeroSym<T>();
+        addDiag(H, kc * cs);
+    }
 }
 
 // Squared continuum spring (springSquaredContinuumEnergy.cpp:9-19): C = |d|^2 / L0^2 - 1, E = 1/2 Keff C^2,
 // d = x0 - x1: both slots have the block Keff ((2/L0^2)^2 d d^T + C (2/L0^2) I); the gradient has sign sg
 // (+1 for slot 0, -1 for slot 1).
-__device__ __forceinline__ void springSquaredContinuumRec(const V3f *x, float L0, float k, float sg, V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void springSquaredContinuumRec(const V3<T> *x, T L0, T k, T sg, V3<T> &g, Sym3<T> &H)
 {
-    const V3f d = x[0] - x[1];
-    const float L2 = L0 * L0;
-    const float C = dot(d, d) / L2 - 1.f, s = 2.f / L2;
+    const V3<T> d = x[0] - x[1];
+    const T L2 = L0 * L0;
+    const T C = dot(d, d) / L2 - T(1), s = T(2) / L2;
     g = (sg * k * C * s) * d;
-    H = zeroSym<float>();
-    addDiag(H, k * C * s);
-    addOuter(H, k * s * s, d);
+    if constexpr (HESS) {
+        H = zeroSym<T>();
+        addDiag(H, k * C * s);
+        addOuter(H, k * s * s, d);
+    }
 }
 
 // Tetrahedra: F = [f0 f1 f2] = [x1 - x0, x2 - x0, x3 - x0] Dinv (Dinv row-major in dv[0..8]) and the slot moves
 // column c by gm[c] (row a - 1 of Dinv, gamma_0 the negated column sums), so it moves F by u gamma^T and
 // w = F gamma is F's rate.
-__device__ __forceinline__ void tetFrameRec(const V3f *x, const float *dv, V3f *f)
+template <class T = float>
+__device__ __forceinline__ void tetFrameRec(const V3<T> *x, const T *dv, V3<T> *f)
 {
-    const V3f u = x[1] - x[0], v = x[2] - x[0], w = x[3] - x[0];
+    const V3<T> u = x[1] - x[0], v = x[2] - x[0], w = x[3] - x[0];
 #pragma unroll
     for (int c = 0; c < 3; ++c) f[c] = dv[c] * u + dv[3 + c] * v + dv[6 + c] * w;
 }
 
 // StVK tetrahedron (stvkEnergy.cpp:41-63, energy form): P = F S, S = lambda tr(E) I + 2 mu E; the slot block is
 // (gamma^T S gamma) I + (lambda + mu) w w^T + mu |gamma|^2 F F^T.
-__device__ __forceinline__ void tetStvkRec(const V3f *x, const float *dv, const float *gm, float lam, float mu,
-                                           V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void tetStvkRec(const V3<T> *x, const T *dv, const T *gm, T lam, T mu, V3<T> &g, Sym3<T> &H)
 {
-    V3f f[3];
-    tetFrameRec(x, dv, f);
-    float S[3][3];
+    V3<T> f[3];
+    tetFrameRec<T>(x, dv, f);
+    T S[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) S[i][j] = mu * (dot(f[i], f[j]) - (i == j ? 1.f : 0.f));
-    const float lt = 0.5f * lam * (dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.f);
+        for (int j = 0; j < 3; ++j) S[i][j] = mu * (dot(f[i], f[j]) - (i == j ? T(1) : T(0)));
+    const T lt = T(0.5) * lam * (dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - T(3));
     S[0][0] += lt;
     S[1][1] += lt;
     S[2][2] += lt;
-    float Sg[3], gSg = 0.f;
+    T Sg[3], gSg = T(0);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         Sg[i] = S[i][0] * gm[0] + S[i][1] * gm[1] + S[i][2] * gm[2];
         gSg += gm[i] * Sg[i];
     }
     g = Sg[0] * f[0] + Sg[1] * f[1] + Sg[2] * f[2];
-    const V3f w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
-    const float g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
-    H = zeroSym<float>();
-    addDiag(H, gSg);
-    addOuter(H, lam + mu, w);
+    if constexpr (HESS) {
+        const V3<T> w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
+        const T g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
+        H = zeroSym<T>();
+        addDiag(H, gSg);
+        addOuter(H, lam + mu, w);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) addOuter(H, mu * g2, f[c]);
+        for (int c = 0; c < 3; ++c) addOuter(H, mu * g2, f[c]);
+    }
 }
 
 // Stable squared Neo-Hookean (stableSquaredNeoHookeanEnergy.cpp:9-31): E = 1/2 lambda (J - 1)^2
 // + 1/2 mu (|F|^2 - 3)^2; J is affine along the slot's rank-one update, so the block is lambda c c^T
 // + mu (4 w w^T + 2 (|F|^2 - 3) |gamma|^2 I) with c = cof(F) gamma.
-__device__ __forceinline__ void tetSsnhRec(const V3f *x, const float *dv, const float *gm, float lam, float mu,
-                                           V3f &g, S3f &H)
+template <class T = float, bool HESS = true>
+__device__ __forceinline__ void tetSsnhRec(const V3<T> *x, const T *dv, const T *gm, T lam, T mu, V3<T> &g, Sym3<T> &H)
 {
-    V3f f[3];
-    tetFrameRec(x, dv, f);
-    const V3f k0 = cross(f[1], f[2]), k1 = cross(f[2], f[0]), k2 = cross(f[0], f[1]);
-    const float J = dot(f[0], k0), s = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - 3.f;
-    const V3f c = gm[0] * k0 + gm[1] * k1 + gm[2] * k2;
-    const V3f w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
-    const float g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
-    g = (lam * (J - 1.f)) * c + (2.f * mu * s) * w;
-    H = zeroSym<float>();
-    addOuter(H, lam, c);
-    addOuter(H, 4.f * mu, w);
-    addDiag(H, 2.f * mu * s * g2);
+    V3<T> f[3];
+    tetFrameRec<T>(x, dv, f);
+    const V3<T> k0 = cross(f[1], f[2]), k1 = cross(f[2], f[0]), k2 = cross(f[0], f[1]);
+    const T J = dot(f[0], k0), s = dot(f[0], f[0]) + dot(f[1], f[1]) + dot(f[2], f[2]) - T(3);
+    const V3<T> c = gm[0] * k0 + gm[1] * k1 + gm[2] * k2;
+    const V3<T> w = gm[0] * f[0] + gm[1] * f[1] + gm[2] * f[2];
+    g = (lam * (J - T(1))) * c + (T(2) * mu * s) * w;
+    if constexpr (HESS) {
+        const T g2 = gm[0] * gm[0] + gm[1] * gm[1] + gm[2] * gm[2];
+        H = zeroSym<T>();
+        addOuter(H, lam, c);
+        addOuter(H, T(4) * mu, w);
+        addDiag(H, T(2) * mu * s * g2);
+    }
 }
 
 // The slot-dependent coefficients from an element's FP64 parameters (the same values the host writes into the
@@ -477,8 +505,10 @@ __device__ __forceinline__ void extRecOf(const V3f *xs, const double *p, V3f &g,
 // closed-form full gradient (implicit_element.h) at the element's positions is taken in one forward-mode pass
 // whose three tangents are the slot's coordinates (the block's columns, symmetrised). The slot is a template
 // parameter so the slot's gradient is picked at compile time (a run-time pick puts the per-slot gradients in
-// scratch memory) and the other slots' outputs are dropped. T: float (solve) or double (residual).
-// TYPE = 0: the energy type is read at run time (`type`); otherwise it is TYPE.
+// scratch memory) and the other slots' outputs are dropped. The per-element derivative hook (extElementKernel)
+// instantiates it in FP32 with the block: the closed forms for types 7 ... 13 and the forward-mode pass for the
+// continuum spring (type 6, whose solve path is the spring class). TYPE = 0: the energy type is read at run time
+// (`type`); otherwise it is TYPE.
 template <int NS, int LOCAL, class T, bool HESS, int TYPE = 0>
 __device__ __forceinline__ void extSlot(int type, const float4 *__restrict__ pos, int4 v,
                                         const double *__restrict__ p, V3<T> &g, Sym3<T> &H)
@@ -588,19 +618,6 @@ __device__ __forceinline__ void extClassRun(const SolveArgs &a, int32_t vid, int
                 tetSsnhRec(xs, dv, gm, q[3].x, q[3].y, ge, He);
         }
         accIf(ok, g, H, ge, He);
-    }
-}
-template <class T, bool HESS>
-__device__ __forceinline__ void extIncidence(int2 in, const float4 *__restrict__ pos, const int4 *__restrict__ extV,
-                                             const double *__restrict__ extP, V3<T> &g, Sym3<T> &H)
-{
-    const int type = in.y >> 2, local = in.y & 3;
-    const int4 v = extV[in.x];
-    const double *p = extP + static_cast<int64_t>(kExtParams) * in.x;
-    switch (fullStencilSize(type)) {
-    case 2: extSlotAny<2, T, HESS>(type, local, pos, v, p, g, H); break;
-    case 3: extSlotAny<3, T, HESS>(type, local, pos, v, p, g, H); break;
-    default: extSlotAny<4, T, HESS>(type, local, pos, v, p, g, H);
     }
 }
 
@@ -948,6 +965,62 @@ __device__ __forceinline__ V3d tetSlotGradD(const double2 *p, bool s0, bool s01,
     return ge;
 }
 
+// The FP64 gradient at slot `slot` of a further-energy element of class C from its exact parameters p: the
+// closed forms of the solve (gradient only), with the slot-dependent coefficients picked by selects.
+template <int C>
+__device__ __forceinline__ V3d extSlotGradD(const V3d *xs, const double *p, int slot)
+{
+    V3d g;
+    S3d H;
+    if constexpr (C == 0) {
+        springSquaredContinuumRec<double, false>(xs, p[0], p[1], slot == 0 ? 1.0 : -1.0, g, H);
+    } else if constexpr (C == 1 || C == 2) {
+        const double4 d = make_double4(p[0], p[1], p[2], p[3]);
+        const double ga = slot == 0 ? -(d.x + d.z) : slot == 1 ? d.x : d.z;
+        const double gb = slot == 0 ? -(d.y + d.w) : slot == 1 ? d.y : d.w;
+        if constexpr (C == 1)
+            membraneBwRec<double, false>(xs, d, p[4], p[5], p[6], ga, gb, g, H);
+        else
+            membraneChoiRec<double, false>(xs, d, make_double4(p[4], p[5], p[6], p[7]), ga, gb, g, H);
+    } else if constexpr (C == 3) {
+        const double cs = slot == 0 ? p[0] : slot == 1 ? p[1] : slot == 2 ? p[2] : p[3];
+        bendingQuadraticRec<double, false>(xs, make_double4(p[0], p[1], p[2], p[3]), p[4], cs, g, H);
+    } else if constexpr (C == 4 || C == 5) {
+        bendingBwRec<C == 5, double, false>(xs, p[0], p[1], slot == 1 || slot == 3, g, H);
+    } else {
+        double dv[9], gm[3];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) dv[i] = p[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            gm[c] = slot == 0 ? -(p[c] + p[3 + c] + p[6 + c]) : slot == 1 ? p[c] : slot == 2 ? p[3 + c] : p[6 + c];
+        if constexpr (C == 6)
+            tetStvkRec<double, false>(xs, dv, gm, p[9], p[10], g, H);
+        else
+            tetSsnhRec<double, false>(xs, dv, gm, p[9], p[10], g, H);
+    }
+    return g;
+}
+
+// One further-energy class of the residual: the vertex's row of (element, its slot in the element) in the solve's
+// class layout, each entry's FP64 slot gradient from the element's exact parameters and slots.
+template <int C, int G>
+__device__ __forceinline__ void extResidualClass(const ResidualArgs &a, int64_t row, int32_t lig, V3d &r)
+{
+    constexpr int NS = xclsStencil(C);
+    const int32_t w = a.xrW[C];
+    for (int32_t j = lig; j < w; j += G) {
+        const int2 e = a.xrR[C][row * w + j];
+        if (e.x < 0) continue;
+        const int4 v = a.extV[e.x];
+        const int32_t vv[4] = {v.x, v.y, v.z, v.w};
+        V3d xs[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) xs[s] = xyzd(a.pos[vv[s]]);
+        r = r - extSlotGradD<C>(xs, a.extP + static_cast<int64_t>(kExtParams) * e.x, e.y);
+    }
+}
+
 // K4: r_v = -(m/h^2)(x_v - x_tilde_v) - sum_e grad_v E_e(x), all in FP64 from the FP32 state and the
 // exact prepared parameters (implicitEulerBase.cpp:102-143 force, x-form of
 // implicitEulerNewtonRobust.cpp:65-66). Writes |r_v|^2 per particle (particle order).
@@ -1032,10 +1105,14 @@ __global__ void __launch_bounds__(256) residualKernel(ResidualArgs a)
             }
         }
         if constexpr (EXT) {
-            for (int32_t k = a.extPtr[vid] + lig; k < a.extPtr[vid + 1]; k += G) {
-                extIncidence<double, false>(a.extInc[k], a.pos, a.extV, a.extP, ge, He);
-                r = r - ge;
-            }
+            extResidualClass<0, G>(a, row, lig, r);
+            extResidualClass<1, G>(a, row, lig, r);
+            extResidualClass<2, G>(a, row, lig, r);
+            extResidualClass<3, G>(a, row, lig, r);
+            extResidualClass<4, G>(a, row, lig, r);
+            extResidualClass<5, G>(a, row, lig, r);
+            extResidualClass<6, G>(a, row, lig, r);
+            extResidualClass<7, G>(a, row, lig, r);
         }
     }
     r.x = groupSum<G>(r.x, lane64);

@@ -247,7 +247,8 @@ def main():
 def _throughput(scene_list, steps):
     """Autotuned, graph-captured steps of the scenes advanced together (each step from the initial
     state); returns (vertex-updates/s, ms per step, launch, event-timed solve launch us, algorithmic
-    GB/s of the solve, all positions finite, context setup s)."""
+    GB/s of the solve, the time and roofline fraction of one FP64 residual evaluation after one substep's
+    sweeps, all positions finite, context setup s)."""
     import numpy as np
     from spg_amd import native
 
@@ -279,11 +280,18 @@ def _throughput(scene_list, steps):
     ctx.set_timing(False)
     finite = all(bool(np.isfinite(ctx.state(o)[0]).all()) for o in range(len(scene_list)))
     mean_ms = st["solve_ms"] / max(1, st["solve_launches"])
+    # one FP64 residual evaluation (the convergence check) at the state after one substep's sweeps
+    ctx.restore_state()
+    ctx.begin_substep(dt, g)
+    ctx.iterate(iters)
+    res_us, res_bytes = ctx.bench_residual(20)
+    ctx.end_substep()
     return {"vertices": V, "colors": ctx.num_colors(), "launch": tuned, "value": V * iters * steps / wall,
             "unit": "vertex-updates/s", "ms_per_step": wall * 1e3 / steps, "steps": steps,
             "solve_launch_us": mean_ms * 1e3,
             "solve_achieved_GBs": st["algorithmic_bytes"] / max(1, st["solve_launches"]) / (mean_ms * 1e-3) / 1e9,
             "fp64_fallbacks": st["fp64_fallbacks"], "skipped_blocks": st["skipped_blocks"],
+            "residual_us": res_us, "residual_frac": res_bytes / (res_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
             "setup_s": setup, "state_finite": finite}
 
 

@@ -327,8 +327,18 @@ __global__ void __launch_bounds__(1024) cgInitFinalKernel(const double *__restri
 // The sum of partial[0, n) in a fixed order, the same value in every thread (and every workgroup).
 __device__ __forceinline__ double blockTotal(const double *__restrict__ partial, int n, double *sh)
 {
+    // thread t adds entries t, t + B, t + 2B, ... in that order; eight of its loads are issued together
     double acc = 0.0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+    const int B = static_cast<int>(blockDim.x);
+    int i = threadIdx.x;
+    for (; i + 7 * B < n; i += 8 * B) {
+        double v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = partial[i + k * B];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc += v[k];
+    }
+    for (; i < n; i += B) acc += partial[i];
     const double t = blockSumFixed(acc, sh);
     if (threadIdx.x == 0) sh[0] = t;
     __syncthreads();

@@ -81,7 +81,6 @@ hipError_t launchSmoothPrediction(const float *y, float *out, const int32_t *nbP
                                   hipStream_t stream);
 hipError_t launchGate(float4 *pos, const float4 *xt, const int32_t *objOfSlot, const int32_t *reject, int32_t n,
                       hipStream_t stream);
-hipError_t launchCorrection(const float4 *pos, const float4 *xt, float4 *corr, int32_t n, hipStream_t stream);
 
 // Columns of the training forward pass's output buffer (the last layer is computed 16 wide).
 constexpr int kMlpInferCols = 16;

@@ -392,8 +392,8 @@ struct spg_vbd_ctx {
     void endSubstep()
     {
         positionsChanged();
-        hipCheck(launchCorrection(pos.p, xt.p, corr.p, static_cast<int32_t>(nv), stream), "correction");
-        hipCheck(launchEndSubstep(pos.p, xold.p, vel.p, static_cast<int32_t>(nv), static_cast<float>(1.0 / h), stream),
+        hipCheck(launchEndSubstep(pos.p, xold.p, xt.p, vel.p, corr.p, static_cast<int32_t>(nv),
+                                  static_cast<float>(1.0 / h), stream),
                  "end substep");
         inSubstep = false;
     }

@@ -192,8 +192,9 @@ struct ResidualArgs {
 // kernel launchers (vbd_kernels.hip); all asynchronous on `stream`
 hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt, const float *mass, int32_t n,
                               float h, float gx, float gy, float gz, hipStream_t stream);
-hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
-                            hipStream_t stream);
+// v = (x - x_old) / h and corr = x - x~
+hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, const float4 *xt, float4 *vel, float4 *corr,
+                            int32_t n, float invh, hipStream_t stream);
 // LDS staging of the element-record rows is available while one workgroup's rows fit this many bytes
 constexpr size_t kSolveMaxLds = 65536;
 // LDS bytes per workgroup of the staged variants (lds_staging 1: element rows, 2: neighbourhood slabs; 0: none)

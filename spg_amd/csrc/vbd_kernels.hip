@@ -67,15 +67,18 @@ __global__ void __launch_bounds__(256) beginSubstepKernel(float4 *__restrict__ p
     xt[i] = make_float4(x.x, x.y, x.z, mass[i]);
 }
 
-// K3 (vbd.cpp:173-185, particleGroup.cpp:65-72): v = (x - x_old) / h.
+// K3 (vbd.cpp:173-185, particleGroup.cpp:65-72): v = (x - x_old) / h; with it the substep's correction
+// x - x~ (the offset the history and learned starts of the next substep use), from the same read of x.
 __global__ void __launch_bounds__(256) endSubstepKernel(const float4 *__restrict__ pos,
-                                                        const float4 *__restrict__ xold, float4 *__restrict__ vel,
-                                                        int32_t n, float invh)
+                                                        const float4 *__restrict__ xold,
+                                                        const float4 *__restrict__ xt, float4 *__restrict__ vel,
+                                                        float4 *__restrict__ corr, int32_t n, float invh)
 {
     const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float4 x = pos[i], xo = xold[i];
+    const float4 x = pos[i], xo = xold[i], t = xt[i];
     vel[i] = make_float4((x.x - xo.x) * invh, (x.y - xo.y) * invh, (x.z - xo.z) * invh, 0.f);
+    corr[i] = make_float4(x.x - t.x, x.y - t.y, x.z - t.z, 0.f);
 }
 
 __global__ void __launch_bounds__(256) copyXtKernel(float4 *__restrict__ pos, const float4 *__restrict__ xt, int32_t n)
@@ -1203,11 +1206,11 @@ hipError_t launchBeginSubstep(float4 *pos, float4 *xold, float4 *vel, float4 *xt
     return hipGetLastError();
 }
 
-hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, float4 *vel, int32_t n, float invh,
-                            hipStream_t stream)
+hipError_t launchEndSubstep(const float4 *pos, const float4 *xold, const float4 *xt, float4 *vel, float4 *corr,
+                            int32_t n, float invh, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    endSubstepKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, xold, vel, n, invh);
+    endSubstepKernel<<<blocksFor(n, 256), 256, 0, stream>>>(pos, xold, xt, vel, corr, n, invh);
     return hipGetLastError();
 }
 

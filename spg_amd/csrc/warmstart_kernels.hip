@@ -116,15 +116,6 @@ __global__ void __launch_bounds__(256) gateKernel(float4 *__restrict__ pos, cons
 }
 
 // c = x* - x~ at the end of a substep (the next substep's inertial-offset feature).
-__global__ void __launch_bounds__(256) correctionKernel(const float4 *__restrict__ pos, const float4 *__restrict__ xt,
-                                                        float4 *__restrict__ corr, int32_t n)
-{
-    const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float4 x = pos[i], t = xt[i];
-    corr[i] = make_float4(x.x - t.x, x.y - t.y, x.z - t.z, 0.f);
-}
-
 inline int blocksOf(int64_t n) { return static_cast<int>((n + 255) / 256); }
 
 }  // namespace
@@ -157,13 +148,6 @@ hipError_t launchGate(float4 *pos, const float4 *xt, const int32_t *objOfSlot, c
 {
     if (n <= 0) return hipSuccess;
     gateKernel<<<blocksOf(n), 256, 0, stream>>>(pos, xt, objOfSlot, reject, n);
-    return hipGetLastError();
-}
-
-hipError_t launchCorrection(const float4 *pos, const float4 *xt, float4 *corr, int32_t n, hipStream_t stream)
-{
-    if (n <= 0) return hipSuccess;
-    correctionKernel<<<blocksOf(n), 256, 0, stream>>>(pos, xt, corr, n);
     return hipGetLastError();
 }
 

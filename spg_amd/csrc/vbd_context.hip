@@ -1229,7 +1229,6 @@ void buildContext(spg_vbd_ctx &c, const spg_vbd_object *objs, int32_t nObjects)
         for (int64_t g = 0; g < c.nv; ++g) ++cnt[c.localColors[g]];
         for (int32_t k = 0; k < c.maxColors; ++k) c.colorBegin[k + 1] = static_cast<int32_t>(c.colorBegin[k] + cnt[k]);
         std::vector<int32_t> fill(c.colorBegin.begin(), c.colorBegin.end() - 1);
-        for (int32_t k = 0; k < c.maxColors; ++k) (void)k;
         // objects in order, particles ascending: within colour k the runs are object-major
         for (int64_t g = 0; g < c.nv; ++g) {
             const int32_t s = fill[c.localColors[g]]++;

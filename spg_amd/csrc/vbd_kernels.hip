@@ -18,7 +18,11 @@ using S3f = Sym3<float>;
 
 __device__ __forceinline__ V3f xyz(float4 p) { return {p.x, p.y, p.z}; }
 
-// accumulate only when `take` (a select: nothing from a dropped contribution reaches g or H)
+// Adds an element's gradient and block only when `take`, by selects: a dropped contribution (a padding record,
+// evaluated on stand-in positions) reaches neither g nor H. The kernel instantiation with the further energies
+// uses it for every class; the five-energy instantiations branch around the evaluation instead (acc), which
+// measured faster there (about 1.5% of the 512x512 cloth's step) and slower with the further energies (2 to 8% of
+// that cloth's colour launch).
 __device__ __forceinline__ void accIf(bool take, V3f &g, S3f &H, const V3f &ge, const S3f &He)
 {
     g.x += take ? ge.x : 0.f;
@@ -32,6 +36,7 @@ __device__ __forceinline__ void accIf(bool take, V3f &g, S3f &H, const V3f &ge, 
     H.zz += take ? He.zz : 0.f;
 }
 
+// Adds an element's gradient and block.
 __device__ __forceinline__ void acc(V3f &g, S3f &H, const V3f &ge, const S3f &He)
 {
     g.x += ge.x;
@@ -43,6 +48,25 @@ __device__ __forceinline__ void acc(V3f &g, S3f &H, const V3f &ge, const S3f &He
     H.yy += He.yy;
     H.yz += He.yz;
     H.zz += He.zz;
+}
+
+// Evaluates one incidence (eval writes its gradient and block) and adds it when `ok`: BRANCH = true skips the
+// evaluation of padding records by a branch, false evaluates every record (padding on stand-in positions) and
+// drops it by selects (accIf). Both add the same values; the choice is by measured speed per instantiation.
+template <bool BRANCH, class F>
+__device__ __forceinline__ void accElement(bool ok, V3f &g, S3f &H, F &&eval)
+{
+    V3f ge;
+    S3f He;
+    if constexpr (BRANCH) {
+        if (ok) {
+            eval(ge, He);
+            acc(g, H, ge, He);
+        }
+    } else {
+        eval(ge, He);
+        accIf(ok, g, H, ge, He);
+    }
 }
 
 // K1 (vbd.cpp:48-83, particleGroup.cpp:57-63): x_old = x; v += h g; x += h v; x_tilde = x.
@@ -680,59 +704,54 @@ __global__ void __launch_bounds__(256) solveColorKernel(SolveArgs a)
             const float4 gg = a.triG[r];
             const float4 kk = a.triM[r];
             const bool ok = o.x >= 0;
-            V3f ge;
-            S3f He;
-            stvkVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), gg.x, gg.y, gg.z, gg.w,
-                              kk.x, kk.y, kk.z, ge, He);
-            accIf(ok, g, H, ge, He);
+            accElement<!EXT>(ok, g, H, [&](V3f &ge, S3f &He) {
+                stvkVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), gg.x, gg.y, gg.z, gg.w,
+                                  kk.x, kk.y, kk.z, ge, He);
+            });
         }
         for (int32_t j = lig; j < a.hinEW; j += G) {
             const int64_t r = row * a.hinEW + j;
             const int4 v = a.hinEV[r];
             const float2 p = a.hinEP[r];
             const bool ok = v.x >= 0;
-            // the vertex is edge end x0 or x1: its own position is xo, only the other end is gathered
-            const bool second = v.y == vid;
-            const V3f pe = xyz(a.pos[ok ? (second ? v.x : v.y) : vid]);
-            V3f ge;
-            S3f He;
-            bendingEdge<float>(second ? pe : xo, second ? xo : pe, xyz(a.pos[ok ? v.z : vid]),
-                               xyz(a.pos[ok ? v.w : vid]), second, p.x, p.y, ge, He);
-            accIf(ok, g, H, ge, He);
+            accElement<!EXT>(ok, g, H, [&](V3f &ge, S3f &He) {
+                // the vertex is edge end x0 or x1: its own position is xo, only the other end is gathered
+                const bool second = v.y == vid;
+                const V3f pe = xyz(a.pos[ok ? (second ? v.x : v.y) : vid]);
+                bendingEdge<float>(second ? pe : xo, second ? xo : pe, xyz(a.pos[ok ? v.z : vid]),
+                                   xyz(a.pos[ok ? v.w : vid]), second, p.x, p.y, ge, He);
+            });
         }
         for (int32_t j = lig; j < a.hinWW; j += G) {
             const int64_t r = row * a.hinWW + j;
             const int4 v = a.hinWV[r];
             const float2 p = a.hinWP[r];
             const bool ok = v.x >= 0;
-            // the vertex is wing x2 or x3: its own position is xo, only the other wing is gathered
-            const bool second = v.w == vid;
-            const V3f pw = xyz(a.pos[ok ? (second ? v.z : v.w) : vid]);
-            V3f ge;
-            S3f He;
-            bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), second ? pw : xo,
-                               second ? xo : pw, second, p.x, p.y, ge, He);
-            accIf(ok, g, H, ge, He);
+            accElement<!EXT>(ok, g, H, [&](V3f &ge, S3f &He) {
+                // the vertex is wing x2 or x3: its own position is xo, only the other wing is gathered
+                const bool second = v.w == vid;
+                const V3f pw = xyz(a.pos[ok ? (second ? v.z : v.w) : vid]);
+                bendingWing<float>(xyz(a.pos[ok ? v.x : vid]), xyz(a.pos[ok ? v.y : vid]), second ? pw : xo,
+                                   second ? xo : pw, second, p.x, p.y, ge, He);
+            });
         }
         for (int32_t j = lig; j < a.tetW; j += G) {
             const int64_t r = row * a.tetW + j;
             const int4 o = a.tetO[r];
             const float4 ga = a.tetGa[r], gb = a.tetGb[r], gc = a.tetGc[r];
             const bool ok = o.x >= 0;
-            V3f ge;
-            S3f He;
-            snhVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]), xyz(a.pos[ok ? o.z : vid]),
-                             xyz(ga), xyz(gb), xyz(gc), ga.w, gb.w, gc.w, ge, He);
-            accIf(ok, g, H, ge, He);
+            accElement<!EXT>(ok, g, H, [&](V3f &ge, S3f &He) {
+                snhVertex<float>(xo, xyz(a.pos[ok ? o.x : vid]), xyz(a.pos[ok ? o.y : vid]),
+                                 xyz(a.pos[ok ? o.z : vid]), xyz(ga), xyz(gb), xyz(gc), ga.w, gb.w, gc.w, ge, He);
+            });
         }
         for (int32_t j = lig; j < a.sprW; j += G) {
             const float4 s = a.spr[row * a.sprW + j];
             const int32_t other = __float_as_int(s.x);
             const bool ok = other >= 0;
-            V3f ge;
-            S3f He;
-            springVertex<float>(xo, xyz(a.pos[ok ? other : vid]), s.y, s.z, ge, He);
-            accIf(ok, g, H, ge, He);
+            accElement<!EXT>(ok, g, H, [&](V3f &ge, S3f &He) {
+                springVertex<float>(xo, xyz(a.pos[ok ? other : vid]), s.y, s.z, ge, He);
+            });
         }
         for (int32_t k = ab + lig; k < ae; k += G) {
             const float4 s = a.anc[k];
@@ -853,55 +872,55 @@ __global__ void __launch_bounds__(256) solveColorNbKernel(SolveArgs a)
         xo = xyz(slab[0]);
         if constexpr ((CLS & kClsTri) != 0) {
             auto tri = [&](const float4 &gg, const float4 &kk) {
-                const int32_t pk = __float_as_int(kk.w);
-                const bool ok = pk >= 0;
-                const int32_t q = ok ? pk : 0;
-                V3f ge;
-                S3f He;
-                stvkVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), gg.x, gg.y, gg.z, gg.w, kk.x,
-                                  kk.y, kk.z, ge, He);
-                accIf(ok, g, H, ge, He);
+                const int32_t q = __float_as_int(kk.w);
+                if (q >= 0) {
+                    V3f ge;
+                    S3f He;
+                    stvkVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), gg.x, gg.y, gg.z, gg.w, kk.x,
+                                      kk.y, kk.z, ge, He);
+                    acc(g, H, ge, He);
+                }
             };
             if (t0) tri(tg, tk);
             for (int32_t j = lig + G; j < a.triW; j += G) tri(a.triG[row * a.triW + j], a.triL[row * a.triW + j]);
         }
         if constexpr ((CLS & kClsHinge) != 0) {
             auto hinE = [&](const float4 &r) {
-                const int32_t pk = __float_as_int(r.z);
-                const bool ok = pk >= 0;
-                const int32_t q = ok ? pk : 0;
-                const int32_t l1 = (q >> 5) & 31;
-                V3f ge;
-                S3f He;
-                bendingEdge<float>(xyz(slab[q & 31]), xyz(slab[l1]), xyz(slab[(q >> 10) & 31]),
-                                   xyz(slab[(q >> 15) & 31]), l1 == 0, r.x, r.y, ge, He);
-                accIf(ok, g, H, ge, He);
+                const int32_t q = __float_as_int(r.z);
+                if (q >= 0) {
+                    const int32_t l1 = (q >> 5) & 31;
+                    V3f ge;
+                    S3f He;
+                    bendingEdge<float>(xyz(slab[q & 31]), xyz(slab[l1]), xyz(slab[(q >> 10) & 31]),
+                                       xyz(slab[(q >> 15) & 31]), l1 == 0, r.x, r.y, ge, He);
+                    acc(g, H, ge, He);
+                }
             };
             if (e0) hinE(he);
             for (int32_t j = lig + G; j < a.hinEW; j += G) hinE(a.hinEL[row * a.hinEW + j]);
             auto hinW = [&](const float4 &r) {
-                const int32_t pk = __float_as_int(r.z);
-                const bool ok = pk >= 0;
-                const int32_t q = ok ? pk : 0;
-                const int32_t l3 = (q >> 15) & 31;
-                V3f ge;
-                S3f He;
-                bendingWing<float>(xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
-                                   xyz(slab[l3]), l3 == 0, r.x, r.y, ge, He);
-                accIf(ok, g, H, ge, He);
+                const int32_t q = __float_as_int(r.z);
+                if (q >= 0) {
+                    const int32_t l3 = (q >> 15) & 31;
+                    V3f ge;
+                    S3f He;
+                    bendingWing<float>(xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
+                                       xyz(slab[l3]), l3 == 0, r.x, r.y, ge, He);
+                    acc(g, H, ge, He);
+                }
             };
             if (w0) hinW(hw);
             for (int32_t j = lig + G; j < a.hinWW; j += G) hinW(a.hinWL[row * a.hinWW + j]);
         }
         if constexpr ((CLS & kClsTet) != 0) {
-            auto tet = [&](int32_t pk, const float4 &ra, const float4 &rb, const float4 &rc) {
-                const bool ok = pk >= 0;
-                const int32_t q = ok ? pk : 0;
-                V3f ge;
-                S3f He;
-                snhVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
-                                 xyz(ra), xyz(rb), xyz(rc), ra.w, rb.w, rc.w, ge, He);
-                accIf(ok, g, H, ge, He);
+            auto tet = [&](int32_t q, const float4 &ra, const float4 &rb, const float4 &rc) {
+                if (q >= 0) {
+                    V3f ge;
+                    S3f He;
+                    snhVertex<float>(xo, xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
+                                     xyz(ra), xyz(rb), xyz(rc), ra.w, rb.w, rc.w, ge, He);
+                    acc(g, H, ge, He);
+                }
             };
             if (q0) tet(tl, ga, gb, gc);
             for (int32_t j = lig + G; j < a.tetW; j += G) {
@@ -911,12 +930,13 @@ __global__ void __launch_bounds__(256) solveColorNbKernel(SolveArgs a)
         }
         if constexpr ((CLS & kClsSpring) != 0) {
             auto spring = [&](const float4 &r) {
-                const int32_t pk = __float_as_int(r.x);
-                const bool ok = pk >= 0;
-                V3f ge;
-                S3f He;
-                springVertex<float>(xo, xyz(slab[ok ? pk : 0]), r.y, r.z, ge, He);
-                accIf(ok, g, H, ge, He);
+                const int32_t q = __float_as_int(r.x);
+                if (q >= 0) {
+                    V3f ge;
+                    S3f He;
+                    springVertex<float>(xo, xyz(slab[q]), r.y, r.z, ge, He);
+                    acc(g, H, ge, He);
+                }
             };
             if (s0) spring(sp);
             for (int32_t j = lig + G; j < a.sprW; j += G) spring(a.sprL[row * a.sprW + j]);

@@ -260,14 +260,17 @@ SPG_HD void bendingEdge(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot1, R th
     const V3<R> eh = iL * e;
     const R pe = dot(p, eh), qe = dot(q, eh);
     const V3<R> pp = p - pe * eh, qp = q - qe * eh;  // perpendicular parts
-    const R rae = slot1 ? pe + L : pe, rbe = slot1 ? qe + L : qe;
-    const V3<R> ra = slot1 ? p + e : p, rb = slot1 ? q + e : q;
-    const R sg = slot1 ? R(1) : R(-1);
+    // slot 1 shifts the lever arms by e: s1 e with s1 = 0 or 1 adds exactly e or nothing, without selects
+    const R s1 = slot1 ? R(1) : R(0);
+    const R rae = s1 * L + pe, rbe = s1 * L + qe;
+    const V3<R> ra = s1 * e + p, rb = s1 * e + q;
+    const R sg = R(2) * s1 - R(1);
     const R ia2 = R(1) / dot(a, a), ib2 = R(1) / dot(b, b);
     const V3<R> gth = (sg * rae * ia2) * a + (-sg * rbe * ib2) * b;
     const V3<R> ua = (iL * ia2) * pp + (R(2) * rae * ia2 * ia2) * cross(a, ra);
     const V3<R> ub = (-iL * ib2) * qp + (R(-2) * rbe * ib2 * ib2) * cross(b, rb);
-    const R theta = hingeAtan2(-dot(cross(a, b), eh), -dot(a, b));
+    // (a x b).e^ = (e.(p x q)) (e.e^) = (a.q) L, since (e x p) x (e x q) = (e.(p x q)) e
+    const R theta = hingeAtan2(-dot(a, q) * L, -dot(a, b));
     const R kC = k * (theta - theta0);
     g = kC * gth;
     H = zeroSym<R>();
@@ -283,10 +286,12 @@ template <class R>
 SPG_HD void bendingWing(V3<R> x0, V3<R> x1, V3<R> x2, V3<R> x3, bool slot3, R theta0, R k, V3<R> &g, Sym3<R> &H)
 {
     const V3<R> e = x1 - x0;
-    const V3<R> a = cross(e, x2 - x1), b = cross(e, x3 - x1);
+    const V3<R> q = x3 - x1;
+    const V3<R> a = cross(e, x2 - x1), b = cross(e, q);
     R L, iL;
     lengthAndInverse(dot(e, e), L, iL);
-    const R theta = hingeAtan2(-dot(cross(a, b), e) * iL, -dot(a, b));
+    // (a x b).e^ = (a.q) L, as in bendingEdge
+    const R theta = hingeAtan2(-dot(a, q) * L, -dot(a, b));
     const V3<R> w = slot3 ? b : a;
     const R sw = slot3 ? R(1) : R(-1);
     const R iw2 = R(1) / dot(w, w);

@@ -171,7 +171,8 @@ struct spg_vbd_ctx {
     // neighbourhood layout of the LDS-staged solve (lds_staging 2; nbW 0: not built)
     int32_t nbW = 0;
     DevBuf<int32_t> nbRow, tetL;
-    DevBuf<float4> triL, hinEL, hinWL, sprL;
+    DevBuf<float4> triL, sprL;
+    DevBuf<NbHinge> hinEL, hinWL;
     // further energies: CSR rows per slot, elements (slots, padded FP64 parameters) for the residual, and the
     // solve's per-class fixed-width record rows
     DevBuf<int32_t> extPtr;
@@ -1013,7 +1014,8 @@ void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows
             if (r[k] == s) return static_cast<int32_t>(k);
         throw VbdError(SPG_VBD_ERR_INVALID, "neighbourhood layout: slot missing from its row");
     };
-    std::vector<float4> triL(triM.size()), hinL[2], sprL(sprRows.size());
+    std::vector<float4> triL(triM.size()), sprL(sprRows.size());
+    std::vector<NbHinge> hinL[2];
     std::vector<int32_t> tetL(tetO.size(), -1);
     for (int64_t v = 0; v < nv; ++v) {
         for (int32_t j = 0; j < c.triW; ++j) {
@@ -1043,7 +1045,7 @@ void buildNeighbourhoodLayout(spg_vbd_ctx &c, const std::vector<float4> &sprRows
                 const int4 h = hinV[cls][r];
                 const int32_t pk =
                     h.x >= 0 ? loc(v, h.x) | loc(v, h.y) << 5 | loc(v, h.z) << 10 | loc(v, h.w) << 15 : -1;
-                hinL[cls][r] = make_float4(hinP[cls][r].x, hinP[cls][r].y, intBitsAsFloat(pk), 0.f);
+                hinL[cls][r] = NbHinge{hinP[cls][r].x, hinP[cls][r].y, pk};
             }
     }
     // the residual's row entries: element id and the slab indices of its slots in stencil order

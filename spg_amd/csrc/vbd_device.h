@@ -67,6 +67,15 @@ __host__ __device__ constexpr int xclsPlanes(int c)
 __host__ __device__ constexpr int xclsStencil(int c) { return c == 0 ? 2 : c <= 2 ? 3 : 4; }
 __host__ __device__ constexpr int xclsType(int c) { return c <= 4 ? c + 7 : c + 6; }  // the energy type of class c
 
+// A hinge record of the neighbourhood-staged solve: the hinge's (theta0, keff) and its four slots as 5-bit indices
+// into the vertex's neighbour row (i0 | i1 << 5 | i2 << 10 | i3 << 15; -1: padding). 12 bytes, read with one
+// three-dword load.
+struct NbHinge {
+    float theta0, keff;
+    int32_t slots;
+};
+static_assert(sizeof(NbHinge) == 12, "hinge records are 12 bytes");
+
 struct SolveArgs {
     float4 *pos;
     const float4 *xt;
@@ -115,14 +124,14 @@ struct SolveArgs {
     float invdt2;
     // neighbourhood layout (lds_staging 2): per vertex slot a row of nbW slots (entry 0 the vertex itself, then
     // its distinct neighbours in order of first use, -1 padding), and records that name their vertices by 5-bit
-    // index into that row: StVK (kd, ko, ks, a | b << 5), hinges (theta0, keff, i0 | i1 << 5 | i2 << 10 |
-    // i3 << 15, 0), SNH a | b << 5 | c << 10 (with tetGa..c), spring (index, L0, k, 0); padding records -1.
+    // index into that row: StVK (kd, ko, ks, a | b << 5), hinges NbHinge, SNH a | b << 5 | c << 10 (with
+    // tetGa..c), spring (index, L0, k, 0); padding records -1.
     // nbW is 0 when the layout does not exist (a vertex with more than 31 neighbours, or further energies).
     int32_t nbW;
     const int32_t *nbRow;
     const float4 *triL;
-    const float4 *hinEL;
-    const float4 *hinWL;
+    const NbHinge *hinEL;
+    const NbHinge *hinWL;
     const int32_t *tetL;
     const float4 *sprL;
     // launch shape, set by the launcher: the kernel reads it with its other arguments instead of from the

@@ -817,7 +817,8 @@ __global__ void __launch_bounds__(256) solveColorNbKernel(SolveArgs a)
     const bool w0 = (CLS & kClsHinge) && active && lig < a.hinWW;
     const bool q0 = (CLS & kClsTet) && active && lig < a.tetW;
     const bool s0 = (CLS & kClsSpring) && active && lig < a.sprW;
-    float4 tg = {}, tk = {}, he = {}, hw = {}, ga = {}, gb = {}, gc = {}, sp = {};
+    float4 tg = {}, tk = {}, ga = {}, gb = {}, gc = {}, sp = {};
+    NbHinge he = {}, hw = {};
     int32_t tl = -1;
     if constexpr ((CLS & kClsTri) != 0) {
         if (t0) {
@@ -885,27 +886,27 @@ __global__ void __launch_bounds__(256) solveColorNbKernel(SolveArgs a)
             for (int32_t j = lig + G; j < a.triW; j += G) tri(a.triG[row * a.triW + j], a.triL[row * a.triW + j]);
         }
         if constexpr ((CLS & kClsHinge) != 0) {
-            auto hinE = [&](const float4 &r) {
-                const int32_t q = __float_as_int(r.z);
+            auto hinE = [&](const NbHinge &r) {
+                const int32_t q = r.slots;
                 if (q >= 0) {
                     const int32_t l1 = (q >> 5) & 31;
                     V3f ge;
                     S3f He;
                     bendingEdge<float>(xyz(slab[q & 31]), xyz(slab[l1]), xyz(slab[(q >> 10) & 31]),
-                                       xyz(slab[(q >> 15) & 31]), l1 == 0, r.x, r.y, ge, He);
+                                       xyz(slab[(q >> 15) & 31]), l1 == 0, r.theta0, r.keff, ge, He);
                     acc(g, H, ge, He);
                 }
             };
             if (e0) hinE(he);
             for (int32_t j = lig + G; j < a.hinEW; j += G) hinE(a.hinEL[row * a.hinEW + j]);
-            auto hinW = [&](const float4 &r) {
-                const int32_t q = __float_as_int(r.z);
+            auto hinW = [&](const NbHinge &r) {
+                const int32_t q = r.slots;
                 if (q >= 0) {
                     const int32_t l3 = (q >> 15) & 31;
                     V3f ge;
                     S3f He;
                     bendingWing<float>(xyz(slab[q & 31]), xyz(slab[(q >> 5) & 31]), xyz(slab[(q >> 10) & 31]),
-                                       xyz(slab[l3]), l3 == 0, r.x, r.y, ge, He);
+                                       xyz(slab[l3]), l3 == 0, r.theta0, r.keff, ge, He);
                     acc(g, H, ge, He);
                 }
             };
